@@ -1,0 +1,231 @@
+"""casmcode_monte_amd -- MI355X-native occupation-Metropolis hot path of libcasm-monte.
+
+The compute path is the HIP library lib/libcasm_monte_mi355x.so, reached
+through the C ABI of include/casm_mi355x/mcgpu.h.  This module is a thin
+ctypes host binding of that ABI (used by the tests and bench.py; the C++
+drop-in classes under include/casm/monte/ bind the same ABI).  There is no CPU
+fallback: loading fails loudly when the library or a gfx950 device is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcasm_monte_mi355x.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(HERE), "include")
+
+MODE_EXACT = 0
+MODE_CHECKERBOARD = 1
+LATTICE_SQUARE = 2
+LATTICE_SC = 3
+OBS_FORMATION_ENERGY = 0
+OBS_PARAM_COMPOSITION = 1
+OBS_POTENTIAL_ENERGY = 2
+OPT_FORCE_STREAMING = 1
+
+# Boltzmann constant, eV/K, as compiled into CASMcode_global (used by the
+# reference's beta = 1/(KB*T), basic_occupation_metropolis.hh:361).
+KB = 8.6173303e-05
+
+EXPORTED = [
+    "mcg_create", "mcg_destroy", "mcg_last_error", "mcg_set_occupation", "mcg_get_occupation",
+    "mcg_rng_dump", "mcg_rng_load", "mcg_run", "mcg_set_active", "mcg_get_counters",
+    "mcg_n_samples", "mcg_get_samples", "mcg_clear_samples", "mcg_stats_compute",
+    "mcg_equilibration", "mcg_last_run_time",
+]
+
+
+class ChainDesc(C.Structure):
+    _fields_ = [("shape", C.c_int64 * 3), ("temperature", C.c_double),
+                ("exchange_potential", C.c_double), ("seed", C.c_uint64)]
+
+
+class Desc(C.Structure):
+    _fields_ = [("mode", C.c_int64), ("lattice", C.c_int64), ("use_nlist", C.c_int64),
+                ("J1", C.c_double), ("J2", C.c_double), ("kb", C.c_double),
+                ("sample_period", C.c_int64), ("n_chains", C.c_int64),
+                ("chains", C.POINTER(ChainDesc)), ("options", C.c_int64)]
+
+
+SAMPLE_DTYPE = np.dtype([("pass", np.int64), ("S1", np.int64), ("S2", np.int64),
+                         ("M", np.int64), ("e_fold", np.float64), ("e_run", np.float64)])
+
+
+class Stats(C.Structure):
+    _fields_ = [("mean", C.c_double), ("calculated_precision", C.c_double),
+                ("variance", C.c_double), ("autocorr_factor", C.c_double), ("n", C.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        L.mcg_create.argtypes = [P(Desc), P(C.c_void_p)]
+        L.mcg_destroy.argtypes = [C.c_void_p]
+        L.mcg_destroy.restype = None
+        L.mcg_last_error.restype = C.c_char_p
+        L.mcg_set_occupation.argtypes = [C.c_void_p, C.c_int64, P(C.c_int8), C.c_int64]
+        L.mcg_get_occupation.argtypes = [C.c_void_p, C.c_int64, P(C.c_int8), C.c_int64]
+        L.mcg_rng_dump.argtypes = [C.c_void_p, C.c_int64, P(C.c_uint64), C.c_int64]
+        L.mcg_rng_load.argtypes = [C.c_void_p, C.c_int64, P(C.c_uint64), C.c_int64]
+        L.mcg_run.argtypes = [C.c_void_p, C.c_int64]
+        L.mcg_set_active.argtypes = [C.c_void_p, P(C.c_uint8), C.c_int64]
+        L.mcg_get_counters.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), P(C.c_int64),
+                                       P(C.c_int64)]
+        L.mcg_n_samples.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64)]
+        L.mcg_get_samples.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_void_p]
+        L.mcg_clear_samples.argtypes = [C.c_void_p]
+        L.mcg_stats_compute.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_double,
+                                        P(Stats)]
+        L.mcg_equilibration.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_double,
+                                        P(C.c_int64), P(C.c_int64)]
+        L.mcg_last_run_time.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+        _lib = L
+    return _lib
+
+
+class McgError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise McgError(f"mcg error {rc}: {lib().mcg_last_error().decode()}")
+
+
+class Run:
+    """One handle of the C ABI: a batch of chains advanced together on the GPU."""
+
+    def __init__(self, chains, mode=MODE_EXACT, lattice=LATTICE_SQUARE, J1=0.1, J2=0.0,
+                 kb=KB, use_nlist=True, sample_period=1, options=0):
+        """chains: sequence of dicts with shape, temperature, exchange_potential, seed."""
+        L = lib()
+        self.n_chains = len(chains)
+        arr = (ChainDesc * self.n_chains)()
+        self.shapes = []
+        for i, ch in enumerate(chains):
+            shp = list(ch["shape"]) + [1] * (3 - len(ch["shape"]))
+            arr[i].shape[:] = shp
+            arr[i].temperature = ch["temperature"]
+            arr[i].exchange_potential = ch.get("exchange_potential", 0.0)
+            arr[i].seed = ch.get("seed", 0)
+            self.shapes.append(tuple(shp))
+        self.chains = [dict(c) for c in chains]
+        d = Desc(mode, lattice, int(use_nlist), J1, J2, kb, sample_period, self.n_chains, arr,
+                 options)
+        self._arr = arr
+        h = C.c_void_p()
+        _check(L.mcg_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.mode, self.lattice, self.J1, self.J2 = mode, lattice, J1, J2
+        self.use_nlist, self.sample_period, self.kb = use_nlist, sample_period, kb
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mcg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def n_sites(self, c):
+        s = self.shapes[c]
+        return s[0] * s[1] * s[2]
+
+    def set_occupation(self, c, occ):
+        o = np.ascontiguousarray(occ, dtype=np.int8)
+        _check(lib().mcg_set_occupation(self.h, c, o.ctypes.data_as(C.POINTER(C.c_int8)), o.size))
+
+    def get_occupation(self, c):
+        o = np.zeros(self.n_sites(c), dtype=np.int8)
+        _check(lib().mcg_get_occupation(self.h, c, o.ctypes.data_as(C.POINTER(C.c_int8)), o.size))
+        return o
+
+    def rng_dump(self, c):
+        n = 313 if self.mode == MODE_EXACT else 4
+        w = np.zeros(n, dtype=np.uint64)
+        _check(lib().mcg_rng_dump(self.h, c, w.ctypes.data_as(C.POINTER(C.c_uint64)), n))
+        return w
+
+    def rng_load(self, c, words):
+        w = np.ascontiguousarray(words, dtype=np.uint64)
+        _check(lib().mcg_rng_load(self.h, c, w.ctypes.data_as(C.POINTER(C.c_uint64)), w.size))
+
+    def run(self, n_passes):
+        _check(lib().mcg_run(self.h, int(n_passes)))
+
+    def set_active(self, mask):
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        _check(lib().mcg_set_active(self.h, m.ctypes.data_as(C.POINTER(C.c_uint8)), m.size))
+
+    def counters(self, c):
+        a, b, d = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().mcg_get_counters(self.h, c, C.byref(a), C.byref(b), C.byref(d)))
+        return a.value, b.value, d.value
+
+    def n_samples(self, c):
+        n = C.c_int64()
+        _check(lib().mcg_n_samples(self.h, c, C.byref(n)))
+        return n.value
+
+    def samples(self, c, first=0, count=None):
+        n = self.n_samples(c)
+        if count is None:
+            count = n - first
+        out = np.zeros(count, dtype=SAMPLE_DTYPE)
+        _check(lib().mcg_get_samples(self.h, c, first, count, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def clear_samples(self):
+        _check(lib().mcg_clear_samples(self.h))
+
+    def stats(self, c, observable, first=0, confidence=0.95):
+        s = Stats()
+        _check(lib().mcg_stats_compute(self.h, c, observable, first, confidence, C.byref(s)))
+        return dict(mean=s.mean, calculated_precision=s.calculated_precision,
+                    variance=s.variance, autocorr_factor=s.autocorr_factor, n=s.n)
+
+    def equilibration(self, c, observable, prec):
+        a, b = C.c_int64(), C.c_int64()
+        _check(lib().mcg_equilibration(self.h, c, observable, prec, C.byref(a), C.byref(b)))
+        return bool(a.value), b.value
+
+    def last_run_time(self):
+        ms, att = C.c_double(), C.c_double()
+        _check(lib().mcg_last_run_time(self.h, C.byref(ms), C.byref(att)))
+        return ms.value, att.value
+
+    # -- the reference's FP64 sampling functions, recomputed from exact integers --
+    def observables(self, c, rec=None):
+        """formation_energy, param_composition, potential_energy per sample, with the
+        reference's rounding (model.hh:266-295, 412-422; basic_semigrand_canonical.hh:165-174)."""
+        if rec is None:
+            rec = self.samples(c)
+        N = np.float64(self.n_sites(c))
+        if self.mode == MODE_EXACT and not self.use_nlist:
+            Ef = rec["e_fold"].astype(np.float64)
+        else:
+            Ef = rec["S1"].astype(np.float64) * np.float64(-self.J1)
+            if self.J2 != 0.0:
+                Ef = Ef + rec["S2"].astype(np.float64) * np.float64(-self.J2)
+        Nx = (np.int64(self.n_sites(c)) + rec["M"]).astype(np.float64) / np.float64(2.0)
+        mu = np.float64(self.chains[c].get("exchange_potential", 0.0))
+        return dict(formation_energy=Ef / N, param_composition=Nx / N,
+                    potential_energy=(Ef - mu * Nx) / N)

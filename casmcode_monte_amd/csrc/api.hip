@@ -1,0 +1,779 @@
+// api.hip -- host implementation of the C ABI declared in include/casm_mi355x/mcgpu.h.
+//
+// Owns the device memory of a run (lattices, RNG states, counters, sample
+// series), builds the per-chain acceptance tables on the host with the same
+// libm exp the reference calls per attempt, and launches the kernels of
+// exact.hip, checker.hip and stats.hip on the handle's stream.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "internal.hh"
+
+namespace mcg {
+void exact_set_max_lds(int bytes);
+void checker_set_max_lds(int bytes);
+int checker_quads_per_item();
+}  // namespace mcg
+
+using namespace mcg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, std::string const &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess)                                                          \
+      return fail(MCG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+constexpr int64_t LDS_LIMIT = 160 * 1024;     // gfx950 LDS per workgroup
+constexpr int64_t EXACT_LDS_MAX = 150 * 1024;  // lattice + fold scratch
+constexpr int64_t CHECKER_LDS_MAX = 128 * 1024;
+
+// libstdc++ canonical double of a word compared to prob: the first word whose
+// canonical value is >= prob (2^64 if none), by bisection on the monotone map.
+// Returns false when every word accepts.
+bool threshold_for(double prob, uint64_t *thr) {
+  if (canonical_from_word(~0ULL) < prob) return false;  // all words accept
+  uint64_t lo = 0, hi = ~0ULL;  // invariant: answer in [lo, hi]
+  while (lo < hi) {
+    uint64_t mid = lo + (hi - lo) / 2;
+    if (canonical_from_word(mid) >= prob)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  *thr = lo;
+  return true;
+}
+
+// Ising formation energy per supercell of a 2D column-major lattice, both
+// variants of model.hh:259-290.
+double ising_per_supercell(std::vector<int8_t> const &occ, int64_t rows, int64_t cols,
+                           double J, bool use_nlist) {
+  auto site = [&](int64_t i, int64_t j) { return (int)occ[i + rows * j]; };
+  if (use_nlist) {
+    // model.hh:265-270: integer bond sum accumulated in a double, then *= -J
+    double e = 0.0;
+    for (int64_t l = 0; l < rows * cols; ++l) {
+      int64_t i = l % rows, j = l / rows;
+      int64_t iu = (i + 1) % rows, ju = (j + 1) % cols;
+      e += site(i, j) * (site(iu, j) + site(i, ju));
+    }
+    e *= -J;
+    return e;
+  }
+  double e = 0.0;
+  for (int64_t i = 0; i < rows; ++i) {
+    int64_t iu = (i + 1) % rows;
+    long dot = 0;
+    for (int64_t j = 0; j < cols; ++j) dot += site(i, j) * site(iu, j);
+    e += -J * static_cast<double>(dot);
+  }
+  for (int64_t j = 0; j < cols; ++j) {
+    int64_t ju = (j + 1) % cols;
+    long dot = 0;
+    for (int64_t i = 0; i < rows; ++i) dot += site(i, j) * site(i, ju);
+    e += -J * static_cast<double>(dot);
+  }
+  return e;
+}
+
+struct LatticeGeom {
+  int ndim;
+  int64_t L[3];
+  int64_t n() const { return L[0] * L[1] * L[2]; }
+};
+
+// Integer observables of a lattice (forward bonds counted once).
+void integer_observables(std::vector<int8_t> const &s, LatticeGeom const &g, bool nnn,
+                         int64_t &S1, int64_t &S2, int64_t &M) {
+  S1 = S2 = M = 0;
+  auto at = [&](int64_t i, int64_t j, int64_t k) {
+    i = (i % g.L[0] + g.L[0]) % g.L[0];
+    j = (j % g.L[1] + g.L[1]) % g.L[1];
+    k = (k % g.L[2] + g.L[2]) % g.L[2];
+    return (int)s[i + g.L[0] * (j + g.L[1] * k)];
+  };
+  for (int64_t k = 0; k < g.L[2]; ++k)
+    for (int64_t j = 0; j < g.L[1]; ++j)
+      for (int64_t i = 0; i < g.L[0]; ++i) {
+        int v = at(i, j, k);
+        M += v;
+        S1 += v * (at(i + 1, j, k) + at(i, j + 1, k));
+        if (g.ndim == 3) S1 += v * at(i, j, k + 1);
+        if (nnn) {
+          S2 += v * (at(i + 1, j + 1, k) + at(i + 1, j - 1, k));
+          if (g.ndim == 3)
+            S2 += v * (at(i + 1, j, k + 1) + at(i + 1, j, k - 1) + at(i, j + 1, k + 1) +
+                       at(i, j + 1, k - 1));
+        }
+      }
+}
+
+}  // namespace
+
+struct mcg_handle {
+  int64_t mode = 0, lattice = 0;
+  bool use_nlist = true;
+  double J1 = 0, J2 = 0, kb = 0;
+  int64_t sample_period = 1;
+  int64_t options = 0;
+  int n_chains = 0;
+  std::vector<mcg_chain_desc> chains;
+  std::vector<int64_t> lat_off;
+  int64_t lat_bytes = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0, last_attempts = 0.0;
+
+  int8_t *d_lat = nullptr;
+  uint8_t *d_active = nullptr;
+  std::vector<uint8_t> active;
+  mcg_sample *d_series = nullptr;
+  int64_t series_cap = 0;
+  std::vector<int64_t> n_samples;  // host mirror of the series length
+  std::vector<int64_t> n_pass;     // host mirror of the pass counters
+  double *d_stats = nullptr;
+  int64_t stats_cap = 0;
+  int64_t *d_eq = nullptr;
+
+  // EXACT
+  ExactParams *d_xp = nullptr;
+  ExactState *d_xs = nullptr;
+  std::vector<ExactParams> xp;
+  int exact_lds = 0;
+  bool exact_in_lds = true;
+
+  // CHECKERBOARD
+  CheckerParams *d_cp = nullptr;
+  CheckerState *d_cs = nullptr;
+  uint32_t *d_tab = nullptr;
+  std::vector<CheckerParams> cp;
+  int32_t *d_work = nullptr;
+  int n_work = 0;
+  bool work_dirty = true;
+  bool checker_lds = true;
+  int checker_lds_bytes = 0;
+  int checker_block = 256;
+  int ncolor = 2;
+
+  ~mcg_handle() {
+    if (stream) hipStreamSynchronize(stream);
+    void *ptrs[] = {d_lat, d_active, d_series, d_stats, d_eq, d_xp, d_xs, d_cp, d_cs, d_tab,
+                    d_work};
+    for (void *p : ptrs)
+      if (p) hipFree(p);
+    if (ev0) hipEventDestroy(ev0);
+    if (ev1) hipEventDestroy(ev1);
+    if (stream) hipStreamDestroy(stream);
+  }
+
+  int64_t sites(int64_t c) const {
+    return chains[c].shape[0] * chains[c].shape[1] * chains[c].shape[2];
+  }
+};
+
+namespace {
+
+int ensure_series(mcg_handle *h, int64_t need) {
+  if (need <= h->series_cap) return MCG_OK;
+  int64_t cap = std::max<int64_t>(need, h->series_cap * 2);
+  cap = std::max<int64_t>(cap, 64);
+  mcg_sample *d = nullptr;
+  size_t bytes = sizeof(mcg_sample) * (size_t)cap * h->n_chains;
+  if (hipMalloc(&d, bytes) != hipSuccess)
+    return fail(MCG_ERR_CAPACITY, "out of device memory for sample series");
+  if (h->d_series) {
+    HIP_TRY(hipMemcpy2DAsync(d, sizeof(mcg_sample) * cap, h->d_series,
+                             sizeof(mcg_sample) * h->series_cap,
+                             sizeof(mcg_sample) * h->series_cap, h->n_chains,
+                             hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    hipFree(h->d_series);
+  }
+  h->d_series = d;
+  h->series_cap = cap;
+  return MCG_OK;
+}
+
+int build_exact(mcg_handle *h) {
+  h->xp.resize(h->n_chains);
+  int max_lds = 0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    auto const &ch = h->chains[c];
+    ExactParams &P = h->xp[c];
+    std::memset(&P, 0, sizeof(P));
+    P.rows = (int32_t)ch.shape[0];
+    P.cols = (int32_t)ch.shape[1];
+    P.n = (int32_t)(ch.shape[0] * ch.shape[1]);
+    P.fold = h->use_nlist ? 0 : 1;
+    P.div_magic = ((1ULL << 40) + (uint64_t)P.rows - 1) / (uint64_t)P.rows;
+    P.negJ = -h->J1;
+    P.lat_off = h->lat_off[c];
+    const double beta = 1.0 / (h->kb * ch.temperature);  // basic_occupation_metropolis.hh:361
+    const double mu = ch.exchange_potential;
+    for (int sidx = 0; sidx < 2; ++sidx) {
+      const int s_old = sidx ? 1 : -1;
+      const int new_occ = -s_old;
+      for (int m = 0; m < 5; ++m) {
+        const int nsum = 2 * m - 4;
+        const int k = sidx * 5 + m;
+        // model.hh:312-314: -J * (new - old) * (sum of 4 neighbours)
+        const double dEf = -h->J1 * (new_occ - s_old) * nsum;
+        // model.hh:432 and basic_semigrand_canonical.hh:188-191
+        const double Ndx = (new_occ - s_old) / 2.0;
+        const double dE = dEf - mu * Ndx;
+        P.dq[k] = dE / (double)P.n;  // occupation_metropolis.hh:141
+        if (dE < 0.0) {
+          P.neg_mask |= 1u << k;
+          continue;
+        }
+        const double prob = std::exp(-dE * beta);  // metropolis.hh:33
+        uint64_t thr = 0;
+        if (threshold_for(prob, &thr))
+          P.thr[k] = thr;
+        else
+          P.full_mask |= 1u << k;
+      }
+    }
+    const int lds = (int)(((P.n + 15) & ~15) + 4 * (P.rows + P.cols));
+    max_lds = std::max(max_lds, lds);
+  }
+  h->exact_in_lds = max_lds <= EXACT_LDS_MAX;
+  if (h->exact_in_lds) {
+    h->exact_lds = max_lds;
+  } else {
+    int scratch = 0;
+    for (auto const &P : h->xp) scratch = std::max(scratch, 4 * (P.rows + P.cols));
+    h->exact_lds = scratch;
+    if (scratch > EXACT_LDS_MAX) return fail(MCG_ERR_ARG, "lattice too large for EXACT mode");
+  }
+  exact_set_max_lds((int)LDS_LIMIT - 4096);
+  HIP_TRY(hipMalloc(&h->d_xp, sizeof(ExactParams) * h->n_chains));
+  HIP_TRY(hipMalloc(&h->d_xs, sizeof(ExactState) * h->n_chains));
+  HIP_TRY(hipMemcpy(h->d_xp, h->xp.data(), sizeof(ExactParams) * h->n_chains,
+                    hipMemcpyHostToDevice));
+  std::vector<ExactState> xs(h->n_chains);
+  for (int c = 0; c < h->n_chains; ++c) {
+    ExactState &S = xs[c];
+    std::memset(&S, 0, sizeof(S));
+    int pos = 0;
+    mt_seed(h->chains[c].seed, S.mt, &pos);
+    S.pos = pos;
+  }
+  HIP_TRY(hipMemcpy(h->d_xs, xs.data(), sizeof(ExactState) * h->n_chains,
+                    hipMemcpyHostToDevice));
+  return MCG_OK;
+}
+
+int build_checker(mcg_handle *h) {
+  const bool nnn = h->J2 != 0.0;
+  const int ndim = h->lattice == MCG_LATTICE_SC ? 3 : 2;
+  h->ncolor = nnn ? (ndim == 3 ? 8 : 4) : 2;
+  const int z1 = ndim == 3 ? 6 : 4;
+  const int z2 = nnn ? (ndim == 3 ? 12 : 4) : 0;
+  const int ntab = 2 * (z1 + 1) * (z2 + 1);
+  h->cp.resize(h->n_chains);
+  std::vector<uint32_t> tabs((size_t)ntab * h->n_chains);
+  int64_t max_n = 0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    auto const &ch = h->chains[c];
+    for (int d = 0; d < ndim; ++d)
+      if (ch.shape[d] < 2 || (ch.shape[d] & 1))
+        return fail(MCG_ERR_ARG, "CHECKERBOARD needs even edge lengths >= 2");
+    CheckerParams &P = h->cp[c];
+    std::memset(&P, 0, sizeof(P));
+    P.L0 = (int32_t)ch.shape[0];
+    P.L1 = (int32_t)ch.shape[1];
+    P.L2 = (int32_t)ch.shape[2];
+    P.ncolor = h->ncolor;
+    P.z1 = z1;
+    P.z2 = z2;
+    P.ndim = ndim;
+    P.key0 = (uint32_t)ch.seed;
+    P.key1 = (uint32_t)(ch.seed >> 32);
+    P.n = h->sites(c);
+    P.lat_off = h->lat_off[c];
+    P.tab_off = (int64_t)ntab * c;
+    max_n = std::max(max_n, P.n);
+    const double beta = 1.0 / (h->kb * ch.temperature);
+    const double mu = ch.exchange_potential;
+    for (int sidx = 0; sidx < 2; ++sidx) {
+      const int s = sidx ? 1 : -1;
+      const double ds = -2.0 * s;
+      for (int a = 0; a <= z1; ++a)
+        for (int b = 0; b <= z2; ++b) {
+          const int nn = 2 * a - z1, nnn_sum = 2 * b - z2;
+          const double dE = -h->J1 * ds * nn - h->J2 * ds * nnn_sum - mu * (ds / 2.0);
+          const double p = std::exp(-beta * dE);
+          uint32_t t;
+          if (!(p < 1.0)) {
+            t = 0xFFFFFFFFu;
+          } else {
+            double v = std::ceil(p * 4294967296.0) - 1.0;
+            if (v < 0.0) v = 0.0;
+            if (v > 4294967295.0) v = 4294967295.0;
+            t = (uint32_t)v;
+          }
+          tabs[(size_t)P.tab_off + (size_t)sidx * (z1 + 1) * (z2 + 1) + a * (z2 + 1) + b] = t;
+        }
+    }
+  }
+  h->checker_lds = max_n <= CHECKER_LDS_MAX && !(h->options & MCG_OPT_FORCE_STREAMING);
+  h->checker_lds_bytes = (int)((max_n + 15) & ~15);
+  checker_set_max_lds((int)LDS_LIMIT - 4096);
+  HIP_TRY(hipMalloc(&h->d_cp, sizeof(CheckerParams) * h->n_chains));
+  HIP_TRY(hipMalloc(&h->d_cs, sizeof(CheckerState) * h->n_chains));
+  HIP_TRY(hipMalloc(&h->d_tab, sizeof(uint32_t) * tabs.size()));
+  HIP_TRY(hipMemcpy(h->d_cp, h->cp.data(), sizeof(CheckerParams) * h->n_chains,
+                    hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_tab, tabs.data(), sizeof(uint32_t) * tabs.size(),
+                    hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(h->d_cs, 0, sizeof(CheckerState) * h->n_chains));
+  return MCG_OK;
+}
+
+// Recompute a chain's integer observables (and EXACT running energy) from its
+// host occupation and store them in the device state.
+int reset_observables(mcg_handle *h, int64_t c, std::vector<int8_t> const &occ) {
+  auto const &ch = h->chains[c];
+  LatticeGeom g{h->lattice == MCG_LATTICE_SC ? 3 : 2, {ch.shape[0], ch.shape[1], ch.shape[2]}};
+  int64_t S1, S2, M;
+  integer_observables(occ, g, h->J2 != 0.0, S1, S2, M);
+  if (h->mode == MCG_MODE_EXACT) {
+    ExactState st;
+    HIP_TRY(hipMemcpy(&st, h->d_xs + c, sizeof(ExactState), hipMemcpyDeviceToHost));
+    st.S = S1;
+    st.M = M;
+    // occupation_metropolis.hh:103-106: potential per_supercell / n_unitcells
+    const int64_t n = g.n();
+    const double Ef = ising_per_supercell(occ, ch.shape[0], ch.shape[1], h->J1, h->use_nlist);
+    const double Nx = static_cast<double>(n + M) / 2.0;
+    const double E = Ef - ch.exchange_potential * Nx;
+    st.e_run = E / (double)n;
+    HIP_TRY(hipMemcpy(h->d_xs + c, &st, sizeof(ExactState), hipMemcpyHostToDevice));
+  } else {
+    CheckerState st;
+    HIP_TRY(hipMemcpy(&st, h->d_cs + c, sizeof(CheckerState), hipMemcpyDeviceToHost));
+    st.S1 = S1;
+    st.S2 = S2;
+    st.M = M;
+    HIP_TRY(hipMemcpy(h->d_cs + c, &st, sizeof(CheckerState), hipMemcpyHostToDevice));
+  }
+  return MCG_OK;
+}
+
+int rebuild_work(mcg_handle *h) {
+  const int qpi = checker_quads_per_item();
+  std::vector<int32_t> work;
+  for (int c = 0; c < h->n_chains; ++c) {
+    if (!h->active[c]) continue;
+    auto const &P = h->cp[c];
+    int64_t half0 = P.L0 / 2, qpl = (half0 + 3) / 4;
+    int64_t nl = P.ncolor == 2 ? (int64_t)P.L1 * P.L2
+                               : (P.ncolor == 4 ? P.L1 / 2 : (int64_t)(P.L1 / 2) * (P.L2 / 2));
+    int64_t nq = nl * qpl;
+    int64_t items = (nq + qpi - 1) / qpi;
+    for (int64_t it = 0; it < items; ++it) {
+      work.push_back(c);
+      work.push_back((int32_t)it);
+    }
+  }
+  if (h->d_work) hipFree(h->d_work);
+  h->d_work = nullptr;
+  h->n_work = (int)(work.size() / 2);
+  if (h->n_work) {
+    HIP_TRY(hipMalloc(&h->d_work, sizeof(int32_t) * work.size()));
+    HIP_TRY(hipMemcpy(h->d_work, work.data(), sizeof(int32_t) * work.size(),
+                      hipMemcpyHostToDevice));
+  }
+  h->work_dirty = false;
+  return MCG_OK;
+}
+
+bool valid_chain(mcg_handle *h, int64_t c) { return h && c >= 0 && c < h->n_chains; }
+
+}  // namespace
+
+extern "C" {
+
+const char *mcg_last_error(void) { return g_err.c_str(); }
+
+int mcg_create(const mcg_desc *desc, mcg_handle **out) {
+  if (!desc || !out) return fail(MCG_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (desc->mode != MCG_MODE_EXACT && desc->mode != MCG_MODE_CHECKERBOARD)
+    return fail(MCG_ERR_ARG, "unknown mode");
+  if (desc->lattice != MCG_LATTICE_SQUARE && desc->lattice != MCG_LATTICE_SC)
+    return fail(MCG_ERR_ARG, "unknown lattice");
+  if (desc->mode == MCG_MODE_EXACT && (desc->lattice != MCG_LATTICE_SQUARE || desc->J2 != 0.0))
+    return fail(MCG_ERR_ARG, "EXACT mode supports the 2D square nearest-neighbour model only");
+  if (desc->n_chains <= 0 || desc->n_chains > (1 << 24) || !desc->chains)
+    return fail(MCG_ERR_ARG, "n_chains out of range");
+  if (desc->sample_period < 1) return fail(MCG_ERR_ARG, "sample_period < 1");
+  if (!(desc->kb > 0.0)) return fail(MCG_ERR_ARG, "kb must be > 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(MCG_ERR_HIP, "no HIP device available (the library has no CPU fallback)");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
+    return fail(MCG_ERR_HIP, std::string("device is not gfx950: ") + prop.gcnArchName);
+
+  auto h = new mcg_handle();
+  h->mode = desc->mode;
+  h->lattice = desc->lattice;
+  h->use_nlist = desc->use_nlist != 0;
+  h->J1 = desc->J1;
+  h->J2 = desc->J2;
+  h->kb = desc->kb;
+  h->sample_period = desc->sample_period;
+  h->options = desc->options;
+  h->n_chains = (int)desc->n_chains;
+  h->chains.assign(desc->chains, desc->chains + desc->n_chains);
+  for (auto &ch : h->chains) {
+    if (h->lattice == MCG_LATTICE_SQUARE) ch.shape[2] = 1;
+    for (int d = 0; d < 3; ++d)
+      if (ch.shape[d] < 1 || ch.shape[d] > (1 << 20)) {
+        delete h;
+        return fail(MCG_ERR_ARG, "shape out of range");
+      }
+    if (!(ch.temperature > 0.0)) {
+      delete h;
+      return fail(MCG_ERR_ARG, "temperature must be > 0");
+    }
+  }
+  int64_t off = 0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    int64_t n = h->sites(c);
+    // EXACT: the walk divides l < 2^24 by rows with a 40-bit reciprocal, which
+    // is exact for rows < 2^16
+    if (h->mode == MCG_MODE_EXACT &&
+        (n > (1LL << 24) || h->chains[c].shape[0] >= (1LL << 16))) {
+      delete h;
+      return fail(MCG_ERR_ARG, "EXACT mode supports at most 2^24 sites and rows < 65536");
+    }
+    if (n > (1LL << 31)) {
+      delete h;
+      return fail(MCG_ERR_ARG, "too many sites in one chain");
+    }
+    h->lat_off.push_back(off);
+    off += (n + 255) & ~255LL;  // 256-byte aligned lattices
+  }
+  h->lat_bytes = off;
+  int rc = MCG_OK;
+  auto bail = [&](int code) {
+    delete h;
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(MCG_ERR_HIP, "hipStreamCreate failed"));
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+    return bail(fail(MCG_ERR_HIP, "hipEventCreate failed"));
+  if (hipMalloc(&h->d_lat, (size_t)h->lat_bytes) != hipSuccess)
+    return bail(fail(MCG_ERR_CAPACITY, "out of device memory for lattices"));
+  if (hipMemset(h->d_lat, 1, (size_t)h->lat_bytes) != hipSuccess)
+    return bail(fail(MCG_ERR_HIP, "hipMemset failed"));
+  h->active.assign(h->n_chains, 1);
+  if (hipMalloc(&h->d_active, h->n_chains) != hipSuccess ||
+      hipMemcpy(h->d_active, h->active.data(), h->n_chains, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(MCG_ERR_HIP, "active mask allocation failed"));
+  h->n_samples.assign(h->n_chains, 0);
+  h->n_pass.assign(h->n_chains, 0);
+  if (hipMalloc(&h->d_eq, 2 * sizeof(int64_t)) != hipSuccess)
+    return bail(fail(MCG_ERR_HIP, "allocation failed"));
+  rc = (h->mode == MCG_MODE_EXACT) ? build_exact(h) : build_checker(h);
+  if (rc != MCG_OK) return bail(rc);
+  // all +1 lattices: set observables
+  for (int c = 0; c < h->n_chains; ++c) {
+    std::vector<int8_t> occ(h->sites(c), 1);
+    rc = reset_observables(h, c, occ);
+    if (rc != MCG_OK) return bail(rc);
+  }
+  rc = ensure_series(h, 64);
+  if (rc != MCG_OK) return bail(rc);
+  *out = h;
+  return MCG_OK;
+}
+
+void mcg_destroy(mcg_handle *h) { delete h; }
+
+int mcg_set_occupation(mcg_handle *h, int64_t chain, const int8_t *occ, int64_t n) {
+  if (!valid_chain(h, chain) || !occ) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "occupation size mismatch");
+  std::vector<int8_t> v(occ, occ + n);
+  for (int8_t s : v)
+    if (s != 1 && s != -1) return fail(MCG_ERR_ARG, "occupation values must be +1 or -1");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemcpy(h->d_lat + h->lat_off[chain], v.data(), n, hipMemcpyHostToDevice));
+  return reset_observables(h, chain, v);
+}
+
+int mcg_get_occupation(mcg_handle *h, int64_t chain, int8_t *occ, int64_t n) {
+  if (!valid_chain(h, chain) || !occ) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "occupation size mismatch");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemcpy(occ, h->d_lat + h->lat_off[chain], n, hipMemcpyDeviceToHost));
+  return MCG_OK;
+}
+
+int mcg_rng_dump(mcg_handle *h, int64_t chain, uint64_t *words, int64_t n_words) {
+  if (!valid_chain(h, chain) || !words) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->mode == MCG_MODE_EXACT) {
+    if (n_words < MT_N + 1) return fail(MCG_ERR_CAPACITY, "need 313 words");
+    ExactState st;
+    HIP_TRY(hipMemcpy(&st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
+    std::memcpy(words, st.mt, sizeof(uint64_t) * MT_N);
+    words[MT_N] = (uint64_t)st.pos;
+  } else {
+    if (n_words < 4) return fail(MCG_ERR_CAPACITY, "need 4 words");
+    CheckerState st;
+    HIP_TRY(hipMemcpy(&st, h->d_cs + chain, sizeof(st), hipMemcpyDeviceToHost));
+    words[0] = h->cp[chain].key0;
+    words[1] = h->cp[chain].key1;
+    words[2] = (uint64_t)st.n_pass;
+    words[3] = 0;
+  }
+  return MCG_OK;
+}
+
+int mcg_rng_load(mcg_handle *h, int64_t chain, const uint64_t *words, int64_t n_words) {
+  if (!valid_chain(h, chain) || !words) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->mode == MCG_MODE_EXACT) {
+    if (n_words < MT_N + 1) return fail(MCG_ERR_CAPACITY, "need 313 words");
+    if (words[MT_N] > (uint64_t)MT_N) return fail(MCG_ERR_ARG, "engine position out of range");
+    ExactState st;
+    HIP_TRY(hipMemcpy(&st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
+    std::memcpy(st.mt, words, sizeof(uint64_t) * MT_N);
+    st.pos = (int64_t)words[MT_N];
+    HIP_TRY(hipMemcpy(h->d_xs + chain, &st, sizeof(st), hipMemcpyHostToDevice));
+  } else {
+    if (n_words < 4) return fail(MCG_ERR_CAPACITY, "need 4 words");
+    h->cp[chain].key0 = (uint32_t)words[0];
+    h->cp[chain].key1 = (uint32_t)words[1];
+    HIP_TRY(hipMemcpy(h->d_cp + chain, &h->cp[chain], sizeof(CheckerParams),
+                      hipMemcpyHostToDevice));
+    CheckerState st;
+    HIP_TRY(hipMemcpy(&st, h->d_cs + chain, sizeof(st), hipMemcpyDeviceToHost));
+    st.n_pass = (int64_t)words[2];
+    HIP_TRY(hipMemcpy(h->d_cs + chain, &st, sizeof(st), hipMemcpyHostToDevice));
+    h->n_pass[chain] = st.n_pass;
+  }
+  return MCG_OK;
+}
+
+int mcg_set_active(mcg_handle *h, const uint8_t *active, int64_t n_chains) {
+  if (!h || !active || n_chains != h->n_chains) return fail(MCG_ERR_ARG, "bad active mask");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  for (int c = 0; c < h->n_chains; ++c) h->active[c] = active[c] ? 1 : 0;
+  HIP_TRY(hipMemcpy(h->d_active, h->active.data(), h->n_chains, hipMemcpyHostToDevice));
+  h->work_dirty = true;
+  return MCG_OK;
+}
+
+int mcg_run(mcg_handle *h, int64_t n_passes) {
+  if (!h) return fail(MCG_ERR_ARG, "null handle");
+  if (n_passes < 0) return fail(MCG_ERR_ARG, "n_passes < 0");
+  h->last_ms = 0.0;
+  h->last_attempts = 0.0;
+  if (n_passes == 0) return MCG_OK;
+  // grow the series to hold every sample this run can append
+  int64_t need = 0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    if (!h->active[c]) continue;
+    int64_t p0 = h->n_pass[c];
+    int64_t add = (p0 + n_passes) / h->sample_period - p0 / h->sample_period;
+    need = std::max(need, h->n_samples[c] + add);
+  }
+  int rc = ensure_series(h, need);
+  if (rc != MCG_OK) return rc;
+  double attempts = 0.0;
+  for (int c = 0; c < h->n_chains; ++c)
+    if (h->active[c]) attempts += (double)h->sites(c) * (double)n_passes;
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  if (h->mode == MCG_MODE_EXACT) {
+    launch_exact(h->d_xp, h->d_xs, h->d_lat, h->d_series, h->series_cap, h->d_active,
+                 h->n_chains, n_passes, h->sample_period, h->exact_lds, h->exact_in_lds,
+                 h->stream);
+  } else if (h->checker_lds) {
+    launch_checker_lds(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_series, h->series_cap,
+                       h->d_active, h->n_chains, n_passes, h->sample_period,
+                       h->checker_lds_bytes, h->checker_block, h->stream);
+  } else {
+    if (h->work_dirty) {
+      rc = rebuild_work(h);
+      if (rc != MCG_OK) return rc;
+    }
+    for (int64_t p = 0; p < n_passes; ++p) {
+      for (int color = 0; color < h->ncolor; ++color)
+        launch_checker_color(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_work, h->n_work, color,
+                             0, h->stream);
+      launch_checker_record(h->d_cp, h->d_cs, h->d_series, h->series_cap, h->d_active,
+                            h->n_chains, h->sample_period, h->stream);
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  h->last_attempts = attempts;
+  for (int c = 0; c < h->n_chains; ++c) {
+    if (!h->active[c]) continue;
+    int64_t p0 = h->n_pass[c];
+    h->n_samples[c] += (p0 + n_passes) / h->sample_period - p0 / h->sample_period;
+    h->n_pass[c] = p0 + n_passes;
+  }
+  return MCG_OK;
+}
+
+int mcg_get_counters(mcg_handle *h, int64_t chain, int64_t *n_pass, int64_t *n_accept,
+                     int64_t *n_reject) {
+  if (!valid_chain(h, chain)) return fail(MCG_ERR_ARG, "bad chain");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->mode == MCG_MODE_EXACT) {
+    ExactState st;
+    HIP_TRY(hipMemcpy(&st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
+    if (n_pass) *n_pass = st.n_pass;
+    if (n_accept) *n_accept = st.n_accept;
+    if (n_reject) *n_reject = st.n_reject;
+  } else {
+    CheckerState st;
+    HIP_TRY(hipMemcpy(&st, h->d_cs + chain, sizeof(st), hipMemcpyDeviceToHost));
+    if (n_pass) *n_pass = st.n_pass;
+    if (n_accept) *n_accept = st.n_accept;
+    if (n_reject) *n_reject = st.n_pass * h->sites(chain) - st.n_accept;
+  }
+  return MCG_OK;
+}
+
+int mcg_n_samples(mcg_handle *h, int64_t chain, int64_t *n_samples) {
+  if (!valid_chain(h, chain) || !n_samples) return fail(MCG_ERR_ARG, "bad chain");
+  *n_samples = h->n_samples[chain];
+  return MCG_OK;
+}
+
+int mcg_get_samples(mcg_handle *h, int64_t chain, int64_t first, int64_t count,
+                    mcg_sample *out) {
+  if (!valid_chain(h, chain) || (!out && count)) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (first < 0 || count < 0 || first + count > h->n_samples[chain])
+    return fail(MCG_ERR_ARG, "sample range out of bounds");
+  if (!count) return MCG_OK;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemcpy(out, h->d_series + chain * h->series_cap + first,
+                    sizeof(mcg_sample) * count, hipMemcpyDeviceToHost));
+  return MCG_OK;
+}
+
+int mcg_clear_samples(mcg_handle *h) {
+  if (!h) return fail(MCG_ERR_ARG, "null handle");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->mode == MCG_MODE_EXACT) {
+    std::vector<ExactState> xs(h->n_chains);
+    HIP_TRY(hipMemcpy(xs.data(), h->d_xs, sizeof(ExactState) * h->n_chains,
+                      hipMemcpyDeviceToHost));
+    for (auto &s : xs) s.n_samples = 0;
+    HIP_TRY(hipMemcpy(h->d_xs, xs.data(), sizeof(ExactState) * h->n_chains,
+                      hipMemcpyHostToDevice));
+  } else {
+    std::vector<CheckerState> cs(h->n_chains);
+    HIP_TRY(hipMemcpy(cs.data(), h->d_cs, sizeof(CheckerState) * h->n_chains,
+                      hipMemcpyDeviceToHost));
+    for (auto &s : cs) s.n_samples = 0;
+    HIP_TRY(hipMemcpy(h->d_cs, cs.data(), sizeof(CheckerState) * h->n_chains,
+                      hipMemcpyHostToDevice));
+  }
+  std::fill(h->n_samples.begin(), h->n_samples.end(), 0);
+  return MCG_OK;
+}
+
+static ObsParams obs_params(mcg_handle *h, int64_t chain, int64_t observable) {
+  ObsParams op;
+  op.n_sites = h->sites(chain);
+  op.negJ1 = -h->J1;
+  op.negJ2 = -h->J2;
+  op.mu = h->chains[chain].exchange_potential;
+  op.fold = (h->mode == MCG_MODE_EXACT && !h->use_nlist) ? 1 : 0;
+  op.observable = (int32_t)observable;
+  return op;
+}
+
+int mcg_stats_compute(mcg_handle *h, int64_t chain, int64_t observable, int64_t first,
+                      double confidence, mcg_stats *out) {
+  if (!valid_chain(h, chain) || !out) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (observable < 0 || observable > 2) return fail(MCG_ERR_ARG, "bad observable");
+  const int64_t count = h->n_samples[chain] - first;
+  if (first < 0 || count <= 0)
+    return fail(MCG_ERR_ARG, "Error in BasicStatisticsCalculator: observations.size()==0");
+  if (count + 4 > h->stats_cap) {
+    if (h->d_stats) hipFree(h->d_stats);
+    h->d_stats = nullptr;
+    h->stats_cap = std::max<int64_t>(count + 4, 2 * h->stats_cap);
+    HIP_TRY(hipMalloc(&h->d_stats, sizeof(double) * h->stats_cap));
+  }
+  launch_stats(h->d_series + chain * h->series_cap, first, count,
+               obs_params(h, chain, observable), h->d_stats, h->stream);
+  HIP_TRY(hipGetLastError());
+  double r[4];
+  HIP_TRY(hipMemcpyAsync(r, h->d_stats, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  // BasicStatistics.cc:127-128 (host libm for the confidence factor)
+  const double one = 1.0, PI = 3.141592653589793238463, a = 0.147;
+  const double x = confidence;
+  const double sgn = (x < 0.0) ? -one : one;
+  const double b = std::log((one - x) * (one + x));
+  const double cc = 2.0 / (PI * a) + b * 0.5;
+  const double d = b / a;
+  const double erfinv = sgn * std::sqrt(std::sqrt(cc * cc - d) - cc);
+  const double f_conf = std::sqrt(2.0) * erfinv;
+  out->mean = r[0];
+  out->variance = r[1];
+  out->autocorr_factor = r[2];
+  out->n = count;
+  out->calculated_precision = f_conf * std::sqrt(r[2] * r[1] / (double)count);
+  return MCG_OK;
+}
+
+int mcg_equilibration(mcg_handle *h, int64_t chain, int64_t observable, double prec,
+                      int64_t *is_equilibrated, int64_t *n_samples_for_equilibration) {
+  if (!valid_chain(h, chain) || !is_equilibrated || !n_samples_for_equilibration)
+    return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (observable < 0 || observable > 2) return fail(MCG_ERR_ARG, "bad observable");
+  const int64_t count = h->n_samples[chain];
+  if (count <= 0) return fail(MCG_ERR_ARG, "Error in equilibration_check: observations.size()==0");
+  launch_equilibration(h->d_series + chain * h->series_cap, count,
+                       obs_params(h, chain, observable), prec, h->d_eq, h->stream);
+  HIP_TRY(hipGetLastError());
+  int64_t r[2];
+  HIP_TRY(hipMemcpyAsync(r, h->d_eq, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  *is_equilibrated = r[0];
+  *n_samples_for_equilibration = r[1];
+  return MCG_OK;
+}
+
+int mcg_last_run_time(mcg_handle *h, double *ms, double *attempts) {
+  if (!h) return fail(MCG_ERR_ARG, "null handle");
+  if (ms) *ms = h->last_ms;
+  if (attempts) *attempts = h->last_attempts;
+  return MCG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+// internal.hh -- device data layout shared by the kernels and the C ABI host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/casm_mi355x/mcgpu.h"
+#include "rng.hh"
+
+namespace mcg {
+
+// ---------------------------------------------------------------------------
+// EXACT mode: one chain per 64-lane workgroup (one wavefront).
+// ---------------------------------------------------------------------------
+
+// Number of (old spin, neighbour sum) classes of the 2D square NN Ising flip:
+// class k = (s_old > 0) * 5 + (nbr_sum + 4) / 2.
+constexpr int EXACT_NCLASS = 10;
+
+// Read-only per-chain parameters (built on the host).
+struct ExactParams {
+  int32_t rows, cols;  // shape[0], shape[1]
+  int32_t n;           // sites
+  int32_t fold;        // 1: compute the row/column-fold formation energy at samples
+  uint64_t div_magic;  // floor(l / rows) = (l * div_magic) >> 40 for l < 2^24
+  uint64_t thr[EXACT_NCLASS];  // accept a drawn word x iff x < thr[k]
+  uint32_t neg_mask;   // bit k: dE < 0, accept without drawing (metropolis.hh:28-30)
+  uint32_t full_mask;  // bit k: every drawn word accepts (thr == 2^64)
+  double dq[EXACT_NCLASS];  // fl(dE_k / n_unitcells), occupation_metropolis.hh:141
+  double negJ;               // -J, for the row/column fold
+  int64_t lat_off;           // byte offset of this chain's lattice in the pool
+};
+
+// Read-write per-chain state.
+struct ExactState {
+  uint64_t mt[MT_N];  // std::mt19937_64 state words
+  int64_t pos;        // index of the next output (312: twist first)
+  int64_t n_pass, n_accept, n_reject;
+  int64_t S, M;       // exact bond sum and magnetisation
+  double e_run;       // running potential energy per unit cell
+  int64_t n_samples;  // records written to the series
+};
+
+void launch_exact(const ExactParams *P, ExactState *S, int8_t *lat_pool, mcg_sample *series,
+                  int64_t series_cap, const uint8_t *active, int n_chains, int64_t n_passes,
+                  int64_t sample_period, int lds_bytes, bool lattice_in_lds,
+                  hipStream_t stream);
+
+// ---------------------------------------------------------------------------
+// CHECKERBOARD mode
+// ---------------------------------------------------------------------------
+
+// Acceptance classes: (old spin, NN sum, NNN sum).  For coordination numbers
+// z1 (NN) and z2 (NNN) the sums take z+1 values each (steps of 2).
+struct CheckerParams {
+  int32_t L0, L1, L2;      // shape
+  int32_t ncolor;          // 2, 4 or 8
+  int32_t z1, z2;          // coordination numbers (z2 = 0 without J2)
+  int32_t ndim;            // 2 or 3
+  int32_t pad0;
+  uint32_t key0, key1;     // Philox key (chain seed)
+  int64_t n;               // sites
+  int64_t lat_off;         // byte offset in the lattice pool
+  int64_t tab_off;         // offset of the chain's threshold table (uint32 entries)
+};
+
+struct CheckerState {
+  int64_t n_pass;
+  int64_t n_accept;
+  int64_t S1, S2, M;       // exact integer observables
+  int64_t n_samples;
+};
+
+void launch_checker_lds(const CheckerParams *P, CheckerState *St, int8_t *lat_pool,
+                        const uint32_t *tables, mcg_sample *series, int64_t series_cap,
+                        const uint8_t *active, int n_chains, int64_t n_passes,
+                        int64_t sample_period, int lds_bytes, int block, hipStream_t stream);
+
+void launch_checker_color(const CheckerParams *P, CheckerState *St, int8_t *lat_pool,
+                          const uint32_t *tables, const int32_t *work, int n_work_items,
+                          int color, int64_t pass, hipStream_t stream);
+
+void launch_checker_record(const CheckerParams *P, CheckerState *St, mcg_sample *series,
+                           int64_t series_cap, const uint8_t *active, int n_chains,
+                           int64_t sample_period, hipStream_t stream);
+
+// ---------------------------------------------------------------------------
+// Statistics on the device sample series
+// ---------------------------------------------------------------------------
+struct ObsParams {
+  int64_t n_sites;
+  double negJ1, negJ2;
+  double mu;
+  int32_t fold;  // use e_fold as E_f
+  int32_t observable;
+};
+
+void launch_stats(const mcg_sample *series, int64_t first, int64_t count, ObsParams op,
+                  double *d_out, hipStream_t stream);
+
+void launch_equilibration(const mcg_sample *series, int64_t count, ObsParams op, double prec,
+                          int64_t *d_out, hipStream_t stream);
+
+}  // namespace mcg

@@ -1,0 +1,173 @@
+// stats.hip -- BasicStatistics and the equilibration check over a chain's
+// device-resident sample series.
+//
+// Reference: BasicStatisticsCalculator::operator() (src/casm/monte/BasicStatistics.cc:114-131),
+// autocorrelation_factor (:24-48), variance (include/casm/monte/misc/math.hh:31-39),
+// _default_equilibration_check (src/casm/monte/checks/EquilibrationCheck.cc:50-117).
+//
+// The observable of each record is formed on the fly with the reference's FP64
+// rounding (the sampling functions of basic_semigrand_canonical.hh:485-590).
+// Reductions run in one 256-thread workgroup: the lag search evaluates 256
+// consecutive lags at a time (one lag per thread, coalesced reads of the
+// centred series from HBM/L2) and stops at the first batch holding a lag with
+// |cov/var0| <= 0.5.  Sums are reduced in a different order than the
+// reference's sequential loops, so results agree to rounding, not bitwise.
+#include "internal.hh"
+
+namespace mcg {
+
+__device__ __forceinline__ double observable_of(const mcg_sample &r, const ObsParams &op) {
+  const double N = (double)op.n_sites;
+  // formation energy per supercell
+  double ef;
+  if (op.fold) {
+    ef = r.e_fold;
+  } else {
+    // fl(S1*(-J1)) [model.hh:266-269], plus the J2 term of the lattice models
+    ef = __dmul_rn((double)r.S1, op.negJ1);
+    if (op.negJ2 != 0.0) ef = __dadd_rn(ef, __dmul_rn((double)r.S2, op.negJ2));
+  }
+  if (op.observable == MCG_OBS_FORMATION_ENERGY) return __ddiv_rn(ef, N);
+  const double Nx = __ddiv_rn((double)(op.n_sites + r.M), 2.0);  // model.hh:415
+  if (op.observable == MCG_OBS_PARAM_COMPOSITION) return __ddiv_rn(Nx, N);
+  // basic_semigrand_canonical.hh:165-174
+  const double e = __dsub_rn(ef, __dmul_rn(op.mu, Nx));
+  return __ddiv_rn(e, N);
+}
+
+template <int BS>
+__device__ double block_sum(double v, double *red) {
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] = __dadd_rn(red[t], red[t + s]);
+    __syncthreads();
+  }
+  double out = red[0];
+  __syncthreads();
+  return out;
+}
+
+// out[0] = mean, out[1] = var0, out[2] = autocorrelation factor, out[3] = lag
+// d_centered: scratch of `count` doubles
+__global__ __launch_bounds__(256) void stats_kernel(const mcg_sample *__restrict__ series,
+                                                    int64_t first, int64_t count, ObsParams op,
+                                                    double *__restrict__ d_centered,
+                                                    double *__restrict__ out) {
+  __shared__ double red[256];
+  __shared__ int best;
+  const int t = threadIdx.x;
+  double s = 0.0;
+  for (int64_t j = t; j < count; j += 256) s = __dadd_rn(s, observable_of(series[first + j], op));
+  const double mean = __ddiv_rn(block_sum<256>(s, red), (double)count);
+  double v = 0.0;
+  for (int64_t j = t; j < count; j += 256) {
+    const double d = __dsub_rn(observable_of(series[first + j], op), mean);
+    d_centered[j] = d;
+    v = __dadd_rn(v, __dmul_rn(d, d));
+  }
+  const double var0 = __ddiv_rn(block_sum<256>(v, red), (double)count);
+  __syncthreads();
+  double factor;
+  double lag = 0.0;
+  if (fabs(var0 / mean) < 1e-8 || var0 == 0.0) {
+    factor = 1.0;
+  } else {
+    factor = 1.7976931348623157e308;  // DBL_MAX: no lag found
+    for (int64_t base = 1; base < count; base += 256) {
+      if (t == 0) best = 0x7fffffff;
+      __syncthreads();
+      const int64_t i = base + t;
+      if (i < count) {
+        const int64_t range = count - i;
+        double cov = 0.0;
+        for (int64_t j = 0; j < range; ++j)
+          cov = __dadd_rn(cov, __dmul_rn(d_centered[j], d_centered[j + i]));
+        cov = __ddiv_rn(cov, (double)range);
+        if (fabs(cov / var0) <= 0.5) atomicMin(&best, t);
+      }
+      __syncthreads();
+      if (best != 0x7fffffff) {
+        lag = (double)(base + best);
+        break;
+      }
+      __syncthreads();
+    }
+    if (lag > 0.0) {
+      const double rho = pow(2.0, -1.0 / lag);
+      factor = (1.0 + rho) / (1.0 - rho);
+    }
+  }
+  if (t == 0) {
+    out[0] = mean;
+    out[1] = var0;
+    out[2] = factor;
+    out[3] = lag;
+  }
+}
+
+// Sequential scan of _default_equilibration_check by one lane.
+// out[0] = is_equilibrated, out[1] = N_samples_for_equilibration
+__global__ void equilibration_kernel(const mcg_sample *__restrict__ series, int64_t count,
+                                     ObsParams op, double prec, int64_t *__restrict__ out) {
+  if (threadIdx.x != 0) return;
+  auto obs = [&](int64_t i) { return observable_of(series[i], op); };
+  const int64_t N = count;
+  const double x0 = obs(0);
+  const double eps = (x0 == 0.0) ? 1e-8 : fabs(x0) * 1e-8;
+  bool all_same = true;
+  for (int64_t i = 0; i < N; ++i)
+    if (fabs(obs(i) - x0) > eps) {
+      all_same = false;
+      break;
+    }
+  if (all_same) {
+    out[0] = 1;
+    out[1] = 0;
+    return;
+  }
+  bool is_even = (N % 2) == 0;
+  int64_t start1 = 0;
+  int64_t start2 = is_even ? N / 2 : N / 2 + 1;
+  double sum1 = 0.0, sum2 = 0.0;
+  for (int64_t i = 0; i < start2; ++i) sum1 = __dadd_rn(sum1, obs(i));
+  for (int64_t i = start2; i < N; ++i) sum2 = __dadd_rn(sum2, obs(i));
+  while (fabs(sum1 / (double)(start2 - start1) - sum2 / (double)(N - start2)) > prec &&
+         start1 < N - 2) {
+    if (is_even) {
+      sum1 = __dsub_rn(sum1, obs(start1));
+      sum1 = __dadd_rn(sum1, obs(start2));
+      sum2 = __dsub_rn(sum2, obs(start2));
+      start2++;
+    } else {
+      sum1 = __dsub_rn(sum1, obs(start1));
+    }
+    start1++;
+    is_even = !is_even;
+  }
+  const double mean_tot = (sum1 + sum2) / (double)(N - start1);
+  if (obs(start1) < mean_tot) {
+    while (obs(start1) < mean_tot && start1 < N - 1) start1++;
+  } else {
+    while (obs(start1) > mean_tot && start1 < N - 1) start1++;
+  }
+  out[0] = (start1 < N - 1) ? 1 : 0;
+  out[1] = start1;
+}
+
+void launch_stats(const mcg_sample *series, int64_t first, int64_t count, ObsParams op,
+                  double *d_out, hipStream_t stream) {
+  // d_out: 4 result doubles followed by `count` doubles of scratch
+  hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(256), 0, stream, series, first, count, op,
+                     d_out + 4, d_out);
+}
+
+void launch_equilibration(const mcg_sample *series, int64_t count, ObsParams op, double prec,
+                          int64_t *d_out, hipStream_t stream) {
+  hipLaunchKernelGGL(equilibration_kernel, dim3(1), dim3(64), 0, stream, series, count, op,
+                     prec, d_out);
+}
+
+}  // namespace mcg

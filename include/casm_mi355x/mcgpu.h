@@ -1,0 +1,204 @@
+/* mcgpu.h -- C ABI of the MI355X occupation-Metropolis library
+ * (casmcode_monte_amd/lib/libcasm_monte_mi355x.so).
+ *
+ * The library replaces the inner loop of libcasm-monte's occupation
+ * Metropolis method for the semi-grand-canonical Ising model (and the
+ * build-defined lattice models of the statistical modes) with hand-written HIP
+ * kernels for gfx950.  The C++ drop-in classes under include/casm/monte/ call
+ * these entry points; nothing else in the reference's API changes.  Each entry
+ * point names the reference code it replaces (paths relative to the reference
+ * repository root).
+ *
+ * Conventions:
+ *  - every function returns MCG_OK (0) or an error code; mcg_last_error()
+ *    returns a thread-local message for the last failure on this thread;
+ *  - a handle owns its device memory and one HIP stream; the caller owns every
+ *    host buffer; there are no callbacks across the ABI;
+ *  - a handle is not thread-safe; separate handles may be driven from separate
+ *    host threads;
+ *  - there is no CPU fallback: without a usable gfx950 device mcg_create fails.
+ */
+#ifndef CASM_MI355X_MCGPU_H
+#define CASM_MI355X_MCGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes -------------------------------------------------------- */
+#define MCG_OK 0
+#define MCG_ERR_ARG 1      /* invalid argument */
+#define MCG_ERR_HIP 2      /* HIP runtime error (no device, launch failure, ...) */
+#define MCG_ERR_STATE 3    /* call not valid in the handle's current state */
+#define MCG_ERR_CAPACITY 4 /* caller buffer too small / device memory exhausted */
+
+/* ---- modes and lattices ----------------------------------------------------- */
+/* EXACT: one sequential chain per wavefront that reproduces the reference's
+ *   random-site order and std::mt19937_64 stream bit for bit
+ *   (basic_occupation_metropolis.hh:381-422 with the event generator of
+ *   basic_semigrand_canonical.hh:308-320).  Square lattice, J1 only.
+ * CHECKERBOARD: sublattice-coloured sweeps with a Philox4x32-10 counter RNG;
+ *   one pass updates every site once.  Square (2D) or simple-cubic (3D)
+ *   lattices with nearest (J1) and next-nearest (J2) couplings. */
+#define MCG_MODE_EXACT 0
+#define MCG_MODE_CHECKERBOARD 1
+
+#define MCG_LATTICE_SQUARE 2 /* 2D, shape[2] == 1 */
+#define MCG_LATTICE_SC 3     /* 3D simple cubic */
+
+/* One chain: one supercell at one (T, mu) condition.
+ * Replaces the (configuration, conditions) pair of IsingState
+ * (ising_cpp/model.hh:141-157) and SemiGrandCanonicalConditions
+ * (ising_cpp/basic_semigrand_canonical.hh:36-73). */
+typedef struct mcg_chain_desc {
+  int64_t shape[3];          /* supercell edge lengths; shape[2] == 1 in 2D  */
+  double temperature;        /* K                                            */
+  double exchange_potential; /* eV, conjugate to the parametric composition  */
+  uint64_t seed;             /* std::mt19937_64::seed value (EXACT) or Philox key */
+} mcg_chain_desc;
+
+/* Run description.  Replaces IsingFormationEnergy / IsingParamComposition /
+ * SemiGrandCanonicalPotential construction (model.hh:164-436,
+ * basic_semigrand_canonical.hh:104-199) and the beta = 1/(KB*T) set-up of
+ * basic_occupation_metropolis.hh:361. */
+typedef struct mcg_desc {
+  int64_t mode;          /* MCG_MODE_*                                        */
+  int64_t lattice;       /* MCG_LATTICE_*                                     */
+  int64_t use_nlist;     /* EXACT: formation-energy variant sampled: 1 = the
+                            neighbour-list sum fl(S*(-J)) (model.hh:261-270),
+                            0 = the row/column fold (model.hh:271-286)        */
+  double J1;             /* nearest-neighbour coupling, eV                    */
+  double J2;             /* next-nearest coupling, eV (CHECKERBOARD only)     */
+  double kb;             /* Boltzmann constant, eV/K                          */
+  int64_t sample_period; /* passes per sample (basic_occupation_metropolis
+                            sample_period)                                   */
+  int64_t n_chains;
+  const mcg_chain_desc *chains; /* n_chains entries */
+  int64_t options;       /* bit mask of MCG_OPT_*                             */
+} mcg_desc;
+
+/* CHECKERBOARD: always use the HBM work-list kernels, even for lattices that
+ * fit in LDS (the two paths produce identical trajectories). */
+#define MCG_OPT_FORCE_STREAMING 1
+
+/* One sampled record of one chain.  Integer observables are exact; the
+ * reference's FP64 sampling functions (basic_semigrand_canonical.hh:485-590)
+ * are recomputed from them on the host with the reference's rounding. */
+typedef struct mcg_sample {
+  int64_t pass;   /* pass count at which the sample was taken                */
+  int64_t S1;     /* sum over nearest-neighbour bonds of s_i s_j             */
+  int64_t S2;     /* sum over next-nearest bonds of s_i s_j                  */
+  int64_t M;      /* sum of s_i                                              */
+  double e_fold;  /* EXACT, use_nlist == 0: the reference's row/column fold of
+                     the formation energy per supercell, computed on device in
+                     the reference's order; otherwise 0                      */
+  double e_run;   /* EXACT: running potential energy per unit cell, updated by
+                     fl(dE / n_unitcells) per accepted step
+                     (occupation_metropolis.hh:141); otherwise 0             */
+} mcg_sample;
+
+typedef struct mcg_handle mcg_handle;
+
+/* Create a handle: validates the description, allocates device memory for
+ * every chain (lattice, RNG state, counters, sample series), fills every
+ * lattice with +1 (IsingConfiguration default, model.hh:23-33) and seeds every
+ * RNG.  Replaces IsingFormationEnergy::set_state (model.hh:210-256; the
+ * neighbour list becomes an implicit stencil) and RandomNumberGenerator
+ * construction (RandomNumberGenerator.hh:20-28). */
+int mcg_create(const mcg_desc *desc, mcg_handle **out);
+
+/* Release all device memory and the stream. */
+void mcg_destroy(mcg_handle *h);
+
+/* Message for the last error on the calling thread ("" if none). */
+const char *mcg_last_error(void);
+
+/* Set / get one chain's occupation (values +1/-1, reference linear order
+ * l = i + shape0*(j + shape1*k), column-major as model.hh:82-99).
+ * Replaces IsingConfiguration::set_occupation / occupation (model.hh:52-60).
+ * Setting recomputes the chain's integer observables and, for EXACT, the
+ * running energy from the initial per-supercell potential
+ * (occupation_metropolis.hh:106). */
+int mcg_set_occupation(mcg_handle *h, int64_t chain, const int8_t *occ, int64_t n);
+int mcg_get_occupation(mcg_handle *h, int64_t chain, int8_t *occ, int64_t n);
+
+/* Dump / load the RNG state of one chain.  EXACT: 313 words, the
+ * std::mt19937_64 state words followed by the index of the next output (the
+ * layout operator<< writes, used by the reference's engine dump/load,
+ * python/src/monte.cpp:495-513).  CHECKERBOARD: 4 words (key lo, key hi,
+ * pass counter, 0). */
+int mcg_rng_dump(mcg_handle *h, int64_t chain, uint64_t *words, int64_t n_words);
+int mcg_rng_load(mcg_handle *h, int64_t chain, const uint64_t *words, int64_t n_words);
+
+/* Advance every active chain by n_passes passes (one pass = one attempt per
+ * site: N random-site attempts in EXACT, one coloured sweep in CHECKERBOARD).
+ * A sample record is appended to a chain's device series whenever its pass
+ * count reaches a multiple of sample_period.  Replaces the main loop of
+ * basic_occupation_metropolis.hh:381-422 between two completion checks (the
+ * drop-in classes call it with the number of passes to the next point at
+ * which the reference's completion check can change its answer). */
+int mcg_run(mcg_handle *h, int64_t n_passes);
+
+/* Mark chains active (1) or inactive (0); inactive chains are skipped by
+ * mcg_run.  Used by the continuous-batching server to retire converged
+ * chains while others keep running. */
+int mcg_set_active(mcg_handle *h, const uint8_t *active, int64_t n_chains);
+
+/* Per-chain counters: pass count, accepted and rejected attempts
+ * (BasicOccupationMetropolisData n_pass/n_accept/n_reject,
+ * basic_occupation_metropolis.hh:71-82). */
+int mcg_get_counters(mcg_handle *h, int64_t chain, int64_t *n_pass, int64_t *n_accept,
+                     int64_t *n_reject);
+
+/* Number of sample records held in a chain's device series. */
+int mcg_n_samples(mcg_handle *h, int64_t chain, int64_t *n_samples);
+
+/* Copy sample records [first, first+count) of a chain to the host.
+ * Replaces the Sampler storage (sampling/Sampler.hh:252-267). */
+int mcg_get_samples(mcg_handle *h, int64_t chain, int64_t first, int64_t count,
+                    mcg_sample *out);
+
+/* Drop all stored sample records of every chain (device memory is kept). */
+int mcg_clear_samples(mcg_handle *h);
+
+/* Observables computed on the device from a chain's sample series:
+ *   MCG_OBS_PARAM_COMPOSITION  x  = fl(fl((N+M)/2)/N)        (model.hh:412-422)
+ *   MCG_OBS_FORMATION_ENERGY   ef = fl(E_f/N)                 (model.hh:293-295)
+ *   MCG_OBS_POTENTIAL_ENERGY   e  = fl(fl(E_f - fl(mu*Nx))/N) (basic_semigrand_canonical.hh:165-174)
+ * where E_f = fl(-J1*S1) - ... per the model (EXACT use_nlist==0: e_fold). */
+#define MCG_OBS_FORMATION_ENERGY 0
+#define MCG_OBS_PARAM_COMPOSITION 1
+#define MCG_OBS_POTENTIAL_ENERGY 2
+
+/* Statistics of one observable over a chain's samples [first, n_samples):
+ * mean and calculated_precision as BasicStatisticsCalculator::operator()
+ * (BasicStatistics.cc:114-131, with autocorrelation_factor :24-48 and
+ * approx_erf_inv math.hh:62-72), computed by a device reduction. */
+typedef struct mcg_stats {
+  double mean;
+  double calculated_precision;
+  double variance;        /* CoVar0 = sum (x - mean)^2 / N                     */
+  double autocorr_factor; /* (1+rho)/(1-rho), 1 if no variation, DBL_MAX if none */
+  int64_t n;              /* number of observations used                        */
+} mcg_stats;
+
+int mcg_stats_compute(mcg_handle *h, int64_t chain, int64_t observable, int64_t first,
+                      double confidence, mcg_stats *out);
+
+/* Equilibration check of one observable over a chain's whole series,
+ * as _default_equilibration_check (src/casm/monte/checks/EquilibrationCheck.cc:50-117)
+ * with absolute precision prec; the partition scan runs on the device. */
+int mcg_equilibration(mcg_handle *h, int64_t chain, int64_t observable, double prec,
+                      int64_t *is_equilibrated, int64_t *n_samples_for_equilibration);
+
+/* Time of the last mcg_run's kernels on the handle's stream, in ms
+ * (HIP events around the launches), and the number of site attempts done. */
+int mcg_last_run_time(mcg_handle *h, double *ms, double *attempts);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CASM_MI355X_MCGPU_H */

@@ -1,0 +1,325 @@
+// capi.cpp -- extern "C" entry points of the oracle (TEST INFRASTRUCTURE ONLY).
+//
+// Built into oracle/lib/liboracle.so by oracle/Makefile and loaded with ctypes
+// by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.  Every
+// entry point wraps a restatement in refcpu.hh; see that file for the
+// reference file:line each one follows.
+#include <cstring>
+#include <sstream>
+
+#include "refcpu.hh"
+
+using namespace refcpu;
+
+extern "C" {
+
+// All fields are 8 bytes wide so the layout has no padding (mirrored in
+// tests/oracle.py with ctypes).
+struct orc_sgc_params {
+  int64_t rows, cols;
+  double J;
+  int64_t use_nlist;
+  double temperature, mu;
+  uint64_t seed;
+  int64_t sample_period;
+  const int32_t *init_occ;  // NULL -> all +1
+  int64_t min_count, min_sample, max_count, max_sample;  // -1: unset
+  int64_t check_begin, check_period, log_spacing;
+  double check_base, check_shift;
+  int64_t check_period_max;
+  double prec_formation_energy, prec_param_composition, prec_potential_energy;  // <=0: unset
+  int64_t trace_len;
+};
+
+struct orc_sgc_output {
+  int64_t capacity;  // capacity of the per-sample arrays
+  double *formation_energy, *param_composition, *potential_energy;
+  int64_t *S, *M;
+  int32_t *final_occ;   // rows*cols
+  int64_t *trace;       // trace_len*5
+  uint64_t *engine;     // 313 words: state[312], position
+  int64_t n_samples, n_pass, n_accept, n_reject, n_checks;
+  int64_t is_complete, all_equilibrated, N_eq, all_converged, N_stats;
+  double mean[3], prec[3];
+  int64_t converged[3];
+};
+
+static void engine_words(std::mt19937_64 const &e, uint64_t *out) {
+  std::ostringstream os;
+  os << e;
+  std::istringstream is(os.str());
+  for (int i = 0; i < 313; ++i) is >> out[i];
+}
+
+int orc_sgc_run(const orc_sgc_params *p, orc_sgc_output *o) {
+  try {
+    IsingConfiguration cfg(p->rows, p->cols, 1);
+    if (p->init_occ)
+      for (long l = 0; l < cfg.n_sites; ++l) cfg.occupation[l] = p->init_occ[l];
+    CompletionCheckParams ccp;
+    if (p->min_count >= 0) ccp.cutoff_params.min_count = p->min_count;
+    if (p->min_sample >= 0) ccp.cutoff_params.min_sample = p->min_sample;
+    if (p->max_count >= 0) ccp.cutoff_params.max_count = p->max_count;
+    if (p->max_sample >= 0) ccp.cutoff_params.max_sample = p->max_sample;
+    ccp.check_begin = p->check_begin;
+    ccp.check_period = p->check_period;
+    ccp.log_spacing = p->log_spacing != 0;
+    ccp.check_base = p->check_base;
+    ccp.check_shift = p->check_shift;
+    ccp.check_period_max = p->check_period_max;
+    const char *names[3] = {"formation_energy", "param_composition", "potential_energy"};
+    double precs[3] = {p->prec_formation_energy, p->prec_param_composition,
+                       p->prec_potential_energy};
+    for (int k = 0; k < 3; ++k)
+      if (precs[k] > 0) {
+        RequestedPrecision rp;
+        rp.abs_convergence_is_required = true;
+        rp.abs_precision = precs[k];
+        ccp.requested_precision[SamplerComponent{names[k], 0}] = rp;
+      }
+    auto eng = std::make_shared<std::mt19937_64>();
+    eng->seed(p->seed);
+    SgcRunOutput r = sgc_basic_run(cfg, p->J, p->use_nlist != 0, p->temperature, p->mu, eng,
+                                   ccp, (int)p->sample_period, (size_t)p->trace_len);
+    int64_t ns = get_n_samples(r.samplers);
+    o->n_samples = ns;
+    if (ns > o->capacity) return 2;
+    auto fe = r.samplers.at("formation_energy")->component(0);
+    auto pc = r.samplers.at("param_composition")->component(0);
+    auto pe = r.samplers.at("potential_energy")->component(0);
+    for (int64_t i = 0; i < ns; ++i) {
+      if (o->formation_energy) o->formation_energy[i] = fe[i];
+      if (o->param_composition) o->param_composition[i] = pc[i];
+      if (o->potential_energy) o->potential_energy[i] = pe[i];
+      if (o->S) o->S[i] = r.S[i];
+      if (o->M) o->M[i] = r.M[i];
+    }
+    if (o->final_occ)
+      for (long l = 0; l < cfg.n_sites; ++l) o->final_occ[l] = r.final_config.occupation[l];
+    if (o->trace)
+      for (size_t t = 0; t < r.trace.size(); ++t) {
+        o->trace[5 * t + 0] = r.trace[t].site;
+        o->trace[5 * t + 1] = r.trace[t].old_occ;
+        o->trace[5 * t + 2] = r.trace[t].nbr_sum;
+        o->trace[5 * t + 3] = r.trace[t].drew;
+        o->trace[5 * t + 4] = r.trace[t].accepted;
+      }
+    if (o->engine) engine_words(*eng, o->engine);
+    o->n_pass = r.n_pass;
+    o->n_accept = r.n_accept;
+    o->n_reject = r.n_reject;
+    o->n_checks = r.n_checks;
+    o->is_complete = r.results.is_complete;
+    o->all_equilibrated = r.results.equilibration_check_results.all_equilibrated;
+    o->N_eq = r.results.equilibration_check_results.N_samples_for_all_to_equilibrate;
+    o->all_converged = r.results.convergence_check_results.all_converged;
+    o->N_stats = r.results.convergence_check_results.N_samples_for_statistics;
+    for (int k = 0; k < 3; ++k) {
+      o->mean[k] = 0.0;
+      o->prec[k] = -1.0;
+      o->converged[k] = -1;
+      auto it = r.results.convergence_check_results.individual_results.find(
+          SamplerComponent{names[k], 0});
+      if (it != r.results.convergence_check_results.individual_results.end()) {
+        o->mean[k] = it->second.stats.mean;
+        o->prec[k] = it->second.stats.calculated_precision;
+        o->converged[k] = it->second.is_converged;
+      }
+    }
+    return 0;
+  } catch (std::exception const &) {
+    return 1;
+  }
+}
+
+// First n outputs of std::mt19937_64 seeded with `seed`.
+void orc_mt64_outputs(uint64_t seed, int64_t n, uint64_t *out) {
+  std::mt19937_64 e(seed);
+  for (int64_t i = 0; i < n; ++i) out[i] = e();
+}
+
+// Engine state words (313: state[312], position) after `n_discard` outputs.
+void orc_mt64_state(uint64_t seed, int64_t n_discard, uint64_t *out) {
+  std::mt19937_64 e(seed);
+  e.discard(n_discard);
+  engine_words(e, out);
+}
+
+// An engine that replays caller-given 64-bit words, to drive the libstdc++
+// distributions with chosen inputs.
+struct ReplayEngine {
+  typedef uint64_t result_type;
+  const uint64_t *xs;
+  int64_t n, pos;
+  static constexpr uint64_t min() { return 0; }
+  static constexpr uint64_t max() { return ~uint64_t(0); }
+  uint64_t operator()() { return pos < n ? xs[pos++] : 0; }
+};
+
+// std::uniform_int_distribution<long>(0, maximum)(engine) on replayed words;
+// returns the value, *consumed = words used (1 unless Lemire rejected).
+long orc_uniform_int_replay(const uint64_t *xs, int64_t n, long maximum, int64_t *consumed) {
+  ReplayEngine e{xs, n, 0};
+  long v = std::uniform_int_distribution<long>(0, maximum)(e);
+  *consumed = e.pos;
+  return v;
+}
+
+// std::uniform_real_distribution<double>(0, 1)(engine) on one replayed word.
+double orc_canonical(uint64_t x) {
+  ReplayEngine e{&x, 1, 0};
+  return std::uniform_real_distribution<double>(0.0, 1.0)(e);
+}
+
+// Metropolis acceptance probability std::exp(-dE * beta), beta = 1/(KB*T)
+// (basic_occupation_metropolis.hh:361, metropolis.hh:33).
+double orc_accept_prob(double dE, double temperature) {
+  double beta = 1.0 / (KB * temperature);
+  return std::exp(-dE * beta);
+}
+
+double orc_kb() { return KB; }
+
+// Statistics restatements (BasicStatistics.cc, EquilibrationCheck.cc).
+void orc_basic_statistics(const double *x, int64_t n, double confidence, double *mean,
+                          double *prec) {
+  std::vector<double> v(x, x + n);
+  BasicStatistics s = basic_statistics(v, confidence);
+  *mean = s.mean;
+  *prec = s.calculated_precision;
+}
+
+double orc_autocorrelation_factor(const double *x, int64_t n) {
+  return autocorrelation_factor(std::vector<double>(x, x + n));
+}
+
+void orc_equilibration(const double *x, int64_t n, double prec, int64_t *is_eq,
+                       int64_t *n_eq) {
+  EquilibrationResult r = default_equilibration_check_impl(std::vector<double>(x, x + n), prec);
+  *is_eq = r.is_equilibrated;
+  *n_eq = r.N_samples_for_equilibration;
+}
+
+double orc_approx_erf_inv(double x) { return approx_erf_inv(x); }
+
+// Ising energies of a given occupation (model.hh per_supercell, both variants)
+double orc_ising_energy(int64_t rows, int64_t cols, const int32_t *occ, double J,
+                        int64_t use_nlist) {
+  IsingConfiguration c(rows, cols, 1);
+  for (long l = 0; l < c.n_sites; ++l) c.occupation[l] = occ[l];
+  IsingFormationEnergy fe(J, use_nlist != 0);
+  fe.set_state(&c);
+  return fe.per_supercell();
+}
+
+double orc_ising_delta(int64_t rows, int64_t cols, const int32_t *occ, double J,
+                       int64_t use_nlist, double mu, int64_t l, int32_t new_occ) {
+  IsingConfiguration c(rows, cols, 1);
+  for (long i = 0; i < c.n_sites; ++i) c.occupation[i] = occ[i];
+  SemiGrandCanonicalPotential pot{IsingFormationEnergy(J, use_nlist != 0)};
+  pot.set_state(&c, mu);
+  return pot.delta(l, new_occ);
+}
+
+double orc_sgc_potential(int64_t rows, int64_t cols, const int32_t *occ, double J,
+                         int64_t use_nlist, double mu) {
+  IsingConfiguration c(rows, cols, 1);
+  for (long i = 0; i < c.n_sites; ++i) c.occupation[i] = occ[i];
+  SemiGrandCanonicalPotential pot{IsingFormationEnergy(J, use_nlist != 0)};
+  pot.set_state(&c, mu);
+  return pot.per_supercell();
+}
+
+// Sequential random-site Metropolis on the generalised Ising lattices; writes
+// per-pass integer observables (S1, S2, M); the occupation is updated in place.
+int orc_lattice_run(int64_t ndim, const int64_t *L, double J1, double J2, double temperature,
+                    double mu, uint64_t seed, int64_t n_passes, int8_t *occ, int64_t *S1,
+                    int64_t *S2, int64_t *M) {
+  LatticeModel m;
+  m.ndim = (int)ndim;
+  for (int d = 0; d < 3; ++d) m.L[d] = d < ndim ? L[d] : 1;
+  m.J1 = J1;
+  m.J2 = J2;
+  std::vector<int8_t> s(occ, occ + m.n_sites());
+  std::mt19937_64 eng(seed);
+  std::vector<long long> a, b, c;
+  lattice_sequential_run(m, s, temperature, mu, eng, n_passes, a, b, c);
+  for (int64_t p = 0; p < n_passes; ++p) {
+    S1[p] = a[p];
+    S2[p] = b[p];
+    M[p] = c[p];
+  }
+  std::copy(s.begin(), s.end(), occ);
+  return 0;
+}
+
+void orc_lattice_observables(int64_t ndim, const int64_t *L, const int8_t *occ, int64_t *out3) {
+  LatticeModel m;
+  m.ndim = (int)ndim;
+  for (int d = 0; d < 3; ++d) m.L[d] = d < ndim ? L[d] : 1;
+  std::vector<int8_t> s(occ, occ + m.n_sites());
+  long long S1, S2, M;
+  lattice_observables(m, s, S1, S2, M);
+  out3[0] = S1;
+  out3[1] = S2;
+  out3[2] = M;
+}
+
+// Exact SGC Boltzmann averages by enumeration of all 2^N states (N <= 26).
+// out[0..5] = <E>, <E^2>, <Nx>, <(Nx)^2>, <E*Nx>, <E_f>, with E the total SGC
+// potential E_f - mu*Nx and Nx = (N+M)/2.
+int orc_lattice_enumerate(int64_t ndim, const int64_t *L, double J1, double J2,
+                          double temperature, double mu, double *out) {
+  LatticeModel m;
+  m.ndim = (int)ndim;
+  for (int d = 0; d < 3; ++d) m.L[d] = d < ndim ? L[d] : 1;
+  m.J1 = J1;
+  m.J2 = J2;
+  long N = m.n_sites();
+  if (N > 26) return 1;
+  // Enumerate bonds once.
+  std::vector<std::pair<long, long>> b1, b2;
+  for (long l = 0; l < N; ++l) {
+    long i, j, k;
+    m.coords(l, i, j, k);
+    for (auto const &o : m.nn_forward()) b1.push_back({l, m.index(i + o[0], j + o[1], k + o[2])});
+    for (auto const &o : m.nnn_forward()) b2.push_back({l, m.index(i + o[0], j + o[1], k + o[2])});
+  }
+  double beta = 1.0 / (KB * temperature);
+  long double Z = 0, sE = 0, sE2 = 0, sX = 0, sX2 = 0, sEX = 0, sEf = 0;
+  std::vector<double> Es(1L << N), Xs(1L << N), Efs(1L << N);
+  double Emin = 1e300;
+  for (long st = 0; st < (1L << N); ++st) {
+    auto spin = [&](long l) { return ((st >> l) & 1) ? 1 : -1; };
+    long long S1 = 0, S2 = 0, M = 0;
+    for (auto const &b : b1) S1 += spin(b.first) * spin(b.second);
+    for (auto const &b : b2) S2 += spin(b.first) * spin(b.second);
+    for (long l = 0; l < N; ++l) M += spin(l);
+    double Ef = -J1 * S1 - J2 * S2;
+    double Nx = (N + M) / 2.0;
+    double E = Ef - mu * Nx;
+    Es[st] = E;
+    Xs[st] = Nx;
+    Efs[st] = Ef;
+    Emin = std::min(Emin, E);
+  }
+  for (long st = 0; st < (1L << N); ++st) {
+    long double w = std::exp(-(long double)beta * (Es[st] - Emin));
+    Z += w;
+    sE += w * Es[st];
+    sE2 += w * Es[st] * Es[st];
+    sX += w * Xs[st];
+    sX2 += w * Xs[st] * Xs[st];
+    sEX += w * Es[st] * Xs[st];
+    sEf += w * Efs[st];
+  }
+  out[0] = (double)(sE / Z);
+  out[1] = (double)(sE2 / Z);
+  out[2] = (double)(sX / Z);
+  out[3] = (double)(sX2 / Z);
+  out[4] = (double)(sEX / Z);
+  out[5] = (double)(sEf / Z);
+  return 0;
+}
+
+}  // extern "C"

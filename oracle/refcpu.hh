@@ -1,0 +1,809 @@
+// refcpu.hh -- CPU restatement of libcasm-monte's occupation-Metropolis hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle: it is compiled into
+// oracle/lib/liboracle.so and is used by tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg, and nowhere else.  The product (the HIP library
+// under casmcode_monte_amd/) never includes, links or calls it.
+//
+// It restates, in plain C++17 over the host libstdc++ <random> and glibc exp,
+// the reference algorithm as read from these files (paths relative to the
+// reference repository root):
+//   include/casm/monte/ising_cpp/model.hh                 (configuration, energies)
+//   include/casm/monte/ising_cpp/basic_semigrand_canonical.hh (SGC potential, events)
+//   include/casm/monte/methods/metropolis.hh              (acceptance)
+//   include/casm/monte/methods/basic_occupation_metropolis.hh (main loop)
+//   include/casm/monte/methods/occupation_metropolis.hh   (RunManager loop)
+//   include/casm/monte/RandomNumberGenerator.hh, definitions.hh (RNG)
+//   include/casm/monte/sampling/Sampler.hh, SamplingParams.hh
+//   include/casm/monte/checks/{CompletionCheck,CutoffCheck,ConvergenceCheck}.hh
+//   src/casm/monte/checks/EquilibrationCheck.cc
+//   src/casm/monte/BasicStatistics.cc, include/casm/monte/misc/math.hh
+//   include/casm/monte/run_management/{RunManager,SamplingFixture}.hh
+// Each function cites the file:line it follows.
+//
+// Models that the reference does not define (2D with next-nearest J2, 3D simple
+// cubic, binary FCC cluster expansion) are defined here in the reference's
+// conventions (E = -J * sum over bonds of s_i s_j; SGC potential E - mu * N * x).
+// Their pinning is by exact enumeration of tiny lattices and closed forms, see
+// tests/test_oracle_models.py.
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <limits>
+#include <map>
+#include <memory>
+#include <optional>
+#include <random>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace refcpu {
+
+// Boltzmann constant in eV/K as compiled into CASMcode_global (value pinned by
+// this project; CASMcode_global is not available to read).
+constexpr double KB = 8.6173303e-05;
+
+typedef long CountType;        // definitions.hh:25
+typedef long long BigCountType;  // definitions.hh:26
+typedef long Index;
+
+// ---------------------------------------------------------------------------
+// RNG: RandomNumberGenerator.hh:14-42 over std::mt19937_64 (definitions.hh:17)
+// ---------------------------------------------------------------------------
+struct RandomNumberGenerator {
+  std::shared_ptr<std::mt19937_64> engine;
+  explicit RandomNumberGenerator(std::shared_ptr<std::mt19937_64> e)
+      : engine(std::move(e)) {
+    if (!engine) throw std::runtime_error("refcpu: null engine");
+  }
+  // RandomNumberGenerator.hh:31-34
+  long random_int(long maximum_value) {
+    return std::uniform_int_distribution<long>(0, maximum_value)(*engine);
+  }
+  // RandomNumberGenerator.hh:38-41
+  double random_real(double maximum_value) {
+    return std::uniform_real_distribution<double>(0, maximum_value)(*engine);
+  }
+};
+
+// metropolis.hh:25-35
+template <typename G>
+bool metropolis_acceptance(double delta_potential_energy, double beta, G &rng) {
+  if (delta_potential_energy < 0.0) return true;
+  double rand = rng.random_real(1.0);
+  double prob = std::exp(-delta_potential_energy * beta);
+  return rand < prob;
+}
+
+// ---------------------------------------------------------------------------
+// Ising model: model.hh:19-436
+// ---------------------------------------------------------------------------
+struct IsingConfiguration {  // model.hh:19-110
+  Index rows = 0, cols = 0;  // shape[0], shape[1]
+  Index n_sites = 0;
+  std::vector<int> occupation;  // column-major: l = rows*col + row
+  IsingConfiguration() = default;
+  IsingConfiguration(Index r, Index c, int fill = 1)
+      : rows(r), cols(c), n_sites(r * c), occupation(r * c, fill) {}
+  int occ(Index l) const { return occupation[l]; }
+  void set_occ(Index l, int v) { occupation[l] = v; }
+  Index within(Index index, int dim) const {  // model.hh:73-79
+    Index s = dim == 0 ? rows : cols;
+    Index r = index % s;
+    if (r < 0) r += s;
+    return r;
+  }
+  Index to_linear(Index i, Index j) const { return rows * j + i; }  // :93-99
+};
+
+struct IsingFormationEnergy {  // model.hh:164-380
+  double J = 1.0;
+  bool use_nlist = true;
+  IsingConfiguration const *config = nullptr;
+  std::vector<std::vector<Index>> nlist, flower_nlist;
+
+  IsingFormationEnergy(double _J, bool _use_nlist) : J(_J), use_nlist(_use_nlist) {}
+
+  void set_state(IsingConfiguration const *c) {  // model.hh:210-256
+    config = c;
+    if (!use_nlist) return;
+    nlist.assign(c->n_sites, {});
+    flower_nlist.assign(c->n_sites, {});
+    for (Index l = 0; l < c->n_sites; ++l) {
+      Index i = l % c->rows, j = l / c->rows;
+      Index in = c->within(i + 1, 0);
+      nlist[l].push_back(c->to_linear(in, j));
+      flower_nlist[l].push_back(c->to_linear(in, j));
+      Index jn = c->within(j + 1, 1);
+      nlist[l].push_back(c->to_linear(i, jn));
+      flower_nlist[l].push_back(c->to_linear(i, jn));
+      in = c->within(i - 1, 0);
+      flower_nlist[l].push_back(c->to_linear(in, j));
+      jn = c->within(j - 1, 1);
+      flower_nlist[l].push_back(c->to_linear(i, jn));
+    }
+  }
+
+  double per_supercell() const {  // model.hh:259-290
+    auto const &occ = config->occupation;
+    if (use_nlist) {
+      double e = 0.0;
+      for (Index l = 0; l < config->n_sites; ++l)
+        e += occ[l] * (occ[nlist[l][0]] + occ[nlist[l][1]]);
+      e *= -J;
+      return e;
+    }
+    Index rows = config->rows, cols = config->cols;
+    auto site = [&](Index i, Index j) { return occ[i + rows * j]; };
+    double e = 0.0;
+    for (Index i = 0; i < rows; ++i) {  // row(i) . row(i+1): integer dot
+      Index in = config->within(i + 1, 0);
+      long dot = 0;
+      for (Index j = 0; j < cols; ++j) dot += site(i, j) * site(in, j);
+      e += -J * static_cast<double>(dot);
+    }
+    for (Index j = 0; j < cols; ++j) {  // col(j) . col(j+1)
+      Index jn = config->within(j + 1, 1);
+      long dot = 0;
+      for (Index i = 0; i < rows; ++i) dot += site(i, j) * site(i, jn);
+      e += -J * static_cast<double>(dot);
+    }
+    return e;
+  }
+  double per_unitcell() const {  // model.hh:293-295
+    return per_supercell() / config->n_sites;
+  }
+  double single_delta(Index l, int new_occ) const {  // model.hh:305-345
+    auto const &occ = config->occupation;
+    if (use_nlist) {
+      return -J * (new_occ - occ[l]) *
+             (occ[flower_nlist[l][0]] + occ[flower_nlist[l][1]] +
+              occ[flower_nlist[l][2]] + occ[flower_nlist[l][3]]);
+    }
+    Index rows = config->rows;
+    Index i = l % rows, j = l / rows;
+    auto site = [&](Index a, Index b) { return occ[a + rows * b]; };
+    double ds = new_occ - site(i, j);
+    return -J * ds *
+           (site(i, config->within(j - 1, 1)) + site(i, config->within(j + 1, 1)) +
+            site(config->within(i - 1, 0), j) + site(config->within(i + 1, 0), j));
+  }
+};
+
+struct IsingParamComposition {  // model.hh:388-436
+  IsingConfiguration const *config = nullptr;
+  double per_supercell() const {  // :412-417
+    long sum = 0;
+    for (int v : config->occupation) sum += v;
+    return static_cast<double>(config->n_sites + sum) / 2.0;
+  }
+  double per_unitcell() const { return per_supercell() / config->n_sites; }  // :420-422
+  double delta(Index l, int new_occ) const {  // :425-435
+    double Ndx = 0.0;
+    Ndx += (new_occ - config->occ(l)) / 2.0;
+    return Ndx;
+  }
+};
+
+// basic_semigrand_canonical.hh:104-199 (one exchange potential component)
+struct SemiGrandCanonicalPotential {
+  IsingFormationEnergy formation_energy;
+  IsingParamComposition param_composition;
+  double mu = 0.0;
+  SemiGrandCanonicalPotential(IsingFormationEnergy fe) : formation_energy(std::move(fe)) {}
+  void set_state(IsingConfiguration const *c, double _mu) {
+    formation_energy.set_state(c);
+    param_composition.config = c;
+    mu = _mu;
+  }
+  double per_supercell() const {  // :165-169
+    return formation_energy.per_supercell() - mu * param_composition.per_supercell();
+  }
+  double per_unitcell() const {  // :172-174
+    return per_supercell() / formation_energy.config->n_sites;
+  }
+  double delta(Index l, int new_occ) const {  // :178-192
+    double dEf = formation_energy.single_delta(l, new_occ);
+    double Ndx = param_composition.delta(l, new_occ);
+    return dEf - mu * Ndx;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Sampling: Sampler.hh:30-112, 128-180
+// ---------------------------------------------------------------------------
+struct Sampler {
+  Index n_components = 1;
+  std::vector<std::vector<double>> rows;  // one row per sample
+  explicit Sampler(Index nc = 1) : n_components(nc) {}
+  void push_back(std::vector<double> const &v) { rows.push_back(v); }
+  CountType n_samples() const { return rows.size(); }
+  std::vector<double> component(Index c) const {
+    std::vector<double> out(rows.size());
+    for (size_t i = 0; i < rows.size(); ++i) out[i] = rows[i][c];
+    return out;
+  }
+};
+typedef std::map<std::string, std::shared_ptr<Sampler>> SamplerMap;
+
+inline CountType get_n_samples(SamplerMap const &s) {  // Sampler.hh:410-416
+  if (s.size()) return s.begin()->second->n_samples();
+  return 0;
+}
+
+struct SamplerComponent {  // Sampler.hh:128-149, 365-370
+  std::string sampler_name;
+  Index component_index = 0;
+  bool operator<(SamplerComponent const &o) const {
+    if (sampler_name == o.sampler_name) return component_index < o.component_index;
+    return sampler_name < o.sampler_name;
+  }
+};
+
+struct RequestedPrecision {  // Sampler.hh:151-180
+  bool abs_convergence_is_required = false;
+  double abs_precision = 0.0;
+  bool rel_convergence_is_required = false;
+  double rel_precision = 0.0;
+};
+
+// ---------------------------------------------------------------------------
+// Statistics: math.hh:10-72, BasicStatistics.cc:24-131
+// ---------------------------------------------------------------------------
+inline double mean_of(std::vector<double> const &x) {
+  // Eigen's mean() is sum()/size(); the sum is a reduction whose
+  // association order is Eigen's (vectorised); here a sequential sum.  The
+  // last-bit difference is why statistics are compared with a tolerance.
+  double s = 0.0;
+  for (double v : x) s += v;
+  return s / static_cast<double>(x.size());
+}
+
+inline double variance(std::vector<double> const &x, double m) {  // math.hh:31-39
+  double cov = 0.0;
+  for (double v : x) {
+    double d = v - m;
+    cov += d * d;
+  }
+  return cov / static_cast<double>(x.size());
+}
+
+inline double approx_erf_inv(double x) {  // math.hh:62-72 (Winitzki, a = 0.147)
+  const double one = 1.0;
+  const double PI = 3.141592653589793238463;
+  const double a = 0.147;
+  double sgn = (x < 0.0) ? -one : one;
+  double b = std::log((one - x) * (one + x));
+  double c = 2.0 / (PI * a) + b * 0.5;
+  double d = b / a;
+  return sgn * std::sqrt(std::sqrt(c * c - d) - c);
+}
+
+inline double autocorrelation_factor(std::vector<double> const &x) {  // BasicStatistics.cc:24-48
+  Index N = x.size();
+  double m = mean_of(x);
+  double v0 = variance(x, m);
+  if (std::abs(v0 / m) < 1e-8 || v0 == 0.0) return 1.0;
+  for (Index i = 1; i < N; ++i) {
+    Index range = N - i;
+    double cov = 0.0;
+    for (Index j = 0; j < range; ++j) cov += (x[j] - m) * (x[j + i] - m);
+    cov /= range;
+    if (std::abs(cov / v0) <= 0.5) {
+      double rho = std::pow(2.0, (-1.0 / (i * 1.0)));
+      return (1.0 + rho) / (1.0 - rho);
+    }
+  }
+  return std::numeric_limits<double>::max();
+}
+
+struct BasicStatistics {  // BasicStatistics.hh:24-36
+  double mean = 0.0;
+  double calculated_precision = std::numeric_limits<double>::max();
+};
+
+inline BasicStatistics basic_statistics(std::vector<double> const &x,
+                                        double confidence = 0.95) {  // BasicStatistics.cc:114-131
+  if (x.empty()) throw std::runtime_error("observations.size()==0");
+  BasicStatistics s;
+  s.mean = mean_of(x);
+  double v0 = variance(x, s.mean);
+  double f = autocorrelation_factor(x);
+  double fc = std::sqrt(2.0) * approx_erf_inv(confidence);
+  s.calculated_precision = fc * std::sqrt(f * v0 / x.size());
+  return s;
+}
+
+// EquilibrationCheck.cc:50-117
+struct EquilibrationResult {
+  bool is_equilibrated = false;
+  CountType N_samples_for_equilibration = 0;
+};
+
+inline EquilibrationResult default_equilibration_check_impl(std::vector<double> const &obs,
+                                                           double prec) {
+  if (obs.empty()) throw std::runtime_error("observations.size()==0");
+  EquilibrationResult r;
+  double eps = (obs[0] == 0.0) ? 1e-8 : std::abs(obs[0]) * 1e-8;
+  CountType N = obs.size();
+  bool is_even = ((N % 2) == 0);
+  bool all_same = true;
+  for (CountType i = 0; i < N; i++)
+    if (std::abs(obs[i] - obs[0]) > eps) {
+      all_same = false;
+      break;
+    }
+  if (all_same) {
+    r.is_equilibrated = true;
+    r.N_samples_for_equilibration = 0;
+    return r;
+  }
+  CountType start1 = 0;
+  CountType start2 = is_even ? N / 2 : (N / 2) + 1;
+  double sum1 = 0.0, sum2 = 0.0;
+  for (CountType i = 0; i < start2; ++i) sum1 += obs[i];
+  for (CountType i = start2; i < N; ++i) sum2 += obs[i];
+  while (std::abs((sum1 / (start2 - start1)) - (sum2 / (N - start2))) > prec &&
+         start1 < N - 2) {
+    if (is_even) {
+      sum1 -= obs[start1];
+      sum1 += obs[start2];
+      sum2 -= obs[start2];
+      start2++;
+    } else {
+      sum1 -= obs[start1];
+    }
+    start1++;
+    is_even = !is_even;
+  }
+  double mean_tot = (sum1 + sum2) / (N - start1);
+  if (obs[start1] < mean_tot) {
+    while (obs[start1] < mean_tot && start1 < N - 1) start1++;
+  } else {
+    while (obs[start1] > mean_tot && start1 < N - 1) start1++;
+  }
+  r.is_equilibrated = (start1 < N - 1);
+  r.N_samples_for_equilibration = start1;
+  return r;
+}
+
+inline EquilibrationResult default_equilibration_check(std::vector<double> const &obs,
+                                                      RequestedPrecision const &rp) {
+  // EquilibrationCheck.cc:119-162 (unweighted branch)
+  double prec;
+  if (rp.abs_convergence_is_required) {
+    prec = rp.abs_precision;
+  } else if (rp.rel_convergence_is_required) {
+    prec = std::abs(mean_of(obs) * rp.rel_precision);
+  } else {
+    EquilibrationResult r;
+    r.is_equilibrated = true;
+    return r;
+  }
+  return default_equilibration_check_impl(obs, prec);
+}
+
+struct EquilibrationCheckResults {  // EquilibrationCheck.hh:31-49
+  bool all_equilibrated = false;
+  CountType N_samples_for_all_to_equilibrate = 0;
+  std::map<SamplerComponent, EquilibrationResult> individual_results;
+};
+
+inline EquilibrationCheckResults equilibration_check(
+    std::map<SamplerComponent, RequestedPrecision> const &rp, SamplerMap const &samplers,
+    bool check_all) {  // EquilibrationCheck.cc:180-225
+  EquilibrationCheckResults res;
+  if (!rp.size()) return res;
+  res.all_equilibrated = true;
+  for (auto const &p : rp) {
+    auto it = samplers.find(p.first.sampler_name);
+    if (it == samplers.end()) throw std::runtime_error("sampler not found");
+    EquilibrationResult cur =
+        default_equilibration_check(it->second->component(p.first.component_index), p.second);
+    res.N_samples_for_all_to_equilibrate =
+        std::max(res.N_samples_for_all_to_equilibrate, cur.N_samples_for_equilibration);
+    res.all_equilibrated &= cur.is_equilibrated;
+    res.individual_results.emplace(p.first, cur);
+    if (!check_all && !res.all_equilibrated) break;
+  }
+  return res;
+}
+
+struct ConvergenceResult {  // ConvergenceCheck.hh:14-23
+  bool is_converged = false;
+  RequestedPrecision requested_precision;
+  BasicStatistics stats;
+};
+
+struct ConvergenceCheckResults {  // ConvergenceCheck.hh:39-59
+  bool all_converged = false;
+  CountType N_samples_for_statistics = 0;
+  std::map<SamplerComponent, ConvergenceResult> individual_results;
+};
+
+inline ConvergenceCheckResults convergence_check(
+    SamplerMap const &samplers, std::map<SamplerComponent, RequestedPrecision> const &rp,
+    CountType N_eq, double confidence) {  // ConvergenceCheck.hh:139-184
+  ConvergenceCheckResults res;
+  CountType N = get_n_samples(samplers);
+  if (!rp.size()) {
+    res.N_samples_for_statistics = N;
+    return res;
+  }
+  if (N_eq >= N) return res;
+  res.N_samples_for_statistics = N - N_eq;
+  res.all_converged = true;
+  for (auto const &p : rp) {
+    auto it = samplers.find(p.first.sampler_name);
+    if (it == samplers.end()) throw std::runtime_error("sampler not found");
+    std::vector<double> comp = it->second->component(p.first.component_index);
+    std::vector<double> tail(comp.end() - res.N_samples_for_statistics, comp.end());
+    ConvergenceResult cur;
+    cur.stats = basic_statistics(tail, confidence);
+    cur.requested_precision = p.second;
+    cur.is_converged = true;  // ConvergenceCheck.hh:74-90
+    if (p.second.abs_convergence_is_required)
+      cur.is_converged &= cur.stats.calculated_precision < p.second.abs_precision;
+    if (p.second.rel_convergence_is_required)
+      cur.is_converged &=
+          std::abs(cur.stats.calculated_precision / cur.stats.mean) < p.second.rel_precision;
+    res.all_converged &= cur.is_converged;
+    res.individual_results.emplace(p.first, cur);
+  }
+  return res;
+}
+
+// CutoffCheck.hh:14-94
+struct CutoffCheckParams {
+  std::optional<CountType> min_count, min_sample, max_count, max_sample;
+  std::optional<double> min_time, min_clocktime, max_time, max_clocktime;
+};
+
+inline bool all_minimums_met(CutoffCheckParams const &p, std::optional<CountType> count,
+                             std::optional<double> time, CountType n_samples,
+                             double clocktime) {
+  if (p.min_sample.has_value() && n_samples < *p.min_sample) return false;
+  if (p.min_count.has_value() && count.has_value() && *count < *p.min_count) return false;
+  if (p.min_time.has_value() && time.has_value() && *time < *p.min_time) return false;
+  if (p.min_clocktime.has_value() && clocktime < *p.min_clocktime) return false;
+  return true;
+}
+
+inline bool any_maximum_met(CutoffCheckParams const &p, std::optional<CountType> count,
+                            std::optional<double> time, CountType n_samples,
+                            double clocktime) {
+  if (p.max_sample.has_value() && n_samples >= *p.max_sample) return true;
+  if (p.max_count.has_value() && count.has_value() && *count >= *p.max_count) return true;
+  if (p.max_time.has_value() && time.has_value() && *time >= *p.max_time) return true;
+  if (p.max_clocktime.has_value() && clocktime >= *p.max_clocktime) return true;
+  return false;
+}
+
+struct CompletionCheckParams {  // CompletionCheck.hh:19-100
+  CutoffCheckParams cutoff_params;
+  std::map<SamplerComponent, RequestedPrecision> requested_precision;
+  double confidence = 0.95;
+  bool log_spacing = false;
+  CountType check_begin = 100;
+  CountType check_period = 100;
+  double check_base = 10.0;
+  double check_shift = 2.0;
+  CountType check_period_max = 10000;
+
+  CountType sample_check_linear(Index n) const { return check_begin + check_period * n; }
+  CountType sample_check_log(Index n) const {
+    return check_begin +
+           static_cast<CountType>(std::round(std::pow(check_base, (n + check_shift))));
+  }
+  Index find_n_begin_linear() const {
+    Index n = 0;
+    auto delta = [&](Index k) { return sample_check_log(k) - sample_check_log(k - 1); };
+    while (delta(n + 1) <= check_period_max) n += 1;
+    return n;
+  }
+  CountType sample_check_log(Index n, Index n_begin_linear) const {
+    if (n <= n_begin_linear) return sample_check_log(n);
+    return sample_check_log(n_begin_linear) + check_period_max * (n - n_begin_linear);
+  }
+};
+
+struct CompletionCheckResults {  // CompletionCheck.hh:103-171
+  std::optional<CountType> count;
+  double clocktime = 0.0;
+  CountType n_samples = 0;
+  bool has_all_minimums_met = false;
+  bool has_any_maximum_met = false;
+  std::optional<CountType> n_samples_at_convergence_check;
+  EquilibrationCheckResults equilibration_check_results;
+  ConvergenceCheckResults convergence_check_results;
+  bool is_complete = false;
+};
+
+class CompletionCheck {  // CompletionCheck.hh:174-376
+ public:
+  explicit CompletionCheck(CompletionCheckParams p)
+      : m_params(std::move(p)), m_n_begin_linear(m_params.find_n_begin_linear()) {
+    m_t0 = std::chrono::steady_clock::now();
+  }
+  CompletionCheckParams const &params() const { return m_params; }
+  CompletionCheckResults const &results() const { return m_results; }
+  Index n_checks() const { return m_n_checks; }
+
+  bool is_complete(SamplerMap const &samplers, std::optional<CountType> count) {
+    CountType n_samples = get_n_samples(samplers);
+    double clocktime = m_last_clocktime;
+    if (n_samples != m_last_n_samples) {
+      clocktime = time_s();
+      m_last_n_samples = n_samples;
+      m_last_clocktime = clocktime;
+    }
+    // partial_reset (CompletionCheck.hh:145-156)
+    m_results.count = count;
+    m_results.clocktime = clocktime;
+    m_results.n_samples = n_samples;
+    m_results.has_all_minimums_met = false;
+    m_results.has_any_maximum_met = false;
+    m_results.is_complete = false;
+
+    m_results.has_all_minimums_met =
+        all_minimums_met(m_params.cutoff_params, count, std::nullopt, n_samples, clocktime);
+    if (!m_results.has_all_minimums_met) return false;
+    m_results.has_any_maximum_met =
+        any_maximum_met(m_params.cutoff_params, count, std::nullopt, n_samples, clocktime);
+    if (m_results.has_any_maximum_met) {
+      m_results.is_complete = true;
+      if (n_samples != m_results.n_samples_at_convergence_check)
+        check_convergence(samplers, n_samples);
+      return true;
+    }
+    Index check_at = m_params.log_spacing
+                         ? m_params.sample_check_log(m_n_checks, m_n_begin_linear)
+                         : m_params.sample_check_linear(m_n_checks);
+    if (n_samples >= check_at) {
+      m_n_checks += 1;
+      check_convergence(samplers, n_samples);
+    }
+    if (m_results.convergence_check_results.all_converged) m_results.is_complete = true;
+    return m_results.is_complete;
+  }
+
+ private:
+  void check_convergence(SamplerMap const &samplers, CountType n_samples) {
+    if (m_params.requested_precision.size()) {
+      m_results.n_samples_at_convergence_check = n_samples;
+      m_results.equilibration_check_results =
+          equilibration_check(m_params.requested_precision, samplers, false);
+      if (m_results.equilibration_check_results.all_equilibrated) {
+        m_results.convergence_check_results = convergence_check(
+            samplers, m_params.requested_precision,
+            m_results.equilibration_check_results.N_samples_for_all_to_equilibrate,
+            m_params.confidence);
+      } else {
+        m_results.convergence_check_results = ConvergenceCheckResults();
+      }
+    }
+  }
+  double time_s() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - m_t0).count();
+  }
+  CompletionCheckParams m_params;
+  CompletionCheckResults m_results;
+  Index m_n_checks = 0;
+  Index m_n_begin_linear = 0;
+  Index m_last_n_samples = 0;
+  double m_last_clocktime = 0.0;
+  std::chrono::steady_clock::time_point m_t0;
+};
+
+// ---------------------------------------------------------------------------
+// The main loop: basic_occupation_metropolis.hh:354-425 with the SGC Ising
+// calculator of basic_semigrand_canonical.hh:415-470 and its three default
+// sampling functions (:485-590).
+// ---------------------------------------------------------------------------
+struct TraceEntry {
+  long site;
+  int old_occ;
+  int nbr_sum;
+  int drew;       // 1 if a uniform was drawn (dE >= 0)
+  int accepted;
+};
+
+struct SgcRunOutput {
+  SamplerMap samplers;           // formation_energy, param_composition, potential_energy
+  std::vector<long long> S, M;   // integer bond sum and magnetisation at each sample
+  CountType n_pass = 0;
+  BigCountType n_accept = 0, n_reject = 0;
+  CompletionCheckResults results;
+  Index n_checks = 0;
+  std::vector<TraceEntry> trace;
+  IsingConfiguration final_config;
+};
+
+inline long long bond_sum(IsingConfiguration const &c) {
+  long long s = 0;
+  for (Index j = 0; j < c.cols; ++j)
+    for (Index i = 0; i < c.rows; ++i) {
+      int v = c.occupation[i + c.rows * j];
+      s += v * (c.occupation[c.within(i + 1, 0) + c.rows * j] +
+                c.occupation[i + c.rows * c.within(j + 1, 1)]);
+    }
+  return s;
+}
+
+inline long long magnetisation(IsingConfiguration const &c) {
+  long long m = 0;
+  for (int v : c.occupation) m += v;
+  return m;
+}
+
+inline SgcRunOutput sgc_basic_run(IsingConfiguration config, double J, bool use_nlist,
+                                  double temperature, double mu,
+                                  std::shared_ptr<std::mt19937_64> engine,
+                                  CompletionCheckParams const &ccp, int sample_period,
+                                  size_t trace_len = 0) {
+  SgcRunOutput out;
+  SemiGrandCanonicalPotential potential{IsingFormationEnergy(J, use_nlist)};
+  potential.set_state(&config, mu);
+  CountType n_steps_per_pass = config.n_sites;  // basic_semigrand_canonical.hh:432
+  Index max_l = config.n_sites - 1;              // :304
+
+  for (auto name : {"formation_energy", "param_composition", "potential_energy"})
+    out.samplers.emplace(name, std::make_shared<Sampler>(1));
+  CompletionCheck cc(ccp);
+
+  double beta = 1.0 / (KB * temperature);  // basic_occupation_metropolis.hh:361
+  RandomNumberGenerator rng(engine);
+  Index n_pass_next_sample = sample_period;
+  CountType n_step = 0;
+  CountType n_pass = 0;
+  while (!cc.is_complete(out.samplers, n_pass)) {  // :381-382
+    long l = rng.random_int(max_l);                // propose, basic_semigrand_canonical.hh:308-315
+    int new_occ = -config.occ(l);
+    double dE = potential.delta(l, new_occ);  // :387
+    bool drew = !(dE < 0.0);
+    bool acc = metropolis_acceptance(dE, beta, rng);
+    if (out.trace.size() < trace_len) {
+      int nsum = 0;
+      Index i = l % config.rows, j = l / config.rows;
+      nsum = config.occ(config.to_linear(config.within(i + 1, 0), j)) +
+             config.occ(config.to_linear(config.within(i - 1, 0), j)) +
+             config.occ(config.to_linear(i, config.within(j + 1, 1))) +
+             config.occ(config.to_linear(i, config.within(j - 1, 1)));
+      out.trace.push_back({l, config.occ(l), nsum, drew ? 1 : 0, acc ? 1 : 0});
+    }
+    if (acc) {
+      out.n_accept++;
+      config.set_occ(l, new_occ);
+    } else {
+      out.n_reject++;
+    }
+    n_step++;
+    if (n_step == n_steps_per_pass) {
+      n_step = 0;
+      n_pass += 1;
+    }
+    if (n_pass == n_pass_next_sample) {  // :406-421
+      n_pass_next_sample += sample_period;
+      out.samplers.at("formation_energy")->push_back({potential.formation_energy.per_unitcell()});
+      out.samplers.at("param_composition")->push_back({potential.param_composition.per_unitcell()});
+      out.samplers.at("potential_energy")->push_back({potential.per_unitcell()});
+      out.S.push_back(bond_sum(config));
+      out.M.push_back(magnetisation(config));
+    }
+  }
+  out.n_pass = n_pass;
+  out.results = cc.results();
+  out.n_checks = cc.n_checks();
+  out.final_config = config;
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// Build-defined Ising models for the statistical modes: 2D square and 3D
+// simple cubic lattices with nearest (J1) and next-nearest (J2) couplings,
+// E_f = -J1 sum_NN s_i s_j - J2 sum_NNN s_i s_j, E_sgc = E_f - mu*(N+M)/2.
+// Linear index: l = i + L0*(j + L1*k) (first index fastest, as in model.hh).
+// ---------------------------------------------------------------------------
+struct LatticeModel {
+  int ndim = 2;
+  long L[3] = {1, 1, 1};
+  double J1 = 0.1, J2 = 0.0;
+  long n_sites() const { return L[0] * L[1] * L[2]; }
+  // NN offsets and NNN offsets (each bond listed once in the "forward" list)
+  std::vector<std::array<int, 3>> nn_forward() const {
+    if (ndim == 2) return {{1, 0, 0}, {0, 1, 0}};
+    return {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  }
+  std::vector<std::array<int, 3>> nnn_forward() const {
+    if (ndim == 2) return {{1, 1, 0}, {1, -1, 0}};
+    return {{1, 1, 0}, {1, -1, 0}, {1, 0, 1}, {1, 0, -1}, {0, 1, 1}, {0, 1, -1}};
+  }
+  long wrap(long a, int d) const {
+    long r = a % L[d];
+    return r < 0 ? r + L[d] : r;
+  }
+  long index(long i, long j, long k) const {
+    return wrap(i, 0) + L[0] * (wrap(j, 1) + L[1] * wrap(k, 2));
+  }
+  void coords(long l, long &i, long &j, long &k) const {
+    i = l % L[0];
+    j = (l / L[0]) % L[1];
+    k = l / (L[0] * L[1]);
+  }
+};
+
+// Neighbour sums at site l: nn and nnn (over all neighbours, both directions)
+inline void neighbour_sums(LatticeModel const &m, std::vector<int8_t> const &s, long l,
+                           int &nn, int &nnn) {
+  long i, j, k;
+  m.coords(l, i, j, k);
+  nn = 0;
+  nnn = 0;
+  for (auto const &o : m.nn_forward()) {
+    nn += s[m.index(i + o[0], j + o[1], k + o[2])];
+    nn += s[m.index(i - o[0], j - o[1], k - o[2])];
+  }
+  if (m.J2 != 0.0) {
+    for (auto const &o : m.nnn_forward()) {
+      nnn += s[m.index(i + o[0], j + o[1], k + o[2])];
+      nnn += s[m.index(i - o[0], j - o[1], k - o[2])];
+    }
+  }
+}
+
+// Integer observables: S1 = sum_NN s s, S2 = sum_NNN s s, M = sum s
+inline void lattice_observables(LatticeModel const &m, std::vector<int8_t> const &s,
+                                long long &S1, long long &S2, long long &M) {
+  S1 = S2 = M = 0;
+  long N = m.n_sites();
+  for (long l = 0; l < N; ++l) {
+    long i, j, k;
+    m.coords(l, i, j, k);
+    M += s[l];
+    for (auto const &o : m.nn_forward()) S1 += s[l] * s[m.index(i + o[0], j + o[1], k + o[2])];
+    for (auto const &o : m.nnn_forward()) S2 += s[l] * s[m.index(i + o[0], j + o[1], k + o[2])];
+  }
+}
+
+// Sequential random-site Metropolis on a LatticeModel (the reference's
+// single-site-flip SGC loop generalised to J2 and 3D), recording the integer
+// observables after every pass.
+inline void lattice_sequential_run(LatticeModel const &m, std::vector<int8_t> &s,
+                                   double temperature, double mu, std::mt19937_64 &eng,
+                                   long n_passes, std::vector<long long> &S1_out,
+                                   std::vector<long long> &S2_out,
+                                   std::vector<long long> &M_out) {
+  RandomNumberGenerator rng(std::shared_ptr<std::mt19937_64>(&eng, [](std::mt19937_64 *) {}));
+  double beta = 1.0 / (KB * temperature);
+  long N = m.n_sites();
+  long long S1, S2, M;
+  lattice_observables(m, s, S1, S2, M);
+  for (long p = 0; p < n_passes; ++p) {
+    for (long step = 0; step < N; ++step) {
+      long l = rng.random_int(N - 1);
+      int nn, nnn;
+      neighbour_sums(m, s, l, nn, nnn);
+      int so = s[l];
+      double ds = -2.0 * so;
+      double dE = -m.J1 * ds * nn - m.J2 * ds * nnn - mu * (ds / 2.0);
+      if (metropolis_acceptance(dE, beta, rng)) {
+        s[l] = static_cast<int8_t>(-so);
+        S1 += -2LL * so * nn;
+        S2 += -2LL * so * nnn;
+        M += -2LL * so;
+      }
+    }
+    S1_out.push_back(S1);
+    S2_out.push_back(S2);
+    M_out.push_back(M);
+  }
+}
+
+}  // namespace refcpu

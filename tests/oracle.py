@@ -1,0 +1,246 @@
+"""ctypes bindings of the oracle (oracle/lib/liboracle.so).
+
+Test infrastructure: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg only.  The oracle restates the reference algorithm on the CPU
+(see oracle/refcpu.hh for the reference file:line each function follows).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "lib", "liboracle.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run __graft_entry__.build())")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class SgcParams(C.Structure):
+    _fields_ = [
+        ("rows", C.c_int64), ("cols", C.c_int64), ("J", C.c_double),
+        ("use_nlist", C.c_int64), ("temperature", C.c_double), ("mu", C.c_double),
+        ("seed", C.c_uint64), ("sample_period", C.c_int64),
+        ("init_occ", C.POINTER(C.c_int32)),
+        ("min_count", C.c_int64), ("min_sample", C.c_int64),
+        ("max_count", C.c_int64), ("max_sample", C.c_int64),
+        ("check_begin", C.c_int64), ("check_period", C.c_int64), ("log_spacing", C.c_int64),
+        ("check_base", C.c_double), ("check_shift", C.c_double),
+        ("check_period_max", C.c_int64),
+        ("prec_formation_energy", C.c_double), ("prec_param_composition", C.c_double),
+        ("prec_potential_energy", C.c_double),
+        ("trace_len", C.c_int64),
+    ]
+
+
+class SgcOutput(C.Structure):
+    _fields_ = [
+        ("capacity", C.c_int64),
+        ("formation_energy", C.POINTER(C.c_double)),
+        ("param_composition", C.POINTER(C.c_double)),
+        ("potential_energy", C.POINTER(C.c_double)),
+        ("S", C.POINTER(C.c_int64)), ("M", C.POINTER(C.c_int64)),
+        ("final_occ", C.POINTER(C.c_int32)),
+        ("trace", C.POINTER(C.c_int64)),
+        ("engine", C.POINTER(C.c_uint64)),
+        ("n_samples", C.c_int64), ("n_pass", C.c_int64), ("n_accept", C.c_int64),
+        ("n_reject", C.c_int64), ("n_checks", C.c_int64),
+        ("is_complete", C.c_int64), ("all_equilibrated", C.c_int64), ("N_eq", C.c_int64),
+        ("all_converged", C.c_int64), ("N_stats", C.c_int64),
+        ("mean", C.c_double * 3), ("prec", C.c_double * 3), ("converged", C.c_int64 * 3),
+    ]
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _declare(L):
+    L.orc_sgc_run.argtypes = [C.POINTER(SgcParams), C.POINTER(SgcOutput)]
+    L.orc_sgc_run.restype = C.c_int
+    L.orc_mt64_outputs.argtypes = [C.c_uint64, C.c_int64, C.POINTER(C.c_uint64)]
+    L.orc_mt64_state.argtypes = [C.c_uint64, C.c_int64, C.POINTER(C.c_uint64)]
+    L.orc_uniform_int_replay.argtypes = [C.POINTER(C.c_uint64), C.c_int64, C.c_long,
+                                         C.POINTER(C.c_int64)]
+    L.orc_uniform_int_replay.restype = C.c_long
+    L.orc_canonical.argtypes = [C.c_uint64]
+    L.orc_canonical.restype = C.c_double
+    L.orc_accept_prob.argtypes = [C.c_double, C.c_double]
+    L.orc_accept_prob.restype = C.c_double
+    L.orc_kb.restype = C.c_double
+    L.orc_basic_statistics.argtypes = [C.POINTER(C.c_double), C.c_int64, C.c_double,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.orc_autocorrelation_factor.argtypes = [C.POINTER(C.c_double), C.c_int64]
+    L.orc_autocorrelation_factor.restype = C.c_double
+    L.orc_equilibration.argtypes = [C.POINTER(C.c_double), C.c_int64, C.c_double,
+                                    C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.orc_approx_erf_inv.argtypes = [C.c_double]
+    L.orc_approx_erf_inv.restype = C.c_double
+    L.orc_ising_energy.argtypes = [C.c_int64, C.c_int64, C.POINTER(C.c_int32), C.c_double,
+                                   C.c_int64]
+    L.orc_ising_energy.restype = C.c_double
+    L.orc_ising_delta.argtypes = [C.c_int64, C.c_int64, C.POINTER(C.c_int32), C.c_double,
+                                  C.c_int64, C.c_double, C.c_int64, C.c_int32]
+    L.orc_ising_delta.restype = C.c_double
+    L.orc_sgc_potential.argtypes = [C.c_int64, C.c_int64, C.POINTER(C.c_int32), C.c_double,
+                                    C.c_int64, C.c_double]
+    L.orc_sgc_potential.restype = C.c_double
+    L.orc_lattice_run.argtypes = [C.c_int64, C.POINTER(C.c_int64), C.c_double, C.c_double,
+                                  C.c_double, C.c_double, C.c_uint64, C.c_int64,
+                                  C.POINTER(C.c_int8), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.orc_lattice_observables.argtypes = [C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int8),
+                                          C.POINTER(C.c_int64)]
+    L.orc_lattice_enumerate.argtypes = [C.c_int64, C.POINTER(C.c_int64), C.c_double,
+                                        C.c_double, C.c_double, C.c_double,
+                                        C.POINTER(C.c_double)]
+
+
+KB = None
+
+
+def kb():
+    return lib().orc_kb()
+
+
+def mt64_outputs(seed, n):
+    out = np.zeros(n, dtype=np.uint64)
+    lib().orc_mt64_outputs(seed, n, _p(out, C.c_uint64))
+    return out
+
+
+def mt64_state(seed, n_discard):
+    out = np.zeros(313, dtype=np.uint64)
+    lib().orc_mt64_state(seed, n_discard, _p(out, C.c_uint64))
+    return out
+
+
+def uniform_int_replay(words, maximum):
+    w = np.ascontiguousarray(words, dtype=np.uint64)
+    used = C.c_int64(0)
+    v = lib().orc_uniform_int_replay(_p(w, C.c_uint64), len(w), maximum, C.byref(used))
+    return v, used.value
+
+
+def canonical(x):
+    return lib().orc_canonical(int(x))
+
+
+def accept_prob(dE, T):
+    return lib().orc_accept_prob(dE, T)
+
+
+def basic_statistics(x, confidence=0.95):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    m, p = C.c_double(), C.c_double()
+    lib().orc_basic_statistics(_p(x, C.c_double), len(x), confidence, C.byref(m), C.byref(p))
+    return m.value, p.value
+
+
+def autocorrelation_factor(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().orc_autocorrelation_factor(_p(x, C.c_double), len(x))
+
+
+def equilibration(x, prec):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    a, b = C.c_int64(), C.c_int64()
+    lib().orc_equilibration(_p(x, C.c_double), len(x), prec, C.byref(a), C.byref(b))
+    return bool(a.value), b.value
+
+
+def ising_energy(occ2d_colmajor, rows, cols, J, use_nlist):
+    o = np.ascontiguousarray(occ2d_colmajor, dtype=np.int32)
+    return lib().orc_ising_energy(rows, cols, _p(o, C.c_int32), J, int(use_nlist))
+
+
+def ising_delta(occ, rows, cols, J, use_nlist, mu, l, new_occ):
+    o = np.ascontiguousarray(occ, dtype=np.int32)
+    return lib().orc_ising_delta(rows, cols, _p(o, C.c_int32), J, int(use_nlist), mu, l, new_occ)
+
+
+def sgc_potential(occ, rows, cols, J, use_nlist, mu):
+    o = np.ascontiguousarray(occ, dtype=np.int32)
+    return lib().orc_sgc_potential(rows, cols, _p(o, C.c_int32), J, int(use_nlist), mu)
+
+
+def sgc_run(rows, cols, J=0.1, use_nlist=True, temperature=2000.0, mu=0.0, seed=42,
+            sample_period=1, init_occ=None, min_count=-1, min_sample=-1, max_count=-1,
+            max_sample=-1, check_begin=100, check_period=100, log_spacing=False,
+            check_base=10.0, check_shift=2.0, check_period_max=10000,
+            prec=(0.0, 0.0, 0.0), trace_len=0, capacity=None):
+    """Run the restated SGC Ising loop; returns a dict of numpy arrays/scalars."""
+    p = SgcParams()
+    p.rows, p.cols, p.J, p.use_nlist = rows, cols, J, int(use_nlist)
+    p.temperature, p.mu, p.seed, p.sample_period = temperature, mu, seed, sample_period
+    occ_keep = None
+    if init_occ is not None:
+        occ_keep = np.ascontiguousarray(init_occ, dtype=np.int32)
+        p.init_occ = _p(occ_keep, C.c_int32)
+    p.min_count, p.min_sample, p.max_count, p.max_sample = min_count, min_sample, max_count, max_sample
+    p.check_begin, p.check_period, p.log_spacing = check_begin, check_period, int(log_spacing)
+    p.check_base, p.check_shift, p.check_period_max = check_base, check_shift, check_period_max
+    p.prec_formation_energy, p.prec_param_composition, p.prec_potential_energy = prec
+    p.trace_len = trace_len
+    if capacity is None:
+        capacity = (max_count if max_count > 0 else 100000) // max(sample_period, 1) + 10
+    fe = np.zeros(capacity); pc = np.zeros(capacity); pe = np.zeros(capacity)
+    S = np.zeros(capacity, dtype=np.int64); M = np.zeros(capacity, dtype=np.int64)
+    fin = np.zeros(rows * cols, dtype=np.int32)
+    tr = np.zeros(max(trace_len, 1) * 5, dtype=np.int64)
+    eng = np.zeros(313, dtype=np.uint64)
+    o = SgcOutput()
+    o.capacity = capacity
+    o.formation_energy, o.param_composition = _p(fe, C.c_double), _p(pc, C.c_double)
+    o.potential_energy = _p(pe, C.c_double)
+    o.S, o.M = _p(S, C.c_int64), _p(M, C.c_int64)
+    o.final_occ, o.trace, o.engine = _p(fin, C.c_int32), _p(tr, C.c_int64), _p(eng, C.c_uint64)
+    rc = lib().orc_sgc_run(C.byref(p), C.byref(o))
+    if rc != 0:
+        raise RuntimeError(f"orc_sgc_run failed rc={rc} (n_samples={o.n_samples})")
+    n = o.n_samples
+    return dict(
+        formation_energy=fe[:n], param_composition=pc[:n], potential_energy=pe[:n],
+        S=S[:n], M=M[:n], final_occ=fin, trace=tr[: trace_len * 5].reshape(-1, 5),
+        engine=eng, n_samples=n, n_pass=o.n_pass, n_accept=o.n_accept, n_reject=o.n_reject,
+        n_checks=o.n_checks, is_complete=bool(o.is_complete),
+        all_equilibrated=bool(o.all_equilibrated), N_eq=o.N_eq,
+        all_converged=bool(o.all_converged), N_stats=o.N_stats,
+        mean=list(o.mean), prec=list(o.prec), converged=list(o.converged),
+    )
+
+
+def lattice_run(ndim, L, J1, J2, T, mu, seed, n_passes, occ):
+    Ls = np.array(list(L) + [1] * (3 - len(L)), dtype=np.int64)
+    o = np.ascontiguousarray(occ, dtype=np.int8).copy()
+    S1 = np.zeros(n_passes, dtype=np.int64); S2 = np.zeros(n_passes, dtype=np.int64)
+    M = np.zeros(n_passes, dtype=np.int64)
+    lib().orc_lattice_run(ndim, _p(Ls, C.c_int64), J1, J2, T, mu, seed, n_passes,
+                          _p(o, C.c_int8), _p(S1, C.c_int64), _p(S2, C.c_int64), _p(M, C.c_int64))
+    return o, S1, S2, M
+
+
+def lattice_observables(ndim, L, occ):
+    Ls = np.array(list(L) + [1] * (3 - len(L)), dtype=np.int64)
+    o = np.ascontiguousarray(occ, dtype=np.int8)
+    out = np.zeros(3, dtype=np.int64)
+    lib().orc_lattice_observables(ndim, _p(Ls, C.c_int64), _p(o, C.c_int8), _p(out, C.c_int64))
+    return tuple(int(v) for v in out)
+
+
+def lattice_enumerate(ndim, L, J1, J2, T, mu):
+    Ls = np.array(list(L) + [1] * (3 - len(L)), dtype=np.int64)
+    out = np.zeros(6)
+    rc = lib().orc_lattice_enumerate(ndim, _p(Ls, C.c_int64), J1, J2, T, mu, _p(out, C.c_double))
+    if rc:
+        raise ValueError("lattice too large to enumerate")
+    return dict(E=out[0], E2=out[1], Nx=out[2], Nx2=out[3], ENx=out[4], Ef=out[5])

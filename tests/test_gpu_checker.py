@@ -1,0 +1,154 @@
+"""CHECKERBOARD mode on the GPU.
+
+Bit-identity properties (the Philox counter depends only on chain seed, pass,
+colour and site):
+  - the LDS-resident kernel and the HBM work-list kernels give the same
+    trajectory;
+  - a chain run alone and the same chain run in a batch of uneven chains give
+    the same trajectory.
+Statistics (seeded, so deterministic):
+  - exact enumeration of tiny lattices (2D NN+NNN, 3D NN+NNN);
+  - Onsager's exact internal energy of the 2D nearest-neighbour model;
+  - agreement within 3 sigma with the oracle's sequential random-site runs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import casmcode_monte_amd as cm
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+KB = cm.KB
+
+
+def energy_terms(rec, J1, J2, mu, N):
+    Ef = -J1 * rec["S1"].astype(float) - J2 * rec["S2"].astype(float)
+    Nx = (N + rec["M"]) / 2.0
+    return Ef - mu * Nx, Nx
+
+
+def run(chains, lattice, J1, J2, passes, options=0, sp=1):
+    r = cm.Run(chains, mode=cm.MODE_CHECKERBOARD, lattice=lattice, J1=J1, J2=J2,
+               sample_period=sp, options=options)
+    r.run(passes)
+    return r
+
+
+@pytest.mark.parametrize("lattice,shape,J2", [
+    (cm.LATTICE_SQUARE, (16, 16), -0.02),
+    (cm.LATTICE_SQUARE, (18, 10), 0.0),
+    (cm.LATTICE_SC, (8, 6, 4), -0.02),
+    (cm.LATTICE_SC, (6, 6, 6), 0.0),
+])
+def test_lds_and_streaming_paths_identical(lattice, shape, J2):
+    print("case: lds vs streaming", lattice, shape, J2, flush=True)
+    ch = [dict(shape=shape, temperature=2200.0, exchange_potential=0.05, seed=77)]
+    a = run(ch, lattice, 0.1, J2, 40)
+    b = run(ch, lattice, 0.1, J2, 40, options=cm.OPT_FORCE_STREAMING)
+    np.testing.assert_array_equal(a.get_occupation(0), b.get_occupation(0))
+    np.testing.assert_array_equal(a.samples(0), b.samples(0))
+    assert a.counters(0) == b.counters(0)
+    # integer observables track the lattice exactly
+    S1, S2, M = O.lattice_observables(3 if lattice == cm.LATTICE_SC else 2, shape,
+                                      a.get_occupation(0))
+    last = a.samples(0)[-1]
+    assert (last["S1"], last["M"]) == (S1, M)
+    if J2 != 0.0:
+        assert last["S2"] == S2
+
+
+@pytest.mark.parametrize("streaming", [0, cm.OPT_FORCE_STREAMING])
+def test_batched_equals_alone(streaming):
+    chains = [dict(shape=(8, 8), temperature=1500.0, exchange_potential=0.2, seed=5),
+              dict(shape=(32, 16), temperature=3000.0, exchange_potential=-0.1, seed=6),
+              dict(shape=(20, 12), temperature=2600.0, exchange_potential=0.0, seed=7)]
+    print("case: batched vs alone streaming", streaming, flush=True)
+    batch = run(chains, cm.LATTICE_SQUARE, 0.1, -0.02, 30, options=streaming)
+    for i, ch in enumerate(chains):
+        alone = run([ch], cm.LATTICE_SQUARE, 0.1, -0.02, 30, options=streaming)
+        np.testing.assert_array_equal(batch.get_occupation(i), alone.get_occupation(0))
+        np.testing.assert_array_equal(batch.samples(i), alone.samples(0))
+
+
+def chain_means(r, n_chains, J1, J2, mu, N, drop):
+    es, xs = [], []
+    for c in range(n_chains):
+        rec = r.samples(c)[drop:]
+        E, Nx = energy_terms(rec, J1, J2, mu, N)
+        es.append(E.mean())
+        xs.append(Nx.mean())
+    return np.array(es), np.array(xs)
+
+
+@pytest.mark.parametrize("ndim,L,J2,T,mu", [
+    (2, (4, 4), -0.02, 3000.0, 0.1),
+    (2, (4, 4), 0.0, 2000.0, -0.05),
+    (3, (2, 2, 4), -0.02, 4000.0, 0.0),
+])
+def test_exact_enumeration(ndim, L, J2, T, mu):
+    J1 = 0.1
+    N = int(np.prod(L))
+    ex = O.lattice_enumerate(ndim, L, J1, J2, T, mu)
+    n_chains, passes, drop = 512, 3000, 200
+    print("case: enumeration", ndim, L, J2, T, mu, flush=True)
+    chains = [dict(shape=L, temperature=T, exchange_potential=mu, seed=1000 + i)
+              for i in range(n_chains)]
+    r = run(chains, cm.LATTICE_SC if ndim == 3 else cm.LATTICE_SQUARE, J1, J2, passes)
+    es, xs = chain_means(r, n_chains, J1, J2, mu, N, drop)
+    for got, exact in ((es, ex["E"]), (xs, ex["Nx"])):
+        se = got.std(ddof=1) / math.sqrt(n_chains)
+        assert abs(got.mean() - exact) < 4 * se + 1e-12, (got.mean(), exact, se)
+
+
+def onsager_energy_per_site(J, T):
+    """Onsager (1944): u = -J coth(2K) [1 + (2/pi)(2 tanh^2(2K) - 1) K1(k^2)]."""
+    from scipy.special import ellipk
+    K = J / (KB * T)
+    k = 2.0 * math.sinh(2 * K) / math.cosh(2 * K) ** 2
+    return -J / math.tanh(2 * K) * (1 + 2 / math.pi * (2 * math.tanh(2 * K) ** 2 - 1)
+                                    * ellipk(k * k))
+
+
+@pytest.mark.parametrize("T", [1500.0, 4000.0])
+def test_onsager_internal_energy(T):
+    J1, L = 0.1, 128
+    n_chains, passes, drop = 16, 2000, 300
+    print("case: onsager T", T, flush=True)
+    chains = [dict(shape=(L, L), temperature=T, exchange_potential=0.0, seed=50 + i)
+              for i in range(n_chains)]
+    r = run(chains, cm.LATTICE_SQUARE, J1, 0.0, passes)
+    es, _ = chain_means(r, n_chains, J1, 0.0, 0.0, L * L, drop)
+    u = es / (L * L)
+    exact = onsager_energy_per_site(J1, T)
+    se = u.std(ddof=1) / math.sqrt(n_chains)
+    assert abs(u.mean() - exact) < 4 * se + 2e-5, (u.mean(), exact, se)
+
+
+@pytest.mark.parametrize("ndim,L,J2,T,mu", [
+    (2, (24, 24), -0.02, 2400.0, 0.05),
+    (3, (8, 8, 8), -0.02, 3500.0, -0.1),
+])
+def test_three_sigma_vs_oracle_sequential(ndim, L, J2, T, mu):
+    """<e> and <x> of the checkerboard chain agree with the oracle's sequential
+    random-site chain within 3 sigma (sigma = BasicStatistics precision / 1.959049
+    for each, combined in quadrature)."""
+    J1 = 0.1
+    N = int(np.prod(L))
+    passes, drop = 6000, 500
+    print("case: 3sigma", ndim, L, J2, T, mu, flush=True)
+    occ0 = np.ones(N, dtype=np.int8)
+    _, S1, S2, M = O.lattice_run(ndim, L, J1, J2, T, mu, 11, passes, occ0)
+    orec = np.zeros(passes, dtype=cm.SAMPLE_DTYPE)
+    orec["S1"], orec["S2"], orec["M"] = S1, S2, M
+    r = run([dict(shape=L, temperature=T, exchange_potential=mu, seed=12)],
+            cm.LATTICE_SC if ndim == 3 else cm.LATTICE_SQUARE, J1, J2, passes)
+    grec = r.samples(0)
+    for idx in (0, 1):
+        a = energy_terms(orec[drop:], J1, J2, mu, N)[idx] / N
+        b = energy_terms(grec[drop:], J1, J2, mu, N)[idx] / N
+        ma, pa = O.basic_statistics(a)
+        mb, pb = O.basic_statistics(b)
+        sigma = math.hypot(pa, pb) / 1.959049
+        assert abs(ma - mb) < 3 * sigma + 1e-12, (idx, ma, mb, sigma)
