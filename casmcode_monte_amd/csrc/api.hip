@@ -219,6 +219,7 @@ int build_exact(mcg_handle *h) {
     P.cols = (int32_t)ch.shape[1];
     P.n = (int32_t)(ch.shape[0] * ch.shape[1]);
     P.fold = h->use_nlist ? 0 : 1;
+    P.self_nb = 2 * (P.rows == 1) + 2 * (P.cols == 1);
     P.div_magic = ((1ULL << 40) + (uint64_t)P.rows - 1) / (uint64_t)P.rows;
     P.negJ = -h->J1;
     P.lat_off = h->lat_off[c];

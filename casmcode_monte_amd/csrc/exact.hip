@@ -79,6 +79,7 @@ __global__ __launch_bounds__(64) void exact_kernel(const ExactParams *__restrict
   const uint64_t magic = P->div_magic;
   const uint32_t neg_mask = P->neg_mask, full_mask = P->full_mask;
   const int fold = P->fold;
+  const int self_nb = P->self_nb;
   int8_t *g_lat = lat_pool + P->lat_off;
   const int lat_bytes_al = (n + 15) & ~15;
   int8_t *lat = LDS_LATTICE ? reinterpret_cast<int8_t *>(smem) : g_lat;
@@ -140,7 +141,9 @@ __global__ __launch_bounds__(64) void exact_kernel(const ExactParams *__restrict
       if (acc) {
         if (lane == 0) lat[l] = (int8_t)(-s);
         if (!LDS_LATTICE) __threadfence();
-        S += -2LL * s * nsum;
+        // the bond sum's self terms (a 1-wide lattice is its own neighbour) never
+        // change; the reference's dE keeps them (model.hh:312-314), S does not
+        S += -2LL * s * (nsum - self_nb * s);
         M += -2LL * s;
         e_run = __dadd_rn(e_run, P->dq[k]);
         ++n_acc;

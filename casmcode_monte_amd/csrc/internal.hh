@@ -23,6 +23,8 @@ struct ExactParams {
   int32_t rows, cols;  // shape[0], shape[1]
   int32_t n;           // sites
   int32_t fold;        // 1: compute the row/column-fold formation energy at samples
+  int32_t self_nb;     // neighbour slots that are the site itself (rows==1, cols==1)
+  int32_t pad0;
   uint64_t div_magic;  // floor(l / rows) = (l * div_magic) >> 40 for l < 2^24
   uint64_t thr[EXACT_NCLASS];  // accept a drawn word x iff x < thr[k]
   uint32_t neg_mask;   // bit k: dE < 0, accept without drawing (metropolis.hh:28-30)

@@ -84,14 +84,17 @@ def chain_means(r, n_chains, J1, J2, mu, N, drop):
 
 @pytest.mark.parametrize("ndim,L,J2,T,mu", [
     (2, (4, 4), -0.02, 3000.0, 0.1),
-    (2, (4, 4), 0.0, 2000.0, -0.05),
+    (2, (4, 4), 0.0, 3500.0, -0.05),
+    (2, (4, 6), 0.0, 2800.0, 0.02),
     (3, (2, 2, 4), -0.02, 4000.0, 0.0),
 ])
 def test_exact_enumeration(ndim, L, J2, T, mu):
+    """Many independent seeded chains per condition (temperatures above the
+    tunnelling regime of these tiny cells, so 500 dropped passes equilibrate)."""
     J1 = 0.1
     N = int(np.prod(L))
     ex = O.lattice_enumerate(ndim, L, J1, J2, T, mu)
-    n_chains, passes, drop = 512, 3000, 200
+    n_chains, passes, drop = 512, 4000, 500
     print("case: enumeration", ndim, L, J2, T, mu, flush=True)
     chains = [dict(shape=L, temperature=T, exchange_potential=mu, seed=1000 + i)
               for i in range(n_chains)]

@@ -71,10 +71,13 @@ def test_exact_variants(shape, T, mu, seed, sp, use_nlist):
                 init=init.astype(np.int8), chunks=[1, 120, 179])
     compare(r, o)
     # the running potential energy (occupation_metropolis.hh:106,141) stays within
-    # accumulated rounding of the recomputed per-unitcell potential
-    rec = r.samples(0)
-    obs = r.observables(0, rec)
-    np.testing.assert_allclose(rec["e_run"], obs["potential_energy"], rtol=0, atol=1e-9)
+    # accumulated rounding of the recomputed per-unitcell potential (not on a
+    # 1-wide lattice: there the reference's dE counts the site as its own
+    # neighbour, model.hh:312-314, while its per_supercell does not change)
+    if rows > 1 and cols > 1:
+        rec = r.samples(0)
+        obs = r.observables(0, rec)
+        np.testing.assert_allclose(rec["e_run"], obs["potential_energy"], rtol=0, atol=1e-9)
 
 
 def test_rng_dump_load_roundtrip():
