@@ -38,7 +38,7 @@ int fail(int code, std::string const &msg) {
   } while (0)
 
 constexpr int64_t LDS_LIMIT = 160 * 1024;     // gfx950 LDS per workgroup
-constexpr int64_t EXACT_LDS_MAX = 150 * 1024;  // lattice + fold scratch
+constexpr int64_t EXACT_LDS_MAX = 144 * 1024;  // lattice + fold scratch (+ ~10 KB static)
 constexpr int64_t CHECKER_LDS_MAX = 128 * 1024;
 
 // libstdc++ canonical double of a word compared to prob: the first word whose
@@ -220,6 +220,9 @@ int build_exact(mcg_handle *h) {
     P.n = (int32_t)(ch.shape[0] * ch.shape[1]);
     P.fold = h->use_nlist ? 0 : 1;
     P.self_nb = 2 * (P.rows == 1) + 2 * (P.cols == 1);
+    // speculative window: ~5*window/n of the lanes see another lane's site in
+    // their neighbourhood; keep that near 2%
+    P.window = (int32_t)std::min<int64_t>(64, std::max<int64_t>(4, P.n / 256));
     P.div_magic = ((1ULL << 40) + (uint64_t)P.rows - 1) / (uint64_t)P.rows;
     P.negJ = -h->J1;
     P.lat_off = h->lat_off[c];
@@ -611,6 +614,7 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
   if (h->mode == MCG_MODE_EXACT) {
     launch_exact(h->d_xp, h->d_xs, h->d_lat, h->d_series, h->series_cap, h->d_active,
                  h->n_chains, n_passes, h->sample_period, h->exact_lds, h->exact_in_lds,
+                 !(h->options & MCG_OPT_EXACT_SCALAR_WALK),
                  h->stream);
   } else if (h->checker_lds) {
     launch_checker_lds(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_series, h->series_cap,

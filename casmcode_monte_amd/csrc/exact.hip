@@ -208,11 +208,308 @@ __global__ __launch_bounds__(64) void exact_kernel(const ExactParams *__restrict
   }
 }
 
+// ---------------------------------------------------------------------------
+// Block-speculative walk (LDS lattice).
+//
+// The stream is consumed in blocks.  For a block starting at stream position
+// P, lane t evaluates position P+t as if it were a site draw: its site (Lemire
+// on word x[P+t]), the old spin and neighbour sum from the block-start lattice,
+// the class k, whether the attempt would draw a uniform (dE >= 0) and whether
+// it would accept (x[P+t+1] < thr[k]).  A scalar pass then walks the real
+// sequence: position t is an attempt; the next attempt is at t+1, or t+2 when t
+// drew a uniform.  Every lane also marks its site in the lattice byte (bit 1;
+// bit 2 when a second lane marks the same site), so a lane whose own site was
+// drawn twice or whose neighbourhood holds another lane's site is flagged as a
+// potential conflict: the scalar pass stops before such a lane (it will be
+// re-evaluated as lane 0 of the next block, on the updated lattice).  Hence
+// every attempt the pass accepts saw exactly the lattice the sequential chain
+// would see.  Flips are committed at the end of the block; the integer
+// observables are block reductions; the FP64 running energy is added in attempt
+// order.  A lane 0 whose Lemire product needs the rejection test takes the
+// one-attempt scalar path.
+// ---------------------------------------------------------------------------
+constexpr uint32_t SPIN_BIT = 1u, MARK_A = 2u, MARK_B = 4u;
+constexpr int RING = 2 * MT_N;
+
+// Out-of-place twist of generation src into dst and temper into the ring slot.
+__device__ void twist_gen(const uint64_t *src, uint64_t *dst, uint64_t *slot, int lane) {
+  for (int k = lane; k < MT_N - MT_M; k += 64)
+    dst[k] = mt_twist_word(src[k], src[k + 1], src[k + MT_M]);
+  __syncthreads();
+  for (int k = MT_N - MT_M + lane; k < MT_N - 1; k += 64)
+    dst[k] = mt_twist_word(src[k], src[k + 1], dst[k - (MT_N - MT_M)]);
+  if (lane == 0) dst[MT_N - 1] = mt_twist_word(src[MT_N - 1], dst[0], dst[MT_M - 1]);
+  __syncthreads();
+  for (int k = lane; k < MT_N; k += 64) slot[k] = mt_temper(dst[k]);
+  __syncthreads();
+}
+
+__device__ __forceinline__ int spin_of(uint32_t b) { return (b & SPIN_BIT) ? 1 : -1; }
+
+__device__ __forceinline__ int wave_isum(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  uint64_t u = __double_as_longlong(v);
+  uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, lane);
+  uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), lane);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__restrict__ Pp,
+                                                         ExactState *__restrict__ Sp,
+                                                         int8_t *__restrict__ lat_pool,
+                                                         mcg_sample *__restrict__ series,
+                                                         int64_t series_cap,
+                                                         const uint8_t *__restrict__ active,
+                                                         int64_t n_passes,
+                                                         int64_t sample_period) {
+  const int c = blockIdx.x;
+  if (!active[c]) return;
+  const int lane = threadIdx.x;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint64_t raw[2][MT_N];
+  __shared__ uint64_t ring[RING];
+  __shared__ uint64_t thr_s[EXACT_NCLASS];
+  __shared__ double dq_s[EXACT_NCLASS];
+
+  const ExactParams *P = Pp + c;
+  ExactState *St = Sp + c;
+  const int32_t rows = P->rows, cols = P->cols, n = P->n;
+  const uint64_t magic = P->div_magic;
+  const uint32_t neg_mask = P->neg_mask, full_mask = P->full_mask;
+  const int fold = P->fold, self_nb = P->self_nb, W = P->window;
+  int8_t *g_lat = lat_pool + P->lat_off;
+  const int lat_bytes_al = (n + 15) & ~15;
+  uint8_t *lat = reinterpret_cast<uint8_t *>(smem);
+  uint32_t *latw = reinterpret_cast<uint32_t *>(smem);
+  int32_t *scratch = reinterpret_cast<int32_t *>(smem + lat_bytes_al);
+
+  for (int i = lane; i < MT_N; i += 64) {
+    raw[0][i] = St->mt[i];
+    ring[i] = mt_temper(raw[0][i]);
+  }
+  if (lane < EXACT_NCLASS) {
+    thr_s[lane] = P->thr[lane];
+    dq_s[lane] = P->dq[lane];
+  }
+  for (int i = lane; i < lat_bytes_al; i += 64) lat[i] = (i < n && g_lat[i] > 0) ? 1u : 0u;
+  __syncthreads();
+
+  const int64_t pos0 = St->pos;
+  int64_t pos = pos0;  // absolute stream position; generation g holds [312g, 312g+312)
+  int64_t G = 0;       // newest generation in the ring (ring holds G-1 and G)
+  int64_t n_pass = St->n_pass, n_acc = St->n_accept, n_rej = St->n_reject;
+  int64_t S = St->S, M = St->M, n_samples = St->n_samples;
+  double e_run = St->e_run;
+  const uint64_t N64 = (uint64_t)n;
+
+  auto ensure = [&](int64_t upto) {  // make positions < upto readable
+    while (upto > (int64_t)MT_N * (G + 1)) {
+      twist_gen(raw[G & 1], raw[(G + 1) & 1], ring + ((G + 1) & 1) * MT_N, lane);
+      ++G;
+    }
+  };
+  auto ring_index = [&](int64_t q) -> int {  // q mod 624, q in [312(G-1), 312(G+1))
+    const int64_t base = (int64_t)MT_N * (G - 1);
+    int r = (int)(q - base) + (int)(((G - 1) & 1) * MT_N);
+    return r >= RING ? r - RING : r;
+  };
+
+  int32_t step = 0;
+  int64_t passes_done = 0;
+  while (passes_done < n_passes) {
+    ensure(pos + W + 1);
+    const bool inwin = lane < W;
+    const int64_t q = pos + lane;
+    const uint64_t x = ring[ring_index(inwin ? q : pos)];
+    const uint64_t x1 = ring[ring_index(inwin ? q + 1 : pos)];
+    const uint64_t lo = x * N64;
+    const uint32_t site = (uint32_t)__umul64hi(x, N64);
+    const bool special = inwin && (lo < N64);
+    const bool valid = inwin && !special;
+    const uint32_t j = (uint32_t)(((uint64_t)site * magic) >> 40);
+    const uint32_t i = site - j * (uint32_t)rows;
+    const uint32_t iu = (i + 1 == (uint32_t)rows) ? 0u : i + 1;
+    const uint32_t id = (i == 0) ? (uint32_t)rows - 1 : i - 1;
+    const uint32_t ju = (j + 1 == (uint32_t)cols) ? 0u : j + 1;
+    const uint32_t jd = (j == 0) ? (uint32_t)cols - 1 : j - 1;
+    const uint32_t sh = (site & 3u) * 8u;
+    if (valid) {
+      const uint32_t old = atomicOr(&latw[site >> 2], MARK_A << sh);
+      if ((old >> sh) & MARK_A) atomicOr(&latw[site >> 2], MARK_B << sh);
+    }
+    __syncthreads();
+    const uint32_t b0 = lat[site];
+    const uint32_t b1 = lat[iu + rows * j], b2 = lat[id + rows * j];
+    const uint32_t b3 = lat[i + rows * ju], b4 = lat[i + rows * jd];
+    const int s = spin_of(b0);
+    const int nsum = spin_of(b1) + spin_of(b2) + spin_of(b3) + spin_of(b4);
+    const bool pc = valid && ((b0 & MARK_B) || ((b1 | b2 | b3 | b4) & MARK_A));
+    const int k = (s > 0 ? 5 : 0) + ((nsum + 4) >> 1);
+    const bool nd = !((neg_mask >> k) & 1u);
+    const bool acc = !nd || ((full_mask >> k) & 1u) || (x1 < thr_s[k]);
+    const uint64_t ND = __ballot(valid && nd);
+    const uint64_t AC = __ballot(valid && acc);
+    const uint64_t PC = __ballot(pc);
+    const uint64_t SP = __ballot(special);
+
+    // scalar pass over the real sequence of attempts
+    int t = 0, attempts = 0;
+    uint64_t F = 0;
+    const int to_pass_end = n - step;
+    if (!(SP & 1ull)) {
+      while (t < W) {
+        const uint64_t bit = 1ull << t;
+        if (t > 0 && ((PC | SP) & bit)) break;
+        if (AC & bit) F |= bit;
+        t += 1 + (int)((ND >> t) & 1ull);
+        if (++attempts == to_pass_end) break;
+      }
+    }
+    // commit the block
+    const bool flipped = (F >> lane) & 1ull;
+    const int dS = flipped ? -2 * s * (nsum - self_nb * s) : 0;
+    const int dM = flipped ? -2 * s : 0;
+    S += wave_isum(dS);
+    M += wave_isum(dM);
+    const double dqv = dq_s[k];
+    for (uint64_t f = F; f; f &= f - 1) e_run = __dadd_rn(e_run, readlane_f64(dqv, __builtin_ctzll(f)));
+    const int nflip = __popcll(F);
+    n_acc += nflip;
+    n_rej += attempts - nflip;
+    if (flipped) atomicXor(&latw[site >> 2], SPIN_BIT << sh);
+    if (valid) atomicAnd(&latw[site >> 2], ~((MARK_A | MARK_B) << sh));
+    __syncthreads();
+    pos += t;
+    step += attempts;
+
+    if (SP & 1ull) {
+      // one attempt with the full Lemire rejection loop (scalar walk)
+      uint64_t xs = ring[ring_index(pos)];
+      ++pos;
+      uint64_t los = xs * N64, his = __umul64hi(xs, N64);
+      if (los < N64) {
+        const uint64_t th = (0ULL - N64) % N64;
+        while (los < th) {
+          ensure(pos + 1);
+          xs = ring[ring_index(pos)];
+          ++pos;
+          los = xs * N64;
+          his = __umul64hi(xs, N64);
+        }
+      }
+      const uint32_t l = (uint32_t)his;
+      const uint32_t jj = (uint32_t)(((uint64_t)l * magic) >> 40);
+      const uint32_t ii = l - jj * (uint32_t)rows;
+      const uint32_t iiu = (ii + 1 == (uint32_t)rows) ? 0u : ii + 1;
+      const uint32_t iid = (ii == 0) ? (uint32_t)rows - 1 : ii - 1;
+      const uint32_t jju = (jj + 1 == (uint32_t)cols) ? 0u : jj + 1;
+      const uint32_t jjd = (jj == 0) ? (uint32_t)cols - 1 : jj - 1;
+      const int ss = spin_of(lat[l]);
+      const int ns = spin_of(lat[iiu + rows * jj]) + spin_of(lat[iid + rows * jj]) +
+                     spin_of(lat[ii + rows * jju]) + spin_of(lat[ii + rows * jjd]);
+      const int kk = (ss > 0 ? 5 : 0) + ((ns + 4) >> 1);
+      bool a;
+      if ((neg_mask >> kk) & 1u) {
+        a = true;
+      } else {
+        ensure(pos + 1);
+        const uint64_t x2 = ring[ring_index(pos)];
+        ++pos;
+        a = ((full_mask >> kk) & 1u) || (x2 < thr_s[kk]);
+      }
+      if (a) {
+        if (lane == 0) atomicXor(&latw[l >> 2], SPIN_BIT << ((l & 3u) * 8u));
+        S += -2LL * ss * (ns - self_nb * ss);
+        M += -2LL * ss;
+        e_run = __dadd_rn(e_run, dq_s[kk]);
+        ++n_acc;
+      } else {
+        ++n_rej;
+      }
+      __syncthreads();
+      ++step;
+    }
+
+    if (step == n) {
+      step = 0;
+      ++passes_done;
+      ++n_pass;
+      if (n_pass % sample_period == 0) {
+        double ef = 0.0;
+        if (fold) {
+          for (int r = lane; r < rows; r += 64) {
+            const int ru = (r + 1 == rows) ? 0 : r + 1;
+            int d = 0;
+            for (int qq = 0; qq < cols; ++qq)
+              d += spin_of(lat[r + rows * qq]) * spin_of(lat[ru + rows * qq]);
+            scratch[r] = d;
+          }
+          for (int qq = lane; qq < cols; qq += 64) {
+            const int qu = (qq + 1 == cols) ? 0 : qq + 1;
+            int d = 0;
+            for (int r = 0; r < rows; ++r)
+              d += spin_of(lat[r + rows * qq]) * spin_of(lat[r + rows * qu]);
+            scratch[rows + qq] = d;
+          }
+          __syncthreads();
+          if (lane == 0) {
+            double e = 0.0;  // model.hh:276-285
+            for (int r = 0; r < rows + cols; ++r)
+              e = __dadd_rn(e, __dmul_rn(P->negJ, (double)scratch[r]));
+            ef = e;
+          }
+          __syncthreads();
+        }
+        if (lane == 0 && n_samples < series_cap) {
+          mcg_sample rec;
+          rec.pass = n_pass;
+          rec.S1 = S;
+          rec.S2 = 0;
+          rec.M = M;
+          rec.e_fold = ef;
+          rec.e_run = e_run;
+          series[(int64_t)c * series_cap + n_samples] = rec;
+        }
+        ++n_samples;
+      }
+    }
+  }
+
+  // write back in the std::mt19937_64 layout: the generation holding the last
+  // consumed output, with index in (0, 312]
+  __syncthreads();
+  int64_t g_fin = 0, p_fin = pos0;
+  if (pos != pos0) {
+    g_fin = (pos - 1) / MT_N;
+    p_fin = pos - (int64_t)MT_N * g_fin;
+  }
+  for (int i2 = lane; i2 < MT_N; i2 += 64) St->mt[i2] = raw[g_fin & 1][i2];
+  for (int i2 = lane; i2 < n; i2 += 64) g_lat[i2] = (lat[i2] & SPIN_BIT) ? (int8_t)1 : (int8_t)-1;
+  if (lane == 0) {
+    St->pos = p_fin;
+    St->n_pass = n_pass;
+    St->n_accept = n_acc;
+    St->n_reject = n_rej;
+    St->S = S;
+    St->M = M;
+    St->e_run = e_run;
+    St->n_samples = n_samples;
+  }
+}
+
 void launch_exact(const ExactParams *P, ExactState *S, int8_t *lat_pool, mcg_sample *series,
                   int64_t series_cap, const uint8_t *active, int n_chains, int64_t n_passes,
-                  int64_t sample_period, int lds_bytes, bool lattice_in_lds,
+                  int64_t sample_period, int lds_bytes, bool lattice_in_lds, bool block_walk,
                   hipStream_t stream) {
-  if (lattice_in_lds) {
+  if (lattice_in_lds && block_walk) {
+    hipLaunchKernelGGL(exact_block_kernel, dim3(n_chains), dim3(64), lds_bytes, stream, P, S,
+                       lat_pool, series, series_cap, active, n_passes, sample_period);
+  } else if (lattice_in_lds) {
     hipLaunchKernelGGL(exact_kernel<true>, dim3(n_chains), dim3(64), lds_bytes, stream, P, S,
                        lat_pool, series, series_cap, active, n_passes, sample_period);
   } else {
@@ -222,6 +519,8 @@ void launch_exact(const ExactParams *P, ExactState *S, int8_t *lat_pool, mcg_sam
 }
 
 void exact_set_max_lds(int bytes) {
+  hipFuncSetAttribute(reinterpret_cast<const void *>(exact_block_kernel),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes - 8 * 1024);
   hipFuncSetAttribute(reinterpret_cast<const void *>(exact_kernel<true>),
                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   hipFuncSetAttribute(reinterpret_cast<const void *>(exact_kernel<false>),

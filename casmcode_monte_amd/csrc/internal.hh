@@ -24,7 +24,7 @@ struct ExactParams {
   int32_t n;           // sites
   int32_t fold;        // 1: compute the row/column-fold formation energy at samples
   int32_t self_nb;     // neighbour slots that are the site itself (rows==1, cols==1)
-  int32_t pad0;
+  int32_t window;      // stream positions evaluated per block by the speculative walk
   uint64_t div_magic;  // floor(l / rows) = (l * div_magic) >> 40 for l < 2^24
   uint64_t thr[EXACT_NCLASS];  // accept a drawn word x iff x < thr[k]
   uint32_t neg_mask;   // bit k: dE < 0, accept without drawing (metropolis.hh:28-30)
@@ -46,7 +46,7 @@ struct ExactState {
 
 void launch_exact(const ExactParams *P, ExactState *S, int8_t *lat_pool, mcg_sample *series,
                   int64_t series_cap, const uint8_t *active, int n_chains, int64_t n_passes,
-                  int64_t sample_period, int lds_bytes, bool lattice_in_lds,
+                  int64_t sample_period, int lds_bytes, bool lattice_in_lds, bool block_walk,
                   hipStream_t stream);
 
 // ---------------------------------------------------------------------------

@@ -82,6 +82,10 @@ typedef struct mcg_desc {
 /* CHECKERBOARD: always use the HBM work-list kernels, even for lattices that
  * fit in LDS (the two paths produce identical trajectories). */
 #define MCG_OPT_FORCE_STREAMING 1
+/* EXACT: run the one-attempt-at-a-time scalar walk instead of the
+ * block-speculative walk (both reproduce the reference stream bit for bit;
+ * the scalar walk is the cross-check of the speculative one). */
+#define MCG_OPT_EXACT_SCALAR_WALK 2
 
 /* One sampled record of one chain.  Integer observables are exact; the
  * reference's FP64 sampling functions (basic_semigrand_canonical.hh:485-590)

@@ -18,9 +18,10 @@ pytestmark = pytest.mark.gpu
 
 
 def run_gpu(rows, cols, T, mu, seed, passes, use_nlist=True, sample_period=1, J=0.1,
-            init=None, chunks=None):
+            init=None, chunks=None, options=0):
     r = cm.Run([dict(shape=(rows, cols), temperature=T, exchange_potential=mu, seed=seed)],
-               mode=cm.MODE_EXACT, J1=J, use_nlist=use_nlist, sample_period=sample_period)
+               mode=cm.MODE_EXACT, J1=J, use_nlist=use_nlist, sample_period=sample_period,
+               options=options)
     if init is not None:
         r.set_occupation(0, init)
     for k in (chunks or [passes]):
@@ -120,3 +121,30 @@ def test_c2_batched_64_chains_bit_exact():
         outs = list(ex.map(orc, conds))
     for i, o in enumerate(outs):
         compare(r, o, c=i)
+
+
+@pytest.mark.parametrize("shape,T,mu", [((128, 128), 2200.0, 0.1), ((9, 11), 3000.0, -0.2),
+                                         ((256, 64), 4000.0, 0.0), ((2, 50), 1500.0, 0.3)])
+def test_speculative_walk_equals_scalar_walk(shape, T, mu):
+    """The block-speculative walk and the one-attempt scalar walk consume the
+    engine identically (so the rare Lemire-rejection path, which is the scalar
+    walk's code, is exercised against the same oracle trajectories)."""
+    rows, cols = shape
+    print("case: speculative vs scalar", shape, T, mu, flush=True)
+    a = run_gpu(rows, cols, T, mu, 31, 0)
+    b = run_gpu(rows, cols, T, mu, 31, 0, options=cm.OPT_EXACT_SCALAR_WALK)
+    for k in (3, 1, 40):
+        a.run(k)
+        b.run(k)
+        np.testing.assert_array_equal(a.get_occupation(0), b.get_occupation(0))
+        np.testing.assert_array_equal(a.rng_dump(0), b.rng_dump(0))
+        np.testing.assert_array_equal(a.samples(0), b.samples(0))
+        assert a.counters(0) == b.counters(0)
+
+
+def test_large_lattice_vs_oracle():
+    """256x256 (window 64, many blocks per pass) bit-exact for 12 passes."""
+    print("case: 256x256 vs oracle", flush=True)
+    o = O.sgc_run(256, 256, temperature=2600.0, mu=0.02, seed=77, max_count=12)
+    r = run_gpu(256, 256, 2600.0, 0.02, 77, 12)
+    compare(r, o)
