@@ -338,52 +338,75 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
     const uint32_t ju = (j + 1 == (uint32_t)cols) ? 0u : j + 1;
     const uint32_t jd = (j == 0) ? (uint32_t)cols - 1 : j - 1;
     const uint32_t sh = (site & 3u) * 8u;
-    if (valid) {
-      const uint32_t old = atomicOr(&latw[site >> 2], MARK_A << sh);
-      if ((old >> sh) & MARK_A) atomicOr(&latw[site >> 2], MARK_B << sh);
-    }
-    __syncthreads();
+    // count the draws of each site in bits 1..7 of its byte (at most 64 draws);
+    // the wave's LDS operations execute in issue order, so the reads below see
+    // every lane's increment (the asm statement keeps the compiler from moving
+    // them above the atomics)
+    if (valid) atomicAdd(&latw[site >> 2], MARK_A << sh);
+    asm volatile("" ::: "memory");
     const uint32_t b0 = lat[site];
     const uint32_t b1 = lat[iu + rows * j], b2 = lat[id + rows * j];
     const uint32_t b3 = lat[i + rows * ju], b4 = lat[i + rows * jd];
     const int s = spin_of(b0);
     const int nsum = spin_of(b1) + spin_of(b2) + spin_of(b3) + spin_of(b4);
-    const bool pc = valid && ((b0 & MARK_B) || ((b1 | b2 | b3 | b4) & MARK_A));
+    // potential conflict: own site drawn twice, or a neighbour drawn at all
+    const bool pc = valid && ((b0 & 0xFCu) || ((b1 | b2 | b3 | b4) & 0xFEu));
     const int k = (s > 0 ? 5 : 0) + ((nsum + 4) >> 1);
+    const uint64_t thr_k = thr_s[k];
+    const double dqv = dq_s[k];
     const bool nd = !((neg_mask >> k) & 1u);
-    const bool acc = !nd || ((full_mask >> k) & 1u) || (x1 < thr_s[k]);
+    const bool acc = !nd || ((full_mask >> k) & 1u) || (x1 < thr_k);
     const uint64_t ND = __ballot(valid && nd);
     const uint64_t AC = __ballot(valid && acc);
-    const uint64_t PC = __ballot(pc);
+    const uint64_t STOP = __ballot(pc || special || !inwin);
     const uint64_t SP = __ballot(special);
 
-    // scalar pass over the real sequence of attempts
-    int t = 0, attempts = 0;
-    uint64_t F = 0;
+    // Attempt positions of the window in closed form: position t is a uniform
+    // draw iff t-1 is an attempt that draws (ND), so inside every run of ND bits
+    // the attempts sit at even offsets from the run start.  E = attempts that
+    // draw, U = E << 1 = uniform positions, attempts = ~U.
+    const uint64_t ALT = 0x5555555555555555ull;
+    const uint64_t run_start = ND & ~(ND << 1);
+    const uint64_t odd_runs = ((ND + (run_start & ~ALT)) ^ ND) & ND;
+    const uint64_t E = ND & (ALT ^ odd_runs);
+    const uint64_t A = ~(E << 1);
+    // execute attempts up to (not including) the first flagged attempt t > 0
+    const uint64_t stops = A & STOP & ~1ull;
+    uint64_t exec = (SP & 1ull) ? 0ull : (A & (stops ? ((1ull << __builtin_ctzll(stops)) - 1) : ~0ull));
+    int attempts = __popcll(exec);
     const int to_pass_end = n - step;
-    if (!(SP & 1ull)) {
-      while (t < W) {
-        const uint64_t bit = 1ull << t;
-        if (t > 0 && ((PC | SP) & bit)) break;
-        if (AC & bit) F |= bit;
-        t += 1 + (int)((ND >> t) & 1ull);
-        if (++attempts == to_pass_end) break;
-      }
+    if (attempts > to_pass_end) {  // end of pass inside the block
+      uint64_t rest = exec;
+      for (int r = 0; r < to_pass_end; ++r) rest &= rest - 1;
+      exec ^= rest;
+      attempts = to_pass_end;
     }
-    // commit the block
+    int t = 0;
+    if (exec) {
+      const int last = 63 - __builtin_clzll(exec);
+      t = last + 1 + (int)((ND >> last) & 1ull);
+    }
+    const uint64_t F = exec & AC;
+
+    // commit the block: in-order FP64 running energy and the integer deltas of
+    // the flipped lanes (packed: dS + 16 in bits 0..15, dM + 2 in bits 16..31)
     const bool flipped = (F >> lane) & 1ull;
-    const int dS = flipped ? -2 * s * (nsum - self_nb * s) : 0;
-    const int dM = flipped ? -2 * s : 0;
-    S += wave_isum(dS);
-    M += wave_isum(dM);
-    const double dqv = dq_s[k];
-    for (uint64_t f = F; f; f &= f - 1) e_run = __dadd_rn(e_run, readlane_f64(dqv, __builtin_ctzll(f)));
+    const uint32_t packed = (uint32_t)(-2 * s * (nsum - self_nb * s) + 16) |
+                            ((uint32_t)(-2 * s + 2) << 16);
+    uint32_t psum = 0;
+    for (uint64_t f = F; f; f &= f - 1) {
+      const int b = __builtin_ctzll(f);
+      e_run = __dadd_rn(e_run, readlane_f64(dqv, b));
+      psum += __builtin_amdgcn_readlane(packed, b);
+    }
     const int nflip = __popcll(F);
+    S += (int64_t)(psum & 0xFFFFu) - 16 * nflip;
+    M += (int64_t)(psum >> 16) - 2 * nflip;
     n_acc += nflip;
     n_rej += attempts - nflip;
     if (flipped) atomicXor(&latw[site >> 2], SPIN_BIT << sh);
-    if (valid) atomicAnd(&latw[site >> 2], ~((MARK_A | MARK_B) << sh));
-    __syncthreads();
+    if (valid) atomicAnd(&latw[site >> 2], ~(0xFEu << sh));
+    asm volatile("" ::: "memory");
     pos += t;
     step += attempts;
 
