@@ -136,7 +136,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--skip-extra", action="store_true", help="only the default workload")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many bytes 10 times "
+                         "(the reference point that scales the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_bytes:
+        import torch
+        a = torch.ones(args.copy_bytes, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.copy_bytes, "launches": 10}))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; independent chains shard "
               "trivially across GPUs (see DESIGN.md) but no multi-GPU figure is measured",
