@@ -1,0 +1,91 @@
+// RAII C++ wrapper of the MI355X library's C ABI (include/casm_mi355x/mcgpu.h),
+// used by the drop-in calculators.  Errors become std::runtime_error, the
+// reference's error model (definitions.hh:107-113).
+#ifndef CASM_MI355X_monte_gpu_Handle
+#define CASM_MI355X_monte_gpu_Handle
+
+#include <sstream>
+
+#include "casm/monte/definitions.hh"
+#include "casm_mi355x/mcgpu.h"
+
+namespace CASM {
+namespace monte {
+namespace gpu {
+
+inline void check(int rc) {
+  if (rc != MCG_OK) throw std::runtime_error(std::string("MI355X backend: ") + mcg_last_error());
+}
+
+/// The 313 words std::mt19937_64's operator<< writes (state words, then the
+/// index of the next output); the same layout the device chain keeps.
+inline std::vector<uint64_t> engine_words(std::mt19937_64 const &e) {
+  std::ostringstream os;
+  os << e;
+  std::istringstream is(os.str());
+  std::vector<uint64_t> w(313);
+  for (auto &x : w) is >> x;
+  return w;
+}
+
+inline void set_engine_words(std::mt19937_64 &e, std::vector<uint64_t> const &w) {
+  std::ostringstream os;
+  for (size_t i = 0; i < w.size(); ++i) os << (i ? " " : "") << w[i];
+  std::istringstream is(os.str());
+  is >> e;
+  if (!is) throw std::runtime_error("MI355X backend: could not restore the engine state");
+}
+
+class Handle {
+ public:
+  explicit Handle(mcg_desc const &d) { check(mcg_create(&d, &m_h)); }
+  ~Handle() {
+    if (m_h) mcg_destroy(m_h);
+  }
+  Handle(Handle const &) = delete;
+  Handle &operator=(Handle const &) = delete;
+
+  mcg_handle *get() const { return m_h; }
+
+  void set_occupation(Index chain, std::vector<int8_t> const &occ) {
+    check(mcg_set_occupation(m_h, chain, occ.data(), (int64_t)occ.size()));
+  }
+  std::vector<int8_t> occupation(Index chain, Index n) const {
+    std::vector<int8_t> occ(n);
+    check(mcg_get_occupation(m_h, chain, occ.data(), n));
+    return occ;
+  }
+  void load_engine(Index chain, std::mt19937_64 const &e) {
+    auto w = engine_words(e);
+    check(mcg_rng_load(m_h, chain, w.data(), (int64_t)w.size()));
+  }
+  void store_engine(Index chain, std::mt19937_64 &e) const {
+    std::vector<uint64_t> w(313);
+    check(mcg_rng_dump(m_h, chain, w.data(), (int64_t)w.size()));
+    set_engine_words(e, w);
+  }
+  void run(CountType n_passes) { check(mcg_run(m_h, n_passes)); }
+  Index n_samples(Index chain) const {
+    int64_t n = 0;
+    check(mcg_n_samples(m_h, chain, &n));
+    return n;
+  }
+  std::vector<mcg_sample> samples(Index chain, Index first, Index count) const {
+    std::vector<mcg_sample> out(count);
+    check(mcg_get_samples(m_h, chain, first, count, out.data()));
+    return out;
+  }
+  void counters(Index chain, int64_t &n_pass, int64_t &n_accept, int64_t &n_reject) const {
+    check(mcg_get_counters(m_h, chain, &n_pass, &n_accept, &n_reject));
+  }
+  void clear_samples() { check(mcg_clear_samples(m_h)); }
+
+ private:
+  mcg_handle *m_h = nullptr;
+};
+
+}  // namespace gpu
+}  // namespace monte
+}  // namespace CASM
+
+#endif

@@ -1,0 +1,146 @@
+// Drop-in counterpart of libcasm-monte's RunManager-driven occupation
+// Metropolis loop (methods/occupation_metropolis.hh:87-154) for the SGC Ising
+// model with single-site flip proposals, with the loop on the MI355X.
+//
+// Reference semantics kept: the potential energy per unit cell is carried in
+// state.properties["potential_energy"] as an FP64 running sum updated by
+// fl(dE / n_unitcells) per accepted step (:103-106, :141) -- the device keeps
+// this sum in the same order; fixtures sample by pass count; the run stops
+// when the RunManager says so (global cutoff or all fixtures complete); the
+// engine shared with the RunManager is advanced exactly as the reference
+// advances it and written back at the end.
+//
+// The device runs whole passes between the points where a fixture samples or
+// a completion check can change its answer (see basic_semigrand_canonical.hh
+// for the argument); sampling by step or by time is not supported.
+#ifndef CASM_MI355X_monte_methods_occupation_metropolis
+#define CASM_MI355X_monte_methods_occupation_metropolis
+
+#include "casm/monte/ising_cpp/basic_semigrand_canonical.hh"
+#include "casm/monte/run_management/RunManager.hh"
+
+namespace CASM {
+namespace monte {
+
+/// SGC potential of the Ising model for occupation_metropolis: E_f - mu*N*x
+/// per supercell (basic_semigrand_canonical.hh:165-169), bound to one state.
+class IsingSemiGrandCanonicalPotential {
+ public:
+  IsingSemiGrandCanonicalPotential(double _J, bool _use_nlist) : J(_J), use_nlist(_use_nlist) {}
+  double J;
+  bool use_nlist;
+  void set(State<ising_cpp::IsingConfiguration> const *s) { m_state = s; }
+  State<ising_cpp::IsingConfiguration> const *state() const { return m_state; }
+  double per_supercell() const {
+    ising_cpp::IsingState st(m_state->configuration, m_state->conditions);
+    ising_cpp::IsingFormationEnergy fe(J, 1, use_nlist, &st);
+    ising_cpp::IsingParamComposition pc(&st);
+    VectorXd mu = m_state->conditions.vector_values.at("exchange_potential");
+    return fe.per_supercell() - mu.dot(pc.per_supercell());
+  }
+
+ private:
+  State<ising_cpp::IsingConfiguration> const *m_state = nullptr;
+};
+
+template <typename StatisticsType, typename EngineType>
+void occupation_metropolis(
+    State<ising_cpp::IsingConfiguration> &state, IsingSemiGrandCanonicalPotential &potential,
+    RandomNumberGenerator<EngineType> &random_number_generator,
+    RunManager<ising_cpp::IsingConfiguration, StatisticsType, EngineType> &run_manager) {
+  if (potential.state() != &state)
+    throw std::runtime_error(
+        "Error in monte::occupation_metropolis: potential not set to correct state");
+  auto &config = state.configuration;
+  const double n_unitcells = static_cast<double>(config.n_sites);
+  state.properties.scalar_values["potential_energy"] = 0.;
+  double &potential_energy_per_unitcell = state.properties.scalar_values["potential_energy"];
+  potential_energy_per_unitcell = potential.per_supercell() / n_unitcells;
+
+  mcg_chain_desc ch{};
+  ch.shape[0] = config.shape[0];
+  ch.shape[1] = config.shape[1];
+  ch.shape[2] = 1;
+  ch.temperature = state.conditions.scalar_values.at("temperature");
+  ch.exchange_potential = state.conditions.vector_values.at("exchange_potential")(0);
+  mcg_desc d{};
+  d.mode = MCG_MODE_EXACT;
+  d.lattice = MCG_LATTICE_SQUARE;
+  d.use_nlist = potential.use_nlist ? 1 : 0;
+  d.J1 = potential.J;
+  d.kb = KB;
+  d.sample_period = 1;
+  d.n_chains = 1;
+  d.chains = &ch;
+  gpu::Handle dev(d);
+  dev.set_occupation(0, std::vector<int8_t>(config.occupation().begin(), config.occupation().end()));
+  dev.load_engine(0, *random_number_generator.engine);
+
+  const CountType N = config.n_sites;
+  run_manager.initialize(N);
+  bool host_functions = false;
+  for (auto &f : run_manager.sampling_fixtures) {
+    auto const &sp = f->params().sampling_params;
+    if (sp.sample_mode != SAMPLE_MODE::BY_PASS)
+      throw std::runtime_error(
+          "occupation_metropolis (MI355X): only SAMPLE_MODE::BY_PASS sampling is supported");
+    if (sp.do_sample_trajectory || !sp.json_sampler_names.empty()) host_functions = true;
+    for (auto const &name : sp.sampler_names)
+      if (f->params().sampling_functions.at(name).device_observable < 0) host_functions = true;
+  }
+  run_manager.sample_data_by_count_if_due(state);
+
+  int64_t acc_prev = 0, rej_prev = 0;
+  auto sync_configuration = [&]() {
+    auto occ = dev.occupation(0, N);
+    config.set_occupation(std::vector<int>(occ.begin(), occ.end()));
+  };
+  while (true) {
+    if (run_manager.is_complete()) break;
+    // the other calls of this pass only repeat periodic checks on the same samples
+    auto pending = [&]() {
+      for (auto &f : run_manager.sampling_fixtures) {
+        auto const &cc = f->completion_check();
+        if (cc.results().is_complete) continue;
+        if (cc.results().has_all_minimums_met &&
+            get_n_samples(f->results().samplers) >= cc.next_check_at())
+          return true;
+      }
+      return false;
+    };
+    for (CountType c = 1; c < N && pending(); ++c) run_manager.is_complete();
+    run_manager.write_status_if_due();
+
+    const CountType p0 = run_manager.sampling_fixtures.empty()
+                             ? 0
+                             : run_manager.sampling_fixtures.front()->counter().pass;
+    CountType target = p0 + 1000000;
+    if (pending()) target = p0 + 1;
+    for (auto &f : run_manager.sampling_fixtures) {
+      target = std::min(target, std::max(f->next_sample_count(), p0 + 1));
+      auto const &cp = f->params().completion_check_params.cutoff_params;
+      if (cp.min_count.has_value() && *cp.min_count > p0) target = std::min(target, *cp.min_count);
+      if (cp.max_count.has_value() && *cp.max_count > p0) target = std::min(target, *cp.max_count);
+    }
+    dev.run(target - p0);
+    mcg_sample last = dev.samples(0, dev.n_samples(0) - 1, 1)[0];
+    dev.clear_samples();
+    int64_t np = 0, na = 0, nr = 0;
+    dev.counters(0, np, na, nr);
+    for (auto &f : run_manager.sampling_fixtures)
+      f->counter().increment_passes(target - p0, na - acc_prev, nr - rej_prev);
+    acc_prev = na;
+    rej_prev = nr;
+    potential_energy_per_unitcell = last.e_run;
+    if (host_functions) sync_configuration();
+    run_manager.sample_data_by_count_if_due(state);
+  }
+  sync_configuration();
+  dev.store_engine(0, *random_number_generator.engine);
+  run_manager.finalize(state);
+}
+
+}  // namespace monte
+}  // namespace CASM
+
+#endif

@@ -705,6 +705,36 @@ inline SgcRunOutput sgc_basic_run(IsingConfiguration config, double J, bool use_
   return out;
 }
 
+// occupation_metropolis.hh:96-153 for the Ising SGC potential with the single
+// site flip proposal: the FP64 running potential energy per unit cell
+// (:106 initial value, :141 update per accepted step) after every pass.
+inline std::vector<double> sgc_running_energy_run(IsingConfiguration config, double J,
+                                                  bool use_nlist, double temperature, double mu,
+                                                  std::shared_ptr<std::mt19937_64> engine,
+                                                  long n_passes, double *initial = nullptr) {
+  SemiGrandCanonicalPotential potential{IsingFormationEnergy(J, use_nlist)};
+  potential.set_state(&config, mu);
+  const double n_unitcells = static_cast<double>(config.n_sites);
+  double pe = potential.per_supercell() / n_unitcells;
+  if (initial) *initial = pe;
+  const double beta = 1.0 / (KB * temperature);
+  RandomNumberGenerator rng(engine);
+  std::vector<double> out;
+  for (long p = 0; p < n_passes; ++p) {
+    for (long s = 0; s < config.n_sites; ++s) {
+      long l = rng.random_int(config.n_sites - 1);
+      int new_occ = -config.occ(l);
+      double dE = potential.delta(l, new_occ);
+      if (metropolis_acceptance(dE, beta, rng)) {
+        config.set_occ(l, new_occ);
+        pe += (dE / n_unitcells);
+      }
+    }
+    out.push_back(pe);
+  }
+  return out;
+}
+
 // ---------------------------------------------------------------------------
 // Build-defined Ising models for the statistical modes: 2D square and 3D
 // simple cubic lattices with nearest (J1) and next-nearest (J2) couplings,

@@ -1,0 +1,360 @@
+// Tests of the C++ drop-in classes (include/casm/monte/...) against the oracle
+// (oracle/refcpu.hh).
+//
+//   dropin_tests --cpu : host logic only (completion check, statistics,
+//                        equilibration, sampler) on synthetic series;
+//   dropin_tests --gpu : SemiGrandCanonicalCalculator::run on the MI355X vs the
+//                        oracle's reference loop, bit for bit.
+// Prints one line per case and "ALL OK" at the end; exits non-zero on failure.
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+
+#include "casm/monte/ising_cpp/basic_semigrand_canonical.hh"
+#include "casm/monte/methods/occupation_metropolis.hh"
+#include "refcpu.hh"
+
+using namespace CASM;
+using namespace CASM::monte;
+namespace sgc = CASM::monte::ising_cpp::basic_semigrand_canonical;
+
+static int g_fail = 0;
+#define EXPECT(cond, msg)                                                  \
+  do {                                                                     \
+    if (!(cond)) {                                                         \
+      std::cout << "FAIL: " << msg << " (" << #cond << ")" << std::endl; \
+      ++g_fail;                                                            \
+    }                                                                      \
+  } while (0)
+
+static bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; }
+
+// ---------------------------------------------------------------------------
+// CPU: host logic vs oracle
+// ---------------------------------------------------------------------------
+static std::vector<double> synthetic_series(int kind, int n, uint64_t seed) {
+  std::mt19937_64 e(seed);
+  std::normal_distribution<double> g(0.0, 1.0);
+  std::vector<double> x(n);
+  double ar = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double eps = g(e);
+    if (kind == 0) x[i] = 1.0 + 0.1 * eps;                       // iid
+    if (kind == 1) x[i] = (ar = 0.9 * ar + eps) + 5.0;           // AR(1) rho=0.9
+    if (kind == 2) x[i] = 3.0 * std::exp(-i / 50.0) + 0.05 * eps;  // relaxation + noise
+    if (kind == 3) x[i] = 2.5;                                   // constant
+  }
+  return x;
+}
+
+static void cpu_tests() {
+  // statistics and equilibration: identical arithmetic to the oracle
+  for (int kind = 0; kind < 4; ++kind)
+    for (int n : {1, 2, 7, 100, 1001}) {
+      auto x = synthetic_series(kind, n, 100 + kind);
+      VectorXd v(x);
+      auto a = BasicStatisticsCalculator()(v);
+      auto b = refcpu::basic_statistics(x);
+      EXPECT(same_bits(a.mean, b.mean) && same_bits(a.calculated_precision, b.calculated_precision),
+             "basic statistics kind " << kind << " n " << n);
+      for (double prec : {1e-3, 0.05, 1.0}) {
+        auto ea = _default_equilibration_check(v, prec);
+        auto eb = refcpu::default_equilibration_check_impl(x, prec);
+        EXPECT(ea.is_equilibrated == eb.is_equilibrated &&
+                   ea.N_samples_for_equilibration == eb.N_samples_for_equilibration,
+               "equilibration kind " << kind << " n " << n << " prec " << prec);
+      }
+    }
+  // closed forms: constant series -> factor 1 and precision 0
+  {
+    VectorXd c(std::vector<double>(50, 2.5));
+    EXPECT(autocorrelation_factor(c) == 1.0, "constant series autocorrelation factor");
+    EXPECT(BasicStatisticsCalculator()(c).calculated_precision == 0.0, "constant precision");
+  }
+  // test_CompletionCheck.py:5-37: max_count = 12 stops at exactly 12
+  {
+    CompletionCheckParams<BasicStatistics> p;
+    p.cutoff_params.max_count = 12;
+    CompletionCheck<BasicStatistics> cc(p);
+    SamplerMap samplers;
+    samplers["e"] = std::make_shared<Sampler>(std::vector<Index>{});
+    samplers["x"] = std::make_shared<Sampler>(std::vector<Index>{3});
+    Sampler weight({});
+    Log log;
+    CountType n_steps = 0;
+    while (!cc.is_complete(samplers, weight, n_steps, log)) {
+      n_steps += 1;
+      if (n_steps % 10 == 0) {
+        samplers["e"]->push_back(VectorXd{0.0});
+        samplers["x"]->push_back(VectorXd{0.0, 0.0, 0.0});
+      }
+    }
+    EXPECT(n_steps == 12, "max_count KAT");
+  }
+  // completion check on identical sample streams: same decisions step by step
+  for (int kind = 0; kind < 3; ++kind) {
+    CompletionCheckParams<BasicStatistics> p;
+    p.cutoff_params.min_sample = 100;
+    p.check_begin = 100;
+    p.check_period = 10;
+    p.requested_precision[SamplerComponent("e", 0, "0")] = RequestedPrecision::abs(0.01);
+    CompletionCheck<BasicStatistics> cc(p);
+    refcpu::CompletionCheckParams q;
+    q.cutoff_params.min_sample = 100;
+    q.check_begin = 100;
+    q.check_period = 10;
+    refcpu::RequestedPrecision rq;
+    rq.abs_convergence_is_required = true;
+    rq.abs_precision = 0.01;
+    q.requested_precision[refcpu::SamplerComponent{"e", 0}] = rq;
+    refcpu::CompletionCheck cr(q);
+    SamplerMap sa;
+    sa["e"] = std::make_shared<Sampler>(std::vector<Index>{});
+    refcpu::SamplerMap sb;
+    sb["e"] = std::make_shared<refcpu::Sampler>(1);
+    Sampler weight({});
+    Log log;
+    auto x = synthetic_series(kind, 20000, 7 + kind);
+    bool da = false, db = false;
+    CountType i = 0;
+    for (; i < (CountType)x.size(); ++i) {
+      da = cc.is_complete(sa, weight, i, log);
+      db = cr.is_complete(sb, i);
+      if (da != db || da) break;
+      sa["e"]->push_back(VectorXd{x[i]});
+      sb["e"]->push_back({x[i]});
+    }
+    EXPECT(da == db && cc.n_checks() == cr.n_checks(),
+           "completion decisions kind " << kind << " at " << i);
+  }
+  std::cout << "cpu: completion check, statistics, equilibration, sampler: done" << std::endl;
+}
+
+// ---------------------------------------------------------------------------
+// GPU: SemiGrandCanonicalCalculator::run vs the oracle
+// ---------------------------------------------------------------------------
+typedef ising_cpp::IsingSystem system_type;
+typedef sgc::SemiGrandCanonicalEventGenerator<std::mt19937_64> event_generator_type;
+typedef sgc::SemiGrandCanonicalCalculator<system_type, event_generator_type> calculator_type;
+
+struct Case {
+  int rows, cols;
+  bool use_nlist;
+  double T, mu;
+  uint64_t seed;
+  int sample_period;
+  long max_count;     // -1: none
+  double prec;        // requested abs precision of param_composition and potential_energy (<=0: none)
+  long min_sample;    // -1: none
+  bool host_function; // add a host-evaluated sampling function
+};
+
+static void gpu_case(Case const &c) {
+  auto system = std::make_shared<system_type>(ising_cpp::IsingFormationEnergy(0.1, 1, c.use_nlist),
+                                              ising_cpp::IsingParamComposition());
+  auto calc = std::make_shared<calculator_type>(system);
+  StateSamplingFunctionMap functions;
+  for (auto f : {sgc::make_parametric_composition_f(calc), sgc::make_formation_energy_f(calc),
+                 sgc::make_potential_energy_f(calc)})
+    functions.emplace(f.name, f);
+  if (c.host_function) {
+    // magnetisation from the host configuration (not a device observable)
+    StateSamplingFunction m("magnetisation", "sum of spins", {}, [calc]() -> VectorXd {
+      double s = 0;
+      for (int v : calc->state->configuration.occupation()) s += v;
+      return VectorXd{s};
+    });
+    functions.emplace(m.name, m);
+  }
+  CompletionCheckParams<BasicStatistics> ccp;
+  if (c.max_count > 0) ccp.cutoff_params.max_count = c.max_count;
+  if (c.min_sample > 0) ccp.cutoff_params.min_sample = c.min_sample;
+  ccp.check_begin = 100;
+  ccp.check_period = 10;
+  if (c.prec > 0) converge(functions, ccp).set_precision("potential_energy", c.prec).set_precision("param_composition", c.prec);
+
+  ValueMap cond;
+  cond.scalar_values["temperature"] = c.T;
+  cond.vector_values["exchange_potential"] = VectorXd{c.mu};
+  ising_cpp::IsingState state(ising_cpp::IsingConfiguration({c.rows, c.cols}, 1), cond);
+  std::mt19937_64 e0(c.seed);
+  // a few random flips so the start is not uniform
+  for (int k = 0; k < 5; ++k) state.configuration.set_occ((k * 7919) % (c.rows * c.cols), -1);
+  auto engine = std::make_shared<std::mt19937_64>(c.seed);
+  MethodLog mlog;
+  auto data = calc->run(state, functions, jsonStateSamplingFunctionMap(), ccp,
+                        event_generator_type(), c.sample_period, mlog, engine,
+                        sgc::write_no_status<calculator_type>);
+
+  // oracle
+  refcpu::IsingConfiguration oc(c.rows, c.cols, 1);
+  for (int k = 0; k < 5; ++k) oc.occupation[(k * 7919) % (c.rows * c.cols)] = -1;
+  refcpu::CompletionCheckParams q;
+  if (c.max_count > 0) q.cutoff_params.max_count = c.max_count;
+  if (c.min_sample > 0) q.cutoff_params.min_sample = c.min_sample;
+  q.check_begin = 100;
+  q.check_period = 10;
+  if (c.prec > 0) {
+    refcpu::RequestedPrecision rp;
+    rp.abs_convergence_is_required = true;
+    rp.abs_precision = c.prec;
+    q.requested_precision[refcpu::SamplerComponent{"potential_energy", 0}] = rp;
+    q.requested_precision[refcpu::SamplerComponent{"param_composition", 0}] = rp;
+  }
+  auto oeng = std::make_shared<std::mt19937_64>(c.seed);
+  auto o = refcpu::sgc_basic_run(oc, 0.1, c.use_nlist, c.T, c.mu, oeng, q, c.sample_period);
+
+  std::string tag = std::to_string(c.rows) + "x" + std::to_string(c.cols) + " T=" +
+                    std::to_string((int)c.T) + " sp=" + std::to_string(c.sample_period);
+  CountType ns = get_n_samples(data->samplers);
+  EXPECT(ns == refcpu::get_n_samples(o.samplers), tag << " n_samples " << ns);
+  for (auto name : {"formation_energy", "param_composition", "potential_energy"}) {
+    VectorXd a = data->samplers.at(name)->component(0);
+    auto b = o.samplers.at(name)->component(0);
+    bool ok = a.size() == (Index)b.size();
+    for (Index i = 0; ok && i < a.size(); ++i) ok = same_bits(a(i), b[i]);
+    EXPECT(ok, tag << " sampled " << name);
+  }
+  if (c.host_function) {
+    VectorXd m = data->samplers.at("magnetisation")->component(0);
+    bool ok = m.size() == (Index)o.M.size();
+    for (Index i = 0; ok && i < m.size(); ++i) ok = m(i) == (double)o.M[i];
+    EXPECT(ok, tag << " host sampling function");
+  }
+  EXPECT(data->n_pass == o.n_pass, tag << " n_pass " << data->n_pass << " vs " << o.n_pass);
+  EXPECT(data->n_accept == o.n_accept && data->n_reject == o.n_reject, tag << " counts");
+  EXPECT(state.configuration.occupation() == o.final_config.occupation, tag << " final config");
+  EXPECT(*engine == *oeng, tag << " engine state");
+  auto const &ra = data->completion_check.results();
+  auto const &rb = o.results;
+  EXPECT(ra.is_complete == rb.is_complete && ra.n_samples == rb.n_samples &&
+             ra.equilibration_check_results.all_equilibrated ==
+                 rb.equilibration_check_results.all_equilibrated &&
+             ra.equilibration_check_results.N_samples_for_all_to_equilibrate ==
+                 rb.equilibration_check_results.N_samples_for_all_to_equilibrate &&
+             ra.convergence_check_results.all_converged ==
+                 rb.convergence_check_results.all_converged &&
+             data->completion_check.n_checks() == o.n_checks,
+         tag << " completion results");
+  for (auto const &p : rb.convergence_check_results.individual_results) {
+    auto it = ra.convergence_check_results.individual_results.find(
+        SamplerComponent(p.first.sampler_name, p.first.component_index, ""));
+    EXPECT(it != ra.convergence_check_results.individual_results.end() &&
+               same_bits(it->second.stats.mean, p.second.stats.mean) &&
+               same_bits(it->second.stats.calculated_precision, p.second.stats.calculated_precision),
+           tag << " stats " << p.first.sampler_name);
+  }
+  std::cout << "gpu: " << tag << " n_samples=" << ns << " n_pass=" << data->n_pass
+            << (g_fail ? " (failures so far)" : " ok") << std::endl;
+}
+
+// RunManager-driven loop: two fixtures, the FP64 running potential energy and
+// a configuration-reading function, vs the oracle.
+static void gpu_occupation_metropolis_tests() {
+  const int R = 20, C = 20;
+  const long P = 300;
+  const double T = 2500.0, mu = 0.1;
+  const uint64_t seed = 5;
+  typedef ising_cpp::IsingConfiguration config_type;
+  ValueMap cond;
+  cond.scalar_values["temperature"] = T;
+  cond.vector_values["exchange_potential"] = VectorXd{mu};
+  State<config_type> state(config_type({R, C}, 1), cond);
+  IsingSemiGrandCanonicalPotential potential(0.1, true);
+  potential.set(&state);
+
+  StateSamplingFunction pe("potential_energy", "running potential energy", {}, [&state]() {
+    return VectorXd{state.properties.scalar_values.at("potential_energy")};
+  });
+  pe.device_observable = MCG_OBS_POTENTIAL_ENERGY;  // reads no configuration
+  StateSamplingFunction mag("magnetisation", "sum of spins", {}, [&state]() {
+    double s = 0;
+    for (int v : state.configuration.occupation()) s += v;
+    return VectorXd{s};
+  });
+  StateSamplingFunctionMap fns;
+  fns.emplace(pe.name, pe);
+  fns.emplace(mag.name, mag);
+
+  SamplingParams sp1;
+  sp1.sampler_names = {"potential_energy"};
+  CompletionCheckParams<BasicStatistics> cc1;
+  cc1.cutoff_params.max_count = P;
+  SamplingParams sp2;
+  sp2.sampler_names = {"magnetisation"};
+  sp2.begin = 0.0;
+  sp2.period = 7.0;
+  CompletionCheckParams<BasicStatistics> cc2;
+  typedef RunManager<config_type, BasicStatistics, std::mt19937_64> rm_type;
+  std::vector<rm_type::sampling_fixture_params_type> fixtures;
+  fixtures.emplace_back("thermo", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), sp1, cc1,
+                        std::vector<std::string>{});
+  fixtures.emplace_back("sparse", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), sp2, cc2,
+                        std::vector<std::string>{});
+  auto engine = std::make_shared<std::mt19937_64>(seed);
+  rm_type rm(engine, fixtures, true);
+  RandomNumberGenerator<std::mt19937_64> rng(engine);
+  occupation_metropolis(state, potential, rng, rm);
+
+  // oracle: running energy per pass and integer observables per pass
+  auto e1 = std::make_shared<std::mt19937_64>(seed);
+  double pe0 = 0.0;
+  auto run_e = refcpu::sgc_running_energy_run(refcpu::IsingConfiguration(R, C, 1), 0.1, true, T,
+                                              mu, e1, P, &pe0);
+  auto e2 = std::make_shared<std::mt19937_64>(seed);
+  refcpu::CompletionCheckParams q;
+  q.cutoff_params.max_count = P;
+  auto o = refcpu::sgc_basic_run(refcpu::IsingConfiguration(R, C, 1), 0.1, true, T, mu, e2, q, 1);
+
+  auto const &thermo = rm.sampling_fixtures[0]->results();
+  VectorXd got = thermo.samplers.at("potential_energy")->component(0);
+  bool ok = got.size() == P;
+  for (long i = 0; ok && i < P; ++i) ok = same_bits(got(i), run_e[i]);
+  EXPECT(ok, "occupation_metropolis running energy (" << got.size() << " samples)");
+  auto const &sparse = rm.sampling_fixtures[1]->results();
+  VectorXd m = sparse.samplers.at("magnetisation")->component(0);
+  ok = m.size() == (P / 7) + 1 && m(0) == (double)(R * C);
+  for (Index k = 1; ok && k < m.size(); ++k) ok = m(k) == (double)o.M[7 * k - 1];
+  EXPECT(ok, "occupation_metropolis second fixture (" << m.size() << " samples)");
+  EXPECT(sparse.sample_count.size() == (size_t)m.size() && sparse.sample_count[1] == 7,
+         "sample counts");
+  EXPECT(*engine == *e1, "occupation_metropolis engine state");
+  EXPECT(state.configuration.occupation() == o.final_config.occupation, "final configuration");
+  EXPECT(thermo.n_accept == o.n_accept && thermo.n_reject == o.n_reject, "acceptance counts");
+  EXPECT(thermo.completion_check_results.is_complete, "thermo fixture complete");
+  std::cout << "gpu: occupation_metropolis RunManager 20x20, 2 fixtures"
+            << (g_fail ? " (failures so far)" : " ok") << std::endl;
+}
+
+static void gpu_tests() {
+  // the reference's SemiGrandCanonicalRun1 set-up, converged on two observables
+  gpu_case({25, 25, true, 2000.0, 0.0, 42, 1, -1, 0.001, 100, false});
+  gpu_case({25, 25, false, 2000.0, 0.0, 43, 1, -1, 0.001, 100, false});
+  // cutoff-terminated, sample period 3, non-zero exchange potential
+  gpu_case({16, 20, true, 3000.0, 0.2, 7, 3, 900, -1.0, -1, false});
+  // a host-evaluated sampling function (lattice refreshed at every sample)
+  gpu_case({12, 12, true, 2500.0, -0.1, 9, 2, 400, 0.002, 100, true});
+  // tiny lattice: pending periodic checks span passes
+  gpu_case({3, 3, true, 4000.0, 0.0, 11, 1, 3000, 1e-4, 500, false});
+  gpu_occupation_metropolis_tests();
+}
+
+int main(int argc, char **argv) {
+  bool cpu = argc > 1 && std::string(argv[1]) == "--cpu";
+  bool gpu = argc > 1 && std::string(argv[1]) == "--gpu";
+  try {
+    if (cpu) cpu_tests();
+    if (gpu) gpu_tests();
+  } catch (std::exception const &e) {
+    std::cout << "EXCEPTION: " << e.what() << std::endl;
+    return 2;
+  }
+  if (g_fail) {
+    std::cout << g_fail << " FAILURES" << std::endl;
+    return 1;
+  }
+  std::cout << "ALL OK" << std::endl;
+  return 0;
+}
