@@ -16,7 +16,7 @@
 namespace mcg {
 void exact_set_max_lds(int bytes);
 void checker_set_max_lds(int bytes);
-int checker_quads_per_item();
+int checker_units_per_item();
 }  // namespace mcg
 
 using namespace mcg;
@@ -88,6 +88,24 @@ double ising_per_supercell(std::vector<int8_t> const &occ, int64_t rows, int64_t
     e += -J * static_cast<double>(dot);
   }
   return e;
+}
+
+// CHECKERBOARD device layout: every line of L0 sites stored as its even-i
+// sites then its odd-i sites, one byte per site, 1 for spin +1 and 0 for -1.
+void to_split_rows(std::vector<int8_t> const &nat, int64_t L0, std::vector<int8_t> &dev) {
+  const int64_t half = L0 / 2, n = (int64_t)nat.size();
+  for (int64_t l = 0; l < n; ++l) {
+    const int64_t line = l / L0, i = l - line * L0;
+    dev[line * L0 + (i & 1) * half + (i >> 1)] = nat[l] > 0 ? 1 : 0;
+  }
+}
+
+void from_split_rows(std::vector<int8_t> const &dev, int64_t L0, std::vector<int8_t> &nat) {
+  const int64_t half = L0 / 2, n = (int64_t)dev.size();
+  for (int64_t l = 0; l < n; ++l) {
+    const int64_t line = l / L0, i = l - line * L0;
+    nat[l] = dev[line * L0 + (i & 1) * half + (i >> 1)] ? 1 : -1;
+  }
 }
 
 struct LatticeGeom {
@@ -311,6 +329,7 @@ int build_checker(mcg_handle *h) {
     P.n = h->sites(c);
     P.lat_off = h->lat_off[c];
     P.tab_off = (int64_t)ntab * c;
+    P.quad_only = (h->options & MCG_OPT_CHECKER_QUAD_PATH) ? 1 : 0;
     max_n = std::max(max_n, P.n);
     const double beta = 1.0 / (h->kb * ch.temperature);
     const double mu = ch.exchange_potential;
@@ -380,16 +399,19 @@ int reset_observables(mcg_handle *h, int64_t c, std::vector<int8_t> const &occ) 
 }
 
 int rebuild_work(mcg_handle *h) {
-  const int qpi = checker_quads_per_item();
+  const int upi = checker_units_per_item();
   std::vector<int32_t> work;
   for (int c = 0; c < h->n_chains; ++c) {
     if (!h->active[c]) continue;
     auto const &P = h->cp[c];
+    // work units of one colour: 16-site chunks when the half line is a
+    // multiple of 16, else 4-site quads (checker.hip)
     int64_t half0 = P.L0 / 2, qpl = (half0 + 3) / 4;
+    int64_t per_line = (half0 % 16 == 0 && !P.quad_only) ? half0 / 16 : qpl;
     int64_t nl = P.ncolor == 2 ? (int64_t)P.L1 * P.L2
                                : (P.ncolor == 4 ? P.L1 / 2 : (int64_t)(P.L1 / 2) * (P.L2 / 2));
-    int64_t nq = nl * qpl;
-    int64_t items = (nq + qpi - 1) / qpi;
+    int64_t nq = nl * per_line;
+    int64_t items = (nq + upi - 1) / upi;
     for (int64_t it = 0; it < items; ++it) {
       work.push_back(c);
       work.push_back((int32_t)it);
@@ -523,7 +545,13 @@ int mcg_set_occupation(mcg_handle *h, int64_t chain, const int8_t *occ, int64_t 
   for (int8_t s : v)
     if (s != 1 && s != -1) return fail(MCG_ERR_ARG, "occupation values must be +1 or -1");
   HIP_TRY(hipStreamSynchronize(h->stream));
-  HIP_TRY(hipMemcpy(h->d_lat + h->lat_off[chain], v.data(), n, hipMemcpyHostToDevice));
+  if (h->mode == MCG_MODE_EXACT) {
+    HIP_TRY(hipMemcpy(h->d_lat + h->lat_off[chain], v.data(), n, hipMemcpyHostToDevice));
+  } else {
+    std::vector<int8_t> dev(n);
+    to_split_rows(v, h->chains[chain].shape[0], dev);
+    HIP_TRY(hipMemcpy(h->d_lat + h->lat_off[chain], dev.data(), n, hipMemcpyHostToDevice));
+  }
   return reset_observables(h, chain, v);
 }
 
@@ -531,7 +559,14 @@ int mcg_get_occupation(mcg_handle *h, int64_t chain, int8_t *occ, int64_t n) {
   if (!valid_chain(h, chain) || !occ) return fail(MCG_ERR_ARG, "bad chain or buffer");
   if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "occupation size mismatch");
   HIP_TRY(hipStreamSynchronize(h->stream));
-  HIP_TRY(hipMemcpy(occ, h->d_lat + h->lat_off[chain], n, hipMemcpyDeviceToHost));
+  if (h->mode == MCG_MODE_EXACT) {
+    HIP_TRY(hipMemcpy(occ, h->d_lat + h->lat_off[chain], n, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<int8_t> dev(n), nat(n);
+    HIP_TRY(hipMemcpy(dev.data(), h->d_lat + h->lat_off[chain], n, hipMemcpyDeviceToHost));
+    from_split_rows(dev, h->chains[chain].shape[0], nat);
+    std::copy(nat.begin(), nat.end(), occ);
+  }
   return MCG_OK;
 }
 

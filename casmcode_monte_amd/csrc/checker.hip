@@ -10,14 +10,26 @@
 // 3D NN: (i+j+k)&1;  3D NN+NNN: (i&1)+2(j&1)+4(k&1) (the face diagonals of one
 // parity sublattice form FCC nearest-neighbour tetrahedra, so 4 colours fail).
 //
-// Work unit = a "quad": 4 sites of one colour that are consecutive along i in
-// one lattice line.  Each quad takes one Philox4x32-10 call keyed by the chain
-// seed with counter (quad index, pass lo, pass hi, colour), so the random
-// stream of a site depends only on (chain seed, pass, site): the LDS-resident
-// kernel, the HBM work-list kernel, a chain run alone and a chain run in a
-// batch all produce the same trajectory bit for bit.
+// Device layout ("split rows"): every lattice line (fixed j,k) of L0 sites is
+// stored as its L0/2 even-i sites followed by its L0/2 odd-i sites, one byte
+// per site holding 1 for spin +1 and 0 for -1.  A colour owns one half of each
+// of its lines, so a sweep reads and writes contiguous runs: the site at half
+// a, index h (i = a + 2h) has its in-line neighbours in the other half at
+// h-1+a and h+a, and its j/k neighbours at index h of the same or the other
+// half of the neighbouring lines.
 //
-// Acceptance: class (old spin, NN sum, NNN sum) -> 32-bit threshold t with
+// Two code paths give identical results:
+//   chunk path (L0 % 32 == 0): a thread owns 16 consecutive sites of one half
+//     line; neighbour counts are formed four sites per 32-bit add (byte lanes)
+//     from 16-byte loads, the in-line shifts with v_alignbyte;
+//   quad path (any even L0): a thread owns 4 consecutive sites, byte loads.
+// Random numbers: site h of colour-class line ln (lines of one colour in
+// order) uses word h&3 of Philox4x32-10 keyed by the chain seed with counter
+// (ln*ceil(L0/8) + h/4, pass lo, pass hi, colour), so a site's stream depends
+// only on (seed, pass, colour, site): both paths, the LDS-resident kernel, the
+// HBM work-list kernel, a chain alone and a chain in a batch agree bit for bit.
+//
+// Acceptance: class (spin, up-neighbour counts) -> 32-bit threshold t with
 // accept iff r <= t (t = 2^32-1 when dE <= 0), tabulated per chain on the host
 // with exp(-beta*dE); no exp per attempt.  Observables are exact integers
 // (S1 = sum_NN s s, S2 = sum_NNN s s, M = sum s) updated from per-flip deltas
@@ -26,16 +38,18 @@
 
 namespace mcg {
 
-struct QuadGeom {
-  int32_t half0;  // sites of one colour per line (L0/2)
-  int32_t qpl;    // quads per line
-  int32_t nl;     // lines per colour
+struct LineGeom {
+  int32_t half;     // sites per half line (L0/2)
+  int32_t qpl;      // quads per half line, ceil(half/4)
+  int32_t nl;       // lines per colour
+  int32_t chunked;  // 16-site chunk path (half % 16 == 0)
 };
 
-__device__ __forceinline__ QuadGeom quad_geom(const CheckerParams &P) {
-  QuadGeom g;
-  g.half0 = P.L0 >> 1;
-  g.qpl = (g.half0 + 3) >> 2;
+__device__ __forceinline__ LineGeom line_geom(const CheckerParams &P) {
+  LineGeom g;
+  g.half = P.L0 >> 1;
+  g.qpl = (g.half + 3) >> 2;
+  g.chunked = ((g.half & 15) == 0) && !P.quad_only;
   if (P.ncolor == 2)
     g.nl = P.L1 * P.L2;
   else if (P.ncolor == 4)
@@ -47,16 +61,9 @@ __device__ __forceinline__ QuadGeom quad_geom(const CheckerParams &P) {
 
 __device__ __forceinline__ int wrapm(int a, int L) { return a < 0 ? a + L : (a >= L ? a - L : a); }
 
-// Update the (up to) 4 sites of quad `qd` of colour `color`.
-// Returns per-quad deltas through references.
-__device__ __forceinline__ void update_quad(const CheckerParams &P, const QuadGeom &g,
-                                            int8_t *__restrict__ lat,
-                                            const uint32_t *__restrict__ tab, int color,
-                                            int64_t pass, int64_t qd, int &dS1, int &dS2,
-                                            int &dM, int &nacc) {
-  const int ln = (int)(qd / g.qpl);
-  const int qi = (int)(qd - (int64_t)ln * g.qpl);
-  int j, k, a;
+// Line (j, k) and own half a of colour-class line ln.
+__device__ __forceinline__ void line_of(const CheckerParams &P, int color, int ln, int &j, int &k,
+                                        int &a) {
   if (P.ncolor == 2) {
     j = ln % P.L1;
     k = ln / P.L1;
@@ -71,55 +78,201 @@ __device__ __forceinline__ void update_quad(const CheckerParams &P, const QuadGe
     j = ((color >> 1) & 1) + 2 * (ln % h1);
     k = ((color >> 2) & 1) + 2 * (ln / h1);
   }
-  const int L0 = P.L0, L1 = P.L1, L2 = P.L2;
-  const int jp = wrapm(j + 1, L1), jm = wrapm(j - 1, L1);
-  const int64_t plane = (int64_t)L0 * L1;
-  const int64_t kb = (int64_t)k * plane;
-  int8_t *row = lat + kb + (int64_t)j * L0;
-  const int8_t *rjp = lat + kb + (int64_t)jp * L0;
-  const int8_t *rjm = lat + kb + (int64_t)jm * L0;
-  const int8_t *rkp = row, *rkm = row;
-  const int8_t *rjp_kp = row, *rjp_km = row, *rjm_kp = row, *rjm_km = row;
+}
+
+// Base pointers of the half lines around line (j, k): own line, j+-1, and in
+// 3D k+-1 and the four (j+-1, k+-1) lines; `a` selects the half.
+struct Lines {
+  uint8_t *own;
+  const uint8_t *jp, *jm, *kp, *km, *jpkp, *jpkm, *jmkp, *jmkm;
+};
+
+__device__ __forceinline__ Lines lines_of(const CheckerParams &P, uint8_t *lat, int j, int k) {
+  const int64_t L0 = P.L0, plane = (int64_t)P.L0 * P.L1;
+  const int jp = wrapm(j + 1, P.L1), jm = wrapm(j - 1, P.L1);
+  Lines Ls;
+  Ls.own = lat + (int64_t)k * plane + j * L0;
+  Ls.jp = lat + (int64_t)k * plane + jp * L0;
+  Ls.jm = lat + (int64_t)k * plane + jm * L0;
+  Ls.kp = Ls.km = Ls.jpkp = Ls.jpkm = Ls.jmkp = Ls.jmkm = Ls.own;
   if (P.ndim == 3) {
-    const int kp = wrapm(k + 1, L2), km = wrapm(k - 1, L2);
-    rkp = lat + (int64_t)kp * plane + (int64_t)j * L0;
-    rkm = lat + (int64_t)km * plane + (int64_t)j * L0;
-    rjp_kp = lat + (int64_t)kp * plane + (int64_t)jp * L0;
-    rjp_km = lat + (int64_t)km * plane + (int64_t)jp * L0;
-    rjm_kp = lat + (int64_t)kp * plane + (int64_t)jm * L0;
-    rjm_km = lat + (int64_t)km * plane + (int64_t)jm * L0;
+    const int kp = wrapm(k + 1, P.L2), km = wrapm(k - 1, P.L2);
+    Ls.kp = lat + (int64_t)kp * plane + j * L0;
+    Ls.km = lat + (int64_t)km * plane + j * L0;
+    Ls.jpkp = lat + (int64_t)kp * plane + jp * L0;
+    Ls.jpkm = lat + (int64_t)km * plane + jp * L0;
+    Ls.jmkp = lat + (int64_t)kp * plane + jm * L0;
+    Ls.jmkm = lat + (int64_t)km * plane + jm * L0;
   }
-  const int64_t qglob = (int64_t)ln * g.qpl + qi;
-  const u32x4 r = philox4x32_10((uint32_t)qglob, (uint32_t)pass, (uint32_t)(pass >> 32),
-                                (uint32_t)color, P.key0, P.key1);
+  return Ls;
+}
+
+__device__ __forceinline__ void accept_site(int b, int c1, int c2, uint32_t r,
+                                            const uint32_t *__restrict__ tab, int z1, int z2,
+                                            bool &flip, int &dS1, int &dS2, int &dM, int &nacc) {
+  const int cls = b * (z1 + 1) * (z2 + 1) + c1 * (z2 + 1) + c2;
+  flip = r <= tab[cls];
+  if (flip) {
+    const int s = 2 * b - 1;
+    dS1 += -2 * s * (2 * c1 - z1);
+    dS2 += -2 * s * (2 * c2 - z2);
+    dM += -2 * s;
+    nacc += 1;
+  }
+}
+
+// --- quad path ------------------------------------------------------------------
+__device__ __forceinline__ void update_quad(const CheckerParams &P, const LineGeom &g,
+                                            uint8_t *__restrict__ lat,
+                                            const uint32_t *__restrict__ tab, int color,
+                                            int64_t pass, int64_t qd, int &dS1, int &dS2,
+                                            int &dM, int &nacc) {
+  const int ln = (int)(qd / g.qpl);
+  const int qi = (int)(qd - (int64_t)ln * g.qpl);
+  int j, k, a;
+  line_of(P, color, ln, j, k, a);
+  const Lines Ls = lines_of(P, lat, j, k);
+  const int half = g.half;
+  const int oa = a * half, ob = (1 - a) * half;  // own half, other half offsets
+  const u32x4 r = philox4x32_10((uint32_t)((int64_t)ln * g.qpl + qi), (uint32_t)pass,
+                                (uint32_t)(pass >> 32), (uint32_t)color, P.key0, P.key1);
   const int z1 = P.z1, z2 = P.z2;
-  const int w2 = z2 + 1;
-  const int cls_s = (z1 + 1) * w2;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const int ip = 4 * qi + u;
-    if (ip >= g.half0) break;
-    const int i = a + 2 * ip;
-    const int ie = wrapm(i + 1, L0), iw = wrapm(i - 1, L0);
-    const int s = row[i];
-    int nn = row[ie] + row[iw] + rjp[i] + rjm[i];
-    int nnn = 0;
-    if (P.ndim == 3) nn += rkp[i] + rkm[i];
+    const int h = 4 * qi + u;
+    if (h >= half) break;
+    const int he = wrapm(h + a, half), hw = wrapm(h - 1 + a, half);
+    const int b = Ls.own[oa + h];
+    int c1 = Ls.own[ob + he] + Ls.own[ob + hw] + Ls.jp[oa + h] + Ls.jm[oa + h];
+    int c2 = 0;
+    if (P.ndim == 3) c1 += Ls.kp[oa + h] + Ls.km[oa + h];
     if (z2 > 0) {
-      nnn = rjp[ie] + rjp[iw] + rjm[ie] + rjm[iw];
+      c2 = Ls.jp[ob + he] + Ls.jp[ob + hw] + Ls.jm[ob + he] + Ls.jm[ob + hw];
       if (P.ndim == 3)
-        nnn += rkp[ie] + rkp[iw] + rkm[ie] + rkm[iw] + rjp_kp[i] + rjp_km[i] + rjm_kp[i] +
-               rjm_km[i];
+        c2 += Ls.kp[ob + he] + Ls.kp[ob + hw] + Ls.km[ob + he] + Ls.km[ob + hw] +
+              Ls.jpkp[oa + h] + Ls.jpkm[oa + h] + Ls.jmkp[oa + h] + Ls.jmkm[oa + h];
     }
-    const int cls = (s > 0 ? cls_s : 0) + ((nn + z1) >> 1) * w2 + ((nnn + z2) >> 1);
-    if (r.v[u] <= tab[cls]) {
-      row[i] = (int8_t)(-s);
-      dS1 += -2 * s * nn;
-      dS2 += -2 * s * nnn;
-      dM += -2 * s;
-      nacc += 1;
+    bool flip;
+    accept_site(b, c1, c2, r.v[u], tab, z1, z2, flip, dS1, dS2, dM, nacc);
+    if (flip) Ls.own[oa + h] = (uint8_t)(b ^ 1);
+  }
+}
+
+// --- chunk path (16 sites, SWAR) -------------------------------------------------
+__device__ __forceinline__ uint4 ld16(const uint8_t *p) {
+  return *reinterpret_cast<const uint4 *>(p);
+}
+
+// element e <- e+1, `last` (low byte) enters at element 15
+__device__ __forceinline__ uint4 shift_down(uint4 v, uint32_t last) {
+  uint4 r;
+  r.x = __builtin_amdgcn_alignbyte(v.y, v.x, 1);
+  r.y = __builtin_amdgcn_alignbyte(v.z, v.y, 1);
+  r.z = __builtin_amdgcn_alignbyte(v.w, v.z, 1);
+  r.w = __builtin_amdgcn_alignbyte(last, v.w, 1);
+  return r;
+}
+
+// element e <- e-1, `first` (low byte) enters at element 0
+__device__ __forceinline__ uint4 shift_up(uint4 v, uint32_t first) {
+  uint4 r;
+  r.x = __builtin_amdgcn_alignbyte(v.x, first << 24, 3);
+  r.y = __builtin_amdgcn_alignbyte(v.y, v.x, 3);
+  r.z = __builtin_amdgcn_alignbyte(v.z, v.y, 3);
+  r.w = __builtin_amdgcn_alignbyte(v.w, v.z, 3);
+  return r;
+}
+
+__device__ __forceinline__ uint4 add4(uint4 a, uint4 b) {
+  return make_uint4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+
+// In-line neighbour pair of the 16 own sites [h0, h0+16) from the other half
+// line `oth` (base of that half): returns east + west.
+__device__ __forceinline__ uint4 inline_pair(const uint8_t *oth, int h0, int a, int half) {
+  const uint4 v = ld16(oth + h0);
+  if (a == 0) {  // east = h, west = h-1
+    const uint32_t first = oth[wrapm(h0 - 1, half)];
+    return add4(v, shift_up(v, first));
+  }
+  // a == 1: west = h, east = h+1
+  const uint32_t last = oth[wrapm(h0 + 16, half)];
+  return add4(v, shift_down(v, last));
+}
+
+__device__ __forceinline__ uint32_t word_of(uint4 v, int w) {
+  return w == 0 ? v.x : (w == 1 ? v.y : (w == 2 ? v.z : v.w));
+}
+
+__device__ __forceinline__ void update_chunk(const CheckerParams &P, const LineGeom &g,
+                                             uint8_t *__restrict__ lat,
+                                             const uint32_t *__restrict__ tab, int color,
+                                             int64_t pass, int64_t cd, int &dS1, int &dS2,
+                                             int &dM, int &nacc) {
+  const int cpl = g.half >> 4;  // chunks per half line
+  const int ln = (int)(cd / cpl);
+  const int h0 = (int)(cd - (int64_t)ln * cpl) << 4;
+  int j, k, a;
+  line_of(P, color, ln, j, k, a);
+  const Lines Ls = lines_of(P, lat, j, k);
+  const int half = g.half;
+  const int oa = a * half, ob = (1 - a) * half;
+  const int z1 = P.z1, z2 = P.z2;
+
+  uint4 own = ld16(Ls.own + oa + h0);
+  uint4 c1 = add4(inline_pair(Ls.own + ob, h0, a, half),
+                  add4(ld16(Ls.jp + oa + h0), ld16(Ls.jm + oa + h0)));
+  if (P.ndim == 3) c1 = add4(c1, add4(ld16(Ls.kp + oa + h0), ld16(Ls.km + oa + h0)));
+  uint4 c2 = make_uint4(0, 0, 0, 0);
+  if (z2 > 0) {
+    c2 = add4(inline_pair(Ls.jp + ob, h0, a, half), inline_pair(Ls.jm + ob, h0, a, half));
+    if (P.ndim == 3) {
+      c2 = add4(c2, add4(inline_pair(Ls.kp + ob, h0, a, half),
+                         inline_pair(Ls.km + ob, h0, a, half)));
+      c2 = add4(c2, add4(add4(ld16(Ls.jpkp + oa + h0), ld16(Ls.jpkm + oa + h0)),
+                         add4(ld16(Ls.jmkp + oa + h0), ld16(Ls.jmkm + oa + h0))));
     }
   }
+  const uint32_t q0 = (uint32_t)((int64_t)ln * g.qpl + (h0 >> 2));
+  uint32_t flipw[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const u32x4 r = philox4x32_10(q0 + w, (uint32_t)pass, (uint32_t)(pass >> 32),
+                                  (uint32_t)color, P.key0, P.key1);
+    const uint32_t ow = word_of(own, w), w1 = word_of(c1, w), w2 = word_of(c2, w);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int b = (ow >> (8 * u)) & 1;
+      const int n1 = (w1 >> (8 * u)) & 0xFF, n2 = (w2 >> (8 * u)) & 0xFF;
+      bool flip;
+      accept_site(b, n1, n2, r.v[u], tab, z1, z2, flip, dS1, dS2, dM, nacc);
+      flipw[w] |= (flip ? 1u : 0u) << (8 * u);
+    }
+  }
+  if (flipw[0] | flipw[1] | flipw[2] | flipw[3]) {
+    own.x ^= flipw[0];
+    own.y ^= flipw[1];
+    own.z ^= flipw[2];
+    own.w ^= flipw[3];
+    *reinterpret_cast<uint4 *>(Ls.own + oa + h0) = own;
+  }
+}
+
+// Work units of one colour of one chain: chunks (chunk path) or quads.
+__device__ __forceinline__ bool chunk_path(const LineGeom &g) { return g.chunked != 0; }
+
+__device__ __forceinline__ int64_t n_units(const LineGeom &g) {
+  return (int64_t)g.nl * (chunk_path(g) ? (g.half >> 4) : g.qpl);
+}
+
+__device__ __forceinline__ void update_unit(const CheckerParams &P, const LineGeom &g,
+                                            uint8_t *lat, const uint32_t *tab, int color,
+                                            int64_t pass, int64_t unit, int &dS1, int &dS2,
+                                            int &dM, int &nacc) {
+  if (chunk_path(g))
+    update_chunk(P, g, lat, tab, color, pass, unit, dS1, dS2, dM, nacc);
+  else
+    update_quad(P, g, lat, tab, color, pass, unit, dS1, dS2, dM, nacc);
 }
 
 __device__ __forceinline__ int wave_sum(int v) {
@@ -139,26 +292,25 @@ __global__ __launch_bounds__(1024) void checker_lds_kernel(
   const int c = blockIdx.x;
   if (!active[c]) return;
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int red[4][32];
+  __shared__ int red[4][16];
   __shared__ uint32_t tab[2 * 7 * 13];
   const CheckerParams P = Pp[c];
   CheckerState *St = Sp + c;
-  int8_t *g_lat = lat_pool + P.lat_off;
-  int8_t *lat = reinterpret_cast<int8_t *>(smem);
+  uint8_t *g_lat = reinterpret_cast<uint8_t *>(lat_pool + P.lat_off);
+  uint8_t *lat = reinterpret_cast<uint8_t *>(smem);
   const int t = threadIdx.x, nt = blockDim.x;
   const int ntab = 2 * (P.z1 + 1) * (P.z2 + 1);
   for (int i = t; i < ntab; i += nt) tab[i] = tables[P.tab_off + i];
   const int64_t n = P.n;
-  if ((n & 15) == 0) {
+  const int64_t n16 = (n + 15) >> 4;  // lattices are padded to 16 bytes
+  {
     const uint4 *src = reinterpret_cast<const uint4 *>(g_lat);
     uint4 *dst = reinterpret_cast<uint4 *>(lat);
-    for (int64_t i = t; i < (n >> 4); i += nt) dst[i] = src[i];
-  } else {
-    for (int64_t i = t; i < n; i += nt) lat[i] = g_lat[i];
+    for (int64_t i = t; i < n16; i += nt) dst[i] = src[i];
   }
   __syncthreads();
-  const QuadGeom g = quad_geom(P);
-  const int64_t nq = (int64_t)g.nl * g.qpl;
+  const LineGeom g = line_geom(P);
+  const int64_t nu = n_units(g);
   int64_t pass = St->n_pass;
   int64_t S1 = St->S1, S2 = St->S2, M = St->M, nacc_tot = St->n_accept;
   int64_t n_samples = St->n_samples;
@@ -166,8 +318,8 @@ __global__ __launch_bounds__(1024) void checker_lds_kernel(
   for (int64_t p = 0; p < n_passes; ++p) {
     int dS1 = 0, dS2 = 0, dM = 0, nacc = 0;
     for (int color = 0; color < P.ncolor; ++color) {
-      for (int64_t qd = t; qd < nq; qd += nt)
-        update_quad(P, g, lat, tab, color, pass, qd, dS1, dS2, dM, nacc);
+      for (int64_t u = t; u < nu; u += nt)
+        update_unit(P, g, lat, tab, color, pass, u, dS1, dS2, dM, nacc);
       __syncthreads();
     }
     dS1 = wave_sum(dS1);
@@ -208,12 +360,10 @@ __global__ __launch_bounds__(1024) void checker_lds_kernel(
       ++n_samples;
     }
   }
-  if ((n & 15) == 0) {
+  {
     const uint4 *src = reinterpret_cast<const uint4 *>(lat);
     uint4 *dst = reinterpret_cast<uint4 *>(g_lat);
-    for (int64_t i = t; i < (n >> 4); i += nt) dst[i] = src[i];
-  } else {
-    for (int64_t i = t; i < n; i += nt) g_lat[i] = lat[i];
+    for (int64_t i = t; i < n16; i += nt) dst[i] = src[i];
   }
   if (t == 0) {
     St->n_pass = pass;
@@ -239,12 +389,12 @@ void checker_set_max_lds(int bytes) {
 }
 
 // ---------------------------------------------------------------------------
-// HBM work-list kernel: one launch per colour over a list of (chain, first
-// quad) work items, QUADS_PER_ITEM quads each; the lattice streams from HBM.
+// HBM work-list kernel: one launch per colour over a list of (chain, item)
+// entries; an item is UNITS_PER_ITEM work units of one chain.
 // ---------------------------------------------------------------------------
 constexpr int HBM_BLOCK = 256;
-constexpr int QUADS_PER_THREAD = 4;
-constexpr int QUADS_PER_ITEM = HBM_BLOCK * QUADS_PER_THREAD;
+constexpr int UNITS_PER_THREAD = 2;
+constexpr int UNITS_PER_ITEM = HBM_BLOCK * UNITS_PER_THREAD;
 
 __global__ __launch_bounds__(HBM_BLOCK) void checker_color_kernel(
     const CheckerParams *__restrict__ Pp, CheckerState *__restrict__ Sp,
@@ -252,20 +402,20 @@ __global__ __launch_bounds__(HBM_BLOCK) void checker_color_kernel(
     const int32_t *__restrict__ work, int color) {
   __shared__ int red[4][HBM_BLOCK / 64];
   const int c = work[2 * blockIdx.x];
-  const int64_t q0 = (int64_t)work[2 * blockIdx.x + 1] * QUADS_PER_ITEM;
+  const int64_t u0 = (int64_t)work[2 * blockIdx.x + 1] * UNITS_PER_ITEM;
   const CheckerParams P = Pp[c];
   CheckerState *St = Sp + c;
-  const QuadGeom g = quad_geom(P);
-  const int64_t nq = (int64_t)g.nl * g.qpl;
+  const LineGeom g = line_geom(P);
+  const int64_t nu = n_units(g);
   const int64_t pass = St->n_pass;
   const uint32_t *tab = tables + P.tab_off;
-  int8_t *lat = lat_pool + P.lat_off;
+  uint8_t *lat = reinterpret_cast<uint8_t *>(lat_pool + P.lat_off);
   int dS1 = 0, dS2 = 0, dM = 0, nacc = 0;
   const int t = threadIdx.x;
 #pragma unroll
-  for (int u = 0; u < QUADS_PER_THREAD; ++u) {
-    const int64_t qd = q0 + (int64_t)u * HBM_BLOCK + t;
-    if (qd < nq) update_quad(P, g, lat, tab, color, pass, qd, dS1, dS2, dM, nacc);
+  for (int u = 0; u < UNITS_PER_THREAD; ++u) {
+    const int64_t unit = u0 + (int64_t)u * HBM_BLOCK + t;
+    if (unit < nu) update_unit(P, g, lat, tab, color, pass, unit, dS1, dS2, dM, nacc);
   }
   dS1 = wave_sum(dS1);
   dS2 = wave_sum(dS2);
@@ -338,6 +488,6 @@ void launch_checker_record(const CheckerParams *P, CheckerState *St, mcg_sample 
                      St, series, series_cap, active, n_chains, sample_period);
 }
 
-int checker_quads_per_item() { return QUADS_PER_ITEM; }
+int checker_units_per_item() { return UNITS_PER_ITEM; }
 
 }  // namespace mcg

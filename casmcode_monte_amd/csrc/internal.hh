@@ -60,7 +60,7 @@ struct CheckerParams {
   int32_t ncolor;          // 2, 4 or 8
   int32_t z1, z2;          // coordination numbers (z2 = 0 without J2)
   int32_t ndim;            // 2 or 3
-  int32_t pad0;
+  int32_t quad_only;       // 1: use the 4-site path even when 16-site chunks fit
   uint32_t key0, key1;     // Philox key (chain seed)
   int64_t n;               // sites
   int64_t lat_off;         // byte offset in the lattice pool

@@ -86,6 +86,10 @@ typedef struct mcg_desc {
  * block-speculative walk (both reproduce the reference stream bit for bit;
  * the scalar walk is the cross-check of the speculative one). */
 #define MCG_OPT_EXACT_SCALAR_WALK 2
+/* CHECKERBOARD: update 4 sites per thread with byte loads even where the
+ * 16-site vector path applies (both give identical trajectories; this is the
+ * cross-check of the vector path). */
+#define MCG_OPT_CHECKER_QUAD_PATH 4
 
 /* One sampled record of one chain.  Integer observables are exact; the
  * reference's FP64 sampling functions (basic_semigrand_canonical.hh:485-590)

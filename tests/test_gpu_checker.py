@@ -155,3 +155,23 @@ def test_three_sigma_vs_oracle_sequential(ndim, L, J2, T, mu):
         mb, pb = O.basic_statistics(b)
         sigma = math.hypot(pa, pb) / 1.959049
         assert abs(ma - mb) < 3 * sigma + 1e-12, (idx, ma, mb, sigma)
+
+
+@pytest.mark.parametrize("lattice,shape,J2,streaming", [
+    (cm.LATTICE_SQUARE, (64, 32), -0.02, 0),
+    (cm.LATTICE_SQUARE, (32, 30), 0.0, cm.OPT_FORCE_STREAMING),
+    (cm.LATTICE_SC, (32, 8, 6), -0.02, 0),
+    (cm.LATTICE_SC, (64, 4, 4), 0.0, cm.OPT_FORCE_STREAMING),
+])
+def test_vector_path_equals_quad_path(lattice, shape, J2, streaming):
+    """16-site vector (SWAR) path vs the 4-site byte path: identical trajectories."""
+    print("case: chunk vs quad", lattice, shape, J2, streaming, flush=True)
+    ch = [dict(shape=shape, temperature=2600.0, exchange_potential=-0.03, seed=91)]
+    a = run(ch, lattice, 0.1, J2, 25, options=streaming)
+    b = run(ch, lattice, 0.1, J2, 25, options=streaming | cm.OPT_CHECKER_QUAD_PATH)
+    np.testing.assert_array_equal(a.get_occupation(0), b.get_occupation(0))
+    np.testing.assert_array_equal(a.samples(0), b.samples(0))
+    S1, S2, M = O.lattice_observables(3 if lattice == cm.LATTICE_SC else 2, shape,
+                                      a.get_occupation(0))
+    last = a.samples(0)[-1]
+    assert (last["S1"], last["S2"] if J2 else S2, last["M"]) == (S1, S2, M)
