@@ -34,7 +34,7 @@ EXPORTED = [
     "mcg_create", "mcg_destroy", "mcg_last_error", "mcg_set_occupation", "mcg_get_occupation",
     "mcg_rng_dump", "mcg_rng_load", "mcg_run", "mcg_set_active", "mcg_get_counters",
     "mcg_n_samples", "mcg_get_samples", "mcg_clear_samples", "mcg_stats_compute",
-    "mcg_equilibration", "mcg_last_run_time",
+    "mcg_equilibration", "mcg_last_run_time", "mcg_assign_chain",
 ]
 
 
@@ -90,6 +90,7 @@ def lib():
         L.mcg_equilibration.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_double,
                                         P(C.c_int64), P(C.c_int64)]
         L.mcg_last_run_time.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+        L.mcg_assign_chain.argtypes = [C.c_void_p, C.c_int64, P(ChainDesc)]
         _lib = L
     return _lib
 
@@ -172,6 +173,18 @@ class Run:
 
     def run(self, n_passes):
         _check(lib().mcg_run(self.h, int(n_passes)))
+
+    def assign(self, c, chain):
+        """Re-initialise slot c for a new run (shape within the slot's capacity)."""
+        shp = list(chain["shape"]) + [1] * (3 - len(chain["shape"]))
+        d = ChainDesc()
+        d.shape[:] = shp
+        d.temperature = chain["temperature"]
+        d.exchange_potential = chain.get("exchange_potential", 0.0)
+        d.seed = chain.get("seed", 0)
+        _check(lib().mcg_assign_chain(self.h, c, C.byref(d)))
+        self.shapes[c] = tuple(shp)
+        self.chains[c] = dict(chain)
 
     def set_active(self, mask):
         m = np.ascontiguousarray(mask, dtype=np.uint8)

@@ -163,6 +163,7 @@ struct mcg_handle {
   int64_t series_cap = 0;
   std::vector<int64_t> n_samples;  // host mirror of the series length
   std::vector<int64_t> n_pass;     // host mirror of the pass counters
+  std::vector<int64_t> capacity;   // sites each chain slot was allocated for
   double *d_stats = nullptr;
   int64_t stats_cap = 0;
   int64_t *d_eq = nullptr;
@@ -226,12 +227,11 @@ int ensure_series(mcg_handle *h, int64_t need) {
   return MCG_OK;
 }
 
-int build_exact(mcg_handle *h) {
-  h->xp.resize(h->n_chains);
-  int max_lds = 0;
-  for (int c = 0; c < h->n_chains; ++c) {
+// Parameters of one EXACT chain: geometry, speculative window and the
+// per-class acceptance tables (returns the LDS bytes the chain needs).
+int fill_exact_params(mcg_handle *h, int c, ExactParams &P) {
+  {
     auto const &ch = h->chains[c];
-    ExactParams &P = h->xp[c];
     std::memset(&P, 0, sizeof(P));
     P.rows = (int32_t)ch.shape[0];
     P.cols = (int32_t)ch.shape[1];
@@ -270,9 +270,14 @@ int build_exact(mcg_handle *h) {
           P.full_mask |= 1u << k;
       }
     }
-    const int lds = (int)(((P.n + 15) & ~15) + 4 * (P.rows + P.cols));
-    max_lds = std::max(max_lds, lds);
+    return (int)(((P.n + 15) & ~15) + 4 * (P.rows + P.cols));
   }
+}
+
+int build_exact(mcg_handle *h) {
+  h->xp.resize(h->n_chains);
+  int max_lds = 0;
+  for (int c = 0; c < h->n_chains; ++c) max_lds = std::max(max_lds, fill_exact_params(h, c, h->xp[c]));
   h->exact_in_lds = max_lds <= EXACT_LDS_MAX;
   if (h->exact_in_lds) {
     h->exact_lds = max_lds;
@@ -300,22 +305,26 @@ int build_exact(mcg_handle *h) {
   return MCG_OK;
 }
 
-int build_checker(mcg_handle *h) {
+int checker_ntab(mcg_handle *h) {
   const bool nnn = h->J2 != 0.0;
   const int ndim = h->lattice == MCG_LATTICE_SC ? 3 : 2;
-  h->ncolor = nnn ? (ndim == 3 ? 8 : 4) : 2;
+  const int z1 = ndim == 3 ? 6 : 4;
+  const int z2 = nnn ? (ndim == 3 ? 12 : 4) : 0;
+  return 2 * (z1 + 1) * (z2 + 1);
+}
+
+// Parameters and acceptance table (ntab entries at tab) of one CHECKERBOARD chain.
+int fill_checker_params(mcg_handle *h, int c, CheckerParams &P, uint32_t *tab) {
+  const bool nnn = h->J2 != 0.0;
+  const int ndim = h->lattice == MCG_LATTICE_SC ? 3 : 2;
   const int z1 = ndim == 3 ? 6 : 4;
   const int z2 = nnn ? (ndim == 3 ? 12 : 4) : 0;
   const int ntab = 2 * (z1 + 1) * (z2 + 1);
-  h->cp.resize(h->n_chains);
-  std::vector<uint32_t> tabs((size_t)ntab * h->n_chains);
-  int64_t max_n = 0;
-  for (int c = 0; c < h->n_chains; ++c) {
+  {
     auto const &ch = h->chains[c];
     for (int d = 0; d < ndim; ++d)
       if (ch.shape[d] < 2 || (ch.shape[d] & 1))
         return fail(MCG_ERR_ARG, "CHECKERBOARD needs even edge lengths >= 2");
-    CheckerParams &P = h->cp[c];
     std::memset(&P, 0, sizeof(P));
     P.L0 = (int32_t)ch.shape[0];
     P.L1 = (int32_t)ch.shape[1];
@@ -330,7 +339,6 @@ int build_checker(mcg_handle *h) {
     P.lat_off = h->lat_off[c];
     P.tab_off = (int64_t)ntab * c;
     P.quad_only = (h->options & MCG_OPT_CHECKER_QUAD_PATH) ? 1 : 0;
-    max_n = std::max(max_n, P.n);
     const double beta = 1.0 / (h->kb * ch.temperature);
     const double mu = ch.exchange_potential;
     for (int sidx = 0; sidx < 2; ++sidx) {
@@ -350,9 +358,25 @@ int build_checker(mcg_handle *h) {
             if (v > 4294967295.0) v = 4294967295.0;
             t = (uint32_t)v;
           }
-          tabs[(size_t)P.tab_off + (size_t)sidx * (z1 + 1) * (z2 + 1) + a * (z2 + 1) + b] = t;
+          tab[(size_t)sidx * (z1 + 1) * (z2 + 1) + a * (z2 + 1) + b] = t;
         }
     }
+  }
+  return MCG_OK;
+}
+
+int build_checker(mcg_handle *h) {
+  const bool nnn = h->J2 != 0.0;
+  const int ndim = h->lattice == MCG_LATTICE_SC ? 3 : 2;
+  h->ncolor = nnn ? (ndim == 3 ? 8 : 4) : 2;
+  const int ntab = checker_ntab(h);
+  h->cp.resize(h->n_chains);
+  std::vector<uint32_t> tabs((size_t)ntab * h->n_chains);
+  int64_t max_n = 0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    int rc = fill_checker_params(h, c, h->cp[c], tabs.data() + (size_t)ntab * c);
+    if (rc != MCG_OK) return rc;
+    max_n = std::max(max_n, h->cp[c].n);
   }
   h->checker_lds = max_n <= CHECKER_LDS_MAX && !(h->options & MCG_OPT_FORCE_STREAMING);
   h->checker_lds_bytes = (int)((max_n + 15) & ~15);
@@ -520,6 +544,7 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
     return bail(fail(MCG_ERR_HIP, "active mask allocation failed"));
   h->n_samples.assign(h->n_chains, 0);
   h->n_pass.assign(h->n_chains, 0);
+  for (int c = 0; c < h->n_chains; ++c) h->capacity.push_back(h->sites(c));
   if (hipMalloc(&h->d_eq, 2 * sizeof(int64_t)) != hipSuccess)
     return bail(fail(MCG_ERR_HIP, "allocation failed"));
   rc = (h->mode == MCG_MODE_EXACT) ? build_exact(h) : build_checker(h);
@@ -615,6 +640,60 @@ int mcg_rng_load(mcg_handle *h, int64_t chain, const uint64_t *words, int64_t n_
     h->n_pass[chain] = st.n_pass;
   }
   return MCG_OK;
+}
+
+int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
+  if (!valid_chain(h, chain) || !desc) return fail(MCG_ERR_ARG, "bad chain or descriptor");
+  mcg_chain_desc nd = *desc;
+  if (h->lattice == MCG_LATTICE_SQUARE) nd.shape[2] = 1;
+  for (int d = 0; d < 3; ++d)
+    if (nd.shape[d] < 1 || nd.shape[d] > (1 << 20)) return fail(MCG_ERR_ARG, "shape out of range");
+  if (!(nd.temperature > 0.0)) return fail(MCG_ERR_ARG, "temperature must be > 0");
+  const int64_t n = nd.shape[0] * nd.shape[1] * nd.shape[2];
+  if (n > h->capacity[chain])
+    return fail(MCG_ERR_CAPACITY, "shape has more sites than the chain slot holds");
+  if (h->mode == MCG_MODE_EXACT && nd.shape[0] >= (1LL << 16))
+    return fail(MCG_ERR_ARG, "EXACT mode supports rows < 65536");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const mcg_chain_desc old = h->chains[chain];
+  h->chains[chain] = nd;
+  if (h->mode == MCG_MODE_EXACT) {
+    ExactParams P;
+    const int lds = fill_exact_params(h, (int)chain, P);
+    const int need = h->exact_in_lds ? lds : 4 * (P.rows + P.cols);
+    if (need > h->exact_lds) {
+      h->chains[chain] = old;
+      return fail(MCG_ERR_CAPACITY, "shape needs more LDS than the handle was created with");
+    }
+    h->xp[chain] = P;
+    HIP_TRY(hipMemcpy(h->d_xp + chain, &P, sizeof(P), hipMemcpyHostToDevice));
+    ExactState S;
+    std::memset(&S, 0, sizeof(S));
+    int pos = 0;
+    mt_seed(nd.seed, S.mt, &pos);
+    S.pos = pos;
+    HIP_TRY(hipMemcpy(h->d_xs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
+  } else {
+    CheckerParams P;
+    std::vector<uint32_t> tab(checker_ntab(h));
+    const int rc = fill_checker_params(h, (int)chain, P, tab.data());
+    if (rc != MCG_OK) {
+      h->chains[chain] = old;
+      return rc;
+    }
+    h->cp[chain] = P;
+    HIP_TRY(hipMemcpy(h->d_cp + chain, &P, sizeof(P), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->d_tab + P.tab_off, tab.data(), sizeof(uint32_t) * tab.size(),
+                      hipMemcpyHostToDevice));
+    CheckerState S;
+    std::memset(&S, 0, sizeof(S));
+    HIP_TRY(hipMemcpy(h->d_cs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
+    h->work_dirty = true;
+  }
+  HIP_TRY(hipMemset(h->d_lat + h->lat_off[chain], 1, (size_t)((h->capacity[chain] + 255) & ~255LL)));
+  h->n_samples[chain] = 0;
+  h->n_pass[chain] = 0;
+  return reset_observables(h, chain, std::vector<int8_t>(n, 1));
 }
 
 int mcg_set_active(mcg_handle *h, const uint8_t *active, int64_t n_chains) {

@@ -154,6 +154,14 @@ int mcg_run(mcg_handle *h, int64_t n_passes);
  * chains while others keep running. */
 int mcg_set_active(mcg_handle *h, const uint8_t *active, int64_t n_chains);
 
+/* Re-initialise one chain slot for a new run: a new shape (at most as many
+ * sites as the slot was created with, same lattice type), conditions and seed;
+ * the lattice becomes all +1 and the counters, sample series and RNG restart.
+ * Replaces building a new state/calculator for the next run of a condition
+ * scan (basic_semigrand_canonical.hh:427-462); used by the continuous-batching
+ * server to admit a request into the slot a finished request freed. */
+int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc);
+
 /* Per-chain counters: pass count, accepted and rejected attempts
  * (BasicOccupationMetropolisData n_pass/n_accept/n_reject,
  * basic_occupation_metropolis.hh:71-82). */
