@@ -34,7 +34,8 @@ EXPORTED = [
     "mcg_create", "mcg_destroy", "mcg_last_error", "mcg_set_occupation", "mcg_get_occupation",
     "mcg_rng_dump", "mcg_rng_load", "mcg_run", "mcg_set_active", "mcg_get_counters",
     "mcg_n_samples", "mcg_get_samples", "mcg_clear_samples", "mcg_stats_compute",
-    "mcg_equilibration", "mcg_last_run_time", "mcg_assign_chain",
+    "mcg_equilibration", "mcg_last_run_time", "mcg_assign_chain", "mcg_autotune",
+    "mcg_get_schedule", "mcg_set_schedule",
 ]
 
 
@@ -52,6 +53,11 @@ class Desc(C.Structure):
 
 SAMPLE_DTYPE = np.dtype([("pass", np.int64), ("S1", np.int64), ("S2", np.int64),
                          ("M", np.int64), ("e_fold", np.float64), ("e_run", np.float64)])
+
+
+class Schedule(C.Structure):
+    _fields_ = [("lds_block", C.c_int64), ("units_per_thread", C.c_int64),
+                ("graph_passes", C.c_int64), ("trial_ms", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -91,6 +97,9 @@ def lib():
                                         P(C.c_int64), P(C.c_int64)]
         L.mcg_last_run_time.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
         L.mcg_assign_chain.argtypes = [C.c_void_p, C.c_int64, P(ChainDesc)]
+        L.mcg_autotune.argtypes = [C.c_void_p, C.c_int64, P(Schedule)]
+        L.mcg_get_schedule.argtypes = [C.c_void_p, P(Schedule)]
+        L.mcg_set_schedule.argtypes = [C.c_void_p, P(Schedule)]
         _lib = L
     return _lib
 
@@ -221,6 +230,26 @@ class Run:
         a, b = C.c_int64(), C.c_int64()
         _check(lib().mcg_equilibration(self.h, c, observable, prec, C.byref(a), C.byref(b)))
         return bool(a.value), b.value
+
+    def autotune(self, trial_passes=64):
+        """Time the candidate launch schedules on this run's chains (state is
+        restored afterwards) and keep the fastest; returns it as a dict."""
+        s = Schedule()
+        _check(lib().mcg_autotune(self.h, trial_passes, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in Schedule._fields_}
+
+    def schedule(self):
+        s = Schedule()
+        _check(lib().mcg_get_schedule(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in Schedule._fields_}
+
+    def set_schedule(self, lds_block=None, units_per_thread=None, graph_passes=None):
+        cur = self.schedule()
+        s = Schedule(lds_block if lds_block is not None else cur["lds_block"],
+                     units_per_thread if units_per_thread is not None
+                     else cur["units_per_thread"],
+                     graph_passes if graph_passes is not None else cur["graph_passes"], 0.0)
+        _check(lib().mcg_set_schedule(self.h, C.byref(s)))
 
     def last_run_time(self):
         ms, att = C.c_double(), C.c_double()

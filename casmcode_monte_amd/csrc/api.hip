@@ -7,7 +7,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
+#include <cstdlib>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -187,9 +191,17 @@ struct mcg_handle {
   int checker_lds_bytes = 0;
   int checker_block = 256;
   int ncolor = 2;
+  int units_per_thread = 2;
+  int graph_passes = 8;
+  double tuned_ms = 0.0;
+  // hipGraph of graph_passes passes of the HBM path (recaptured when the work
+  // list, the series buffer or the schedule change)
+  hipGraphExec_t graph_exec = nullptr;
+  bool graph_dirty = true;
 
   ~mcg_handle() {
     if (stream) hipStreamSynchronize(stream);
+    if (graph_exec) hipGraphExecDestroy(graph_exec);
     void *ptrs[] = {d_lat, d_active, d_series, d_stats, d_eq, d_xp, d_xs, d_cp, d_cs, d_tab,
                     d_work};
     for (void *p : ptrs)
@@ -223,6 +235,7 @@ int ensure_series(mcg_handle *h, int64_t need) {
     hipFree(h->d_series);
   }
   h->d_series = d;
+  h->graph_dirty = true;  // captured launches hold the old buffer and capacity
   h->series_cap = cap;
   return MCG_OK;
 }
@@ -451,6 +464,99 @@ int rebuild_work(mcg_handle *h) {
   }
   h->work_dirty = false;
   return MCG_OK;
+}
+
+// One pass of the HBM path: one launch per colour over the work list, then
+// the pass-counter/sample-record launch.
+void enqueue_pass(mcg_handle *h) {
+  if (h->n_work > 0)
+    for (int color = 0; color < h->ncolor; ++color)
+      launch_checker_color(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_work, h->n_work, color,
+                           h->units_per_thread, h->stream);
+  launch_checker_record(h->d_cp, h->d_cs, h->d_series, h->series_cap, h->d_active, h->n_chains,
+                        h->sample_period, h->stream);
+}
+
+// Capture graph_passes passes into a hipGraph (no memset nodes: every launch
+// reads its pass counter from device state, so replays are independent).
+int capture_graph(mcg_handle *h) {
+  if (h->graph_exec) {
+    HIP_TRY(hipGraphExecDestroy(h->graph_exec));
+    h->graph_exec = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  for (int p = 0; p < h->graph_passes; ++p) enqueue_pass(h);
+  HIP_TRY(hipStreamEndCapture(h->stream, &g));
+  HIP_TRY(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
+  HIP_TRY(hipGraphDestroy(g));
+  h->graph_dirty = false;
+  return MCG_OK;
+}
+
+// Autotuner cache key: path (LDS-resident or HBM), lattice, colouring and
+// the size class (log2 of the sites per chain and of the chain count).
+std::string tune_key(const mcg_handle *h) {
+  int64_t sites = 0;
+  for (int c = 0; c < h->n_chains; ++c) sites = std::max(sites, h->sites(c));
+  auto lg = [](int64_t v) {
+    int k = 0;
+    while ((int64_t(1) << k) < v) ++k;
+    return k;
+  };
+  return std::string(h->checker_lds ? "lds" : "hbm") + ":" + std::to_string(h->lattice) + ":" +
+         std::to_string(h->ncolor) + ":" + std::to_string(lg(sites)) + ":" +
+         std::to_string(lg(h->n_chains));
+}
+
+std::mutex &tune_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+std::map<std::string, mcg_schedule> &tune_cache() {
+  static std::map<std::string, mcg_schedule> m;
+  return m;
+}
+
+// File format, one schedule per line: key lds_block units_per_thread graph_passes trial_ms
+bool tune_cache_lookup(const std::string &key, mcg_schedule *out) {
+  std::lock_guard<std::mutex> lock(tune_mutex());
+  auto &m = tune_cache();
+  auto it = m.find(key);
+  if (it != m.end()) {
+    *out = it->second;
+    return true;
+  }
+  const char *path = std::getenv("MCG_TUNE_CACHE");
+  if (!path || !*path) return false;
+  FILE *f = std::fopen(path, "r");
+  if (!f) return false;
+  char k[128];
+  long long b, u, g;
+  double ms;
+  bool found = false;
+  while (std::fscanf(f, "%127s %lld %lld %lld %lf", k, &b, &u, &g, &ms) == 5) {
+    if (key == k) {  // the last line for a key wins
+      *out = mcg_schedule{b, u, g, ms};
+      found = true;
+    }
+  }
+  std::fclose(f);
+  if (found) m[key] = *out;
+  return found;
+}
+
+void tune_cache_store(const std::string &key, const mcg_schedule &s) {
+  std::lock_guard<std::mutex> lock(tune_mutex());
+  tune_cache()[key] = s;
+  const char *path = std::getenv("MCG_TUNE_CACHE");
+  if (!path || !*path) return;
+  if (FILE *f = std::fopen(path, "a")) {
+    std::fprintf(f, "%s %lld %lld %lld %.6f\n", key.c_str(), (long long)s.lds_block,
+                 (long long)s.units_per_thread, (long long)s.graph_passes, s.trial_ms);
+    std::fclose(f);
+  }
 }
 
 bool valid_chain(mcg_handle *h, int64_t c) { return h && c >= 0 && c < h->n_chains; }
@@ -724,6 +830,19 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
   double attempts = 0.0;
   for (int c = 0; c < h->n_chains; ++c)
     if (h->active[c]) attempts += (double)h->sites(c) * (double)n_passes;
+  const bool hbm_path = h->mode != MCG_MODE_EXACT && !h->checker_lds;
+  const int G = h->graph_passes;
+  if (hbm_path) {
+    if (h->work_dirty) {
+      rc = rebuild_work(h);
+      if (rc != MCG_OK) return rc;
+      h->graph_dirty = true;
+    }
+    if (G > 1 && n_passes >= G && h->graph_dirty) {
+      rc = capture_graph(h);
+      if (rc != MCG_OK) return rc;
+    }
+  }
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
   if (h->mode == MCG_MODE_EXACT) {
     launch_exact(h->d_xp, h->d_xs, h->d_lat, h->d_series, h->series_cap, h->d_active,
@@ -735,17 +854,10 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
                        h->d_active, h->n_chains, n_passes, h->sample_period,
                        h->checker_lds_bytes, h->checker_block, h->stream);
   } else {
-    if (h->work_dirty) {
-      rc = rebuild_work(h);
-      if (rc != MCG_OK) return rc;
-    }
-    for (int64_t p = 0; p < n_passes; ++p) {
-      for (int color = 0; color < h->ncolor; ++color)
-        launch_checker_color(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_work, h->n_work, color,
-                             0, h->stream);
-      launch_checker_record(h->d_cp, h->d_cs, h->d_series, h->series_cap, h->d_active,
-                            h->n_chains, h->sample_period, h->stream);
-    }
+    int64_t done = 0;
+    if (G > 1 && n_passes >= G)
+      for (; done + G <= n_passes; done += G) HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
+    for (; done < n_passes; ++done) enqueue_pass(h);
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
@@ -885,6 +997,89 @@ int mcg_equilibration(mcg_handle *h, int64_t chain, int64_t observable, double p
   HIP_TRY(hipStreamSynchronize(h->stream));
   *is_equilibrated = r[0];
   *n_samples_for_equilibration = r[1];
+  return MCG_OK;
+}
+
+int mcg_get_schedule(mcg_handle *h, mcg_schedule *out) {
+  if (!h || !out) return fail(MCG_ERR_ARG, "null argument");
+  out->lds_block = h->checker_block;
+  out->units_per_thread = h->units_per_thread;
+  out->graph_passes = h->graph_passes;
+  out->trial_ms = h->tuned_ms;
+  return MCG_OK;
+}
+
+int mcg_set_schedule(mcg_handle *h, const mcg_schedule *s) {
+  if (!h || !s) return fail(MCG_ERR_ARG, "null argument");
+  if (s->lds_block != 256 && s->lds_block != 512 && s->lds_block != 1024)
+    return fail(MCG_ERR_ARG, "lds_block must be 256, 512 or 1024");
+  if (s->units_per_thread != 1 && s->units_per_thread != 2 && s->units_per_thread != 4)
+    return fail(MCG_ERR_ARG, "units_per_thread must be 1, 2 or 4");
+  if (s->graph_passes < 1 || s->graph_passes > 256)
+    return fail(MCG_ERR_ARG, "graph_passes must be in [1, 256]");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->checker_block = (int)s->lds_block;
+  h->units_per_thread = (int)s->units_per_thread;
+  h->graph_passes = (int)s->graph_passes;
+  h->graph_dirty = true;
+  return MCG_OK;
+}
+
+int mcg_autotune(mcg_handle *h, int64_t trial_passes, mcg_schedule *chosen) {
+  if (!h) return fail(MCG_ERR_ARG, "null handle");
+  if (h->mode == MCG_MODE_EXACT) return chosen ? mcg_get_schedule(h, chosen) : MCG_OK;
+  const std::string key = tune_key(h);
+  mcg_schedule best{};
+  if (!tune_cache_lookup(key, &best)) {
+    trial_passes = std::max<int64_t>(trial_passes, 16);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    // snapshot of everything a trial run changes
+    int8_t *lat_snap = nullptr;
+    CheckerState *cs_snap = nullptr;
+    HIP_TRY(hipMalloc(&lat_snap, (size_t)h->lat_bytes));
+    HIP_TRY(hipMalloc(&cs_snap, sizeof(CheckerState) * h->n_chains));
+    HIP_TRY(hipMemcpy(lat_snap, h->d_lat, (size_t)h->lat_bytes, hipMemcpyDeviceToDevice));
+    HIP_TRY(hipMemcpy(cs_snap, h->d_cs, sizeof(CheckerState) * h->n_chains,
+                      hipMemcpyDeviceToDevice));
+    const auto ns_snap = h->n_samples, np_snap = h->n_pass;
+    auto restore = [&]() -> int {
+      HIP_TRY(hipMemcpy(h->d_lat, lat_snap, (size_t)h->lat_bytes, hipMemcpyDeviceToDevice));
+      HIP_TRY(hipMemcpy(h->d_cs, cs_snap, sizeof(CheckerState) * h->n_chains,
+                        hipMemcpyDeviceToDevice));
+      h->n_samples = ns_snap;
+      h->n_pass = np_snap;
+      return MCG_OK;
+    };
+    std::vector<mcg_schedule> cands;
+    if (h->checker_lds) {
+      for (int64_t b : {256, 512, 1024}) cands.push_back({b, h->units_per_thread, 1, 0.0});
+    } else {
+      for (int64_t u : {1, 2, 4})
+        for (int64_t g : {1, 8}) cands.push_back({h->checker_block, u, g, 0.0});
+    }
+    double best_ms = 1e300;
+    int rc = MCG_OK;
+    for (auto &c : cands) {
+      if ((rc = mcg_set_schedule(h, &c)) != MCG_OK) break;
+      if ((rc = mcg_run(h, std::min<int64_t>(trial_passes, 8))) != MCG_OK) break;  // warm-up
+      if ((rc = restore()) != MCG_OK) break;
+      if ((rc = mcg_run(h, trial_passes)) != MCG_OK) break;
+      c.trial_ms = h->last_ms;
+      if ((rc = restore()) != MCG_OK) break;
+      if (c.trial_ms < best_ms) {
+        best_ms = c.trial_ms;
+        best = c;
+      }
+    }
+    (void)hipFree(lat_snap);
+    (void)hipFree(cs_snap);
+    if (rc != MCG_OK) return rc;
+    tune_cache_store(key, best);
+  }
+  int rc = mcg_set_schedule(h, &best);
+  if (rc != MCG_OK) return rc;
+  h->tuned_ms = best.trial_ms;
+  if (chosen) *chosen = best;
   return MCG_OK;
 }
 

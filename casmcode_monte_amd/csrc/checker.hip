@@ -393,16 +393,23 @@ void checker_set_max_lds(int bytes) {
 // entries; an item is UNITS_PER_ITEM work units of one chain.
 // ---------------------------------------------------------------------------
 constexpr int HBM_BLOCK = 256;
-constexpr int UNITS_PER_THREAD = 2;
-constexpr int UNITS_PER_ITEM = HBM_BLOCK * UNITS_PER_THREAD;
+constexpr int MAX_UNITS_PER_THREAD = 4;
+constexpr int UNITS_PER_ITEM = HBM_BLOCK * MAX_UNITS_PER_THREAD;
 
+// A work item is UNITS_PER_ITEM units of one chain; a block covers
+// HBM_BLOCK * UPT units, so an item takes MAX_UNITS_PER_THREAD / UPT blocks.
+template <int UPT>
 __global__ __launch_bounds__(HBM_BLOCK) void checker_color_kernel(
     const CheckerParams *__restrict__ Pp, CheckerState *__restrict__ Sp,
     int8_t *__restrict__ lat_pool, const uint32_t *__restrict__ tables,
     const int32_t *__restrict__ work, int color) {
+  constexpr int BLOCKS_PER_ITEM = MAX_UNITS_PER_THREAD / UPT;
   __shared__ int red[4][HBM_BLOCK / 64];
-  const int c = work[2 * blockIdx.x];
-  const int64_t u0 = (int64_t)work[2 * blockIdx.x + 1] * UNITS_PER_ITEM;
+  const int item = blockIdx.x / BLOCKS_PER_ITEM;
+  const int sub = blockIdx.x - item * BLOCKS_PER_ITEM;
+  const int c = work[2 * item];
+  const int64_t u0 =
+      (int64_t)work[2 * item + 1] * UNITS_PER_ITEM + (int64_t)sub * HBM_BLOCK * UPT;
   const CheckerParams P = Pp[c];
   CheckerState *St = Sp + c;
   const LineGeom g = line_geom(P);
@@ -413,7 +420,7 @@ __global__ __launch_bounds__(HBM_BLOCK) void checker_color_kernel(
   int dS1 = 0, dS2 = 0, dM = 0, nacc = 0;
   const int t = threadIdx.x;
 #pragma unroll
-  for (int u = 0; u < UNITS_PER_THREAD; ++u) {
+  for (int u = 0; u < UPT; ++u) {
     const int64_t unit = u0 + (int64_t)u * HBM_BLOCK + t;
     if (unit < nu) update_unit(P, g, lat, tab, color, pass, unit, dS1, dS2, dM, nacc);
   }
@@ -475,10 +482,16 @@ __global__ void checker_record_kernel(const CheckerParams *__restrict__ Pp,
 
 void launch_checker_color(const CheckerParams *P, CheckerState *St, int8_t *lat_pool,
                           const uint32_t *tables, const int32_t *work, int n_work_items,
-                          int color, int64_t pass, hipStream_t stream) {
-  (void)pass;
-  hipLaunchKernelGGL(checker_color_kernel, dim3(n_work_items), dim3(HBM_BLOCK), 0, stream, P,
-                     St, lat_pool, tables, work, color);
+                          int color, int upt, hipStream_t stream) {
+  if (upt == 1)
+    hipLaunchKernelGGL(checker_color_kernel<1>, dim3(n_work_items * 4), dim3(HBM_BLOCK), 0,
+                       stream, P, St, lat_pool, tables, work, color);
+  else if (upt == 2)
+    hipLaunchKernelGGL(checker_color_kernel<2>, dim3(n_work_items * 2), dim3(HBM_BLOCK), 0,
+                       stream, P, St, lat_pool, tables, work, color);
+  else
+    hipLaunchKernelGGL(checker_color_kernel<4>, dim3(n_work_items), dim3(HBM_BLOCK), 0, stream,
+                       P, St, lat_pool, tables, work, color);
 }
 
 void launch_checker_record(const CheckerParams *P, CheckerState *St, mcg_sample *series,

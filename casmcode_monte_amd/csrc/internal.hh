@@ -81,7 +81,7 @@ void launch_checker_lds(const CheckerParams *P, CheckerState *St, int8_t *lat_po
 
 void launch_checker_color(const CheckerParams *P, CheckerState *St, int8_t *lat_pool,
                           const uint32_t *tables, const int32_t *work, int n_work_items,
-                          int color, int64_t pass, hipStream_t stream);
+                          int color, int upt, hipStream_t stream);
 
 void launch_checker_record(const CheckerParams *P, CheckerState *St, mcg_sample *series,
                            int64_t series_cap, const uint8_t *active, int n_chains,

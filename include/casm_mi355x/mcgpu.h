@@ -209,6 +209,26 @@ int mcg_stats_compute(mcg_handle *h, int64_t chain, int64_t observable, int64_t 
 int mcg_equilibration(mcg_handle *h, int64_t chain, int64_t observable, double prec,
                       int64_t *is_equilibrated, int64_t *n_samples_for_equilibration);
 
+/* Launch schedule of a CHECKERBOARD handle (EXACT handles have nothing to
+ * tune: one wavefront per chain).  Every schedule produces the same
+ * trajectories; only the speed changes. */
+typedef struct mcg_schedule {
+  int64_t lds_block;        /* LDS-resident kernel: threads per workgroup (256, 512, 1024) */
+  int64_t units_per_thread; /* HBM kernel: work units per thread (1, 2, 4)                */
+  int64_t graph_passes;     /* HBM path: passes captured per hipGraph replay (1 = none)   */
+  double trial_ms;          /* mcg_autotune: time of the chosen schedule on its trial     */
+} mcg_schedule;
+
+/* Autotuner: times the candidate schedules on `trial_passes` passes of this
+ * handle's own chains (from a device snapshot that is restored afterwards, so
+ * the run's state and trajectory are unchanged), keeps the fastest and caches
+ * it per (lattice, colouring, size class) for the process and, when the
+ * environment variable MCG_TUNE_CACHE names a file, across processes.
+ * No counterpart in the reference (its loop has no launch parameters). */
+int mcg_autotune(mcg_handle *h, int64_t trial_passes, mcg_schedule *chosen);
+int mcg_get_schedule(mcg_handle *h, mcg_schedule *out);
+int mcg_set_schedule(mcg_handle *h, const mcg_schedule *s);
+
 /* Time of the last mcg_run's kernels on the handle's stream, in ms
  * (HIP events around the launches), and the number of site attempts done. */
 int mcg_last_run_time(mcg_handle *h, double *ms, double *attempts);

@@ -199,3 +199,56 @@ def test_heat_capacity_and_susceptibility_vs_enumeration():
     for vals, exact in ((np.array(Cs), C_exact), (np.array(chis), chi_exact)):
         se = vals.std(ddof=1) / math.sqrt(n_chains)
         assert abs(vals.mean() - exact) < 4 * se + 0.01 * abs(exact), (vals.mean(), exact, se)
+
+
+SCHED_CHAINS = [dict(shape=(64, 48), temperature=2400.0, exchange_potential=0.05, seed=31),
+                dict(shape=(40, 36), temperature=1800.0, exchange_potential=-0.1, seed=32)]
+
+
+@pytest.mark.parametrize("streaming,schedules", [
+    (0, [dict(lds_block=256), dict(lds_block=512), dict(lds_block=1024)]),
+    (cm.OPT_FORCE_STREAMING, [dict(units_per_thread=u, graph_passes=g)
+                              for u in (1, 2, 4) for g in (1, 8, 5)]),
+])
+def test_schedules_give_identical_trajectories(streaming, schedules):
+    """Launch schedules (LDS block size, units per thread, graph replay of
+    several passes) change nothing but speed; 37 passes leave a remainder
+    after the graph replays."""
+    print("case: schedules streaming", streaming, flush=True)
+    ref = None
+    for s in schedules:
+        r = cm.Run(SCHED_CHAINS, mode=cm.MODE_CHECKERBOARD, lattice=cm.LATTICE_SQUARE,
+                   J1=0.1, J2=-0.02, options=streaming)
+        r.set_schedule(**s)
+        r.run(37)
+        r.run(11)
+        got = [(r.get_occupation(c), r.samples(c), r.counters(c)) for c in range(2)]
+        if ref is None:
+            ref = got
+            continue
+        for (oa, sa, ca), (ob, sb, cb) in zip(ref, got):
+            np.testing.assert_array_equal(oa, ob)
+            np.testing.assert_array_equal(sa, sb)
+            assert ca == cb
+
+
+@pytest.mark.parametrize("streaming", [0, cm.OPT_FORCE_STREAMING])
+def test_autotune_leaves_the_run_unchanged(streaming):
+    print("case: autotune streaming", streaming, flush=True)
+    a = cm.Run(SCHED_CHAINS, mode=cm.MODE_CHECKERBOARD, lattice=cm.LATTICE_SQUARE,
+               J1=0.1, J2=-0.02, options=streaming)
+    b = cm.Run(SCHED_CHAINS, mode=cm.MODE_CHECKERBOARD, lattice=cm.LATTICE_SQUARE,
+               J1=0.1, J2=-0.02, options=streaming)
+    a.run(10)
+    b.run(10)
+    chosen = a.autotune(24)
+    assert chosen["trial_ms"] > 0.0
+    assert a.schedule()["units_per_thread"] == chosen["units_per_thread"]
+    a.run(30)
+    b.run(30)
+    for c in range(2):
+        np.testing.assert_array_equal(a.get_occupation(c), b.get_occupation(c))
+        np.testing.assert_array_equal(a.samples(c), b.samples(c))
+        assert a.counters(c) == b.counters(c)
+    # a second handle of the same size class hits the cache
+    assert b.autotune(24) == chosen
