@@ -99,9 +99,11 @@ def bench_checker_family(shape, lattice, passes, chains=16):
     r = cm.Run(conds, mode=cm.MODE_CHECKERBOARD, lattice=lattice, J1=0.1, J2=-0.02,
                sample_period=1)
     r.run(2)
+    sched = r.autotune(16)
     w, k, a = timed_run(r, passes)
     r.close()
     return dict(shape=list(shape), chains=chains, passes=passes, wall_s=w, kernel_s=k,
+                schedule={k2: sched[k2] for k2 in ("units_per_thread", "graph_passes")},
                 attempts=a, attempts_per_s=a / w,
                 hbm_gbs_algorithmic=2.0 * a / k / 1e9,
                 hbm_frac=2.0 * a / k / 1e9 / HBM_PEAK_GBS)
@@ -151,7 +153,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--skip-extra", action="store_true", help="only the default workload")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--served-requests", type=int, default=400)
+    ap.add_argument("--served-requests", type=int, default=400,
+                    help="requests of the served scan (0 skips it)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes 10 times "
                          "(the reference point that scales the FETCH_SIZE/WRITE_SIZE counters)")
@@ -222,13 +225,15 @@ def main():
             "square_4096x4096_J1J2_1chain": fam2_plain,
             "sc_256cubed_J1J2_16T": fam3,
         }
-        served = bench_served(args.served_requests)
-        print(f"[bench] served done: {served['attempts_per_s']:.3e} attempts/s", file=sys.stderr,
-              flush=True)
-        line["served"] = served
-        line["latency_s"] = {"p50": served["p50_latency_s"], "p99": served["p99_latency_s"],
-                             "note": "served scan run latency (arrival to result)",
-                             "exact_batch_wall_s": ex["wall_s"]}
+        if args.served_requests > 0:
+            served = bench_served(args.served_requests)
+            print(f"[bench] served done: {served['attempts_per_s']:.3e} attempts/s",
+                  file=sys.stderr, flush=True)
+            line["served"] = served
+            line["latency_s"] = {"p50": served["p50_latency_s"],
+                                 "p99": served["p99_latency_s"],
+                                 "note": "served scan run latency (arrival to result)",
+                                 "exact_batch_wall_s": ex["wall_s"]}
         line["roofline_checkerboard"] = {
             "bound": "hbm", "achieved": fam2["hbm_gbs_algorithmic"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": fam2["hbm_frac"], "traffic": None,

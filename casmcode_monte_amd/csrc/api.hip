@@ -194,7 +194,7 @@ struct mcg_handle {
   int units_per_thread = 2;
   int graph_passes = 8;
   double tuned_ms = 0.0;
-  // hipGraph of graph_passes passes of the HBM path (recaptured when the work
+  // hipGraph of graph_passes passes of the HBM path (rebuilt when the work
   // list, the series buffer or the schedule change)
   hipGraphExec_t graph_exec = nullptr;
   bool graph_dirty = true;
@@ -218,6 +218,18 @@ struct mcg_handle {
 
 namespace {
 
+// Host<->device copies and fills go through the handle's own non-blocking
+// stream (the legacy stream would not be ordered with the handle's kernels).
+hipError_t copy_sync(mcg_handle *h, void *dst, const void *src, size_t n, hipMemcpyKind k) {
+  hipError_t e = hipMemcpyAsync(dst, src, n, k, h->stream);
+  return e != hipSuccess ? e : hipStreamSynchronize(h->stream);
+}
+
+hipError_t fill_sync(mcg_handle *h, void *dst, int v, size_t n) {
+  hipError_t e = hipMemsetAsync(dst, v, n, h->stream);
+  return e != hipSuccess ? e : hipStreamSynchronize(h->stream);
+}
+
 int ensure_series(mcg_handle *h, int64_t need) {
   if (need <= h->series_cap) return MCG_OK;
   int64_t cap = std::max<int64_t>(need, h->series_cap * 2);
@@ -235,7 +247,7 @@ int ensure_series(mcg_handle *h, int64_t need) {
     hipFree(h->d_series);
   }
   h->d_series = d;
-  h->graph_dirty = true;  // captured launches hold the old buffer and capacity
+  h->graph_dirty = true;  // graph nodes hold the old buffer and capacity
   h->series_cap = cap;
   return MCG_OK;
 }
@@ -303,7 +315,7 @@ int build_exact(mcg_handle *h) {
   exact_set_max_lds((int)LDS_LIMIT - 4096);
   HIP_TRY(hipMalloc(&h->d_xp, sizeof(ExactParams) * h->n_chains));
   HIP_TRY(hipMalloc(&h->d_xs, sizeof(ExactState) * h->n_chains));
-  HIP_TRY(hipMemcpy(h->d_xp, h->xp.data(), sizeof(ExactParams) * h->n_chains,
+  HIP_TRY(copy_sync(h, h->d_xp, h->xp.data(), sizeof(ExactParams) * h->n_chains,
                     hipMemcpyHostToDevice));
   std::vector<ExactState> xs(h->n_chains);
   for (int c = 0; c < h->n_chains; ++c) {
@@ -313,7 +325,7 @@ int build_exact(mcg_handle *h) {
     mt_seed(h->chains[c].seed, S.mt, &pos);
     S.pos = pos;
   }
-  HIP_TRY(hipMemcpy(h->d_xs, xs.data(), sizeof(ExactState) * h->n_chains,
+  HIP_TRY(copy_sync(h, h->d_xs, xs.data(), sizeof(ExactState) * h->n_chains,
                     hipMemcpyHostToDevice));
   return MCG_OK;
 }
@@ -397,11 +409,11 @@ int build_checker(mcg_handle *h) {
   HIP_TRY(hipMalloc(&h->d_cp, sizeof(CheckerParams) * h->n_chains));
   HIP_TRY(hipMalloc(&h->d_cs, sizeof(CheckerState) * h->n_chains));
   HIP_TRY(hipMalloc(&h->d_tab, sizeof(uint32_t) * tabs.size()));
-  HIP_TRY(hipMemcpy(h->d_cp, h->cp.data(), sizeof(CheckerParams) * h->n_chains,
+  HIP_TRY(copy_sync(h, h->d_cp, h->cp.data(), sizeof(CheckerParams) * h->n_chains,
                     hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(h->d_tab, tabs.data(), sizeof(uint32_t) * tabs.size(),
+  HIP_TRY(copy_sync(h, h->d_tab, tabs.data(), sizeof(uint32_t) * tabs.size(),
                     hipMemcpyHostToDevice));
-  HIP_TRY(hipMemset(h->d_cs, 0, sizeof(CheckerState) * h->n_chains));
+  HIP_TRY(fill_sync(h, h->d_cs, 0, sizeof(CheckerState) * h->n_chains));
   return MCG_OK;
 }
 
@@ -414,7 +426,7 @@ int reset_observables(mcg_handle *h, int64_t c, std::vector<int8_t> const &occ) 
   integer_observables(occ, g, h->J2 != 0.0, S1, S2, M);
   if (h->mode == MCG_MODE_EXACT) {
     ExactState st;
-    HIP_TRY(hipMemcpy(&st, h->d_xs + c, sizeof(ExactState), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(h, &st, h->d_xs + c, sizeof(ExactState), hipMemcpyDeviceToHost));
     st.S = S1;
     st.M = M;
     // occupation_metropolis.hh:103-106: potential per_supercell / n_unitcells
@@ -423,14 +435,14 @@ int reset_observables(mcg_handle *h, int64_t c, std::vector<int8_t> const &occ) 
     const double Nx = static_cast<double>(n + M) / 2.0;
     const double E = Ef - ch.exchange_potential * Nx;
     st.e_run = E / (double)n;
-    HIP_TRY(hipMemcpy(h->d_xs + c, &st, sizeof(ExactState), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(h, h->d_xs + c, &st, sizeof(ExactState), hipMemcpyHostToDevice));
   } else {
     CheckerState st;
-    HIP_TRY(hipMemcpy(&st, h->d_cs + c, sizeof(CheckerState), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(h, &st, h->d_cs + c, sizeof(CheckerState), hipMemcpyDeviceToHost));
     st.S1 = S1;
     st.S2 = S2;
     st.M = M;
-    HIP_TRY(hipMemcpy(h->d_cs + c, &st, sizeof(CheckerState), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(h, h->d_cs + c, &st, sizeof(CheckerState), hipMemcpyHostToDevice));
   }
   return MCG_OK;
 }
@@ -459,7 +471,7 @@ int rebuild_work(mcg_handle *h) {
   h->n_work = (int)(work.size() / 2);
   if (h->n_work) {
     HIP_TRY(hipMalloc(&h->d_work, sizeof(int32_t) * work.size()));
-    HIP_TRY(hipMemcpy(h->d_work, work.data(), sizeof(int32_t) * work.size(),
+    HIP_TRY(copy_sync(h, h->d_work, work.data(), sizeof(int32_t) * work.size(),
                       hipMemcpyHostToDevice));
   }
   h->work_dirty = false;
@@ -477,19 +489,26 @@ void enqueue_pass(mcg_handle *h) {
                         h->sample_period, h->stream);
 }
 
-// Capture graph_passes passes into a hipGraph (no memset nodes: every launch
-// reads its pass counter from device state, so replays are independent).
-int capture_graph(mcg_handle *h) {
+// Build a hipGraph of graph_passes passes of the HBM path (kernel nodes added
+// explicitly; every launch reads its pass counter from device state, so
+// replays are independent).
+int build_graph(mcg_handle *h) {
   if (h->graph_exec) {
     HIP_TRY(hipGraphExecDestroy(h->graph_exec));
     h->graph_exec = nullptr;
   }
   hipGraph_t g = nullptr;
-  HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-  for (int p = 0; p < h->graph_passes; ++p) enqueue_pass(h);
-  HIP_TRY(hipStreamEndCapture(h->stream, &g));
-  HIP_TRY(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
-  HIP_TRY(hipGraphDestroy(g));
+  HIP_TRY(hipGraphCreate(&g, 0));
+  hipError_t e = checker_graph_add_passes(
+      g, h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_work, h->n_work, h->ncolor,
+      h->units_per_thread, h->d_series, h->series_cap, h->d_active, h->n_chains,
+      h->sample_period, h->graph_passes);
+  if (e == hipSuccess) e = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    h->graph_exec = nullptr;
+    return fail(MCG_ERR_HIP, std::string("graph build: ") + hipGetErrorString(e));
+  }
   h->graph_dirty = false;
   return MCG_OK;
 }
@@ -642,11 +661,11 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
     return bail(fail(MCG_ERR_HIP, "hipEventCreate failed"));
   if (hipMalloc(&h->d_lat, (size_t)h->lat_bytes) != hipSuccess)
     return bail(fail(MCG_ERR_CAPACITY, "out of device memory for lattices"));
-  if (hipMemset(h->d_lat, 1, (size_t)h->lat_bytes) != hipSuccess)
+  if (fill_sync(h, h->d_lat, 1, (size_t)h->lat_bytes) != hipSuccess)
     return bail(fail(MCG_ERR_HIP, "hipMemset failed"));
   h->active.assign(h->n_chains, 1);
   if (hipMalloc(&h->d_active, h->n_chains) != hipSuccess ||
-      hipMemcpy(h->d_active, h->active.data(), h->n_chains, hipMemcpyHostToDevice) != hipSuccess)
+      copy_sync(h, h->d_active, h->active.data(), h->n_chains, hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(MCG_ERR_HIP, "active mask allocation failed"));
   h->n_samples.assign(h->n_chains, 0);
   h->n_pass.assign(h->n_chains, 0);
@@ -677,11 +696,11 @@ int mcg_set_occupation(mcg_handle *h, int64_t chain, const int8_t *occ, int64_t 
     if (s != 1 && s != -1) return fail(MCG_ERR_ARG, "occupation values must be +1 or -1");
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (h->mode == MCG_MODE_EXACT) {
-    HIP_TRY(hipMemcpy(h->d_lat + h->lat_off[chain], v.data(), n, hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(h, h->d_lat + h->lat_off[chain], v.data(), n, hipMemcpyHostToDevice));
   } else {
     std::vector<int8_t> dev(n);
     to_split_rows(v, h->chains[chain].shape[0], dev);
-    HIP_TRY(hipMemcpy(h->d_lat + h->lat_off[chain], dev.data(), n, hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(h, h->d_lat + h->lat_off[chain], dev.data(), n, hipMemcpyHostToDevice));
   }
   return reset_observables(h, chain, v);
 }
@@ -691,10 +710,10 @@ int mcg_get_occupation(mcg_handle *h, int64_t chain, int8_t *occ, int64_t n) {
   if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "occupation size mismatch");
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (h->mode == MCG_MODE_EXACT) {
-    HIP_TRY(hipMemcpy(occ, h->d_lat + h->lat_off[chain], n, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(h, occ, h->d_lat + h->lat_off[chain], n, hipMemcpyDeviceToHost));
   } else {
     std::vector<int8_t> dev(n), nat(n);
-    HIP_TRY(hipMemcpy(dev.data(), h->d_lat + h->lat_off[chain], n, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(h, dev.data(), h->d_lat + h->lat_off[chain], n, hipMemcpyDeviceToHost));
     from_split_rows(dev, h->chains[chain].shape[0], nat);
     std::copy(nat.begin(), nat.end(), occ);
   }
@@ -707,13 +726,13 @@ int mcg_rng_dump(mcg_handle *h, int64_t chain, uint64_t *words, int64_t n_words)
   if (h->mode == MCG_MODE_EXACT) {
     if (n_words < MT_N + 1) return fail(MCG_ERR_CAPACITY, "need 313 words");
     ExactState st;
-    HIP_TRY(hipMemcpy(&st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(h, &st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
     std::memcpy(words, st.mt, sizeof(uint64_t) * MT_N);
     words[MT_N] = (uint64_t)st.pos;
   } else {
     if (n_words < 4) return fail(MCG_ERR_CAPACITY, "need 4 words");
     CheckerState st;
-    HIP_TRY(hipMemcpy(&st, h->d_cs + chain, sizeof(st), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(h, &st, h->d_cs + chain, sizeof(st), hipMemcpyDeviceToHost));
     words[0] = h->cp[chain].key0;
     words[1] = h->cp[chain].key1;
     words[2] = (uint64_t)st.n_pass;
@@ -729,20 +748,20 @@ int mcg_rng_load(mcg_handle *h, int64_t chain, const uint64_t *words, int64_t n_
     if (n_words < MT_N + 1) return fail(MCG_ERR_CAPACITY, "need 313 words");
     if (words[MT_N] > (uint64_t)MT_N) return fail(MCG_ERR_ARG, "engine position out of range");
     ExactState st;
-    HIP_TRY(hipMemcpy(&st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(h, &st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
     std::memcpy(st.mt, words, sizeof(uint64_t) * MT_N);
     st.pos = (int64_t)words[MT_N];
-    HIP_TRY(hipMemcpy(h->d_xs + chain, &st, sizeof(st), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(h, h->d_xs + chain, &st, sizeof(st), hipMemcpyHostToDevice));
   } else {
     if (n_words < 4) return fail(MCG_ERR_CAPACITY, "need 4 words");
     h->cp[chain].key0 = (uint32_t)words[0];
     h->cp[chain].key1 = (uint32_t)words[1];
-    HIP_TRY(hipMemcpy(h->d_cp + chain, &h->cp[chain], sizeof(CheckerParams),
+    HIP_TRY(copy_sync(h, h->d_cp + chain, &h->cp[chain], sizeof(CheckerParams),
                       hipMemcpyHostToDevice));
     CheckerState st;
-    HIP_TRY(hipMemcpy(&st, h->d_cs + chain, sizeof(st), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(h, &st, h->d_cs + chain, sizeof(st), hipMemcpyDeviceToHost));
     st.n_pass = (int64_t)words[2];
-    HIP_TRY(hipMemcpy(h->d_cs + chain, &st, sizeof(st), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(h, h->d_cs + chain, &st, sizeof(st), hipMemcpyHostToDevice));
     h->n_pass[chain] = st.n_pass;
   }
   return MCG_OK;
@@ -772,13 +791,13 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
       return fail(MCG_ERR_CAPACITY, "shape needs more LDS than the handle was created with");
     }
     h->xp[chain] = P;
-    HIP_TRY(hipMemcpy(h->d_xp + chain, &P, sizeof(P), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(h, h->d_xp + chain, &P, sizeof(P), hipMemcpyHostToDevice));
     ExactState S;
     std::memset(&S, 0, sizeof(S));
     int pos = 0;
     mt_seed(nd.seed, S.mt, &pos);
     S.pos = pos;
-    HIP_TRY(hipMemcpy(h->d_xs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(h, h->d_xs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
   } else {
     CheckerParams P;
     std::vector<uint32_t> tab(checker_ntab(h));
@@ -788,15 +807,15 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
       return rc;
     }
     h->cp[chain] = P;
-    HIP_TRY(hipMemcpy(h->d_cp + chain, &P, sizeof(P), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(h->d_tab + P.tab_off, tab.data(), sizeof(uint32_t) * tab.size(),
+    HIP_TRY(copy_sync(h, h->d_cp + chain, &P, sizeof(P), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(h, h->d_tab + P.tab_off, tab.data(), sizeof(uint32_t) * tab.size(),
                       hipMemcpyHostToDevice));
     CheckerState S;
     std::memset(&S, 0, sizeof(S));
-    HIP_TRY(hipMemcpy(h->d_cs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(h, h->d_cs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
     h->work_dirty = true;
   }
-  HIP_TRY(hipMemset(h->d_lat + h->lat_off[chain], 1, (size_t)((h->capacity[chain] + 255) & ~255LL)));
+  HIP_TRY(fill_sync(h, h->d_lat + h->lat_off[chain], 1, (size_t)((h->capacity[chain] + 255) & ~255LL)));
   h->n_samples[chain] = 0;
   h->n_pass[chain] = 0;
   return reset_observables(h, chain, std::vector<int8_t>(n, 1));
@@ -806,7 +825,7 @@ int mcg_set_active(mcg_handle *h, const uint8_t *active, int64_t n_chains) {
   if (!h || !active || n_chains != h->n_chains) return fail(MCG_ERR_ARG, "bad active mask");
   HIP_TRY(hipStreamSynchronize(h->stream));
   for (int c = 0; c < h->n_chains; ++c) h->active[c] = active[c] ? 1 : 0;
-  HIP_TRY(hipMemcpy(h->d_active, h->active.data(), h->n_chains, hipMemcpyHostToDevice));
+  HIP_TRY(copy_sync(h, h->d_active, h->active.data(), h->n_chains, hipMemcpyHostToDevice));
   h->work_dirty = true;
   return MCG_OK;
 }
@@ -839,7 +858,7 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
       h->graph_dirty = true;
     }
     if (G > 1 && n_passes >= G && h->graph_dirty) {
-      rc = capture_graph(h);
+      rc = build_graph(h);
       if (rc != MCG_OK) return rc;
     }
   }
@@ -881,13 +900,13 @@ int mcg_get_counters(mcg_handle *h, int64_t chain, int64_t *n_pass, int64_t *n_a
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (h->mode == MCG_MODE_EXACT) {
     ExactState st;
-    HIP_TRY(hipMemcpy(&st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(h, &st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
     if (n_pass) *n_pass = st.n_pass;
     if (n_accept) *n_accept = st.n_accept;
     if (n_reject) *n_reject = st.n_reject;
   } else {
     CheckerState st;
-    HIP_TRY(hipMemcpy(&st, h->d_cs + chain, sizeof(st), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(h, &st, h->d_cs + chain, sizeof(st), hipMemcpyDeviceToHost));
     if (n_pass) *n_pass = st.n_pass;
     if (n_accept) *n_accept = st.n_accept;
     if (n_reject) *n_reject = st.n_pass * h->sites(chain) - st.n_accept;
@@ -908,7 +927,7 @@ int mcg_get_samples(mcg_handle *h, int64_t chain, int64_t first, int64_t count,
     return fail(MCG_ERR_ARG, "sample range out of bounds");
   if (!count) return MCG_OK;
   HIP_TRY(hipStreamSynchronize(h->stream));
-  HIP_TRY(hipMemcpy(out, h->d_series + chain * h->series_cap + first,
+  HIP_TRY(copy_sync(h, out, h->d_series + chain * h->series_cap + first,
                     sizeof(mcg_sample) * count, hipMemcpyDeviceToHost));
   return MCG_OK;
 }
@@ -918,17 +937,17 @@ int mcg_clear_samples(mcg_handle *h) {
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (h->mode == MCG_MODE_EXACT) {
     std::vector<ExactState> xs(h->n_chains);
-    HIP_TRY(hipMemcpy(xs.data(), h->d_xs, sizeof(ExactState) * h->n_chains,
+    HIP_TRY(copy_sync(h, xs.data(), h->d_xs, sizeof(ExactState) * h->n_chains,
                       hipMemcpyDeviceToHost));
     for (auto &s : xs) s.n_samples = 0;
-    HIP_TRY(hipMemcpy(h->d_xs, xs.data(), sizeof(ExactState) * h->n_chains,
+    HIP_TRY(copy_sync(h, h->d_xs, xs.data(), sizeof(ExactState) * h->n_chains,
                       hipMemcpyHostToDevice));
   } else {
     std::vector<CheckerState> cs(h->n_chains);
-    HIP_TRY(hipMemcpy(cs.data(), h->d_cs, sizeof(CheckerState) * h->n_chains,
+    HIP_TRY(copy_sync(h, cs.data(), h->d_cs, sizeof(CheckerState) * h->n_chains,
                       hipMemcpyDeviceToHost));
     for (auto &s : cs) s.n_samples = 0;
-    HIP_TRY(hipMemcpy(h->d_cs, cs.data(), sizeof(CheckerState) * h->n_chains,
+    HIP_TRY(copy_sync(h, h->d_cs, cs.data(), sizeof(CheckerState) * h->n_chains,
                       hipMemcpyHostToDevice));
   }
   std::fill(h->n_samples.begin(), h->n_samples.end(), 0);
@@ -1038,13 +1057,13 @@ int mcg_autotune(mcg_handle *h, int64_t trial_passes, mcg_schedule *chosen) {
     CheckerState *cs_snap = nullptr;
     HIP_TRY(hipMalloc(&lat_snap, (size_t)h->lat_bytes));
     HIP_TRY(hipMalloc(&cs_snap, sizeof(CheckerState) * h->n_chains));
-    HIP_TRY(hipMemcpy(lat_snap, h->d_lat, (size_t)h->lat_bytes, hipMemcpyDeviceToDevice));
-    HIP_TRY(hipMemcpy(cs_snap, h->d_cs, sizeof(CheckerState) * h->n_chains,
+    HIP_TRY(copy_sync(h, lat_snap, h->d_lat, (size_t)h->lat_bytes, hipMemcpyDeviceToDevice));
+    HIP_TRY(copy_sync(h, cs_snap, h->d_cs, sizeof(CheckerState) * h->n_chains,
                       hipMemcpyDeviceToDevice));
     const auto ns_snap = h->n_samples, np_snap = h->n_pass;
     auto restore = [&]() -> int {
-      HIP_TRY(hipMemcpy(h->d_lat, lat_snap, (size_t)h->lat_bytes, hipMemcpyDeviceToDevice));
-      HIP_TRY(hipMemcpy(h->d_cs, cs_snap, sizeof(CheckerState) * h->n_chains,
+      HIP_TRY(copy_sync(h, h->d_lat, lat_snap, (size_t)h->lat_bytes, hipMemcpyDeviceToDevice));
+      HIP_TRY(copy_sync(h, h->d_cs, cs_snap, sizeof(CheckerState) * h->n_chains,
                         hipMemcpyDeviceToDevice));
       h->n_samples = ns_snap;
       h->n_pass = np_snap;

@@ -501,6 +501,51 @@ void launch_checker_record(const CheckerParams *P, CheckerState *St, mcg_sample 
                      St, series, series_cap, active, n_chains, sample_period);
 }
 
+// `passes` passes of the HBM path as a chain of kernel nodes of graph g (an
+// explicitly built graph: no stream capture, so building one never interacts
+// with work other threads issue meanwhile).
+hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, CheckerState *St,
+                                    int8_t *lat_pool, const uint32_t *tables,
+                                    const int32_t *work, int n_work_items, int ncolor, int upt,
+                                    mcg_sample *series, int64_t series_cap,
+                                    const uint8_t *active, int n_chains, int64_t sample_period,
+                                    int passes) {
+  void *color_fn = upt == 1   ? reinterpret_cast<void *>(&checker_color_kernel<1>)
+                   : upt == 2 ? reinterpret_cast<void *>(&checker_color_kernel<2>)
+                              : reinterpret_cast<void *>(&checker_color_kernel<4>);
+  const unsigned color_grid = (unsigned)n_work_items * (unsigned)(4 / upt);
+  hipGraphNode_t prev = nullptr;
+  auto add = [&](void *fn, dim3 grid, dim3 block, void **args) -> hipError_t {
+    hipKernelNodeParams kp{};
+    kp.func = fn;
+    kp.gridDim = grid;
+    kp.blockDim = block;
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    hipError_t e = hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &kp);
+    prev = node;
+    return e;
+  };
+  for (int p = 0; p < passes; ++p) {
+    if (n_work_items > 0)
+      for (int color = 0; color < ncolor; ++color) {
+        void *args[] = {(void *)&P, (void *)&St, (void *)&lat_pool, (void *)&tables,
+                        (void *)&work, (void *)&color};
+        hipError_t e = add(color_fn, dim3(color_grid), dim3(HBM_BLOCK), args);
+        if (e != hipSuccess) return e;
+      }
+    void *args[] = {(void *)&P,          (void *)&St,     (void *)&series,
+                    (void *)&series_cap, (void *)&active, (void *)&n_chains,
+                    (void *)&sample_period};
+    hipError_t e = add(reinterpret_cast<void *>(&checker_record_kernel),
+                       dim3((n_chains + 63) / 64), dim3(64), args);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 int checker_units_per_item() { return UNITS_PER_ITEM; }
 
 }  // namespace mcg

@@ -87,6 +87,13 @@ void launch_checker_record(const CheckerParams *P, CheckerState *St, mcg_sample 
                            int64_t series_cap, const uint8_t *active, int n_chains,
                            int64_t sample_period, hipStream_t stream);
 
+hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, CheckerState *St,
+                                    int8_t *lat_pool, const uint32_t *tables,
+                                    const int32_t *work, int n_work_items, int ncolor, int upt,
+                                    mcg_sample *series, int64_t series_cap,
+                                    const uint8_t *active, int n_chains, int64_t sample_period,
+                                    int passes);
+
 // ---------------------------------------------------------------------------
 // Statistics on the device sample series
 // ---------------------------------------------------------------------------
