@@ -19,6 +19,7 @@ MODE_EXACT = 0
 MODE_CHECKERBOARD = 1
 LATTICE_SQUARE = 2
 LATTICE_SC = 3
+LATTICE_FCC = 4
 OBS_FORMATION_ENERGY = 0
 OBS_PARAM_COMPOSITION = 1
 OBS_POTENTIAL_ENERGY = 2
@@ -48,11 +49,13 @@ class Desc(C.Structure):
     _fields_ = [("mode", C.c_int64), ("lattice", C.c_int64), ("use_nlist", C.c_int64),
                 ("J1", C.c_double), ("J2", C.c_double), ("kb", C.c_double),
                 ("sample_period", C.c_int64), ("n_chains", C.c_int64),
-                ("chains", C.POINTER(ChainDesc)), ("options", C.c_int64)]
+                ("chains", C.POINTER(ChainDesc)), ("options", C.c_int64),
+                ("eci", C.POINTER(C.c_double))]
 
 
 SAMPLE_DTYPE = np.dtype([("pass", np.int64), ("S1", np.int64), ("S2", np.int64),
-                         ("M", np.int64), ("e_fold", np.float64), ("e_run", np.float64)])
+                         ("M", np.int64), ("e_fold", np.float64), ("e_run", np.float64),
+                         ("S3", np.int64), ("T3", np.int64)])
 
 
 class Schedule(C.Structure):
@@ -117,8 +120,10 @@ class Run:
     """One handle of the C ABI: a batch of chains advanced together on the GPU."""
 
     def __init__(self, chains, mode=MODE_EXACT, lattice=LATTICE_SQUARE, J1=0.1, J2=0.0,
-                 kb=KB, use_nlist=True, sample_period=1, options=0):
-        """chains: sequence of dicts with shape, temperature, exchange_potential, seed."""
+                 kb=KB, use_nlist=True, sample_period=1, options=0, eci=None):
+        """chains: sequence of dicts with shape, temperature, exchange_potential, seed.
+        FCC (lattice=LATTICE_FCC, mode CHECKERBOARD): shape in conventional cells and
+        eci = (V0, V1, V2, V3, V4, Vt)."""
         L = lib()
         self.n_chains = len(chains)
         arr = (ChainDesc * self.n_chains)()
@@ -131,9 +136,16 @@ class Run:
             arr[i].seed = ch.get("seed", 0)
             self.shapes.append(tuple(shp))
         self.chains = [dict(c) for c in chains]
+        self.eci = None
+        eci_arr = None
+        if lattice == LATTICE_FCC:
+            if eci is None or len(eci) != 6:
+                raise ValueError("the FCC cluster expansion needs 6 ECIs")
+            self.eci = tuple(float(v) for v in eci)
+            eci_arr = (C.c_double * 6)(*self.eci)
         d = Desc(mode, lattice, int(use_nlist), J1, J2, kb, sample_period, self.n_chains, arr,
-                 options)
-        self._arr = arr
+                 options, eci_arr)
+        self._arr, self._eci_arr = arr, eci_arr
         h = C.c_void_p()
         _check(L.mcg_create(C.byref(d), C.byref(h)))
         self.h = h
@@ -159,7 +171,7 @@ class Run:
 
     def n_sites(self, c):
         s = self.shapes[c]
-        return s[0] * s[1] * s[2]
+        return (4 if self.lattice == LATTICE_FCC else 1) * s[0] * s[1] * s[2]
 
     def set_occupation(self, c, occ):
         o = np.ascontiguousarray(occ, dtype=np.int8)
@@ -263,6 +275,17 @@ class Run:
         if rec is None:
             rec = self.samples(c)
         N = np.float64(self.n_sites(c))
+        mu = np.float64(self.chains[c].get("exchange_potential", 0.0))
+        if self.lattice == LATTICE_FCC:
+            # E_f = N V0 + V1 N1 + V2/6 P1 + V3/3 P2 + V4/12 P3 + Vt/8 T (device order)
+            v = [np.float64(x) for x in self.eci]
+            coef = [v[0], v[1], v[2] / 6.0, v[3] / 3.0, v[4] / 12.0, v[5] / 8.0]
+            Ef = coef[0] * N
+            for k, f in zip(coef[1:], ("M", "S1", "S2", "S3", "T3")):
+                Ef = Ef + k * rec[f].astype(np.float64)
+            N1 = rec["M"].astype(np.float64)
+            return dict(formation_energy=Ef / N, param_composition=N1 / N,
+                        potential_energy=(Ef - mu * N1) / N)
         if self.mode == MODE_EXACT and not self.use_nlist:
             Ef = rec["e_fold"].astype(np.float64)
         else:

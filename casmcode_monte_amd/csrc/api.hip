@@ -15,11 +15,13 @@
 #include <string>
 #include <vector>
 
+#include "fcc.hh"
 #include "internal.hh"
 
 namespace mcg {
 void exact_set_max_lds(int bytes);
 void checker_set_max_lds(int bytes);
+hipError_t fcc_set_max_lds(int bytes);
 int checker_units_per_item();
 }  // namespace mcg
 
@@ -44,6 +46,7 @@ int fail(int code, std::string const &msg) {
 constexpr int64_t LDS_LIMIT = 160 * 1024;     // gfx950 LDS per workgroup
 constexpr int64_t EXACT_LDS_MAX = 144 * 1024;  // lattice + fold scratch (+ ~10 KB static)
 constexpr int64_t CHECKER_LDS_MAX = 128 * 1024;
+constexpr int64_t FCC_LDS_MAX = 142 * 1024;  // lattice; tables and lists take 17 KB static
 
 // libstdc++ canonical double of a word compared to prob: the first word whose
 // canonical value is >= prob (2^64 if none), by bisection on the monotone map.
@@ -144,6 +147,84 @@ void integer_observables(std::vector<int8_t> const &s, LatticeGeom const &g, boo
       }
 }
 
+// ---- FCC cluster expansion (host side) ------------------------------------------
+struct FccOffset {
+  int dx, dy, dz;
+};
+
+std::vector<FccOffset> fcc_offsets(int shell) {
+  std::vector<FccOffset> v;
+#define MCG_FCC_PUSH(k, dx, dy, dz) v.push_back({dx, dy, dz});
+  if (shell == 1) {
+    MCG_FCC_NN(MCG_FCC_PUSH)
+  } else if (shell == 2) {
+    MCG_FCC_N2(MCG_FCC_PUSH)
+  } else {
+    MCG_FCC_N3(MCG_FCC_PUSH)
+  }
+#undef MCG_FCC_PUSH
+  return v;
+}
+
+bool fcc_is_nn(int dx, int dy, int dz) { return dx * dx + dy * dy + dz * dz == 2; }
+
+// Triangle count of a nearest-neighbour occupancy mask (bit k = neighbour k of
+// MCG_FCC_NN): the occupied pairs among the neighbour pairs that are
+// themselves nearest neighbours.
+std::vector<uint8_t> fcc_tri_table() {
+  const auto nn = fcc_offsets(1);
+  std::vector<std::pair<int, int>> tp;
+  for (int a = 0; a < 12; ++a)
+    for (int b = a + 1; b < 12; ++b)
+      if (fcc_is_nn(nn[b].dx - nn[a].dx, nn[b].dy - nn[a].dy, nn[b].dz - nn[a].dz))
+        tp.push_back({a, b});
+  std::vector<uint8_t> tab(4096);
+  for (int m = 0; m < 4096; ++m) {
+    int t = 0;
+    for (auto const &pr : tp) t += ((m >> pr.first) & 1) & ((m >> pr.second) & 1);
+    tab[m] = (uint8_t)t;
+  }
+  return tab;
+}
+
+// Counts of an FCC occupation (site order l = h + L0*(y + 2*L1*z)).
+void fcc_counts(std::vector<int8_t> const &occ, const int64_t *L, int64_t &N1, int64_t &P1,
+                int64_t &P2, int64_t &P3, int64_t &T) {
+  const int64_t L0 = L[0], ry = 2 * L[1], rz = 2 * L[2], rx = 2 * L[0];
+  auto at = [&](int64_t x, int64_t y, int64_t z) -> int {
+    x = (x % rx + rx) % rx;
+    y = (y % ry + ry) % ry;
+    z = (z % rz + rz) % rz;
+    return occ[(x >> 1) + L0 * (y + ry * z)];
+  };
+  const auto s1 = fcc_offsets(1), s2 = fcc_offsets(2), s3 = fcc_offsets(3);
+  std::vector<std::pair<int, int>> tp;
+  for (int a = 0; a < 12; ++a)
+    for (int b = a + 1; b < 12; ++b)
+      if (fcc_is_nn(s1[b].dx - s1[a].dx, s1[b].dy - s1[a].dy, s1[b].dz - s1[a].dz))
+        tp.push_back({a, b});
+  int64_t p1 = 0, p2 = 0, p3 = 0, t = 0;
+  N1 = 0;
+  for (int64_t l = 0; l < (int64_t)occ.size(); ++l) {
+    if (!occ[l]) continue;
+    const int64_t h = l % L0, line = l / L0, y = line % ry, z = line / ry;
+    const int64_t x = ((y + z) & 1) + 2 * h;
+    ++N1;
+    int nb[12];
+    for (int k = 0; k < 12; ++k) {
+      nb[k] = at(x + s1[k].dx, y + s1[k].dy, z + s1[k].dz);
+      p1 += nb[k];
+    }
+    for (auto const &o : s2) p2 += at(x + o.dx, y + o.dy, z + o.dz);
+    for (auto const &o : s3) p3 += at(x + o.dx, y + o.dy, z + o.dz);
+    for (auto const &pr : tp) t += nb[pr.first] & nb[pr.second];
+  }
+  P1 = p1 / 2;
+  P2 = p2 / 2;
+  P3 = p3 / 2;
+  T = t / 3;
+}
+
 }  // namespace
 
 struct mcg_handle {
@@ -152,6 +233,7 @@ struct mcg_handle {
   double J1 = 0, J2 = 0, kb = 0;
   int64_t sample_period = 1;
   int64_t options = 0;
+  double eci[6] = {0, 0, 0, 0, 0, 0};  // FCC
   int n_chains = 0;
   std::vector<mcg_chain_desc> chains;
   std::vector<int64_t> lat_off;
@@ -199,11 +281,16 @@ struct mcg_handle {
   hipGraphExec_t graph_exec = nullptr;
   bool graph_dirty = true;
 
+  // FCC cluster expansion (CHECKERBOARD with lattice FCC; shares cp/d_cp/d_cs)
+  double *d_ftab = nullptr;  // FCC_NTAB doubles per chain
+  uint8_t *d_tri = nullptr;  // 4096-entry triangle-count table
+  bool fcc() const { return lattice == MCG_LATTICE_FCC; }
+
   ~mcg_handle() {
     if (stream) hipStreamSynchronize(stream);
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     void *ptrs[] = {d_lat, d_active, d_series, d_stats, d_eq, d_xp, d_xs, d_cp, d_cs, d_tab,
-                    d_work};
+                    d_work, d_ftab, d_tri};
     for (void *p : ptrs)
       if (p) hipFree(p);
     if (ev0) hipEventDestroy(ev0);
@@ -212,7 +299,7 @@ struct mcg_handle {
   }
 
   int64_t sites(int64_t c) const {
-    return chains[c].shape[0] * chains[c].shape[1] * chains[c].shape[2];
+    return (fcc() ? 4 : 1) * chains[c].shape[0] * chains[c].shape[1] * chains[c].shape[2];
   }
 };
 
@@ -417,10 +504,91 @@ int build_checker(mcg_handle *h) {
   return MCG_OK;
 }
 
+// Parameters and acceptance tables (FCC_NTAB doubles at tab) of one FCC chain:
+// A[s][n1][n2] = exp(-beta ds (V1 - mu + V2/6 n1 + V3/3 n2)) and
+// B[s][n3][t] = exp(-beta ds (V4/12 n3 + Vt/8 t)), ds = 1 - 2s.
+int fill_fcc_params(mcg_handle *h, int c, CheckerParams &P, double *tab) {
+  auto const &ch = h->chains[c];
+  for (int d = 0; d < 3; ++d)
+    if (ch.shape[d] < 3 || ch.shape[d] > FCC_LMAX)
+      return fail(MCG_ERR_ARG, "FCC edges must be 3..127 conventional cells");
+  std::memset(&P, 0, sizeof(P));
+  P.L0 = (int32_t)ch.shape[0];
+  P.L1 = (int32_t)ch.shape[1];
+  P.L2 = (int32_t)ch.shape[2];
+  P.ncolor = 64;
+  P.z1 = FCC_Z1;
+  P.z2 = FCC_Z2;
+  P.ndim = 3;
+  P.key0 = (uint32_t)ch.seed;
+  P.key1 = (uint32_t)(ch.seed >> 32);
+  P.n = h->sites(c);
+  P.lat_off = h->lat_off[c];
+  P.tab_off = (int64_t)FCC_NTAB * c;
+  const double beta = 1.0 / (h->kb * ch.temperature);
+  const double mu = ch.exchange_potential;
+  const double *v = h->eci;
+  double worst = 0.0;
+  for (int sg = 0; sg < 2; ++sg) {
+    const double ds = 1.0 - 2.0 * sg;
+    for (int n1 = 0; n1 <= FCC_Z1; ++n1)
+      for (int n2 = 0; n2 <= FCC_Z2; ++n2) {
+        const double x = -beta * ds * (v[1] - mu + v[2] / 6.0 * n1 + v[3] / 3.0 * n2);
+        worst = std::max(worst, std::fabs(x));
+        tab[sg * FCC_TAB_A + n1 * (FCC_Z2 + 1) + n2] = std::exp(x);
+      }
+    for (int n3 = 0; n3 <= FCC_Z3; ++n3)
+      for (int t = 0; t <= FCC_ZT; ++t) {
+        const double x = -beta * ds * (v[4] / 12.0 * n3 + v[5] / 8.0 * t);
+        worst = std::max(worst, std::fabs(x));
+        tab[2 * FCC_TAB_A + sg * FCC_TAB_B + n3 * (FCC_ZT + 1) + t] = std::exp(x);
+      }
+  }
+  // each factor must be a finite, normal double for the product to decide
+  if (!(worst < 700.0))
+    return fail(MCG_ERR_ARG, "FCC: beta*|dE| term exceeds 700 (temperature too low for the ECIs)");
+  return MCG_OK;
+}
+
+int build_fcc(mcg_handle *h) {
+  h->ncolor = 64;
+  h->checker_block = 1024;
+  h->cp.resize(h->n_chains);
+  std::vector<double> tabs((size_t)FCC_NTAB * h->n_chains);
+  int64_t max_n = 0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    int rc = fill_fcc_params(h, c, h->cp[c], tabs.data() + (size_t)FCC_NTAB * c);
+    if (rc != MCG_OK) return rc;
+    max_n = std::max(max_n, h->cp[c].n);
+  }
+  h->checker_lds = max_n <= FCC_LDS_MAX && !(h->options & MCG_OPT_FORCE_STREAMING);
+  h->checker_lds_bytes = h->checker_lds ? (int)((max_n + 15) & ~15) : 0;
+  HIP_TRY(fcc_set_max_lds((int)FCC_LDS_MAX));
+  const std::vector<uint8_t> tri = fcc_tri_table();
+  HIP_TRY(hipMalloc(&h->d_cp, sizeof(CheckerParams) * h->n_chains));
+  HIP_TRY(hipMalloc(&h->d_cs, sizeof(CheckerState) * h->n_chains));
+  HIP_TRY(hipMalloc(&h->d_ftab, sizeof(double) * tabs.size()));
+  HIP_TRY(hipMalloc(&h->d_tri, tri.size()));
+  HIP_TRY(copy_sync(h, h->d_cp, h->cp.data(), sizeof(CheckerParams) * h->n_chains,
+                    hipMemcpyHostToDevice));
+  HIP_TRY(copy_sync(h, h->d_ftab, tabs.data(), sizeof(double) * tabs.size(),
+                    hipMemcpyHostToDevice));
+  HIP_TRY(copy_sync(h, h->d_tri, tri.data(), tri.size(), hipMemcpyHostToDevice));
+  HIP_TRY(fill_sync(h, h->d_cs, 0, sizeof(CheckerState) * h->n_chains));
+  return MCG_OK;
+}
+
 // Recompute a chain's integer observables (and EXACT running energy) from its
 // host occupation and store them in the device state.
 int reset_observables(mcg_handle *h, int64_t c, std::vector<int8_t> const &occ) {
   auto const &ch = h->chains[c];
+  if (h->fcc()) {
+    CheckerState st;
+    HIP_TRY(copy_sync(h, &st, h->d_cs + c, sizeof(CheckerState), hipMemcpyDeviceToHost));
+    fcc_counts(occ, ch.shape, st.M, st.S1, st.S2, st.S3, st.T3);
+    HIP_TRY(copy_sync(h, h->d_cs + c, &st, sizeof(CheckerState), hipMemcpyHostToDevice));
+    return MCG_OK;
+  }
   LatticeGeom g{h->lattice == MCG_LATTICE_SC ? 3 : 2, {ch.shape[0], ch.shape[1], ch.shape[2]}};
   int64_t S1, S2, M;
   integer_observables(occ, g, h->J2 != 0.0, S1, S2, M);
@@ -591,8 +759,12 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
   *out = nullptr;
   if (desc->mode != MCG_MODE_EXACT && desc->mode != MCG_MODE_CHECKERBOARD)
     return fail(MCG_ERR_ARG, "unknown mode");
-  if (desc->lattice != MCG_LATTICE_SQUARE && desc->lattice != MCG_LATTICE_SC)
+  if (desc->lattice != MCG_LATTICE_SQUARE && desc->lattice != MCG_LATTICE_SC &&
+      desc->lattice != MCG_LATTICE_FCC)
     return fail(MCG_ERR_ARG, "unknown lattice");
+  if (desc->lattice == MCG_LATTICE_FCC &&
+      (desc->mode != MCG_MODE_CHECKERBOARD || desc->eci == nullptr))
+    return fail(MCG_ERR_ARG, "the FCC cluster expansion runs in CHECKERBOARD mode with 6 ECIs");
   if (desc->mode == MCG_MODE_EXACT && (desc->lattice != MCG_LATTICE_SQUARE || desc->J2 != 0.0))
     return fail(MCG_ERR_ARG, "EXACT mode supports the 2D square nearest-neighbour model only");
   if (desc->n_chains <= 0 || desc->n_chains > (1 << 24) || !desc->chains)
@@ -618,6 +790,8 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
   h->kb = desc->kb;
   h->sample_period = desc->sample_period;
   h->options = desc->options;
+  if (h->fcc())
+    for (int i = 0; i < 6; ++i) h->eci[i] = desc->eci[i];
   h->n_chains = (int)desc->n_chains;
   h->chains.assign(desc->chains, desc->chains + desc->n_chains);
   for (auto &ch : h->chains) {
@@ -661,7 +835,8 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
     return bail(fail(MCG_ERR_HIP, "hipEventCreate failed"));
   if (hipMalloc(&h->d_lat, (size_t)h->lat_bytes) != hipSuccess)
     return bail(fail(MCG_ERR_CAPACITY, "out of device memory for lattices"));
-  if (fill_sync(h, h->d_lat, 1, (size_t)h->lat_bytes) != hipSuccess)
+  // Ising lattices start all +1 (stored as 1), FCC lattices empty
+  if (fill_sync(h, h->d_lat, h->fcc() ? 0 : 1, (size_t)h->lat_bytes) != hipSuccess)
     return bail(fail(MCG_ERR_HIP, "hipMemset failed"));
   h->active.assign(h->n_chains, 1);
   if (hipMalloc(&h->d_active, h->n_chains) != hipSuccess ||
@@ -672,11 +847,11 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
   for (int c = 0; c < h->n_chains; ++c) h->capacity.push_back(h->sites(c));
   if (hipMalloc(&h->d_eq, 2 * sizeof(int64_t)) != hipSuccess)
     return bail(fail(MCG_ERR_HIP, "allocation failed"));
-  rc = (h->mode == MCG_MODE_EXACT) ? build_exact(h) : build_checker(h);
+  rc = (h->mode == MCG_MODE_EXACT) ? build_exact(h) : (h->fcc() ? build_fcc(h) : build_checker(h));
   if (rc != MCG_OK) return bail(rc);
-  // all +1 lattices: set observables
+  // initial lattices: set observables
   for (int c = 0; c < h->n_chains; ++c) {
-    std::vector<int8_t> occ(h->sites(c), 1);
+    std::vector<int8_t> occ(h->sites(c), h->fcc() ? 0 : 1);
     rc = reset_observables(h, c, occ);
     if (rc != MCG_OK) return bail(rc);
   }
@@ -692,10 +867,15 @@ int mcg_set_occupation(mcg_handle *h, int64_t chain, const int8_t *occ, int64_t 
   if (!valid_chain(h, chain) || !occ) return fail(MCG_ERR_ARG, "bad chain or buffer");
   if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "occupation size mismatch");
   std::vector<int8_t> v(occ, occ + n);
-  for (int8_t s : v)
-    if (s != 1 && s != -1) return fail(MCG_ERR_ARG, "occupation values must be +1 or -1");
+  if (h->fcc()) {
+    for (int8_t s : v)
+      if (s != 0 && s != 1) return fail(MCG_ERR_ARG, "FCC occupation values must be 0 or 1");
+  } else {
+    for (int8_t s : v)
+      if (s != 1 && s != -1) return fail(MCG_ERR_ARG, "occupation values must be +1 or -1");
+  }
   HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->mode == MCG_MODE_EXACT) {
+  if (h->mode == MCG_MODE_EXACT || h->fcc()) {
     HIP_TRY(copy_sync(h, h->d_lat + h->lat_off[chain], v.data(), n, hipMemcpyHostToDevice));
   } else {
     std::vector<int8_t> dev(n);
@@ -709,7 +889,7 @@ int mcg_get_occupation(mcg_handle *h, int64_t chain, int8_t *occ, int64_t n) {
   if (!valid_chain(h, chain) || !occ) return fail(MCG_ERR_ARG, "bad chain or buffer");
   if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "occupation size mismatch");
   HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->mode == MCG_MODE_EXACT) {
+  if (h->mode == MCG_MODE_EXACT || h->fcc()) {
     HIP_TRY(copy_sync(h, occ, h->d_lat + h->lat_off[chain], n, hipMemcpyDeviceToHost));
   } else {
     std::vector<int8_t> dev(n), nat(n);
@@ -774,7 +954,7 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
   for (int d = 0; d < 3; ++d)
     if (nd.shape[d] < 1 || nd.shape[d] > (1 << 20)) return fail(MCG_ERR_ARG, "shape out of range");
   if (!(nd.temperature > 0.0)) return fail(MCG_ERR_ARG, "temperature must be > 0");
-  const int64_t n = nd.shape[0] * nd.shape[1] * nd.shape[2];
+  const int64_t n = (h->fcc() ? 4 : 1) * nd.shape[0] * nd.shape[1] * nd.shape[2];
   if (n > h->capacity[chain])
     return fail(MCG_ERR_CAPACITY, "shape has more sites than the chain slot holds");
   if (h->mode == MCG_MODE_EXACT && nd.shape[0] >= (1LL << 16))
@@ -798,6 +978,25 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
     mt_seed(nd.seed, S.mt, &pos);
     S.pos = pos;
     HIP_TRY(copy_sync(h, h->d_xs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
+  } else if (h->fcc()) {
+    CheckerParams P;
+    std::vector<double> tab(FCC_NTAB);
+    const int rc = fill_fcc_params(h, (int)chain, P, tab.data());
+    if (rc != MCG_OK) {
+      h->chains[chain] = old;
+      return rc;
+    }
+    if (h->checker_lds && P.n > h->checker_lds_bytes) {
+      h->chains[chain] = old;
+      return fail(MCG_ERR_CAPACITY, "shape needs more LDS than the handle was created with");
+    }
+    h->cp[chain] = P;
+    HIP_TRY(copy_sync(h, h->d_cp + chain, &P, sizeof(P), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(h, h->d_ftab + P.tab_off, tab.data(), sizeof(double) * tab.size(),
+                      hipMemcpyHostToDevice));
+    CheckerState S;
+    std::memset(&S, 0, sizeof(S));
+    HIP_TRY(copy_sync(h, h->d_cs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
   } else {
     CheckerParams P;
     std::vector<uint32_t> tab(checker_ntab(h));
@@ -815,10 +1014,12 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
     HIP_TRY(copy_sync(h, h->d_cs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
     h->work_dirty = true;
   }
-  HIP_TRY(fill_sync(h, h->d_lat + h->lat_off[chain], 1, (size_t)((h->capacity[chain] + 255) & ~255LL)));
+  const int fill = h->fcc() ? 0 : 1;
+  HIP_TRY(fill_sync(h, h->d_lat + h->lat_off[chain], fill,
+                    (size_t)((h->capacity[chain] + 255) & ~255LL)));
   h->n_samples[chain] = 0;
   h->n_pass[chain] = 0;
-  return reset_observables(h, chain, std::vector<int8_t>(n, 1));
+  return reset_observables(h, chain, std::vector<int8_t>(n, (int8_t)fill));
 }
 
 int mcg_set_active(mcg_handle *h, const uint8_t *active, int64_t n_chains) {
@@ -849,7 +1050,7 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
   double attempts = 0.0;
   for (int c = 0; c < h->n_chains; ++c)
     if (h->active[c]) attempts += (double)h->sites(c) * (double)n_passes;
-  const bool hbm_path = h->mode != MCG_MODE_EXACT && !h->checker_lds;
+  const bool hbm_path = h->mode != MCG_MODE_EXACT && !h->checker_lds && !h->fcc();
   const int G = h->graph_passes;
   if (hbm_path) {
     if (h->work_dirty) {
@@ -868,6 +1069,17 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
                  h->n_chains, n_passes, h->sample_period, h->exact_lds, h->exact_in_lds,
                  !(h->options & MCG_OPT_EXACT_SCALAR_WALK),
                  h->stream);
+  } else if (h->fcc()) {
+    HIP_TRY(hipGetLastError());
+    launch_fcc(h->d_cp, h->d_cs, h->d_lat, h->d_ftab, h->d_tri, h->d_series, h->series_cap,
+               h->d_active, h->n_chains, n_passes, h->sample_period, h->checker_lds_bytes,
+               h->checker_lds, h->checker_block, h->stream);
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess)
+      return fail(MCG_ERR_HIP, std::string("fcc launch (block ") +
+                                   std::to_string(h->checker_block) + ", lds " +
+                                   std::to_string(h->checker_lds_bytes) +
+                                   "): " + hipGetErrorString(le));
   } else if (h->checker_lds) {
     launch_checker_lds(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_series, h->series_cap,
                        h->d_active, h->n_chains, n_passes, h->sample_period,
@@ -962,6 +1174,14 @@ static ObsParams obs_params(mcg_handle *h, int64_t chain, int64_t observable) {
   op.mu = h->chains[chain].exchange_potential;
   op.fold = (h->mode == MCG_MODE_EXACT && !h->use_nlist) ? 1 : 0;
   op.observable = (int32_t)observable;
+  op.fcc = h->fcc() ? 1 : 0;
+  op.pad = 0;
+  op.coef[0] = h->eci[0];
+  op.coef[1] = h->eci[1];
+  op.coef[2] = h->eci[2] / 6.0;
+  op.coef[3] = h->eci[3] / 3.0;
+  op.coef[4] = h->eci[4] / 12.0;
+  op.coef[5] = h->eci[5] / 8.0;
   return op;
 }
 
@@ -1070,7 +1290,7 @@ int mcg_autotune(mcg_handle *h, int64_t trial_passes, mcg_schedule *chosen) {
       return MCG_OK;
     };
     std::vector<mcg_schedule> cands;
-    if (h->checker_lds) {
+    if (h->checker_lds || h->fcc()) {
       for (int64_t b : {256, 512, 1024}) cands.push_back({b, h->units_per_thread, 1, 0.0});
     } else {
       for (int64_t u : {1, 2, 4})

@@ -348,7 +348,7 @@ __global__ __launch_bounds__(1024) void checker_lds_kernel(
     ++pass;
     if (pass % sample_period == 0) {
       if (t == 0 && n_samples < series_cap) {
-        mcg_sample rec;
+        mcg_sample rec = {};
         rec.pass = pass;
         rec.S1 = S1;
         rec.S2 = S2;
@@ -467,7 +467,7 @@ __global__ void checker_record_kernel(const CheckerParams *__restrict__ Pp,
   if (pass % sample_period == 0) {
     const int64_t ns = St->n_samples;
     if (ns < series_cap) {
-      mcg_sample rec;
+      mcg_sample rec = {};
       rec.pass = pass;
       rec.S1 = St->S1;
       rec.S2 = St->S2;

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(64) void exact_kernel(const ExactParams *__restrict
         __syncthreads();
       }
       if (lane == 0 && n_samples < series_cap) {
-        mcg_sample rec;
+        mcg_sample rec = {};
         rec.pass = n_pass;
         rec.S1 = S;
         rec.S2 = 0;
@@ -489,7 +489,7 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
           __syncthreads();
         }
         if (lane == 0 && n_samples < series_cap) {
-          mcg_sample rec;
+          mcg_sample rec = {};
           rec.pass = n_pass;
           rec.S1 = S;
           rec.S2 = 0;

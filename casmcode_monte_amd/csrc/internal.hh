@@ -70,8 +70,9 @@ struct CheckerParams {
 struct CheckerState {
   int64_t n_pass;
   int64_t n_accept;
-  int64_t S1, S2, M;       // exact integer observables
+  int64_t S1, S2, M;       // exact integer observables (FCC: P1, P2, N1)
   int64_t n_samples;
+  int64_t S3, T3;          // FCC: P3 and the triangle count
 };
 
 void launch_checker_lds(const CheckerParams *P, CheckerState *St, int8_t *lat_pool,
@@ -94,6 +95,14 @@ hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, Checke
                                     const uint8_t *active, int n_chains, int64_t sample_period,
                                     int passes);
 
+// FCC cluster expansion (fcc.hip): CheckerParams with ncolor = 64, tab_off in
+// doubles into the per-chain acceptance tables, tri = the 4096-entry
+// triangle-count table of the nearest-neighbour occupancy mask.
+void launch_fcc(const CheckerParams *P, CheckerState *St, int8_t *lat_pool, const double *tables,
+                const uint8_t *tri, mcg_sample *series, int64_t series_cap,
+                const uint8_t *active, int n_chains, int64_t n_passes, int64_t sample_period,
+                int lds_bytes, bool lattice_in_lds, int block, hipStream_t stream);
+
 // ---------------------------------------------------------------------------
 // Statistics on the device sample series
 // ---------------------------------------------------------------------------
@@ -103,6 +112,9 @@ struct ObsParams {
   double mu;
   int32_t fold;  // use e_fold as E_f
   int32_t observable;
+  int32_t fcc;   // FCC cluster expansion: E_f from the counts and coef
+  int32_t pad;
+  double coef[6];  // FCC: V0, V1, V2/6, V3/3, V4/12, Vt/8
 };
 
 void launch_stats(const mcg_sample *series, int64_t first, int64_t count, ObsParams op,

@@ -18,6 +18,19 @@ namespace mcg {
 
 __device__ __forceinline__ double observable_of(const mcg_sample &r, const ObsParams &op) {
   const double N = (double)op.n_sites;
+  if (op.fcc) {
+    // E_f = N V0 + V1 N1 + V2/6 P1 + V3/3 P2 + V4/12 P3 + Vt/8 T, summed in this order
+    double ef = __dmul_rn(op.coef[0], N);
+    ef = __dadd_rn(ef, __dmul_rn(op.coef[1], (double)r.M));
+    ef = __dadd_rn(ef, __dmul_rn(op.coef[2], (double)r.S1));
+    ef = __dadd_rn(ef, __dmul_rn(op.coef[3], (double)r.S2));
+    ef = __dadd_rn(ef, __dmul_rn(op.coef[4], (double)r.S3));
+    ef = __dadd_rn(ef, __dmul_rn(op.coef[5], (double)r.T3));
+    if (op.observable == MCG_OBS_FORMATION_ENERGY) return __ddiv_rn(ef, N);
+    const double N1 = (double)r.M;
+    if (op.observable == MCG_OBS_PARAM_COMPOSITION) return __ddiv_rn(N1, N);
+    return __ddiv_rn(__dsub_rn(ef, __dmul_rn(op.mu, N1)), N);
+  }
   // formation energy per supercell
   double ef;
   if (op.fold) {

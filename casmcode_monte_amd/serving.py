@@ -1,7 +1,8 @@
-"""Continuous-batching server of SGC Ising runs on one MI355X.
+"""Continuous-batching server of SGC Metropolis runs on one MI355X.
 
-A request is one run of the 2D square-lattice Ising model (edge L, temperature,
-exchange potential, seed) that stops when the precision of <x> (the
+A request is one run of the 2D square-lattice Ising model (edge L) or of the
+binary FCC cluster expansion (L^3 conventional cells), at a temperature,
+exchange potential and seed, that stops when the precision of <x> (the
 parametric composition) reaches the requested value or after max_passes.
 Stopping follows libcasm-monte's completion check with one requested
 precision (checks/CompletionCheck.hh:272-376): at every check the series is
@@ -11,10 +12,11 @@ equilibration samples must reach calculated_precision < requested
 (ConvergenceCheck.hh:74-119, BasicStatistics.cc:114-131).  Both reductions run
 on the device over the device-resident sample series.
 
-Slots: requests are routed by lattice size to pools (one library handle and
-one host thread each): LDS-resident classes up to 128 KB lattices, and an HBM
-class above.  A pool runs `chunk` passes of all its active slots per round
-(one launch), then checks the slots whose own pass count reached a multiple
+Slots: requests are routed by model and lattice size to pools (one library
+handle and one host thread each): LDS-resident classes up to 128 KB (Ising) or
+142 KB (FCC) lattices, and an HBM class above.  A pool runs `chunk` passes of
+all its active slots per round (one launch), then checks the slots whose own
+pass count reached a multiple
 of `check_period`, retires finished requests and admits waiting ones into the
 freed slots (mcg_assign_chain) -- requests that arrive while others run join
 the next round.  Admission also respects a global HBM footprint budget
@@ -32,8 +34,9 @@ import zlib
 
 import numpy as np
 
-from . import (KB, LATTICE_SQUARE, MODE_CHECKERBOARD, OBS_PARAM_COMPOSITION, SAMPLE_DTYPE,
-               Run)
+from . import (KB, LATTICE_FCC, LATTICE_SQUARE, MODE_CHECKERBOARD, OBS_PARAM_COMPOSITION,
+               OPT_FORCE_STREAMING, SAMPLE_DTYPE, Run)
+from .fcc import DEFAULT_ECI
 
 
 @dataclasses.dataclass
@@ -46,9 +49,16 @@ class Request:
     arrival: float = 0.0       # seconds after the start of the trace
     max_passes: int = 20000
     precision: float = 1e-3    # requested absolute precision of <x>
+    model: str = "ising"       # "ising" (2D, edge L) or "fcc" (L^3 conventional cells)
+
+    def sites(self):
+        return 4 * self.edge ** 3 if self.model == "fcc" else self.edge * self.edge
+
+    def shape(self):
+        return (self.edge,) * 3 if self.model == "fcc" else (self.edge, self.edge)
 
     def footprint(self, sample_bytes=SAMPLE_DTYPE.itemsize):
-        return self.edge * self.edge + self.max_passes * sample_bytes
+        return self.sites() + self.max_passes * sample_bytes
 
 
 @dataclasses.dataclass
@@ -73,16 +83,26 @@ class Result:
 # (max edge, slots, LDS-resident) per size class
 DEFAULT_CLASSES = ((64, 96, True), (128, 64, True), (256, 48, True), (362, 32, True),
                    (512, 16, False))
+# FCC: edges in conventional cells (4 L^3 sites; LDS-resident up to L = 33)
+DEFAULT_FCC_CLASSES = ((12, 64, True), (20, 48, True), (33, 32, True), (48, 12, False))
 
 
 class _Pool:
-    def __init__(self, max_edge, n_slots, lds, J1, J2, chunk, check_period):
-        e = max_edge - (max_edge & 1)
+    def __init__(self, model, max_edge, n_slots, lds, J1, J2, eci, chunk, check_period):
+        self.model = model
         self.max_edge = max_edge
-        chains = [dict(shape=(e, e), temperature=1000.0, exchange_potential=0.0, seed=i)
-                  for i in range(n_slots)]
-        self.run = Run(chains, mode=MODE_CHECKERBOARD, lattice=LATTICE_SQUARE, J1=J1, J2=J2,
-                       options=0 if lds else 1)
+        opts = 0 if lds else OPT_FORCE_STREAMING
+        if model == "fcc":
+            chains = [dict(shape=(max_edge,) * 3, temperature=1000.0, exchange_potential=0.0,
+                           seed=i) for i in range(n_slots)]
+            self.run = Run(chains, mode=MODE_CHECKERBOARD, lattice=LATTICE_FCC, eci=eci,
+                           options=opts)
+        else:
+            e = max_edge - (max_edge & 1)
+            chains = [dict(shape=(e, e), temperature=1000.0, exchange_potential=0.0, seed=i)
+                      for i in range(n_slots)]
+            self.run = Run(chains, mode=MODE_CHECKERBOARD, lattice=LATTICE_SQUARE, J1=J1,
+                           J2=J2, options=opts)
         self.active = np.zeros(n_slots, dtype=np.uint8)
         self.run.set_active(self.active)
         self.slot_req = [None] * n_slots
@@ -94,13 +114,17 @@ class _Pool:
 
 
 class Server:
-    """Continuous-batching Metropolis server (CHECKERBOARD mode, 2D square lattice)."""
+    """Continuous-batching Metropolis server (CHECKERBOARD mode: 2D square-lattice
+    Ising and binary FCC cluster expansion pools)."""
 
     def __init__(self, classes=DEFAULT_CLASSES, J1=0.1, J2=0.0, chunk=100, check_period=500,
-                 hbm_budget=8 << 30):
+                 hbm_budget=8 << 30, fcc_classes=(), eci=DEFAULT_ECI):
         if check_period % chunk:
             raise ValueError("check_period must be a multiple of chunk")
-        self.pools = [_Pool(e, n, lds, J1, J2, chunk, check_period) for e, n, lds in classes]
+        self.pools = [_Pool("ising", e, n, lds, J1, J2, None, chunk, check_period)
+                      for e, n, lds in classes]
+        self.pools += [_Pool("fcc", e, n, lds, J1, J2, eci, chunk, check_period)
+                       for e, n, lds in fcc_classes]
         self.hbm_budget = hbm_budget
         self.in_use = 0
         self.max_in_use = 0
@@ -109,12 +133,14 @@ class Server:
         self.errors = []
 
     def pool_for(self, req):
-        if req.edge % 2 or req.edge < 2:
-            raise ValueError("edge lengths must be even (checkerboard colouring)")
+        if req.model == "ising" and (req.edge % 2 or req.edge < 2):
+            raise ValueError("Ising edge lengths must be even (checkerboard colouring)")
+        if req.model == "fcc" and req.edge < 3:
+            raise ValueError("FCC edges must be >= 3 conventional cells")
         for p in self.pools:
-            if req.edge <= p.max_edge:
+            if p.model == req.model and req.edge <= p.max_edge:
                 return p
-        raise ValueError(f"edge {req.edge} exceeds the largest slot class")
+        raise ValueError(f"no {req.model} slot class holds edge {req.edge}")
 
     # --- one pool's loop ----------------------------------------------------------
     def _admit(self, p, now):
@@ -131,7 +157,7 @@ class Server:
                 self.max_in_use = max(self.max_in_use, self.in_use)
             p.queue.pop(0)
             s = int(free[0])
-            p.run.assign(s, dict(shape=(req.edge, req.edge), temperature=req.temperature,
+            p.run.assign(s, dict(shape=req.shape(), temperature=req.temperature,
                                  exchange_potential=req.exchange_potential, seed=req.seed))
             p.slot_req[s] = req
             p.passes[s] = 0
@@ -225,9 +251,13 @@ class Server:
 
 
 def poisson_trace(n, rate=50.0, seed=2024, edge_range=(16, 512), T_range=(300.0, 3000.0),
-                  mu_range=(-1.0, 1.0), max_passes=20000, precision=1e-3):
-    """Request trace: Poisson arrivals at `rate` per second, edges log-uniform in
-    edge_range rounded to the nearest even integer, T and mu uniform."""
+                  mu_range=(-1.0, 1.0), max_passes=20000, precision=1e-3, fcc_fraction=0.0,
+                  fcc_edge_range=(8, 48)):
+    """Request trace: Poisson arrivals at `rate` per second, T and mu uniform.
+    Ising edges are log-uniform in edge_range rounded to the nearest even
+    integer; with fcc_fraction > 0 each request is an FCC run with that
+    probability (drawn after the Ising trace, so fcc_fraction = 0 reproduces it),
+    its edge log-uniform in fcc_edge_range rounded to the nearest integer."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, size=n))
     lo, hi = np.log(edge_range[0]), np.log(edge_range[1])
@@ -235,5 +265,13 @@ def poisson_trace(n, rate=50.0, seed=2024, edge_range=(16, 512), T_range=(300.0,
     edges = np.clip(2 * np.round(edges / 2), edge_range[0], edge_range[1]).astype(int)
     T = rng.uniform(*T_range, size=n)
     mu = rng.uniform(*mu_range, size=n)
+    models = ["ising"] * n
+    if fcc_fraction > 0.0:
+        is_fcc = rng.random(n) < fcc_fraction
+        flo, fhi = np.log(fcc_edge_range[0]), np.log(fcc_edge_range[1])
+        fedges = np.clip(np.round(np.exp(rng.uniform(flo, fhi, size=n))), *fcc_edge_range)
+        for i in np.flatnonzero(is_fcc):
+            models[i] = "fcc"
+            edges[i] = int(fedges[i])
     return [Request(i, int(edges[i]), float(T[i]), float(mu[i]), 0x5E00000 + i, float(t[i]),
-                    max_passes, precision) for i in range(n)]
+                    max_passes, precision, models[i]) for i in range(n)]

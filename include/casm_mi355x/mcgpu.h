@@ -47,6 +47,19 @@ extern "C" {
 
 #define MCG_LATTICE_SQUARE 2 /* 2D, shape[2] == 1 */
 #define MCG_LATTICE_SC 3     /* 3D simple cubic */
+/* Binary FCC cluster expansion (CHECKERBOARD only; a build-defined model with
+ * no counterpart among the reference's lattice models).  shape = conventional
+ * cells per edge (each 3..127), 4*L0*L1*L2 sites at the doubled coordinates
+ * (x, y, z), x+y+z even, of a periodic 2L0 x 2L1 x 2L2 grid; linear site
+ * order l = h + L0*(y + 2*L1*z) with x = ((y+z)&1) + 2h.  Occupations are
+ * 0/1.  Energy per supercell, with N sites, N1 = sum of occupations and Pk /
+ * T the occupied pair (shells 1-3) and nearest-neighbour triangle counts:
+ *   E = N*V0 + V1*N1 + V2/6*P1 + V3/3*P2 + V4/12*P3 + Vt/8*T
+ * (orbit averages per primitive cell; per-site multiplicities 6, 3, 12, 8),
+ * ECIs given in mcg_desc.eci = {V0, V1, V2, V3, V4, Vt}.  Colouring: each
+ * doubled coordinate is coloured mod 4 (a seam of two sites gets colours 1,
+ * 2 when 2L = 2 mod 4), at most 64 colour classes per pass. */
+#define MCG_LATTICE_FCC 4
 
 /* One chain: one supercell at one (T, mu) condition.
  * Replaces the (configuration, conditions) pair of IsingState
@@ -77,6 +90,8 @@ typedef struct mcg_desc {
   int64_t n_chains;
   const mcg_chain_desc *chains; /* n_chains entries */
   int64_t options;       /* bit mask of MCG_OPT_*                             */
+  const double *eci;     /* MCG_LATTICE_FCC: 6 ECIs (V0, V1, V2, V3, V4, Vt), eV;
+                            otherwise ignored                                 */
 } mcg_desc;
 
 /* CHECKERBOARD: always use the HBM work-list kernels, even for lattices that
@@ -105,7 +120,12 @@ typedef struct mcg_sample {
   double e_run;   /* EXACT: running potential energy per unit cell, updated by
                      fl(dE / n_unitcells) per accepted step
                      (occupation_metropolis.hh:141); otherwise 0             */
+  int64_t S3;     /* FCC: third-shell pair count; otherwise 0                 */
+  int64_t T3;     /* FCC: nearest-neighbour triangle count; otherwise 0       */
 } mcg_sample;
+/* FCC records: M = N1 (occupied sites), S1 = P1, S2 = P2, S3 = P3 (pairs of
+ * shells 1-3 with both sites occupied), T3 = occupied nearest-neighbour
+ * triangles. */
 
 typedef struct mcg_handle mcg_handle;
 
@@ -128,7 +148,8 @@ const char *mcg_last_error(void);
  * Replaces IsingConfiguration::set_occupation / occupation (model.hh:52-60).
  * Setting recomputes the chain's integer observables and, for EXACT, the
  * running energy from the initial per-supercell potential
- * (occupation_metropolis.hh:106). */
+ * (occupation_metropolis.hh:106).  FCC: values 0/1 in the FCC site order;
+ * lattices start empty (all 0). */
 int mcg_set_occupation(mcg_handle *h, int64_t chain, const int8_t *occ, int64_t n);
 int mcg_get_occupation(mcg_handle *h, int64_t chain, int8_t *occ, int64_t n);
 

@@ -322,4 +322,70 @@ int orc_lattice_enumerate(int64_t ndim, const int64_t *L, double J1, double J2,
   return 0;
 }
 
+// ---- binary FCC cluster expansion --------------------------------------------
+static FccModel fcc_model(const int64_t *L, const double *eci) {
+  FccModel m;
+  for (int d = 0; d < 3; ++d) m.L[d] = L[d];
+  if (eci)
+    for (int i = 0; i < 6; ++i) m.eci[i] = eci[i];
+  return m;
+}
+
+// out5 = N1, P1, P2, P3, T
+void orc_fcc_counts(const int64_t *L, const int8_t *occ, int64_t *out5) {
+  FccModel m = fcc_model(L, nullptr);
+  std::vector<int8_t> s(occ, occ + m.n_sites());
+  FccCounts c = fcc_counts(m, s);
+  out5[0] = c.N1;
+  out5[1] = c.P1;
+  out5[2] = c.P2;
+  out5[3] = c.P3;
+  out5[4] = c.T;
+}
+
+// out4 = n1, n2, n3, t of site l
+void orc_fcc_local(const int64_t *L, const int8_t *occ, int64_t l, int64_t *out4) {
+  FccModel m = fcc_model(L, nullptr);
+  std::vector<int8_t> s(occ, occ + m.n_sites());
+  int n1, n2, n3, t;
+  fcc_local(m, s, (long)l, n1, n2, n3, t);
+  out4[0] = n1;
+  out4[1] = n2;
+  out4[2] = n3;
+  out4[3] = t;
+}
+
+// shell sizes (12, 6, 24) and triangle pairs per site (24)
+void orc_fcc_shell_sizes(int64_t *out4) {
+  out4[0] = (int64_t)FccModel::shell(2).size();
+  out4[1] = (int64_t)FccModel::shell(4).size();
+  out4[2] = (int64_t)FccModel::shell(6).size();
+  out4[3] = (int64_t)FccModel::triangle_pairs().size();
+}
+
+double orc_fcc_energy(const int64_t *L, const double *eci, const int8_t *occ) {
+  FccModel m = fcc_model(L, eci);
+  std::vector<int8_t> s(occ, occ + m.n_sites());
+  return fcc_energy(m, fcc_counts(m, s));
+}
+
+// counts: n_passes x 5 (N1, P1, P2, P3, T after each pass); occ in/out
+int orc_fcc_run(const int64_t *L, const double *eci, double temperature, double mu,
+                uint64_t seed, int64_t n_passes, int8_t *occ, int64_t *counts) {
+  FccModel m = fcc_model(L, eci);
+  std::vector<int8_t> s(occ, occ + m.n_sites());
+  std::mt19937_64 eng(seed);
+  std::vector<FccCounts> out;
+  fcc_sequential_run(m, s, temperature, mu, eng, n_passes, out);
+  for (int64_t p = 0; p < n_passes; ++p) {
+    counts[5 * p + 0] = out[p].N1;
+    counts[5 * p + 1] = out[p].P1;
+    counts[5 * p + 2] = out[p].P2;
+    counts[5 * p + 3] = out[p].P3;
+    counts[5 * p + 4] = out[p].T;
+  }
+  std::copy(s.begin(), s.end(), occ);
+  return 0;
+}
+
 }  // extern "C"

@@ -24,8 +24,9 @@
 // Models that the reference does not define (2D with next-nearest J2, 3D simple
 // cubic, binary FCC cluster expansion) are defined here in the reference's
 // conventions (E = -J * sum over bonds of s_i s_j; SGC potential E - mu * N * x).
-// Their pinning is by exact enumeration of tiny lattices and closed forms, see
-// tests/test_oracle_models.py.
+// Their pinning is by exact enumeration of tiny lattices, closed forms and
+// hand-counted ordered structures, in tests/test_oracle_kats.py and
+// tests/test_oracle_fcc.py.
 #pragma once
 
 #include <algorithm>
@@ -833,6 +834,157 @@ inline void lattice_sequential_run(LatticeModel const &m, std::vector<int8_t> &s
     S1_out.push_back(S1);
     S2_out.push_back(S2);
     M_out.push_back(M);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Binary FCC cluster expansion (build-defined; the reference has no lattice
+// model of its own beyond the 2D Ising one).
+//
+// Sites are the points (x, y, z) of a periodic 2L0 x 2L1 x 2L2 integer grid
+// with x + y + z even (doubled conventional coordinates, L = conventional
+// cells per edge, 4 L0 L1 L2 sites).  Linear index l = h + L0 * (y + 2 L1 z)
+// with x = ((y + z) & 1) + 2h.  Occupation sigma in {0, 1}.
+//
+// Energy per supercell, correlations as orbit averages per primitive cell
+// (one site) with per-site orbit multiplicities 6, 3, 12 (pairs of shells
+// 1-3) and 8 (nearest-neighbour triangles):
+//   E = N V0 + V1 N1 + (V2/6) P1 + (V3/3) P2 + (V4/12) P3 + (Vt/8) T
+// with N1 = sum sigma, Pk = number of shell-k pairs with both sites occupied,
+// T = number of nearest-neighbour triangles with all three occupied.
+// The flip of site i (dsigma = 1 - 2 sigma_i) changes E by
+//   dsigma * (V1 + (V2/6) n1 + (V3/3) n2 + (V4/12) n3 + (Vt/8) t)
+// with nk = occupied shell-k neighbours and t = triangles through i whose two
+// other sites are occupied.
+// ---------------------------------------------------------------------------
+struct FccModel {
+  long L[3] = {3, 3, 3};
+  double eci[6] = {0, 0, 0, 0, 0, 0};  // V0, V1, V2 (NN), V3 (2NN), V4 (3NN), Vt
+  long n_sites() const { return 4 * L[0] * L[1] * L[2]; }
+
+  // shell offsets by squared length in doubled coordinates: 2, 4, 6
+  static std::vector<std::array<int, 3>> shell(int d2) {
+    std::vector<std::array<int, 3>> out;
+    for (int a = -2; a <= 2; ++a)
+      for (int b = -2; b <= 2; ++b)
+        for (int c = -2; c <= 2; ++c)
+          if (((a + b + c) & 1) == 0 && a * a + b * b + c * c == d2) out.push_back({a, b, c});
+    return out;
+  }
+  // unordered pairs of nearest-neighbour offsets that are nearest neighbours
+  // of each other: the triangles through a site
+  static std::vector<std::pair<int, int>> triangle_pairs() {
+    auto nn = shell(2);
+    std::vector<std::pair<int, int>> out;
+    for (size_t a = 0; a < nn.size(); ++a)
+      for (size_t b = a + 1; b < nn.size(); ++b) {
+        int dx = nn[b][0] - nn[a][0], dy = nn[b][1] - nn[a][1], dz = nn[b][2] - nn[a][2];
+        if (dx * dx + dy * dy + dz * dz == 2) out.push_back({(int)a, (int)b});
+      }
+    return out;
+  }
+  long wrap(long a, int d) const {
+    long n = 2 * L[d], r = a % n;
+    return r < 0 ? r + n : r;
+  }
+  long index(long x, long y, long z) const {
+    x = wrap(x, 0);
+    y = wrap(y, 1);
+    z = wrap(z, 2);
+    long h = (x - ((y + z) & 1)) / 2;
+    return h + L[0] * (y + 2 * L[1] * z);
+  }
+  void coords(long l, long &x, long &y, long &z) const {
+    long h = l % L[0], line = l / L[0];
+    y = line % (2 * L[1]);
+    z = line / (2 * L[1]);
+    x = ((y + z) & 1) + 2 * h;
+  }
+};
+
+struct FccCounts {
+  long long N1 = 0, P1 = 0, P2 = 0, P3 = 0, T = 0;
+};
+
+inline FccCounts fcc_counts(FccModel const &m, std::vector<int8_t> const &s) {
+  const auto s1 = FccModel::shell(2), s2 = FccModel::shell(4), s3 = FccModel::shell(6);
+  const auto tp = FccModel::triangle_pairs();
+  FccCounts c;
+  long long p1 = 0, p2 = 0, p3 = 0, t = 0;
+  const long N = m.n_sites();
+  for (long l = 0; l < N; ++l) {
+    if (!s[l]) continue;
+    long x, y, z;
+    m.coords(l, x, y, z);
+    c.N1 += 1;
+    auto occ = [&](std::array<int, 3> const &o) {
+      return (long long)s[m.index(x + o[0], y + o[1], z + o[2])];
+    };
+    for (auto const &o : s1) p1 += occ(o);
+    for (auto const &o : s2) p2 += occ(o);
+    for (auto const &o : s3) p3 += occ(o);
+    for (auto const &pr : tp) t += occ(s1[pr.first]) * occ(s1[pr.second]);
+  }
+  c.P1 = p1 / 2;  // every pair seen from both ends
+  c.P2 = p2 / 2;
+  c.P3 = p3 / 2;
+  c.T = t / 3;  // every triangle seen from its three corners
+  return c;
+}
+
+inline void fcc_local(FccModel const &m, std::vector<int8_t> const &s, long l, int &n1, int &n2,
+                      int &n3, int &t) {
+  static const auto s1 = FccModel::shell(2), s2 = FccModel::shell(4), s3 = FccModel::shell(6);
+  static const auto tp = FccModel::triangle_pairs();
+  long x, y, z;
+  m.coords(l, x, y, z);
+  auto occ = [&](std::array<int, 3> const &o) {
+    return (int)s[m.index(x + o[0], y + o[1], z + o[2])];
+  };
+  n1 = n2 = n3 = t = 0;
+  for (auto const &o : s1) n1 += occ(o);
+  for (auto const &o : s2) n2 += occ(o);
+  for (auto const &o : s3) n3 += occ(o);
+  for (auto const &pr : tp) t += occ(s1[pr.first]) * occ(s1[pr.second]);
+}
+
+inline double fcc_energy(FccModel const &m, FccCounts const &c) {
+  const double *v = m.eci;
+  return v[0] * (double)m.n_sites() + v[1] * (double)c.N1 + v[2] / 6.0 * (double)c.P1 +
+         v[3] / 3.0 * (double)c.P2 + v[4] / 12.0 * (double)c.P3 + v[5] / 8.0 * (double)c.T;
+}
+
+// Sequential random-site SGC Metropolis on the FCC model: the reference's
+// loop (basic_occupation_metropolis.hh:381-403: propose a random site,
+// dE of the SGC potential, metropolis_acceptance, apply) with the FCC
+// energy, recording the integer counts after every pass.
+inline void fcc_sequential_run(FccModel const &m, std::vector<int8_t> &s, double temperature,
+                               double mu, std::mt19937_64 &eng, long n_passes,
+                               std::vector<FccCounts> &out) {
+  RandomNumberGenerator rng(std::shared_ptr<std::mt19937_64>(&eng, [](std::mt19937_64 *) {}));
+  const double beta = 1.0 / (KB * temperature);
+  const long N = m.n_sites();
+  const double *v = m.eci;
+  FccCounts c = fcc_counts(m, s);
+  for (long p = 0; p < n_passes; ++p) {
+    for (long step = 0; step < N; ++step) {
+      long l = rng.random_int(N - 1);
+      int n1, n2, n3, t;
+      fcc_local(m, s, l, n1, n2, n3, t);
+      const int ds = 1 - 2 * s[l];
+      const double dE = ds * (v[1] + v[2] / 6.0 * n1 + v[3] / 3.0 * n2 + v[4] / 12.0 * n3 +
+                              v[5] / 8.0 * t) -
+                        mu * ds;
+      if (metropolis_acceptance(dE, beta, rng)) {
+        s[l] = static_cast<int8_t>(1 - s[l]);
+        c.N1 += ds;
+        c.P1 += ds * n1;
+        c.P2 += ds * n2;
+        c.P3 += ds * n3;
+        c.T += ds * t;
+      }
+    }
+    out.push_back(c);
   }
 }
 

@@ -103,6 +103,13 @@ def _declare(L):
     L.orc_lattice_enumerate.argtypes = [C.c_int64, C.POINTER(C.c_int64), C.c_double,
                                         C.c_double, C.c_double, C.c_double,
                                         C.POINTER(C.c_double)]
+    P64, P8, PD = C.POINTER(C.c_int64), C.POINTER(C.c_int8), C.POINTER(C.c_double)
+    L.orc_fcc_counts.argtypes = [P64, P8, P64]
+    L.orc_fcc_local.argtypes = [P64, P8, C.c_int64, P64]
+    L.orc_fcc_shell_sizes.argtypes = [P64]
+    L.orc_fcc_energy.argtypes = [P64, PD, P8]
+    L.orc_fcc_energy.restype = C.c_double
+    L.orc_fcc_run.argtypes = [P64, PD, C.c_double, C.c_double, C.c_uint64, C.c_int64, P8, P64]
 
 
 KB = None
@@ -244,3 +251,61 @@ def lattice_enumerate(ndim, L, J1, J2, T, mu):
     if rc:
         raise ValueError("lattice too large to enumerate")
     return dict(E=out[0], E2=out[1], Nx=out[2], Nx2=out[3], ENx=out[4], Ef=out[5])
+
+
+# ---- binary FCC cluster expansion ------------------------------------------------
+FCC_FIELDS = ("N1", "P1", "P2", "P3", "T")
+
+
+def fcc_counts(L, occ):
+    Ls = np.array(L, dtype=np.int64)
+    o = np.ascontiguousarray(occ, dtype=np.int8)
+    out = np.zeros(5, dtype=np.int64)
+    lib().orc_fcc_counts(_p(Ls, C.c_int64), _p(o, C.c_int8), _p(out, C.c_int64))
+    return dict(zip(FCC_FIELDS, (int(v) for v in out)))
+
+
+def fcc_local(L, occ, l):
+    Ls = np.array(L, dtype=np.int64)
+    o = np.ascontiguousarray(occ, dtype=np.int8)
+    out = np.zeros(4, dtype=np.int64)
+    lib().orc_fcc_local(_p(Ls, C.c_int64), _p(o, C.c_int8), l, _p(out, C.c_int64))
+    return tuple(int(v) for v in out)
+
+
+def fcc_shell_sizes():
+    out = np.zeros(4, dtype=np.int64)
+    lib().orc_fcc_shell_sizes(_p(out, C.c_int64))
+    return tuple(int(v) for v in out)
+
+
+def fcc_energy(L, eci, occ):
+    Ls = np.array(L, dtype=np.int64)
+    e = np.ascontiguousarray(eci, dtype=np.float64)
+    o = np.ascontiguousarray(occ, dtype=np.int8)
+    return lib().orc_fcc_energy(_p(Ls, C.c_int64), _p(e, C.c_double), _p(o, C.c_int8))
+
+
+def fcc_run(L, eci, T, mu, seed, n_passes, occ):
+    """Sequential random-site SGC Metropolis; returns (final occ, counts[n_passes, 5])."""
+    Ls = np.array(L, dtype=np.int64)
+    e = np.ascontiguousarray(eci, dtype=np.float64)
+    o = np.ascontiguousarray(occ, dtype=np.int8).copy()
+    out = np.zeros((n_passes, 5), dtype=np.int64)
+    lib().orc_fcc_run(_p(Ls, C.c_int64), _p(e, C.c_double), T, mu, seed, n_passes,
+                      _p(o, C.c_int8), _p(out, C.c_int64))
+    return o, out
+
+
+def fcc_coords(L):
+    """Doubled coordinates (x, y, z) of every site in the linear order
+    l = h + L0*(y + 2*L1*z), x = ((y+z)&1) + 2h."""
+    L0, L1, L2 = L
+    l = np.arange(4 * L0 * L1 * L2)
+    h = l % L0
+    line = l // L0
+    y = line % (2 * L1)
+    z = line // (2 * L1)
+    x = ((y + z) & 1) + 2 * h
+    return x, y, z
+

@@ -52,3 +52,32 @@ def test_hbm_budget_admission():
     srv.close()
     assert len(res) == len(reqs)
     assert 0 < stats["max_hbm_in_use"] <= budget
+
+
+FCC_CLASSES = ((6, 2, True), (10, 2, False))
+
+
+def test_ising_fcc_mix_equals_alone():
+    """Ising and FCC requests served together; each result equals its run alone."""
+    reqs = []
+    for i in range(10):
+        fcc = i % 2 == 1
+        edge = (4 + i // 2) if fcc else 16 + 12 * i
+        reqs.append(serving.Request(rid=i, edge=edge, temperature=700.0 + 150.0 * i,
+                                    exchange_potential=-0.3 + 0.07 * i, seed=7700 + i,
+                                    arrival=0.005 * i, max_passes=1500, precision=3e-3,
+                                    model="fcc" if fcc else "ising"))
+    print("case: serving ising+fcc mix", flush=True)
+    srv = serving.Server(classes=CLASSES, fcc_classes=FCC_CLASSES, chunk=100, check_period=500)
+    mixed, stats = srv.serve(reqs)
+    srv.close()
+    assert len(mixed) == len(reqs)
+    for req in reqs:
+        alone_srv = serving.Server(classes=CLASSES, fcc_classes=FCC_CLASSES, chunk=100,
+                                   check_period=500)
+        alone, _ = alone_srv.serve([serving.Request(**{**req.__dict__, "arrival": 0.0})])
+        alone_srv.close()
+        a, m = alone[req.rid], mixed[req.rid]
+        key = lambda r: (r.passes, r.n_samples, r.converged, r.n_equilibration, r.mean_x,
+                         r.precision_x, r.lattice_crc)
+        assert key(a) == key(m), (req, a, m)
