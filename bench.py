@@ -12,6 +12,10 @@ Also measured in the same run and reported beside `value`:
                 LDS-resident launch vs the plain one-chain launch run condition
                 by condition; and the 4096^2 / 256^3 J1+J2 16-temperature
                 families streamed from HBM (batched and one chain);
+  fcc        -- the binary FCC cluster expansion: 256 uneven chains (16^3-32^3
+                conventional cells) in one autotuned launch vs one chain alone;
+  served     -- a Poisson(50/s) stream of 50/50 Ising/FCC run requests through
+                the continuous-batching server: throughput and p50/p99 latency;
   cpu_baseline -- the oracle (a restatement of the reference loop with its
                 per-step completion check and per-pass sampling) timed on the
                 host cores on a bounded sample of the default workload.
@@ -109,18 +113,48 @@ def bench_checker_family(shape, lattice, passes, chains=16):
                 hbm_frac=2.0 * a / k / 1e9 / HBM_PEAK_GBS)
 
 
+def bench_fcc(passes):
+    """Binary FCC cluster expansion: 256 chains of L = 16 + (i mod 17) conventional
+    cells (16384-131072 sites), T ~ U(300, 2000) K, mu ~ U(-1, 1) eV, default ECIs,
+    all in one autotuned launch; plain = one 32^3 chain alone."""
+    import casmcode_monte_amd as cm
+    from casmcode_monte_amd import fcc
+    conds = fcc.batch_conditions()
+    r = cm.Run(conds, mode=cm.MODE_CHECKERBOARD, lattice=cm.LATTICE_FCC, eci=fcc.DEFAULT_ECI)
+    r.run(2)
+    sched = r.autotune(8)
+    w, k, a = timed_run(r, passes)
+    r.close()
+    p = cm.Run([conds[16]], mode=cm.MODE_CHECKERBOARD, lattice=cm.LATTICE_FCC,
+               eci=fcc.DEFAULT_ECI)
+    p.run(2)
+    pp = max(passes // 10, 10)
+    pw, pk, pa = timed_run(p, pp)
+    p.close()
+    lds_gbs = 44.0 * a / k / 1e9  # own read + 42 neighbour reads + write per attempt
+    return dict(chains=len(conds), passes=passes, wall_s=w, kernel_s=k, attempts=a,
+                attempts_per_s=a / w, lds_block=sched["lds_block"],
+                plain_attempts_per_s=pa / pw, plain_passes=pp,
+                batched_over_plain=(a / w) / (pa / pw),
+                roofline={"bound": "lds", "achieved": lds_gbs, "peak": LDS_PEAK_GBS,
+                          "unit": "GB/s", "frac": lds_gbs / LDS_PEAK_GBS,
+                          "bytes_per_attempt": 44})
+
+
 def bench_served(n_requests):
-    """Served scan: a Poisson (50/s) stream of 2D Ising run requests, edges
-    log-uniform in [16, 512] (rounded to even), T in 300-3000 K, mu in -1..1 eV,
-    each stopping at <x> precision 1e-3 or 20000 passes."""
+    """Served scan: a Poisson (50/s) stream of run requests, half 2D Ising (edges
+    log-uniform in [16, 512], rounded to even) and half FCC cluster expansion
+    (edges log-uniform in [8, 48] conventional cells), T in 300-3000 K, mu in
+    -1..1 eV, each stopping at <x> precision 1e-3 or 20000 passes."""
     from casmcode_monte_amd import serving
-    reqs = serving.poisson_trace(n_requests, rate=50.0)
-    srv = serving.Server()
+    reqs = serving.poisson_trace(n_requests, rate=50.0, fcc_fraction=0.5)
+    srv = serving.Server(fcc_classes=serving.DEFAULT_FCC_CLASSES)
     res, st = srv.serve(reqs)
     srv.close()
     st["converged"] = sum(r.converged for r in res.values())
-    st["trace"] = (f"first {n_requests} requests of the Poisson(50/s) trace, seed 2024; "
-                   "2D Ising J1=0.1 eV only (no FCC requests)")
+    st["fcc_requests"] = sum(r.model == "fcc" for r in reqs)
+    st["trace"] = (f"first {n_requests} requests of the Poisson(50/s) trace, seed 2024, "
+                   "50/50 2D Ising (J1=0.1 eV) / FCC cluster expansion (default ECIs)")
     return st
 
 
@@ -219,6 +253,10 @@ def main():
         fam2_plain = bench_checker_family((4096, 4096, 1), cm.LATTICE_SQUARE, 50, chains=1)
         fam3 = bench_checker_family((256, 256, 256), cm.LATTICE_SC, 20)
         print("[bench] checker 256^3 done", file=sys.stderr, flush=True)
+        fc = bench_fcc(200)
+        print(f"[bench] fcc batch done: {fc['attempts_per_s']:.3e} attempts/s", file=sys.stderr,
+              flush=True)
+        line["fcc_cluster_expansion_256_chains"] = fc
         line["checkerboard"] = {
             "scan_128x128_64_conditions": scan,
             "square_4096x4096_J1J2_16T": fam2,

@@ -550,19 +550,24 @@ int fill_fcc_params(mcg_handle *h, int c, CheckerParams &P, double *tab) {
   return MCG_OK;
 }
 
+// LDS bytes of an FCC lattice: 4*L1*L2 lines of fcc_lds_stride(L0) bytes
+int64_t fcc_lds_bytes(CheckerParams const &P) {
+  return ((int64_t)4 * P.L1 * P.L2 * fcc_lds_stride(P.L0) + 15) & ~15LL;
+}
+
 int build_fcc(mcg_handle *h) {
   h->ncolor = 64;
   h->checker_block = 1024;
   h->cp.resize(h->n_chains);
   std::vector<double> tabs((size_t)FCC_NTAB * h->n_chains);
-  int64_t max_n = 0;
+  int64_t max_lds = 0;
   for (int c = 0; c < h->n_chains; ++c) {
     int rc = fill_fcc_params(h, c, h->cp[c], tabs.data() + (size_t)FCC_NTAB * c);
     if (rc != MCG_OK) return rc;
-    max_n = std::max(max_n, h->cp[c].n);
+    max_lds = std::max(max_lds, fcc_lds_bytes(h->cp[c]));
   }
-  h->checker_lds = max_n <= FCC_LDS_MAX && !(h->options & MCG_OPT_FORCE_STREAMING);
-  h->checker_lds_bytes = h->checker_lds ? (int)((max_n + 15) & ~15) : 0;
+  h->checker_lds = max_lds <= FCC_LDS_MAX && !(h->options & MCG_OPT_FORCE_STREAMING);
+  h->checker_lds_bytes = h->checker_lds ? (int)max_lds : 0;
   HIP_TRY(fcc_set_max_lds((int)FCC_LDS_MAX));
   const std::vector<uint8_t> tri = fcc_tri_table();
   HIP_TRY(hipMalloc(&h->d_cp, sizeof(CheckerParams) * h->n_chains));
@@ -986,7 +991,7 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
       h->chains[chain] = old;
       return rc;
     }
-    if (h->checker_lds && P.n > h->checker_lds_bytes) {
+    if (h->checker_lds && fcc_lds_bytes(P) > h->checker_lds_bytes) {
       h->chains[chain] = old;
       return fail(MCG_ERR_CAPACITY, "shape needs more LDS than the handle was created with");
     }

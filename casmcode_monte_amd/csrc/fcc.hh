@@ -45,4 +45,11 @@ __host__ __device__ inline int fcc_ring_colour(int v, int n) {
   return v == n - 2 ? 1 : 2;
 }
 
+// LDS line stride of the FCC kernel: L0 sites plus two ghost bytes, plus one
+// more when that is a multiple of 4 (lines 4 apart then fall in distinct banks).
+__host__ __device__ inline int fcc_lds_stride(int L0) {
+  const int s = L0 + 2;
+  return (s & 3) == 0 ? s + 1 : s;
+}
+
 }  // namespace mcg

@@ -26,7 +26,9 @@
 // a chain alone or in a batch give identical trajectories.
 //
 // Kernel: one workgroup per chain runs n_passes; the lattice lives in LDS when
-// it fits (<= 142 KB) and is read and written in place in HBM otherwise.
+// it fits (<= 142 KB, lines padded with ghost copies of their end sites) and
+// is read and written in place in HBM otherwise.  A thread gathers the
+// neighbourhoods of its unit's four sites before flipping any of them.
 // Counts (N1, P1, P2, P3, T) are exact integers updated from per-flip deltas.
 #include "fcc.hh"
 #include "internal.hh"
@@ -36,10 +38,10 @@ namespace mcg {
 namespace {
 
 struct FccLists {
-  uint8_t y[4][FCC_LIST];      // y values of each colour
-  uint8_t z[4][FCC_LIST];      // z values of each colour
-  uint8_t xh[4][2][FCC_LIST];  // h = x >> 1 of each colour and parity of x
-  int ny[4], nz[4], nx[4][2];
+  uint8_t y[4][2][FCC_LIST];   // y values of each colour, by parity
+  uint8_t z[4][2][FCC_LIST];   // z values of each colour, by parity
+  uint8_t xh[4][2][FCC_LIST];  // h = x >> 1 of each colour, by parity of x
+  int ny[4][2], nz[4][2], nx[4][2];
 };
 
 __device__ __forceinline__ int fcc_wave_sum(int v) {
@@ -54,72 +56,136 @@ struct FccDelta {
   int N1 = 0, P1 = 0, P2 = 0, P3 = 0, T = 0, acc = 0;
 };
 
-// Update one work unit (up to 4 sites of class col on line (y, z)).
-__device__ __forceinline__ void fcc_unit(uint8_t *__restrict__ lat, const double *__restrict__ tab,
+// Work units of one class: four blocks (py, pz) of lines whose y and z have
+// parities py, pz, each line holding ceil(nx[cx][py^pz] / 4) units.
+struct FccClass {
+  int cx, cy, cz, col;
+  int start[5];  // unit prefix sums of the four blocks
+  int nq[4];     // units per line in each block
+};
+
+__device__ __forceinline__ FccClass fcc_class(const FccLists &Ls, int col) {
+  FccClass k;
+  k.col = col;
+  k.cx = col & 3;
+  k.cy = (col >> 2) & 3;
+  k.cz = col >> 4;
+  k.start[0] = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int py = b >> 1, pz = b & 1;
+    k.nq[b] = (Ls.nx[k.cx][py ^ pz] + 3) >> 2;
+    k.start[b + 1] = k.start[b] + Ls.ny[k.cy][py] * Ls.nz[k.cz][pz] * k.nq[b];
+  }
+  return k;
+}
+
+// Lattice access.  LDS: every line is stored as [site L0-1, sites 0..L0-1,
+// site 0] (ghost copies of the two ends) with a stride that spreads the lines
+// of one colour over the banks, so a neighbour at h + dh (|dh| <= 1) is a
+// constant offset.  Global memory: plain lines of L0 sites, h wrapped.
+template <bool LDS>
+struct FccLat {
+  uint8_t *p;
+  int stride, L0;
+  __device__ __forceinline__ int base(int line) const { return line * stride + (LDS ? 1 : 0); }
+  __device__ __forceinline__ int at(int b, int h, int dh) const {
+    if (LDS) return p[b + h + dh];
+    int x = h + dh;
+    x = x < 0 ? x + L0 : (x >= L0 ? x - L0 : x);
+    return p[b + x];
+  }
+  __device__ __forceinline__ void flip(int b, int h, uint8_t v) const {
+    p[b + h] = v;
+    if (LDS) {
+      if (h == 0) p[b + L0] = v;       // right ghost of the line
+      if (h == L0 - 1) p[b - 1] = v;   // left ghost of the line
+    }
+  }
+};
+
+// Update one work unit: up to 4 sites of class k on one line.  The sites do
+// not interact, so the neighbourhoods of a group of FCC_GATHER sites are all
+// read before any of them flips (independent loads in flight).
+constexpr int FCC_GATHER = 1;
+template <bool LDS>
+__device__ __forceinline__ void fcc_unit(const FccLat<LDS> &lat, const double *__restrict__ tab,
                                          const uint8_t *__restrict__ tri, const FccLists &Ls,
-                                         int L0, int L1, int L2, int cx, int cy, int cz, int col,
-                                         int maxq, int ny, int u, int64_t pass, uint32_t key0,
-                                         uint32_t key1, FccDelta &d) {
-  const int q = u % maxq;
-  const int li = u / maxq;
-  const int yi = li % ny, zi = li / ny;
-  const int y = Ls.y[cy][yi], z = Ls.z[cz][zi];
-  const int p = (y + z) & 1;
-  const int cnt = Ls.nx[cx][p];
+                                         int L1, int L2, const FccClass &k, int u, int64_t pass,
+                                         uint32_t key0, uint32_t key1, FccDelta &d) {
+  const int b = (u >= k.start[1]) + (u >= k.start[2]) + (u >= k.start[3]);
+  const int py = b >> 1, pz = b & 1, p = py ^ pz;
+  const int r0 = u - (b == 0 ? 0 : b == 1 ? k.start[1] : b == 2 ? k.start[2] : k.start[3]);
+  const int nq = b == 0 ? k.nq[0] : b == 1 ? k.nq[1] : b == 2 ? k.nq[2] : k.nq[3];
+  const int q = r0 % nq;
+  const int li = r0 / nq;
+  const int nyb = Ls.ny[k.cy][py];
+  const int y = Ls.y[k.cy][py][li % nyb], z = Ls.z[k.cz][pz][li / nyb];
+  const int cnt = Ls.nx[k.cx][p];
   const int s0 = q << 2;
-  if (s0 >= cnt) return;
   const int ry = 2 * L1, rz = 2 * L2;
   int ys[5], zs[5];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    ys[k] = wrap_ring(y + k - 2, ry);
-    zs[k] = wrap_ring(z + k - 2, rz) * ry;
+  for (int j = 0; j < 5; ++j) {
+    ys[j] = wrap_ring(y + j - 2, ry);
+    zs[j] = wrap_ring(z + j - 2, rz) * ry;
   }
   const int line = y + ry * z;
   const u32x4 r = philox4x32_10((uint32_t)(line * 32 + q), (uint32_t)pass,
-                                (uint32_t)(pass >> 32), (uint32_t)col, key0, key1);
+                                (uint32_t)(pass >> 32), (uint32_t)k.col, key0, key1);
+  const int bl = lat.base(line);
+  // sites in groups of FCC_GATHER: gather their neighbourhoods, then decide
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int s = s0 + v;
-    if (s >= cnt) break;
-    const int h = Ls.xh[cx][p][s];
-    const int hm = h == 0 ? L0 - 1 : h - 1;
-    const int hp = h == L0 - 1 ? 0 : h + 1;
-    const int hA = p ? h : hm;  // x - 1 on a line of the other parity
-    const int hB = p ? hp : h;  // x + 1 on a line of the other parity
-#define MCG_FCC_AT(dx, dy, dz)                                                       \
-  ((int)lat[(ys[(dy) + 2] + zs[(dz) + 2]) * L0 +                                     \
-            ((((dy) + (dz)) & 1) == 0 ? ((dx) == -2 ? hm : ((dx) == 0 ? h : hp))     \
-                                      : ((dx) == -1 ? hA : hB))])
-    uint32_t mask = 0;
-#define MCG_FCC_BIT(k, dx, dy, dz) mask |= (uint32_t)MCG_FCC_AT(dx, dy, dz) << (k);
-    MCG_FCC_NN(MCG_FCC_BIT)
-    int n2 = 0;
-#define MCG_FCC_ADD2(k, dx, dy, dz) n2 += MCG_FCC_AT(dx, dy, dz);
-    MCG_FCC_N2(MCG_FCC_ADD2)
-    int n3 = 0;
-#define MCG_FCC_ADD3(k, dx, dy, dz) n3 += MCG_FCC_AT(dx, dy, dz);
-    MCG_FCC_N3(MCG_FCC_ADD3)
+  for (int g0 = 0; g0 < 4; g0 += FCC_GATHER) {
+    uint32_t mask[FCC_GATHER];
+    int n2[FCC_GATHER], n3[FCC_GATHER], sig[FCC_GATHER], hh[FCC_GATHER];
+#pragma unroll
+    for (int j = 0; j < FCC_GATHER; ++j) {
+      const int v = g0 + j;
+      const int s = s0 + v < cnt ? s0 + v : cnt - 1;  // tail slots repeat the last site
+      const int h = Ls.xh[k.cx][p][s];
+      const int hq = h + p;  // anchor of the odd-(dy+dz) offsets: x -+ 1 lands at hq-1, hq
+      hh[j] = h;
+#define MCG_FCC_AT(dx, dy, dz)                                                           \
+  lat.at(lat.base(ys[(dy) + 2] + zs[(dz) + 2]), ((((dy) + (dz)) & 1) == 0) ? h : hq,     \
+         ((((dy) + (dz)) & 1) == 0) ? (dx) / 2 : ((dx) == -1 ? -1 : 0))
+      uint32_t m = 0;
+#define MCG_FCC_BIT(kk, dx, dy, dz) m |= (uint32_t)MCG_FCC_AT(dx, dy, dz) << (kk);
+      MCG_FCC_NN(MCG_FCC_BIT)
+      int a2 = 0;
+#define MCG_FCC_ADD2(kk, dx, dy, dz) a2 += MCG_FCC_AT(dx, dy, dz);
+      MCG_FCC_N2(MCG_FCC_ADD2)
+      int a3 = 0;
+#define MCG_FCC_ADD3(kk, dx, dy, dz) a3 += MCG_FCC_AT(dx, dy, dz);
+      MCG_FCC_N3(MCG_FCC_ADD3)
 #undef MCG_FCC_ADD3
 #undef MCG_FCC_ADD2
 #undef MCG_FCC_BIT
 #undef MCG_FCC_AT
-    const int n1 = __popc(mask);
-    const int t = tri[mask];
-    const int site = line * L0 + h;
-    const int sig = lat[site];
-    const double pr = tab[sig * FCC_TAB_A + n1 * (FCC_Z2 + 1) + n2] *
-                      tab[2 * FCC_TAB_A + sig * FCC_TAB_B + n3 * (FCC_ZT + 1) + t];
-    const double uu = ((double)r.v[v] + 0.5) * 0x1p-32;
-    if (uu < pr) {
-      lat[site] = (uint8_t)(1 - sig);
-      const int ds = 1 - 2 * sig;
-      d.N1 += ds;
-      d.P1 += ds * n1;
-      d.P2 += ds * n2;
-      d.P3 += ds * n3;
-      d.T += ds * t;
-      d.acc += 1;
+      mask[j] = m;
+      n2[j] = a2;
+      n3[j] = a3;
+      sig[j] = lat.at(bl, h, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < FCC_GATHER; ++j) {
+      const int v = g0 + j;
+      const int n1 = __popc(mask[j]);
+      const int t = tri[mask[j]];
+      const int sg = sig[j];
+      const double pr = tab[sg * FCC_TAB_A + n1 * (FCC_Z2 + 1) + n2[j]] *
+                        tab[2 * FCC_TAB_A + sg * FCC_TAB_B + n3[j] * (FCC_ZT + 1) + t];
+      const double uu = ((double)r.v[v] + 0.5) * 0x1p-32;
+      if (s0 + v < cnt && uu < pr) {
+        lat.flip(bl, hh[j], (uint8_t)(1 - sg));
+        const int ds = 1 - 2 * sg;
+        d.N1 += ds;
+        d.P1 += ds * n1;
+        d.P2 += ds * n2[j];
+        d.P3 += ds * n3[j];
+        d.T += ds * t;
+        d.acc += 1;
+      }
     }
   }
 }
@@ -145,34 +211,51 @@ __global__ __launch_bounds__(1024) void fcc_kernel(
   for (int i = t; i < 4096; i += nt) tri[i] = tri_g[i];
   const int L0 = P.L0, L1 = P.L1, L2 = P.L2;
   if (t < 4) {  // thread a lists the coordinates of colour a on the three rings
-    int k = 0;
-    for (int v = 0; v < 2 * L1; ++v)
-      if (fcc_ring_colour(v, 2 * L1) == t) Ls.y[t][k++] = (uint8_t)v;
-    Ls.ny[t] = k;
-    k = 0;
-    for (int v = 0; v < 2 * L2; ++v)
-      if (fcc_ring_colour(v, 2 * L2) == t) Ls.z[t][k++] = (uint8_t)v;
-    Ls.nz[t] = k;
     int k0 = 0, k1 = 0;
+    for (int v = 0; v < 2 * L1; ++v)
+      if (fcc_ring_colour(v, 2 * L1) == t) {
+        if (v & 1) Ls.y[t][1][k1++] = (uint8_t)v;
+        else Ls.y[t][0][k0++] = (uint8_t)v;
+      }
+    Ls.ny[t][0] = k0;
+    Ls.ny[t][1] = k1;
+    k0 = k1 = 0;
+    for (int v = 0; v < 2 * L2; ++v)
+      if (fcc_ring_colour(v, 2 * L2) == t) {
+        if (v & 1) Ls.z[t][1][k1++] = (uint8_t)v;
+        else Ls.z[t][0][k0++] = (uint8_t)v;
+      }
+    Ls.nz[t][0] = k0;
+    Ls.nz[t][1] = k1;
+    k0 = k1 = 0;
     for (int v = 0; v < 2 * L0; ++v)
       if (fcc_ring_colour(v, 2 * L0) == t) {
-        if (v & 1)
-          Ls.xh[t][1][k1++] = (uint8_t)(v >> 1);
-        else
-          Ls.xh[t][0][k0++] = (uint8_t)(v >> 1);
+        if (v & 1) Ls.xh[t][1][k1++] = (uint8_t)(v >> 1);
+        else Ls.xh[t][0][k0++] = (uint8_t)(v >> 1);
       }
     Ls.nx[t][0] = k0;
     Ls.nx[t][1] = k1;
   }
-  const int64_t n16 = (P.n + 15) >> 4;  // lattices are padded to 256 bytes
-  uint8_t *lat;
+  // In LDS every line takes S = L0 + 2 bytes (+1 when that is a multiple of 4)
+  // with ghost copies of its end sites: the lines a wavefront reads at once
+  // are 4 apart (one colour), and this stride puts them in distinct banks.
+  const int n_lines = 4 * L1 * L2;
+  FccLat<LAT_IN_LDS> lat;
+  lat.L0 = L0;
   if (LAT_IN_LDS) {
-    lat = reinterpret_cast<uint8_t *>(smem);
-    const uint4 *src = reinterpret_cast<const uint4 *>(g_lat);
-    uint4 *dst = reinterpret_cast<uint4 *>(lat);
-    for (int64_t i = t; i < n16; i += nt) dst[i] = src[i];
+    const int S = fcc_lds_stride(L0);
+    lat.p = reinterpret_cast<uint8_t *>(smem);
+    lat.stride = S;
+    for (int ln = t; ln < n_lines; ln += nt) {
+      uint8_t *dst = lat.p + ln * S;
+      const uint8_t *src = g_lat + ln * L0;
+      for (int h = 0; h < L0; ++h) dst[1 + h] = src[h];
+      dst[0] = src[L0 - 1];
+      dst[L0 + 1] = src[0];
+    }
   } else {
-    lat = g_lat;
+    lat.p = g_lat;
+    lat.stride = L0;
   }
   __syncthreads();
   int64_t pass = St->n_pass;
@@ -182,15 +265,11 @@ __global__ __launch_bounds__(1024) void fcc_kernel(
   for (int64_t it = 0; it < n_passes; ++it) {
     FccDelta d;
     for (int col = 0; col < 64; ++col) {
-      const int cx = col & 3, cy = (col >> 2) & 3, cz = col >> 4;
-      const int ny = Ls.ny[cy], nz = Ls.nz[cz];
-      const int mx = Ls.nx[cx][0] > Ls.nx[cx][1] ? Ls.nx[cx][0] : Ls.nx[cx][1];
-      const int maxq = (mx + 3) >> 2;
-      const int units = ny * nz * maxq;
+      const FccClass k = fcc_class(Ls, col);
+      const int units = k.start[4];
       if (units == 0) continue;  // the same for every thread
       for (int u = t; u < units; u += nt)
-        fcc_unit(lat, tab, tri, Ls, L0, L1, L2, cx, cy, cz, col, maxq, ny, u, pass, P.key0,
-                 P.key1, d);
+        fcc_unit<LAT_IN_LDS>(lat, tab, tri, Ls, L1, L2, k, u, pass, P.key0, P.key1, d);
       __syncthreads();
     }
     const int v0 = fcc_wave_sum(d.N1), v1 = fcc_wave_sum(d.P1), v2 = fcc_wave_sum(d.P2);
@@ -230,11 +309,9 @@ __global__ __launch_bounds__(1024) void fcc_kernel(
       ++n_samples;
     }
   }
-  if (LAT_IN_LDS) {
-    const uint4 *src = reinterpret_cast<const uint4 *>(lat);
-    uint4 *dst = reinterpret_cast<uint4 *>(g_lat);
-    for (int64_t i = t; i < n16; i += nt) dst[i] = src[i];
-  }
+  if (LAT_IN_LDS)
+    for (int ln = t; ln < n_lines; ln += nt)
+      for (int h = 0; h < L0; ++h) g_lat[ln * L0 + h] = lat.p[ln * lat.stride + 1 + h];
   if (t == 0) {
     St->n_pass = pass;
     St->M = N1;

@@ -83,8 +83,8 @@ class Result:
 # (max edge, slots, LDS-resident) per size class
 DEFAULT_CLASSES = ((64, 96, True), (128, 64, True), (256, 48, True), (362, 32, True),
                    (512, 16, False))
-# FCC: edges in conventional cells (4 L^3 sites; LDS-resident up to L = 33)
-DEFAULT_FCC_CLASSES = ((12, 64, True), (20, 48, True), (33, 32, True), (48, 12, False))
+# FCC: edges in conventional cells (4 L^3 sites; LDS-resident up to L = 32)
+DEFAULT_FCC_CLASSES = ((12, 64, True), (20, 48, True), (32, 32, True), (48, 12, False))
 
 
 class _Pool:
