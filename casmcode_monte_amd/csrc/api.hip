@@ -275,6 +275,12 @@ struct mcg_handle {
   int ncolor = 2;
   int units_per_thread = 2;
   int graph_passes = 8;
+  // 2D band path: lines per band (-1 = chosen from the sizes, 0 = off) and the
+  // (chain, first line, lines) list
+  int band_lines = -1;
+  int32_t *d_band = nullptr;
+  int n_bands = 0;
+  int band_ring_bytes = 0;
   double tuned_ms = 0.0;
   // hipGraph of graph_passes passes of the HBM path (rebuilt when the work
   // list, the series buffer or the schedule change)
@@ -290,7 +296,7 @@ struct mcg_handle {
     if (stream) hipStreamSynchronize(stream);
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     void *ptrs[] = {d_lat, d_active, d_series, d_stats, d_eq, d_xp, d_xs, d_cp, d_cs, d_tab,
-                    d_work, d_ftab, d_tri};
+                    d_work, d_ftab, d_tri, d_band};
     for (void *p : ptrs)
       if (p) hipFree(p);
     if (ev0) hipEventDestroy(ev0);
@@ -417,13 +423,8 @@ int build_exact(mcg_handle *h) {
   return MCG_OK;
 }
 
-int checker_ntab(mcg_handle *h) {
-  const bool nnn = h->J2 != 0.0;
-  const int ndim = h->lattice == MCG_LATTICE_SC ? 3 : 2;
-  const int z1 = ndim == 3 ? 6 : 4;
-  const int z2 = nnn ? (ndim == 3 ? 12 : 4) : 0;
-  return 2 * (z1 + 1) * (z2 + 1);
-}
+// Entries of a chain's acceptance table (indexed by checker_class).
+int checker_ntab(mcg_handle *) { return CHECKER_NTAB; }
 
 // Parameters and acceptance table (ntab entries at tab) of one CHECKERBOARD chain.
 int fill_checker_params(mcg_handle *h, int c, CheckerParams &P, uint32_t *tab) {
@@ -431,7 +432,8 @@ int fill_checker_params(mcg_handle *h, int c, CheckerParams &P, uint32_t *tab) {
   const int ndim = h->lattice == MCG_LATTICE_SC ? 3 : 2;
   const int z1 = ndim == 3 ? 6 : 4;
   const int z2 = nnn ? (ndim == 3 ? 12 : 4) : 0;
-  const int ntab = 2 * (z1 + 1) * (z2 + 1);
+  const int ntab = CHECKER_NTAB;
+  std::fill(tab, tab + ntab, 0u);
   {
     auto const &ch = h->chains[c];
     for (int d = 0; d < ndim; ++d)
@@ -470,7 +472,7 @@ int fill_checker_params(mcg_handle *h, int c, CheckerParams &P, uint32_t *tab) {
             if (v > 4294967295.0) v = 4294967295.0;
             t = (uint32_t)v;
           }
-          tab[(size_t)sidx * (z1 + 1) * (z2 + 1) + a * (z2 + 1) + b] = t;
+          tab[checker_class(sidx, a, b)] = t;
         }
     }
   }
@@ -647,6 +649,47 @@ int rebuild_work(mcg_handle *h) {
     HIP_TRY(copy_sync(h, h->d_work, work.data(), sizeof(int32_t) * work.size(),
                       hipMemcpyHostToDevice));
   }
+  // 2D band path (checker.hip): every active chain square, 16-site chunks,
+  // lines short enough for the 8-line ring
+  if (h->d_band) hipFree(h->d_band);
+  h->d_band = nullptr;
+  h->n_bands = 0;
+  h->band_ring_bytes = 0;
+  bool band_ok = h->lattice == MCG_LATTICE_SQUARE && h->band_lines != 0 && !h->fcc() &&
+                 !(h->options & MCG_OPT_CHECKER_QUAD_PATH);
+  int64_t maxL0 = 0, total_lines = 0;
+  for (int c = 0; c < h->n_chains && band_ok; ++c) {
+    if (!h->active[c]) continue;
+    auto const &P = h->cp[c];
+    if (P.L0 % 32 != 0 || P.L0 > 8192) band_ok = false;
+    maxL0 = std::max<int64_t>(maxL0, P.L0);
+    total_lines += P.L1;
+  }
+  if (band_ok && total_lines > 0) {
+    int target = h->band_lines;
+    if (target < 0) {  // about 2048 bands or more, bands of at most 64 lines
+      target = 64;
+      while (target > 8 && total_lines / target < 2048) target /= 2;
+    }
+    std::vector<int32_t> band;
+    for (int c = 0; c < h->n_chains; ++c) {
+      if (!h->active[c]) continue;
+      const int L1 = h->cp[c].L1;
+      int H = std::max(2, target & ~1);
+      while (H > 2 && L1 % H != 0) H -= 2;  // an even divisor of L1 (L1 is even)
+      for (int j0 = 0; j0 < L1; j0 += H) {
+        band.push_back(c);
+        band.push_back(j0);
+        band.push_back(H);
+      }
+    }
+    h->n_bands = (int)(band.size() / 3);
+    h->band_ring_bytes = (int)(checker_band_ring_lines() * maxL0);
+    HIP_TRY(checker_band_set_max_lds(h->band_ring_bytes));
+    HIP_TRY(hipMalloc(&h->d_band, sizeof(int32_t) * band.size()));
+    HIP_TRY(copy_sync(h, h->d_band, band.data(), sizeof(int32_t) * band.size(),
+                      hipMemcpyHostToDevice));
+  }
   h->work_dirty = false;
   return MCG_OK;
 }
@@ -654,7 +697,10 @@ int rebuild_work(mcg_handle *h) {
 // One pass of the HBM path: one launch per colour over the work list, then
 // the pass-counter/sample-record launch.
 void enqueue_pass(mcg_handle *h) {
-  if (h->n_work > 0)
+  if (h->n_bands > 0)
+    launch_checker_band(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_band, h->n_bands,
+                        h->band_ring_bytes, h->stream);
+  else if (h->n_work > 0)
     for (int color = 0; color < h->ncolor; ++color)
       launch_checker_color(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_work, h->n_work, color,
                            h->units_per_thread, h->stream);
@@ -675,7 +721,7 @@ int build_graph(mcg_handle *h) {
   hipError_t e = checker_graph_add_passes(
       g, h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_work, h->n_work, h->ncolor,
       h->units_per_thread, h->d_series, h->series_cap, h->d_active, h->n_chains,
-      h->sample_period, h->graph_passes);
+      h->sample_period, h->graph_passes, h->d_band, h->n_bands, h->band_ring_bytes);
   if (e == hipSuccess) e = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
@@ -711,7 +757,8 @@ std::map<std::string, mcg_schedule> &tune_cache() {
   return m;
 }
 
-// File format, one schedule per line: key lds_block units_per_thread graph_passes trial_ms
+// File format, one schedule per line:
+//   key lds_block units_per_thread graph_passes trial_ms band_lines
 bool tune_cache_lookup(const std::string &key, mcg_schedule *out) {
   std::lock_guard<std::mutex> lock(tune_mutex());
   auto &m = tune_cache();
@@ -725,12 +772,12 @@ bool tune_cache_lookup(const std::string &key, mcg_schedule *out) {
   FILE *f = std::fopen(path, "r");
   if (!f) return false;
   char k[128];
-  long long b, u, g;
+  long long b, u, g, bl;
   double ms;
   bool found = false;
-  while (std::fscanf(f, "%127s %lld %lld %lld %lf", k, &b, &u, &g, &ms) == 5) {
+  while (std::fscanf(f, "%127s %lld %lld %lld %lf %lld", k, &b, &u, &g, &ms, &bl) == 6) {
     if (key == k) {  // the last line for a key wins
-      *out = mcg_schedule{b, u, g, ms};
+      *out = mcg_schedule{b, u, g, ms, bl};
       found = true;
     }
   }
@@ -745,8 +792,9 @@ void tune_cache_store(const std::string &key, const mcg_schedule &s) {
   const char *path = std::getenv("MCG_TUNE_CACHE");
   if (!path || !*path) return;
   if (FILE *f = std::fopen(path, "a")) {
-    std::fprintf(f, "%s %lld %lld %lld %.6f\n", key.c_str(), (long long)s.lds_block,
-                 (long long)s.units_per_thread, (long long)s.graph_passes, s.trial_ms);
+    std::fprintf(f, "%s %lld %lld %lld %.6f %lld\n", key.c_str(), (long long)s.lds_block,
+                 (long long)s.units_per_thread, (long long)s.graph_passes, s.trial_ms,
+                 (long long)s.band_lines);
     std::fclose(f);
   }
 }
@@ -1249,6 +1297,7 @@ int mcg_get_schedule(mcg_handle *h, mcg_schedule *out) {
   out->lds_block = h->checker_block;
   out->units_per_thread = h->units_per_thread;
   out->graph_passes = h->graph_passes;
+  out->band_lines = h->band_lines;
   out->trial_ms = h->tuned_ms;
   return MCG_OK;
 }
@@ -1261,10 +1310,14 @@ int mcg_set_schedule(mcg_handle *h, const mcg_schedule *s) {
     return fail(MCG_ERR_ARG, "units_per_thread must be 1, 2 or 4");
   if (s->graph_passes < 1 || s->graph_passes > 256)
     return fail(MCG_ERR_ARG, "graph_passes must be in [1, 256]");
+  if (s->band_lines != -1 && (s->band_lines < 0 || s->band_lines > 4096 || (s->band_lines & 1)))
+    return fail(MCG_ERR_ARG, "band_lines must be -1 (automatic), 0 (off) or even <= 4096");
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->checker_block = (int)s->lds_block;
   h->units_per_thread = (int)s->units_per_thread;
   h->graph_passes = (int)s->graph_passes;
+  if (h->band_lines != (int)s->band_lines) h->work_dirty = true;
+  h->band_lines = (int)s->band_lines;
   h->graph_dirty = true;
   return MCG_OK;
 }
@@ -1296,10 +1349,15 @@ int mcg_autotune(mcg_handle *h, int64_t trial_passes, mcg_schedule *chosen) {
     };
     std::vector<mcg_schedule> cands;
     if (h->checker_lds || h->fcc()) {
-      for (int64_t b : {256, 512, 1024}) cands.push_back({b, h->units_per_thread, 1, 0.0});
+      for (int64_t b : {256, 512, 1024})
+        cands.push_back({b, h->units_per_thread, 1, 0.0, h->band_lines});
     } else {
       for (int64_t u : {1, 2, 4})
-        for (int64_t g : {1, 8}) cands.push_back({h->checker_block, u, g, 0.0});
+        for (int64_t g : {1, 8}) cands.push_back({h->checker_block, u, g, 0.0, 0});
+      if (h->lattice == MCG_LATTICE_SQUARE)
+        for (int64_t bl : {8, 16, 32, 64})
+          for (int64_t g : {1, 8})
+            cands.push_back({h->checker_block, h->units_per_thread, g, 0.0, bl});
     }
     double best_ms = 1e300;
     int rc = MCG_OK;

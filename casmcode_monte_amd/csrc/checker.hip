@@ -24,14 +24,17 @@
 //     from 16-byte loads, the in-line shifts with v_alignbyte;
 //   quad path (any even L0): a thread owns 4 consecutive sites, byte loads.
 // Random numbers: site h of colour-class line ln (lines of one colour in
-// order) uses word h&3 of Philox4x32-10 keyed by the chain seed with counter
-// (ln*ceil(L0/8) + h/4, pass lo, pass hi, colour), so a site's stream depends
-// only on (seed, pass, colour, site): both paths, the LDS-resident kernel, the
-// HBM work-list kernel, a chain alone and a chain in a batch agree bit for bit.
+// order) draws a 32-bit uniform from Philox4x32-10 keyed by the chain seed:
+// its high 16 bits from call (ln*ceil(L0/16) + h/8, pass lo, pass hi, colour),
+// its low 16 bits, read only when they can decide, from call
+// (ln*L0/2 + h, pass lo, pass hi, colour | 2^31) (see SiteRng).  A site's
+// stream depends only on (seed, pass, colour, site): both paths, the
+// LDS-resident kernel, the HBM kernels, a chain alone and a chain in a batch
+// agree bit for bit.
 //
 // Acceptance: class (spin, up-neighbour counts) -> 32-bit threshold t with
 // accept iff r <= t (t = 2^32-1 when dE <= 0), tabulated per chain on the host
-// with exp(-beta*dE); no exp per attempt.  Observables are exact integers
+// with exp(-beta*dE) at index checker_class(b, c1, c2); no exp per attempt.  Observables are exact integers
 // (S1 = sum_NN s s, S2 = sum_NNN s s, M = sum s) updated from per-flip deltas
 // reduced per workgroup.
 #include "internal.hh"
@@ -41,6 +44,7 @@ namespace mcg {
 struct LineGeom {
   int32_t half;     // sites per half line (L0/2)
   int32_t qpl;      // quads per half line, ceil(half/4)
+  int32_t opl;      // 8-site groups per half line, ceil(half/8) (Philox calls)
   int32_t nl;       // lines per colour
   int32_t chunked;  // 16-site chunk path (half % 16 == 0)
 };
@@ -49,6 +53,7 @@ __device__ __forceinline__ LineGeom line_geom(const CheckerParams &P) {
   LineGeom g;
   g.half = P.L0 >> 1;
   g.qpl = (g.half + 3) >> 2;
+  g.opl = (g.half + 7) >> 3;
   g.chunked = ((g.half & 15) == 0) && !P.quad_only;
   if (P.ncolor == 2)
     g.nl = P.L1 * P.L2;
@@ -107,18 +112,48 @@ __device__ __forceinline__ Lines lines_of(const CheckerParams &P, uint8_t *lat, 
   return Ls;
 }
 
-__device__ __forceinline__ void accept_site(int b, int c1, int c2, uint32_t r,
-                                            const uint32_t *__restrict__ tab, int z1, int z2,
-                                            bool &flip, int &dS1, int &dS2, int &dM, int &nacc) {
-  const int cls = b * (z1 + 1) * (z2 + 1) + c1 * (z2 + 1) + c2;
-  flip = r <= tab[cls];
-  if (flip) {
-    const int s = 2 * b - 1;
-    dS1 += -2 * s * (2 * c1 - z1);
-    dS2 += -2 * s * (2 * c2 - z2);
-    dM += -2 * s;
-    nacc += 1;
-  }
+// Random words.  A site's 32-bit uniform r is split in two 16-bit halves: the
+// high half comes from the main stream, one Philox4x32-10 call per 8 sites of
+// a colour-class half line (counter (ln*opl + h/8, pass lo, pass hi, colour);
+// site h takes bits 16*(h&1) of word (h&7)>>1), and the low half, needed only
+// when the high half equals the threshold's (probability 2^-16), from word 0
+// of the refinement call (ln*half + h, pass lo, pass hi, colour | 2^31).  The
+// acceptance r <= t is therefore decided exactly as for a 32-bit uniform,
+// with half the Philox work of one call per 4 sites.
+struct SiteRng {
+  uint32_t key0, key1, pass_lo, pass_hi, color;
+};
+
+__device__ __forceinline__ uint32_t hi16_of(const u32x4 &r, int e) {
+  const uint32_t w = r.v[(e & 7) >> 1];
+  return (e & 1) ? (w >> 16) : (w & 0xFFFFu);
+}
+
+__device__ __forceinline__ uint32_t refine_lo16(const SiteRng &rc, uint32_t site_ctr) {
+  return philox4x32_10(site_ctr, rc.pass_lo, rc.pass_hi, rc.color | 0x80000000u, rc.key0,
+                       rc.key1).v[0] & 0xFFFFu;
+}
+
+// Acceptance class of (old spin bit b, up-neighbour counts c1, c2) and the
+// decision r <= t from the high half when it settles it: returns 1 (flip),
+// 0 (keep) or -1 (the low half decides: flip iff lo16 <= (t & 0xFFFF)).
+__device__ __forceinline__ int decide_hi(int b, int c1, int c2, uint32_t hi16,
+                                         const uint32_t *__restrict__ tab, int z1, int z2,
+                                         uint32_t &t) {
+  const int cls = checker_class(b, c1, c2);
+  t = tab[cls];
+  if (t == 0xFFFFFFFFu) return 1;
+  const uint32_t thi = t >> 16;
+  return hi16 < thi ? 1 : (hi16 > thi ? 0 : -1);
+}
+
+__device__ __forceinline__ void count_flip(int b, int c1, int c2, int z1, int z2, int &dS1,
+                                           int &dS2, int &dM, int &nacc) {
+  const int s = 2 * b - 1;
+  dS1 += -2 * s * (2 * c1 - z1);
+  dS2 += -2 * s * (2 * c2 - z2);
+  dM += -2 * s;
+  nacc += 1;
 }
 
 // --- quad path ------------------------------------------------------------------
@@ -134,9 +169,13 @@ __device__ __forceinline__ void update_quad(const CheckerParams &P, const LineGe
   const Lines Ls = lines_of(P, lat, j, k);
   const int half = g.half;
   const int oa = a * half, ob = (1 - a) * half;  // own half, other half offsets
-  const u32x4 r = philox4x32_10((uint32_t)((int64_t)ln * g.qpl + qi), (uint32_t)pass,
-                                (uint32_t)(pass >> 32), (uint32_t)color, P.key0, P.key1);
+  const SiteRng rc{P.key0, P.key1, (uint32_t)pass, (uint32_t)(pass >> 32), (uint32_t)color};
+  const u32x4 r = philox4x32_10((uint32_t)((int64_t)ln * g.opl + (qi >> 1)), rc.pass_lo,
+                                rc.pass_hi, rc.color, P.key0, P.key1);
   const int z1 = P.z1, z2 = P.z2;
+  uint32_t need = 0u;  // sites whose low random half decides (threshold kept in tneed)
+  uint32_t tneed[4] = {0u, 0u, 0u, 0u};
+  int cneed[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int h = 4 * qi + u;
@@ -152,9 +191,28 @@ __device__ __forceinline__ void update_quad(const CheckerParams &P, const LineGe
         c2 += Ls.kp[ob + he] + Ls.kp[ob + hw] + Ls.km[ob + he] + Ls.km[ob + hw] +
               Ls.jpkp[oa + h] + Ls.jpkm[oa + h] + Ls.jmkp[oa + h] + Ls.jmkm[oa + h];
     }
-    bool flip;
-    accept_site(b, c1, c2, r.v[u], tab, z1, z2, flip, dS1, dS2, dM, nacc);
-    if (flip) Ls.own[oa + h] = (uint8_t)(b ^ 1);
+    uint32_t t;
+    const int flip = decide_hi(b, c1, c2, hi16_of(r, h), tab, z1, z2, t);
+    if (flip > 0) {
+      count_flip(b, c1, c2, z1, z2, dS1, dS2, dM, nacc);
+      Ls.own[oa + h] = (uint8_t)(b ^ 1);
+    } else if (flip < 0) {
+      need |= 1u << u;
+      tneed[u] = t;
+      cneed[u] = (b << 16) | (c1 << 8) | c2;
+    }
+  }
+  while (need) {  // rare: ties of the high halves, decided by the refinement words
+    const int u = __builtin_ctz(need);
+    need &= need - 1u;
+    const int h = 4 * qi + u;
+    const uint32_t t = u == 0 ? tneed[0] : u == 1 ? tneed[1] : u == 2 ? tneed[2] : tneed[3];
+    const int cc = u == 0 ? cneed[0] : u == 1 ? cneed[1] : u == 2 ? cneed[2] : cneed[3];
+    if (refine_lo16(rc, (uint32_t)((int64_t)ln * half + h)) <= (t & 0xFFFFu)) {
+      const int b = cc >> 16;
+      count_flip(b, (cc >> 8) & 0xFF, cc & 0xFF, z1, z2, dS1, dS2, dM, nacc);
+      Ls.own[oa + h] = (uint8_t)(b ^ 1);
+    }
   }
 }
 
@@ -204,17 +262,13 @@ __device__ __forceinline__ uint32_t word_of(uint4 v, int w) {
   return w == 0 ? v.x : (w == 1 ? v.y : (w == 2 ? v.z : v.w));
 }
 
-__device__ __forceinline__ void update_chunk(const CheckerParams &P, const LineGeom &g,
-                                             uint8_t *__restrict__ lat,
-                                             const uint32_t *__restrict__ tab, int color,
-                                             int64_t pass, int64_t cd, int &dS1, int &dS2,
-                                             int &dM, int &nacc) {
-  const int cpl = g.half >> 4;  // chunks per half line
-  const int ln = (int)(cd / cpl);
-  const int h0 = (int)(cd - (int64_t)ln * cpl) << 4;
-  int j, k, a;
-  line_of(P, color, ln, j, k, a);
-  const Lines Ls = lines_of(P, lat, j, k);
+// The 16 sites [h0, h0+16) of half a of the line whose half lines are at Ls,
+// colour-class line ln (main-stream Philox counters ln*opl + h0/8 and the next).
+__device__ __forceinline__ void update_chunk_at(const CheckerParams &P, const LineGeom &g,
+                                                const Lines &Ls, int a, int ln, int h0,
+                                                const uint32_t *__restrict__ tab, int color,
+                                                int64_t pass, int &dS1, int &dS2, int &dM,
+                                                int &nacc) {
   const int half = g.half;
   const int oa = a * half, ob = (1 - a) * half;
   const int z1 = P.z1, z2 = P.z2;
@@ -233,29 +287,82 @@ __device__ __forceinline__ void update_chunk(const CheckerParams &P, const LineG
                          add4(ld16(Ls.jmkp + oa + h0), ld16(Ls.jmkm + oa + h0))));
     }
   }
-  const uint32_t q0 = (uint32_t)((int64_t)ln * g.qpl + (h0 >> 2));
-  uint32_t flipw[4] = {0u, 0u, 0u, 0u};
+  const SiteRng rc{P.key0, P.key1, (uint32_t)pass, (uint32_t)(pass >> 32), (uint32_t)color};
+  const uint32_t o0 = (uint32_t)((int64_t)ln * g.opl + (h0 >> 3));
+  // class byte of every site, checker_class(b, c1, c2) (< 256: no carries
+  // between the bytes of a word)
+  uint32_t fw[4] = {0u, 0u, 0u, 0u};  // 1 in the byte of every site that flips
+  uint32_t need = 0u;                 // sites whose low random half decides
 #pragma unroll
-  for (int w = 0; w < 4; ++w) {
-    const u32x4 r = philox4x32_10(q0 + w, (uint32_t)pass, (uint32_t)(pass >> 32),
-                                  (uint32_t)color, P.key0, P.key1);
-    const uint32_t ow = word_of(own, w), w1 = word_of(c1, w), w2 = word_of(c2, w);
+  for (int hw = 0; hw < 2; ++hw) {
+    const u32x4 r = philox4x32_10(o0 + hw, rc.pass_lo, rc.pass_hi, rc.color, P.key0, P.key1);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int b = (ow >> (8 * u)) & 1;
-      const int n1 = (w1 >> (8 * u)) & 0xFF, n2 = (w2 >> (8 * u)) & 0xFF;
-      bool flip;
-      accept_site(b, n1, n2, r.v[u], tab, z1, z2, flip, dS1, dS2, dM, nacc);
-      flipw[w] |= (flip ? 1u : 0u) << (8 * u);
+    for (int w = 2 * hw; w < 2 * hw + 2; ++w) {
+      const uint32_t cw = (word_of(own, w) << 7) + (word_of(c1, w) << 4) + word_of(c2, w);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t thi = tab[(cw >> (8 * u)) & 0xFFu] >> 16;
+        const uint32_t hi = hi16_of(r, 4 * w + u);
+        fw[w] |= (hi < thi ? 1u : 0u) << (8 * u);
+        need |= (hi == thi ? 1u : 0u) << (4 * w + u);
+      }
     }
   }
-  if (flipw[0] | flipw[1] | flipw[2] | flipw[3]) {
-    own.x ^= flipw[0];
-    own.y ^= flipw[1];
-    own.z ^= flipw[2];
-    own.w ^= flipw[3];
+  while (need) {  // rare: ties of the high halves, decided by the refinement words
+    const int e = __builtin_ctz(need);
+    need &= need - 1u;
+    const int w = e >> 2, u = e & 3;
+    const uint32_t cw = (word_of(own, w) << 7) + (word_of(c1, w) << 4) + word_of(c2, w);
+    const uint32_t t = tab[(cw >> (8 * u)) & 0xFFu];
+    if (refine_lo16(rc, (uint32_t)((int64_t)ln * half + h0 + e)) <= (t & 0xFFFFu)) {
+      if (w == 0) fw[0] |= 1u << (8 * u);
+      if (w == 1) fw[1] |= 1u << (8 * u);
+      if (w == 2) fw[2] |= 1u << (8 * u);
+      if (w == 3) fw[3] |= 1u << (8 * u);
+    }
+  }
+  // observable changes, four sites per word: a flip of a site with spin bit b
+  // and up-neighbour counts c1, c2 changes S1 by -2(2b-1)(2c1-z1), S2 by
+  // -2(2b-1)(2c2-z2) and M by -2(2b-1); byte dot products give the sums over
+  // the flipped sites of 1, b, c1, b*c1, c2 and b*c2
+  const uint32_t ones = 0x01010101u;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const uint32_t f = fw[w];
+    if (!f) continue;
+    const uint32_t fb = f & word_of(own, w);
+    const int nf = (int)__builtin_amdgcn_udot4(f, ones, 0u, false);
+    const int nb = (int)__builtin_amdgcn_udot4(fb, ones, 0u, false);
+    const int s1 = (int)__builtin_amdgcn_udot4(f, word_of(c1, w), 0u, false);
+    const int b1 = (int)__builtin_amdgcn_udot4(fb, word_of(c1, w), 0u, false);
+    const int s2 = (int)__builtin_amdgcn_udot4(f, word_of(c2, w), 0u, false);
+    const int b2 = (int)__builtin_amdgcn_udot4(fb, word_of(c2, w), 0u, false);
+    dM += -2 * (2 * nb - nf);
+    dS1 += -2 * (4 * b1 - 2 * z1 * nb - 2 * s1 + z1 * nf);
+    dS2 += -2 * (4 * b2 - 2 * z2 * nb - 2 * s2 + z2 * nf);
+    nacc += nf;
+  }
+  if (fw[0] | fw[1] | fw[2] | fw[3]) {
+    own.x ^= fw[0];
+    own.y ^= fw[1];
+    own.z ^= fw[2];
+    own.w ^= fw[3];
     *reinterpret_cast<uint4 *>(Ls.own + oa + h0) = own;
   }
+}
+
+__device__ __forceinline__ void update_chunk(const CheckerParams &P, const LineGeom &g,
+                                             uint8_t *__restrict__ lat,
+                                             const uint32_t *__restrict__ tab, int color,
+                                             int64_t pass, int64_t cd, int &dS1, int &dS2,
+                                             int &dM, int &nacc) {
+  const int cpl = g.half >> 4;  // chunks per half line
+  const int ln = (int)(cd / cpl);
+  const int h0 = (int)(cd - (int64_t)ln * cpl) << 4;
+  int j, k, a;
+  line_of(P, color, ln, j, k, a);
+  const Lines Ls = lines_of(P, lat, j, k);
+  update_chunk_at(P, g, Ls, a, ln, h0, tab, color, pass, dS1, dS2, dM, nacc);
 }
 
 // Work units of one colour of one chain: chunks (chunk path) or quads.
@@ -293,14 +400,13 @@ __global__ __launch_bounds__(1024) void checker_lds_kernel(
   if (!active[c]) return;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int red[4][16];
-  __shared__ uint32_t tab[2 * 7 * 13];
+  __shared__ uint32_t tab[CHECKER_NTAB];
   const CheckerParams P = Pp[c];
   CheckerState *St = Sp + c;
   uint8_t *g_lat = reinterpret_cast<uint8_t *>(lat_pool + P.lat_off);
   uint8_t *lat = reinterpret_cast<uint8_t *>(smem);
   const int t = threadIdx.x, nt = blockDim.x;
-  const int ntab = 2 * (P.z1 + 1) * (P.z2 + 1);
-  for (int i = t; i < ntab; i += nt) tab[i] = tables[P.tab_off + i];
+  for (int i = t; i < CHECKER_NTAB; i += nt) tab[i] = tables[P.tab_off + i];
   const int64_t n = P.n;
   const int64_t n16 = (n + 15) >> 4;  // lattices are padded to 16 bytes
   {
@@ -452,6 +558,211 @@ __global__ __launch_bounds__(HBM_BLOCK) void checker_color_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// 2D band kernels (HBM path, 16-site chunk layout): one pass in two launches
+// that read and write every line about once, instead of one launch per colour
+// that each read the whole lattice.
+//
+// A pass splits into two waves of updates: A (2 colours: colour 0 on every
+// line; 4 colours: colours 0 then 1 on the even lines) and B (colour 1 on
+// every line; colours 2 then 3 on the odd lines).  A(j) reads the B sites of
+// lines j-1..j+1 before the pass touches them; B(j) needs A done on
+// j-1..j+1.  A workgroup owns a band of H lines [j0, j0+H) and streams it
+// through an LDS ring of BAND_RING lines: step t runs A(t+1) beside
+// B(t-1) (2 colours) or B(t-2) (4 colours, odd t only), pairs that neither
+// read nor write each other's sites.  B on the band's edge lines j0 and
+// j0+H-1 needs lines of the neighbouring bands and runs in the second launch
+// (checker_seam_kernel), from HBM.  Every site's update reads the same
+// neighbour values as in the colour-by-colour sweep and uses the same Philox
+// words, so trajectories are identical to the per-colour kernels.
+//
+// Write-back: a line goes back to HBM once its last update of the band is
+// done, and only the halves the band changed (the edge lines hold only A
+// updates), so no band writes bytes another band reads.  A band's ring copies
+// of lines j0-1 and j0+H are read-only; of them only the halves holding B
+// sites are used (the other halves may be changing in the neighbouring band).
+// ---------------------------------------------------------------------------
+constexpr int BAND_BLOCK = 256;
+constexpr int BAND_RING = 8;
+
+struct BandCtx {
+  const CheckerParams *P;
+  LineGeom g;
+  uint8_t *ring;
+  int L0, L1, j0, H;
+  __device__ __forceinline__ uint8_t *slot(int j) const {
+    return ring + ((j - j0 + 1) & (BAND_RING - 1)) * L0;
+  }
+};
+
+// One colour's half-line update of line j (all its chunks), by threads
+// [t0, t0 + nthreads) of the block.
+__device__ __forceinline__ void band_half(const BandCtx &b, int j, int color, int t0,
+                                          int nthreads, const uint32_t *tab, int64_t pass,
+                                          int &dS1, int &dS2, int &dM, int &nacc) {
+  const int cpl = b.g.half >> 4;
+  const int a = b.P->ncolor == 2 ? ((color + j) & 1) : (color & 1);
+  const int ln = b.P->ncolor == 2 ? j : (j >> 1);
+  Lines Ls;
+  Ls.own = b.slot(j);
+  Ls.jp = b.slot(j + 1);
+  Ls.jm = b.slot(j - 1);
+  Ls.kp = Ls.km = Ls.jpkp = Ls.jpkm = Ls.jmkp = Ls.jmkm = Ls.own;
+  for (int ch = (int)threadIdx.x - t0; ch < cpl; ch += nthreads)
+    update_chunk_at(*b.P, b.g, Ls, a, ln, ch << 4, tab, color, pass, dS1, dS2, dM, nacc);
+}
+
+// Lines of the band: copy line j (global, wrapped) into its ring slot.
+__device__ __forceinline__ void band_load(const BandCtx &b, const uint8_t *lat, int j) {
+  const int jw = j < 0 ? j + b.L1 : (j >= b.L1 ? j - b.L1 : j);
+  const uint4 *src = reinterpret_cast<const uint4 *>(lat + (int64_t)jw * b.L0);
+  uint4 *dst = reinterpret_cast<uint4 *>(b.slot(j));
+  for (int i = threadIdx.x; i < (b.L0 >> 4); i += BAND_BLOCK) dst[i] = src[i];
+}
+
+// Write back half `which` (0, 1) or both halves (2) of band line j.
+__device__ __forceinline__ void band_store(const BandCtx &b, uint8_t *lat, int j, int which) {
+  const int n16 = b.g.half >> 4;
+  const int first = which == 1 ? n16 : 0;
+  const int count = which == 2 ? 2 * n16 : n16;
+  const uint4 *src = reinterpret_cast<const uint4 *>(b.slot(j));
+  uint4 *dst = reinterpret_cast<uint4 *>(lat + (int64_t)j * b.L0);
+  for (int i = threadIdx.x; i < count; i += BAND_BLOCK) dst[first + i] = src[first + i];
+}
+
+__device__ __forceinline__ void block_add_deltas(CheckerState *St, int dS1, int dS2, int dM,
+                                                 int nacc, int (*red)[BAND_BLOCK / 64]) {
+  dS1 = wave_sum(dS1);
+  dS2 = wave_sum(dS2);
+  dM = wave_sum(dM);
+  nacc = wave_sum(nacc);
+  const int t = threadIdx.x, wid = t >> 6, lane = t & 63;
+  if (lane == 0) {
+    red[0][wid] = dS1;
+    red[1][wid] = dS2;
+    red[2][wid] = dM;
+    red[3][wid] = nacc;
+  }
+  __syncthreads();
+  if (t == 0) {
+    long long a1 = 0, a2 = 0, am = 0, an = 0;
+    for (int w = 0; w < BAND_BLOCK / 64; ++w) {
+      a1 += red[0][w];
+      a2 += red[1][w];
+      am += red[2][w];
+      an += red[3][w];
+    }
+    if (a1) atomicAdd(reinterpret_cast<unsigned long long *>(&St->S1), (unsigned long long)a1);
+    if (a2) atomicAdd(reinterpret_cast<unsigned long long *>(&St->S2), (unsigned long long)a2);
+    if (am) atomicAdd(reinterpret_cast<unsigned long long *>(&St->M), (unsigned long long)am);
+    if (an)
+      atomicAdd(reinterpret_cast<unsigned long long *>(&St->n_accept), (unsigned long long)an);
+  }
+}
+
+// work: (chain, j0, H) per band
+__global__ __launch_bounds__(BAND_BLOCK) void checker_band_kernel(
+    const CheckerParams *__restrict__ Pp, CheckerState *__restrict__ Sp,
+    int8_t *__restrict__ lat_pool, const uint32_t *__restrict__ tables,
+    const int32_t *__restrict__ work) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t tab[CHECKER_NTAB];
+  __shared__ int red[4][BAND_BLOCK / 64];
+  const int c = work[3 * blockIdx.x], j0 = work[3 * blockIdx.x + 1], H = work[3 * blockIdx.x + 2];
+  const CheckerParams P = Pp[c];
+  CheckerState *St = Sp + c;
+  const int64_t pass = St->n_pass;
+  for (int i = threadIdx.x; i < CHECKER_NTAB; i += BAND_BLOCK) tab[i] = tables[P.tab_off + i];
+  uint8_t *lat = reinterpret_cast<uint8_t *>(lat_pool + P.lat_off);
+  BandCtx b;
+  b.P = &P;
+  b.g = line_geom(P);
+  b.ring = reinterpret_cast<uint8_t *>(smem);
+  b.L0 = P.L0;
+  b.L1 = P.L1;
+  b.j0 = j0;
+  b.H = H;
+  const bool four = P.ncolor == 4;
+  const int stride = four ? 2 : 1;
+  for (int j = j0 - 1; j <= j0 + 1; ++j) band_load(b, lat, j);
+  __syncthreads();
+  int dS1 = 0, dS2 = 0, dM = 0, nacc = 0;
+  const int jlast = j0 + H - 1;
+  for (int t = j0 - 1; t <= jlast; t += stride) {
+    if (!four) {
+      const bool doA = t + 1 <= jlast;
+      const bool doB = t - 1 >= j0 + 1 && t - 1 <= jlast - 1;
+      // A(t+1) on threads [0, nA), B(t-1) on the rest
+      const int nA = doA ? (doB ? BAND_BLOCK / 2 : BAND_BLOCK) : 0;
+      if (doA && (int)threadIdx.x < nA)
+        band_half(b, t + 1, 0, 0, nA, tab, pass, dS1, dS2, dM, nacc);
+      if (doB && (int)threadIdx.x >= nA)
+        band_half(b, t - 1, 1, nA, BAND_BLOCK - nA, tab, pass, dS1, dS2, dM, nacc);
+    } else {
+      const bool doA = t + 1 <= jlast - 1;
+      const bool doB = t - 2 >= j0 + 1 && t - 2 <= jlast - 2;
+      const int nA = doA ? (doB ? BAND_BLOCK / 2 : BAND_BLOCK) : 0;
+      if (doA && (int)threadIdx.x < nA)
+        band_half(b, t + 1, 0, 0, nA, tab, pass, dS1, dS2, dM, nacc);
+      if (doB && (int)threadIdx.x >= nA)
+        band_half(b, t - 2, 2, nA, BAND_BLOCK - nA, tab, pass, dS1, dS2, dM, nacc);
+      __syncthreads();
+      if (doA && (int)threadIdx.x < nA)
+        band_half(b, t + 1, 1, 0, nA, tab, pass, dS1, dS2, dM, nacc);
+      if (doB && (int)threadIdx.x >= nA)
+        band_half(b, t - 2, 3, nA, BAND_BLOCK - nA, tab, pass, dS1, dS2, dM, nacc);
+    }
+    // the next lines into their ring slots (slots of lines no step reads any more)
+    for (int j = t + 3; j < t + 3 + stride; ++j)
+      if (j <= jlast + 1) band_load(b, lat, j);
+    __syncthreads();
+    // lines whose last update of this band was this step
+    if (!four) {
+      // A-only edge lines j0 and jlast: the A half; inner lines: after B(j) at step j+1
+      if (t + 1 == j0 || (t + 1 == jlast && jlast != j0)) band_store(b, lat, t + 1, (t + 1) & 1);
+      if (t - 1 >= j0 + 1 && t - 1 <= jlast - 1) band_store(b, lat, t - 1, 2);
+    } else {
+      if (t + 1 <= jlast - 1) band_store(b, lat, t + 1, 2);           // even line after A
+      if (t - 2 >= j0 + 1 && t - 2 <= jlast - 2) band_store(b, lat, t - 2, 2);  // odd after B
+    }
+  }
+  block_add_deltas(St, dS1, dS2, dM, nacc, red);
+}
+
+// B on the band edge lines (2 colours: j0 and j0+H-1; 4 colours: j0+H-1),
+// from HBM after every band's first launch.
+__global__ __launch_bounds__(BAND_BLOCK) void checker_seam_kernel(
+    const CheckerParams *__restrict__ Pp, CheckerState *__restrict__ Sp,
+    int8_t *__restrict__ lat_pool, const uint32_t *__restrict__ tables,
+    const int32_t *__restrict__ work) {
+  __shared__ int red[4][BAND_BLOCK / 64];
+  const int c = work[3 * blockIdx.x], j0 = work[3 * blockIdx.x + 1], H = work[3 * blockIdx.x + 2];
+  const CheckerParams P = Pp[c];
+  CheckerState *St = Sp + c;
+  const int64_t pass = St->n_pass;
+  const uint32_t *tab = tables + P.tab_off;
+  uint8_t *lat = reinterpret_cast<uint8_t *>(lat_pool + P.lat_off);
+  const LineGeom g = line_geom(P);
+  const int cpl = g.half >> 4;
+  int dS1 = 0, dS2 = 0, dM = 0, nacc = 0;
+  const int jlast = j0 + H - 1;
+  if (P.ncolor == 2) {
+    const int nl = jlast != j0 ? 2 : 1;
+    for (int task = threadIdx.x; task < nl * cpl; task += BAND_BLOCK) {
+      const int j = task < cpl ? j0 : jlast;
+      const int ch = task < cpl ? task : task - cpl;
+      update_chunk(P, g, lat, tab, 1, pass, (int64_t)j * cpl + ch, dS1, dS2, dM, nacc);
+    }
+  } else {
+    for (int ch = threadIdx.x; ch < cpl; ch += BAND_BLOCK)
+      update_chunk(P, g, lat, tab, 2, pass, (int64_t)(jlast >> 1) * cpl + ch, dS1, dS2, dM, nacc);
+    __syncthreads();
+    for (int ch = threadIdx.x; ch < cpl; ch += BAND_BLOCK)
+      update_chunk(P, g, lat, tab, 3, pass, (int64_t)(jlast >> 1) * cpl + ch, dS1, dS2, dM, nacc);
+  }
+  block_add_deltas(St, dS1, dS2, dM, nacc, red);
+}
+
 // After the last colour of a pass: advance the pass counter and append a
 // sample record when due.
 __global__ void checker_record_kernel(const CheckerParams *__restrict__ Pp,
@@ -509,18 +820,19 @@ hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, Checke
                                     const int32_t *work, int n_work_items, int ncolor, int upt,
                                     mcg_sample *series, int64_t series_cap,
                                     const uint8_t *active, int n_chains, int64_t sample_period,
-                                    int passes) {
+                                    int passes, const int32_t *band_work, int n_bands,
+                                    int ring_bytes) {
   void *color_fn = upt == 1   ? reinterpret_cast<void *>(&checker_color_kernel<1>)
                    : upt == 2 ? reinterpret_cast<void *>(&checker_color_kernel<2>)
                               : reinterpret_cast<void *>(&checker_color_kernel<4>);
   const unsigned color_grid = (unsigned)n_work_items * (unsigned)(4 / upt);
   hipGraphNode_t prev = nullptr;
-  auto add = [&](void *fn, dim3 grid, dim3 block, void **args) -> hipError_t {
+  auto add = [&](void *fn, dim3 grid, dim3 block, void **args, unsigned shmem = 0) -> hipError_t {
     hipKernelNodeParams kp{};
     kp.func = fn;
     kp.gridDim = grid;
     kp.blockDim = block;
-    kp.sharedMemBytes = 0;
+    kp.sharedMemBytes = shmem;
     kp.kernelParams = args;
     kp.extra = nullptr;
     hipGraphNode_t node = nullptr;
@@ -529,7 +841,16 @@ hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, Checke
     return e;
   };
   for (int p = 0; p < passes; ++p) {
-    if (n_work_items > 0)
+    if (n_bands > 0) {
+      void *args[] = {(void *)&P, (void *)&St, (void *)&lat_pool, (void *)&tables,
+                      (void *)&band_work};
+      hipError_t e = add(reinterpret_cast<void *>(&checker_band_kernel), dim3(n_bands),
+                         dim3(BAND_BLOCK), args, (unsigned)ring_bytes);
+      if (e == hipSuccess)
+        e = add(reinterpret_cast<void *>(&checker_seam_kernel), dim3(n_bands), dim3(BAND_BLOCK),
+                args);
+      if (e != hipSuccess) return e;
+    } else if (n_work_items > 0)
       for (int color = 0; color < ncolor; ++color) {
         void *args[] = {(void *)&P, (void *)&St, (void *)&lat_pool, (void *)&tables,
                         (void *)&work, (void *)&color};
@@ -544,6 +865,22 @@ hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, Checke
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+void launch_checker_band(const CheckerParams *P, CheckerState *St, int8_t *lat_pool,
+                         const uint32_t *tables, const int32_t *band_work, int n_bands,
+                         int ring_bytes, hipStream_t stream) {
+  hipLaunchKernelGGL(checker_band_kernel, dim3(n_bands), dim3(BAND_BLOCK), ring_bytes, stream,
+                     P, St, lat_pool, tables, band_work);
+  hipLaunchKernelGGL(checker_seam_kernel, dim3(n_bands), dim3(BAND_BLOCK), 0, stream, P, St,
+                     lat_pool, tables, band_work);
+}
+
+int checker_band_ring_lines() { return BAND_RING; }
+
+hipError_t checker_band_set_max_lds(int bytes) {
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(checker_band_kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
 int checker_units_per_item() { return UNITS_PER_ITEM; }

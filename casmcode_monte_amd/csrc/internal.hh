@@ -67,6 +67,14 @@ struct CheckerParams {
   int64_t tab_off;         // offset of the chain's threshold table (uint32 entries)
 };
 
+// Acceptance table of a chain: entry (b << 7) | (c1 << 4) | c2 for old spin
+// bit b and up-neighbour counts c1 <= 6, c2 <= 12 (unused entries 0), so the
+// class bytes of four sites come from one packed-word shift-and-add.
+constexpr int CHECKER_NTAB = 256;
+__host__ __device__ inline int checker_class(int b, int c1, int c2) {
+  return (b << 7) | (c1 << 4) | c2;
+}
+
 struct CheckerState {
   int64_t n_pass;
   int64_t n_accept;
@@ -93,7 +101,16 @@ hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, Checke
                                     const int32_t *work, int n_work_items, int ncolor, int upt,
                                     mcg_sample *series, int64_t series_cap,
                                     const uint8_t *active, int n_chains, int64_t sample_period,
-                                    int passes);
+                                    int passes, const int32_t *band_work, int n_bands,
+                                    int ring_bytes);
+
+// 2D band path: one launch over (chain, first line, lines) bands, then the
+// band-edge launch; ring_bytes = checker_band_ring_lines() * L0 (largest chain).
+void launch_checker_band(const CheckerParams *P, CheckerState *St, int8_t *lat_pool,
+                         const uint32_t *tables, const int32_t *band_work, int n_bands,
+                         int ring_bytes, hipStream_t stream);
+int checker_band_ring_lines();
+hipError_t checker_band_set_max_lds(int bytes);
 
 // FCC cluster expansion (fcc.hip): CheckerParams with ncolor = 64, tab_off in
 // doubles into the per-chain acceptance tables, tri = the 4096-entry

@@ -239,6 +239,8 @@ typedef struct mcg_schedule {
   int64_t units_per_thread; /* HBM kernel: work units per thread (1, 2, 4)                */
   int64_t graph_passes;     /* HBM path: passes captured per hipGraph replay (1 = none)   */
   double trial_ms;          /* mcg_autotune: time of the chosen schedule on its trial     */
+  int64_t band_lines;       /* 2D HBM path: lines per band of the two-launch band kernels
+                               (-1 = chosen from the sizes, 0 = one launch per colour)  */
 } mcg_schedule;
 
 /* Autotuner: times the candidate schedules on `trial_passes` passes of this

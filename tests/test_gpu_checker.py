@@ -252,3 +252,28 @@ def test_autotune_leaves_the_run_unchanged(streaming):
         assert a.counters(c) == b.counters(c)
     # a second handle of the same size class hits the cache
     assert b.autotune(24) == chosen
+
+
+@pytest.mark.parametrize("J2", [0.0, -0.02])
+@pytest.mark.parametrize("band_lines", [2, 4, 8, 64, -1])
+def test_band_path_equals_colour_kernels(J2, band_lines):
+    """The 2D two-launch band kernels (bands of band_lines lines, including one
+    band per chain and bands of 2 lines) give the per-colour kernels'
+    trajectories bit for bit, and those equal the LDS-resident kernel's."""
+    print("case: band path", J2, band_lines, flush=True)
+    chains = [dict(shape=(64, 24), temperature=2300.0, exchange_potential=0.04, seed=71),
+              dict(shape=(32, 2), temperature=1700.0, exchange_potential=-0.2, seed=72),
+              dict(shape=(96, 10), temperature=3100.0, exchange_potential=0.0, seed=73),
+              dict(shape=(32, 64), temperature=900.0, exchange_potential=0.1, seed=74)]
+    ref = cm.Run(chains, mode=cm.MODE_CHECKERBOARD, lattice=cm.LATTICE_SQUARE, J1=0.1, J2=J2)
+    ref.run(21)
+    for G in (1, 8):
+        for bl in (0, band_lines):
+            r = cm.Run(chains, mode=cm.MODE_CHECKERBOARD, lattice=cm.LATTICE_SQUARE, J1=0.1,
+                       J2=J2, options=cm.OPT_FORCE_STREAMING)
+            r.set_schedule(band_lines=bl, graph_passes=G)
+            r.run(21)
+            for c in range(len(chains)):
+                np.testing.assert_array_equal(ref.get_occupation(c), r.get_occupation(c))
+                np.testing.assert_array_equal(ref.samples(c), r.samples(c))
+                assert ref.counters(c) == r.counters(c)
