@@ -50,6 +50,7 @@ def c3_conditions(shape, n=16):
 
 
 def timed_run(run, passes):
+    run.reserve_samples(run.n_samples(0) + passes + 1)  # no series growth while timed
     t0 = time.perf_counter()
     run.run(passes)
     wall = time.perf_counter() - t0

@@ -36,7 +36,7 @@ EXPORTED = [
     "mcg_rng_dump", "mcg_rng_load", "mcg_run", "mcg_set_active", "mcg_get_counters",
     "mcg_n_samples", "mcg_get_samples", "mcg_clear_samples", "mcg_stats_compute",
     "mcg_equilibration", "mcg_last_run_time", "mcg_assign_chain", "mcg_autotune",
-    "mcg_get_schedule", "mcg_set_schedule",
+    "mcg_get_schedule", "mcg_set_schedule", "mcg_reserve_samples",
 ]
 
 
@@ -102,6 +102,7 @@ def lib():
         L.mcg_last_run_time.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
         L.mcg_assign_chain.argtypes = [C.c_void_p, C.c_int64, P(ChainDesc)]
         L.mcg_autotune.argtypes = [C.c_void_p, C.c_int64, P(Schedule)]
+        L.mcg_reserve_samples.argtypes = [C.c_void_p, C.c_int64]
         L.mcg_get_schedule.argtypes = [C.c_void_p, P(Schedule)]
         L.mcg_set_schedule.argtypes = [C.c_void_p, P(Schedule)]
         _lib = L
@@ -243,6 +244,10 @@ class Run:
         a, b = C.c_int64(), C.c_int64()
         _check(lib().mcg_equilibration(self.h, c, observable, prec, C.byref(a), C.byref(b)))
         return bool(a.value), b.value
+
+    def reserve_samples(self, n_records):
+        """Grow every chain's device sample series to n_records records."""
+        _check(lib().mcg_reserve_samples(self.h, int(n_records)))
 
     def autotune(self, trial_passes=64):
         """Time the candidate launch schedules on this run's chains (state is

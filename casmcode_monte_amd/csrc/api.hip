@@ -1197,6 +1197,12 @@ int mcg_get_samples(mcg_handle *h, int64_t chain, int64_t first, int64_t count,
   return MCG_OK;
 }
 
+int mcg_reserve_samples(mcg_handle *h, int64_t n_records) {
+  if (!h || n_records < 0 || n_records > (int64_t(1) << 40))
+    return fail(MCG_ERR_ARG, "bad handle or record count");
+  return ensure_series(h, n_records);
+}
+
 int mcg_clear_samples(mcg_handle *h) {
   if (!h) return fail(MCG_ERR_ARG, "null handle");
   HIP_TRY(hipStreamSynchronize(h->stream));

@@ -200,6 +200,11 @@ int mcg_get_samples(mcg_handle *h, int64_t chain, int64_t first, int64_t count,
 /* Drop all stored sample records of every chain (device memory is kept). */
 int mcg_clear_samples(mcg_handle *h);
 
+/* Grow the sample series so every chain can hold n_records records without a
+ * reallocation inside later mcg_run calls (the counterpart of the Sampler's
+ * capacity increment, sampling/Sampler.hh:252-267). */
+int mcg_reserve_samples(mcg_handle *h, int64_t n_records);
+
 /* Observables computed on the device from a chain's sample series:
  *   MCG_OBS_PARAM_COMPOSITION  x  = fl(fl((N+M)/2)/N)        (model.hh:412-422)
  *   MCG_OBS_FORMATION_ENERGY   ef = fl(E_f/N)                 (model.hh:293-295)
