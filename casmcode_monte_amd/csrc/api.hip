@@ -281,6 +281,9 @@ struct mcg_handle {
   int32_t *d_band = nullptr;
   int n_bands = 0;
   int band_ring_bytes = 0;
+  // line-class path (4/8 colourings): (chain, first line, lines) items
+  int32_t *d_pair = nullptr;
+  int n_pairs = 0;
   double tuned_ms = 0.0;
   // hipGraph of graph_passes passes of the HBM path (rebuilt when the work
   // list, the series buffer or the schedule change)
@@ -296,7 +299,7 @@ struct mcg_handle {
     if (stream) hipStreamSynchronize(stream);
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     void *ptrs[] = {d_lat, d_active, d_series, d_stats, d_eq, d_xp, d_xs, d_cp, d_cs, d_tab,
-                    d_work, d_ftab, d_tri, d_band};
+                    d_work, d_ftab, d_tri, d_band, d_pair};
     for (void *p : ptrs)
       if (p) hipFree(p);
     if (ev0) hipEventDestroy(ev0);
@@ -690,6 +693,36 @@ int rebuild_work(mcg_handle *h) {
     HIP_TRY(copy_sync(h, h->d_band, band.data(), sizeof(int32_t) * band.size(),
                       hipMemcpyHostToDevice));
   }
+  // line-class path: whole colour-class lines per workgroup (used when the
+  // band path is not)
+  if (h->d_pair) hipFree(h->d_pair);
+  h->d_pair = nullptr;
+  h->n_pairs = 0;
+  if (h->n_bands == 0 && (h->ncolor == 4 || h->ncolor == 8) && !h->fcc()) {
+    const int blk = checker_pair_block();
+    std::vector<int32_t> items;
+    for (int c = 0; c < h->n_chains; ++c) {
+      if (!h->active[c]) continue;
+      auto const &P = h->cp[c];
+      const int64_t half = P.L0 / 2;
+      const bool chunked = half % 16 == 0 && !P.quad_only;
+      const int64_t upl = chunked ? half / 16 : (half + 3) / 4;
+      const int64_t nl = P.ncolor == 4 ? P.L1 / 2 : (int64_t)(P.L1 / 2) * (P.L2 / 2);
+      // a workgroup takes units_per_thread units of each colour per thread
+      const int64_t lpb = h->units_per_thread * std::max<int64_t>(1, blk / upl);
+      for (int64_t l0 = 0; l0 < nl; l0 += lpb) {
+        items.push_back(c);
+        items.push_back((int32_t)l0);
+        items.push_back((int32_t)std::min(lpb, nl - l0));
+      }
+    }
+    h->n_pairs = (int)(items.size() / 3);
+    if (h->n_pairs) {
+      HIP_TRY(hipMalloc(&h->d_pair, sizeof(int32_t) * items.size()));
+      HIP_TRY(copy_sync(h, h->d_pair, items.data(), sizeof(int32_t) * items.size(),
+                        hipMemcpyHostToDevice));
+    }
+  }
   h->work_dirty = false;
   return MCG_OK;
 }
@@ -700,6 +733,10 @@ void enqueue_pass(mcg_handle *h) {
   if (h->n_bands > 0)
     launch_checker_band(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_band, h->n_bands,
                         h->band_ring_bytes, h->stream);
+  else if (h->n_pairs > 0)
+    for (int lc = 0; lc < h->ncolor / 2; ++lc)
+      launch_checker_pair(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_pair, h->n_pairs, lc,
+                          h->stream);
   else if (h->n_work > 0)
     for (int color = 0; color < h->ncolor; ++color)
       launch_checker_color(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_work, h->n_work, color,
@@ -721,7 +758,8 @@ int build_graph(mcg_handle *h) {
   hipError_t e = checker_graph_add_passes(
       g, h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_work, h->n_work, h->ncolor,
       h->units_per_thread, h->d_series, h->series_cap, h->d_active, h->n_chains,
-      h->sample_period, h->graph_passes, h->d_band, h->n_bands, h->band_ring_bytes);
+      h->sample_period, h->graph_passes, h->d_band, h->n_bands, h->band_ring_bytes, h->d_pair,
+      h->n_pairs);
   if (e == hipSuccess) e = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
@@ -1320,9 +1358,10 @@ int mcg_set_schedule(mcg_handle *h, const mcg_schedule *s) {
     return fail(MCG_ERR_ARG, "band_lines must be -1 (automatic), 0 (off) or even <= 4096");
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->checker_block = (int)s->lds_block;
+  if (h->band_lines != (int)s->band_lines || h->units_per_thread != (int)s->units_per_thread)
+    h->work_dirty = true;
   h->units_per_thread = (int)s->units_per_thread;
   h->graph_passes = (int)s->graph_passes;
-  if (h->band_lines != (int)s->band_lines) h->work_dirty = true;
   h->band_lines = (int)s->band_lines;
   h->graph_dirty = true;
   return MCG_OK;

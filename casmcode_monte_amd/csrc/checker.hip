@@ -559,6 +559,69 @@ __global__ __launch_bounds__(HBM_BLOCK) void checker_color_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// HBM line-class kernel (4 and 8 colourings): the two colours of one line
+// class (halves a = 0 and a = 1 of the same lines) in one launch.  Colour
+// 2m+1 depends on colour 2m only through the in-line neighbours of the same
+// line (its other neighbours belong to other line classes), so a workgroup that
+// owns whole lines updates half 0, synchronises, and updates half 1: half the
+// launches, and half the passes over the lattice, of one launch per colour.
+// work: (chain, first colour-class line, lines) per item.
+// ---------------------------------------------------------------------------
+constexpr int PAIR_BLOCK = 256;
+
+__global__ __launch_bounds__(PAIR_BLOCK) void checker_pair_kernel(
+    const CheckerParams *__restrict__ Pp, CheckerState *__restrict__ Sp,
+    int8_t *__restrict__ lat_pool, const uint32_t *__restrict__ tables,
+    const int32_t *__restrict__ work, int line_class) {
+  __shared__ int red[4][PAIR_BLOCK / 64];
+  const int c = work[3 * blockIdx.x], l0 = work[3 * blockIdx.x + 1];
+  const int nlines = work[3 * blockIdx.x + 2];
+  const CheckerParams P = Pp[c];
+  CheckerState *St = Sp + c;
+  const LineGeom g = line_geom(P);
+  const int64_t pass = St->n_pass;
+  const uint32_t *tab = tables + P.tab_off;
+  uint8_t *lat = reinterpret_cast<uint8_t *>(lat_pool + P.lat_off);
+  const int upl = chunk_path(g) ? (g.half >> 4) : g.qpl;  // units per half line
+  int dS1 = 0, dS2 = 0, dM = 0, nacc = 0;
+  for (int a = 0; a < 2; ++a) {
+    const int color = 2 * line_class + a;
+    for (int task = threadIdx.x; task < nlines * upl; task += PAIR_BLOCK) {
+      const int ln = l0 + task / upl;
+      const int64_t unit = (int64_t)ln * upl + (task - (task / upl) * upl);
+      update_unit(P, g, lat, tab, color, pass, unit, dS1, dS2, dM, nacc);
+    }
+    __syncthreads();
+  }
+  dS1 = wave_sum(dS1);
+  dS2 = wave_sum(dS2);
+  dM = wave_sum(dM);
+  nacc = wave_sum(nacc);
+  const int t = threadIdx.x, wid = t >> 6, lane = t & 63;
+  if (lane == 0) {
+    red[0][wid] = dS1;
+    red[1][wid] = dS2;
+    red[2][wid] = dM;
+    red[3][wid] = nacc;
+  }
+  __syncthreads();
+  if (t == 0) {
+    long long a1 = 0, a2 = 0, am = 0, an = 0;
+    for (int w = 0; w < PAIR_BLOCK / 64; ++w) {
+      a1 += red[0][w];
+      a2 += red[1][w];
+      am += red[2][w];
+      an += red[3][w];
+    }
+    if (a1) atomicAdd(reinterpret_cast<unsigned long long *>(&St->S1), (unsigned long long)a1);
+    if (a2) atomicAdd(reinterpret_cast<unsigned long long *>(&St->S2), (unsigned long long)a2);
+    if (am) atomicAdd(reinterpret_cast<unsigned long long *>(&St->M), (unsigned long long)am);
+    if (an)
+      atomicAdd(reinterpret_cast<unsigned long long *>(&St->n_accept), (unsigned long long)an);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 2D band kernels (HBM path, 16-site chunk layout): one pass in two launches
 // that read and write every line about once, instead of one launch per colour
 // that each read the whole lattice.
@@ -821,7 +884,7 @@ hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, Checke
                                     mcg_sample *series, int64_t series_cap,
                                     const uint8_t *active, int n_chains, int64_t sample_period,
                                     int passes, const int32_t *band_work, int n_bands,
-                                    int ring_bytes) {
+                                    int ring_bytes, const int32_t *pair_work, int n_pairs) {
   void *color_fn = upt == 1   ? reinterpret_cast<void *>(&checker_color_kernel<1>)
                    : upt == 2 ? reinterpret_cast<void *>(&checker_color_kernel<2>)
                               : reinterpret_cast<void *>(&checker_color_kernel<4>);
@@ -850,6 +913,14 @@ hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, Checke
         e = add(reinterpret_cast<void *>(&checker_seam_kernel), dim3(n_bands), dim3(BAND_BLOCK),
                 args);
       if (e != hipSuccess) return e;
+    } else if (n_pairs > 0) {
+      for (int lc = 0; lc < ncolor / 2; ++lc) {
+        void *args[] = {(void *)&P, (void *)&St, (void *)&lat_pool, (void *)&tables,
+                        (void *)&pair_work, (void *)&lc};
+        hipError_t e = add(reinterpret_cast<void *>(&checker_pair_kernel), dim3(n_pairs),
+                           dim3(PAIR_BLOCK), args);
+        if (e != hipSuccess) return e;
+      }
     } else if (n_work_items > 0)
       for (int color = 0; color < ncolor; ++color) {
         void *args[] = {(void *)&P, (void *)&St, (void *)&lat_pool, (void *)&tables,
@@ -875,6 +946,15 @@ void launch_checker_band(const CheckerParams *P, CheckerState *St, int8_t *lat_p
   hipLaunchKernelGGL(checker_seam_kernel, dim3(n_bands), dim3(BAND_BLOCK), 0, stream, P, St,
                      lat_pool, tables, band_work);
 }
+
+void launch_checker_pair(const CheckerParams *P, CheckerState *St, int8_t *lat_pool,
+                         const uint32_t *tables, const int32_t *pair_work, int n_items,
+                         int line_class, hipStream_t stream) {
+  hipLaunchKernelGGL(checker_pair_kernel, dim3(n_items), dim3(PAIR_BLOCK), 0, stream, P, St,
+                     lat_pool, tables, pair_work, line_class);
+}
+
+int checker_pair_block() { return PAIR_BLOCK; }
 
 int checker_band_ring_lines() { return BAND_RING; }
 

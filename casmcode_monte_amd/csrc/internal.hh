@@ -102,7 +102,14 @@ hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, Checke
                                     mcg_sample *series, int64_t series_cap,
                                     const uint8_t *active, int n_chains, int64_t sample_period,
                                     int passes, const int32_t *band_work, int n_bands,
-                                    int ring_bytes);
+                                    int ring_bytes, const int32_t *pair_work, int n_pairs);
+
+// Line-class path (4 and 8 colourings): one launch per line class over
+// (chain, first colour-class line, lines) items, both halves of each line.
+void launch_checker_pair(const CheckerParams *P, CheckerState *St, int8_t *lat_pool,
+                         const uint32_t *tables, const int32_t *pair_work, int n_items,
+                         int line_class, hipStream_t stream);
+int checker_pair_block();
 
 // 2D band path: one launch over (chain, first line, lines) bands, then the
 // band-edge launch; ring_bytes = checker_band_ring_lines() * L0 (largest chain).

@@ -241,7 +241,7 @@ int mcg_equilibration(mcg_handle *h, int64_t chain, int64_t observable, double p
 typedef struct mcg_schedule {
   int64_t lds_block;        /* LDS-resident and FCC kernels: threads per workgroup
                                (256, 512, 1024)                                         */
-  int64_t units_per_thread; /* HBM kernel: work units per thread (1, 2, 4)                */
+  int64_t units_per_thread; /* HBM kernels: work units per thread and colour (1, 2, 4)    */
   int64_t graph_passes;     /* HBM path: passes captured per hipGraph replay (1 = none)   */
   double trial_ms;          /* mcg_autotune: time of the chosen schedule on its trial     */
   int64_t band_lines;       /* 2D HBM path: lines per band of the two-launch band kernels

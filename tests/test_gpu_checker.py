@@ -41,6 +41,8 @@ def run(chains, lattice, J1, J2, passes, options=0, sp=1):
     (cm.LATTICE_SQUARE, (18, 10), 0.0),
     (cm.LATTICE_SC, (8, 6, 4), -0.02),
     (cm.LATTICE_SC, (6, 6, 6), 0.0),
+    (cm.LATTICE_SC, (64, 6, 4), -0.02),
+    (cm.LATTICE_SC, (32, 10, 6), 0.0),
 ])
 def test_lds_and_streaming_paths_identical(lattice, shape, J2):
     print("case: lds vs streaming", lattice, shape, J2, flush=True)
