@@ -215,13 +215,14 @@ __global__ __launch_bounds__(64) void exact_kernel(const ExactParams *__restrict
 // P, lane t evaluates position P+t as if it were a site draw: its site (Lemire
 // on word x[P+t]), the old spin and neighbour sum from the block-start lattice,
 // the class k, whether the attempt would draw a uniform (dE >= 0) and whether
-// it would accept (x[P+t+1] < thr[k]).  A scalar pass then walks the real
-// sequence: position t is an attempt; the next attempt is at t+1, or t+2 when t
-// drew a uniform.  Every lane also marks its site in the lattice byte (bit 1;
-// bit 2 when a second lane marks the same site), so a lane whose own site was
-// drawn twice or whose neighbourhood holds another lane's site is flagged as a
-// potential conflict: the scalar pass stops before such a lane (it will be
-// re-evaluated as lane 0 of the next block, on the updated lattice).  Hence
+// it would accept (x[P+t+1] < thr[k]).  The real sequence follows in closed
+// form: position t is an attempt; the next attempt is at t+1, or t+2 when t
+// drew a uniform.  Every attempt lane then marks its site in the lattice byte
+// (a draw count in bits 1..7; positions that are uniform draws mark nothing),
+// so an attempt whose own site was drawn twice or whose neighbourhood holds
+// another attempt's site is flagged as a potential conflict: the block stops
+// before such a lane (it will be re-evaluated as lane 0 of the next block, on
+// the updated lattice).  Hence
 // every attempt the pass accepts saw exactly the lattice the sequential chain
 // would see.  Flips are committed at the end of the block; the integer
 // observables are block reductions; the FP64 running energy is added in attempt
@@ -246,17 +247,17 @@ __device__ void twist_gen(const uint64_t *src, uint64_t *dst, uint64_t *slot, in
 
 __device__ __forceinline__ int spin_of(uint32_t b) { return (b & SPIN_BIT) ? 1 : -1; }
 
-__device__ __forceinline__ int wave_isum(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
 __device__ __forceinline__ double readlane_f64(double v, int lane) {
   uint64_t u = __double_as_longlong(v);
   uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, lane);
   uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), lane);
   return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__device__ __forceinline__ int wave_isum(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
 __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__restrict__ Pp,
@@ -338,19 +339,10 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
     const uint32_t ju = (j + 1 == (uint32_t)cols) ? 0u : j + 1;
     const uint32_t jd = (j == 0) ? (uint32_t)cols - 1 : j - 1;
     const uint32_t sh = (site & 3u) * 8u;
-    // count the draws of each site in bits 1..7 of its byte (at most 64 draws);
-    // the wave's LDS operations execute in issue order, so the reads below see
-    // every lane's increment (the asm statement keeps the compiler from moving
-    // them above the atomics)
-    if (valid) atomicAdd(&latw[site >> 2], MARK_A << sh);
-    asm volatile("" ::: "memory");
-    const uint32_t b0 = lat[site];
-    const uint32_t b1 = lat[iu + rows * j], b2 = lat[id + rows * j];
-    const uint32_t b3 = lat[i + rows * ju], b4 = lat[i + rows * jd];
-    const int s = spin_of(b0);
-    const int nsum = spin_of(b1) + spin_of(b2) + spin_of(b3) + spin_of(b4);
-    // potential conflict: own site drawn twice, or a neighbour drawn at all
-    const bool pc = valid && ((b0 & 0xFCu) || ((b1 | b2 | b3 | b4) & 0xFEu));
+    const uint32_t a0 = site, a1 = iu + rows * j, a2 = id + rows * j;
+    const uint32_t a3 = i + rows * ju, a4 = i + rows * jd;
+    const int s = spin_of(lat[a0]);
+    const int nsum = spin_of(lat[a1]) + spin_of(lat[a2]) + spin_of(lat[a3]) + spin_of(lat[a4]);
     const int k = (s > 0 ? 5 : 0) + ((nsum + 4) >> 1);
     const uint64_t thr_k = thr_s[k];
     const double dqv = dq_s[k];
@@ -358,7 +350,6 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
     const bool acc = !nd || ((full_mask >> k) & 1u) || (x1 < thr_k);
     const uint64_t ND = __ballot(valid && nd);
     const uint64_t AC = __ballot(valid && acc);
-    const uint64_t STOP = __ballot(pc || special || !inwin);
     const uint64_t SP = __ballot(special);
 
     // Attempt positions of the window in closed form: position t is a uniform
@@ -370,6 +361,17 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
     const uint64_t odd_runs = ((ND + (run_start & ~ALT)) ^ ND) & ND;
     const uint64_t E = ND & (ALT ^ odd_runs);
     const uint64_t A = ~(E << 1);
+    // count the site draws of the attempt positions in bits 1..7 of each site's
+    // byte (at most 64 draws); the wave's LDS operations execute in issue order,
+    // so the reads below see every lane's increment (the asm statement keeps
+    // the compiler from moving them above the atomics)
+    const bool att = valid && ((A >> lane) & 1ull);
+    if (att) atomicAdd(&latw[site >> 2], MARK_A << sh);
+    asm volatile("" ::: "memory");
+    const uint32_t m0 = lat[a0], m1 = lat[a1], m2 = lat[a2], m3 = lat[a3], m4 = lat[a4];
+    // potential conflict: own site drawn twice, or a neighbour drawn at all
+    const bool pc = att && ((m0 & 0xFCu) || ((m1 | m2 | m3 | m4) & 0xFEu));
+    const uint64_t STOP = __ballot(pc || special || !inwin);
     // execute attempts up to (not including) the first flagged attempt t > 0
     const uint64_t stops = A & STOP & ~1ull;
     uint64_t exec = (SP & 1ull) ? 0ull : (A & (stops ? ((1ull << __builtin_ctzll(stops)) - 1) : ~0ull));
@@ -405,7 +407,7 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
     n_acc += nflip;
     n_rej += attempts - nflip;
     if (flipped) atomicXor(&latw[site >> 2], SPIN_BIT << sh);
-    if (valid) atomicAnd(&latw[site >> 2], ~(0xFEu << sh));
+    if (att) atomicAnd(&latw[site >> 2], ~(0xFEu << sh));
     asm volatile("" ::: "memory");
     pos += t;
     step += attempts;
