@@ -57,7 +57,29 @@ def build(verbose=False):
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     shutil.rmtree(objdir, ignore_errors=True)
+    build_python(verbose)
     return LIB
+
+
+def build_python(verbose=False):
+    """Compile the pybind11 module _libcasm_monte (python/libcasm_monte.cpp): the
+    libcasm.monte names over the C++ drop-in, linked to the HIP library."""
+    import sysconfig
+
+    import pybind11
+    ext = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    out = os.path.join(HERE, "_libcasm_monte" + ext)
+    tmp = out + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
+           "-I", os.path.join(ROOT, "include"), "-I", pybind11.get_include(),
+           "-I", sysconfig.get_paths()["include"],
+           os.path.join(HERE, "python", "libcasm_monte.cpp"),
+           "-L", LIBDIR, "-lcasm_monte_mi355x", "-Wl,-rpath,$ORIGIN/lib", "-o", tmp]
+    o = _run(cmd)
+    if verbose and o.strip():
+        print(o)
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
