@@ -48,8 +48,20 @@ template <typename ConfigType, typename StatisticsType>
 using ResultsAnalysisFunctionMap =
     std::map<std::string, ResultsAnalysisFunction<ConfigType, StatisticsType>>;
 
+/// Resident memory of the process in MiB (0 when /proc is unavailable); the
+/// reference records it at the start and end of every run (Results.hh).
+inline double memory_used_MiB() {
+  std::ifstream f("/proc/self/status");
+  std::string line;
+  while (std::getline(f, line))
+    if (line.rfind("VmRSS:", 0) == 0) return std::stod(line.substr(6)) / 1024.0;
+  return 0.0;
+}
+
 template <typename ConfigType, typename StatisticsType>
 struct Results {
+  typedef ConfigType config_type;
+  typedef StatisticsType stats_type;
   Results(std::vector<std::string> _sampler_names,
           StateSamplingFunctionMap const &_sampling_functions,
           std::vector<std::string> _json_sampler_names,
@@ -67,6 +79,8 @@ struct Results {
   jsonStateSamplingFunctionMap json_sampling_functions;
   ResultsAnalysisFunctionMap<ConfigType, StatisticsType> analysis_functions;
   std::optional<TimeType> elapsed_clocktime;
+  double initial_memory_used_MiB = 0.0;
+  double final_memory_used_MiB = 0.0;
   SamplerMap samplers;
   std::map<std::string, std::shared_ptr<jsonSampler>> json_samplers;
   std::map<std::string, VectorXd> analysis;
@@ -81,6 +95,8 @@ struct Results {
 
   void reset() {
     elapsed_clocktime.reset();
+    initial_memory_used_MiB = memory_used_MiB();
+    final_memory_used_MiB = 0.0;
     samplers.clear();
     json_samplers.clear();
     analysis.clear();
@@ -319,6 +335,7 @@ class SamplingFixture {
   void finalize(state_type const &state, Index run_index) {
     Log &log = m_params.method_log.log;
     m_results.elapsed_clocktime = log.time_s();
+    m_results.final_memory_used_MiB = memory_used_MiB();
     m_results.completion_check_results = m_completion_check.results();
     m_results.analysis =
         make_analysis(m_results, m_params.analysis_functions, m_params.analysis_names);
