@@ -77,6 +77,55 @@ def bench_exact(K, W):
     return out
 
 
+def canonical_conditions(shape=(128, 128)):
+    """8 temperatures x 8 compositions x = n(+1)/N in 0.1..0.8, random initial
+    configurations at that composition (seeded)."""
+    import numpy as np
+    n = shape[0] * shape[1]
+    out, occs = [], []
+    for a in range(8):
+        for b in range(8):
+            out.append(dict(shape=shape, temperature=500.0 + a * 2500.0 / 7.0,
+                            seed=2000 + 8 * a + b))
+            s = -np.ones(n, dtype=np.int8)
+            s[np.random.default_rng(2000 + 8 * a + b).permutation(n)[:round((0.1 + 0.1 * b) * n)]] = 1
+            occs.append(s)
+    return out, occs
+
+
+def bench_canonical(passes):
+    """Canonical pair-swap chains (CANONICAL mode, bit-exact): 64 chains of 128x128
+    in one launch vs one chain alone, and the oracle's rate on one host core."""
+    import casmcode_monte_amd as cm
+    conds, occs = canonical_conditions()
+    r = cm.Run(conds, mode=cm.MODE_CANONICAL, J1=0.1)
+    for i, o in enumerate(occs):
+        r.set_occupation(i, o)
+    r.run(2)
+    w, k, a = timed_run(r, passes)
+    out = dict(chains=len(conds), passes=passes, wall_s=w, kernel_s=k, attempts=a,
+               attempts_per_s=a / w)
+    r.close()
+    p = cm.Run(conds[:1], mode=cm.MODE_CANONICAL, J1=0.1)
+    p.set_occupation(0, occs[0])
+    p.run(1)
+    pp = max(passes // 10, 10)
+    pw, pk, pa = timed_run(p, pp)
+    out["plain_attempts_per_s"] = pa / pw
+    out["plain_passes"] = pp
+    out["batched_over_plain"] = out["attempts_per_s"] / out["plain_attempts_per_s"]
+    p.close()
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as O
+    cp = 20
+    t0 = time.perf_counter()
+    O.canonical_run(128, 128, 0.1, conds[0]["temperature"], cp, occs[0], O.mt64_state(2000, 0))
+    cw = time.perf_counter() - t0
+    out["cpu_port"] = dict(attempts_per_s=cp * 128 * 128 / cw, cores=1,
+                           sample=f"1 chain 128x128, {cp} passes, oracle restatement")
+    return out
+
+
 def bench_checker_scan(passes):
     """64-condition 128^2 scan: batched LDS-resident launch vs one chain at a time."""
     import casmcode_monte_amd as cm
@@ -258,6 +307,10 @@ def main():
         print(f"[bench] fcc batch done: {fc['attempts_per_s']:.3e} attempts/s", file=sys.stderr,
               flush=True)
         line["fcc_cluster_expansion_256_chains"] = fc
+        cn = bench_canonical(max(K // 4, 20))
+        print(f"[bench] canonical batch done: {cn['attempts_per_s']:.3e} attempts/s",
+              file=sys.stderr, flush=True)
+        line["canonical_exact_64_chains"] = cn
         line["checkerboard"] = {
             "scan_128x128_64_conditions": scan,
             "square_4096x4096_J1J2_16T": fam2,

@@ -17,6 +17,7 @@ INCLUDE_DIR = os.path.join(os.path.dirname(HERE), "include")
 
 MODE_EXACT = 0
 MODE_CHECKERBOARD = 1
+MODE_CANONICAL = 2
 LATTICE_SQUARE = 2
 LATTICE_SC = 3
 LATTICE_FCC = 4
@@ -36,7 +37,8 @@ EXPORTED = [
     "mcg_rng_dump", "mcg_rng_load", "mcg_run", "mcg_set_active", "mcg_get_counters",
     "mcg_n_samples", "mcg_get_samples", "mcg_clear_samples", "mcg_stats_compute",
     "mcg_equilibration", "mcg_last_run_time", "mcg_assign_chain", "mcg_autotune",
-    "mcg_get_schedule", "mcg_set_schedule", "mcg_reserve_samples",
+    "mcg_get_schedule", "mcg_set_schedule", "mcg_reserve_samples", "mcg_get_occ_location",
+    "mcg_set_occ_location",
 ]
 
 
@@ -105,6 +107,8 @@ def lib():
         L.mcg_reserve_samples.argtypes = [C.c_void_p, C.c_int64]
         L.mcg_get_schedule.argtypes = [C.c_void_p, P(Schedule)]
         L.mcg_set_schedule.argtypes = [C.c_void_p, P(Schedule)]
+        L.mcg_get_occ_location.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), C.c_int64]
+        L.mcg_set_occ_location.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), C.c_int64]
         _lib = L
     return _lib
 
@@ -184,8 +188,20 @@ class Run:
         _check(lib().mcg_get_occupation(self.h, c, o.ctypes.data_as(C.POINTER(C.c_int8)), o.size))
         return o
 
+    def occ_location(self, c):
+        """CANONICAL: the candidate lists (species-0 sites in list order, then species 1)."""
+        v = np.zeros(self.n_sites(c), dtype=np.int64)
+        _check(lib().mcg_get_occ_location(self.h, c, v.ctypes.data_as(C.POINTER(C.c_int64)),
+                                          v.size))
+        return v
+
+    def set_occ_location(self, c, loc):
+        v = np.ascontiguousarray(loc, dtype=np.int64)
+        _check(lib().mcg_set_occ_location(self.h, c, v.ctypes.data_as(C.POINTER(C.c_int64)),
+                                          v.size))
+
     def rng_dump(self, c):
-        n = 313 if self.mode == MODE_EXACT else 4
+        n = 313 if self.mode in (MODE_EXACT, MODE_CANONICAL) else 4
         w = np.zeros(n, dtype=np.uint64)
         _check(lib().mcg_rng_dump(self.h, c, w.ctypes.data_as(C.POINTER(C.c_uint64)), n))
         return w
@@ -294,7 +310,7 @@ class Run:
             N1 = rec["M"].astype(np.float64)
             return dict(formation_energy=Ef / N, param_composition=N1 / N,
                         potential_energy=(Ef - mu * N1) / N)
-        if self.mode == MODE_EXACT and not self.use_nlist:
+        if self.mode in (MODE_EXACT, MODE_CANONICAL) and not self.use_nlist:
             Ef = rec["e_fold"].astype(np.float64)
         else:
             Ef = rec["S1"].astype(np.float64) * np.float64(-self.J1)
@@ -302,5 +318,7 @@ class Run:
                 Ef = Ef + rec["S2"].astype(np.float64) * np.float64(-self.J2)
         Nx = (np.int64(self.n_sites(c)) + rec["M"]).astype(np.float64) / np.float64(2.0)
         mu = np.float64(self.chains[c].get("exchange_potential", 0.0))
+        if self.mode == MODE_CANONICAL:
+            mu = np.float64(0.0)  # the canonical potential is the formation energy
         return dict(formation_energy=Ef / N, param_composition=Nx / N,
                     potential_energy=(Ef - mu * Nx) / N)

@@ -254,12 +254,22 @@ struct mcg_handle {
   int64_t stats_cap = 0;
   int64_t *d_eq = nullptr;
 
-  // EXACT
+  // EXACT (and CANONICAL, which shares the state records d_xs)
   ExactParams *d_xp = nullptr;
   ExactState *d_xs = nullptr;
   std::vector<ExactParams> xp;
   int exact_lds = 0;
   bool exact_in_lds = true;
+
+  // CANONICAL: per-chain parameters and the candidate lists (int32 entries at
+  // the lattice offsets: species-0 sites then species-1 sites)
+  CanonParams *d_kp = nullptr;
+  std::vector<CanonParams> kp;
+  int32_t *d_list = nullptr;
+  int canon_lds = 0;
+  bool canonical() const { return mode == MCG_MODE_CANONICAL; }
+  // one sequential chain per wavefront, MT19937-64 state, native lattice layout
+  bool sequential() const { return mode == MCG_MODE_EXACT || mode == MCG_MODE_CANONICAL; }
 
   // CHECKERBOARD
   CheckerParams *d_cp = nullptr;
@@ -299,7 +309,7 @@ struct mcg_handle {
     if (stream) hipStreamSynchronize(stream);
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     void *ptrs[] = {d_lat, d_active, d_series, d_stats, d_eq, d_xp, d_xs, d_cp, d_cs, d_tab,
-                    d_work, d_ftab, d_tri, d_band, d_pair};
+                    d_work, d_ftab, d_tri, d_band, d_pair, d_kp, d_list};
     for (void *p : ptrs)
       if (p) hipFree(p);
     if (ev0) hipEventDestroy(ev0);
@@ -423,6 +433,96 @@ int build_exact(mcg_handle *h) {
   }
   HIP_TRY(copy_sync(h, h->d_xs, xs.data(), sizeof(ExactState) * h->n_chains,
                     hipMemcpyHostToDevice));
+  return MCG_OK;
+}
+
+// Parameters of one CANONICAL chain with nA sites of species 0 (spin -1): the
+// 25 swap classes (Sa, Sb) of the pair delta model.hh:354-379 (site a
+// -1 -> +1 first, then site b +1 -> -1 with a already flipped).
+void fill_canonical_params(mcg_handle *h, int c, int64_t nA, CanonParams &P) {
+  auto const &ch = h->chains[c];
+  std::memset(&P, 0, sizeof(P));
+  P.rows = (int32_t)ch.shape[0];
+  P.cols = (int32_t)ch.shape[1];
+  P.n = (int32_t)(ch.shape[0] * ch.shape[1]);
+  P.nA = (int32_t)nA;
+  P.nB = P.n - P.nA;
+  P.fold = h->use_nlist ? 0 : 1;
+  P.self_nb = 2 * (P.rows == 1) + 2 * (P.cols == 1);
+  P.window = (h->options & MCG_OPT_EXACT_SCALAR_WALK) ? 1 : 64;
+  uint32_t marks = 64;
+  while (marks < (uint32_t)P.n && marks < 8192u) marks <<= 1;
+  P.mark_mask = marks - 1;
+  P.div_magic = ((1ULL << 40) + (uint64_t)P.rows - 1) / (uint64_t)P.rows;
+  P.negJ = -h->J1;
+  P.lat_off = h->lat_off[c];
+  const double beta = 1.0 / (h->kb * ch.temperature);  // occupation_metropolis.hh:108-109
+  for (int ia = 0; ia < 5; ++ia)
+    for (int ib = 0; ib < 5; ++ib) {
+      const int Sa = 2 * ia - 4, Sb = 2 * ib - 4, k = ia * 5 + ib;
+      // model.hh:305-314 on spins: -J * (new - old) * (neighbour sum), summed
+      // from 0.0 in event order (model.hh:362-369)
+      double dE = 0.0;
+      dE += -h->J1 * (1 - (-1)) * Sa;
+      dE += -h->J1 * ((-1) - 1) * Sb;
+      P.dq[k] = dE / (double)P.n;  // occupation_metropolis.hh:141
+      if (dE < 0.0) {
+        P.neg_mask |= 1u << k;
+        continue;
+      }
+      uint64_t thr = 0;
+      if (threshold_for(std::exp(-dE * beta), &thr))  // metropolis.hh:33
+        P.thr[k] = thr;
+      else
+        P.full_mask |= 1u << k;
+    }
+}
+
+int build_canonical(mcg_handle *h) {
+  h->kp.resize(h->n_chains);
+  int max_lds = 0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    fill_canonical_params(h, c, 0, h->kp[c]);
+    max_lds = std::max(max_lds, canon_lds_bytes(h->kp[c]));
+  }
+  if (max_lds > EXACT_LDS_MAX)
+    return fail(MCG_ERR_ARG,
+                "CANONICAL mode keeps a chain's lattice, lists and conflict marks in LDS: "
+                "lattice too large");
+  h->canon_lds = max_lds;
+  HIP_TRY(canonical_set_max_lds((int)EXACT_LDS_MAX));
+  HIP_TRY(hipMalloc(&h->d_kp, sizeof(CanonParams) * h->n_chains));
+  HIP_TRY(hipMalloc(&h->d_xs, sizeof(ExactState) * h->n_chains));
+  HIP_TRY(hipMalloc(&h->d_list, sizeof(int32_t) * (size_t)h->lat_bytes));
+  HIP_TRY(copy_sync(h, h->d_kp, h->kp.data(), sizeof(CanonParams) * h->n_chains,
+                    hipMemcpyHostToDevice));
+  std::vector<ExactState> xs(h->n_chains);
+  for (int c = 0; c < h->n_chains; ++c) {
+    ExactState &S = xs[c];
+    std::memset(&S, 0, sizeof(S));
+    int pos = 0;
+    mt_seed(h->chains[c].seed, S.mt, &pos);
+    S.pos = pos;
+  }
+  HIP_TRY(copy_sync(h, h->d_xs, xs.data(), sizeof(ExactState) * h->n_chains,
+                    hipMemcpyHostToDevice));
+  return MCG_OK;
+}
+
+// OccLocation::initialize (OccLocation.cc:39-118) for one chain: both lists in
+// site order; uploads them and the list sizes.
+int canonical_init_lists(mcg_handle *h, int64_t c, std::vector<int8_t> const &occ) {
+  std::vector<int32_t> lists;
+  lists.reserve(occ.size());
+  for (int64_t l = 0; l < (int64_t)occ.size(); ++l)
+    if (occ[l] < 0) lists.push_back((int32_t)l);
+  const int64_t nA = (int64_t)lists.size();
+  for (int64_t l = 0; l < (int64_t)occ.size(); ++l)
+    if (occ[l] > 0) lists.push_back((int32_t)l);
+  HIP_TRY(copy_sync(h, h->d_list + h->lat_off[c], lists.data(), sizeof(int32_t) * lists.size(),
+                    hipMemcpyHostToDevice));
+  fill_canonical_params(h, (int)c, nA, h->kp[c]);
+  HIP_TRY(copy_sync(h, h->d_kp + c, &h->kp[c], sizeof(CanonParams), hipMemcpyHostToDevice));
   return MCG_OK;
 }
 
@@ -602,16 +702,17 @@ int reset_observables(mcg_handle *h, int64_t c, std::vector<int8_t> const &occ) 
   LatticeGeom g{h->lattice == MCG_LATTICE_SC ? 3 : 2, {ch.shape[0], ch.shape[1], ch.shape[2]}};
   int64_t S1, S2, M;
   integer_observables(occ, g, h->J2 != 0.0, S1, S2, M);
-  if (h->mode == MCG_MODE_EXACT) {
+  if (h->sequential()) {
     ExactState st;
     HIP_TRY(copy_sync(h, &st, h->d_xs + c, sizeof(ExactState), hipMemcpyDeviceToHost));
     st.S = S1;
     st.M = M;
     // occupation_metropolis.hh:103-106: potential per_supercell / n_unitcells
+    // (the canonical potential is the formation energy alone)
     const int64_t n = g.n();
     const double Ef = ising_per_supercell(occ, ch.shape[0], ch.shape[1], h->J1, h->use_nlist);
     const double Nx = static_cast<double>(n + M) / 2.0;
-    const double E = Ef - ch.exchange_potential * Nx;
+    const double E = h->canonical() ? Ef : Ef - ch.exchange_potential * Nx;
     st.e_run = E / (double)n;
     HIP_TRY(copy_sync(h, h->d_xs + c, &st, sizeof(ExactState), hipMemcpyHostToDevice));
   } else {
@@ -848,7 +949,8 @@ const char *mcg_last_error(void) { return g_err.c_str(); }
 int mcg_create(const mcg_desc *desc, mcg_handle **out) {
   if (!desc || !out) return fail(MCG_ERR_ARG, "null argument");
   *out = nullptr;
-  if (desc->mode != MCG_MODE_EXACT && desc->mode != MCG_MODE_CHECKERBOARD)
+  if (desc->mode != MCG_MODE_EXACT && desc->mode != MCG_MODE_CHECKERBOARD &&
+      desc->mode != MCG_MODE_CANONICAL)
     return fail(MCG_ERR_ARG, "unknown mode");
   if (desc->lattice != MCG_LATTICE_SQUARE && desc->lattice != MCG_LATTICE_SC &&
       desc->lattice != MCG_LATTICE_FCC)
@@ -858,6 +960,9 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
     return fail(MCG_ERR_ARG, "the FCC cluster expansion runs in CHECKERBOARD mode with 6 ECIs");
   if (desc->mode == MCG_MODE_EXACT && (desc->lattice != MCG_LATTICE_SQUARE || desc->J2 != 0.0))
     return fail(MCG_ERR_ARG, "EXACT mode supports the 2D square nearest-neighbour model only");
+  if (desc->mode == MCG_MODE_CANONICAL &&
+      (desc->lattice != MCG_LATTICE_SQUARE || desc->J2 != 0.0))
+    return fail(MCG_ERR_ARG, "CANONICAL mode supports the 2D square nearest-neighbour model only");
   if (desc->n_chains <= 0 || desc->n_chains > (1 << 24) || !desc->chains)
     return fail(MCG_ERR_ARG, "n_chains out of range");
   if (desc->sample_period < 1) return fail(MCG_ERR_ARG, "sample_period < 1");
@@ -907,6 +1012,10 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
       delete h;
       return fail(MCG_ERR_ARG, "EXACT mode supports at most 2^24 sites and rows < 65536");
     }
+    if (h->canonical() && n > (1LL << 16)) {
+      delete h;
+      return fail(MCG_ERR_ARG, "CANONICAL mode supports at most 65536 sites per chain");
+    }
     if (n > (1LL << 31)) {
       delete h;
       return fail(MCG_ERR_ARG, "too many sites in one chain");
@@ -938,7 +1047,10 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
   for (int c = 0; c < h->n_chains; ++c) h->capacity.push_back(h->sites(c));
   if (hipMalloc(&h->d_eq, 2 * sizeof(int64_t)) != hipSuccess)
     return bail(fail(MCG_ERR_HIP, "allocation failed"));
-  rc = (h->mode == MCG_MODE_EXACT) ? build_exact(h) : (h->fcc() ? build_fcc(h) : build_checker(h));
+  rc = (h->mode == MCG_MODE_EXACT)  ? build_exact(h)
+       : h->canonical()              ? build_canonical(h)
+       : h->fcc()                    ? build_fcc(h)
+                                     : build_checker(h);
   if (rc != MCG_OK) return bail(rc);
   // initial lattices: set observables
   for (int c = 0; c < h->n_chains; ++c) {
@@ -966,8 +1078,12 @@ int mcg_set_occupation(mcg_handle *h, int64_t chain, const int8_t *occ, int64_t 
       if (s != 1 && s != -1) return fail(MCG_ERR_ARG, "occupation values must be +1 or -1");
   }
   HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->mode == MCG_MODE_EXACT || h->fcc()) {
+  if (h->sequential() || h->fcc()) {
     HIP_TRY(copy_sync(h, h->d_lat + h->lat_off[chain], v.data(), n, hipMemcpyHostToDevice));
+    if (h->canonical()) {
+      const int rc = canonical_init_lists(h, chain, v);
+      if (rc != MCG_OK) return rc;
+    }
   } else {
     std::vector<int8_t> dev(n);
     to_split_rows(v, h->chains[chain].shape[0], dev);
@@ -980,7 +1096,7 @@ int mcg_get_occupation(mcg_handle *h, int64_t chain, int8_t *occ, int64_t n) {
   if (!valid_chain(h, chain) || !occ) return fail(MCG_ERR_ARG, "bad chain or buffer");
   if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "occupation size mismatch");
   HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->mode == MCG_MODE_EXACT || h->fcc()) {
+  if (h->sequential() || h->fcc()) {
     HIP_TRY(copy_sync(h, occ, h->d_lat + h->lat_off[chain], n, hipMemcpyDeviceToHost));
   } else {
     std::vector<int8_t> dev(n), nat(n);
@@ -994,7 +1110,7 @@ int mcg_get_occupation(mcg_handle *h, int64_t chain, int8_t *occ, int64_t n) {
 int mcg_rng_dump(mcg_handle *h, int64_t chain, uint64_t *words, int64_t n_words) {
   if (!valid_chain(h, chain) || !words) return fail(MCG_ERR_ARG, "bad chain or buffer");
   HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->mode == MCG_MODE_EXACT) {
+  if (h->sequential()) {
     if (n_words < MT_N + 1) return fail(MCG_ERR_CAPACITY, "need 313 words");
     ExactState st;
     HIP_TRY(copy_sync(h, &st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
@@ -1015,7 +1131,7 @@ int mcg_rng_dump(mcg_handle *h, int64_t chain, uint64_t *words, int64_t n_words)
 int mcg_rng_load(mcg_handle *h, int64_t chain, const uint64_t *words, int64_t n_words) {
   if (!valid_chain(h, chain) || !words) return fail(MCG_ERR_ARG, "bad chain or buffer");
   HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->mode == MCG_MODE_EXACT) {
+  if (h->sequential()) {
     if (n_words < MT_N + 1) return fail(MCG_ERR_CAPACITY, "need 313 words");
     if (words[MT_N] > (uint64_t)MT_N) return fail(MCG_ERR_ARG, "engine position out of range");
     ExactState st;
@@ -1050,6 +1166,9 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
     return fail(MCG_ERR_CAPACITY, "shape has more sites than the chain slot holds");
   if (h->mode == MCG_MODE_EXACT && nd.shape[0] >= (1LL << 16))
     return fail(MCG_ERR_ARG, "EXACT mode supports rows < 65536");
+  if (h->canonical() && n > (1LL << 16))
+    return fail(MCG_ERR_ARG, "CANONICAL mode supports at most 65536 sites per chain");
+
   HIP_TRY(hipStreamSynchronize(h->stream));
   const mcg_chain_desc old = h->chains[chain];
   h->chains[chain] = nd;
@@ -1063,6 +1182,22 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
     }
     h->xp[chain] = P;
     HIP_TRY(copy_sync(h, h->d_xp + chain, &P, sizeof(P), hipMemcpyHostToDevice));
+    ExactState S;
+    std::memset(&S, 0, sizeof(S));
+    int pos = 0;
+    mt_seed(nd.seed, S.mt, &pos);
+    S.pos = pos;
+    HIP_TRY(copy_sync(h, h->d_xs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
+  } else if (h->canonical()) {
+    CanonParams P;
+    fill_canonical_params(h, (int)chain, 0, P);
+    if (canon_lds_bytes(P) > h->canon_lds) {
+      h->chains[chain] = old;
+      return fail(MCG_ERR_CAPACITY, "shape needs more LDS than the handle was created with");
+    }
+    h->kp[chain] = P;
+    HIP_TRY(copy_sync(h, h->d_kp + chain, &h->kp[chain], sizeof(CanonParams),
+                      hipMemcpyHostToDevice));
     ExactState S;
     std::memset(&S, 0, sizeof(S));
     int pos = 0;
@@ -1113,6 +1248,42 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
   return reset_observables(h, chain, std::vector<int8_t>(n, (int8_t)fill));
 }
 
+int mcg_get_occ_location(mcg_handle *h, int64_t chain, int64_t *loc, int64_t n) {
+  if (!valid_chain(h, chain) || !loc) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (!h->canonical()) return fail(MCG_ERR_STATE, "occupant lists exist in CANONICAL mode only");
+  if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "list size mismatch");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  std::vector<int32_t> v(n);
+  HIP_TRY(copy_sync(h, v.data(), h->d_list + h->lat_off[chain], sizeof(int32_t) * n,
+                    hipMemcpyDeviceToHost));
+  std::copy(v.begin(), v.end(), loc);
+  return MCG_OK;
+}
+
+int mcg_set_occ_location(mcg_handle *h, int64_t chain, const int64_t *loc, int64_t n) {
+  if (!valid_chain(h, chain) || !loc) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (!h->canonical()) return fail(MCG_ERR_STATE, "occupant lists exist in CANONICAL mode only");
+  if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "list size mismatch");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  std::vector<int8_t> occ(n);
+  HIP_TRY(copy_sync(h, occ.data(), h->d_lat + h->lat_off[chain], n, hipMemcpyDeviceToHost));
+  const int64_t nA = h->kp[chain].nA;
+  std::vector<uint8_t> seen(n, 0);
+  std::vector<int32_t> v(n);
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t l = loc[i];
+    if (l < 0 || l >= n || seen[l])
+      return fail(MCG_ERR_ARG, "occupant lists must hold every site exactly once");
+    if ((occ[l] < 0) != (i < nA))
+      return fail(MCG_ERR_ARG, "occupant lists do not match the occupation");
+    seen[l] = 1;
+    v[i] = (int32_t)l;
+  }
+  HIP_TRY(copy_sync(h, h->d_list + h->lat_off[chain], v.data(), sizeof(int32_t) * n,
+                    hipMemcpyHostToDevice));
+  return MCG_OK;
+}
+
 int mcg_set_active(mcg_handle *h, const uint8_t *active, int64_t n_chains) {
   if (!h || !active || n_chains != h->n_chains) return fail(MCG_ERR_ARG, "bad active mask");
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1141,7 +1312,12 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
   double attempts = 0.0;
   for (int c = 0; c < h->n_chains; ++c)
     if (h->active[c]) attempts += (double)h->sites(c) * (double)n_passes;
-  const bool hbm_path = h->mode != MCG_MODE_EXACT && !h->checker_lds && !h->fcc();
+  if (h->canonical())
+    for (int c = 0; c < h->n_chains; ++c)
+      if (h->active[c] && (h->kp[c].nA == 0 || h->kp[c].nB == 0))
+        // OccEventProposal.hh:136-139 (occupation_metropolis.hh:121-127 stops the run)
+        return fail(MCG_ERR_STATE, "Error in choose_canonical_swap: No events possible.");
+  const bool hbm_path = !h->sequential() && !h->checker_lds && !h->fcc();
   const int G = h->graph_passes;
   if (hbm_path) {
     if (h->work_dirty) {
@@ -1160,6 +1336,10 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
                  h->n_chains, n_passes, h->sample_period, h->exact_lds, h->exact_in_lds,
                  !(h->options & MCG_OPT_EXACT_SCALAR_WALK),
                  h->stream);
+  } else if (h->canonical()) {
+    launch_canonical(h->d_kp, h->d_xs, h->d_lat, h->d_list, h->d_series, h->series_cap,
+                     h->d_active, h->n_chains, n_passes, h->sample_period, h->canon_lds,
+                     h->stream);
   } else if (h->fcc()) {
     HIP_TRY(hipGetLastError());
     launch_fcc(h->d_cp, h->d_cs, h->d_lat, h->d_ftab, h->d_tri, h->d_series, h->series_cap,
@@ -1201,7 +1381,7 @@ int mcg_get_counters(mcg_handle *h, int64_t chain, int64_t *n_pass, int64_t *n_a
                      int64_t *n_reject) {
   if (!valid_chain(h, chain)) return fail(MCG_ERR_ARG, "bad chain");
   HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->mode == MCG_MODE_EXACT) {
+  if (h->sequential()) {
     ExactState st;
     HIP_TRY(copy_sync(h, &st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
     if (n_pass) *n_pass = st.n_pass;
@@ -1244,7 +1424,7 @@ int mcg_reserve_samples(mcg_handle *h, int64_t n_records) {
 int mcg_clear_samples(mcg_handle *h) {
   if (!h) return fail(MCG_ERR_ARG, "null handle");
   HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->mode == MCG_MODE_EXACT) {
+  if (h->sequential()) {
     std::vector<ExactState> xs(h->n_chains);
     HIP_TRY(copy_sync(h, xs.data(), h->d_xs, sizeof(ExactState) * h->n_chains,
                       hipMemcpyDeviceToHost));
@@ -1268,8 +1448,8 @@ static ObsParams obs_params(mcg_handle *h, int64_t chain, int64_t observable) {
   op.n_sites = h->sites(chain);
   op.negJ1 = -h->J1;
   op.negJ2 = -h->J2;
-  op.mu = h->chains[chain].exchange_potential;
-  op.fold = (h->mode == MCG_MODE_EXACT && !h->use_nlist) ? 1 : 0;
+  op.mu = h->canonical() ? 0.0 : h->chains[chain].exchange_potential;
+  op.fold = (h->sequential() && !h->use_nlist) ? 1 : 0;
   op.observable = (int32_t)observable;
   op.fcc = h->fcc() ? 1 : 0;
   op.pad = 0;
@@ -1369,7 +1549,7 @@ int mcg_set_schedule(mcg_handle *h, const mcg_schedule *s) {
 
 int mcg_autotune(mcg_handle *h, int64_t trial_passes, mcg_schedule *chosen) {
   if (!h) return fail(MCG_ERR_ARG, "null handle");
-  if (h->mode == MCG_MODE_EXACT) return chosen ? mcg_get_schedule(h, chosen) : MCG_OK;
+  if (h->sequential()) return chosen ? mcg_get_schedule(h, chosen) : MCG_OK;
   const std::string key = tune_key(h);
   mcg_schedule best{};
   if (!tune_cache_lookup(key, &best)) {

@@ -21,18 +21,9 @@
 // int->double conversion happens per attempt.  The FP64 tails (running energy
 // and the row/column formation-energy fold) use explicitly rounded operations.
 #include "internal.hh"
+#include "walk.hh"
 
 namespace mcg {
-
-__device__ __forceinline__ uint64_t rfl64(uint64_t v) {
-  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ int32_t rfl32(int32_t v) {
-  return __builtin_amdgcn_readfirstlane(v);
-}
 
 // Twist of the 312-word state by one wavefront.  Phase 1 (k < 156) reads only
 // old words; phase 2 (k >= 156) reads old x[k], x[k+1] and new x[k-156], x[0].
@@ -230,35 +221,8 @@ __global__ __launch_bounds__(64) void exact_kernel(const ExactParams *__restrict
 // one-attempt scalar path.
 // ---------------------------------------------------------------------------
 constexpr uint32_t SPIN_BIT = 1u, MARK_A = 2u, MARK_B = 4u;
-constexpr int RING = 2 * MT_N;
-
-// Out-of-place twist of generation src into dst and temper into the ring slot.
-__device__ void twist_gen(const uint64_t *src, uint64_t *dst, uint64_t *slot, int lane) {
-  for (int k = lane; k < MT_N - MT_M; k += 64)
-    dst[k] = mt_twist_word(src[k], src[k + 1], src[k + MT_M]);
-  __syncthreads();
-  for (int k = MT_N - MT_M + lane; k < MT_N - 1; k += 64)
-    dst[k] = mt_twist_word(src[k], src[k + 1], dst[k - (MT_N - MT_M)]);
-  if (lane == 0) dst[MT_N - 1] = mt_twist_word(src[MT_N - 1], dst[0], dst[MT_M - 1]);
-  __syncthreads();
-  for (int k = lane; k < MT_N; k += 64) slot[k] = mt_temper(dst[k]);
-  __syncthreads();
-}
 
 __device__ __forceinline__ int spin_of(uint32_t b) { return (b & SPIN_BIT) ? 1 : -1; }
-
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
-  uint64_t u = __double_as_longlong(v);
-  uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, lane);
-  uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), lane);
-  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-
-__device__ __forceinline__ int wave_isum(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
 __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__restrict__ Pp,
                                                          ExactState *__restrict__ Sp,

@@ -50,6 +50,45 @@ void launch_exact(const ExactParams *P, ExactState *S, int8_t *lat_pool, mcg_sam
                   hipStream_t stream);
 
 // ---------------------------------------------------------------------------
+// CANONICAL mode: pair swaps at fixed composition, one chain per wavefront.
+// ---------------------------------------------------------------------------
+
+// Classes of a swap (site a: -1 -> +1, then site b: +1 -> -1): the neighbour
+// sum of a before the swap and the neighbour sum of b after a has flipped,
+// k = ((Sa + 4) / 2) * 5 + (Sb + 4) / 2.
+constexpr int CANON_NCLASS = 25;
+
+struct CanonParams {
+  int32_t rows, cols, n;
+  int32_t nA, nB;        // sizes of the species-0 (spin -1) and species-1 (+1) lists
+  int32_t fold;          // 1: compute the row/column-fold formation energy at samples
+  int32_t self_nb;       // neighbour slots that are the site itself
+  int32_t window;        // stream positions evaluated per block
+  uint32_t mark_mask;    // conflict marks: entry (site & mark_mask)
+  uint32_t neg_mask;     // bit k: dE < 0, accept without drawing
+  uint32_t full_mask;    // bit k: every drawn word accepts
+  int32_t pad;
+  uint64_t div_magic;    // floor(l / rows) = (l * div_magic) >> 40
+  uint64_t thr[CANON_NCLASS];
+  double dq[CANON_NCLASS];  // fl(dE_k / n_unitcells)
+  double negJ;
+  int64_t lat_off;       // lattice byte offset; the lists start at entry lat_off of the list pool
+};
+
+// LDS bytes of one chain: lattice, conflict marks, the two lists, fold scratch.
+inline int canon_lds_bytes(const CanonParams &P) {
+  return ((P.n + 15) & ~15) + 4 * (int)(P.mark_mask + 1) + ((2 * P.n + 15) & ~15) +
+         4 * (P.rows + P.cols);
+}
+
+void launch_canonical(const CanonParams *P, ExactState *S, int8_t *lat_pool, int32_t *list_pool,
+                      mcg_sample *series, int64_t series_cap, const uint8_t *active,
+                      int n_chains, int64_t n_passes, int64_t sample_period, int lds_bytes,
+                      hipStream_t stream);
+// Dynamic LDS limit of the canonical kernel (its static LDS is ~10 KB).
+hipError_t canonical_set_max_lds(int bytes);
+
+// ---------------------------------------------------------------------------
 // CHECKERBOARD mode
 // ---------------------------------------------------------------------------
 

@@ -374,7 +374,8 @@ PYBIND11_MODULE(_libcasm_monte, m) {
       .def_readonly("n_unitcells", &IsingConfiguration::n_unitcells)
       .def("occ", &IsingConfiguration::occ)
       .def("set_occ", &IsingConfiguration::set_occ)
-      .def("occupation", &IsingConfiguration::occupation)
+      .def("occupation",
+           [](IsingConfiguration const &c) { return std::vector<int>(c.occupation()); })
       .def("set_occupation", &IsingConfiguration::set_occupation);
 
   py::class_<IsingState>(mi, "IsingState")

@@ -55,6 +55,15 @@ class Handle {
     check(mcg_get_occupation(m_h, chain, occ.data(), n));
     return occ;
   }
+  /// CANONICAL: the candidate lists, species-0 sites then species-1 sites
+  std::vector<int64_t> occ_location(Index chain, Index n) const {
+    std::vector<int64_t> loc(n);
+    check(mcg_get_occ_location(m_h, chain, loc.data(), n));
+    return loc;
+  }
+  void set_occ_location(Index chain, std::vector<int64_t> const &loc) {
+    check(mcg_set_occ_location(m_h, chain, loc.data(), (int64_t)loc.size()));
+  }
   void load_engine(Index chain, std::mt19937_64 const &e) {
     auto w = engine_words(e);
     check(mcg_rng_load(m_h, chain, w.data(), (int64_t)w.size()));

@@ -23,40 +23,14 @@
 #ifndef CASM_MI355X_monte_ising_cpp_basic_semigrand_canonical
 #define CASM_MI355X_monte_ising_cpp_basic_semigrand_canonical
 
+#include "casm/monte/RandomNumberGenerator.hh"
 #include "casm/monte/checks/CompletionCheck.hh"
+#include "casm/monte/events/OccEvent.hh"
 #include "casm/monte/gpu/Handle.hh"
 #include "casm/monte/ising_cpp/model.hh"
 
 namespace CASM {
 namespace monte {
-
-/// Occupant change proposed by an event generator (events/OccEvent.hh:55-71).
-struct OccEvent {
-  std::vector<Index> linear_site_index;
-  std::vector<int> new_occ;
-};
-
-/// RandomNumberGenerator.hh:14-42
-template <typename EngineType = default_engine_type>
-struct RandomNumberGenerator {
-  std::shared_ptr<EngineType> engine;
-  RandomNumberGenerator(std::shared_ptr<EngineType> _engine = std::shared_ptr<EngineType>())
-      : engine(_engine) {
-    if (engine == nullptr) {
-      engine = std::make_shared<EngineType>();
-      std::random_device device;
-      engine->seed(device());
-    }
-  }
-  template <typename IntType>
-  IntType random_int(IntType maximum_value) {
-    return std::uniform_int_distribution<IntType>(0, maximum_value)(*engine);
-  }
-  template <typename RealType>
-  RealType random_real(RealType maximum_value) {
-    return std::uniform_real_distribution<RealType>(0, maximum_value)(*engine);
-  }
-};
 
 namespace methods {
 

@@ -25,6 +25,9 @@ class IsingConfiguration {
   Index n_sites, n_variable_sites, n_unitcells;
 
   std::vector<int> const &occupation() const { return m_occupation; }
+  /// mutable occupation, the form OccLocation::apply updates (through
+  /// get_occupation in canonical runs)
+  std::vector<int> &occupation() { return m_occupation; }
   void set_occupation(std::vector<int> const &occ) {
     if (occ.size() != m_occupation.size())
       throw std::runtime_error("Error in set_occupation: size mismatch");

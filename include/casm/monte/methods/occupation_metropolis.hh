@@ -13,10 +13,18 @@
 // The device runs whole passes between the points where a fixture samples or
 // a completion check can change its answer (see basic_semigrand_canonical.hh
 // for the argument); sampling by step or by time is not supported.
+//
+// Two entry points: the SGC Ising potential with single-site flips (EXACT
+// mode), and the reference's generic signature with an OccLocation and
+// propose_canonical_event (canonical pair swaps, CANONICAL mode).
 #ifndef CASM_MI355X_monte_methods_occupation_metropolis
 #define CASM_MI355X_monte_methods_occupation_metropolis
 
+#include <iostream>
+#include <type_traits>
+
 #include "casm/monte/ising_cpp/basic_semigrand_canonical.hh"
+#include "casm/monte/ising_cpp/canonical.hh"
 #include "casm/monte/run_management/RunManager.hh"
 
 namespace CASM {
@@ -43,40 +51,18 @@ class IsingSemiGrandCanonicalPotential {
   State<ising_cpp::IsingConfiguration> const *m_state = nullptr;
 };
 
-template <typename StatisticsType, typename EngineType>
-void occupation_metropolis(
-    State<ising_cpp::IsingConfiguration> &state, IsingSemiGrandCanonicalPotential &potential,
-    RandomNumberGenerator<EngineType> &random_number_generator,
-    RunManager<ising_cpp::IsingConfiguration, StatisticsType, EngineType> &run_manager) {
-  if (potential.state() != &state)
-    throw std::runtime_error(
-        "Error in monte::occupation_metropolis: potential not set to correct state");
-  auto &config = state.configuration;
-  const double n_unitcells = static_cast<double>(config.n_sites);
-  state.properties.scalar_values["potential_energy"] = 0.;
+namespace detail {
+
+/// Drive a RunManager over device passes of chain 0 of `dev` (one pass = N
+/// steps): the device runs whole passes between the points where a fixture
+/// samples or a completion check can change its answer; counters, the running
+/// potential energy and, when a sampling function reads it, the configuration
+/// (sync_configuration) are brought to the host at those points.
+template <typename ConfigType, typename StatisticsType, typename EngineType, typename SyncF>
+void run_device_passes(State<ConfigType> &state, gpu::Handle &dev, CountType N,
+                       RunManager<ConfigType, StatisticsType, EngineType> &run_manager,
+                       SyncF sync_configuration) {
   double &potential_energy_per_unitcell = state.properties.scalar_values["potential_energy"];
-  potential_energy_per_unitcell = potential.per_supercell() / n_unitcells;
-
-  mcg_chain_desc ch{};
-  ch.shape[0] = config.shape[0];
-  ch.shape[1] = config.shape[1];
-  ch.shape[2] = 1;
-  ch.temperature = state.conditions.scalar_values.at("temperature");
-  ch.exchange_potential = state.conditions.vector_values.at("exchange_potential")(0);
-  mcg_desc d{};
-  d.mode = MCG_MODE_EXACT;
-  d.lattice = MCG_LATTICE_SQUARE;
-  d.use_nlist = potential.use_nlist ? 1 : 0;
-  d.J1 = potential.J;
-  d.kb = KB;
-  d.sample_period = 1;
-  d.n_chains = 1;
-  d.chains = &ch;
-  gpu::Handle dev(d);
-  dev.set_occupation(0, std::vector<int8_t>(config.occupation().begin(), config.occupation().end()));
-  dev.load_engine(0, *random_number_generator.engine);
-
-  const CountType N = config.n_sites;
   run_manager.initialize(N);
   bool host_functions = false;
   for (auto &f : run_manager.sampling_fixtures) {
@@ -91,10 +77,6 @@ void occupation_metropolis(
   run_manager.sample_data_by_count_if_due(state);
 
   int64_t acc_prev = 0, rej_prev = 0;
-  auto sync_configuration = [&]() {
-    auto occ = dev.occupation(0, N);
-    config.set_occupation(std::vector<int>(occ.begin(), occ.end()));
-  };
   while (true) {
     if (run_manager.is_complete()) break;
     // the other calls of this pass only repeat periodic checks on the same samples
@@ -136,6 +118,139 @@ void occupation_metropolis(
     run_manager.sample_data_by_count_if_due(state);
   }
   sync_configuration();
+}
+
+}  // namespace detail
+
+template <typename StatisticsType, typename EngineType>
+void occupation_metropolis(
+    State<ising_cpp::IsingConfiguration> &state, IsingSemiGrandCanonicalPotential &potential,
+    RandomNumberGenerator<EngineType> &random_number_generator,
+    RunManager<ising_cpp::IsingConfiguration, StatisticsType, EngineType> &run_manager) {
+  if (potential.state() != &state)
+    throw std::runtime_error(
+        "Error in monte::occupation_metropolis: potential not set to correct state");
+  auto &config = state.configuration;
+  const double n_unitcells = static_cast<double>(config.n_sites);
+  state.properties.scalar_values["potential_energy"] = 0.;
+  state.properties.scalar_values["potential_energy"] = potential.per_supercell() / n_unitcells;
+
+  mcg_chain_desc ch{};
+  ch.shape[0] = config.shape[0];
+  ch.shape[1] = config.shape[1];
+  ch.shape[2] = 1;
+  ch.temperature = state.conditions.scalar_values.at("temperature");
+  ch.exchange_potential = state.conditions.vector_values.at("exchange_potential")(0);
+  mcg_desc d{};
+  d.mode = MCG_MODE_EXACT;
+  d.lattice = MCG_LATTICE_SQUARE;
+  d.use_nlist = potential.use_nlist ? 1 : 0;
+  d.J1 = potential.J;
+  d.kb = KB;
+  d.sample_period = 1;
+  d.n_chains = 1;
+  d.chains = &ch;
+  gpu::Handle dev(d);
+  dev.set_occupation(0, std::vector<int8_t>(config.occupation().begin(), config.occupation().end()));
+  dev.load_engine(0, *random_number_generator.engine);
+
+  const CountType N = config.n_sites;
+  detail::run_device_passes(state, dev, N, run_manager, [&]() {
+    auto occ = dev.occupation(0, N);
+    config.set_occupation(std::vector<int>(occ.begin(), occ.end()));
+  });
+  dev.store_engine(0, *random_number_generator.engine);
+  run_manager.finalize(state);
+}
+
+/// Canonical runs: the reference's generic signature
+/// (occupation_metropolis.hh:28-35) with propose_canonical_event over an
+/// OccLocation, for the binary square-lattice Ising model of occupant indices
+/// (ising_cpp/canonical.hh).  The chain runs on the MI355X in CANONICAL mode;
+/// at the end the configuration, the OccLocation lists (whose order decides
+/// later draws) and the engine are where the reference's loop leaves them.
+template <typename ProposeOccEventFuntionType, typename GeneratorType, typename StatisticsType,
+          typename EngineType>
+void occupation_metropolis(
+    State<ising_cpp::IsingConfiguration> &state, OccLocation &occ_location,
+    ising_cpp::IsingOccupantPotential &potential, std::vector<OccSwap> const &possible_swaps,
+    ProposeOccEventFuntionType propose_event_f, GeneratorType &random_number_generator,
+    RunManager<ising_cpp::IsingConfiguration, StatisticsType, EngineType> &run_manager) {
+  if (potential.state() != &state)
+    throw std::runtime_error(
+        "Error in monte::occupation_metropolis: potential not set to correct state");
+  typedef OccEvent &(*canonical_f)(OccEvent &, OccLocation const &, std::vector<OccSwap> const &,
+                                   GeneratorType &);
+  bool canonical = false;
+  if constexpr (std::is_convertible_v<ProposeOccEventFuntionType, canonical_f>)
+    canonical = static_cast<canonical_f>(propose_event_f) == &propose_canonical_event<GeneratorType>;
+  if (!canonical)
+    throw std::runtime_error(
+        "occupation_metropolis (MI355X): the device runs propose_canonical_event only");
+  auto const &cv = occ_location.convert();
+  auto const &cl = occ_location.candidate_list();
+  if (cv.asym_size() != 1 || cv.occ_size(0) != 2 || cl.size() != 2 ||
+      possible_swaps.size() != 1 || possible_swaps[0] != OccSwap(cl[0], cl[1]))
+    throw std::runtime_error(
+        "occupation_metropolis (MI355X): canonical runs need one binary asymmetric unit and "
+        "its single canonical swap");
+  auto &config = state.configuration;
+  const CountType N = config.n_sites;
+  if (occ_location.mol_size() != N)
+    throw std::runtime_error("occupation_metropolis (MI355X): every site must mutate");
+  for (Index l = 0; l < N; ++l)
+    if (config.occ(l) != 0 && config.occ(l) != 1)
+      throw std::runtime_error(
+          "occupation_metropolis (MI355X): canonical configurations hold occupant indices 0/1");
+
+  // occupation_metropolis.hh:96-106
+  const double n_unitcells = static_cast<double>(config.n_unitcells);
+  state.properties.scalar_values["potential_energy"] = 0.;
+  state.properties.scalar_values["potential_energy"] = potential.per_supercell() / n_unitcells;
+  if (occ_location.cand_size(0) == 0 || occ_location.cand_size(1) == 0) {
+    // OccEventProposal.hh:136-139 throws; occupation_metropolis.hh:121-127 logs and stops
+    std::cerr << "\nError proposing event: Error in choose_canonical_swap: No events possible."
+              << std::endl;
+    run_manager.initialize(N);
+    run_manager.sample_data_by_count_if_due(state);
+    run_manager.finalize(state);
+    return;
+  }
+
+  mcg_chain_desc ch{};
+  ch.shape[0] = config.shape[0];
+  ch.shape[1] = config.shape[1];
+  ch.shape[2] = 1;
+  ch.temperature = state.conditions.scalar_values.at("temperature");
+  mcg_desc d{};
+  d.mode = MCG_MODE_CANONICAL;
+  d.lattice = MCG_LATTICE_SQUARE;
+  d.use_nlist = potential.use_nlist ? 1 : 0;
+  d.J1 = potential.J;
+  d.kb = KB;
+  d.sample_period = 1;
+  d.n_chains = 1;
+  d.chains = &ch;
+  gpu::Handle dev(d);
+  std::vector<int8_t> spins(N);
+  for (Index l = 0; l < N; ++l) spins[l] = config.occ(l) ? 1 : -1;
+  dev.set_occupation(0, spins);
+  std::vector<int64_t> loc;
+  for (auto const &v : occ_location.locations())
+    for (Index id : v) loc.push_back(occ_location.mol(id).l);
+  dev.set_occ_location(0, loc);
+  dev.load_engine(0, *random_number_generator.engine);
+
+  detail::run_device_passes(state, dev, N, run_manager, [&]() {
+    auto occ = dev.occupation(0, N);
+    for (Index l = 0; l < N; ++l) config.set_occ(l, occ[l] > 0 ? 1 : 0);
+  });
+  auto dl = dev.occ_location(0, N);
+  std::vector<std::vector<Index>> lists(2);
+  const Index nA = occ_location.cand_size(0);
+  for (Index i = 0; i < N; ++i) lists[i < nA ? 0 : 1].push_back(occ_location.l_to_mol_id(dl[i]));
+  for (Index l = 0; l < N; ++l) occ_location.mol(occ_location.l_to_mol_id(l)).species_index = config.occ(l);
+  occ_location.set_locations(lists);
   dev.store_engine(0, *random_number_generator.engine);
   run_manager.finalize(state);
 }

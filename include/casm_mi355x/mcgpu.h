@@ -41,9 +41,20 @@ extern "C" {
  *   basic_semigrand_canonical.hh:308-320).  Square lattice, J1 only.
  * CHECKERBOARD: sublattice-coloured sweeps with a Philox4x32-10 counter RNG;
  *   one pass updates every site once.  Square (2D) or simple-cubic (3D)
- *   lattices with nearest (J1) and next-nearest (J2) couplings. */
+ *   lattices with nearest (J1) and next-nearest (J2) couplings.
+ * CANONICAL: one sequential chain per wavefront running the reference's
+ *   canonical pair-swap Metropolis bit for bit: the generic loop of
+ *   occupation_metropolis.hh:87-153 with propose_canonical_event
+ *   (OccEventProposal.hh:109-236) over an OccLocation (OccLocation.cc:39-118,
+ *   216-283), std::mt19937_64.  The lattice is binary with one asymmetric
+ *   unit: species 0 = occupation -1, species 1 = +1, one canonical swap type;
+ *   the potential is the square-lattice formation energy (J1 only, the
+ *   multi-site delta of model.hh:354-379); the composition never changes and
+ *   exchange_potential is ignored.  At most 65536 sites per chain, held in LDS
+ *   with the two candidate lists. */
 #define MCG_MODE_EXACT 0
 #define MCG_MODE_CHECKERBOARD 1
+#define MCG_MODE_CANONICAL 2
 
 #define MCG_LATTICE_SQUARE 2 /* 2D, shape[2] == 1 */
 #define MCG_LATTICE_SC 3     /* 3D simple cubic */
@@ -149,9 +160,20 @@ const char *mcg_last_error(void);
  * Setting recomputes the chain's integer observables and, for EXACT, the
  * running energy from the initial per-supercell potential
  * (occupation_metropolis.hh:106).  FCC: values 0/1 in the FCC site order;
- * lattices start empty (all 0). */
+ * lattices start empty (all 0).  CANONICAL: setting also rebuilds the
+ * candidate lists in site order, as OccLocation::initialize does
+ * (OccLocation.cc:39-118). */
 int mcg_set_occupation(mcg_handle *h, int64_t chain, const int8_t *occ, int64_t n);
 int mcg_get_occupation(mcg_handle *h, int64_t chain, int8_t *occ, int64_t n);
+
+/* CANONICAL: get / set one chain's candidate lists, n = sites entries: the
+ * sites of species 0 (occupation -1) in list order, then the sites of species
+ * 1.  They are OccLocation's m_loc[0] and m_loc[1] (OccLocation.hh:204-206;
+ * Mol ids are site indices because every site mutates) and decide which site
+ * each later choose_mol draw picks (OccLocation.hh:254-259).  Setting checks
+ * that every site appears once, in the list of its species. */
+int mcg_get_occ_location(mcg_handle *h, int64_t chain, int64_t *loc, int64_t n);
+int mcg_set_occ_location(mcg_handle *h, int64_t chain, const int64_t *loc, int64_t n);
 
 /* Dump / load the RNG state of one chain.  EXACT: 313 words, the
  * std::mt19937_64 state words followed by the index of the next output (the
@@ -162,7 +184,10 @@ int mcg_rng_dump(mcg_handle *h, int64_t chain, uint64_t *words, int64_t n_words)
 int mcg_rng_load(mcg_handle *h, int64_t chain, const uint64_t *words, int64_t n_words);
 
 /* Advance every active chain by n_passes passes (one pass = one attempt per
- * site: N random-site attempts in EXACT, one coloured sweep in CHECKERBOARD).
+ * site: N random-site attempts in EXACT, N pair-swap attempts in CANONICAL,
+ * one coloured sweep in CHECKERBOARD).  CANONICAL fails with MCG_ERR_STATE
+ * when an active chain has no site of one species (the reference's
+ * "No events possible").
  * A sample record is appended to a chain's device series whenever its pass
  * count reaches a multiple of sample_period.  Replaces the main loop of
  * basic_occupation_metropolis.hh:381-422 between two completion checks (the

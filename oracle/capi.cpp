@@ -4,6 +4,7 @@
 // by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.  Every
 // entry point wraps a restatement in refcpu.hh; see that file for the
 // reference file:line each one follows.
+#include <cstdio>
 #include <cstring>
 #include <sstream>
 
@@ -386,6 +387,66 @@ int orc_fcc_run(const int64_t *L, const double *eci, double temperature, double 
   }
   std::copy(s.begin(), s.end(), occ);
   return 0;
+}
+
+// Canonical pair-swap run (refcpu::canonical_run).  occ: n = rows*cols spins
+// (+1/-1) in, final spins out.  engine: 313 words (state[312], position) in and
+// out.  loc_in: NULL (OccLocation::initialize from occ) or n mol ids, the
+// species-0 list then the species-1 list; loc_out: the final lists in the same
+// form; mol_loc_out: n entries, Mol.loc of each site.  e_run and S: n_passes
+// entries.  counters: n_accept, n_reject.  use_nlist: the per_supercell
+// variant of the initial energy.  Returns 0, or 1 with the message in
+// err (256 bytes) when the reference would stop with an exception.
+int orc_canonical_run(int64_t rows, int64_t cols, double J, double temperature,
+                      int64_t n_passes, int8_t *occ, uint64_t *engine, const int64_t *loc_in,
+                      int64_t *loc_out, int64_t *mol_loc_out, double *e_run, int64_t *S,
+                      int64_t *counters, double *initial, int64_t use_nlist, char *err) {
+  try {
+    const int64_t n = rows * cols;
+    std::vector<int> o(n);
+    for (int64_t l = 0; l < n; ++l) o[l] = occ[l] > 0 ? 1 : 0;
+    std::mt19937_64 eng;
+    {
+      std::ostringstream os;
+      for (int i = 0; i < 313; ++i) os << engine[i] << (i < 312 ? " " : "");
+      std::istringstream is(os.str());
+      is >> eng;
+    }
+    BinaryOccLocation ol0;
+    const BinaryOccLocation *init = nullptr;
+    if (loc_in) {
+      ol0.initialize(o);
+      int64_t n0 = (int64_t)ol0.loc[0].size();
+      for (int c = 0; c < 2; ++c) {
+        int64_t base = c ? n0 : 0, cnt = c ? n - n0 : n0;
+        for (int64_t i = 0; i < cnt; ++i) {
+          Index id = loc_in[base + i];
+          ol0.loc[c][i] = id;
+          ol0.mol[id].loc = i;
+        }
+      }
+      init = &ol0;
+    }
+    CanonicalRunOutput r =
+        canonical_run(rows, cols, o, J, temperature, eng, n_passes, init, use_nlist != 0);
+    for (int64_t l = 0; l < n; ++l) occ[l] = r.final_occ[l] ? 1 : -1;
+    engine_words(eng, engine);
+    int64_t k = 0;
+    for (int c = 0; c < 2; ++c)
+      for (Index id : r.ol.loc[c]) loc_out[k++] = id;
+    for (int64_t l = 0; l < n; ++l) mol_loc_out[l] = r.ol.mol[l].loc;
+    for (int64_t p = 0; p < n_passes; ++p) {
+      e_run[p] = r.e_run[p];
+      S[p] = r.S[p];
+    }
+    counters[0] = r.n_accept;
+    counters[1] = r.n_reject;
+    *initial = r.initial;
+    return 0;
+  } catch (std::exception &ex) {
+    std::snprintf(err, 256, "%s", ex.what());
+    return 1;
+  }
 }
 
 }  // extern "C"

@@ -19,6 +19,8 @@
 //   src/casm/monte/checks/EquilibrationCheck.cc
 //   src/casm/monte/BasicStatistics.cc, include/casm/monte/misc/math.hh
 //   include/casm/monte/run_management/{RunManager,SamplingFixture}.hh
+//   src/casm/monte/events/{OccLocation,OccCandidate}.cc,
+//   include/casm/monte/events/{OccLocation,OccEventProposal}.hh (canonical swaps)
 // Each function cites the file:line it follows.
 //
 // Models that the reference does not define (2D with next-nearest J2, 3D simple
@@ -986,6 +988,177 @@ inline void fcc_sequential_run(FccModel const &m, std::vector<int8_t> &s, double
     }
     out.push_back(c);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Canonical (Kawasaki pair-swap) Metropolis: the generic occupation_metropolis
+// loop (occupation_metropolis.hh:87-153) with propose_canonical_event
+// (OccEventProposal.hh:109-236) over an OccLocation (OccLocation.cc:39-118,
+// 216-283; OccLocation.hh:254-259), for a binary lattice with one asymmetric
+// unit: occupant index o in {0, 1}, species index == occupant index,
+// candidates [(0,0), (0,1)] (OccCandidate.cc:31-52) and the single canonical
+// swap ((0,0),(0,1)) that make_canonical_swaps builds (OccCandidate.cc:99-110).
+// The potential is the square-lattice Ising formation energy of model.hh
+// evaluated on the spins s = 2o - 1, with the multi-site delta of
+// model.hh:354-379 (each site's single delta in event order, the earlier
+// sites applied).
+// ---------------------------------------------------------------------------
+struct OccMol {  // OccEvent.hh Mol: id, l, asym, species_index, loc
+  Index id = 0, l = 0, asym = 0, species_index = 0, loc = 0;
+};
+
+struct BinaryOccLocation {
+  std::vector<OccMol> mol;
+  std::vector<Index> loc[2];  // m_loc[cand_index] -> mol ids
+  std::vector<Index> l_to_mol;
+
+  // OccLocation.cc:39-118 (every site mutates, so mol_id == l)
+  void initialize(std::vector<int> const &occupation) {
+    mol.clear();
+    l_to_mol.clear();
+    loc[0].clear();
+    loc[1].clear();
+    mol.resize(occupation.size());
+    for (Index l = 0; l < (Index)occupation.size(); ++l) {
+      Index cand = occupation[l];
+      OccMol &m = mol[l];
+      m.id = l;
+      m.l = l;
+      m.asym = 0;
+      m.species_index = cand;
+      m.loc = (Index)loc[cand].size();
+      loc[cand].push_back(l);
+      l_to_mol.push_back(l);
+    }
+  }
+  Index cand_size(Index cand) const { return (Index)loc[cand].size(); }
+  // OccLocation.hh:254-259: uniform over [0, size-1] with size_t bounds
+  OccMol const &choose_mol(Index cand, std::mt19937_64 &engine) const {
+    std::size_t maxv = loc[cand].size() - 1;
+    std::size_t i = std::uniform_int_distribution<std::size_t>(0, maxv)(engine);
+    return mol[loc[cand][i]];
+  }
+  // OccLocation.cc:254-283 (the occ_transform part; no atom tracking)
+  void apply_transform(Index mol_id, Index from_species, Index to_species,
+                       std::vector<int> &occupation) {
+    OccMol &m = mol[mol_id];
+    if (m.species_index != from_species)
+      throw std::runtime_error("refcpu: species mismatch in apply");
+    occupation[m.l] = (int)to_species;
+    Index cand = m.species_index;
+    Index back = loc[cand].back();
+    loc[cand][m.loc] = back;
+    mol[back].loc = m.loc;
+    loc[cand].pop_back();
+    m.species_index = to_species;
+    cand = m.species_index;
+    m.loc = (Index)loc[cand].size();
+    loc[cand].push_back(m.id);
+  }
+};
+
+struct CanonicalEvent {  // OccEvent: linear_site_index, new_occ, occ_transform
+  Index mol_id[2], l[2], from[2], to[2];
+  int new_occ[2];
+};
+
+// OccEventProposal.hh:109-152 (one swap type: the draw is made and the swap is
+// always the first one), then :173-208
+inline void propose_canonical_event(CanonicalEvent &e, BinaryOccLocation const &ol,
+                                    std::mt19937_64 &engine) {
+  double tsum0 = 0.0;
+  double tsum1 = tsum0 + ((double)ol.cand_size(0)) * ((double)ol.cand_size(1));
+  if (tsum1 == 0.0) throw std::runtime_error("Error in choose_canonical_swap: No events possible.");
+  double rand = std::uniform_real_distribution<double>(0, tsum1)(engine);
+  if (!(rand < tsum1)) throw std::runtime_error("Error in choose_canonical_swap");
+  OccMol const &ma = ol.choose_mol(0, engine);
+  e.mol_id[0] = ma.id;
+  e.l[0] = ma.l;
+  e.from[0] = 0;
+  e.to[0] = 1;
+  OccMol const &mb = ol.choose_mol(1, engine);
+  e.mol_id[1] = mb.id;
+  e.l[1] = mb.l;
+  e.from[1] = 1;
+  e.to[1] = 0;
+  for (int i = 0; i < 2; ++i) e.new_occ[i] = (int)e.to[i];  // Conversions::occ_index
+}
+
+// model.hh:305-345 single-site delta on spins s = 2o - 1 (both variants give
+// the same value: the four-neighbour sum is an exact integer)
+inline double lattice_gas_single_delta(IsingConfiguration const &c, std::vector<int> const &occ,
+                                       double J, Index l, int new_occ) {
+  auto spin = [&](Index q) { return 2 * occ[q] - 1; };
+  Index i = l % c.rows, j = l / c.rows;
+  int sum = spin(c.to_linear(c.within(i + 1, 0), j)) + spin(c.to_linear(i, c.within(j + 1, 1))) +
+            spin(c.to_linear(c.within(i - 1, 0), j)) + spin(c.to_linear(i, c.within(j - 1, 1)));
+  return -J * ((2 * new_occ - 1) - spin(l)) * sum;
+}
+
+// model.hh:354-379
+inline double lattice_gas_delta(IsingConfiguration const &c, std::vector<int> &occ, double J,
+                                CanonicalEvent const &e) {
+  double dE = 0.0;
+  int orig[2];
+  for (int i = 0; i < 2; ++i) {
+    dE += lattice_gas_single_delta(c, occ, J, e.l[i], e.new_occ[i]);
+    orig[i] = occ[e.l[i]];
+    occ[e.l[i]] = e.new_occ[i];
+  }
+  for (int i = 0; i < 2; ++i) occ[e.l[i]] = orig[i];
+  return dE;
+}
+
+struct CanonicalRunOutput {
+  std::vector<double> e_run;      // potential energy per unit cell after each pass
+  std::vector<long long> S;       // bond sum after each pass
+  std::vector<int> final_occ;     // occupant indices
+  BinaryOccLocation ol;
+  BigCountType n_accept = 0, n_reject = 0;
+  double initial = 0.0;
+};
+
+// occupation_metropolis.hh:96-153: n_passes passes of mol_size() steps
+inline CanonicalRunOutput canonical_run(Index rows, Index cols, std::vector<int> occ, double J,
+                                        double temperature, std::mt19937_64 &engine,
+                                        long n_passes, BinaryOccLocation const *init_ol = nullptr,
+                                        bool use_nlist = true) {
+  CanonicalRunOutput out;
+  IsingConfiguration shape(rows, cols, 1);
+  IsingConfiguration spins(rows, cols, 1);
+  for (Index l = 0; l < shape.n_sites; ++l) spins.occupation[l] = 2 * occ[l] - 1;
+  IsingFormationEnergy fe(J, use_nlist);  // per_supercell variant (model.hh:259-290)
+  fe.set_state(&spins);
+  const double n_unitcells = static_cast<double>(shape.n_sites);
+  double pe = fe.per_supercell() / n_unitcells;
+  out.initial = pe;
+  BinaryOccLocation ol;
+  if (init_ol)
+    ol = *init_ol;
+  else
+    ol.initialize(occ);
+  const double beta = 1.0 / (KB * temperature);
+  RandomNumberGenerator rng(std::shared_ptr<std::mt19937_64>(&engine, [](std::mt19937_64 *) {}));
+  CanonicalEvent e;
+  for (long p = 0; p < n_passes; ++p) {
+    for (Index s = 0; s < (Index)ol.mol.size(); ++s) {
+      propose_canonical_event(e, ol, engine);
+      double dE = lattice_gas_delta(shape, occ, J, e);
+      if (metropolis_acceptance(dE, beta, rng)) {
+        out.n_accept++;
+        for (int i = 0; i < 2; ++i) ol.apply_transform(e.mol_id[i], e.from[i], e.to[i], occ);
+        pe += (dE / n_unitcells);
+      } else {
+        out.n_reject++;
+      }
+    }
+    out.e_run.push_back(pe);
+    for (Index l = 0; l < shape.n_sites; ++l) spins.occupation[l] = 2 * occ[l] - 1;
+    out.S.push_back(bond_sum(spins));
+  }
+  out.final_occ = occ;
+  out.ol = ol;
+  return out;
 }
 
 }  // namespace refcpu

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <iostream>
+#include <numeric>
 
 #include "casm/monte/ising_cpp/basic_semigrand_canonical.hh"
 #include "casm/monte/methods/occupation_metropolis.hh"
@@ -328,6 +329,144 @@ static void gpu_occupation_metropolis_tests() {
             << (g_fail ? " (failures so far)" : " ok") << std::endl;
 }
 
+// ---------------------------------------------------------------------------
+// Canonical pair swaps: host event machinery (CPU) and the device run (GPU)
+// ---------------------------------------------------------------------------
+static std::vector<int> random_occupants(int n, int n_one, uint64_t seed) {
+  std::vector<int> occ(n, 0);
+  std::vector<int> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::mt19937_64 e(seed);
+  std::shuffle(idx.begin(), idx.end(), e);
+  for (int i = 0; i < n_one; ++i) occ[idx[i]] = 1;
+  return occ;
+}
+
+// The invariants the reference's OccLocation test checks after every
+// proposal and apply (tests/unit/monte/OccLocation_test.cpp:16-66).
+static bool occ_location_consistent(OccLocation const &ol, std::vector<int> const &occ) {
+  for (Index id = 0; id < ol.mol_size(); ++id)
+    if (ol.mol(id).id != id) return false;
+  for (Index l = 0; l < (Index)occ.size(); ++l) {
+    Mol const &m = ol.mol(ol.l_to_mol_id(l));
+    if (m.l != l || m.species_index != occ[l]) return false;
+    if (ol.mol_id(ol.candidate_list().index(m.asym, m.species_index), m.loc) != m.id) return false;
+  }
+  return true;
+}
+
+static void cpu_canonical_tests() {
+  const int R = 9, C = 7, N = R * C;
+  ising_cpp::BinaryLatticeEvents ev(N);
+  EXPECT(ev.candidate_list.size() == 2 && ev.canonical_swaps.size() == 1 &&
+             ev.canonical_swaps[0] == OccSwap(OccCandidate(0, 0), OccCandidate(0, 1)),
+         "binary candidate list and canonical swap");
+  EXPECT(ev.semigrand_canonical_swaps.size() == 2, "semi-grand canonical swaps (both orders)");
+  std::vector<int> occ = random_occupants(N, 25, 3);
+  OccLocation ol(ev.convert, ev.candidate_list);
+  ol.initialize(occ);
+  EXPECT(occ_location_consistent(ol, occ), "OccLocation::initialize invariants");
+  // the oracle's restatement of the same proposal and apply, same stream
+  refcpu::BinaryOccLocation ro;
+  ro.initialize(occ);
+  std::vector<int> rocc = occ;
+  auto eng = std::make_shared<std::mt19937_64>(77);
+  std::mt19937_64 reng(77);
+  RandomNumberGenerator<std::mt19937_64> rng(eng);
+  OccEvent e;
+  refcpu::CanonicalEvent re;
+  bool same = true, inv = true;
+  for (int k = 0; k < 20000; ++k) {
+    propose_canonical_event(e, ol, ev.canonical_swaps, rng);
+    refcpu::propose_canonical_event(re, ro, reng);
+    same = same && e.linear_site_index[0] == re.l[0] && e.linear_site_index[1] == re.l[1] &&
+           e.new_occ[0] == re.new_occ[0] && e.new_occ[1] == re.new_occ[1];
+    if (k % 3 != 2) {  // apply two of every three proposals
+      ol.apply(e, occ);
+      for (int i = 0; i < 2; ++i) ro.apply_transform(re.mol_id[i], re.from[i], re.to[i], rocc);
+    }
+    if (k % 500 == 0) inv = inv && occ_location_consistent(ol, occ);
+  }
+  EXPECT(same, "propose_canonical_event matches the oracle's stream use");
+  EXPECT(inv && occ_location_consistent(ol, occ), "OccLocation::apply invariants");
+  EXPECT(occ == rocc && ol.locations()[0] == ro.loc[0] && ol.locations()[1] == ro.loc[1],
+         "lists after 13334 applied swaps");
+  EXPECT(*eng == reng, "engine state after 20000 proposals");
+  // semi-grand canonical proposal: one site of the chosen type changes species
+  propose_semigrand_canonical_event(e, ol, ev.semigrand_canonical_swaps, rng);
+  EXPECT(e.linear_site_index.size() == 1 && e.new_occ[0] == 1 - occ[e.linear_site_index[0]],
+         "semi-grand canonical proposal flips one site");
+  std::vector<int> all_zero(N, 0);
+  ol.initialize(all_zero);
+  bool threw = false;
+  try {
+    propose_canonical_event(e, ol, ev.canonical_swaps, rng);
+  } catch (std::runtime_error const &ex) {
+    threw = std::string(ex.what()).find("No events possible") != std::string::npos;
+  }
+  EXPECT(threw, "no canonical event possible at x = 0");
+  std::cout << "cpu: canonical event machinery" << (g_fail ? " (failures so far)" : " ok")
+            << std::endl;
+}
+
+static void gpu_canonical_tests() {
+  const int R = 18, C = 14, N = R * C;
+  const long P = 45;
+  const double T = 1700.0, J = 0.1;
+  const uint64_t seed = 31;
+  typedef ising_cpp::IsingConfiguration config_type;
+  std::vector<int> occ0 = random_occupants(N, 101, 9);
+  config_type cfg({R, C}, 0);
+  cfg.set_occupation(occ0);
+  ValueMap cond;
+  cond.scalar_values["temperature"] = T;
+  State<config_type> state(cfg, cond);
+  ising_cpp::IsingOccupantPotential potential(J, true);
+  potential.set(&state);
+  ising_cpp::BinaryLatticeEvents ev(N);
+  OccLocation ol(ev.convert, ev.candidate_list);
+  ol.initialize(state.configuration.occupation());
+
+  StateSamplingFunction pe("potential_energy", "running potential energy", {}, [&state]() {
+    return VectorXd{state.properties.scalar_values.at("potential_energy")};
+  });
+  pe.device_observable = MCG_OBS_POTENTIAL_ENERGY;
+  StateSamplingFunctionMap fns;
+  fns.emplace(pe.name, pe);
+  SamplingParams sp;
+  sp.sampler_names = {"potential_energy"};
+  CompletionCheckParams<BasicStatistics> cc;
+  cc.cutoff_params.max_count = P;
+  typedef RunManager<config_type, BasicStatistics, std::mt19937_64> rm_type;
+  std::vector<rm_type::sampling_fixture_params_type> fixtures;
+  fixtures.emplace_back("thermo", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), sp, cc,
+                        std::vector<std::string>{});
+  auto engine = std::make_shared<std::mt19937_64>(seed);
+  rm_type rm(engine, fixtures, true);
+  typedef RandomNumberGenerator<std::mt19937_64> generator_type;
+  generator_type rng(engine);
+  occupation_metropolis(state, ol, potential, ev.canonical_swaps,
+                        propose_canonical_event<generator_type>, rng, rm);
+
+  std::mt19937_64 oe(seed);
+  auto o = refcpu::canonical_run(R, C, occ0, J, T, oe, P);
+  auto const &res = rm.sampling_fixtures[0]->results();
+  VectorXd got = res.samplers.at("potential_energy")->component(0);
+  bool ok = got.size() == P;
+  for (long i = 0; ok && i < P; ++i) ok = same_bits(got(i), o.e_run[i]);
+  EXPECT(ok, "canonical running energy (" << got.size() << " samples)");
+  EXPECT(state.configuration.occupation() == o.final_occ, "canonical final configuration");
+  EXPECT(ol.locations()[0] == o.ol.loc[0] && ol.locations()[1] == o.ol.loc[1],
+         "canonical OccLocation lists");
+  EXPECT(occ_location_consistent(ol, state.configuration.occupation()),
+         "canonical OccLocation invariants after the run");
+  EXPECT(*engine == oe, "canonical engine state");
+  EXPECT(res.n_accept == o.n_accept && res.n_reject == o.n_reject, "canonical acceptance counts");
+  std::cout << "gpu: canonical occupation_metropolis 18x14" << (g_fail ? " (failures so far)" : " ok")
+            << std::endl;
+}
+
 static void gpu_tests() {
   // the reference's SemiGrandCanonicalRun1 set-up, converged on two observables
   gpu_case({25, 25, true, 2000.0, 0.0, 42, 1, -1, 0.001, 100, false});
@@ -339,13 +478,17 @@ static void gpu_tests() {
   // tiny lattice: pending periodic checks span passes
   gpu_case({3, 3, true, 4000.0, 0.0, 11, 1, 3000, 1e-4, 500, false});
   gpu_occupation_metropolis_tests();
+  gpu_canonical_tests();
 }
 
 int main(int argc, char **argv) {
   bool cpu = argc > 1 && std::string(argv[1]) == "--cpu";
   bool gpu = argc > 1 && std::string(argv[1]) == "--gpu";
   try {
-    if (cpu) cpu_tests();
+    if (cpu) {
+      cpu_tests();
+      cpu_canonical_tests();
+    }
     if (gpu) gpu_tests();
   } catch (std::exception const &e) {
     std::cout << "EXCEPTION: " << e.what() << std::endl;

@@ -110,6 +110,11 @@ def _declare(L):
     L.orc_fcc_energy.argtypes = [P64, PD, P8]
     L.orc_fcc_energy.restype = C.c_double
     L.orc_fcc_run.argtypes = [P64, PD, C.c_double, C.c_double, C.c_uint64, C.c_int64, P8, P64]
+    PU = C.POINTER(C.c_uint64)
+    L.orc_canonical_run.argtypes = [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_int64,
+                                    P8, PU, P64, P64, P64, PD, P64, P64, PD, C.c_int64,
+                                    C.c_char_p]
+    L.orc_canonical_run.restype = C.c_int
 
 
 KB = None
@@ -309,3 +314,38 @@ def fcc_coords(L):
     x = ((y + z) & 1) + 2 * h
     return x, y, z
 
+
+
+class CanonicalError(RuntimeError):
+    """The reference's canonical loop would stop with this exception."""
+
+
+def canonical_run(rows, cols, J, T, n_passes, occ, engine, loc=None, use_nlist=True):
+    """Canonical pair-swap Metropolis (occupation_metropolis + propose_canonical_event
+    over an OccLocation) on spins occ (+1/-1, column-major).
+
+    engine: 313 words (std::mt19937_64 state, position), e.g. mt64_state(seed, 0).
+    loc: None (lists built by OccLocation::initialize) or the species-0 list then
+    the species-1 list.  use_nlist: variant of the initial per_supercell energy.  Returns a dict with e_run and S per pass, final occ,
+    engine, loc (lists), mol_loc, n_accept, n_reject and the initial energy.
+    """
+    n = rows * cols
+    o = np.ascontiguousarray(occ, dtype=np.int8).copy()
+    eng = np.ascontiguousarray(engine, dtype=np.uint64).copy()
+    li = None if loc is None else np.ascontiguousarray(loc, dtype=np.int64)
+    lo = np.zeros(n, dtype=np.int64)
+    ml = np.zeros(n, dtype=np.int64)
+    e = np.zeros(max(n_passes, 1), dtype=np.float64)
+    S = np.zeros(max(n_passes, 1), dtype=np.int64)
+    cnt = np.zeros(2, dtype=np.int64)
+    init = C.c_double(0.0)
+    err = C.create_string_buffer(256)
+    rc = lib().orc_canonical_run(rows, cols, J, T, n_passes, _p(o, C.c_int8),
+                                 _p(eng, C.c_uint64), None if li is None else _p(li, C.c_int64),
+                                 _p(lo, C.c_int64), _p(ml, C.c_int64), _p(e, C.c_double),
+                                 _p(S, C.c_int64), _p(cnt, C.c_int64), C.byref(init),
+                                 int(use_nlist), err)
+    if rc:
+        raise CanonicalError(err.value.decode())
+    return dict(e_run=e[:n_passes], S=S[:n_passes], occ=o, engine=eng, loc=lo, mol_loc=ml,
+                n_accept=int(cnt[0]), n_reject=int(cnt[1]), initial=init.value)
