@@ -9,6 +9,7 @@
 #define CASM_MI355X_monte_RunManager
 
 #include "casm/monte/checks/CompletionCheck.hh"
+#include "casm/monte/checks/io/json/CompletionCheck_json_io.hh"
 
 namespace CASM {
 namespace monte {
@@ -285,10 +286,13 @@ class SamplingFixture {
     if (m_params.method_log.logfile_path.empty()) return;
     Log &log = m_params.method_log.log;
     m_params.method_log.reset();
-    auto const &r = m_completion_check.results();
-    log << "{\"run_index\": " << run_index << ", \"time\": " << log.time_s()
-        << ", \"completion_check_results\": {\"is_complete\": " << (r.is_complete ? "true" : "false")
-        << ", \"n_samples\": " << r.n_samples << "}}" << std::endl;
+    jsonParser json = jsonParser::object();
+    json["run_index"] = run_index;
+    json["time"] = log.time_s();
+    to_json(m_completion_check.results(), json["completion_check_results"]);
+    std::ostringstream os;
+    json.print(os);
+    log << os.str() << std::endl;
     log.begin_lap();
   }
 

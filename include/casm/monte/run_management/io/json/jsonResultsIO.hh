@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "casm/casm_io/json/jsonParser.hh"
+#include "casm/monte/checks/io/json/CompletionCheck_json_io.hh"
 #include "casm/monte/run_management/RunManager.hh"
 
 namespace CASM {
@@ -27,13 +28,6 @@ inline jsonParser to_json_vector(VectorXd const &v) {
   jsonParser j = jsonParser::array();
   for (Index i = 0; i < v.size(); ++i) j.push_back(v(i));
   return j;
-}
-
-inline jsonParser &to_json(BasicStatistics const &stats, jsonParser &json) {
-  json.put_obj();
-  json["mean"] = stats.mean;
-  json["calculated_precision"] = stats.calculated_precision;
-  return json;
 }
 
 /// BasicStatistics.cc:190-206
@@ -355,99 +349,6 @@ class jsonResultsIO {
     }
   }
 };
-
-// --- CompletionCheckParams from JSON (checks/io/json/CompletionCheck_json_io.hh:295-369) --
-// Errors are reported by exceptions carrying the reference's messages.
-
-inline CutoffCheckParams cutoff_params_from_json(jsonParser const &json) {
-  CutoffCheckParams p;
-  auto minmax = [&](std::string const &key, auto &mn, auto &mx) {
-    if (!json.contains(key)) return;
-    auto const &j = json[key];
-    using MinT = typename std::decay<decltype(mn)>::type::value_type;
-    if (j.contains("min")) mn = static_cast<MinT>(j["min"].get<double>());
-    if (j.contains("max")) mx = static_cast<MinT>(j["max"].get<double>());
-  };
-  minmax("count", p.min_count, p.max_count);
-  minmax("sample", p.min_sample, p.max_sample);
-  minmax("time", p.min_time, p.max_time);
-  minmax("clocktime", p.min_clocktime, p.max_clocktime);
-  return p;
-}
-
-inline CompletionCheckParams<BasicStatistics> completion_check_params_from_json(
-    jsonParser const &json, StateSamplingFunctionMap const &sampling_functions) {
-  CompletionCheckParams<BasicStatistics> p;
-  double confidence = json.contains("confidence") ? json["confidence"].get<double>() : 0.95;
-  long method = json.contains("weighted_observations_method")
-                    ? json["weighted_observations_method"].get<long>()
-                    : 1;
-  long n_resamples = json.contains("n_resamples") ? json["n_resamples"].get<long>() : 10000;
-  p.equilibration_check_f = default_equilibration_check;
-  p.calc_statistics_f = BasicStatisticsCalculator(confidence, method, n_resamples);
-  if (json.contains("cutoff")) p.cutoff_params = cutoff_params_from_json(json["cutoff"]);
-  if (json.contains("convergence")) {
-    for (auto const &c : json["convergence"].elements()) {
-      std::string q = c["quantity"].get<std::string>();
-      auto it = sampling_functions.find(q);
-      if (it == sampling_functions.end())
-        throw std::runtime_error("Error: \"" + q + "\" is not a sampling option.");
-      RequestedPrecision rp;
-      if (c.contains("precision")) {
-        rp.abs_convergence_is_required = true;
-        rp.abs_precision = c["precision"].get<double>();
-      }
-      if (c.contains("abs_precision")) {
-        rp.abs_convergence_is_required = true;
-        rp.abs_precision = c["abs_precision"].get<double>();
-      }
-      if (c.contains("rel_precision")) {
-        rp.rel_convergence_is_required = true;
-        rp.rel_precision = c["rel_precision"].get<double>();
-      }
-      auto const &names = it->second.component_names;
-      std::set<Index> selected;
-      if (c.contains("component_index"))
-        for (long i : c["component_index"].get<std::vector<long>>()) selected.insert(i);
-      if (c.contains("component_name"))
-        for (auto const &n : c["component_name"].get<std::vector<std::string>>()) {
-          auto f = std::find(names.begin(), names.end(), n);
-          if (f == names.end())
-            throw std::runtime_error("Error: \"" + n + "\" is not a component of \"" + q + "\"");
-          selected.insert(f - names.begin());
-        }
-      if (selected.empty())
-        for (Index i = 0; i < (Index)names.size(); ++i) selected.insert(i);
-      for (Index i : selected) {
-        if (i < 0 || i >= (Index)names.size())
-          throw std::runtime_error("Error: component_index out of range for \"" + q + "\"");
-        p.requested_precision[SamplerComponent(q, i, names[i])] = rp;
-      }
-    }
-  }
-  std::string spacing = json.contains("spacing") ? json["spacing"].get<std::string>() : "linear";
-  if (spacing == "linear") {
-    p.log_spacing = false;
-    p.check_begin = 100;
-    p.check_period = 100;
-  } else if (spacing == "log") {
-    p.log_spacing = true;
-    p.check_begin = 0;
-    p.check_base = 10.0;
-    p.check_shift = 2.0;
-    p.check_period_max = 10000;
-  } else {
-    throw std::runtime_error("Error: \"spacing\" must be one of \"linear\", \"log\".");
-  }
-  if (json.contains("begin")) p.check_begin = json["begin"].get<long>();
-  if (json.contains("period")) p.check_period = json["period"].get<long>();
-  if (p.check_period <= 1) throw std::runtime_error("Error: \"period\" must > 0.");
-  if (json.contains("base")) p.check_base = json["base"].get<double>();
-  if (json.contains("shift")) p.check_shift = json["shift"].get<double>();
-  if (json.contains("period_max")) p.check_period_max = json["period_max"].get<long>();
-  if (p.check_base <= 1.0) throw std::runtime_error("Error: \"base\" must > 1.0");
-  return p;
-}
 
 }  // namespace monte
 }  // namespace CASM

@@ -1,11 +1,14 @@
 // json_io_tests <dir>: writes two runs of a sampling fixture's results with
 // jsonResultsIO into <dir> (summary.json appended per run, run.<i>/observations.json
-// and trajectory.json), reads a CompletionCheckParams from JSON and checks a
-// jsonParser round trip.  The file contents are checked by tests/test_json_io.py.
+// and trajectory.json), reads a CompletionCheckParams, SamplingParams and
+// SamplingFixtureParams from JSON, writes the completion check parameters and
+// results as JSON, and checks a jsonParser round trip.  The file contents are
+// checked by tests/test_json_io.py.
 #include <cassert>
 #include <iostream>
 
 #include "casm/monte/ising_cpp/model.hh"
+#include "casm/monte/run_management/io/json/SamplingFixtureParams_json_io.hh"
 #include "casm/monte/run_management/io/json/jsonResultsIO.hh"
 
 using namespace CASM;
@@ -100,6 +103,110 @@ int main(int argc, char **argv) {
     cond.scalar_values["temperature"] = 1000.0 + 500.0 * run;
     cond.vector_values["exchange_potential"] = VectorXd{0.1 * run};
     io.write(r, cond, run);
+  }
+
+  // SamplingParams from JSON (SamplingParams_json_io.cc:70-176) and back
+  {
+    std::set<std::string> names{"param_composition", "potential_energy"};
+    SamplingParams sp = sampling_params_from_json(
+        jsonParser::parse(R"({"sample_by": "pass", "spacing": "log", "period": 10.0,
+                              "quantities": ["potential_energy"]})"),
+        names, {}, false);
+    check(sp.sample_mode == SAMPLE_MODE::BY_PASS && sp.sample_method == SAMPLE_METHOD::LOG,
+          "sampling mode/method");
+    check(sp.begin == 0.0 && sp.period == 10.0 && sp.shift == 10.0 &&
+              sp.base == std::pow(10.0, 0.1),
+          "sampling defaults");
+    check(sp.sampler_names == std::vector<std::string>{"potential_energy"}, "quantities");
+    auto fails = [&](const char *text, const char *needle) {
+      try {
+        sampling_params_from_json(jsonParser::parse(text), names, {}, false);
+      } catch (std::runtime_error const &e) {
+        return std::string(e.what()).find(needle) != std::string::npos;
+      }
+      return false;
+    };
+    check(fails(R"({"sample_by": "pass", "period": 1, "quantities": ["nope"]})",
+                "\"nope\" is not a sampling option"),
+          "unknown quantity");
+    check(fails(R"({"sample_by": "time", "period": 1})", "must be one of \"pass\" or \"step\""),
+          "time sampling not allowed");
+    check(fails(R"({"sample_by": "pass", "spacing": "log", "period": 1.0})", "must > 1.0"),
+          "log period");
+    check(fails(R"({"sample_by": "pass"})", "period"), "period required");
+    jsonParser back;
+    to_json(sp, back);
+    check(back["spacing"].get<std::string>() == "log" && back["sample_by"].get<std::string>() == "pass",
+          "SamplingParams to_json");
+  }
+
+  // SamplingFixtureParams from JSON (SamplingFixtureParams_json_io.hh:37-147)
+  {
+    jsonParser fj = jsonParser::parse(R"({
+      "sampling": {"sample_by": "pass", "period": 1, "quantities": ["param_composition"]},
+      "completion_check": {"cutoff": {"count": {"max": 200}},
+                           "convergence": [{"quantity": "param_composition", "abs_precision": 0.01}]},
+      "analysis": {"functions": []},
+      "results_io": {"method": "json", "kwargs": {"output_dir": "OUT", "write_observations": true}},
+      "log": {"file": "STATUS", "frequency_in_s": 0.0}})");
+    fj["results_io"]["kwargs"]["output_dir"] = (dir / "fixture_out").string();
+    fj["log"]["file"] = (dir / "fixture_out" / "status.json").string();
+    auto params = sampling_fixture_params_from_json<IsingConfiguration>(
+        fj, "thermo", f, jsonStateSamplingFunctionMap(),
+        ResultsAnalysisFunctionMap<IsingConfiguration, BasicStatistics>(), false);
+    check(params.label == "thermo" && params.sampling_params.sampler_names.size() == 1,
+          "fixture sampling params");
+    check(params.completion_check_params.cutoff_params.max_count == 200, "fixture cutoff");
+    check(params.results_io != nullptr, "fixture results_io");
+    check(params.method_log.log_frequency == 0.0, "fixture log frequency");
+    jsonParser pj;
+    to_json(params, pj);
+    pj.write(dir / "fixture_params.json");
+    // the results writer the fixture was given: one run into output_dir
+    std::vector<std::string> names{"param_composition"};
+    results_type r(names, f, {}, jsonStateSamplingFunctionMap(),
+                   ResultsAnalysisFunctionMap<IsingConfiguration, BasicStatistics>());
+    r.reset();
+    for (int k = 0; k < 50; ++k) {
+      r.samplers.at("param_composition")->push_back(VectorXd{0.25 + 0.001 * (k % 7)});
+      r.sample_count.push_back(k + 1);
+    }
+    r.completion_check_results.params = params.completion_check_params;
+    ValueMap cond;
+    cond.scalar_values["temperature"] = 700.0;
+    params.results_io(r, cond, 0);
+    bool threw = false;
+    try {
+      jsonParser bad = fj;
+      bad["results_io"]["method"] = "hdf5";
+      sampling_fixture_params_from_json<IsingConfiguration>(
+          bad, "x", f, jsonStateSamplingFunctionMap(),
+          ResultsAnalysisFunctionMap<IsingConfiguration, BasicStatistics>(), false);
+    } catch (std::runtime_error const &e) {
+      threw = std::string(e.what()).find("must be one of: \"json\"") != std::string::npos;
+    }
+    check(threw, "unknown results_io method");
+  }
+
+  // completion check parameters and results as JSON (CompletionCheck_json_io.hh:371-440)
+  {
+    jsonParser pj;
+    to_json(ccp, pj);
+    pj.write(dir / "completion_check_params.json");
+    CompletionCheck<BasicStatistics> cc(ccp);
+    SamplerMap samplers;
+    samplers.emplace("param_composition", std::make_shared<Sampler>(std::vector<Index>{1}));
+    samplers.emplace("potential_energy", std::make_shared<Sampler>(std::vector<Index>{}));
+    for (int k = 0; k < 400; ++k) {
+      samplers.at("param_composition")->push_back(VectorXd{0.5 + 0.1 * std::sin(0.37 * k)});
+      samplers.at("potential_energy")->push_back(VectorXd{-0.2 + 0.02 * std::cos(0.9 * k)});
+    }
+    Sampler weight(std::vector<Index>{});
+    Log log;
+    cc.is_complete(samplers, weight, CountType(400), log);
+    jsonParser rj;
+    to_json(cc.results(), rj);
+    rj.write(dir / "completion_check_results.json");
   }
   std::cout << "ALL OK" << std::endl;
   return 0;

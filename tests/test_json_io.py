@@ -74,3 +74,57 @@ def test_observations_and_trajectory(out):
         t = json.load(open(out / f"run.{run}" / "trajectory.json"))
         assert len(t) == 3 and all(c["shape"] == [4, 4] for c in t)
         assert sum(v == -1 for v in t[-1]["occupation"]) == 3
+
+
+def test_completion_check_params_round_trip(out):
+    """CompletionCheckParams to_json (CompletionCheck_json_io.hh:371-417) of the
+    parameters read from JSON above."""
+    p = json.load(open(out / "completion_check_params.json"))
+    assert p["cutoff"] == {"count": {"min": 10, "max": 5000}, "sample": {"min": 100}}
+    assert p["spacing"] == "linear" and p["begin"] == 100 and p["period"] == 10
+    conv = {c["quantity"]: c for c in p["convergence"]}
+    assert conv["potential_energy"]["abs_precision"] == 0.001
+    assert conv["param_composition"]["abs_precision"] == 0.002
+    assert conv["param_composition"]["component_index"] == [0]
+    assert conv["param_composition"]["component_name"] == ["0"]
+
+
+def test_completion_check_results_json(out):
+    """CompletionCheckResults to_json (CompletionCheck_json_io.hh:420-440, with the
+    equilibration and convergence results of EquilibrationCheck_json_io.cc and
+    ConvergenceCheck_json_io.hh:33-62); the statistics equal the oracle's
+    BasicStatistics of the same post-equilibration tail."""
+    r = json.load(open(out / "completion_check_results.json"))
+    for key in ("has_all_minimums_met", "has_any_maximum_met", "count", "time", "clocktime",
+                "n_samples", "is_complete", "n_samples_at_convergence_check",
+                "equilibration_check_results", "convergence_check_results"):
+        assert key in r, key
+    assert r["count"] == 400 and r["n_samples"] == 400 and r["time"] is None
+    eq = r["equilibration_check_results"]
+    assert eq["all_equilibrated"] is True
+    n_eq = eq["N_samples_for_all_to_equilibrate"]
+    series = {
+        "param_composition": np.array([0.5 + 0.1 * math.sin(0.37 * k) for k in range(400)]),
+        "potential_energy": np.array([-0.2 + 0.02 * math.cos(0.9 * k) for k in range(400)]),
+    }
+    cv = r["convergence_check_results"]
+    assert cv["N_samples_for_statistics"] == 400 - n_eq
+    for ind in cv["individual_results"]:
+        mean, prec = O.basic_statistics(series[ind["sampler_name"]][n_eq:], 0.95)
+        assert ind["stats"]["mean"] == pytest.approx(mean, rel=1e-13)
+        assert ind["stats"]["calculated_precision"] == pytest.approx(prec, rel=1e-10)
+        assert ind["is_converged"] == (prec < ind["requested_precision"]["abs_precision"])
+
+
+def test_sampling_fixture_params_json(out):
+    """SamplingFixtureParams read from JSON (SamplingFixtureParams_json_io.hh:37-147)
+    and written back; its results_io writes a run into its output_dir."""
+    p = json.load(open(out / "fixture_params.json"))
+    assert p["sampling"] == {"sample_by": "pass", "spacing": "linear", "begin": 0, "period": 1,
+                             "quantities": ["param_composition"]}
+    assert p["completion_check"]["cutoff"] == {"count": {"max": 200}}
+    assert p["results_io"] == {"method": "json"}
+    assert p["log"]["frequency_in_s"] == 0.0 and p["log"]["file"].endswith("status.json")
+    s = json.load(open(out / "fixture_out" / "summary.json"))
+    assert s["conditions"]["temperature"]["value"] == [700.0]
+    assert os.path.exists(out / "fixture_out" / "run.0" / "observations.json")
