@@ -157,7 +157,8 @@ def bench_checker_family(shape, lattice, passes, chains=16):
     w, k, a = timed_run(r, passes)
     r.close()
     return dict(shape=list(shape), chains=chains, passes=passes, wall_s=w, kernel_s=k,
-                schedule={k2: sched[k2] for k2 in ("units_per_thread", "graph_passes")},
+                schedule={k2: sched[k2] for k2 in ("units_per_thread", "graph_passes", "band_lines",
+                                                   "line_tiles")},
                 attempts=a, attempts_per_s=a / w,
                 hbm_gbs_algorithmic=2.0 * a / k / 1e9,
                 hbm_frac=2.0 * a / k / 1e9 / HBM_PEAK_GBS)

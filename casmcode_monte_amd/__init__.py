@@ -63,7 +63,7 @@ SAMPLE_DTYPE = np.dtype([("pass", np.int64), ("S1", np.int64), ("S2", np.int64),
 class Schedule(C.Structure):
     _fields_ = [("lds_block", C.c_int64), ("units_per_thread", C.c_int64),
                 ("graph_passes", C.c_int64), ("trial_ms", C.c_double),
-                ("band_lines", C.c_int64)]
+                ("band_lines", C.c_int64), ("line_tiles", C.c_int64)]
 
 
 class Stats(C.Structure):
@@ -278,13 +278,14 @@ class Run:
         return {k: getattr(s, k) for k, _ in Schedule._fields_}
 
     def set_schedule(self, lds_block=None, units_per_thread=None, graph_passes=None,
-                     band_lines=None):
+                     band_lines=None, line_tiles=None):
         cur = self.schedule()
         s = Schedule(lds_block if lds_block is not None else cur["lds_block"],
                      units_per_thread if units_per_thread is not None
                      else cur["units_per_thread"],
                      graph_passes if graph_passes is not None else cur["graph_passes"], 0.0,
-                     band_lines if band_lines is not None else cur["band_lines"])
+                     band_lines if band_lines is not None else cur["band_lines"],
+                     line_tiles if line_tiles is not None else cur["line_tiles"])
         _check(lib().mcg_set_schedule(self.h, C.byref(s)))
 
     def last_run_time(self):

@@ -288,12 +288,16 @@ struct mcg_handle {
   // 2D band path: lines per band (-1 = chosen from the sizes, 0 = off) and the
   // (chain, first line, lines) list
   int band_lines = -1;
+  int line_tiles = 0;  // line-class path: LDS tiles (1) or lines from HBM (0)
   int32_t *d_band = nullptr;
   int n_bands = 0;
   int band_ring_bytes = 0;
   // line-class path (4/8 colourings): (chain, first line, lines) items
   int32_t *d_pair = nullptr;
   int n_pairs = 0;
+  int pair_lds = 0;  // dynamic LDS of the line-class launches (largest tile)
+  int tile_grid = 0;  // persistent workgroups of the tiled line-class launches
+  int n_cu = 256;     // compute units of the device
   double tuned_ms = 0.0;
   // hipGraph of graph_passes passes of the HBM path (rebuilt when the work
   // list, the series buffer or the schedule change)
@@ -795,12 +799,22 @@ int rebuild_work(mcg_handle *h) {
                       hipMemcpyHostToDevice));
   }
   // line-class path: whole colour-class lines per workgroup (used when the
-  // band path is not)
+  // band path is not); LDS tiles of class lines with their neighbour lines on
+  // the 16-site chunk path, else lines straight from HBM
   if (h->d_pair) hipFree(h->d_pair);
   h->d_pair = nullptr;
   h->n_pairs = 0;
+  h->pair_lds = 0;
   if (h->n_bands == 0 && (h->ncolor == 4 || h->ncolor == 8) && !h->fcc()) {
     const int blk = checker_pair_block();
+    // LDS per tile: units_per_thread 1, 2, 4 -> 24, 36, 48 KB
+    const int64_t budget = std::min<int64_t>(
+        checker_tile_max_bytes(),
+        1024 * (h->units_per_thread == 1 ? 24 : (h->units_per_thread == 2 ? 36 : 48)));
+    // tiles when requested and every active chain runs the 16-site chunk path
+    bool tiles = h->line_tiles != 0;
+    for (int c = 0; c < h->n_chains; ++c)
+      if (h->active[c] && (h->cp[c].L0 / 2 % 16 != 0 || h->cp[c].quad_only)) tiles = false;
     std::vector<int32_t> items;
     for (int c = 0; c < h->n_chains; ++c) {
       if (!h->active[c]) continue;
@@ -808,16 +822,55 @@ int rebuild_work(mcg_handle *h) {
       const int64_t half = P.L0 / 2;
       const bool chunked = half % 16 == 0 && !P.quad_only;
       const int64_t upl = chunked ? half / 16 : (half + 3) / 4;
+      const bool d3 = P.ndim == 3;
+      const int64_t h1 = P.L1 / 2, h2 = d3 ? P.L2 / 2 : 1;
       const int64_t nl = P.ncolor == 4 ? P.L1 / 2 : (int64_t)(P.L1 / 2) * (P.L2 / 2);
+      // tile TJ x TK with (2TJ+1)(2TK+1) lines of L0 bytes within the budget,
+      // the most class lines per staged line
+      int64_t TJ = 0, TK = 0;
+      if (tiles) {
+        const int64_t lines = budget / P.L0;
+        double best = 0.0;
+        for (int64_t tk = 1; tk <= (d3 ? h2 : 1); ++tk)
+          for (int64_t tj = 1; tj <= h1; ++tj) {
+            const int64_t staged = (2 * tj + 1) * (d3 ? 2 * tk + 1 : 1);
+            if (staged > lines || tj > 0xFFFF || tk > 0x7FFF) break;
+            const double score = (double)(tj * tk) / staged + 1e-9 * (double)(tj * tk);
+            if (score > best) {
+              best = score;
+              TJ = tj;
+              TK = tk;
+            }
+          }
+      }
+      if (TJ > 0) {
+        h->pair_lds = std::max<int>(
+            h->pair_lds, (int)((2 * TJ + 1) * (d3 ? 2 * TK + 1 : 1) * P.L0));
+        for (int64_t kh = 0; kh < h2; kh += TK)
+          for (int64_t jh = 0; jh < h1; jh += TJ) {
+            const int64_t tj = std::min(TJ, h1 - jh), tk = std::min(TK, h2 - kh);
+            items.push_back(c);
+            items.push_back((int32_t)jh);
+            items.push_back((int32_t)kh);
+            items.push_back((int32_t)(tj | (tk << 16)));
+          }
+        continue;
+      }
       // a workgroup takes units_per_thread units of each colour per thread
       const int64_t lpb = h->units_per_thread * std::max<int64_t>(1, blk / upl);
       for (int64_t l0 = 0; l0 < nl; l0 += lpb) {
         items.push_back(c);
         items.push_back((int32_t)l0);
         items.push_back((int32_t)std::min(lpb, nl - l0));
+        items.push_back(0);
       }
     }
-    h->n_pairs = (int)(items.size() / 3);
+    h->n_pairs = (int)(items.size() / 4);
+    if (h->pair_lds > 0) {
+      HIP_TRY(checker_pair_set_max_lds(h->pair_lds));
+      const int per_cu = std::max(1, std::min(8, (int)(LDS_LIMIT / (h->pair_lds + 1024))));
+      h->tile_grid = std::min(h->n_pairs, h->n_cu * per_cu);
+    }
     if (h->n_pairs) {
       HIP_TRY(hipMalloc(&h->d_pair, sizeof(int32_t) * items.size()));
       HIP_TRY(copy_sync(h, h->d_pair, items.data(), sizeof(int32_t) * items.size(),
@@ -837,7 +890,7 @@ void enqueue_pass(mcg_handle *h) {
   else if (h->n_pairs > 0)
     for (int lc = 0; lc < h->ncolor / 2; ++lc)
       launch_checker_pair(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_pair, h->n_pairs, lc,
-                          h->stream);
+                          h->pair_lds, h->tile_grid, h->stream);
   else if (h->n_work > 0)
     for (int color = 0; color < h->ncolor; ++color)
       launch_checker_color(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_work, h->n_work, color,
@@ -860,7 +913,7 @@ int build_graph(mcg_handle *h) {
       g, h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_work, h->n_work, h->ncolor,
       h->units_per_thread, h->d_series, h->series_cap, h->d_active, h->n_chains,
       h->sample_period, h->graph_passes, h->d_band, h->n_bands, h->band_ring_bytes, h->d_pair,
-      h->n_pairs);
+      h->n_pairs, h->pair_lds, h->tile_grid);
   if (e == hipSuccess) e = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
@@ -897,7 +950,7 @@ std::map<std::string, mcg_schedule> &tune_cache() {
 }
 
 // File format, one schedule per line:
-//   key lds_block units_per_thread graph_passes trial_ms band_lines
+//   key lds_block units_per_thread graph_passes trial_ms band_lines line_tiles
 bool tune_cache_lookup(const std::string &key, mcg_schedule *out) {
   std::lock_guard<std::mutex> lock(tune_mutex());
   auto &m = tune_cache();
@@ -911,12 +964,13 @@ bool tune_cache_lookup(const std::string &key, mcg_schedule *out) {
   FILE *f = std::fopen(path, "r");
   if (!f) return false;
   char k[128];
-  long long b, u, g, bl;
+  long long b, u, g, bl, lt;
   double ms;
   bool found = false;
-  while (std::fscanf(f, "%127s %lld %lld %lld %lf %lld", k, &b, &u, &g, &ms, &bl) == 6) {
+  while (std::fscanf(f, "%127s %lld %lld %lld %lf %lld %lld", k, &b, &u, &g, &ms, &bl, &lt) ==
+         7) {
     if (key == k) {  // the last line for a key wins
-      *out = mcg_schedule{b, u, g, ms, bl};
+      *out = mcg_schedule{b, u, g, ms, bl, lt};
       found = true;
     }
   }
@@ -931,9 +985,9 @@ void tune_cache_store(const std::string &key, const mcg_schedule &s) {
   const char *path = std::getenv("MCG_TUNE_CACHE");
   if (!path || !*path) return;
   if (FILE *f = std::fopen(path, "a")) {
-    std::fprintf(f, "%s %lld %lld %lld %.6f %lld\n", key.c_str(), (long long)s.lds_block,
+    std::fprintf(f, "%s %lld %lld %lld %.6f %lld %lld\n", key.c_str(), (long long)s.lds_block,
                  (long long)s.units_per_thread, (long long)s.graph_passes, s.trial_ms,
-                 (long long)s.band_lines);
+                 (long long)s.band_lines, (long long)s.line_tiles);
     std::fclose(f);
   }
 }
@@ -978,6 +1032,7 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
     return fail(MCG_ERR_HIP, std::string("device is not gfx950: ") + prop.gcnArchName);
 
   auto h = new mcg_handle();
+  h->n_cu = std::max(1, prop.multiProcessorCount);
   h->mode = desc->mode;
   h->lattice = desc->lattice;
   h->use_nlist = desc->use_nlist != 0;
@@ -1522,6 +1577,7 @@ int mcg_get_schedule(mcg_handle *h, mcg_schedule *out) {
   out->units_per_thread = h->units_per_thread;
   out->graph_passes = h->graph_passes;
   out->band_lines = h->band_lines;
+  out->line_tiles = h->line_tiles;
   out->trial_ms = h->tuned_ms;
   return MCG_OK;
 }
@@ -1536,10 +1592,13 @@ int mcg_set_schedule(mcg_handle *h, const mcg_schedule *s) {
     return fail(MCG_ERR_ARG, "graph_passes must be in [1, 256]");
   if (s->band_lines != -1 && (s->band_lines < 0 || s->band_lines > 4096 || (s->band_lines & 1)))
     return fail(MCG_ERR_ARG, "band_lines must be -1 (automatic), 0 (off) or even <= 4096");
+  if (s->line_tiles != 0 && s->line_tiles != 1) return fail(MCG_ERR_ARG, "line_tiles must be 0 or 1");
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->checker_block = (int)s->lds_block;
-  if (h->band_lines != (int)s->band_lines || h->units_per_thread != (int)s->units_per_thread)
+  if (h->band_lines != (int)s->band_lines || h->units_per_thread != (int)s->units_per_thread ||
+      h->line_tiles != (int)s->line_tiles)
     h->work_dirty = true;
+  h->line_tiles = (int)s->line_tiles;
   h->units_per_thread = (int)s->units_per_thread;
   h->graph_passes = (int)s->graph_passes;
   h->band_lines = (int)s->band_lines;
@@ -1575,14 +1634,18 @@ int mcg_autotune(mcg_handle *h, int64_t trial_passes, mcg_schedule *chosen) {
     std::vector<mcg_schedule> cands;
     if (h->checker_lds || h->fcc()) {
       for (int64_t b : {256, 512, 1024})
-        cands.push_back({b, h->units_per_thread, 1, 0.0, h->band_lines});
+        cands.push_back({b, h->units_per_thread, 1, 0.0, h->band_lines, h->line_tiles});
     } else {
-      for (int64_t u : {1, 2, 4})
-        for (int64_t g : {1, 8}) cands.push_back({h->checker_block, u, g, 0.0, 0});
+      const bool line_class = h->ncolor == 4 || h->ncolor == 8;
+      for (int64_t lt : {0, 1}) {
+        if (lt && !line_class) continue;
+        for (int64_t u : {1, 2, 4})
+          for (int64_t g : {1, 8}) cands.push_back({h->checker_block, u, g, 0.0, 0, lt});
+      }
       if (h->lattice == MCG_LATTICE_SQUARE)
         for (int64_t bl : {8, 16, 32, 64})
           for (int64_t g : {1, 8})
-            cands.push_back({h->checker_block, h->units_per_thread, g, 0.0, bl});
+            cands.push_back({h->checker_block, h->units_per_thread, g, 0.0, bl, 0});
     }
     double best_ms = 1e300;
     int rc = MCG_OK;

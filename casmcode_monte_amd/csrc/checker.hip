@@ -565,17 +565,18 @@ __global__ __launch_bounds__(HBM_BLOCK) void checker_color_kernel(
 // line (its other neighbours belong to other line classes), so a workgroup that
 // owns whole lines updates half 0, synchronises, and updates half 1: half the
 // launches, and half the passes over the lattice, of one launch per colour.
-// work: (chain, first colour-class line, lines) per item.
 // ---------------------------------------------------------------------------
 constexpr int PAIR_BLOCK = 256;
 
+// Work items are (chain, first colour-class line, lines, 0): lines straight
+// from HBM (every class line re-reads its neighbour lines).
 __global__ __launch_bounds__(PAIR_BLOCK) void checker_pair_kernel(
     const CheckerParams *__restrict__ Pp, CheckerState *__restrict__ Sp,
     int8_t *__restrict__ lat_pool, const uint32_t *__restrict__ tables,
     const int32_t *__restrict__ work, int line_class) {
   __shared__ int red[4][PAIR_BLOCK / 64];
-  const int c = work[3 * blockIdx.x], l0 = work[3 * blockIdx.x + 1];
-  const int nlines = work[3 * blockIdx.x + 2];
+  const int32_t *item = work + 4 * blockIdx.x;
+  const int c = item[0];
   const CheckerParams P = Pp[c];
   CheckerState *St = Sp + c;
   const LineGeom g = line_geom(P);
@@ -584,6 +585,7 @@ __global__ __launch_bounds__(PAIR_BLOCK) void checker_pair_kernel(
   uint8_t *lat = reinterpret_cast<uint8_t *>(lat_pool + P.lat_off);
   const int upl = chunk_path(g) ? (g.half >> 4) : g.qpl;  // units per half line
   int dS1 = 0, dS2 = 0, dM = 0, nacc = 0;
+  const int l0 = item[1], nlines = item[2];
   for (int a = 0; a < 2; ++a) {
     const int color = 2 * line_class + a;
     for (int task = threadIdx.x; task < nlines * upl; task += PAIR_BLOCK) {
@@ -619,6 +621,175 @@ __global__ __launch_bounds__(PAIR_BLOCK) void checker_pair_kernel(
     if (an)
       atomicAdd(reinterpret_cast<unsigned long long *>(&St->n_accept), (unsigned long long)an);
   }
+}
+
+// LDS-tiled line-class kernel (16-site chunk path).  Work items are (chain,
+// jh0, kh0, TJ | TK << 16): the TJ x TK class lines j = cj + 2*jh,
+// k = ck + 2*kh with jh in [jh0, jh0+TJ), kh in [kh0, kh0+TK) (2D: TK = 1,
+// k = 0).  The (2TJ+1) x (2TK+1) lines around them -- every line their sites
+// read -- are staged in LDS once, both halves are updated there and only the
+// class lines are written back, so each lattice line is read about once per
+// launch instead of once per class line that neighbours it.  Neighbour lines
+// have the other j or k parity, so they are never class lines of this launch
+// and tiles do not interact.  Workgroups are persistent: while a workgroup
+// updates one tile, the lines of its next tile are already on their way into
+// registers (TILE_PF = 12 16-byte words per thread, at most 48 KB of lines
+// per tile).  Random counters are those of the per-colour kernels (same line
+// index ln), so trajectories are identical.
+constexpr int TILE_PF = 12;
+
+struct TileGeom {
+  int c, jh0, kh0, TJ, TK, RJ, RK, n16;
+};
+
+__device__ __forceinline__ TileGeom tile_geom(const CheckerParams *__restrict__ Pp,
+                                              const int32_t *__restrict__ work, int it) {
+  TileGeom t;
+  const int32_t *item = work + 4 * it;
+  t.c = item[0];
+  t.jh0 = item[1];
+  t.kh0 = item[2];
+  t.TJ = item[3] & 0xFFFF;
+  t.TK = item[3] >> 16;
+  const bool d3 = Pp[t.c].ndim == 3;
+  t.RJ = 2 * t.TJ + 1;
+  t.RK = d3 ? 2 * t.TK + 1 : 1;
+  t.n16 = t.RJ * t.RK * (Pp[t.c].L0 >> 4);
+  return t;
+}
+
+__global__ __launch_bounds__(PAIR_BLOCK) void checker_tile_kernel(
+    const CheckerParams *__restrict__ Pp, CheckerState *__restrict__ Sp,
+    int8_t *__restrict__ lat_pool, const uint32_t *__restrict__ tables,
+    const int32_t *__restrict__ work, int n_items, int line_class) {
+  extern __shared__ __align__(16) uint8_t region[];
+  __shared__ int red[4][PAIR_BLOCK / 64];
+  __shared__ uint32_t tab[CHECKER_NTAB];  // acceptance table of the current tile's chain
+  uint4 *reg16 = reinterpret_cast<uint4 *>(region);
+  const int cj = line_class & 1, ck = (line_class >> 1) & 1;
+  // prefetch words as named registers (an array of this size would live in scratch)
+  uint4 pf0, pf1, pf2, pf3, pf4, pf5, pf6, pf7, pf8, pf9, pf10, pf11;
+// lines of tile ITEM into pf (thread t holds words t + r*PAIR_BLOCK)
+#define MCG_TILE_FETCH(ITEM)                                                                   \
+  do {                                                                                         \
+    const TileGeom ft = tile_geom(Pp, work, (ITEM));                                           \
+    const CheckerParams &Q = Pp[ft.c];                                                         \
+    const uint8_t *qlat = reinterpret_cast<const uint8_t *>(lat_pool + Q.lat_off);             \
+    const int qv16 = Q.L0 >> 4;                                                                \
+    const bool q3 = Q.ndim == 3;                                                               \
+    const int qjb = cj + 2 * ft.jh0 - 1, qkb = q3 ? ck + 2 * ft.kh0 - 1 : 0;                   \
+    const int64_t qplane = (int64_t)Q.L0 * Q.L1;                                               \
+    auto ld = [&](int r) -> uint4 {                                                            \
+      const int idx = threadIdx.x + r * PAIR_BLOCK;                                            \
+      if (idx >= ft.n16) return make_uint4(0, 0, 0, 0);                                        \
+      const int row = idx / qv16, col = idx - row * qv16;                                      \
+      const int dk = row / ft.RJ, dj = row - dk * ft.RJ;                                       \
+      const int jj = wrapm(qjb + dj, Q.L1), kk = q3 ? wrapm(qkb + dk, Q.L2) : 0;               \
+      return *reinterpret_cast<const uint4 *>(qlat + kk * qplane + (int64_t)jj * Q.L0 +       \
+                                              col * 16);                                       \
+    };                                                                                         \
+    pf0 = ld(0); pf1 = ld(1); pf2 = ld(2); pf3 = ld(3); pf4 = ld(4); pf5 = ld(5);             \
+    pf6 = ld(6); pf7 = ld(7); pf8 = ld(8); pf9 = ld(9); pf10 = ld(10); pf11 = ld(11);         \
+  } while (0)
+// pf into the LDS region of tile ITEM, and its chain's acceptance table
+#define MCG_TILE_STAGE(ITEM)                                                                   \
+  do {                                                                                         \
+    const TileGeom sg = tile_geom(Pp, work, (ITEM));                                           \
+    const int sn16 = sg.n16;                                                                   \
+    for (int i = threadIdx.x; i < CHECKER_NTAB; i += PAIR_BLOCK)                               \
+      tab[i] = tables[Pp[sg.c].tab_off + i];                                                   \
+    auto st = [&](int r, const uint4 &v) {                                                     \
+      const int idx = threadIdx.x + r * PAIR_BLOCK;                                            \
+      if (idx < sn16) reg16[idx] = v;                                                          \
+    };                                                                                         \
+    st(0, pf0); st(1, pf1); st(2, pf2); st(3, pf3); st(4, pf4); st(5, pf5);                   \
+    st(6, pf6); st(7, pf7); st(8, pf8); st(9, pf9); st(10, pf10); st(11, pf11);               \
+  } while (0)
+
+  int it = blockIdx.x;
+  if (it < n_items) {
+    MCG_TILE_FETCH(it);
+    MCG_TILE_STAGE(it);
+  }
+  __syncthreads();
+  while (it < n_items) {
+    const int nxt = it + (int)gridDim.x;
+    if (nxt < n_items) MCG_TILE_FETCH(nxt);  // in flight while this tile is updated
+    const TileGeom t = tile_geom(Pp, work, it);
+    const CheckerParams P = Pp[t.c];
+    CheckerState *St = Sp + t.c;
+    const LineGeom g = line_geom(P);
+    const int64_t pass = St->n_pass;
+    uint8_t *lat = reinterpret_cast<uint8_t *>(lat_pool + P.lat_off);
+    const int upl = g.half >> 4;
+    const bool d3 = P.ndim == 3;
+    const int L0 = P.L0, v16 = L0 >> 4, h1 = P.L1 >> 1;
+    const int64_t plane = (int64_t)L0 * P.L1;
+    auto line_at = [&](int dj, int dk) { return region + (dj + t.RJ * dk) * L0; };
+    int dS1 = 0, dS2 = 0, dM = 0, nacc = 0;
+    for (int a = 0; a < 2; ++a) {
+      const int color = 2 * line_class + a;
+      for (int task = threadIdx.x; task < t.TJ * t.TK * upl; task += PAIR_BLOCK) {
+        const int li = task / upl, u = task - li * upl;
+        const int tk = li / t.TJ, tj = li - tk * t.TJ;
+        const int ln = (t.jh0 + tj) + h1 * (t.kh0 + tk);
+        const int dj = 2 * tj + 1, dk = d3 ? 2 * tk + 1 : 0;
+        Lines Ls;
+        Ls.own = line_at(dj, dk);
+        Ls.jp = line_at(dj + 1, dk);
+        Ls.jm = line_at(dj - 1, dk);
+        Ls.kp = Ls.km = Ls.jpkp = Ls.jpkm = Ls.jmkp = Ls.jmkm = Ls.own;
+        if (d3) {
+          Ls.kp = line_at(dj, dk + 1);
+          Ls.km = line_at(dj, dk - 1);
+          Ls.jpkp = line_at(dj + 1, dk + 1);
+          Ls.jpkm = line_at(dj + 1, dk - 1);
+          Ls.jmkp = line_at(dj - 1, dk + 1);
+          Ls.jmkm = line_at(dj - 1, dk - 1);
+        }
+        update_chunk_at(P, g, Ls, a, ln, u << 4, tab, color, pass, dS1, dS2, dM, nacc);
+      }
+      __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < t.TJ * t.TK * v16; idx += PAIR_BLOCK) {
+      const int li = idx / v16, col = idx - li * v16;
+      const int tk = li / t.TJ, tj = li - tk * t.TJ;
+      const int j = cj + 2 * (t.jh0 + tj), k = d3 ? ck + 2 * (t.kh0 + tk) : 0;
+      *reinterpret_cast<uint4 *>(lat + k * plane + (int64_t)j * L0 + col * 16) =
+          reinterpret_cast<const uint4 *>(line_at(2 * tj + 1, d3 ? 2 * tk + 1 : 0))[col];
+    }
+    dS1 = wave_sum(dS1);
+    dS2 = wave_sum(dS2);
+    dM = wave_sum(dM);
+    nacc = wave_sum(nacc);
+    const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+    if (lane == 0) {
+      red[0][wid] = dS1;
+      red[1][wid] = dS2;
+      red[2][wid] = dM;
+      red[3][wid] = nacc;
+    }
+    __syncthreads();  // also: every read of this tile's lines is done
+    if (tid == 0) {
+      long long a1 = 0, a2 = 0, am = 0, an = 0;
+      for (int w = 0; w < PAIR_BLOCK / 64; ++w) {
+        a1 += red[0][w];
+        a2 += red[1][w];
+        am += red[2][w];
+        an += red[3][w];
+      }
+      if (a1) atomicAdd(reinterpret_cast<unsigned long long *>(&St->S1), (unsigned long long)a1);
+      if (a2) atomicAdd(reinterpret_cast<unsigned long long *>(&St->S2), (unsigned long long)a2);
+      if (am) atomicAdd(reinterpret_cast<unsigned long long *>(&St->M), (unsigned long long)am);
+      if (an)
+        atomicAdd(reinterpret_cast<unsigned long long *>(&St->n_accept), (unsigned long long)an);
+    }
+    if (nxt < n_items) MCG_TILE_STAGE(nxt);
+    __syncthreads();
+    it = nxt;
+  }
+#undef MCG_TILE_FETCH
+#undef MCG_TILE_STAGE
 }
 
 // ---------------------------------------------------------------------------
@@ -884,7 +1055,8 @@ hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, Checke
                                     mcg_sample *series, int64_t series_cap,
                                     const uint8_t *active, int n_chains, int64_t sample_period,
                                     int passes, const int32_t *band_work, int n_bands,
-                                    int ring_bytes, const int32_t *pair_work, int n_pairs) {
+                                    int ring_bytes, const int32_t *pair_work, int n_pairs,
+                                    int pair_lds, int tile_grid) {
   void *color_fn = upt == 1   ? reinterpret_cast<void *>(&checker_color_kernel<1>)
                    : upt == 2 ? reinterpret_cast<void *>(&checker_color_kernel<2>)
                               : reinterpret_cast<void *>(&checker_color_kernel<4>);
@@ -915,10 +1087,19 @@ hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, Checke
       if (e != hipSuccess) return e;
     } else if (n_pairs > 0) {
       for (int lc = 0; lc < ncolor / 2; ++lc) {
-        void *args[] = {(void *)&P, (void *)&St, (void *)&lat_pool, (void *)&tables,
-                        (void *)&pair_work, (void *)&lc};
-        hipError_t e = add(reinterpret_cast<void *>(&checker_pair_kernel), dim3(n_pairs),
-                           dim3(PAIR_BLOCK), args);
+        hipError_t e;
+        if (pair_lds > 0) {
+          void *args[] = {(void *)&P,         (void *)&St,      (void *)&lat_pool,
+                          (void *)&tables,    (void *)&pair_work, (void *)&n_pairs,
+                          (void *)&lc};
+          e = add(reinterpret_cast<void *>(&checker_tile_kernel), dim3(tile_grid),
+                  dim3(PAIR_BLOCK), args, (unsigned)pair_lds);
+        } else {
+          void *args[] = {(void *)&P, (void *)&St, (void *)&lat_pool, (void *)&tables,
+                          (void *)&pair_work, (void *)&lc};
+          e = add(reinterpret_cast<void *>(&checker_pair_kernel), dim3(n_pairs),
+                  dim3(PAIR_BLOCK), args);
+        }
         if (e != hipSuccess) return e;
       }
     } else if (n_work_items > 0)
@@ -949,10 +1130,21 @@ void launch_checker_band(const CheckerParams *P, CheckerState *St, int8_t *lat_p
 
 void launch_checker_pair(const CheckerParams *P, CheckerState *St, int8_t *lat_pool,
                          const uint32_t *tables, const int32_t *pair_work, int n_items,
-                         int line_class, hipStream_t stream) {
-  hipLaunchKernelGGL(checker_pair_kernel, dim3(n_items), dim3(PAIR_BLOCK), 0, stream, P, St,
-                     lat_pool, tables, pair_work, line_class);
+                         int line_class, int lds_bytes, int tile_grid, hipStream_t stream) {
+  if (lds_bytes > 0)
+    hipLaunchKernelGGL(checker_tile_kernel, dim3(tile_grid), dim3(PAIR_BLOCK), lds_bytes, stream,
+                       P, St, lat_pool, tables, pair_work, n_items, line_class);
+  else
+    hipLaunchKernelGGL(checker_pair_kernel, dim3(n_items), dim3(PAIR_BLOCK), 0, stream, P, St,
+                       lat_pool, tables, pair_work, line_class);
 }
+
+hipError_t checker_pair_set_max_lds(int bytes) {
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(checker_tile_kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+int checker_tile_max_bytes() { return TILE_PF * PAIR_BLOCK * 16; }
 
 int checker_pair_block() { return PAIR_BLOCK; }
 

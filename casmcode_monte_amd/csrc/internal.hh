@@ -141,14 +141,21 @@ hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, Checke
                                     mcg_sample *series, int64_t series_cap,
                                     const uint8_t *active, int n_chains, int64_t sample_period,
                                     int passes, const int32_t *band_work, int n_bands,
-                                    int ring_bytes, const int32_t *pair_work, int n_pairs);
+                                    int ring_bytes, const int32_t *pair_work, int n_pairs,
+                                    int pair_lds, int tile_grid);
 
-// Line-class path (4 and 8 colourings): one launch per line class over
-// (chain, first colour-class line, lines) items, both halves of each line.
+// Line-class path (4 and 8 colourings): one launch per line class, both
+// halves of each line.  lds_bytes == 0: items (chain, first colour-class line,
+// lines, 0) read straight from HBM, one workgroup per item.  lds_bytes > 0:
+// tiles (chain, jh0, kh0, TJ | TK << 16) of class lines staged with their
+// neighbour lines in LDS ((2TJ+1) x (2TK+1) x L0 bytes, 2D (2TJ+1) x L0, at
+// most checker_tile_max_bytes()), tile_grid persistent workgroups.
 void launch_checker_pair(const CheckerParams *P, CheckerState *St, int8_t *lat_pool,
                          const uint32_t *tables, const int32_t *pair_work, int n_items,
-                         int line_class, hipStream_t stream);
+                         int line_class, int lds_bytes, int tile_grid, hipStream_t stream);
+hipError_t checker_pair_set_max_lds(int bytes);
 int checker_pair_block();
+int checker_tile_max_bytes();
 
 // 2D band path: one launch over (chain, first line, lines) bands, then the
 // band-edge launch; ring_bytes = checker_band_ring_lines() * L0 (largest chain).

@@ -271,6 +271,9 @@ typedef struct mcg_schedule {
   double trial_ms;          /* mcg_autotune: time of the chosen schedule on its trial     */
   int64_t band_lines;       /* 2D HBM path: lines per band of the two-launch band kernels
                                (-1 = chosen from the sizes, 0 = one launch per colour)  */
+  int64_t line_tiles;       /* 4/8-colour HBM path (not banded): 1 = stage tiles of class
+                               lines with their neighbour lines in LDS, 0 = read lines
+                               from HBM per class line                                  */
 } mcg_schedule;
 
 /* Autotuner: times the candidate schedules on `trial_passes` passes of this

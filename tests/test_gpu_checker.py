@@ -279,3 +279,35 @@ def test_band_path_equals_colour_kernels(J2, band_lines):
                 np.testing.assert_array_equal(ref.get_occupation(c), r.get_occupation(c))
                 np.testing.assert_array_equal(ref.samples(c), r.samples(c))
                 assert ref.counters(c) == r.counters(c)
+
+
+@pytest.mark.parametrize("lattice,shape", [
+    (cm.LATTICE_SC, (64, 10, 6)),
+    (cm.LATTICE_SC, (32, 14, 10)),
+    (cm.LATTICE_SC, (96, 6, 18)),
+    (cm.LATTICE_SC, (32, 2, 2)),
+    (cm.LATTICE_SQUARE, (64, 30)),
+    (cm.LATTICE_SQUARE, (128, 6)),
+])
+def test_line_class_tiles_equal_other_paths(lattice, shape):
+    """The LDS-tiled line-class kernel (tiles sized by units_per_thread, with
+    partial tiles at the lattice edges and wrap-around neighbour lines) gives
+    the LDS-resident kernel's and the untiled line-class kernel's trajectories
+    bit for bit."""
+    print("case: line-class tiles", lattice, shape, flush=True)
+    ch = [dict(shape=shape, temperature=2500.0, exchange_potential=0.03, seed=123),
+          dict(shape=shape, temperature=1400.0, exchange_potential=-0.1, seed=124)]
+    ref = run(ch, lattice, 0.1, -0.02, 19)
+    quad = run(ch, lattice, 0.1, -0.02, 19,
+               options=cm.OPT_FORCE_STREAMING | cm.OPT_CHECKER_QUAD_PATH)
+    for u in (1, 2, 4):
+        for G in (1, 8):
+            r = cm.Run(ch, mode=cm.MODE_CHECKERBOARD, lattice=lattice, J1=0.1, J2=-0.02,
+                       options=cm.OPT_FORCE_STREAMING)
+            r.set_schedule(units_per_thread=u, graph_passes=G, line_tiles=1)
+            r.run(19)
+            for c in range(2):
+                for other in (ref, quad):
+                    np.testing.assert_array_equal(other.get_occupation(c), r.get_occupation(c))
+                    np.testing.assert_array_equal(other.samples(c), r.samples(c))
+                    assert other.counters(c) == r.counters(c)
