@@ -18,6 +18,7 @@ INCLUDE_DIR = os.path.join(os.path.dirname(HERE), "include")
 MODE_EXACT = 0
 MODE_CHECKERBOARD = 1
 MODE_CANONICAL = 2
+MODE_NFOLD = 3
 LATTICE_SQUARE = 2
 LATTICE_SC = 3
 LATTICE_FCC = 4
@@ -52,12 +53,13 @@ class Desc(C.Structure):
                 ("J1", C.c_double), ("J2", C.c_double), ("kb", C.c_double),
                 ("sample_period", C.c_int64), ("n_chains", C.c_int64),
                 ("chains", C.POINTER(ChainDesc)), ("options", C.c_int64),
-                ("eci", C.POINTER(C.c_double))]
+                ("eci", C.POINTER(C.c_double)), ("sample_time_period", C.c_double)]
 
 
 SAMPLE_DTYPE = np.dtype([("pass", np.int64), ("S1", np.int64), ("S2", np.int64),
                          ("M", np.int64), ("e_fold", np.float64), ("e_run", np.float64),
-                         ("S3", np.int64), ("T3", np.int64)])
+                         ("S3", np.int64), ("T3", np.int64), ("time", np.float64),
+                         ("weight", np.float64)])
 
 
 class Schedule(C.Structure):
@@ -126,7 +128,8 @@ class Run:
     """One handle of the C ABI: a batch of chains advanced together on the GPU."""
 
     def __init__(self, chains, mode=MODE_EXACT, lattice=LATTICE_SQUARE, J1=0.1, J2=0.0,
-                 kb=KB, use_nlist=True, sample_period=1, options=0, eci=None):
+                 kb=KB, use_nlist=True, sample_period=1, options=0, eci=None,
+                 sample_time_period=0.0):
         """chains: sequence of dicts with shape, temperature, exchange_potential, seed.
         FCC (lattice=LATTICE_FCC, mode CHECKERBOARD): shape in conventional cells and
         eci = (V0, V1, V2, V3, V4, Vt)."""
@@ -150,7 +153,7 @@ class Run:
             self.eci = tuple(float(v) for v in eci)
             eci_arr = (C.c_double * 6)(*self.eci)
         d = Desc(mode, lattice, int(use_nlist), J1, J2, kb, sample_period, self.n_chains, arr,
-                 options, eci_arr)
+                 options, eci_arr, sample_time_period)
         self._arr, self._eci_arr = arr, eci_arr
         h = C.c_void_p()
         _check(L.mcg_create(C.byref(d), C.byref(h)))
@@ -201,7 +204,7 @@ class Run:
                                           v.size))
 
     def rng_dump(self, c):
-        n = 313 if self.mode in (MODE_EXACT, MODE_CANONICAL) else 4
+        n = 313 if self.mode in (MODE_EXACT, MODE_CANONICAL, MODE_NFOLD) else 4
         w = np.zeros(n, dtype=np.uint64)
         _check(lib().mcg_rng_dump(self.h, c, w.ctypes.data_as(C.POINTER(C.c_uint64)), n))
         return w

@@ -17,7 +17,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libcasm_monte_mi355x.so")
-SOURCES = ["api.hip", "exact.hip", "canonical.hip", "checker.hip", "fcc.hip", "stats.hip"]
+SOURCES = ["api.hip", "exact.hip", "canonical.hip", "nfold.hip", "checker.hip", "fcc.hip",
+           "stats.hip"]
 HEADERS = ["internal.hh", "rng.hh", "walk.hh", "fcc.hh"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -268,8 +268,16 @@ struct mcg_handle {
   int32_t *d_list = nullptr;
   int canon_lds = 0;
   bool canonical() const { return mode == MCG_MODE_CANONICAL; }
+  // NFOLD: per-chain parameters (the class arrays share d_list)
+  NfoldParams *d_np = nullptr;
+  std::vector<NfoldParams> np_;
+  int nfold_lds = 0;
+  double sample_time_period = 0.0;
+  bool nfold() const { return mode == MCG_MODE_NFOLD; }
   // one sequential chain per wavefront, MT19937-64 state, native lattice layout
-  bool sequential() const { return mode == MCG_MODE_EXACT || mode == MCG_MODE_CANONICAL; }
+  bool sequential() const {
+    return mode == MCG_MODE_EXACT || mode == MCG_MODE_CANONICAL || mode == MCG_MODE_NFOLD;
+  }
 
   // CHECKERBOARD
   CheckerParams *d_cp = nullptr;
@@ -313,7 +321,7 @@ struct mcg_handle {
     if (stream) hipStreamSynchronize(stream);
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     void *ptrs[] = {d_lat, d_active, d_series, d_stats, d_eq, d_xp, d_xs, d_cp, d_cs, d_tab,
-                    d_work, d_ftab, d_tri, d_band, d_pair, d_kp, d_list};
+                    d_work, d_ftab, d_tri, d_band, d_pair, d_kp, d_list, d_np};
     for (void *p : ptrs)
       if (p) hipFree(p);
     if (ev0) hipEventDestroy(ev0);
@@ -530,6 +538,88 @@ int canonical_init_lists(mcg_handle *h, int64_t c, std::vector<int8_t> const &oc
   return MCG_OK;
 }
 
+// Parameters of one NFOLD chain: geometry and the ten class rates
+// min(1, exp(-beta dE_k)) for the SGC potential delta of class k (as the
+// exact mode's classes, basic_semigrand_canonical.hh:178-192).
+void fill_nfold_params(mcg_handle *h, int c, NfoldParams &P) {
+  auto const &ch = h->chains[c];
+  std::memset(&P, 0, sizeof(P));
+  P.rows = (int32_t)ch.shape[0];
+  P.cols = (int32_t)ch.shape[1];
+  P.n = (int32_t)(ch.shape[0] * ch.shape[1]);
+  P.div_magic = ((1ULL << 40) + (uint64_t)P.rows - 1) / (uint64_t)P.rows;
+  P.time_period = h->sample_time_period;
+  P.lat_off = h->lat_off[c];
+  const double beta = 1.0 / (h->kb * ch.temperature);
+  const double mu = ch.exchange_potential;
+  for (int sidx = 0; sidx < 2; ++sidx)
+    for (int m = 0; m < 5; ++m) {
+      const int s_old = sidx ? 1 : -1, new_occ = -s_old, nsum = 2 * m - 4;
+      const double dEf = -h->J1 * (new_occ - s_old) * nsum;
+      const double Ndx = (new_occ - s_old) / 2.0;
+      const double dE = dEf - mu * Ndx;
+      P.rate[sidx * 5 + m] = dE < 0.0 ? 1.0 : std::exp(-dE * beta);
+    }
+}
+
+void nfold_reset_state(ExactState &S, uint64_t seed) {
+  std::memset(&S, 0, sizeof(S));
+  int pos = 0;
+  mt_seed(seed, S.mt, &pos);
+  S.pos = pos;
+  S.pending_site = -1;
+  S.next_k = 1;
+}
+
+int build_nfold(mcg_handle *h) {
+  h->np_.resize(h->n_chains);
+  int max_lds = 0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    fill_nfold_params(h, c, h->np_[c]);
+    max_lds = std::max(max_lds, nfold_lds_bytes(h->np_[c]));
+  }
+  if (max_lds > EXACT_LDS_MAX)
+    return fail(MCG_ERR_ARG, "NFOLD mode keeps a chain's lattice and class array in LDS: "
+                             "lattice too large");
+  h->nfold_lds = max_lds;
+  HIP_TRY(nfold_set_max_lds((int)EXACT_LDS_MAX));
+  HIP_TRY(hipMalloc(&h->d_np, sizeof(NfoldParams) * h->n_chains));
+  HIP_TRY(hipMalloc(&h->d_xs, sizeof(ExactState) * h->n_chains));
+  HIP_TRY(hipMalloc(&h->d_list, sizeof(int32_t) * (size_t)h->lat_bytes));
+  HIP_TRY(copy_sync(h, h->d_np, h->np_.data(), sizeof(NfoldParams) * h->n_chains,
+                    hipMemcpyHostToDevice));
+  std::vector<ExactState> xs(h->n_chains);
+  for (int c = 0; c < h->n_chains; ++c) nfold_reset_state(xs[c], h->chains[c].seed);
+  HIP_TRY(copy_sync(h, h->d_xs, xs.data(), sizeof(ExactState) * h->n_chains,
+                    hipMemcpyHostToDevice));
+  return MCG_OK;
+}
+
+// The class-partitioned site array of an NFOLD chain, classes in order and
+// sites in site order within a class; drops the chain's pending event (it was
+// selected on the previous occupation).
+int nfold_init_array(mcg_handle *h, int64_t c, std::vector<int8_t> const &occ) {
+  const int64_t rows = h->chains[c].shape[0], cols = h->chains[c].shape[1], n = rows * cols;
+  auto cls = [&](int64_t l) {
+    const int64_t i = l % rows, j = l / rows;
+    const int nsum = occ[(i + 1) % rows + rows * j] + occ[(i + rows - 1) % rows + rows * j] +
+                     occ[i + rows * ((j + 1) % cols)] + occ[i + rows * ((j + cols - 1) % cols)];
+    return (occ[l] > 0 ? 5 : 0) + (nsum + 4) / 2;
+  };
+  std::vector<int32_t> arr;
+  arr.reserve(n);
+  for (int k = 0; k < 10; ++k)
+    for (int64_t l = 0; l < n; ++l)
+      if (cls(l) == k) arr.push_back((int32_t)l);
+  HIP_TRY(copy_sync(h, h->d_list + h->lat_off[c], arr.data(), sizeof(int32_t) * n,
+                    hipMemcpyHostToDevice));
+  ExactState st;
+  HIP_TRY(copy_sync(h, &st, h->d_xs + c, sizeof(ExactState), hipMemcpyDeviceToHost));
+  st.pending_site = -1;
+  HIP_TRY(copy_sync(h, h->d_xs + c, &st, sizeof(ExactState), hipMemcpyHostToDevice));
+  return MCG_OK;
+}
+
 // Entries of a chain's acceptance table (indexed by checker_class).
 int checker_ntab(mcg_handle *) { return CHECKER_NTAB; }
 
@@ -717,7 +807,8 @@ int reset_observables(mcg_handle *h, int64_t c, std::vector<int8_t> const &occ) 
     const double Ef = ising_per_supercell(occ, ch.shape[0], ch.shape[1], h->J1, h->use_nlist);
     const double Nx = static_cast<double>(n + M) / 2.0;
     const double E = h->canonical() ? Ef : Ef - ch.exchange_potential * Nx;
-    st.e_run = E / (double)n;
+    // NFOLD keeps S and M only (nfold.hh sets no state properties)
+    st.e_run = h->nfold() ? 0.0 : E / (double)n;
     HIP_TRY(copy_sync(h, h->d_xs + c, &st, sizeof(ExactState), hipMemcpyHostToDevice));
   } else {
     CheckerState st;
@@ -1004,8 +1095,13 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
   if (!desc || !out) return fail(MCG_ERR_ARG, "null argument");
   *out = nullptr;
   if (desc->mode != MCG_MODE_EXACT && desc->mode != MCG_MODE_CHECKERBOARD &&
-      desc->mode != MCG_MODE_CANONICAL)
+      desc->mode != MCG_MODE_CANONICAL && desc->mode != MCG_MODE_NFOLD)
     return fail(MCG_ERR_ARG, "unknown mode");
+  if (desc->mode == MCG_MODE_NFOLD && (desc->lattice != MCG_LATTICE_SQUARE || desc->J2 != 0.0))
+    return fail(MCG_ERR_ARG, "NFOLD mode supports the 2D square nearest-neighbour model only");
+  if (desc->sample_time_period < 0.0 ||
+      (desc->sample_time_period > 0.0 && desc->mode != MCG_MODE_NFOLD))
+    return fail(MCG_ERR_ARG, "sample_time_period must be 0, or > 0 in NFOLD mode");
   if (desc->lattice != MCG_LATTICE_SQUARE && desc->lattice != MCG_LATTICE_SC &&
       desc->lattice != MCG_LATTICE_FCC)
     return fail(MCG_ERR_ARG, "unknown lattice");
@@ -1041,6 +1137,7 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
   h->kb = desc->kb;
   h->sample_period = desc->sample_period;
   h->options = desc->options;
+  h->sample_time_period = desc->sample_time_period;
   if (h->fcc())
     for (int i = 0; i < 6; ++i) h->eci[i] = desc->eci[i];
   h->n_chains = (int)desc->n_chains;
@@ -1067,9 +1164,9 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
       delete h;
       return fail(MCG_ERR_ARG, "EXACT mode supports at most 2^24 sites and rows < 65536");
     }
-    if (h->canonical() && n > (1LL << 16)) {
+    if ((h->canonical() || h->nfold()) && n > (1LL << 16)) {
       delete h;
-      return fail(MCG_ERR_ARG, "CANONICAL mode supports at most 65536 sites per chain");
+      return fail(MCG_ERR_ARG, "CANONICAL and NFOLD modes support at most 65536 sites per chain");
     }
     if (n > (1LL << 31)) {
       delete h;
@@ -1104,6 +1201,7 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
     return bail(fail(MCG_ERR_HIP, "allocation failed"));
   rc = (h->mode == MCG_MODE_EXACT)  ? build_exact(h)
        : h->canonical()              ? build_canonical(h)
+       : h->nfold()                  ? build_nfold(h)
        : h->fcc()                    ? build_fcc(h)
                                      : build_checker(h);
   if (rc != MCG_OK) return bail(rc);
@@ -1111,6 +1209,7 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
   for (int c = 0; c < h->n_chains; ++c) {
     std::vector<int8_t> occ(h->sites(c), h->fcc() ? 0 : 1);
     rc = reset_observables(h, c, occ);
+    if (rc == MCG_OK && h->nfold()) rc = nfold_init_array(h, c, occ);
     if (rc != MCG_OK) return bail(rc);
   }
   rc = ensure_series(h, 64);
@@ -1137,6 +1236,10 @@ int mcg_set_occupation(mcg_handle *h, int64_t chain, const int8_t *occ, int64_t 
     HIP_TRY(copy_sync(h, h->d_lat + h->lat_off[chain], v.data(), n, hipMemcpyHostToDevice));
     if (h->canonical()) {
       const int rc = canonical_init_lists(h, chain, v);
+      if (rc != MCG_OK) return rc;
+    }
+    if (h->nfold()) {
+      const int rc = nfold_init_array(h, chain, v);
       if (rc != MCG_OK) return rc;
     }
   } else {
@@ -1221,8 +1324,8 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
     return fail(MCG_ERR_CAPACITY, "shape has more sites than the chain slot holds");
   if (h->mode == MCG_MODE_EXACT && nd.shape[0] >= (1LL << 16))
     return fail(MCG_ERR_ARG, "EXACT mode supports rows < 65536");
-  if (h->canonical() && n > (1LL << 16))
-    return fail(MCG_ERR_ARG, "CANONICAL mode supports at most 65536 sites per chain");
+  if ((h->canonical() || h->nfold()) && n > (1LL << 16))
+    return fail(MCG_ERR_ARG, "CANONICAL and NFOLD modes support at most 65536 sites per chain");
 
   HIP_TRY(hipStreamSynchronize(h->stream));
   const mcg_chain_desc old = h->chains[chain];
@@ -1242,6 +1345,18 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
     int pos = 0;
     mt_seed(nd.seed, S.mt, &pos);
     S.pos = pos;
+    HIP_TRY(copy_sync(h, h->d_xs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
+  } else if (h->nfold()) {
+    NfoldParams P;
+    fill_nfold_params(h, (int)chain, P);
+    if (nfold_lds_bytes(P) > h->nfold_lds) {
+      h->chains[chain] = old;
+      return fail(MCG_ERR_CAPACITY, "shape needs more LDS than the handle was created with");
+    }
+    h->np_[chain] = P;
+    HIP_TRY(copy_sync(h, h->d_np + chain, &P, sizeof(P), hipMemcpyHostToDevice));
+    ExactState S;
+    nfold_reset_state(S, nd.seed);
     HIP_TRY(copy_sync(h, h->d_xs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
   } else if (h->canonical()) {
     CanonParams P;
@@ -1300,12 +1415,15 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
                     (size_t)((h->capacity[chain] + 255) & ~255LL)));
   h->n_samples[chain] = 0;
   h->n_pass[chain] = 0;
-  return reset_observables(h, chain, std::vector<int8_t>(n, (int8_t)fill));
+  const int rc = reset_observables(h, chain, std::vector<int8_t>(n, (int8_t)fill));
+  if (rc != MCG_OK || !h->nfold()) return rc;
+  return nfold_init_array(h, chain, std::vector<int8_t>(n, (int8_t)fill));
 }
 
 int mcg_get_occ_location(mcg_handle *h, int64_t chain, int64_t *loc, int64_t n) {
   if (!valid_chain(h, chain) || !loc) return fail(MCG_ERR_ARG, "bad chain or buffer");
-  if (!h->canonical()) return fail(MCG_ERR_STATE, "occupant lists exist in CANONICAL mode only");
+  if (!h->canonical() && !h->nfold())
+    return fail(MCG_ERR_STATE, "site lists exist in CANONICAL and NFOLD modes only");
   if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "list size mismatch");
   HIP_TRY(hipStreamSynchronize(h->stream));
   std::vector<int32_t> v(n);
@@ -1395,6 +1513,9 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
     launch_canonical(h->d_kp, h->d_xs, h->d_lat, h->d_list, h->d_series, h->series_cap,
                      h->d_active, h->n_chains, n_passes, h->sample_period, h->canon_lds,
                      h->stream);
+  } else if (h->nfold()) {
+    launch_nfold(h->d_np, h->d_xs, h->d_lat, h->d_list, h->d_series, h->series_cap,
+                 h->d_active, h->n_chains, n_passes, h->sample_period, h->nfold_lds, h->stream);
   } else if (h->fcc()) {
     HIP_TRY(hipGetLastError());
     launch_fcc(h->d_cp, h->d_cs, h->d_lat, h->d_ftab, h->d_tri, h->d_series, h->series_cap,
@@ -1423,11 +1544,27 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
   HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   h->last_ms = ms;
   h->last_attempts = attempts;
+  const bool time_grid = h->nfold() && h->sample_time_period > 0.0;
   for (int c = 0; c < h->n_chains; ++c) {
     if (!h->active[c]) continue;
     int64_t p0 = h->n_pass[c];
-    h->n_samples[c] += (p0 + n_passes) / h->sample_period - p0 / h->sample_period;
+    if (!time_grid) h->n_samples[c] += (p0 + n_passes) / h->sample_period - p0 / h->sample_period;
     h->n_pass[c] = p0 + n_passes;
+  }
+  if (time_grid) {  // the number of time-grid records is known after the run
+    std::vector<ExactState> xs(h->n_chains);
+    HIP_TRY(copy_sync(h, xs.data(), h->d_xs, sizeof(ExactState) * h->n_chains,
+                      hipMemcpyDeviceToHost));
+    int64_t over = 0;
+    for (int c = 0; c < h->n_chains; ++c) {
+      if (!h->active[c]) continue;
+      over = std::max(over, xs[c].n_samples - h->series_cap);
+      h->n_samples[c] = std::min(xs[c].n_samples, h->series_cap);
+    }
+    if (over > 0)
+      return fail(MCG_ERR_CAPACITY,
+                  "NFOLD time-grid samples exceeded the sample series by " + std::to_string(over) +
+                      " records (they were dropped); reserve more with mcg_reserve_samples");
   }
   return MCG_OK;
 }
@@ -1521,6 +1658,9 @@ int mcg_stats_compute(mcg_handle *h, int64_t chain, int64_t observable, int64_t 
                       double confidence, mcg_stats *out) {
   if (!valid_chain(h, chain) || !out) return fail(MCG_ERR_ARG, "bad chain or buffer");
   if (observable < 0 || observable > 2) return fail(MCG_ERR_ARG, "bad observable");
+  if (h->nfold())
+    return fail(MCG_ERR_STATE,
+                "NFOLD samples carry weights: compute statistics from the records' weights");
   const int64_t count = h->n_samples[chain] - first;
   if (first < 0 || count <= 0)
     return fail(MCG_ERR_ARG, "Error in BasicStatisticsCalculator: observations.size()==0");
@@ -1558,6 +1698,9 @@ int mcg_equilibration(mcg_handle *h, int64_t chain, int64_t observable, double p
   if (!valid_chain(h, chain) || !is_equilibrated || !n_samples_for_equilibration)
     return fail(MCG_ERR_ARG, "bad chain or buffer");
   if (observable < 0 || observable > 2) return fail(MCG_ERR_ARG, "bad observable");
+  if (h->nfold())
+    return fail(MCG_ERR_STATE,
+                "NFOLD samples carry weights: compute statistics from the records' weights");
   const int64_t count = h->n_samples[chain];
   if (count <= 0) return fail(MCG_ERR_ARG, "Error in equilibration_check: observations.size()==0");
   launch_equilibration(h->d_series + chain * h->series_cap, count,

@@ -34,14 +34,19 @@ struct ExactParams {
   int64_t lat_off;           // byte offset of this chain's lattice in the pool
 };
 
-// Read-write per-chain state.
+// Read-write per-chain state (EXACT, CANONICAL and NFOLD).
 struct ExactState {
   uint64_t mt[MT_N];  // std::mt19937_64 state words
   int64_t pos;        // index of the next output (312: twist first)
   int64_t n_pass, n_accept, n_reject;
   int64_t S, M;       // exact bond sum and magnetisation
   double e_run;       // running potential energy per unit cell
-  int64_t n_samples;  // records written to the series
+  int64_t n_samples;  // records the run produced (written while below the capacity)
+  // NFOLD
+  double time;           // simulated time
+  int64_t pending_site;  // event selected at the end of a sampled pass, or -1
+  double pending_dt;     // its time increment
+  int64_t next_k;        // next time-grid sample index (sample at next_k * period)
 };
 
 void launch_exact(const ExactParams *P, ExactState *S, int8_t *lat_pool, mcg_sample *series,
@@ -87,6 +92,25 @@ void launch_canonical(const CanonParams *P, ExactState *S, int8_t *lat_pool, int
                       hipStream_t stream);
 // Dynamic LDS limit of the canonical kernel (its static LDS is ~10 KB).
 hipError_t canonical_set_max_lds(int bytes);
+
+// ---------------------------------------------------------------------------
+// NFOLD mode: rejection-free single-site flips, one chain per wavefront.
+// ---------------------------------------------------------------------------
+struct NfoldParams {
+  int32_t rows, cols, n;
+  int32_t pad;
+  uint64_t div_magic;   // floor(l / rows) = (l * div_magic) >> 40
+  double rate[10];      // Metropolis rate of class k = (s > 0) * 5 + (nbr_sum + 4) / 2
+  double time_period;   // > 0: samples on the time grid k * time_period; else by pass
+  int64_t lat_off;      // lattice byte offset; the class array starts at entry lat_off
+};
+
+inline int nfold_lds_bytes(const NfoldParams &P) { return ((P.n + 15) & ~15) + 4 * P.n + 64; }
+
+void launch_nfold(const NfoldParams *P, ExactState *S, int8_t *lat_pool, int32_t *list_pool,
+                  mcg_sample *series, int64_t series_cap, const uint8_t *active, int n_chains,
+                  int64_t n_passes, int64_t sample_period, int lds_bytes, hipStream_t stream);
+hipError_t nfold_set_max_lds(int bytes);
 
 // ---------------------------------------------------------------------------
 // CHECKERBOARD mode

@@ -55,6 +55,21 @@ extern "C" {
 #define MCG_MODE_EXACT 0
 #define MCG_MODE_CHECKERBOARD 1
 #define MCG_MODE_CANONICAL 2
+/* NFOLD: the N-fold way (rejection-free kinetic Monte Carlo) of nfold.hh:78-149
+ *   for the SGC square-lattice Ising model (J1), one sequential chain per
+ *   wavefront.  Every event flips one site chosen with probability
+ *   proportional to its Metropolis rate min(1, exp(-beta dE)); the time
+ *   advances by -log1p(-u)/R (R the total rate).  The reference takes its event
+ *   selector from outside the repository; the one here groups sites by rate
+ *   class (10 classes, one class-partitioned site array), draws a class with
+ *   one canonical word, a site in it with one Lemire word and the time with one
+ *   more word.  One pass = N events.  Samples: by pass (the state after the
+ *   pass, weighted by the time increment of the next event, which is selected
+ *   then) or, with mcg_desc.sample_time_period > 0, on the time grid (one record
+ *   per residence interval holding grid points, weighted by their number).
+ *   Occupations, counts and the event sequence are bit-identical to the
+ *   oracle's; times agree to the rounding of log1p. */
+#define MCG_MODE_NFOLD 3
 
 #define MCG_LATTICE_SQUARE 2 /* 2D, shape[2] == 1 */
 #define MCG_LATTICE_SC 3     /* 3D simple cubic */
@@ -103,6 +118,8 @@ typedef struct mcg_desc {
   int64_t options;       /* bit mask of MCG_OPT_*                             */
   const double *eci;     /* MCG_LATTICE_FCC: 6 ECIs (V0, V1, V2, V3, V4, Vt), eV;
                             otherwise ignored                                 */
+  double sample_time_period; /* NFOLD: > 0 samples on the time grid k*period
+                                (nfold.hh:105-108 BY_TIME); 0 samples by pass   */
 } mcg_desc;
 
 /* CHECKERBOARD: always use the HBM work-list kernels, even for lattices that
@@ -133,6 +150,10 @@ typedef struct mcg_sample {
                      (occupation_metropolis.hh:141); otherwise 0             */
   int64_t S3;     /* FCC: third-shell pair count; otherwise 0                 */
   int64_t T3;     /* FCC: nearest-neighbour triangle count; otherwise 0       */
+  double time;    /* NFOLD: simulated time of the sampled state (time grid: the
+                     first grid point of the record); otherwise 0           */
+  double weight;  /* NFOLD: sample weight (by pass: residence time of the state;
+                     time grid: number of grid points); otherwise 0         */
 } mcg_sample;
 /* FCC records: M = N1 (occupied sites), S1 = P1, S2 = P2, S3 = P3 (pairs of
  * shells 1-3 with both sites occupied), T3 = occupied nearest-neighbour
@@ -171,7 +192,8 @@ int mcg_get_occupation(mcg_handle *h, int64_t chain, int8_t *occ, int64_t n);
  * 1.  They are OccLocation's m_loc[0] and m_loc[1] (OccLocation.hh:204-206;
  * Mol ids are site indices because every site mutates) and decide which site
  * each later choose_mol draw picks (OccLocation.hh:254-259).  Setting checks
- * that every site appears once, in the list of its species. */
+ * that every site appears once, in the list of its species.  NFOLD (get only):
+ * the site array partitioned by rate class, in class order. */
 int mcg_get_occ_location(mcg_handle *h, int64_t chain, int64_t *loc, int64_t n);
 int mcg_set_occ_location(mcg_handle *h, int64_t chain, const int64_t *loc, int64_t n);
 

@@ -449,4 +449,39 @@ int orc_canonical_run(int64_t rows, int64_t cols, double J, double temperature,
   }
 }
 
+// N-fold way run (refcpu::nfold_run).  occ: rows*cols spins in, final spins
+// out; engine: 313 words in and out.  Per sample (at most capacity entries;
+// returns the number of samples): S, M, time, weight.  arr_out: the class-partitioned site array
+// (rows*cols).  scalars: final time, pending dt; pending site in *pending.
+int64_t orc_nfold_run(int64_t rows, int64_t cols, double J, double temperature, double mu,
+                      int64_t n_passes, int64_t sample_period, double time_period,
+                      int64_t capacity, int8_t *occ, uint64_t *engine, int64_t *S, int64_t *M,
+                      double *time, double *weight, int64_t *arr_out, double *scalars,
+                      int64_t *pending) {
+  IsingConfiguration cfg(rows, cols, 1);
+  for (long l = 0; l < cfg.n_sites; ++l) cfg.occupation[l] = occ[l] > 0 ? 1 : -1;
+  std::mt19937_64 eng;
+  {
+    std::ostringstream os;
+    for (int i = 0; i < 313; ++i) os << engine[i] << (i < 312 ? " " : "");
+    std::istringstream is(os.str());
+    is >> eng;
+  }
+  NfoldRunOutput r =
+      nfold_run(cfg, J, temperature, mu, eng, n_passes, sample_period, time_period);
+  for (long l = 0; l < cfg.n_sites; ++l) occ[l] = (int8_t)r.final_occ[l];
+  engine_words(eng, engine);
+  for (size_t k = 0; k < r.S.size() && (int64_t)k < capacity; ++k) {
+    S[k] = r.S[k];
+    M[k] = r.M[k];
+    time[k] = r.time[k];
+    weight[k] = r.weight[k];
+  }
+  for (long l = 0; l < cfg.n_sites; ++l) arr_out[l] = r.arr[l];
+  scalars[0] = r.final_time;
+  scalars[1] = r.pending_dt;
+  *pending = r.pending_site;
+  return (int64_t)r.S.size();
+}
+
 }  // extern "C"

@@ -1161,4 +1161,207 @@ inline CanonicalRunOutput canonical_run(Index rows, Index cols, std::vector<int>
   return out;
 }
 
+// ---------------------------------------------------------------------------
+// N-fold way (rejection-free) for the SGC Ising model: the loop of
+// nfold.hh:78-149 (select an event and its time increment, sample by count
+// before applying it with the increment as the sample weight, apply, advance
+// time) with a class-based event selector defined here (the reference takes
+// its selector from outside the repository).  Events are single-site flips;
+// a site's rate is its Metropolis probability min(1, exp(-beta dE)) for the
+// SGC potential delta of class k = (s > 0) * 5 + (nbr_sum + 4) / 2 (the
+// exact mode's classes).  Sites are kept in one array partitioned by class
+// (class k at [start[k], start[k+1])); per event:
+//   R = sum_k n_k r_k (k = 0..9, FP64 in order);
+//   class: the first k whose running sum exceeds u1 * R (u1 the canonical
+//     double of one engine word; the last non-empty class if rounding leaves
+//     none);
+//   site: arr[start[k] + p], p uniform in [0, n_k) (one Lemire word);
+//   dt = -log1p(-u3) / R (u3 the canonical double of one more word);
+//   flip the site, then re-class the site and its neighbours (+i, -i, +j,
+//   -j) in that order; a site moving from class a to b < a swaps with the
+//   first entry of each segment it leaves, to b > a with the last.
+// ---------------------------------------------------------------------------
+struct NfoldChain {
+  IsingConfiguration config;
+  double J = 0.1, mu = 0.0, beta = 1.0;
+  double rate[10];
+  std::vector<long> arr, pos;
+  long start[11];
+  double time = 0.0;
+
+  int class_of(long l) const {
+    Index i = l % config.rows, j = l / config.rows;
+    int nsum = config.occ(config.to_linear(config.within(i + 1, 0), j)) +
+               config.occ(config.to_linear(config.within(i - 1, 0), j)) +
+               config.occ(config.to_linear(i, config.within(j + 1, 1))) +
+               config.occ(config.to_linear(i, config.within(j - 1, 1)));
+    return (config.occ(l) > 0 ? 5 : 0) + (nsum + 4) / 2;
+  }
+
+  NfoldChain(IsingConfiguration c, double _J, double temperature, double _mu)
+      : config(std::move(c)), J(_J), mu(_mu), beta(1.0 / (KB * temperature)) {
+    for (int sidx = 0; sidx < 2; ++sidx)
+      for (int m = 0; m < 5; ++m) {
+        const int s_old = sidx ? 1 : -1, new_occ = -s_old, nsum = 2 * m - 4;
+        // model.hh:312-314 and basic_semigrand_canonical.hh:188-191, as the exact mode
+        const double dEf = -J * (new_occ - s_old) * nsum;
+        const double Ndx = (new_occ - s_old) / 2.0;
+        const double dE = dEf - mu * Ndx;
+        rate[sidx * 5 + m] = dE < 0.0 ? 1.0 : std::exp(-dE * beta);
+      }
+    const long n = config.n_sites;
+    arr.resize(n);
+    pos.resize(n);
+    long count[10] = {0};
+    for (long l = 0; l < n; ++l) count[class_of(l)]++;
+    start[0] = 0;
+    for (int k = 0; k < 10; ++k) start[k + 1] = start[k] + count[k];
+    long fill[10];
+    for (int k = 0; k < 10; ++k) fill[k] = start[k];
+    for (long l = 0; l < n; ++l) {
+      const int k = class_of(l);
+      arr[fill[k]] = l;
+      pos[l] = fill[k]++;
+    }
+  }
+
+  int class_at(long p) const {
+    int k = 0;
+    while (start[k + 1] <= p) ++k;
+    return k;
+  }
+
+  void swap_at(long p, long q) {
+    std::swap(arr[p], arr[q]);
+    pos[arr[p]] = p;
+    pos[arr[q]] = q;
+  }
+
+  void move(long l, int to) {
+    int a = class_at(pos[l]);
+    while (a < to) {  // become the first entry of segment a+1
+      swap_at(pos[l], start[a + 1] - 1);
+      start[a + 1]--;
+      ++a;
+    }
+    while (a > to) {  // become the last entry of segment a-1
+      swap_at(pos[l], start[a]);
+      start[a]++;
+      --a;
+    }
+  }
+
+  // select an event: the site and its time increment (three words)
+  double select(std::mt19937_64 &engine, long *site) const {
+    double R = 0.0;
+    for (int k = 0; k < 10; ++k) R += (double)(start[k + 1] - start[k]) * rate[k];
+    const double target = std::generate_canonical<double, 53>(engine) * R;
+    double run = 0.0;
+    int k = -1, last = -1;
+    for (int q = 0; q < 10; ++q) {
+      const long nq = start[q + 1] - start[q];
+      if (nq == 0) continue;
+      last = q;
+      run += (double)nq * rate[q];
+      if (target < run) {
+        k = q;
+        break;
+      }
+    }
+    if (k < 0) k = last;
+    const long nk = start[k + 1] - start[k];
+    const long p = std::uniform_int_distribution<long>(0, nk - 1)(engine);
+    *site = arr[start[k] + p];
+    const double u3 = std::generate_canonical<double, 53>(engine);
+    return -std::log1p(-u3) / R;
+  }
+
+  void apply(long l) {
+    config.set_occ(l, -config.occ(l));
+    Index i = l % config.rows, j = l / config.rows;
+    const long nb[5] = {l, config.to_linear(config.within(i + 1, 0), j),
+                        config.to_linear(config.within(i - 1, 0), j),
+                        config.to_linear(i, config.within(j + 1, 1)),
+                        config.to_linear(i, config.within(j - 1, 1))};
+    for (long x : nb) move(x, class_of(x));
+  }
+};
+
+struct NfoldRunOutput {
+  std::vector<long long> S, M;  // state at each sample
+  std::vector<double> time, weight;
+  std::vector<int> final_occ;
+  std::vector<long> arr;
+  double final_time = 0.0;
+  long pending_site = -1;
+  double pending_dt = 0.0;
+};
+
+// n_passes passes of N events (nfold.hh:112-146), sampled one of two ways:
+//  * by pass (time_period <= 0): the sample of pass count p
+//    (p % sample_period == 0) holds the state after p*N events and, as its
+//    weight, the time increment of the event that follows (nfold.hh:93-107).
+//    That event is selected at the end of pass p and applied first in pass
+//    p+1, which leaves the stream order of one selection per event unchanged.
+//    (Every event flips one spin, so with N even these samples only see
+//    states of one flip parity; see tests/test_oracle_nfold.py.)
+//  * by time (time_period > 0): samples at times k * time_period (k >= 1),
+//    each holding the state in force at that time, weight 1
+//    (nfold.hh:105-108, 128-129); the grid points inside one residence
+//    interval are one record, weighted by their number, timed at the first.
+inline NfoldRunOutput nfold_run(IsingConfiguration config, double J, double temperature,
+                                double mu, std::mt19937_64 &engine, long n_passes,
+                                long sample_period = 1, double time_period = 0.0) {
+  NfoldChain ch(std::move(config), J, temperature, mu);
+  NfoldRunOutput out;
+  const long n = ch.config.n_sites;
+  long pending = -1;
+  double pending_dt = 0.0;
+  long next_k = 1;
+  for (long p = 1; p <= n_passes; ++p) {
+    for (long s = 0; s < n; ++s) {
+      long l;
+      double dt;
+      if (pending >= 0) {
+        l = pending;
+        dt = pending_dt;
+        pending = -1;
+      } else {
+        dt = ch.select(engine, &l);
+      }
+      if (time_period > 0.0) {
+        // first grid index k with k * period >= t_end (the predicate decides at the edge)
+        const double t_end = ch.time + dt;
+        long k = (long)std::ceil(t_end / time_period);
+        if (k < next_k) k = next_k;
+        while (k > next_k && (double)(k - 1) * time_period >= t_end) --k;
+        while ((double)k * time_period < t_end) ++k;
+        const long m = k - next_k;
+        if (m > 0) {
+          out.S.push_back(bond_sum(ch.config));
+          out.M.push_back(magnetisation(ch.config));
+          out.time.push_back((double)next_k * time_period);
+          out.weight.push_back((double)m);
+          next_k += m;
+        }
+      }
+      ch.apply(l);
+      ch.time += dt;
+    }
+    if (time_period <= 0.0 && p % sample_period == 0) {
+      pending_dt = ch.select(engine, &pending);
+      out.S.push_back(bond_sum(ch.config));
+      out.M.push_back(magnetisation(ch.config));
+      out.time.push_back(ch.time);
+      out.weight.push_back(pending_dt);
+    }
+  }
+  out.final_occ = ch.config.occupation;
+  out.arr = ch.arr;
+  out.final_time = ch.time;
+  out.pending_site = pending;
+  out.pending_dt = pending_dt;
+  return out;
+}
+
 }  // namespace refcpu

@@ -115,6 +115,10 @@ def _declare(L):
                                     P8, PU, P64, P64, P64, PD, P64, P64, PD, C.c_int64,
                                     C.c_char_p]
     L.orc_canonical_run.restype = C.c_int
+    L.orc_nfold_run.argtypes = [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                C.c_int64, C.c_int64, C.c_double, C.c_int64, P8, PU, P64, P64,
+                                PD, PD, P64, PD, P64]
+    L.orc_nfold_run.restype = C.c_int64
 
 
 KB = None
@@ -349,3 +353,30 @@ def canonical_run(rows, cols, J, T, n_passes, occ, engine, loc=None, use_nlist=T
         raise CanonicalError(err.value.decode())
     return dict(e_run=e[:n_passes], S=S[:n_passes], occ=o, engine=eng, loc=lo, mol_loc=ml,
                 n_accept=int(cnt[0]), n_reject=int(cnt[1]), initial=init.value)
+
+
+def nfold_run(rows, cols, J, T, mu, n_passes, occ, engine, sample_period=1, time_period=0.0,
+              capacity=None):
+    """N-fold way (class-based rejection-free selector) on spins occ, sampled by
+    pass (time_period = 0) or on a time grid; returns a dict with S, M, time,
+    weight per sample, final occ, engine, the class array, final time and the
+    pending event (site, dt)."""
+    n = rows * cols
+    o = np.ascontiguousarray(occ, dtype=np.int8).copy()
+    eng = np.ascontiguousarray(engine, dtype=np.uint64).copy()
+    m = capacity if capacity is not None else n_passes // sample_period
+    S = np.zeros(max(m, 1), dtype=np.int64)
+    M = np.zeros(max(m, 1), dtype=np.int64)
+    t = np.zeros(max(m, 1), dtype=np.float64)
+    w = np.zeros(max(m, 1), dtype=np.float64)
+    arr = np.zeros(n, dtype=np.int64)
+    sc = np.zeros(2, dtype=np.float64)
+    pend = C.c_int64(0)
+    got = lib().orc_nfold_run(rows, cols, J, T, mu, n_passes, sample_period, time_period, m,
+                              _p(o, C.c_int8), _p(eng, C.c_uint64), _p(S, C.c_int64),
+                              _p(M, C.c_int64), _p(t, C.c_double), _p(w, C.c_double),
+                              _p(arr, C.c_int64), _p(sc, C.c_double), C.byref(pend))
+    m = min(m, got)
+    return dict(S=S[:m], M=M[:m], time=t[:m], weight=w[:m], occ=o, engine=eng, arr=arr,
+                n_total=got,
+                final_time=float(sc[0]), pending_dt=float(sc[1]), pending_site=pend.value)

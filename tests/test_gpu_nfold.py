@@ -1,0 +1,110 @@
+"""NFOLD mode on the GPU vs the oracle (refcpu::nfold_run).
+
+The event sequence is bit-identical: per sample the integer observables, at
+the end the occupation, the class-partitioned site array and the engine words.
+Times and by-pass weights agree to the rounding of log1p (device vs host libm):
+relative 1e-12.  Time-grid records (grid counts and times) are exact.
+"""
+import numpy as np
+import pytest
+
+import casmcode_monte_amd as cm
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+J = 0.1
+
+
+def _run_gpu(chains, occs, passes, chunks=None, sample_period=1, time_period=0.0, reserve=0):
+    r = cm.Run(chains, mode=cm.MODE_NFOLD, J1=J, sample_period=sample_period,
+               sample_time_period=time_period)
+    for i, o in enumerate(occs):
+        r.set_occupation(i, o)
+    if reserve:
+        r.reserve_samples(reserve)
+    for k in (chunks or [passes]):
+        r.run(k)
+    return r
+
+
+def _compare(r, c, o, exact_time=False):
+    rec = r.samples(c)
+    assert len(rec) == len(o["S"])
+    np.testing.assert_array_equal(rec["S1"], o["S"])
+    np.testing.assert_array_equal(rec["M"], o["M"])
+    if exact_time:
+        np.testing.assert_array_equal(rec["time"], o["time"])
+        np.testing.assert_array_equal(rec["weight"], o["weight"])
+    else:
+        np.testing.assert_allclose(rec["time"], o["time"], rtol=1e-12)
+        np.testing.assert_allclose(rec["weight"], o["weight"], rtol=1e-12)
+    np.testing.assert_array_equal(r.get_occupation(c), o["occ"])
+    np.testing.assert_array_equal(r.occ_location(c), o["arr"])
+    np.testing.assert_array_equal(r.rng_dump(c), o["engine"])
+
+
+@pytest.mark.parametrize("rows,cols,T,mu,seed,passes,sp", [
+    (8, 8, 2000.0, 0.0, 5, 40, 1),
+    (7, 9, 1500.0, 0.1, 6, 40, 3),
+    (16, 16, 900.0, -0.2, 7, 20, 1),
+    (2, 9, 2500.0, 0.05, 8, 60, 1),
+    (1, 12, 1800.0, 0.0, 9, 60, 2),
+    (64, 64, 2200.0, 0.0, 10, 6, 1),
+    (128, 128, 1200.0, 0.3, 11, 2, 1),
+])
+def test_nfold_by_pass_matches_oracle(rows, cols, T, mu, seed, passes, sp):
+    print("case:", rows, cols, T, mu, seed, passes, sp, flush=True)
+    n = rows * cols
+    occ = np.where(np.random.default_rng(seed).random(n) < 0.6, 1, -1).astype(np.int8)
+    o = O.nfold_run(rows, cols, J, T, mu, passes, occ, O.mt64_state(seed, 0), sample_period=sp)
+    r = _run_gpu([dict(shape=(rows, cols), temperature=T, exchange_potential=mu, seed=seed)],
+                 [occ], passes, chunks=[1, passes // 2, passes - 1 - passes // 2],
+                 sample_period=sp)
+    _compare(r, 0, o)
+    n_pass, n_ev, n_rej = r.counters(0)
+    assert (n_pass, n_ev, n_rej) == (passes, passes * n, 0)
+    r.close()
+
+
+def test_nfold_time_grid_matches_oracle():
+    rows, cols, T, mu, seed, passes = 12, 10, 1600.0, 0.05, 21, 30
+    occ = np.ones(rows * cols, dtype=np.int8)
+    period = O.nfold_run(rows, cols, J, T, mu, passes, occ, O.mt64_state(seed, 0),
+                         time_period=1e30, capacity=1)["final_time"] / 400
+    o = O.nfold_run(rows, cols, J, T, mu, passes, occ, O.mt64_state(seed, 0),
+                    time_period=period, capacity=10000)
+    r = _run_gpu([dict(shape=(rows, cols), temperature=T, exchange_potential=mu, seed=seed)],
+                 [occ], passes, chunks=[7, passes - 7], time_period=period, reserve=10000)
+    _compare(r, 0, o, exact_time=True)
+    r.close()
+
+
+def test_nfold_batch():
+    rng = np.random.default_rng(33)
+    shapes = [(8, 8), (9, 7), (20, 12), (5, 5), (32, 16), (6, 30)]
+    chains, occs = [], []
+    for i, (a, b) in enumerate(shapes):
+        chains.append(dict(shape=(a, b), temperature=700.0 + 500.0 * i,
+                           exchange_potential=-0.3 + 0.12 * i, seed=500 + i))
+        occs.append(np.where(rng.random(a * b) < 0.5, 1, -1).astype(np.int8))
+    r = _run_gpu(chains, occs, 12)
+    for i, ((a, b), occ) in enumerate(zip(shapes, occs)):
+        o = O.nfold_run(a, b, J, chains[i]["temperature"], chains[i]["exchange_potential"], 12,
+                        occ, O.mt64_state(500 + i, 0))
+        _compare(r, i, o)
+    r.close()
+
+
+def test_nfold_errors():
+    r = _run_gpu([dict(shape=(6, 6), temperature=1500.0, seed=1)],
+                 [np.ones(36, dtype=np.int8)], 2)
+    with pytest.raises(cm.McgError, match="weights"):
+        r.stats(0, cm.OBS_POTENTIAL_ENERGY)
+    r.close()
+    r = cm.Run([dict(shape=(6, 6), temperature=5000.0, seed=1)], mode=cm.MODE_NFOLD, J1=J,
+               sample_time_period=1e-6)
+    r.reserve_samples(10)
+    with pytest.raises(cm.McgError, match="exceeded"):
+        r.run(2)
+    r.close()

@@ -1,0 +1,113 @@
+"""Pins of the oracle's N-fold way chain (refcpu::nfold_run).
+
+The reference's nfold loop (nfold.hh:78-149) takes its event selector from
+outside the repository, so the selector is defined by this build (class-based
+rates, one partitioned site array); the pins are:
+  * the partition: the site array holds every site once, grouped by class in
+    class order, after runs;
+  * stream use: exactly three engine words per selected event;
+  * the time-weighted averages (weights = residence times) of E and N x equal
+    the SGC ensemble averages from exact enumeration of a 4x4 lattice, and
+    mean residence time 1/N at infinite temperature (all rates 1);
+  * the integer observables equal those recomputed from the final state.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+J = 0.1
+
+
+def _class_of(s, rows, cols, l):
+    i, j = l % rows, l // rows
+    nb = (s[(i + 1) % rows + rows * j] + s[(i - 1) % rows + rows * j] +
+          s[i + rows * ((j + 1) % cols)] + s[i + rows * ((j - 1) % cols)])
+    return (5 if s[l] > 0 else 0) + (nb + 4) // 2
+
+
+def _bond_sum(s, rows, cols):
+    g = s.reshape(cols, rows).T.astype(np.int64)
+    return int((g * np.roll(g, -1, 0)).sum() + (g * np.roll(g, -1, 1)).sum())
+
+
+@pytest.mark.parametrize("rows,cols,T,mu", [(8, 8, 2000.0, 0.0), (6, 10, 900.0, 0.2),
+                                           (5, 7, 3500.0, -0.3)])
+def test_partition_and_observables(rows, cols, T, mu):
+    n = rows * cols
+    occ = np.where(np.random.default_rng(rows).random(n) < 0.5, 1, -1).astype(np.int8)
+    r = oracle.nfold_run(rows, cols, J, T, mu, 30, occ, oracle.mt64_state(3, 0))
+    s = r["occ"]
+    assert sorted(r["arr"].tolist()) == list(range(n))
+    classes = [_class_of(s, rows, cols, l) for l in r["arr"]]
+    assert classes == sorted(classes)
+    assert r["S"][-1] == _bond_sum(s, rows, cols) and r["M"][-1] == int(s.sum())
+    assert np.all(np.diff(r["time"]) > 0) and np.all(r["weight"] > 0)
+
+
+def test_three_words_per_event():
+    rows = cols = 6
+    seed = 11
+    r = oracle.nfold_run(rows, cols, J, 2500.0, 0.0, 3, np.ones(36, dtype=np.int8),
+                         oracle.mt64_state(seed, 0))
+    events = 3 * 36 + 1  # three passes and the selected event the last sample weighs
+    assert np.array_equal(oracle.mt64_state(seed, 3 * events), r["engine"])
+
+
+def test_infinite_temperature_residence_time():
+    """All rates 1: R = N and dt = -log(1-u)/N, mean 1/N."""
+    n = 64
+    r = oracle.nfold_run(8, 8, J, 1e12, 0.0, 4000, np.ones(n, dtype=np.int8),
+                         oracle.mt64_state(7, 0))
+    w = r["weight"]
+    assert abs(w.mean() * n - 1.0) < 4 * w.std() * n / np.sqrt(len(w))
+    assert abs(r["final_time"] / (4000 * n) * n - 1.0) < 0.01
+
+
+def _check_vs_enumeration(r, rows, cols, T, mu, drop):
+    n = rows * cols
+    ex = oracle.lattice_enumerate(2, (rows, cols), J, 0.0, T, mu)
+    S, M, w = r["S"][drop:], r["M"][drop:], r["weight"][drop:]
+    Ef = -J * S
+    Nx = (n + M) / 2.0
+    E = Ef - mu * Nx  # SGC potential per supercell
+    nb = 100
+    m = len(w) // nb * nb
+    for x, exact in ((E, ex["E"]), (Nx, ex["Nx"])):
+        xb = (x[:m] * w[:m]).reshape(nb, -1).sum(1) / w[:m].reshape(nb, -1).sum(1)
+        mean = (x * w).sum() / w.sum()
+        err = xb.std(ddof=1) / np.sqrt(nb)
+        assert abs(mean - exact) < 4 * err + 1e-9, (mean, exact, err)
+
+
+@pytest.mark.parametrize("T,mu", [(2000.0, 0.0), (1500.0, 0.15)])
+def test_time_grid_sampling_vs_enumeration(T, mu):
+    """Samples on a time grid (one record per residence interval, weighted by
+    its number of grid points) give the ensemble averages (4x4)."""
+    args = (4, 4, J, T, mu, 60000, np.ones(16, dtype=np.int8), oracle.mt64_state(2024, 0))
+    period = oracle.nfold_run(*args, time_period=1e30, capacity=1)["final_time"] / 500000
+    r = oracle.nfold_run(*args, time_period=period, capacity=600_000)
+    w, t = r["weight"], r["time"]
+    assert r["n_total"] == len(r["S"]) and np.all(w >= 1.0) and np.all(w == np.round(w))
+    # records are runs of consecutive grid points: k_next = k + weight
+    k = np.round(t / period)
+    assert np.allclose(t, k * period) and np.array_equal(k[1:], k[:-1] + w[:-1])
+    _check_vs_enumeration(r, 4, 4, T, mu, 1000)
+
+
+def test_pass_sampling_odd_lattice_vs_enumeration():
+    """By-pass samples weighted by residence times give the ensemble averages
+    when N is odd (3x5): consecutive pass samples alternate flip parity."""
+    r = oracle.nfold_run(3, 5, J, 1800.0, 0.05, 60000, np.ones(15, dtype=np.int8),
+                         oracle.mt64_state(99, 0))
+    _check_vs_enumeration(r, 3, 5, 1800.0, 0.05, 500)
+
+
+def test_pass_sampling_even_lattice_sees_one_parity():
+    """With N even every pass sample has an even number of flips since the
+    start: the number of -1 sites keeps the initial parity (this is why the
+    time grid is the sampling to use for even N)."""
+    r = oracle.nfold_run(4, 4, J, 2000.0, 0.0, 2000, np.ones(16, dtype=np.int8),
+                         oracle.mt64_state(5, 0))
+    n_down = (16 - r["M"]) // 2
+    assert np.all(n_down % 2 == 0)
