@@ -126,6 +126,44 @@ def bench_canonical(passes):
     return out
 
 
+def bench_nfold(passes):
+    """N-fold way chains (NFOLD mode): the 64 conditions of the exact scan
+    (128x128, 8 T x 8 mu) in one launch vs one chain alone.  Besides events/s
+    the Metropolis-equivalent rate: a random-site Metropolis attempt accepts
+    with probability R/N, so the simulated time (sum of 1/R per event on
+    average) times N counts the attempts the events stand for."""
+    import casmcode_monte_amd as cm
+    conds = c2_conditions()
+    r = cm.Run(conds, mode=cm.MODE_NFOLD, J1=0.1)
+    r.run(2)
+    t0 = [r.samples(c)["time"][-1] for c in range(len(conds))]
+    w, k, a = timed_run(r, passes)
+    t1 = [r.samples(c)["time"][-1] for c in range(len(conds))]
+    n = 128 * 128
+    equiv = sum(n * (b - e) for b, e in zip(t1, t0))
+    out = dict(chains=len(conds), passes=passes, wall_s=w, kernel_s=k, events=a,
+               events_per_s=a / w, metropolis_equivalent_attempts_per_s=equiv / w)
+    r.close()
+    p = cm.Run(conds[:1], mode=cm.MODE_NFOLD, J1=0.1)
+    p.run(1)
+    pp = max(passes // 10, 5)
+    pw, pk, pa = timed_run(p, pp)
+    out["plain_events_per_s"] = pa / pw
+    out["batched_over_plain"] = out["events_per_s"] / out["plain_events_per_s"]
+    p.close()
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle as O
+    cp = 5
+    tc = time.perf_counter()
+    O.nfold_run(128, 128, 0.1, conds[0]["temperature"], conds[0]["exchange_potential"], cp,
+                np.ones(n, dtype=np.int8), O.mt64_state(1000, 0))
+    cw = time.perf_counter() - tc
+    out["cpu_port"] = dict(events_per_s=cp * n / cw, cores=1,
+                           sample=f"1 chain 128x128, {cp} passes, oracle restatement")
+    return out
+
+
 def bench_checker_scan(passes):
     """64-condition 128^2 scan: batched LDS-resident launch vs one chain at a time."""
     import casmcode_monte_amd as cm
@@ -312,6 +350,10 @@ def main():
         print(f"[bench] canonical batch done: {cn['attempts_per_s']:.3e} attempts/s",
               file=sys.stderr, flush=True)
         line["canonical_exact_64_chains"] = cn
+        nf = bench_nfold(max(K // 10, 20))
+        print(f"[bench] nfold batch done: {nf['events_per_s']:.3e} events/s", file=sys.stderr,
+              flush=True)
+        line["nfold_64_chains"] = nf
         line["checkerboard"] = {
             "scan_128x128_64_conditions": scan,
             "square_4096x4096_J1J2_16T": fam2,

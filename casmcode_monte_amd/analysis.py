@@ -1,5 +1,6 @@
-"""Fluctuation observables of a sampled SGC run: heat capacity and the
-parametric-composition susceptibility, with block error bars.
+"""Fluctuation observables of a sampled SGC run (heat capacity and the
+parametric-composition susceptibility) and the weighted mean of a weighted
+(N-fold way) series, with block error bars.
 
 The reference names these analyses ("heat_capacity", "param_susc",
 run_management/io/json/SamplingFixtureParams_json_io.hh:54-59) but implements
@@ -31,3 +32,19 @@ def heat_capacity(e_per_unitcell, n_unitcells, temperature, kb=KB, blocks=20):
 def susceptibility(x, n_unitcells, temperature, kb=KB, blocks=20):
     f = lambda s: n_unitcells * np.var(s) / (kb * temperature)
     return _blocked(x, f, blocks)
+
+
+def weighted_mean(x, w, blocks=20):
+    """Weighted mean sum(x w) / sum(w) of a weighted series (NFOLD samples: w are
+    residence times or time-grid counts) with a block error: the series is cut
+    into `blocks` consecutive blocks of equal length and
+    sigma = std(block weighted means) / sqrt(blocks)."""
+    x = np.asarray(x, dtype=np.float64)
+    w = np.asarray(w, dtype=np.float64)
+    mean = float((x * w).sum() / w.sum())
+    n = len(x) // blocks
+    if n == 0:
+        return mean, float("nan")
+    xb = (x[:n * blocks] * w[:n * blocks]).reshape(blocks, n).sum(1)
+    wb = w[:n * blocks].reshape(blocks, n).sum(1)
+    return mean, float((xb / wb).std(ddof=1) / np.sqrt(blocks))

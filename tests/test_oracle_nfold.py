@@ -111,3 +111,17 @@ def test_pass_sampling_even_lattice_sees_one_parity():
                          oracle.mt64_state(5, 0))
     n_down = (16 - r["M"]) // 2
     assert np.all(n_down % 2 == 0)
+
+
+def test_weighted_mean_helper():
+    """analysis.weighted_mean on a by-pass series equals the direct weighted sum
+    and agrees with enumeration (odd N, 3x5)."""
+    from casmcode_monte_amd import analysis
+    r = oracle.nfold_run(3, 5, J, 1800.0, 0.05, 20000, np.ones(15, dtype=np.int8),
+                         oracle.mt64_state(4, 0))
+    Nx = (15 + r["M"][200:]) / 2.0
+    w = r["weight"][200:]
+    mean, err = analysis.weighted_mean(Nx, w, blocks=50)
+    assert mean == pytest.approx((Nx * w).sum() / w.sum(), rel=1e-15)
+    ex = oracle.lattice_enumerate(2, (3, 5), J, 0.0, 1800.0, 0.05)
+    assert abs(mean - ex["Nx"]) < 4 * err
