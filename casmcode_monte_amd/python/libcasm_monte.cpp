@@ -9,7 +9,9 @@
 // StateSamplingFunction, CompletionCheck*, RequestedPrecision, converge),
 // python/src/monte_ising_cpp.cpp and
 // python/src/monte_ising_cpp_semigrand_canonical.cpp:503-576 (the Ising
-// system and the SGC calculator).  The calculator's run() drives the MI355X
+// system and the SGC calculator), python/src/monte_events.cpp (occupant
+// candidates, swaps, OccLocation and the event proposal functions, over the
+// one-asymmetric-unit Conversions of this build).  The calculator's run() drives the MI355X
 // library through the C++ drop-in (include/casm/monte/ising_cpp/
 // basic_semigrand_canonical.hh); nothing here runs a Metropolis step on the
 // host.
@@ -19,6 +21,7 @@
 #include <pybind11/stl.h>
 #include <pybind11/stl_bind.h>
 
+#include "casm/monte/events/OccEventProposal.hh"
 #include "casm/monte/ising_cpp/basic_semigrand_canonical.hh"
 
 namespace py = pybind11;
@@ -28,6 +31,7 @@ using namespace CASM::monte::ising_cpp;
 namespace sgc = CASM::monte::ising_cpp::basic_semigrand_canonical;
 
 typedef std::mt19937_64 engine_type;
+typedef RandomNumberGenerator<engine_type> generator_type;
 typedef sgc::SemiGrandCanonicalEventGenerator<engine_type> event_generator_type;
 typedef sgc::SemiGrandCanonicalCalculator<IsingSystem, event_generator_type> calculator_type;
 typedef std::map<SamplerComponent, RequestedPrecision> RequestedPrecisionMap;
@@ -181,6 +185,182 @@ PYBIND11_MODULE(_libcasm_monte, m) {
       .def(py::init<>())
       .def("seed", [](engine_type &e, uint64_t s) { e.seed(s); })
       .def("__call__", [](engine_type &e) { return e(); });
+
+  // python/src/monte.cpp RandomNumberGenerator: random_int / random_real over a
+  // shared engine
+  py::class_<generator_type>(mm, "RandomNumberGenerator")
+      .def(py::init([](std::shared_ptr<engine_type> e) { return generator_type(e); }),
+           py::arg("engine") = nullptr)
+      .def("random_int", [](generator_type &g, uint64_t m) { return g.random_int(m); },
+           py::arg("maximum_value"))
+      .def("random_real", [](generator_type &g, double m) { return g.random_real(m); },
+           py::arg("maximum_value"))
+      .def_property_readonly("engine", [](generator_type const &g) { return g.engine; });
+
+  // ---- libcasm.monte.events ------------------------------------------------------
+  py::module_ me = m.def_submodule("events");
+  py::class_<Conversions, std::shared_ptr<Conversions>>(me, "Conversions")
+      .def(py::init<Index, std::vector<std::string>>(), py::arg("n_sites"),
+           py::arg("occupants"))
+      .def("l_size", &Conversions::l_size)
+      .def("l_to_asym", &Conversions::l_to_asym, py::arg("l"))
+      .def("asym_size", &Conversions::asym_size)
+      .def("occ_size", &Conversions::occ_size, py::arg("asym"))
+      .def("species_index",
+           py::overload_cast<Index, Index>(&Conversions::species_index, py::const_),
+           py::arg("asym"), py::arg("occ_index"))
+      .def("occ_index", &Conversions::occ_index, py::arg("asym"), py::arg("species_index"))
+      .def("species_allowed", &Conversions::species_allowed, py::arg("asym"),
+           py::arg("species_index"))
+      .def("species_size", &Conversions::species_size);
+  py::class_<Mol>(me, "Mol")
+      .def(py::init<>())
+      .def_readwrite("id", &Mol::id)
+      .def_readwrite("component_id", &Mol::component)
+      .def_readwrite("linear_site_index", &Mol::l)
+      .def_readwrite("asymmetric_unit_index", &Mol::asym)
+      .def_readwrite("species_index", &Mol::species_index)
+      .def_readwrite("mol_location_index", &Mol::loc);
+  py::class_<OccTransform>(me, "OccTransform")
+      .def(py::init<>())
+      .def_readwrite("linear_site_index", &OccTransform::l)
+      .def_readwrite("mol_id", &OccTransform::mol_id)
+      .def_readwrite("asym", &OccTransform::asym)
+      .def_readwrite("from_species", &OccTransform::from_species)
+      .def_readwrite("to_species", &OccTransform::to_species);
+  py::class_<OccEvent>(me, "OccEvent")
+      .def(py::init<>())
+      .def_readwrite("linear_site_index", &OccEvent::linear_site_index)
+      .def_readwrite("new_occ", &OccEvent::new_occ)
+      .def_readwrite("occ_transform", &OccEvent::occ_transform)
+      .def("__copy__", [](OccEvent const &e) { return OccEvent(e); });
+  py::class_<OccCandidate>(me, "OccCandidate")
+      .def(py::init<Index, Index>(), py::arg("asymmetric_unit_index"), py::arg("species_index"))
+      .def_readwrite("asymmetric_unit_index", &OccCandidate::asym)
+      .def_readwrite("species_index", &OccCandidate::species_index)
+      .def("__lt__", [](OccCandidate const &a, OccCandidate const &b) { return a < b; })
+      .def("__eq__", [](OccCandidate const &a, OccCandidate const &b) { return a == b; })
+      .def("__ne__", [](OccCandidate const &a, OccCandidate const &b) { return a != b; })
+      .def("__repr__", [](OccCandidate const &c) {
+        return "OccCandidate(" + std::to_string(c.asym) + ", " +
+               std::to_string(c.species_index) + ")";
+      });
+  py::class_<OccSwap>(me, "OccSwap")
+      .def(py::init<OccCandidate const &, OccCandidate const &>(), py::arg("first"),
+           py::arg("second"))
+      .def_readwrite("first", &OccSwap::cand_a)
+      .def_readwrite("second", &OccSwap::cand_b)
+      .def("reverse", &OccSwap::reverse)
+      .def("sort", [](OccSwap &s) { s.sort(); })
+      .def("sorted", &OccSwap::sorted)
+      .def("__lt__", [](OccSwap const &a, OccSwap const &b) { return a < b; })
+      .def("__eq__", [](OccSwap const &a, OccSwap const &b) { return a == b; })
+      .def("__ne__", [](OccSwap const &a, OccSwap const &b) { return a != b; });
+  py::class_<OccCandidateList>(me, "OccCandidateList")
+      .def(py::init<Conversions const &>(), py::arg("convert"))
+      .def(py::init<std::vector<OccCandidate>, Conversions const &>(), py::arg("candidates"),
+           py::arg("convert"))
+      .def("index", py::overload_cast<OccCandidate const &>(&OccCandidateList::index,
+                                                            py::const_),
+           py::arg("candidate"))
+      .def("__getitem__", [](OccCandidateList const &l, Index i) { return l[i]; })
+      .def("__len__", [](OccCandidateList const &l) { return l.size(); })
+      .def(
+          "__iter__",
+          [](OccCandidateList const &l) { return py::make_iterator(l.begin(), l.end()); },
+          py::keep_alive<0, 1>());
+  me.def("is_allowed_canonical_swap", &allowed_canonical_swap, py::arg("convert"),
+         py::arg("cand_a"), py::arg("cand_b"));
+  me.def("make_canonical_swaps", &make_canonical_swaps, py::arg("convert"),
+         py::arg("occ_candidate_list"));
+  me.def("is_allowed_semigrand_canonical_swap", &allowed_semigrand_canonical_swap,
+         py::arg("convert"), py::arg("cand_a"), py::arg("cand_b"));
+  me.def("make_semigrand_canonical_swaps", &make_semigrand_canonical_swaps, py::arg("convert"),
+         py::arg("occ_candidate_list"));
+  // OccLocation keeps references to its Conversions and OccCandidateList
+  py::class_<OccLocation, std::shared_ptr<OccLocation>>(me, "OccLocation")
+      .def(py::init<Conversions const &, OccCandidateList const &, bool, bool, bool>(),
+           py::arg("convert"), py::arg("occ_candidate_list"), py::arg("update_atoms") = false,
+           py::arg("track_unique_atoms") = false, py::arg("save_atom_info") = false,
+           py::keep_alive<1, 2>(), py::keep_alive<1, 3>())
+      .def("initialize",
+           [](OccLocation &ol, std::vector<int> const &occ, std::optional<double> t) {
+             ol.initialize(occ, t);
+           },
+           py::arg("occupation"), py::arg("time") = std::nullopt)
+      .def("apply",
+           [](OccLocation &ol, OccEvent const &e,
+              py::array_t<int, py::array::c_style> occupation, std::optional<double> t) {
+             // the occupation array is updated in place, as the reference's
+             // Eigen::Ref argument is
+             auto buf = occupation.mutable_unchecked<1>();
+             std::vector<int> v(buf.shape(0));
+             for (py::ssize_t i = 0; i < buf.shape(0); ++i) v[i] = buf(i);
+             ol.apply(e, v, t);
+             for (py::ssize_t i = 0; i < buf.shape(0); ++i) buf(i) = v[i];
+           },
+           py::arg("event"), py::arg("occupation"), py::arg("time") = std::nullopt)
+      .def("choose_mol_by_candidate_index",
+           [](OccLocation const &ol, Index c, generator_type &g) { return ol.choose_mol(c, g); },
+           py::arg("candidate_index"), py::arg("random_number_generator"))
+      .def("choose_mol",
+           [](OccLocation const &ol, OccCandidate const &c, generator_type &g) {
+             return ol.choose_mol(c, g);
+           },
+           py::arg("candidate"), py::arg("random_number_generator"))
+      .def("mol_size", &OccLocation::mol_size)
+      .def("mol", [](OccLocation const &ol, Index id) { return ol.mol(id); }, py::arg("mol_id"))
+      .def("candidate_list", &OccLocation::candidate_list, py::return_value_policy::reference_internal)
+      .def("cand_size_by_candidate_index",
+           [](OccLocation const &ol, Index c) { return ol.cand_size(c); },
+           py::arg("candidate_index"))
+      .def("cand_size",
+           [](OccLocation const &ol, OccCandidate const &c) { return ol.cand_size(c); },
+           py::arg("candidate"))
+      .def("mol_id_by_candidate_index",
+           [](OccLocation const &ol, Index c, Index loc) { return ol.mol_id(c, loc); },
+           py::arg("candidate_index"), py::arg("location_index"))
+      .def("mol_id",
+           [](OccLocation const &ol, OccCandidate const &c, Index loc) {
+             return ol.mol_id(c, loc);
+           },
+           py::arg("candidate"), py::arg("location_index"))
+      .def("linear_site_index_to_mol_id", &OccLocation::l_to_mol_id,
+           py::arg("linear_site_index"))
+      .def("convert", &OccLocation::convert, py::return_value_policy::reference_internal);
+  me.def("choose_canonical_swap",
+         [](OccLocation const &ol, std::vector<OccSwap> const &swaps, generator_type &g) {
+           return choose_canonical_swap(ol, swaps, g);
+         },
+         py::arg("occ_location"), py::arg("canonical_swaps"), py::arg("random_number_generator"));
+  me.def("propose_canonical_event_from_swap",
+         [](OccEvent &e, OccLocation const &ol, OccSwap const &swap, generator_type &g) {
+           propose_canonical_event_from_swap(e, ol, swap, g);
+         },
+         py::arg("event"), py::arg("occ_location"), py::arg("swap"),
+         py::arg("random_number_generator"));
+  me.def("propose_canonical_event",
+         [](OccEvent &e, OccLocation const &ol, std::vector<OccSwap> const &swaps,
+            generator_type &g) { propose_canonical_event(e, ol, swaps, g); },
+         py::arg("event"), py::arg("occ_location"), py::arg("canonical_swaps"),
+         py::arg("random_number_generator"));
+  me.def("choose_semigrand_canonical_swap",
+         [](OccLocation const &ol, std::vector<OccSwap> const &swaps, generator_type &g) {
+           return choose_semigrand_canonical_swap(ol, swaps, g);
+         },
+         py::arg("occ_location"), py::arg("semigrand_canonical_swaps"),
+         py::arg("random_number_generator"));
+  me.def("propose_semigrand_canonical_event_from_swap",
+         [](OccEvent &e, OccLocation const &ol, OccSwap const &swap, generator_type &g) {
+           propose_semigrand_canonical_event_from_swap(e, ol, swap, g);
+         },
+         py::arg("event"), py::arg("occ_location"), py::arg("swap"),
+         py::arg("random_number_generator"));
+  me.def("propose_semigrand_canonical_event",
+         [](OccEvent &e, OccLocation const &ol, std::vector<OccSwap> const &swaps,
+            generator_type &g) { propose_semigrand_canonical_event(e, ol, swaps, g); },
+         py::arg("event"), py::arg("occ_location"), py::arg("semigrand_canonical_swaps"),
+         py::arg("random_number_generator"));
 
   // ---- libcasm.monte.sampling ----------------------------------------------------
   py::module_ ms = m.def_submodule("sampling");
