@@ -1558,6 +1558,12 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
     int64_t over = 0;
     for (int c = 0; c < h->n_chains; ++c) {
       if (!h->active[c]) continue;
+      if (xs[c].frozen)
+        return fail(MCG_ERR_STATE,
+                    "NFOLD chain " + std::to_string(c) +
+                        ": the total rate is 0 (every flip has probability 0 at this "
+                        "temperature), so the next event never comes and the time grid "
+                        "would never end; the chain's state is undefined");
       over = std::max(over, xs[c].n_samples - h->series_cap);
       h->n_samples[c] = std::min(xs[c].n_samples, h->series_cap);
     }

@@ -47,6 +47,7 @@ struct ExactState {
   int64_t pending_site;  // event selected at the end of a sampled pass, or -1
   double pending_dt;     // its time increment
   int64_t next_k;        // next time-grid sample index (sample at next_k * period)
+  int64_t frozen;        // set when the total rate reached 0 on the time grid
 };
 
 void launch_exact(const ExactParams *P, ExactState *S, int8_t *lat_pool, mcg_sample *series,

@@ -53,6 +53,24 @@ __device__ __forceinline__ int class_of(const uint8_t *lat, uint32_t l, int32_t 
   return (lat[l] ? 5 : 0) + ((nsum + 4) >> 1);
 }
 
+// A cross-lane read only sees lanes that are active when it executes.  The
+// empty asm consumes the result where the call stands (all 64 lanes active),
+// so the compiler cannot sink the read into a branch that masks off its
+// source lanes.
+template <typename T>
+__device__ __forceinline__ T pinned(T v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+// lane k receives lane k+1's value (row_shl:1) / lane k-1's (row_shr:1)
+// within its row of 16 lanes; the walk uses lanes 0..10
+__device__ __forceinline__ int row_shl1(int v) {
+  return pinned(__builtin_amdgcn_update_dpp(0, v, 0x101, 0xf, 0xf, false));
+}
+__device__ __forceinline__ int row_shr1(int v) {
+  return pinned(__builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
+}
+
 }  // namespace
 
 __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict__ Pp,
@@ -112,6 +130,9 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
   double time = St->time;
   int64_t pending = St->pending_site;
   double pending_dt = St->pending_dt;
+  const int self_nb = 2 * (rows == 1) + 2 * (cols == 1);
+  const bool narrow = rows <= 2 || cols <= 2;  // a neighbour can be listed twice
+  bool frozen = false;  // total rate 0: the next event never comes
 
   auto ensure = [&](int64_t upto) {
     while (upto > (int64_t)MT_N * (G + 1)) {
@@ -119,78 +140,98 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
       ++G;
     }
   };
-  auto word = [&]() -> uint64_t {
-    ensure(pos_mt + 1);
+  // stream position q, which ensure() has made available
+  auto ring_at = [&](int64_t q) -> uint64_t {
     const int64_t base = (int64_t)MT_N * (G - 1);
-    int r = (int)(pos_mt - base) + (int)(((G - 1) & 1) * MT_N);
+    int r = (int)(q - base) + (int)(((G - 1) & 1) * MT_N);
     if (r >= RING) r -= RING;
-    ++pos_mt;
     return rfl64(ring[r]);
   };
-  // select an event: site and time increment (three words and Lemire redraws)
-  auto select = [&](uint32_t &site) -> double {
-    double R = 0.0;
-    for (int k = 0; k < 10; ++k)
-      R = __dadd_rn(R, __dmul_rn((double)(start_of(k + 1) - start_of(k)), readlane_f64(ratev, k)));
-    const double target = __dmul_rn(canonical_from_word(word()), R);
-    double run = 0.0;
-    int kk = -1, last = -1;
-    for (int q = 0; q < 10; ++q) {
-      const int nq = start_of(q + 1) - start_of(q);
-      if (nq == 0) continue;
-      last = q;
-      run = __dadd_rn(run, __dmul_rn((double)nq, readlane_f64(ratev, q)));
-      if (target < run) {
-        kk = q;
-        break;
-      }
+  auto word = [&]() -> uint64_t {
+    ensure(pos_mt + 1);
+    const uint64_t w = ring_at(pos_mt);
+    ++pos_mt;
+    return w;
+  };
+  // select an event: the site l, its position, the time-increment word u3 and
+  // the total rate R.  Lane k (k < 10) holds n_k and its running rate sum; the
+  // sum is formed in class order, so running sums equal the sequential ones
+  // (an empty class adds +0.0).
+  auto select = [&](uint32_t &site, int &site_pos, double &u3, double &R) {
+    const int nxt = row_shl1(startv);
+    const int nv = lane < 10 ? nxt - startv : 0;
+    const double tv = __dmul_rn((double)nv, ratev);
+    double run = 0.0, pre = 0.0;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+      run = __dadd_rn(run, readlane_f64(tv, k));
+      pre = lane == k ? run : pre;
     }
-    if (kk < 0) kk = last;
-    const uint64_t nk = (uint64_t)(start_of(kk + 1) - start_of(kk));
-    uint64_t x = word();
-    uint64_t lo = x * nk, hi = __umul64hi(x, nk);
+    R = run;
+    ensure(pos_mt + 3);  // the usual three words, read together
+    const uint64_t w0 = ring_at(pos_mt), w1 = ring_at(pos_mt + 1), w2 = ring_at(pos_mt + 2);
+    pos_mt += 2;
+    const double target = __dmul_rn(canonical_from_word(w0), R);
+    const uint64_t hit = __ballot(lane < 10 && nv > 0 && target < pre);
+    int kk;
+    if (hit) {
+      kk = __ffsll((unsigned long long)hit) - 1;
+    } else {  // rounding left none: the last non-empty class
+      kk = 63 - __clzll((unsigned long long)__ballot(lane < 10 && nv > 0));
+    }
+    const uint64_t nk = (uint64_t)__builtin_amdgcn_readlane(nv, kk);
+    uint64_t x = w1, lo = x * nk, hi = __umul64hi(x, nk);
+    bool redrawn = false;
     if (lo < nk) {
       const uint64_t th = (0ULL - nk) % nk;
       while (lo < th) {
         x = word();
         lo = x * nk;
         hi = __umul64hi(x, nk);
+        redrawn = true;
       }
     }
-    site = rfl32((int32_t)arr[start_of(kk) + (int)hi]);
-    const double u3 = canonical_from_word(word());
-    return -log1p(-u3) / R;
+    uint64_t w3 = w2;
+    if (redrawn)
+      w3 = word();
+    else
+      ++pos_mt;
+    u3 = canonical_from_word(w3);
+    site_pos = __builtin_amdgcn_readlane(startv, kk) + (int)hi;
+    site = rfl32((int32_t)arr[site_pos]);
   };
-  // move site x (now at position p) from class a to class b
-  auto move = [&](uint32_t x, int b) {
-    int p = rfl32((int32_t)pos[x]);
-    int a = 0;
-    for (int k = 1; k <= 10; ++k) a += start_of(k) <= p ? 1 : 0;
-    while (a < b) {  // become the first entry of segment a+1
-      const int q = start_of(a + 1) - 1;
-      const uint32_t y = rfl32((int32_t)arr[q]);
-      if (lane == 0) {
-        arr[q] = (uint16_t)x;
-        arr[p] = (uint16_t)y;
-        pos[y] = (uint16_t)p;
-      }
-      p = q;
-      if (lane == a + 1) --startv;
-      ++a;
+  // Move site x from class a to class b at position p.  Going up, x swaps in
+  // turn with the last entry of segments a, a+1, .., b-1 (each of which then
+  // starts one lower); going down, with the first entry of segments a, a-1,
+  // .., b+1 (each then starts one higher).  The chain of swaps is a rotation:
+  // lane k holds the boundary between segments k-1 and k, reads the entry
+  // there and writes it into the previous hole, all lanes at once; a boundary
+  // that coincides with the previous one (empty segment) is a swap of x with
+  // itself and writes nothing.
+  auto move = [&](uint32_t x, int a, int b, int p_known) {
+    const bool up = a < b;
+    // lane k: the last entry of segment k-1 (up) or the first of segment k (down)
+    const int q = up ? startv - 1 : startv;
+    const int from_below = row_shr1(q), from_above = row_shl1(q);
+    const bool act = up ? (lane > a && lane <= b) : (lane > b && lane <= a);
+    uint32_t y = arr[act ? q : 0], px = pos[x];
+    asm volatile("" : "+v"(y), "+v"(px));  // both reads issued before either is used
+    const int p = p_known >= 0 ? p_known : rfl32((int32_t)px);
+    int qp = lane == (up ? a + 1 : a) ? p : up ? from_below : from_above;
+    const int qx = __builtin_amdgcn_readlane(q, up ? b : b + 1);
+    // lane 11 (outside the segment lanes) carries x to its final slot in the
+    // same writes
+    bool wr = act && q != qp;
+    if (lane == 11) {
+      y = x;
+      qp = qx;
+      wr = true;
     }
-    while (a > b) {  // become the last entry of segment a-1
-      const int q = start_of(a);
-      const uint32_t y = rfl32((int32_t)arr[q]);
-      if (lane == 0) {
-        arr[q] = (uint16_t)x;
-        arr[p] = (uint16_t)y;
-        pos[y] = (uint16_t)p;
-      }
-      p = q;
-      if (lane == a) ++startv;
-      --a;
+    if (wr) {
+      arr[qp] = (uint16_t)y;
+      pos[y] = (uint16_t)qp;
     }
-    if (lane == 0) pos[x] = (uint16_t)p;
+    startv += act ? (up ? -1 : 1) : 0;
     // the wave's LDS operations execute in issue order; keep the compiler from
     // moving the next move's reads above these writes
     asm volatile("" ::: "memory");
@@ -208,54 +249,137 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
     ++n_samples;
   };
 
+  // Deferred time increments: lane j holds event j of the current batch (its
+  // u3 and R, or its known increment with R = -1, and the state before it).
+  // A flush turns them into increments on all lanes at once, adds them to the
+  // time in event order and, on the time grid, writes the batch's records.
+  double bu = 0.0, bR = 1.0;
+  int64_t bS = 0, bM = 0;
+  int nb_ev = 0;
+  auto flush = [&]() {
+    if (nb_ev == 0) return;
+    const double dtv = bR < 0.0 ? bu : -log1p(-bu) / bR;
+    double tend = 0.0;
+    for (int j = 0; j < nb_ev; ++j) {
+      time = __dadd_rn(time, readlane_f64(dtv, j));
+      tend = lane == j ? time : tend;
+    }
+    if (by_time) {
+      // first grid index k >= next_k with k * period >= t_end (the predicate
+      // decides at the edge); t_end is non-decreasing, so each lane can
+      // search from the batch's starting index
+      int64_t k = next_k;
+      const bool never = lane < nb_ev && !(tend <= 1.79769313486231570815e308);
+      if (__ballot(never)) frozen = true;
+      if (lane < nb_ev && !never) {
+        k = (int64_t)ceil(tend / period);
+        if (k < next_k) k = next_k;
+        while (k > next_k && (double)(k - 1) * period >= tend) --k;
+        while ((double)k * period < tend) ++k;
+      }
+      const int64_t ku = pinned(__shfl_up(k, 1));
+      const int64_t kp = lane == 0 ? next_k : ku;
+      const int64_t m = lane < nb_ev ? k - kp : 0;
+      const uint64_t rec = __ballot(m > 0);
+      if (m > 0) {
+        const int64_t ns =
+            n_samples + __popcll((unsigned long long)rec & ((1ULL << lane) - 1ULL));
+        if (ns < series_cap) {
+          mcg_sample r = {};
+          r.pass = n_pass;
+          r.S1 = bS;
+          r.M = bM;
+          r.time = (double)kp * period;
+          r.weight = (double)m;
+          series[(int64_t)c * series_cap + ns] = r;
+        }
+      }
+      n_samples += __popcll((unsigned long long)rec);
+      const int32_t klo = __builtin_amdgcn_readlane((int32_t)k, nb_ev - 1);
+      const int32_t khi = __builtin_amdgcn_readlane((int32_t)(k >> 32), nb_ev - 1);
+      next_k = ((int64_t)khi << 32) | (uint32_t)klo;
+    }
+    nb_ev = 0;
+  };
+
+  // flip site l (at position l_pos when known, else -1), update S and M, and
+  // re-class it and its four neighbours
+  auto apply = [&](uint32_t l, int l_pos) {
+    // lanes 0..4 take l, +i, -i, +j, -j: their spin, neighbour sum, and how
+    // many of their neighbour slots hold l
+    const uint32_t lj = (uint32_t)(((uint64_t)l * magic) >> 40);
+    const uint32_t li = l - lj * (uint32_t)rows;
+    uint32_t xi = li, xj = lj;
+    if (lane == 1) xi = li + 1 == (uint32_t)rows ? 0u : li + 1;
+    if (lane == 2) xi = li == 0 ? (uint32_t)rows - 1 : li - 1;
+    if (lane == 3) xj = lj + 1 == (uint32_t)cols ? 0u : lj + 1;
+    if (lane == 4) xj = lj == 0 ? (uint32_t)cols - 1 : lj - 1;
+    const uint32_t xs = xi + (uint32_t)rows * xj;
+    const uint32_t xu = xi + 1 == (uint32_t)rows ? 0u : xi + 1;
+    const uint32_t xd = xi == 0 ? (uint32_t)rows - 1 : xi - 1;
+    const uint32_t yu = xj + 1 == (uint32_t)cols ? 0u : xj + 1;
+    const uint32_t yd = xj == 0 ? (uint32_t)cols - 1 : xj - 1;
+    const uint32_t xn[4] = {xu + (uint32_t)rows * xj, xd + (uint32_t)rows * xj,
+                            xi + (uint32_t)rows * yu, xi + (uint32_t)rows * yd};
+    const int sx = spin_at(lat, xs);
+    const int nsx = spin_at(lat, xn[0]) + spin_at(lat, xn[1]) + spin_at(lat, xn[2]) +
+                    spin_at(lat, xn[3]);
+    const int mx = (xn[0] == l) + (xn[1] == l) + (xn[2] == l) + (xn[3] == l);
+    const int s_old = __builtin_amdgcn_readlane(sx, 0);
+    const int nsum = __builtin_amdgcn_readlane(nsx, 0);
+    const int c_pre = (sx > 0 ? 5 : 0) + ((nsx + 4) >> 1);
+    const int sp = xs == l ? -sx : sx;
+    const int nsp = nsx - 2 * s_old * mx;
+    const int c_post = (sp > 0 ? 5 : 0) + ((nsp + 4) >> 1);
+    S += -2LL * s_old * (nsum - self_nb * s_old);
+    M += -2LL * s_old;
+    if (lane == 0) lat[l] ^= 1u;
+    asm volatile("" ::: "memory");
+    for (int t = 0; t < 5; ++t) {
+      const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)xs, t);
+      const int a = __builtin_amdgcn_readlane(c_pre, t), b = __builtin_amdgcn_readlane(c_post, t);
+      bool dup = false;  // listed twice (a lattice 1 or 2 sites wide): moved once
+      if (narrow)
+        for (int u = 0; u < t; ++u) dup |= (uint32_t)__builtin_amdgcn_readlane((int)xs, u) == x;
+      if (dup || a == b) continue;
+      move(x, a, b, t == 0 ? l_pos : -1);
+    }
+  };
+
   for (int64_t pp = 0; pp < n_passes; ++pp) {
     for (int32_t st = 0; st < n; ++st) {
       uint32_t l;
-      double dt;
+      int l_pos = -1;
+      double u3, R;
       if (pending >= 0) {
         l = (uint32_t)pending;
-        dt = pending_dt;
+        u3 = pending_dt;
+        R = -1.0;
         pending = -1;
       } else {
-        dt = select(l);
+        select(l, l_pos, u3, R);
       }
-      if (by_time) {  // grid points inside [time, time + dt) see the current state
-        // first grid index k with k * period >= t_end (the predicate decides at the edge)
-        const double t_end = time + dt;
-        int64_t k = (int64_t)ceil(t_end / period);
-        if (k < next_k) k = next_k;
-        while (k > next_k && (double)(k - 1) * period >= t_end) --k;
-        while ((double)k * period < t_end) ++k;
-        const int64_t m = k - next_k;
-        if (m > 0) {
-          record((double)next_k * period, (double)m);
-          next_k += m;
-        }
+      if (lane == nb_ev) {
+        bu = u3;
+        bR = R;
+        bS = S;
+        bM = M;
       }
-      // flip, observables, then the five classes
-      uint32_t nb[4];
-      nbrs(l, rows, cols, magic, nb);
-      const int s_old = spin_at(lat, l);
-      const int nsum = spin_at(lat, nb[0]) + spin_at(lat, nb[1]) + spin_at(lat, nb[2]) +
-                       spin_at(lat, nb[3]);
-      const int self_nb = 2 * (rows == 1) + 2 * (cols == 1);
-      S += -2LL * s_old * (nsum - self_nb * s_old);
-      M += -2LL * s_old;
-      __syncthreads();
-      if (lane == 0) lat[l] ^= 1u;
-      __syncthreads();
-      const uint32_t xs = lane == 1 ? nb[0] : lane == 2 ? nb[1] : lane == 3 ? nb[2]
-                          : lane == 4 ? nb[3] : l;
-      const int xc = lane < 5 ? class_of(lat, xs, rows, cols, magic) : 0;
-      for (int t = 0; t < 5; ++t)
-        move((uint32_t)__builtin_amdgcn_readlane((int)xs, t), __builtin_amdgcn_readlane(xc, t));
-      time += dt;
+      ++nb_ev;
+      apply(l, l_pos);
       ++n_ev;
+      if (nb_ev == 64) flush();
+      if (frozen) break;
     }
+    flush();
+    if (frozen) break;
     ++n_pass;
     if (!by_time && n_pass % sample_period == 0) {
       uint32_t l;
-      pending_dt = select(l);
+      int l_pos;
+      double u3, R;
+      select(l, l_pos, u3, R);
+      pending_dt = -log1p(-u3) / R;
       pending = l;
       record(time, pending_dt);
     }
@@ -283,6 +407,7 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
     St->pending_site = pending;
     St->pending_dt = pending_dt;
     St->next_k = next_k;
+    St->frozen = frozen ? 1 : 0;
   }
 }
 

@@ -49,7 +49,10 @@ def _compare(r, c, o, exact_time=False):
     (7, 9, 1500.0, 0.1, 6, 40, 3),
     (16, 16, 900.0, -0.2, 7, 20, 1),
     (2, 9, 2500.0, 0.05, 8, 60, 1),
+    (9, 2, 2100.0, -0.05, 12, 60, 1),
+    (2, 2, 3000.0, 0.0, 13, 200, 1),
     (1, 12, 1800.0, 0.0, 9, 60, 2),
+    (12, 1, 1400.0, 0.1, 14, 60, 1),
     (64, 64, 2200.0, 0.0, 10, 6, 1),
     (128, 128, 1200.0, 0.3, 11, 2, 1),
 ])
@@ -77,6 +80,41 @@ def test_nfold_time_grid_matches_oracle():
     r = _run_gpu([dict(shape=(rows, cols), temperature=T, exchange_potential=mu, seed=seed)],
                  [occ], passes, chunks=[7, passes - 7], time_period=period, reserve=10000)
     _compare(r, 0, o, exact_time=True)
+    r.close()
+
+
+def test_nfold_time_grid_batch():
+    """Several chains on the time grid, a grid fine enough that most residence
+    intervals hold several grid points and coarse enough that many hold none."""
+    shapes = [(10, 10), (7, 3), (16, 8)]
+    chains, occs, outs = [], [], []
+    rng = np.random.default_rng(44)
+    for i, (a, b) in enumerate(shapes):
+        T, mu, seed = 900.0 + 700.0 * i, 0.1 - 0.1 * i, 800 + i
+        occ = np.where(rng.random(a * b) < 0.5, 1, -1).astype(np.int8)
+        chains.append(dict(shape=(a, b), temperature=T, exchange_potential=mu, seed=seed))
+        occs.append(occ)
+    period = 0.02
+    for i, (a, b) in enumerate(shapes):
+        outs.append(O.nfold_run(a, b, J, chains[i]["temperature"],
+                                chains[i]["exchange_potential"], 9, occs[i],
+                                O.mt64_state(chains[i]["seed"], 0), time_period=period,
+                                capacity=20000))
+    r = _run_gpu(chains, occs, 9, chunks=[4, 5], time_period=period, reserve=20000)
+    for i in range(len(shapes)):
+        assert len(outs[i]["S"]) > 10
+        _compare(r, i, outs[i], exact_time=True)
+    r.close()
+
+
+def test_nfold_frozen_time_grid():
+    """All +1 at 1 K with mu = 0.4 eV: every flip raises the potential by 1.2 eV,
+    its Metropolis probability underflows to 0, the total rate is 0 and the
+    next event never comes; the time grid stops with an error."""
+    r = cm.Run([dict(shape=(8, 8), temperature=1.0, exchange_potential=0.4, seed=1)],
+               mode=cm.MODE_NFOLD, J1=J, sample_time_period=1e-3)
+    with pytest.raises(cm.McgError, match="total rate is 0"):
+        r.run(1)
     r.close()
 
 
