@@ -1164,9 +1164,10 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
       delete h;
       return fail(MCG_ERR_ARG, "EXACT mode supports at most 2^24 sites and rows < 65536");
     }
-    if ((h->canonical() || h->nfold()) && n > (1LL << 16)) {
+    if ((h->canonical() && n > (1LL << 16)) || (h->nfold() && n > (1LL << 15))) {
       delete h;
-      return fail(MCG_ERR_ARG, "CANONICAL and NFOLD modes support at most 65536 sites per chain");
+      return fail(MCG_ERR_ARG, "CANONICAL mode supports at most 65536 sites per chain, "
+                               "NFOLD at most 32768");
     }
     if (n > (1LL << 31)) {
       delete h;
@@ -1324,8 +1325,9 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
     return fail(MCG_ERR_CAPACITY, "shape has more sites than the chain slot holds");
   if (h->mode == MCG_MODE_EXACT && nd.shape[0] >= (1LL << 16))
     return fail(MCG_ERR_ARG, "EXACT mode supports rows < 65536");
-  if ((h->canonical() || h->nfold()) && n > (1LL << 16))
-    return fail(MCG_ERR_ARG, "CANONICAL and NFOLD modes support at most 65536 sites per chain");
+  if ((h->canonical() && n > (1LL << 16)) || (h->nfold() && n > (1LL << 15)))
+    return fail(MCG_ERR_ARG, "CANONICAL mode supports at most 65536 sites per chain, "
+                             "NFOLD at most 32768");
 
   HIP_TRY(hipStreamSynchronize(h->stream));
   const mcg_chain_desc old = h->chains[chain];

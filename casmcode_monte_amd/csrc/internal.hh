@@ -106,7 +106,8 @@ struct NfoldParams {
   int64_t lat_off;      // lattice byte offset; the class array starts at entry lat_off
 };
 
-inline int nfold_lds_bytes(const NfoldParams &P) { return ((P.n + 15) & ~15) + 4 * P.n + 64; }
+// class array and positions (16-bit each; a position's bit 15 is the spin)
+inline int nfold_lds_bytes(const NfoldParams &P) { return 4 * P.n + 64; }
 
 void launch_nfold(const NfoldParams *P, ExactState *S, int8_t *lat_pool, int32_t *list_pool,
                   mcg_sample *series, int64_t series_cap, const uint8_t *active, int n_chains,

@@ -28,7 +28,12 @@ namespace mcg {
 
 namespace {
 
-__device__ __forceinline__ int spin_at(const uint8_t *lat, uint32_t l) { return lat[l] ? 1 : -1; }
+// pos[l] holds l's position in the class array (bits 0..14) and its spin
+// (bit 15: 1 = +1)
+constexpr uint16_t SPIN_BIT = 0x8000u;
+__device__ __forceinline__ int spin_at(const uint16_t *pos, uint32_t l) {
+  return (pos[l] & SPIN_BIT) ? 1 : -1;
+}
 
 __device__ __forceinline__ void nbrs(uint32_t l, int32_t rows, int32_t cols, uint64_t magic,
                                      uint32_t out[4]) {
@@ -44,13 +49,13 @@ __device__ __forceinline__ void nbrs(uint32_t l, int32_t rows, int32_t cols, uin
   out[3] = i + (uint32_t)rows * jd;
 }
 
-__device__ __forceinline__ int class_of(const uint8_t *lat, uint32_t l, int32_t rows,
+__device__ __forceinline__ int class_of(const uint16_t *pos, uint32_t l, int32_t rows,
                                         int32_t cols, uint64_t magic) {
   uint32_t nb[4];
   nbrs(l, rows, cols, magic, nb);
-  const int nsum = spin_at(lat, nb[0]) + spin_at(lat, nb[1]) + spin_at(lat, nb[2]) +
-                   spin_at(lat, nb[3]);
-  return (lat[l] ? 5 : 0) + ((nsum + 4) >> 1);
+  const int nsum = spin_at(pos, nb[0]) + spin_at(pos, nb[1]) + spin_at(pos, nb[2]) +
+                   spin_at(pos, nb[3]);
+  return (spin_at(pos, l) > 0 ? 5 : 0) + ((nsum + 4) >> 1);
 }
 
 // A cross-lane read only sees lanes that are active when it executes.  The
@@ -97,8 +102,7 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
   const bool by_time = period > 0.0;
   int8_t *g_lat = lat_pool + P->lat_off;
   int32_t *g_arr = list_pool + P->lat_off;
-  uint8_t *lat = smem;
-  uint16_t *arr = reinterpret_cast<uint16_t *>(smem + ((n + 15) & ~15));
+  uint16_t *arr = reinterpret_cast<uint16_t *>(smem);
   uint16_t *pos = arr + n;
 
   for (int i = lane; i < MT_N; i += 64) {
@@ -106,14 +110,13 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
     ring[i] = mt_temper(raw[0][i]);
   }
   if (lane < 16) hist[lane] = 0;
-  for (int i = lane; i < n; i += 64) lat[i] = g_lat[i] > 0 ? 1u : 0u;
-  __syncthreads();
   for (int i = lane; i < n; i += 64) {
     const uint32_t l = (uint32_t)g_arr[i];
     arr[i] = (uint16_t)l;
-    pos[l] = (uint16_t)i;
-    atomicAdd(&hist[class_of(lat, l, rows, cols, magic)], 1);
+    pos[l] = (uint16_t)(i | (g_lat[l] > 0 ? SPIN_BIT : 0u));
   }
+  __syncthreads();
+  for (int l = lane; l < n; l += 64) atomicAdd(&hist[class_of(pos, l, rows, cols, magic)], 1);
   __syncthreads();
   // lane k holds start[k] (k <= 10) and rate[k] (k < 10)
   int startv = 0;
@@ -216,20 +219,23 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
     const bool act = up ? (lane > a && lane <= b) : (lane > b && lane <= a);
     uint32_t y = arr[act ? q : 0], px = pos[x];
     asm volatile("" : "+v"(y), "+v"(px));  // both reads issued before either is used
-    const int p = p_known >= 0 ? p_known : rfl32((int32_t)px);
+    const int p = p_known >= 0 ? p_known : rfl32((int32_t)(px & 0x7fffu));
     int qp = lane == (up ? a + 1 : a) ? p : up ? from_below : from_above;
     const int qx = __builtin_amdgcn_readlane(q, up ? b : b + 1);
-    // lane 11 (outside the segment lanes) carries x to its final slot in the
-    // same writes
+    // an entry keeps its class (k - 1 going up, k going down), which fixes its
+    // spin; lane 11 (outside the segment lanes) carries x to its final slot,
+    // in class b, in the same writes
+    int cls = up ? lane - 1 : lane;
     bool wr = act && q != qp;
     if (lane == 11) {
       y = x;
       qp = qx;
+      cls = b;
       wr = true;
     }
     if (wr) {
       arr[qp] = (uint16_t)y;
-      pos[y] = (uint16_t)qp;
+      pos[y] = (uint16_t)(qp | (cls >= 5 ? SPIN_BIT : 0u));
     }
     startv += act ? (up ? -1 : 1) : 0;
     // the wave's LDS operations execute in issue order; keep the compiler from
@@ -321,9 +327,9 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
     const uint32_t yd = xj == 0 ? (uint32_t)cols - 1 : xj - 1;
     const uint32_t xn[4] = {xu + (uint32_t)rows * xj, xd + (uint32_t)rows * xj,
                             xi + (uint32_t)rows * yu, xi + (uint32_t)rows * yd};
-    const int sx = spin_at(lat, xs);
-    const int nsx = spin_at(lat, xn[0]) + spin_at(lat, xn[1]) + spin_at(lat, xn[2]) +
-                    spin_at(lat, xn[3]);
+    const int sx = spin_at(pos, xs);
+    const int nsx = spin_at(pos, xn[0]) + spin_at(pos, xn[1]) + spin_at(pos, xn[2]) +
+                    spin_at(pos, xn[3]);
     const int mx = (xn[0] == l) + (xn[1] == l) + (xn[2] == l) + (xn[3] == l);
     const int s_old = __builtin_amdgcn_readlane(sx, 0);
     const int nsum = __builtin_amdgcn_readlane(nsx, 0);
@@ -333,8 +339,7 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
     const int c_post = (sp > 0 ? 5 : 0) + ((nsp + 4) >> 1);
     S += -2LL * s_old * (nsum - self_nb * s_old);
     M += -2LL * s_old;
-    if (lane == 0) lat[l] ^= 1u;
-    asm volatile("" ::: "memory");
+    // the flip itself is written with l's move (its spin bit follows its class)
     for (int t = 0; t < 5; ++t) {
       const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)xs, t);
       const int a = __builtin_amdgcn_readlane(c_pre, t), b = __builtin_amdgcn_readlane(c_post, t);
@@ -393,7 +398,7 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
   }
   for (int i = lane; i < MT_N; i += 64) St->mt[i] = raw[g_fin & 1][i];
   for (int i = lane; i < n; i += 64) {
-    g_lat[i] = lat[i] ? (int8_t)1 : (int8_t)-1;
+    g_lat[i] = (pos[i] & SPIN_BIT) ? (int8_t)1 : (int8_t)-1;
     g_arr[i] = (int32_t)arr[i];
   }
   if (lane == 0) {

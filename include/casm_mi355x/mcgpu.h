@@ -68,7 +68,10 @@ extern "C" {
  *   then) or, with mcg_desc.sample_time_period > 0, on the time grid (one record
  *   per residence interval holding grid points, weighted by their number).
  *   Occupations, counts and the event sequence are bit-identical to the
- *   oracle's; times agree to the rounding of log1p. */
+ *   oracle's; times agree to the rounding of log1p.  At most 32768 sites per
+ *   chain, held in LDS as the class array and the positions (4 bytes a site).
+ *   On the time grid, a chain whose total rate reaches 0 (every flip's
+ *   probability underflows) fails the run with MCG_ERR_STATE. */
 #define MCG_MODE_NFOLD 3
 
 #define MCG_LATTICE_SQUARE 2 /* 2D, shape[2] == 1 */
