@@ -144,6 +144,17 @@ def bench_nfold(passes):
     out = dict(chains=len(conds), passes=passes, wall_s=w, kernel_s=k, events=a,
                events_per_s=a / w, metropolis_equivalent_attempts_per_s=equiv / w)
     r.close()
+    # the chip holds two 128x128 chains per CU: 8 independent replicas (seeds)
+    # of each condition fill it
+    reps = [dict(c, seed=c["seed"] + 100000 * k) for k in range(8) for c in conds]
+    f = cm.Run(reps, mode=cm.MODE_NFOLD, J1=0.1)
+    f.run(1)
+    fp = max(passes // 8, 5)
+    fw, fk, fa = timed_run(f, fp)
+    out["full_chip"] = dict(chains=len(reps), passes=fp, wall_s=fw, kernel_s=fk, events=fa,
+                            events_per_s=fa / fw,
+                            note="8 seeds of each of the 64 conditions, two chains per CU")
+    f.close()
     p = cm.Run(conds[:1], mode=cm.MODE_NFOLD, J1=0.1)
     p.run(1)
     pp = max(passes // 10, 5)
