@@ -222,9 +222,71 @@ __global__ __launch_bounds__(64) void exact_kernel(const ExactParams *__restrict
 // ---------------------------------------------------------------------------
 constexpr uint32_t SPIN_BIT = 1u, MARK_A = 2u, MARK_B = 4u;
 
+// The engine's generations come from a second wavefront of the workgroup.
+// raw[g % 3] holds generation g untempered and ring[g % 3] its tempered
+// outputs.  The walk reads two generations at a time (G-1 and G, G the newest
+// it has), so the producer can write generation G+1 into the third slot while
+// the walk works.  The waves meet at one workgroup barrier per generation:
+//   B_g: generation g+1 is in its slots; the walk passes it when its window
+//   first needs generation g+1, the producer after writing it.
+// The producer twists generation g right after B_{g-2}: by then the walk's
+// window reaches into generation g-1, so its last consumed word lies in
+// generation g-2 or later and the slots of generation g-3 are free.  Both waves
+// count the barriers.  When the walk is done it writes the number its final
+// barrier will have into `done` and passes that barrier, which is the one the
+// producer waits at; the producer returns after the barrier whose number equals
+// `done` (a read that races ahead of the final barrier sees a larger number and
+// goes on to the final barrier).
+__device__ void mt_producer(uint64_t (*raw)[MT_N], uint64_t (*ring)[MT_N],
+                            volatile const int *done, int L) {
+  const bool third = L < MT_N - MT_M - 128;
+  const bool wraps = L == MT_N - MT_M - 128 - 1;
+  int nbar = 0;
+  for (int64_t g = 1;; ++g) {
+    const uint64_t *src = raw[(g - 1) % 3];
+    uint64_t *dst = raw[g % 3];
+    uint64_t *slot = ring[g % 3];
+    // generation g exactly as twist_gen forms it (lane L: words L, L+64,
+    // L+128, L+156, L+220, L+284)
+    const uint64_t a0 = src[L], a0n = src[L + 1], a1 = src[L + 64], a1n = src[L + 65];
+    const uint64_t b0 = src[L + 156], b0n = src[L + 157], b1 = src[L + 220], b1n = src[L + 221];
+    uint64_t a2 = 0, a2n = 0, b2 = 0, b2n = 0;
+    if (third) {
+      a2 = src[L + 128];
+      a2n = src[L + 129];
+      b2 = src[L + 284];
+      if (!wraps) b2n = src[L + 285];
+    }
+    const uint64_t d0 = mt_twist_word(a0, a0n, b0);
+    const uint64_t d1 = mt_twist_word(a1, a1n, b1);
+    const uint64_t d2 = mt_twist_word(a2, a2n, b2);
+    const uint64_t new0 = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(d0 >> 32), 0) << 32) |
+                          (uint32_t)__builtin_amdgcn_readlane((uint32_t)d0, 0);
+    const uint64_t e0 = mt_twist_word(b0, b0n, d0);
+    const uint64_t e1 = mt_twist_word(b1, b1n, d1);
+    const uint64_t e2 = mt_twist_word(b2, wraps ? new0 : b2n, d2);
+    dst[L] = d0;
+    dst[L + 64] = d1;
+    dst[L + 156] = e0;
+    dst[L + 220] = e1;
+    slot[L] = mt_temper(d0);
+    slot[L + 64] = mt_temper(d1);
+    slot[L + 156] = mt_temper(e0);
+    slot[L + 220] = mt_temper(e1);
+    if (third) {
+      dst[L + 128] = d2;
+      dst[L + 284] = e2;
+      slot[L + 128] = mt_temper(d2);
+      slot[L + 284] = mt_temper(e2);
+    }
+    __syncthreads();  // B_{g-1}
+    if (*done == ++nbar) return;
+  }
+}
+
 __device__ __forceinline__ int spin_of(uint32_t b) { return (b & SPIN_BIT) ? 1 : -1; }
 
-__global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__restrict__ Pp,
+__global__ __launch_bounds__(128) void exact_block_kernel(const ExactParams *__restrict__ Pp,
                                                          ExactState *__restrict__ Sp,
                                                          int8_t *__restrict__ lat_pool,
                                                          mcg_sample *__restrict__ series,
@@ -234,12 +296,14 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
                                                          int64_t sample_period) {
   const int c = blockIdx.x;
   if (!active[c]) return;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool producer = threadIdx.x >= 64;  // wave 1 twists the engine; wave 0 walks
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ uint64_t raw[2][MT_N];
-  __shared__ uint64_t ring[RING];
+  __shared__ uint64_t raw[3][MT_N];
+  __shared__ uint64_t ring[3][MT_N];
   __shared__ uint64_t thr_s[EXACT_NCLASS];
   __shared__ double dq_s[EXACT_NCLASS];
+  __shared__ int done;
 
   const ExactParams *P = Pp + c;
   ExactState *St = Sp + c;
@@ -253,16 +317,23 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
   uint32_t *latw = reinterpret_cast<uint32_t *>(smem);
   int32_t *scratch = reinterpret_cast<int32_t *>(smem + lat_bytes_al);
 
-  for (int i = lane; i < MT_N; i += 64) {
-    raw[0][i] = St->mt[i];
-    ring[i] = mt_temper(raw[0][i]);
+  if (!producer) {
+    for (int i = lane; i < MT_N; i += 64) {
+      raw[0][i] = St->mt[i];
+      ring[0][i] = mt_temper(raw[0][i]);
+    }
+    if (lane < EXACT_NCLASS) {
+      thr_s[lane] = P->thr[lane];
+      dq_s[lane] = P->dq[lane];
+    }
+    for (int i = lane; i < lat_bytes_al; i += 64) lat[i] = (i < n && g_lat[i] > 0) ? 1u : 0u;
+    if (lane == 0) done = 0;
   }
-  if (lane < EXACT_NCLASS) {
-    thr_s[lane] = P->thr[lane];
-    dq_s[lane] = P->dq[lane];
-  }
-  for (int i = lane; i < lat_bytes_al; i += 64) lat[i] = (i < n && g_lat[i] > 0) ? 1u : 0u;
   __syncthreads();
+  if (producer) {
+    mt_producer(raw, ring, &done, lane);
+    return;
+  }
 
   const int64_t pos0 = St->pos;
   int64_t pos = pos0;  // absolute stream position; generation g holds [312g, 312g+312)
@@ -272,16 +343,20 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
   double e_run = St->e_run;
   const uint64_t N64 = (uint64_t)n;
 
+  int nbar = 0;  // hand-off barriers passed
+  int sG = 0;    // G % 3
   auto ensure = [&](int64_t upto) {  // make positions < upto readable
     while (upto > (int64_t)MT_N * (G + 1)) {
-      twist_gen(raw[G & 1], raw[(G + 1) & 1], ring + ((G + 1) & 1) * MT_N, lane);
+      __syncthreads();  // B_G: generation G+1 is in its slots
+      ++nbar;
       ++G;
+      sG = sG == 2 ? 0 : sG + 1;
     }
   };
-  auto ring_index = [&](int64_t q) -> int {  // q mod 624, q in [312(G-1), 312(G+1))
-    const int64_t base = (int64_t)MT_N * (G - 1);
-    int r = (int)(q - base) + (int)(((G - 1) & 1) * MT_N);
-    return r >= RING ? r - RING : r;
+  const uint64_t *ringf = &ring[0][0];
+  auto ring_index = [&](int64_t q) -> int {  // slot offset of q in [312(G-1), 312(G+1))
+    const int r = (int)(q - (int64_t)MT_N * G);
+    return r >= 0 ? r + MT_N * sG : r + MT_N * (sG == 0 ? 3 : sG);
   };
 
   int32_t step = 0;
@@ -290,10 +365,13 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
     ensure(pos + W + 1);
     const bool inwin = lane < W;
     const int64_t q = pos + lane;
-    const uint64_t x = ring[ring_index(inwin ? q : pos)];
-    const uint64_t x1 = ring[ring_index(inwin ? q + 1 : pos)];
-    const uint64_t lo = x * N64;
-    const uint32_t site = (uint32_t)__umul64hi(x, N64);
+    const uint64_t x = ringf[ring_index(inwin ? q : pos)];
+    const uint64_t x1 = ringf[ring_index(inwin ? q + 1 : pos)];
+    // x * n (n < 2^24) from two 32x32->64 products: site = high word, lo = low
+    const uint64_t p0 = (uint64_t)(uint32_t)x * (uint32_t)n;
+    const uint64_t p1 = (uint64_t)(uint32_t)(x >> 32) * (uint32_t)n + (p0 >> 32);
+    const uint64_t lo = (p1 << 32) | (uint32_t)p0;
+    const uint32_t site = (uint32_t)(p1 >> 32);
     const bool special = inwin && (lo < N64);
     const bool valid = inwin && !special;
     const uint32_t j = (uint32_t)(((uint64_t)site * magic) >> 40);
@@ -305,15 +383,13 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
     const uint32_t sh = (site & 3u) * 8u;
     const uint32_t a0 = site, a1 = iu + rows * j, a2 = id + rows * j;
     const uint32_t a3 = i + rows * ju, a4 = i + rows * jd;
-    const int s = spin_of(lat[a0]);
-    const int nsum = spin_of(lat[a1]) + spin_of(lat[a2]) + spin_of(lat[a3]) + spin_of(lat[a4]);
-    const int k = (s > 0 ? 5 : 0) + ((nsum + 4) >> 1);
-    const uint64_t thr_k = thr_s[k];
-    const double dqv = dq_s[k];
+    // at a block's start every mark is clear, so each byte is its spin bit:
+    // the class (s > 0) * 5 + (nsum + 4) / 2 is 5 b0 + b1 + b2 + b3 + b4
+    const int b0 = lat[a0], nup = lat[a1] + lat[a2] + lat[a3] + lat[a4];
+    const int s = 2 * b0 - 1, nsum = 2 * nup - 4;
+    const int k = 5 * b0 + nup;
     const bool nd = !((neg_mask >> k) & 1u);
-    const bool acc = !nd || ((full_mask >> k) & 1u) || (x1 < thr_k);
     const uint64_t ND = __ballot(valid && nd);
-    const uint64_t AC = __ballot(valid && acc);
     const uint64_t SP = __ballot(special);
 
     // Attempt positions of the window in closed form: position t is a uniform
@@ -332,7 +408,14 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
     const bool att = valid && ((A >> lane) & 1ull);
     if (att) atomicAdd(&latw[site >> 2], MARK_A << sh);
     asm volatile("" ::: "memory");
-    const uint32_t m0 = lat[a0], m1 = lat[a1], m2 = lat[a2], m3 = lat[a3], m4 = lat[a4];
+    // the class tables go out with the mark reads: one round trip (the asm
+    // takes all seven values, so none of the reads moves into a branch)
+    uint32_t m0 = lat[a0], m1 = lat[a1], m2 = lat[a2], m3 = lat[a3], m4 = lat[a4];
+    uint64_t thr_k = thr_s[k];
+    double dqv = dq_s[k];
+    asm volatile("" : "+v"(m0), "+v"(m1), "+v"(m2), "+v"(m3), "+v"(m4), "+v"(thr_k), "+v"(dqv));
+    const bool acc = !nd || ((full_mask >> k) & 1u) || (x1 < thr_k);
+    const uint64_t AC = __ballot(valid && acc);
     // potential conflict: own site drawn twice, or a neighbour drawn at all
     const bool pc = att && ((m0 & 0xFCu) || ((m1 | m2 | m3 | m4) & 0xFEu));
     const uint64_t STOP = __ballot(pc || special || !inwin);
@@ -378,14 +461,14 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
 
     if (SP & 1ull) {
       // one attempt with the full Lemire rejection loop (scalar walk)
-      uint64_t xs = ring[ring_index(pos)];
+      uint64_t xs = ringf[ring_index(pos)];
       ++pos;
       uint64_t los = xs * N64, his = __umul64hi(xs, N64);
       if (los < N64) {
         const uint64_t th = (0ULL - N64) % N64;
         while (los < th) {
           ensure(pos + 1);
-          xs = ring[ring_index(pos)];
+          xs = ringf[ring_index(pos)];
           ++pos;
           los = xs * N64;
           his = __umul64hi(xs, N64);
@@ -407,7 +490,7 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
         a = true;
       } else {
         ensure(pos + 1);
-        const uint64_t x2 = ring[ring_index(pos)];
+        const uint64_t x2 = ringf[ring_index(pos)];
         ++pos;
         a = ((full_mask >> kk) & 1u) || (x2 < thr_s[kk]);
       }
@@ -420,7 +503,10 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
       } else {
         ++n_rej;
       }
-      __syncthreads();
+      // one wavefront walks: its LDS operations execute in issue order, and
+      // the asm keeps the compiler from reordering them (no workgroup barrier
+      // here: the producer wave takes part only in the engine hand-offs)
+      asm volatile("" ::: "memory");
       ++step;
     }
 
@@ -445,14 +531,14 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
               d += spin_of(lat[r + rows * qq]) * spin_of(lat[r + rows * qu]);
             scratch[rows + qq] = d;
           }
-          __syncthreads();
+          asm volatile("" ::: "memory");
           if (lane == 0) {
             double e = 0.0;  // model.hh:276-285
             for (int r = 0; r < rows + cols; ++r)
               e = __dadd_rn(e, __dmul_rn(P->negJ, (double)scratch[r]));
             ef = e;
           }
-          __syncthreads();
+          asm volatile("" ::: "memory");
         }
         if (lane == 0 && n_samples < series_cap) {
           mcg_sample rec = {};
@@ -469,15 +555,18 @@ __global__ __launch_bounds__(64) void exact_block_kernel(const ExactParams *__re
     }
   }
 
+  // release the producer: it is waiting at its next barrier, which this one
+  // (number nbar + 1) matches; it then reads the number and returns
+  if (lane == 0) done = nbar + 1;
+  __syncthreads();
   // write back in the std::mt19937_64 layout: the generation holding the last
   // consumed output, with index in (0, 312]
-  __syncthreads();
   int64_t g_fin = 0, p_fin = pos0;
   if (pos != pos0) {
     g_fin = (pos - 1) / MT_N;
     p_fin = pos - (int64_t)MT_N * g_fin;
   }
-  for (int i2 = lane; i2 < MT_N; i2 += 64) St->mt[i2] = raw[g_fin & 1][i2];
+  for (int i2 = lane; i2 < MT_N; i2 += 64) St->mt[i2] = raw[g_fin % 3][i2];
   for (int i2 = lane; i2 < n; i2 += 64) g_lat[i2] = (lat[i2] & SPIN_BIT) ? (int8_t)1 : (int8_t)-1;
   if (lane == 0) {
     St->pos = p_fin;
@@ -496,7 +585,7 @@ void launch_exact(const ExactParams *P, ExactState *S, int8_t *lat_pool, mcg_sam
                   int64_t sample_period, int lds_bytes, bool lattice_in_lds, bool block_walk,
                   hipStream_t stream) {
   if (lattice_in_lds && block_walk) {
-    hipLaunchKernelGGL(exact_block_kernel, dim3(n_chains), dim3(64), lds_bytes, stream, P, S,
+    hipLaunchKernelGGL(exact_block_kernel, dim3(n_chains), dim3(128), lds_bytes, stream, P, S,
                        lat_pool, series, series_cap, active, n_passes, sample_period);
   } else if (lattice_in_lds) {
     hipLaunchKernelGGL(exact_kernel<true>, dim3(n_chains), dim3(64), lds_bytes, stream, P, S,
@@ -509,7 +598,7 @@ void launch_exact(const ExactParams *P, ExactState *S, int8_t *lat_pool, mcg_sam
 
 void exact_set_max_lds(int bytes) {
   hipFuncSetAttribute(reinterpret_cast<const void *>(exact_block_kernel),
-                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes - 8 * 1024);
+                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes - 12 * 1024);
   hipFuncSetAttribute(reinterpret_cast<const void *>(exact_kernel<true>),
                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   hipFuncSetAttribute(reinterpret_cast<const void *>(exact_kernel<false>),

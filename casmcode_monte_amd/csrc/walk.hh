@@ -32,18 +32,50 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
   return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-// Out-of-place twist of generation src into dst (one 64-lane workgroup) and
-// temper it into the ring slot.
+// Out-of-place twist of generation src into dst by one 64-lane wavefront
+// (the workgroup), tempering the new words into the ring slot.  Lane L owns
+// words L, L+64, L+128 (the first 156, which read only old words) and L+156,
+// L+220, L+284 (the rest, each of which reads the new word 156 below it: one
+// this lane computed).  Word 311 also reads the new word 0 (lane 0's).  All
+// reads go out before any arithmetic, and since src and dst differ no write can
+// disturb a read: one LDS round trip per generation.  The wave's LDS operations
+// execute in issue order, so the caller's later reads see these writes.
 __device__ inline void twist_gen(const uint64_t *src, uint64_t *dst, uint64_t *slot, int lane) {
-  for (int k = lane; k < MT_N - MT_M; k += 64)
-    dst[k] = mt_twist_word(src[k], src[k + 1], src[k + MT_M]);
-  __syncthreads();
-  for (int k = MT_N - MT_M + lane; k < MT_N - 1; k += 64)
-    dst[k] = mt_twist_word(src[k], src[k + 1], dst[k - (MT_N - MT_M)]);
-  if (lane == 0) dst[MT_N - 1] = mt_twist_word(src[MT_N - 1], dst[0], dst[MT_M - 1]);
-  __syncthreads();
-  for (int k = lane; k < MT_N; k += 64) slot[k] = mt_temper(dst[k]);
-  __syncthreads();
+  const int L = lane;
+  const bool third = L < MT_N - MT_M - 128;  // words L+128 and L+284 exist (L < 28)
+  const bool wraps = L == MT_N - MT_M - 128 - 1;  // L+284 == 311
+  const uint64_t a0 = src[L], a0n = src[L + 1], a1 = src[L + 64], a1n = src[L + 65];
+  const uint64_t b0 = src[L + 156], b0n = src[L + 157], b1 = src[L + 220], b1n = src[L + 221];
+  uint64_t a2 = 0, a2n = 0, b2 = 0, b2n = 0;
+  if (third) {
+    a2 = src[L + 128];
+    a2n = src[L + 129];
+    b2 = src[L + 284];
+    if (!wraps) b2n = src[L + 285];
+  }
+  const uint64_t d0 = mt_twist_word(a0, a0n, b0);
+  const uint64_t d1 = mt_twist_word(a1, a1n, b1);
+  const uint64_t d2 = mt_twist_word(a2, a2n, b2);
+  const uint64_t new0 = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(d0 >> 32), 0) << 32) |
+                        (uint32_t)__builtin_amdgcn_readlane((uint32_t)d0, 0);
+  const uint64_t e0 = mt_twist_word(b0, b0n, d0);
+  const uint64_t e1 = mt_twist_word(b1, b1n, d1);
+  const uint64_t e2 = mt_twist_word(b2, wraps ? new0 : b2n, d2);
+  dst[L] = d0;
+  dst[L + 64] = d1;
+  dst[L + 156] = e0;
+  dst[L + 220] = e1;
+  slot[L] = mt_temper(d0);
+  slot[L + 64] = mt_temper(d1);
+  slot[L + 156] = mt_temper(e0);
+  slot[L + 220] = mt_temper(e1);
+  if (third) {
+    dst[L + 128] = d2;
+    dst[L + 284] = e2;
+    slot[L + 128] = mt_temper(d2);
+    slot[L + 284] = mt_temper(e2);
+  }
+  asm volatile("" ::: "memory");
 }
 
 }  // namespace mcg
