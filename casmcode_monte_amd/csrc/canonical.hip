@@ -1,6 +1,7 @@
 // canonical.hip -- CANONICAL mode: the reference's canonical pair-swap chain
 // (Kawasaki exchanges between any two sites of different species), reproduced
-// bit for bit on one wavefront per chain.
+// bit for bit on one wavefront per chain; a second wavefront of the workgroup
+// twists the engine's next generation meanwhile (mt_producer, walk.hh).
 //
 // Reference: the generic loop of occupation_metropolis.hh:87-153 with
 // propose_canonical_event (OccEventProposal.hh:109-236) over an OccLocation
@@ -63,7 +64,7 @@ __device__ __forceinline__ Nbrs nbrs_of(uint32_t l, int32_t rows, int32_t cols, 
 
 }  // namespace
 
-__global__ __launch_bounds__(64) void canonical_kernel(const CanonParams *__restrict__ Pp,
+__global__ __launch_bounds__(128) void canonical_kernel(const CanonParams *__restrict__ Pp,
                                                        ExactState *__restrict__ Sp,
                                                        int8_t *__restrict__ lat_pool,
                                                        int32_t *__restrict__ list_pool,
@@ -74,10 +75,12 @@ __global__ __launch_bounds__(64) void canonical_kernel(const CanonParams *__rest
                                                        int64_t sample_period) {
   const int c = blockIdx.x;
   if (!active[c]) return;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool producer = threadIdx.x >= 64;
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ uint64_t raw[2][MT_N];
-  __shared__ uint64_t ring[RING];
+  __shared__ uint64_t raw[3][MT_N];
+  __shared__ uint64_t ring[3][MT_N];
+  __shared__ int done;
   __shared__ uint64_t thr_s[CANON_NCLASS];
   __shared__ double dq_s[CANON_NCLASS];
 
@@ -98,51 +101,50 @@ __global__ __launch_bounds__(64) void canonical_kernel(const CanonParams *__rest
   int32_t *scratch = reinterpret_cast<int32_t *>(smem + lat_al + 4 * (mmask + 1) +
                                                  ((2 * n + 15) & ~15));
 
-  for (int i = lane; i < MT_N; i += 64) {
-    raw[0][i] = St->mt[i];
-    ring[i] = mt_temper(raw[0][i]);
+  if (!producer) {
+    for (int i = lane; i < MT_N; i += 64) {
+      raw[0][i] = St->mt[i];
+      ring[0][i] = mt_temper(raw[0][i]);
+    }
+    if (lane < CANON_NCLASS) {
+      thr_s[lane] = P->thr[lane];
+      dq_s[lane] = P->dq[lane];
+    }
+    for (int i = lane; i < n; i += 64) {
+      lat[i] = g_lat[i] > 0 ? 1u : 0u;
+      list0[i] = (uint16_t)g_list[i];
+    }
+    for (uint32_t i = lane; i <= mmask; i += 64) marks[i] = NO_MARK;
+    if (lane == 0) done = 0;
   }
-  if (lane < CANON_NCLASS) {
-    thr_s[lane] = P->thr[lane];
-    dq_s[lane] = P->dq[lane];
-  }
-  for (int i = lane; i < n; i += 64) {
-    lat[i] = g_lat[i] > 0 ? 1u : 0u;
-    list0[i] = (uint16_t)g_list[i];
-  }
-  for (uint32_t i = lane; i <= mmask; i += 64) marks[i] = NO_MARK;
   __syncthreads();
+  if (producer) {  // wave 1 twists the engine (walk.hh); wave 0 walks
+    mt_producer(raw, ring, &done, lane);
+    return;
+  }
 
   const int64_t pos0 = St->pos;
   int64_t pos = pos0;
-  int64_t G = 0;
+  MtRing3 eng;  // engine generations from the producer wave
   int64_t n_pass = St->n_pass, n_acc = St->n_accept, n_rej = St->n_reject;
   int64_t S = St->S, M = St->M, n_samples = St->n_samples;
   double e_run = St->e_run;
   const uint64_t NA64 = nA, NB64 = nB;
 
-  auto ensure = [&](int64_t upto) {
-    while (upto > (int64_t)MT_N * (G + 1)) {
-      twist_gen(raw[G & 1], raw[(G + 1) & 1], ring + ((G + 1) & 1) * MT_N, lane);
-      ++G;
-    }
-  };
-  auto ring_index = [&](int64_t q) -> int {
-    const int64_t base = (int64_t)MT_N * (G - 1);
-    int r = (int)(q - base) + (int)(((G - 1) & 1) * MT_N);
-    return r >= RING ? r - RING : r;
-  };
+  auto ensure = [&](int64_t upto) { eng.ensure(upto); };
+  const uint64_t *ringf = &ring[0][0];
+  auto ring_index = [&](int64_t q) -> int { return eng.index(q); };
   // libstdc++ uniform_int_distribution over [0, m-1] with the rejection loop
   auto lemire_scalar = [&](uint64_t m) -> uint32_t {
     ensure(pos + 1);
-    uint64_t x = ring[ring_index(pos)];
+    uint64_t x = ringf[ring_index(pos)];
     ++pos;
     uint64_t lo = x * m, hi = __umul64hi(x, m);
     if (lo < m) {
       const uint64_t th = (0ULL - m) % m;
       while (lo < th) {
         ensure(pos + 1);
-        x = ring[ring_index(pos)];
+        x = ringf[ring_index(pos)];
         ++pos;
         lo = x * m;
         hi = __umul64hi(x, m);
@@ -157,9 +159,9 @@ __global__ __launch_bounds__(64) void canonical_kernel(const CanonParams *__rest
     ensure(pos + W + 3);
     const bool inwin = lane < W;
     const int64_t q = inwin ? pos + lane : pos;
-    const uint64_t x1 = ring[ring_index(q + 1)];
-    const uint64_t x2 = ring[ring_index(q + 2)];
-    const uint64_t x3 = ring[ring_index(q + 3)];
+    const uint64_t x1 = ringf[ring_index(q + 1)];
+    const uint64_t x2 = ringf[ring_index(q + 2)];
+    const uint64_t x3 = ringf[ring_index(q + 3)];
     const uint32_t ia = (uint32_t)__umul64hi(x1, NA64);
     const uint32_t ib = (uint32_t)__umul64hi(x2, NB64);
     const bool special = inwin && ((x1 * NA64) < NA64 || (x2 * NB64) < NB64);
@@ -273,11 +275,14 @@ __global__ __launch_bounds__(64) void canonical_kernel(const CanonParams *__rest
         ok = true;
       } else {
         ensure(pos + 1);
-        const uint64_t xr = ring[ring_index(pos)];
+        const uint64_t xr = ringf[ring_index(pos)];
         ++pos;
         ok = ((full_mask >> kk) & 1u) || (xr < thr_s[kk]);
       }
-      __syncthreads();
+      // one wavefront walks: its LDS operations execute in issue order and the
+      // asm keeps the compiler from reordering them (the producer wave takes
+      // part only in the engine hand-offs)
+      asm volatile("" ::: "memory");
       if (ok) {
         if (lane == 0) {
           const uint16_t bk = list0[nA - 1];
@@ -293,7 +298,7 @@ __global__ __launch_bounds__(64) void canonical_kernel(const CanonParams *__rest
       } else {
         ++n_rej;
       }
-      __syncthreads();
+      asm volatile("" ::: "memory");
       ++step;
     }
 
@@ -304,7 +309,7 @@ __global__ __launch_bounds__(64) void canonical_kernel(const CanonParams *__rest
       if (n_pass % sample_period == 0) {
         double ef = 0.0;
         if (fold) {
-          __syncthreads();
+          asm volatile("" ::: "memory");
           for (int r = lane; r < rows; r += 64) {
             const int ru = (r + 1 == rows) ? 0 : r + 1;
             int d = 0;
@@ -319,14 +324,14 @@ __global__ __launch_bounds__(64) void canonical_kernel(const CanonParams *__rest
               d += spin_bit(lat[r + rows * qq]) * spin_bit(lat[r + rows * qu]);
             scratch[rows + qq] = d;
           }
-          __syncthreads();
+          asm volatile("" ::: "memory");
           if (lane == 0) {
             double e = 0.0;  // model.hh:276-285
             for (int r = 0; r < rows + cols; ++r)
               e = __dadd_rn(e, __dmul_rn(P->negJ, (double)scratch[r]));
             ef = e;
           }
-          __syncthreads();
+          asm volatile("" ::: "memory");
         }
         if (lane == 0 && n_samples < series_cap) {
           mcg_sample rec = {};
@@ -342,13 +347,14 @@ __global__ __launch_bounds__(64) void canonical_kernel(const CanonParams *__rest
     }
   }
 
-  __syncthreads();
+  // release the producer (it waits at the barrier this one matches)
+  eng.finish(&done, lane);
   int64_t g_fin = 0, p_fin = pos0;
   if (pos != pos0) {
     g_fin = (pos - 1) / MT_N;
     p_fin = pos - (int64_t)MT_N * g_fin;
   }
-  for (int i = lane; i < MT_N; i += 64) St->mt[i] = raw[g_fin & 1][i];
+  for (int i = lane; i < MT_N; i += 64) St->mt[i] = raw[g_fin % 3][i];
   for (int i = lane; i < n; i += 64) {
     g_lat[i] = lat[i] ? (int8_t)1 : (int8_t)-1;
     g_list[i] = (int32_t)list0[i];
@@ -369,7 +375,7 @@ void launch_canonical(const CanonParams *P, ExactState *S, int8_t *lat_pool, int
                       mcg_sample *series, int64_t series_cap, const uint8_t *active,
                       int n_chains, int64_t n_passes, int64_t sample_period, int lds_bytes,
                       hipStream_t stream) {
-  hipLaunchKernelGGL(canonical_kernel, dim3(n_chains), dim3(64), lds_bytes, stream, P, S,
+  hipLaunchKernelGGL(canonical_kernel, dim3(n_chains), dim3(128), lds_bytes, stream, P, S,
                      lat_pool, list_pool, series, series_cap, active, n_passes, sample_period);
 }
 

@@ -7,11 +7,14 @@
 // metropolis_acceptance (metropolis.hh:25-35) and RandomNumberGenerator over
 // std::mt19937_64 (RandomNumberGenerator.hh:31-41).
 //
-// Layout: workgroup = one 64-lane wavefront = one chain.  The chain's int8
-// lattice (column-major, l = i + rows*j) lives in LDS when it fits, else in HBM.
-// The MT19937-64 state lives in LDS and is twisted by all 64 lanes together
-// (two dependency-free phases of 156 words); every other step is a scalar walk
-// that all lanes execute in lockstep with scalar (readfirstlane) operands.
+// Layout: one chain per workgroup.  The chain's int8 lattice (column-major,
+// l = i + rows*j) lives in LDS when it fits, else in HBM.  The scalar walk
+// (exact_kernel) is one 64-lane wavefront that twists the MT19937-64 state in
+// LDS itself (two dependency-free phases of 156 words) and executes every other
+// step in lockstep with scalar (readfirstlane) operands.  The block walk
+// (exact_block_kernel) has two wavefronts: wave 0 walks, wave 1 twists the
+// engine's next generation while wave 0 consumes the current one (mt_producer,
+// walk.hh).
 //
 // Exactness: a site is drawn by Lemire's method on one engine word; the
 // Metropolis decision for class k (old spin, neighbour sum) is either "dE < 0,
@@ -222,68 +225,6 @@ __global__ __launch_bounds__(64) void exact_kernel(const ExactParams *__restrict
 // ---------------------------------------------------------------------------
 constexpr uint32_t SPIN_BIT = 1u, MARK_A = 2u, MARK_B = 4u;
 
-// The engine's generations come from a second wavefront of the workgroup.
-// raw[g % 3] holds generation g untempered and ring[g % 3] its tempered
-// outputs.  The walk reads two generations at a time (G-1 and G, G the newest
-// it has), so the producer can write generation G+1 into the third slot while
-// the walk works.  The waves meet at one workgroup barrier per generation:
-//   B_g: generation g+1 is in its slots; the walk passes it when its window
-//   first needs generation g+1, the producer after writing it.
-// The producer twists generation g right after B_{g-2}: by then the walk's
-// window reaches into generation g-1, so its last consumed word lies in
-// generation g-2 or later and the slots of generation g-3 are free.  Both waves
-// count the barriers.  When the walk is done it writes the number its final
-// barrier will have into `done` and passes that barrier, which is the one the
-// producer waits at; the producer returns after the barrier whose number equals
-// `done` (a read that races ahead of the final barrier sees a larger number and
-// goes on to the final barrier).
-__device__ void mt_producer(uint64_t (*raw)[MT_N], uint64_t (*ring)[MT_N],
-                            volatile const int *done, int L) {
-  const bool third = L < MT_N - MT_M - 128;
-  const bool wraps = L == MT_N - MT_M - 128 - 1;
-  int nbar = 0;
-  for (int64_t g = 1;; ++g) {
-    const uint64_t *src = raw[(g - 1) % 3];
-    uint64_t *dst = raw[g % 3];
-    uint64_t *slot = ring[g % 3];
-    // generation g exactly as twist_gen forms it (lane L: words L, L+64,
-    // L+128, L+156, L+220, L+284)
-    const uint64_t a0 = src[L], a0n = src[L + 1], a1 = src[L + 64], a1n = src[L + 65];
-    const uint64_t b0 = src[L + 156], b0n = src[L + 157], b1 = src[L + 220], b1n = src[L + 221];
-    uint64_t a2 = 0, a2n = 0, b2 = 0, b2n = 0;
-    if (third) {
-      a2 = src[L + 128];
-      a2n = src[L + 129];
-      b2 = src[L + 284];
-      if (!wraps) b2n = src[L + 285];
-    }
-    const uint64_t d0 = mt_twist_word(a0, a0n, b0);
-    const uint64_t d1 = mt_twist_word(a1, a1n, b1);
-    const uint64_t d2 = mt_twist_word(a2, a2n, b2);
-    const uint64_t new0 = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(d0 >> 32), 0) << 32) |
-                          (uint32_t)__builtin_amdgcn_readlane((uint32_t)d0, 0);
-    const uint64_t e0 = mt_twist_word(b0, b0n, d0);
-    const uint64_t e1 = mt_twist_word(b1, b1n, d1);
-    const uint64_t e2 = mt_twist_word(b2, wraps ? new0 : b2n, d2);
-    dst[L] = d0;
-    dst[L + 64] = d1;
-    dst[L + 156] = e0;
-    dst[L + 220] = e1;
-    slot[L] = mt_temper(d0);
-    slot[L + 64] = mt_temper(d1);
-    slot[L + 156] = mt_temper(e0);
-    slot[L + 220] = mt_temper(e1);
-    if (third) {
-      dst[L + 128] = d2;
-      dst[L + 284] = e2;
-      slot[L + 128] = mt_temper(d2);
-      slot[L + 284] = mt_temper(e2);
-    }
-    __syncthreads();  // B_{g-1}
-    if (*done == ++nbar) return;
-  }
-}
-
 __device__ __forceinline__ int spin_of(uint32_t b) { return (b & SPIN_BIT) ? 1 : -1; }
 
 __global__ __launch_bounds__(128) void exact_block_kernel(const ExactParams *__restrict__ Pp,
@@ -337,27 +278,15 @@ __global__ __launch_bounds__(128) void exact_block_kernel(const ExactParams *__r
 
   const int64_t pos0 = St->pos;
   int64_t pos = pos0;  // absolute stream position; generation g holds [312g, 312g+312)
-  int64_t G = 0;       // newest generation in the ring (ring holds G-1 and G)
+  MtRing3 eng;         // engine generations from the producer wave
   int64_t n_pass = St->n_pass, n_acc = St->n_accept, n_rej = St->n_reject;
   int64_t S = St->S, M = St->M, n_samples = St->n_samples;
   double e_run = St->e_run;
   const uint64_t N64 = (uint64_t)n;
 
-  int nbar = 0;  // hand-off barriers passed
-  int sG = 0;    // G % 3
-  auto ensure = [&](int64_t upto) {  // make positions < upto readable
-    while (upto > (int64_t)MT_N * (G + 1)) {
-      __syncthreads();  // B_G: generation G+1 is in its slots
-      ++nbar;
-      ++G;
-      sG = sG == 2 ? 0 : sG + 1;
-    }
-  };
+  auto ensure = [&](int64_t upto) { eng.ensure(upto); };
   const uint64_t *ringf = &ring[0][0];
-  auto ring_index = [&](int64_t q) -> int {  // slot offset of q in [312(G-1), 312(G+1))
-    const int r = (int)(q - (int64_t)MT_N * G);
-    return r >= 0 ? r + MT_N * sG : r + MT_N * (sG == 0 ? 3 : sG);
-  };
+  auto ring_index = [&](int64_t q) -> int { return eng.index(q); };
 
   int32_t step = 0;
   int64_t passes_done = 0;
@@ -556,9 +485,8 @@ __global__ __launch_bounds__(128) void exact_block_kernel(const ExactParams *__r
   }
 
   // release the producer: it is waiting at its next barrier, which this one
-  // (number nbar + 1) matches; it then reads the number and returns
-  if (lane == 0) done = nbar + 1;
-  __syncthreads();
+  // matches; it then reads the barrier's number and returns
+  eng.finish(&done, lane);
   // write back in the std::mt19937_64 layout: the generation holding the last
   // consumed output, with index in (0, 312]
   int64_t g_fin = 0, p_fin = pos0;
