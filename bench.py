@@ -383,7 +383,8 @@ def main():
         line["roofline_checkerboard"] = {
             "bound": "hbm", "achieved": fam2["hbm_gbs_algorithmic"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": fam2["hbm_frac"], "traffic": None,
-            "kernel": "checker_color_kernel, 16 x 4096^2 J1+J2 (2 B/attempt algorithmic)"}
+            "kernel": ("checker_band_kernel + checker_seam_kernel" if fam2["schedule"].get("band_lines")
+                       else "checker_color_kernel") + ", 16 x 4096^2 J1+J2 (2 B/attempt algorithmic)"}
     if not args.skip_cpu:
         line["cpu_baseline"] = cpu_baseline(16, 1000)
         print("[bench] cpu baseline done", file=sys.stderr, flush=True)
