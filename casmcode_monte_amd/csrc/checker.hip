@@ -570,12 +570,23 @@ constexpr int PAIR_BLOCK = 256;
 
 // Work items are (chain, first colour-class line, lines, 0): lines straight
 // from HBM (every class line re-reads its neighbour lines).
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup i to XCD
+// i % 8), and each XCD has its own L2.  Work items are ordered so that nearby
+// items read the same neighbour lines; this bijection hands every XCD one
+// contiguous run of items (XCD x takes items [start_x, start_x + count_x) in
+// order), so those shared lines meet in one L2.  Results do not depend on the
+// order: items update disjoint lines and sum integers.
+__device__ __forceinline__ int xcd_item(int i, int n) {
+  const int x = i & 7, local = i >> 3, q = n >> 3, r = n & 7;
+  return x < r ? x * (q + 1) + local : r * (q + 1) + (x - r) * q + local;
+}
+
 __global__ __launch_bounds__(PAIR_BLOCK) void checker_pair_kernel(
     const CheckerParams *__restrict__ Pp, CheckerState *__restrict__ Sp,
     int8_t *__restrict__ lat_pool, const uint32_t *__restrict__ tables,
     const int32_t *__restrict__ work, int line_class) {
   __shared__ int red[4][PAIR_BLOCK / 64];
-  const int32_t *item = work + 4 * blockIdx.x;
+  const int32_t *item = work + 4 * xcd_item(blockIdx.x, gridDim.x);
   const int c = item[0];
   const CheckerParams P = Pp[c];
   CheckerState *St = Sp + c;
@@ -586,14 +597,22 @@ __global__ __launch_bounds__(PAIR_BLOCK) void checker_pair_kernel(
   const int upl = chunk_path(g) ? (g.half >> 4) : g.qpl;  // units per half line
   int dS1 = 0, dS2 = 0, dM = 0, nacc = 0;
   const int l0 = item[1], nlines = item[2];
-  for (int a = 0; a < 2; ++a) {
-    const int color = 2 * line_class + a;
-    for (int task = threadIdx.x; task < nlines * upl; task += PAIR_BLOCK) {
-      const int ln = l0 + task / upl;
-      const int64_t unit = (int64_t)ln * upl + (task - (task / upl) * upl);
-      update_unit(P, g, lat, tab, color, pass, unit, dS1, dS2, dM, nacc);
+  // Both colours of a group of lines (one unit per thread) before the next
+  // group: colour 1 of a line reads only that line's colour-0 half among the
+  // sites this launch changes, so the order is free, and a group's lines are
+  // still in cache for its second colour.
+  const int per_group = upl < PAIR_BLOCK ? PAIR_BLOCK / upl : 1;
+  for (int g0 = 0; g0 < nlines; g0 += per_group) {
+    const int gl = nlines - g0 < per_group ? nlines - g0 : per_group;
+    for (int a = 0; a < 2; ++a) {
+      const int color = 2 * line_class + a;
+      for (int task = threadIdx.x; task < gl * upl; task += PAIR_BLOCK) {
+        const int ln = l0 + g0 + task / upl;
+        const int64_t unit = (int64_t)ln * upl + (task - (task / upl) * upl);
+        update_unit(P, g, lat, tab, color, pass, unit, dS1, dS2, dM, nacc);
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
   dS1 = wave_sum(dS1);
   dS2 = wave_sum(dS2);
