@@ -37,6 +37,7 @@ EXPORTED = [
     "mcg_create", "mcg_destroy", "mcg_last_error", "mcg_set_occupation", "mcg_get_occupation",
     "mcg_rng_dump", "mcg_rng_load", "mcg_run", "mcg_set_active", "mcg_get_counters",
     "mcg_n_samples", "mcg_get_samples", "mcg_clear_samples", "mcg_stats_compute",
+    "mcg_stats_weighted",
     "mcg_equilibration", "mcg_last_run_time", "mcg_assign_chain", "mcg_autotune",
     "mcg_get_schedule", "mcg_set_schedule", "mcg_reserve_samples", "mcg_get_occ_location",
     "mcg_set_occ_location",
@@ -103,6 +104,8 @@ def lib():
                                         P(Stats)]
         L.mcg_equilibration.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_double,
                                         P(C.c_int64), P(C.c_int64)]
+        L.mcg_stats_weighted.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_double,
+                                         C.c_int64, C.c_int64, P(Stats)]
         L.mcg_last_run_time.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
         L.mcg_assign_chain.argtypes = [C.c_void_p, C.c_int64, P(ChainDesc)]
         L.mcg_autotune.argtypes = [C.c_void_p, C.c_int64, P(Schedule)]
@@ -256,6 +259,16 @@ class Run:
     def stats(self, c, observable, first=0, confidence=0.95):
         s = Stats()
         _check(lib().mcg_stats_compute(self.h, c, observable, first, confidence, C.byref(s)))
+        return dict(mean=s.mean, calculated_precision=s.calculated_precision,
+                    variance=s.variance, autocorr_factor=s.autocorr_factor, n=s.n)
+
+    def stats_weighted(self, c, observable, first=0, confidence=0.95, method=1,
+                       n_resamples=10000):
+        """Statistics of weighted (NFOLD) records: BasicStatisticsCalculator's
+        weighted branch (BasicStatistics.cc:144-188)."""
+        s = Stats()
+        _check(lib().mcg_stats_weighted(self.h, c, observable, first, confidence, method,
+                                        n_resamples, C.byref(s)))
         return dict(mean=s.mean, calculated_precision=s.calculated_precision,
                     variance=s.variance, autocorr_factor=s.autocorr_factor, n=s.n)
 

@@ -1649,7 +1649,8 @@ static ObsParams obs_params(mcg_handle *h, int64_t chain, int64_t observable) {
   op.negJ1 = -h->J1;
   op.negJ2 = -h->J2;
   op.mu = h->canonical() ? 0.0 : h->chains[chain].exchange_potential;
-  op.fold = (h->sequential() && !h->use_nlist) ? 1 : 0;
+  // only EXACT and CANONICAL records carry the row/column fold
+  op.fold = ((h->mode == MCG_MODE_EXACT || h->canonical()) && !h->use_nlist) ? 1 : 0;
   op.observable = (int32_t)observable;
   op.fcc = h->fcc() ? 1 : 0;
   op.pad = 0;
@@ -1662,13 +1663,60 @@ static ObsParams obs_params(mcg_handle *h, int64_t chain, int64_t observable) {
   return op;
 }
 
+// f_confidence = sqrt(2) * approx_erf_inv(confidence) (BasicStatistics.cc:127,
+// misc/math.hh:62-72; host libm)
+static double confidence_factor(double confidence) {
+  const double one = 1.0, PI = 3.141592653589793238463, a = 0.147;
+  const double x = confidence;
+  const double sgn = (x < 0.0) ? -one : one;
+  const double b = std::log((one - x) * (one + x));
+  const double cc = 2.0 / (PI * a) + b * 0.5;
+  const double d = b / a;
+  return std::sqrt(2.0) * (sgn * std::sqrt(std::sqrt(cc * cc - d) - cc));
+}
+
+int mcg_stats_weighted(mcg_handle *h, int64_t chain, int64_t observable, int64_t first,
+                       double confidence, int64_t method, int64_t n_resamples, mcg_stats *out) {
+  if (!valid_chain(h, chain) || !out) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (observable < 0 || observable > 2) return fail(MCG_ERR_ARG, "bad observable");
+  if (!h->nfold())
+    return fail(MCG_ERR_STATE, "weighted statistics need weighted records (NFOLD mode)");
+  if (method != 1 && method != 2)
+    return fail(MCG_ERR_ARG, "Error in BasicStatisticsCalculator: invalid method");
+  if (n_resamples < 1 || n_resamples > (1LL << 26))
+    return fail(MCG_ERR_ARG, "n_resamples must be in [1, 2^26]");
+  const int64_t count = h->n_samples[chain] - first;
+  if (first < 0 || count <= 0)
+    return fail(MCG_ERR_ARG, "Error in BasicStatisticsCalculator: observations.size()==0");
+  if (n_resamples + 8 > h->stats_cap) {
+    if (h->d_stats) hipFree(h->d_stats);
+    h->d_stats = nullptr;
+    h->stats_cap = std::max<int64_t>(n_resamples + 8, 2 * h->stats_cap);
+    HIP_TRY(hipMalloc(&h->d_stats, sizeof(double) * h->stats_cap));
+  }
+  launch_weighted_stats(h->d_series + chain * h->series_cap, first, count,
+                        obs_params(h, chain, observable), (int)method, n_resamples, h->d_stats,
+                        h->stream);
+  HIP_TRY(hipGetLastError());
+  double r[8];
+  HIP_TRY(hipMemcpyAsync(r, h->d_stats, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  out->mean = r[0];
+  out->variance = r[1];
+  out->autocorr_factor = r[2];
+  out->n = count;
+  // BasicStatistics.cc:176-178 (method 1: / W) and :127-128 via :183 (method 2: / n)
+  out->calculated_precision = confidence_factor(confidence) * std::sqrt(r[2] * r[1] / r[4]);
+  return MCG_OK;
+}
+
 int mcg_stats_compute(mcg_handle *h, int64_t chain, int64_t observable, int64_t first,
                       double confidence, mcg_stats *out) {
   if (!valid_chain(h, chain) || !out) return fail(MCG_ERR_ARG, "bad chain or buffer");
   if (observable < 0 || observable > 2) return fail(MCG_ERR_ARG, "bad observable");
-  if (h->nfold())
-    return fail(MCG_ERR_STATE,
-                "NFOLD samples carry weights: compute statistics from the records' weights");
+  // weighted records: the reference calculator's defaults (BasicStatistics.hh:
+  // method 1, 10000 resamples)
+  if (h->nfold()) return mcg_stats_weighted(h, chain, observable, first, confidence, 1, 10000, out);
   const int64_t count = h->n_samples[chain] - first;
   if (first < 0 || count <= 0)
     return fail(MCG_ERR_ARG, "Error in BasicStatisticsCalculator: observations.size()==0");
@@ -1685,14 +1733,7 @@ int mcg_stats_compute(mcg_handle *h, int64_t chain, int64_t observable, int64_t 
   HIP_TRY(hipMemcpyAsync(r, h->d_stats, sizeof(r), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   // BasicStatistics.cc:127-128 (host libm for the confidence factor)
-  const double one = 1.0, PI = 3.141592653589793238463, a = 0.147;
-  const double x = confidence;
-  const double sgn = (x < 0.0) ? -one : one;
-  const double b = std::log((one - x) * (one + x));
-  const double cc = 2.0 / (PI * a) + b * 0.5;
-  const double d = b / a;
-  const double erfinv = sgn * std::sqrt(std::sqrt(cc * cc - d) - cc);
-  const double f_conf = std::sqrt(2.0) * erfinv;
+  const double f_conf = confidence_factor(confidence);
   out->mean = r[0];
   out->variance = r[1];
   out->autocorr_factor = r[2];
@@ -1706,13 +1747,12 @@ int mcg_equilibration(mcg_handle *h, int64_t chain, int64_t observable, double p
   if (!valid_chain(h, chain) || !is_equilibrated || !n_samples_for_equilibration)
     return fail(MCG_ERR_ARG, "bad chain or buffer");
   if (observable < 0 || observable > 2) return fail(MCG_ERR_ARG, "bad observable");
-  if (h->nfold())
-    return fail(MCG_ERR_STATE,
-                "NFOLD samples carry weights: compute statistics from the records' weights");
   const int64_t count = h->n_samples[chain];
   if (count <= 0) return fail(MCG_ERR_ARG, "Error in equilibration_check: observations.size()==0");
+  // NFOLD records are weighted: EquilibrationCheck.cc:134-160
   launch_equilibration(h->d_series + chain * h->series_cap, count,
-                       obs_params(h, chain, observable), prec, h->d_eq, h->stream);
+                       obs_params(h, chain, observable), prec, h->nfold() ? 1 : 0, h->d_eq,
+                       h->stream);
   HIP_TRY(hipGetLastError());
   int64_t r[2];
   HIP_TRY(hipMemcpyAsync(r, h->d_eq, sizeof(r), hipMemcpyDeviceToHost, h->stream));

@@ -216,7 +216,9 @@ struct ObsParams {
 void launch_stats(const mcg_sample *series, int64_t first, int64_t count, ObsParams op,
                   double *d_out, hipStream_t stream);
 
+void launch_weighted_stats(const mcg_sample *series, int64_t first, int64_t count, ObsParams op,
+                           int method, int64_t n_resamples, double *d_out, hipStream_t stream);
 void launch_equilibration(const mcg_sample *series, int64_t count, ObsParams op, double prec,
-                          int64_t *d_out, hipStream_t stream);
+                          int weighted, int64_t *d_out, hipStream_t stream);
 
 }  // namespace mcg

@@ -1,9 +1,10 @@
 // stats.hip -- BasicStatistics and the equilibration check over a chain's
 // device-resident sample series.
 //
-// Reference: BasicStatisticsCalculator::operator() (src/casm/monte/BasicStatistics.cc:114-131),
-// autocorrelation_factor (:24-48), variance (include/casm/monte/misc/math.hh:31-39),
-// _default_equilibration_check (src/casm/monte/checks/EquilibrationCheck.cc:50-117).
+// Reference: BasicStatisticsCalculator::operator() (src/casm/monte/BasicStatistics.cc:114-131,
+// weighted :144-188), autocorrelation_factor (:24-48), resample (:50-73), variance and
+// weighted_variance (include/casm/monte/misc/math.hh:31-54), _default_equilibration_check
+// (src/casm/monte/checks/EquilibrationCheck.cc:50-117) and its weighted form (:119-162).
 //
 // The observable of each record is formed on the fly with the reference's FP64
 // rounding (the sampling functions of basic_semigrand_canonical.hh:485-590).
@@ -63,6 +64,46 @@ __device__ double block_sum(double v, double *red) {
   return out;
 }
 
+// autocorrelation_factor (BasicStatistics.cc:24-48) of a centred series
+// d[0..count) with variance var0 and mean `mean`; `increment` is the interval
+// between resampled observations (1 for unweighted ones).  The whole
+// workgroup takes part; the lag found is returned in *lag_out (0 if none).
+__device__ double lag_factor(const double *__restrict__ d, int64_t count, double var0,
+                             double mean, double increment, int *best, double *lag_out) {
+  const int t = threadIdx.x;
+  double lag = 0.0;
+  double factor;
+  if (fabs(var0 / mean) < 1e-8 || var0 == 0.0) {
+    factor = 1.0;
+  } else {
+    factor = 1.7976931348623157e308;  // DBL_MAX: no lag found
+    for (int64_t base = 1; base < count; base += 256) {
+      if (t == 0) *best = 0x7fffffff;
+      __syncthreads();
+      const int64_t i = base + t;
+      if (i < count) {
+        const int64_t range = count - i;
+        double cov = 0.0;
+        for (int64_t j = 0; j < range; ++j) cov = __dadd_rn(cov, __dmul_rn(d[j], d[j + i]));
+        cov = __ddiv_rn(cov, (double)range);
+        if (fabs(cov / var0) <= 0.5) atomicMin(best, t);
+      }
+      __syncthreads();
+      if (*best != 0x7fffffff) {
+        lag = (double)(base + *best);
+        break;
+      }
+      __syncthreads();
+    }
+    if (lag > 0.0) {
+      const double rho = pow(2.0, -1.0 / (lag * increment));
+      factor = (1.0 + rho) / (1.0 - rho);
+    }
+  }
+  *lag_out = lag;
+  return factor;
+}
+
 // out[0] = mean, out[1] = var0, out[2] = autocorrelation factor, out[3] = lag
 // d_centered: scratch of `count` doubles
 __global__ __launch_bounds__(256) void stats_kernel(const mcg_sample *__restrict__ series,
@@ -83,36 +124,8 @@ __global__ __launch_bounds__(256) void stats_kernel(const mcg_sample *__restrict
   }
   const double var0 = __ddiv_rn(block_sum<256>(v, red), (double)count);
   __syncthreads();
-  double factor;
-  double lag = 0.0;
-  if (fabs(var0 / mean) < 1e-8 || var0 == 0.0) {
-    factor = 1.0;
-  } else {
-    factor = 1.7976931348623157e308;  // DBL_MAX: no lag found
-    for (int64_t base = 1; base < count; base += 256) {
-      if (t == 0) best = 0x7fffffff;
-      __syncthreads();
-      const int64_t i = base + t;
-      if (i < count) {
-        const int64_t range = count - i;
-        double cov = 0.0;
-        for (int64_t j = 0; j < range; ++j)
-          cov = __dadd_rn(cov, __dmul_rn(d_centered[j], d_centered[j + i]));
-        cov = __ddiv_rn(cov, (double)range);
-        if (fabs(cov / var0) <= 0.5) atomicMin(&best, t);
-      }
-      __syncthreads();
-      if (best != 0x7fffffff) {
-        lag = (double)(base + best);
-        break;
-      }
-      __syncthreads();
-    }
-    if (lag > 0.0) {
-      const double rho = pow(2.0, -1.0 / lag);
-      factor = (1.0 + rho) / (1.0 - rho);
-    }
-  }
+  double lag;
+  const double factor = lag_factor(d_centered, count, var0, mean, 1.0, &best, &lag);
   if (t == 0) {
     out[0] = mean;
     out[1] = var0;
@@ -121,12 +134,98 @@ __global__ __launch_bounds__(256) void stats_kernel(const mcg_sample *__restrict
   }
 }
 
-// Sequential scan of _default_equilibration_check by one lane.
+// Weighted observations (the records' weights), BasicStatistics.cc:144-188:
+// the series read at n equally spaced points of its cumulative weight
+// (resample, :50-73, one lane in order), then method 1 (weighted mean and
+// variance, misc/math.hh:45-54; autocorrelation of the resampled series with
+// increment W/n) or method 2 (unweighted statistics of the resampled series).
+// out[0] = mean, out[1] = variance, out[2] = autocorrelation factor,
+// out[3] = lag, out[4] = W (the sum the precision divides by: W or n);
+// d_eq: scratch of n doubles
+__global__ __launch_bounds__(256) void weighted_stats_kernel(
+    const mcg_sample *__restrict__ series, int64_t first, int64_t count, ObsParams op,
+    int method, int64_t n, double *__restrict__ d_eq, double *__restrict__ out) {
+  __shared__ double red[256];
+  __shared__ int best;
+  const int t = threadIdx.x;
+  const mcg_sample *S = series + first;
+  double sw = 0.0, sxw = 0.0;
+  for (int64_t j = t; j < count; j += 256) {
+    const double w = S[j].weight;
+    sw = __dadd_rn(sw, w);
+    sxw = __dadd_rn(sxw, __dmul_rn(observable_of(S[j], op), w));
+  }
+  const double W = block_sum<256>(sw, red);
+  const double XW = block_sum<256>(sxw, red);
+  const double increment = __ddiv_rn(W, (double)n);
+  if (t == 0) {
+    int64_t j = 0;
+    double Wj = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      const double target = __dmul_rn((double)i, increment);
+      while (j + 1 < count && __dadd_rn(Wj, S[j].weight) < target) {
+        Wj = __dadd_rn(Wj, S[j].weight);
+        ++j;
+      }
+      d_eq[i] = observable_of(S[j], op);
+    }
+  }
+  __syncthreads();
+  double se = 0.0;
+  for (int64_t i = t; i < n; i += 256) se = __dadd_rn(se, d_eq[i]);
+  const double eq_mean = __ddiv_rn(block_sum<256>(se, red), (double)n);
+  double ve = 0.0;
+  for (int64_t i = t; i < n; i += 256) {
+    const double d = __dsub_rn(d_eq[i], eq_mean);
+    d_eq[i] = d;
+    ve = __dadd_rn(ve, __dmul_rn(d, d));
+  }
+  const double eq_var = __ddiv_rn(block_sum<256>(ve, red), (double)n);
+  __syncthreads();
+  double mean, var, denom, lag, factor;
+  if (method == 1) {
+    mean = __ddiv_rn(XW, W);
+    double sv = 0.0;
+    for (int64_t j = t; j < count; j += 256) {
+      const double d = __dsub_rn(observable_of(S[j], op), mean);
+      sv = __dadd_rn(sv, __dmul_rn(__dmul_rn(S[j].weight, d), d));  // w(i) * d * d
+    }
+    var = __ddiv_rn(block_sum<256>(sv, red), W);
+    denom = W;
+    factor = lag_factor(d_eq, n, eq_var, eq_mean, increment, &best, &lag);
+  } else {
+    mean = eq_mean;
+    var = eq_var;
+    denom = (double)n;
+    factor = lag_factor(d_eq, n, eq_var, eq_mean, 1.0, &best, &lag);
+  }
+  if (t == 0) {
+    out[0] = mean;
+    out[1] = var;
+    out[2] = factor;
+    out[3] = lag;
+    out[4] = denom;
+  }
+}
+
+// Sequential scan of _default_equilibration_check by one lane.  With weights
+// (default_equilibration_check, EquilibrationCheck.cc:134-160) each observation
+// is scaled by (N/W) * w_i first, W the sum of the weights in order.
 // out[0] = is_equilibrated, out[1] = N_samples_for_equilibration
 __global__ void equilibration_kernel(const mcg_sample *__restrict__ series, int64_t count,
-                                     ObsParams op, double prec, int64_t *__restrict__ out) {
+                                     ObsParams op, double prec, int weighted,
+                                     int64_t *__restrict__ out) {
   if (threadIdx.x != 0) return;
-  auto obs = [&](int64_t i) { return observable_of(series[i], op); };
+  double wf = 0.0;
+  if (weighted) {
+    double W = 0.0;
+    for (int64_t i = 0; i < count; ++i) W = __dadd_rn(W, series[i].weight);
+    wf = __ddiv_rn((double)count, W);
+  }
+  auto obs = [&](int64_t i) {
+    const double x = observable_of(series[i], op);
+    return weighted ? __dmul_rn(x, __dmul_rn(wf, series[i].weight)) : x;
+  };
   const int64_t N = count;
   const double x0 = obs(0);
   const double eps = (x0 == 0.0) ? 1e-8 : fabs(x0) * 1e-8;
@@ -177,10 +276,17 @@ void launch_stats(const mcg_sample *series, int64_t first, int64_t count, ObsPar
                      d_out + 4, d_out);
 }
 
+void launch_weighted_stats(const mcg_sample *series, int64_t first, int64_t count, ObsParams op,
+                           int method, int64_t n_resamples, double *d_out, hipStream_t stream) {
+  // d_out: 8 result doubles followed by n_resamples doubles of scratch
+  hipLaunchKernelGGL(weighted_stats_kernel, dim3(1), dim3(256), 0, stream, series, first, count,
+                     op, method, n_resamples, d_out + 8, d_out);
+}
+
 void launch_equilibration(const mcg_sample *series, int64_t count, ObsParams op, double prec,
-                          int64_t *d_out, hipStream_t stream) {
+                          int weighted, int64_t *d_out, hipStream_t stream) {
   hipLaunchKernelGGL(equilibration_kernel, dim3(1), dim3(64), 0, stream, series, count, op,
-                     prec, d_out);
+                     prec, weighted, d_out);
 }
 
 }  // namespace mcg

@@ -276,12 +276,30 @@ typedef struct mcg_stats {
   int64_t n;              /* number of observations used                        */
 } mcg_stats;
 
+/* NFOLD records are weighted: there mcg_stats_compute is mcg_stats_weighted
+ * with the calculator's defaults (method 1, 10000 resamples). */
 int mcg_stats_compute(mcg_handle *h, int64_t chain, int64_t observable, int64_t first,
                       double confidence, mcg_stats *out);
 
+/* Statistics of weighted records (NFOLD mode; weight = residence time by pass,
+ * number of grid points on the time grid), as BasicStatisticsCalculator's
+ * weighted branch (BasicStatistics.cc:144-188): the series is read at
+ * n_resamples equally spaced points of its cumulative weight (resample,
+ * :50-73); method 1 gives the weighted mean and variance
+ * (weighted_variance, misc/math.hh:45-54) with the autocorrelation factor of
+ * the resampled series (increment W/n_resamples) and precision
+ * f * sqrt(factor * variance / W); method 2 gives the plain statistics of the
+ * resampled series.  Sums are reduced in another order than the reference's
+ * (results agree to rounding).  Errors: MCG_ERR_STATE outside NFOLD mode,
+ * MCG_ERR_ARG for another method or n_resamples outside [1, 2^26]. */
+int mcg_stats_weighted(mcg_handle *h, int64_t chain, int64_t observable, int64_t first,
+                       double confidence, int64_t method, int64_t n_resamples, mcg_stats *out);
+
 /* Equilibration check of one observable over a chain's whole series,
  * as _default_equilibration_check (src/casm/monte/checks/EquilibrationCheck.cc:50-117)
- * with absolute precision prec; the partition scan runs on the device. */
+ * with absolute precision prec; the partition scan runs on the device.  NFOLD
+ * records are weighted: each observation is scaled by (N/W) w_i first, W the
+ * in-order sum of the weights (EquilibrationCheck.cc:134-160). */
 int mcg_equilibration(mcg_handle *h, int64_t chain, int64_t observable, double prec,
                       int64_t *is_equilibrated, int64_t *n_samples_for_equilibration);
 

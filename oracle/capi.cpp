@@ -190,6 +190,30 @@ void orc_basic_statistics(const double *x, int64_t n, double confidence, double 
   *prec = s.calculated_precision;
 }
 
+// 0 on success, 1 on an error (empty or mismatched input, bad method)
+int orc_weighted_statistics(const double *x, const double *w, int64_t n, double confidence,
+                            int64_t method, int64_t n_resamples, double *out4) {
+  try {
+    WeightedStatistics s = weighted_statistics(std::vector<double>(x, x + n),
+                                               std::vector<double>(w, w + n), confidence,
+                                               (int)method, n_resamples);
+    out4[0] = s.mean;
+    out4[1] = s.calculated_precision;
+    out4[2] = s.variance;
+    out4[3] = s.autocorr;
+    return 0;
+  } catch (std::exception const &) {
+    return 1;
+  }
+}
+
+void orc_resample(const double *x, const double *w, int64_t n, double w_sum, int64_t m,
+                  double *out) {
+  std::vector<double> r = resample(std::vector<double>(x, x + n), std::vector<double>(w, w + n),
+                                   w_sum, m);
+  std::copy(r.begin(), r.end(), out);
+}
+
 double orc_autocorrelation_factor(const double *x, int64_t n) {
   return autocorrelation_factor(std::vector<double>(x, x + n));
 }

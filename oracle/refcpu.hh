@@ -289,7 +289,9 @@ inline double approx_erf_inv(double x) {  // math.hh:62-72 (Winitzki, a = 0.147)
   return sgn * std::sqrt(std::sqrt(c * c - d) - c);
 }
 
-inline double autocorrelation_factor(std::vector<double> const &x) {  // BasicStatistics.cc:24-48
+// BasicStatistics.cc:24-48; `increment` is the interval between resampled
+// observations (1.0 for unweighted ones)
+inline double autocorrelation_factor(std::vector<double> const &x, double increment = 1.0) {
   Index N = x.size();
   double m = mean_of(x);
   double v0 = variance(x, m);
@@ -300,11 +302,41 @@ inline double autocorrelation_factor(std::vector<double> const &x) {  // BasicSt
     for (Index j = 0; j < range; ++j) cov += (x[j] - m) * (x[j + i] - m);
     cov /= range;
     if (std::abs(cov / v0) <= 0.5) {
-      double rho = std::pow(2.0, (-1.0 / (i * 1.0)));
+      double rho = std::pow(2.0, (-1.0 / (i * increment)));
       return (1.0 + rho) / (1.0 - rho);
     }
   }
   return std::numeric_limits<double>::max();
+}
+
+// BasicStatistics.cc:50-73: the weighted series as a time series read at
+// n equally spaced times i * W / n
+inline std::vector<double> resample(std::vector<double> const &x, std::vector<double> const &w,
+                                    double w_sum, Index n) {
+  const double increment = w_sum / n;
+  std::vector<double> out(n);
+  Index j = 0;
+  double W_j = 0.0;
+  for (Index i = 0; i < n; ++i) {
+    const double target = i * increment;
+    while (W_j + w[j] < target) {
+      W_j += w[j];
+      ++j;
+    }
+    out[i] = x[j];
+  }
+  return out;
+}
+
+// misc/math.hh:45-54
+inline double weighted_variance(std::vector<double> const &x, double m,
+                                std::vector<double> const &w, double w_sum) {
+  double cov = 0.0;
+  for (size_t i = 0; i < x.size(); ++i) {
+    const double d = x[i] - m;
+    cov += w[i] * d * d;
+  }
+  return cov / w_sum;
 }
 
 struct BasicStatistics {  // BasicStatistics.hh:24-36
@@ -321,6 +353,42 @@ inline BasicStatistics basic_statistics(std::vector<double> const &x,
   double f = autocorrelation_factor(x);
   double fc = std::sqrt(2.0) * approx_erf_inv(confidence);
   s.calculated_precision = fc * std::sqrt(f * v0 / x.size());
+  return s;
+}
+
+// BasicStatistics.cc:144-188 (weighted observations; method 1: weighted mean
+// and variance, autocorrelation from the resampled series; method 2: all
+// statistics from the resampled series).  Eigen's sum and dot are restated as
+// sequential sums (their reduction order is Eigen's own).
+struct WeightedStatistics {
+  double mean = 0.0, calculated_precision = 0.0, variance = 0.0, autocorr = 0.0;
+};
+inline WeightedStatistics weighted_statistics(std::vector<double> const &x,
+                                              std::vector<double> const &w, double confidence,
+                                              int method, Index n_resamples) {
+  if (x.empty()) throw std::runtime_error("observations.size()==0");
+  if (x.size() != w.size()) throw std::runtime_error("observations.size() != sample_weight.size()");
+  double W = 0.0;
+  for (double v : w) W += v;
+  const double increment = W / n_resamples;
+  std::vector<double> eq = resample(x, w, W, n_resamples);
+  const double fc = std::sqrt(2.0) * approx_erf_inv(confidence);
+  WeightedStatistics s;
+  if (method == 1) {
+    double xw = 0.0;
+    for (size_t i = 0; i < x.size(); ++i) xw += x[i] * w[i];
+    s.mean = xw / W;
+    s.variance = weighted_variance(x, s.mean, w, W);
+    s.autocorr = autocorrelation_factor(eq, increment);
+    s.calculated_precision = fc * std::sqrt(s.autocorr * s.variance / W);
+  } else if (method == 2) {
+    s.mean = mean_of(eq);
+    s.variance = variance(eq, s.mean);
+    s.autocorr = autocorrelation_factor(eq);
+    s.calculated_precision = fc * std::sqrt(s.autocorr * s.variance / eq.size());
+  } else {
+    throw std::runtime_error("invalid method");
+  }
   return s;
 }
 

@@ -79,6 +79,13 @@ def _declare(L):
     L.orc_kb.restype = C.c_double
     L.orc_basic_statistics.argtypes = [C.POINTER(C.c_double), C.c_int64, C.c_double,
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.orc_weighted_statistics.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.c_int64, C.c_double, C.c_int64, C.c_int64,
+                                          C.POINTER(C.c_double)]
+    L.orc_weighted_statistics.restype = C.c_int
+    L.orc_resample.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64,
+                               C.c_double, C.c_int64, C.POINTER(C.c_double)]
+    L.orc_resample.restype = None
     L.orc_autocorrelation_factor.argtypes = [C.POINTER(C.c_double), C.c_int64]
     L.orc_autocorrelation_factor.restype = C.c_double
     L.orc_equilibration.argtypes = [C.POINTER(C.c_double), C.c_int64, C.c_double,
@@ -160,6 +167,29 @@ def basic_statistics(x, confidence=0.95):
     m, p = C.c_double(), C.c_double()
     lib().orc_basic_statistics(_p(x, C.c_double), len(x), confidence, C.byref(m), C.byref(p))
     return m.value, p.value
+
+
+def weighted_statistics(x, w, confidence=0.95, method=1, n_resamples=10000):
+    """BasicStatisticsCalculator on weighted observations (BasicStatistics.cc:144-188):
+    dict of mean, calculated_precision, variance, autocorr_factor."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    if len(x) != len(w):
+        raise ValueError("observations.size() != sample_weight.size()")
+    out = np.zeros(4)
+    if lib().orc_weighted_statistics(_p(x, C.c_double), _p(w, C.c_double), len(x), confidence,
+                                     method, n_resamples, _p(out, C.c_double)):
+        raise ValueError("weighted statistics: empty or mismatched input, or bad method")
+    return dict(mean=out[0], calculated_precision=out[1], variance=out[2], autocorr_factor=out[3])
+
+
+def resample(x, w, w_sum, m):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    out = np.zeros(m)
+    lib().orc_resample(_p(x, C.c_double), _p(w, C.c_double), len(x), w_sum, m,
+                       _p(out, C.c_double))
+    return out
 
 
 def autocorrelation_factor(x):

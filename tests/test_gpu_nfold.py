@@ -137,9 +137,16 @@ def test_nfold_batch():
 def test_nfold_errors():
     r = _run_gpu([dict(shape=(6, 6), temperature=1500.0, seed=1)],
                  [np.ones(36, dtype=np.int8)], 2)
-    with pytest.raises(cm.McgError, match="weights"):
-        r.stats(0, cm.OBS_POTENTIAL_ENERGY)
+    with pytest.raises(cm.McgError, match="invalid method"):
+        r.stats_weighted(0, cm.OBS_POTENTIAL_ENERGY, method=3)
+    with pytest.raises(cm.McgError, match="n_resamples"):
+        r.stats_weighted(0, cm.OBS_POTENTIAL_ENERGY, n_resamples=0)
     r.close()
+    e = cm.Run([dict(shape=(6, 6), temperature=1500.0, seed=1)], mode=cm.MODE_EXACT, J1=J)
+    e.run(2)
+    with pytest.raises(cm.McgError, match="weighted records"):
+        e.stats_weighted(0, cm.OBS_POTENTIAL_ENERGY)
+    e.close()
     r = cm.Run([dict(shape=(6, 6), temperature=5000.0, seed=1)], mode=cm.MODE_NFOLD, J1=J,
                sample_time_period=1e-6)
     r.reserve_samples(10)
