@@ -19,8 +19,8 @@
 #include "internal.hh"
 
 namespace mcg {
-void exact_set_max_lds(int bytes);
-void checker_set_max_lds(int bytes);
+hipError_t exact_set_max_lds(int bytes);
+hipError_t checker_set_max_lds(int bytes);
 hipError_t fcc_set_max_lds(int bytes);
 int checker_units_per_item();
 }  // namespace mcg
@@ -318,15 +318,16 @@ struct mcg_handle {
   bool fcc() const { return lattice == MCG_LATTICE_FCC; }
 
   ~mcg_handle() {
-    if (stream) hipStreamSynchronize(stream);
-    if (graph_exec) hipGraphExecDestroy(graph_exec);
+    // teardown: errors here have nowhere to go
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     void *ptrs[] = {d_lat, d_active, d_series, d_stats, d_eq, d_xp, d_xs, d_cp, d_cs, d_tab,
                     d_work, d_ftab, d_tri, d_band, d_pair, d_kp, d_list, d_np};
     for (void *p : ptrs)
-      if (p) hipFree(p);
-    if (ev0) hipEventDestroy(ev0);
-    if (ev1) hipEventDestroy(ev1);
-    if (stream) hipStreamDestroy(stream);
+      if (p) (void)hipFree(p);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 
   int64_t sites(int64_t c) const {
@@ -430,7 +431,7 @@ int build_exact(mcg_handle *h) {
     h->exact_lds = scratch;
     if (scratch > EXACT_LDS_MAX) return fail(MCG_ERR_ARG, "lattice too large for EXACT mode");
   }
-  exact_set_max_lds((int)LDS_LIMIT - 4096);
+  HIP_TRY(exact_set_max_lds((int)LDS_LIMIT - 4096));
   HIP_TRY(hipMalloc(&h->d_xp, sizeof(ExactParams) * h->n_chains));
   HIP_TRY(hipMalloc(&h->d_xs, sizeof(ExactState) * h->n_chains));
   HIP_TRY(copy_sync(h, h->d_xp, h->xp.data(), sizeof(ExactParams) * h->n_chains,
@@ -691,7 +692,7 @@ int build_checker(mcg_handle *h) {
   }
   h->checker_lds = max_n <= CHECKER_LDS_MAX && !(h->options & MCG_OPT_FORCE_STREAMING);
   h->checker_lds_bytes = (int)((max_n + 15) & ~15);
-  checker_set_max_lds((int)LDS_LIMIT - 4096);
+  HIP_TRY(checker_set_max_lds((int)LDS_LIMIT - 4096));
   HIP_TRY(hipMalloc(&h->d_cp, sizeof(CheckerParams) * h->n_chains));
   HIP_TRY(hipMalloc(&h->d_cs, sizeof(CheckerState) * h->n_chains));
   HIP_TRY(hipMalloc(&h->d_tab, sizeof(uint32_t) * tabs.size()));

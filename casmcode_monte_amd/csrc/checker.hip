@@ -489,9 +489,9 @@ void launch_checker_lds(const CheckerParams *P, CheckerState *St, int8_t *lat_po
                      lat_pool, tables, series, series_cap, active, n_passes, sample_period);
 }
 
-void checker_set_max_lds(int bytes) {
-  hipFuncSetAttribute(reinterpret_cast<const void *>(checker_lds_kernel),
-                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+hipError_t checker_set_max_lds(int bytes) {
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(checker_lds_kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
 // ---------------------------------------------------------------------------

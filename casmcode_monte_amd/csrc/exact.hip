@@ -223,7 +223,7 @@ __global__ __launch_bounds__(64) void exact_kernel(const ExactParams *__restrict
 // order.  A lane 0 whose Lemire product needs the rejection test takes the
 // one-attempt scalar path.
 // ---------------------------------------------------------------------------
-constexpr uint32_t SPIN_BIT = 1u, MARK_A = 2u, MARK_B = 4u;
+constexpr uint32_t SPIN_BIT = 1u, MARK_A = 2u;
 
 __device__ __forceinline__ int spin_of(uint32_t b) { return (b & SPIN_BIT) ? 1 : -1; }
 
@@ -524,13 +524,17 @@ void launch_exact(const ExactParams *P, ExactState *S, int8_t *lat_pool, mcg_sam
   }
 }
 
-void exact_set_max_lds(int bytes) {
-  hipFuncSetAttribute(reinterpret_cast<const void *>(exact_block_kernel),
-                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes - 12 * 1024);
-  hipFuncSetAttribute(reinterpret_cast<const void *>(exact_kernel<true>),
-                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  hipFuncSetAttribute(reinterpret_cast<const void *>(exact_kernel<false>),
-                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+// The block walk's static LDS (three raw and three tempered engine
+// generations, ~15 KB) leaves it 12 KB less dynamic LDS than the scalar walk.
+hipError_t exact_set_max_lds(int bytes) {
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(exact_block_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes - 12 * 1024);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(reinterpret_cast<const void *>(exact_kernel<true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(exact_kernel<false>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
 }  // namespace mcg

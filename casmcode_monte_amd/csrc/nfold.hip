@@ -123,7 +123,6 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
   for (int k = 0; k < 10; ++k)
     if (k < lane) startv += hist[k];
   const double ratev = lane < 10 ? P->rate[lane] : 0.0;
-  auto start_of = [&](int k) -> int { return __builtin_amdgcn_readlane(startv, k); };
 
   const int64_t pos0 = St->pos;
   int64_t pos_mt = pos0;
