@@ -120,3 +120,18 @@ def test_canonical_errors():
         r.set_occ_location(0, np.zeros(16, dtype=np.int64))
     r.run(2)
     r.close()
+
+
+def test_canonical_many_short_launches():
+    """One-pass launches (each starts and ends the engine hand-off between the
+    walking and the twisting wavefront) equal one uninterrupted run."""
+    rows, cols, n_up, T, seed, passes = 6, 6, 14, 1800.0, 21, 50
+    occ = _spins(np.random.default_rng(seed), rows, cols, n_up)
+    o = O.canonical_run(rows, cols, J, T, passes, occ, O.mt64_state(seed, 0))
+    r = cm.Run([dict(shape=(rows, cols), temperature=T, seed=seed)], mode=cm.MODE_CANONICAL,
+               J1=J)
+    r.set_occupation(0, occ)
+    for _ in range(passes):
+        r.run(1)
+    _compare(r, 0, o, rows, cols)
+    r.close()

@@ -148,3 +148,19 @@ def test_large_lattice_vs_oracle():
     o = O.sgc_run(256, 256, temperature=2600.0, mu=0.02, seed=77, max_count=12)
     r = run_gpu(256, 256, 2600.0, 0.02, 77, 12)
     compare(r, o)
+
+
+@pytest.mark.parametrize("shape,T,seed", [((4, 4), 3000.0, 17), ((2, 6), 800.0, 18),
+                                          ((10, 10), 1500.0, 19)])
+def test_many_short_launches(shape, T, seed):
+    """Many one- and two-pass launches: each launch starts and ends the engine
+    hand-off between the walking and the twisting wavefront, at every offset
+    into a generation; the trajectory must equal one uninterrupted run."""
+    rows, cols = shape
+    passes = 90
+    print("case: short launches", shape, T, seed, flush=True)
+    o = O.sgc_run(rows, cols, J=0.1, use_nlist=True, temperature=T, mu=0.05, seed=seed,
+                  max_count=passes)
+    r = run_gpu(rows, cols, T, 0.05, seed, passes, chunks=[1, 2] * 30)
+    compare(r, o)
+    r.close()
