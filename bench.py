@@ -274,10 +274,47 @@ def cpu_baseline(n_chains, passes):
         list(ex.map(one, conds))
     wall = time.perf_counter() - t0
     attempts = n_chains * 128 * 128 * passes
-    return dict(value=attempts / wall, unit="site-attempts/s", cores=threads, kind="port",
-                sample=f"{n_chains} of the 64 chains x {passes} passes of 128x128 "
-                       f"(reference loop with per-step completion check and per-pass "
-                       f"sampling), {threads} threads")
+    out = dict(value=attempts / wall, unit="site-attempts/s", cores=threads, kind="port",
+               sample=f"{n_chains} of the 64 chains x {passes} passes of 128x128 "
+                      f"(reference loop with per-step completion check and per-pass "
+                      f"sampling), {threads} threads")
+    out["configs"] = cpu_configs(O)
+    return out
+
+
+def cpu_configs(O):
+    """The reference's CPU path (the oracle's restatement: sequential random-site
+    Metropolis) on bounded samples of the other configurations, one core each."""
+    import numpy as np
+    from casmcode_monte_amd import fcc
+    res = {}
+
+    def timed(name, attempts, f, sample):
+        t0 = time.perf_counter()
+        f()
+        w = time.perf_counter() - t0
+        res[name] = dict(attempts_per_s=attempts / w, cores=1, sample=sample)
+
+    timed("c1_25x25_end_to_end", 11000 * 625,
+          lambda: O.sgc_run(25, 25, J=0.1, use_nlist=True, temperature=2000.0, mu=0.0, seed=42,
+                            max_count=11000),
+          "25x25, all +1, T=2000 K, seed 42, 11000 passes with per-step completion checks")
+    n2 = 4096 * 4096
+    timed("c3_4096x4096_J1J2", n2,
+          lambda: O.lattice_run(2, (4096, 4096), 0.1, -0.02, 1000.0, 0.0, 0xC3000, 1,
+                                np.ones(n2, dtype=np.int8)),
+          "one random-site pass of 4096^2 J1+J2 at 1000 K")
+    n3 = 256 ** 3
+    timed("c3_256cubed_J1J2", n3,
+          lambda: O.lattice_run(3, (256, 256, 256), 0.1, -0.02, 1000.0, 0.0, 0xC3000, 1,
+                                np.ones(n3, dtype=np.int8)),
+          "one random-site pass of 256^3 J1+J2 at 1000 K")
+    nf = 4 * 32 ** 3
+    timed("c4_fcc_32cubed", 5 * nf,
+          lambda: O.fcc_run((32, 32, 32), fcc.DEFAULT_ECI, 1000.0, 0.0, 7, 5,
+                            np.zeros(nf, dtype=np.int8)),
+          "5 random-site passes of one 32^3 FCC chain at 1000 K")
+    return res
 
 
 def main():
