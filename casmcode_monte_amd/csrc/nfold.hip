@@ -142,13 +142,13 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
       ++G;
     }
   };
-  // stream position q, which ensure() has made available
-  auto ring_at = [&](int64_t q) -> uint64_t {
+  // ring slot of stream position q, which ensure() has made available
+  auto ring_slot = [&](int64_t q) -> int {
     const int64_t base = (int64_t)MT_N * (G - 1);
     int r = (int)(q - base) + (int)(((G - 1) & 1) * MT_N);
-    if (r >= RING) r -= RING;
-    return rfl64(ring[r]);
+    return r >= RING ? r - RING : r;
   };
+  auto ring_at = [&](int64_t q) -> uint64_t { return rfl64(ring[ring_slot(q)]); };
   auto word = [&]() -> uint64_t {
     ensure(pos_mt + 1);
     const uint64_t w = ring_at(pos_mt);
@@ -160,6 +160,10 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
   // sum is formed in class order, so running sums equal the sequential ones
   // (an empty class adds +0.0).
   auto select = [&](uint32_t &site, int &site_pos, double &u3, double &R) {
+    // the usual three words go out first and arrive while the rates are summed
+    ensure(pos_mt + 3);
+    const uint64_t v0 = ring[ring_slot(pos_mt)], v1 = ring[ring_slot(pos_mt + 1)],
+                   v2 = ring[ring_slot(pos_mt + 2)];
     const int nxt = row_shl1(startv);
     const int nv = lane < 10 ? nxt - startv : 0;
     const double tv = __dmul_rn((double)nv, ratev);
@@ -170,8 +174,7 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
       pre = lane == k ? run : pre;
     }
     R = run;
-    ensure(pos_mt + 3);  // the usual three words, read together
-    const uint64_t w0 = ring_at(pos_mt), w1 = ring_at(pos_mt + 1), w2 = ring_at(pos_mt + 2);
+    const uint64_t w0 = rfl64(v0), w1 = rfl64(v1), w2 = rfl64(v2);
     pos_mt += 2;
     const double target = __dmul_rn(canonical_from_word(w0), R);
     const uint64_t hit = __ballot(lane < 10 && nv > 0 && target < pre);
@@ -265,9 +268,12 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
     if (nb_ev == 0) return;
     const double dtv = bR < 0.0 ? bu : -log1p(-bu) / bR;
     double tend = 0.0;
-    for (int j = 0; j < nb_ev; ++j) {
-      time = __dadd_rn(time, readlane_f64(dtv, j));
-      tend = lane == j ? time : tend;
+#pragma unroll
+    for (int j = 0; j < 64; ++j) {  // constant lane indices: no lane-select hazards
+      if (j < nb_ev) {
+        time = __dadd_rn(time, readlane_f64(dtv, j));
+        tend = lane == j ? time : tend;
+      }
     }
     if (by_time) {
       // first grid index k >= next_k with k * period >= t_end (the predicate
