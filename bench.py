@@ -74,6 +74,17 @@ def bench_exact(K, W):
     out["plain"] = dict(passes=kp, wall_s=pw, kernel_s=pk, attempts=pa,
                         attempts_per_s=pa / pw)
     p.close()
+    # the 64 chains use 64 of the 256 CUs; 16 independent seeds of each
+    # condition (1024 chains, four per CU) show what the chip holds
+    reps = [dict(c, seed=c["seed"] + 100000 * k) for k in range(16) for c in conds]
+    f = cm.Run(reps, mode=cm.MODE_EXACT, J1=0.1, use_nlist=True)
+    f.run(2)
+    fp = max(K // 20, 10)
+    fw, fk, fa = timed_run(f, fp)
+    out["full_chip"] = dict(chains=len(reps), passes=fp, wall_s=fw, kernel_s=fk, attempts=fa,
+                            attempts_per_s=fa / fw,
+                            note="16 seeds of each of the 64 conditions, four chains per CU")
+    f.close()
     return out
 
 
@@ -380,6 +391,7 @@ def main():
         "plain": {"attempts_per_s": ex["plain"]["attempts_per_s"],
                   "batched_over_plain": value / ex["plain"]["attempts_per_s"],
                   "passes": ex["plain"]["passes"]},
+        "exact_full_chip": ex["full_chip"],
     }
     if not args.skip_extra:
         scan = bench_checker_scan(200)
