@@ -352,7 +352,8 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; independent chains shard "
-              "trivially across GPUs (see DESIGN.md) but no multi-GPU figure is measured",
+              "trivially across GPUs (each GPU its own handle, no communication) but no "
+              "multi-GPU figure is measured",
               file=sys.stderr)
         sys.exit(2)
     import casmcode_monte_amd as cm

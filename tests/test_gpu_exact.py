@@ -96,7 +96,7 @@ def test_rng_dump_load_roundtrip():
 
 
 def c2_conditions():
-    """BASELINE batched exact scan: T = 500 + a*2500/7, mu = -0.4 + b*0.8/7, seed 1000+i."""
+    """Batched exact scan: T = 500 + a*2500/7, mu = -0.4 + b*0.8/7, seed 1000+i."""
     out = []
     for a in range(8):
         for b in range(8):
