@@ -351,7 +351,7 @@ def main():
         print(json.dumps({"copy_bytes": args.copy_bytes, "launches": 10}))
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; independent chains shard "
+        print("bench.py: the library runs on one GPU; independent chains shard "
               "trivially across GPUs (each GPU its own handle, no communication) but no "
               "multi-GPU figure is measured",
               file=sys.stderr)

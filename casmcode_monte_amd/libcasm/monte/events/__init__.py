@@ -1,6 +1,6 @@
 """libcasm.monte.events: occupant candidates, swaps, OccLocation and the event
 proposal functions (python/src/monte_events.cpp), over the one-asymmetric-unit
-Conversions of this build (Conversions(n_sites, occupants))."""
+Conversions of this library (Conversions(n_sites, occupants))."""
 from casmcode_monte_amd._libcasm_monte import events as _e
 
 Conversions = _e.Conversions

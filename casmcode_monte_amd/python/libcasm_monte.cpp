@@ -11,7 +11,7 @@
 // python/src/monte_ising_cpp_semigrand_canonical.cpp:503-576 (the Ising
 // system and the SGC calculator), python/src/monte_events.cpp (occupant
 // candidates, swaps, OccLocation and the event proposal functions, over the
-// one-asymmetric-unit Conversions of this build).  The calculator's run() drives the MI355X
+// one-asymmetric-unit Conversions of this library).  The calculator's run() drives the MI355X
 // library through the C++ drop-in (include/casm/monte/ising_cpp/
 // basic_semigrand_canonical.hh); nothing here runs a Metropolis step on the
 // host.

@@ -3,7 +3,7 @@
 // runs: one asymmetric unit whose sites all allow the same ordered occupants,
 // species index == occupant index.  The reference builds Conversions from an
 // xtal::BasicStructure prim and a supercell matrix (CASMcode_crystallography,
-// not a dependency of this build); the member names and meanings used by
+// not a dependency of this library); the member names and meanings used by
 // OccCandidateList, OccLocation and the event proposal functions are kept.
 #ifndef CASM_MI355X_monte_Conversions
 #define CASM_MI355X_monte_Conversions

@@ -10,7 +10,7 @@
 // slot, then is appended to the list of its new species.  The order of the
 // lists decides which site later draws pick, so it is kept exactly.
 //
-// Not supported by this build: atom tracking (update_atoms, unique atom ids
+// Not supported here: atom tracking (update_atoms, unique atom ids
 // and atom info; only kinetic Monte Carlo with molecules uses them).  Added
 // for moving the lists to and from the device: locations() and
 // set_locations().

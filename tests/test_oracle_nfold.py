@@ -1,7 +1,7 @@
 """Pins of the oracle's N-fold way chain (refcpu::nfold_run).
 
 The reference's nfold loop (nfold.hh:78-149) takes its event selector from
-outside the repository, so the selector is defined by this build (class-based
+outside the repository, so the selector is defined here (class-based
 rates, one partitioned site array); the pins are:
   * the partition: the site array holds every site once, grouped by class in
     class order, after runs;
