@@ -1684,8 +1684,9 @@ int mcg_stats_weighted(mcg_handle *h, int64_t chain, int64_t observable, int64_t
     return fail(MCG_ERR_STATE, "weighted statistics need weighted records (NFOLD mode)");
   if (method != 1 && method != 2)
     return fail(MCG_ERR_ARG, "Error in BasicStatisticsCalculator: invalid method");
-  if (n_resamples < 1 || n_resamples > (1LL << 26))
-    return fail(MCG_ERR_ARG, "n_resamples must be in [1, 2^26]");
+  // the lag search is quadratic in n_resamples: bound it
+  if (n_resamples < 1 || n_resamples > (1LL << 22))
+    return fail(MCG_ERR_ARG, "n_resamples must be in [1, 2^22]");
   const int64_t count = h->n_samples[chain] - first;
   if (first < 0 || count <= 0)
     return fail(MCG_ERR_ARG, "Error in BasicStatisticsCalculator: observations.size()==0");

@@ -291,7 +291,7 @@ int mcg_stats_compute(mcg_handle *h, int64_t chain, int64_t observable, int64_t 
  * f * sqrt(factor * variance / W); method 2 gives the plain statistics of the
  * resampled series.  Sums are reduced in another order than the reference's
  * (results agree to rounding).  Errors: MCG_ERR_STATE outside NFOLD mode,
- * MCG_ERR_ARG for another method or n_resamples outside [1, 2^26]. */
+ * MCG_ERR_ARG for another method or n_resamples outside [1, 2^22]. */
 int mcg_stats_weighted(mcg_handle *h, int64_t chain, int64_t observable, int64_t first,
                        double confidence, int64_t method, int64_t n_resamples, mcg_stats *out);
 
