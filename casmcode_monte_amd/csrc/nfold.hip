@@ -15,12 +15,15 @@
 // a class by swapping with the first entry of the segment left, up by
 // swapping with the last.
 //
-// Layout: workgroup = one 64-lane wavefront = one chain.  LDS holds the
-// lattice (one byte per site, 1 = spin +1), the class array and each site's
-// position in it (16-bit), and the MT19937-64 ring.  The segment starts live
-// in registers (lane k holds start[k]); the rates likewise (lane k, k < 10).
-// The walk is sequential: all lanes execute it with uniform operands; lanes
-// 0..4 recompute the five classes after a flip in parallel.
+// Layout: workgroup = one 64-lane wavefront = one chain.  LDS holds the class
+// array and each site's position in it (16-bit each; bit 15 of a position is
+// the site's spin, which its class also fixes: 4 bytes a site, so two 128x128
+// chains share a CU) and the MT19937-64 ring.  The segment starts live in
+// registers (lane k holds start[k]); the rates likewise (lane k, k < 10).
+// The walk is sequential; its parallel parts: the running rate sums and the
+// class choice (ballot over lanes 0..9), the five sites' neighbourhoods (lanes
+// 0..4, one LDS round trip), each class move as one rotation (one lane per
+// segment boundary crossed), and the time increments of 64 events at once.
 #include "internal.hh"
 #include "walk.hh"
 
