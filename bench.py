@@ -117,6 +117,18 @@ def bench_canonical(passes):
     out = dict(chains=len(conds), passes=passes, wall_s=w, kernel_s=k, attempts=a,
                attempts_per_s=a / w)
     r.close()
+    # one 128x128 chain per CU: 4 seeds of each condition fill the chip
+    reps = [dict(c, seed=c["seed"] + 100000 * k) for k in range(4) for c in conds]
+    f = cm.Run(reps, mode=cm.MODE_CANONICAL, J1=0.1)
+    for i in range(len(reps)):
+        f.set_occupation(i, occs[i % len(occs)])
+    f.run(1)
+    fp = max(passes // 4, 10)
+    fw, fk, fa = timed_run(f, fp)
+    out["full_chip"] = dict(chains=len(reps), passes=fp, wall_s=fw, kernel_s=fk, attempts=fa,
+                            attempts_per_s=fa / fw,
+                            note="4 seeds of each of the 64 conditions, one chain per CU")
+    f.close()
     p = cm.Run(conds[:1], mode=cm.MODE_CANONICAL, J1=0.1)
     p.set_occupation(0, occs[0])
     p.run(1)
