@@ -164,3 +164,18 @@ def test_many_short_launches(shape, T, seed):
     r = run_gpu(rows, cols, T, 0.05, seed, passes, chunks=[1, 2] * 30)
     compare(r, o)
     r.close()
+
+
+@pytest.mark.parametrize("shape,use_nlist", [((360, 400), True), ((380, 380), False)])
+def test_lattice_at_the_lds_limit(shape, use_nlist):
+    """Lattices that just fit the block walk's LDS (144000 and 144400 lattice
+    bytes plus the fold scratch, at most 147456 bytes, beside the walk's ~15 KB
+    of engine generations): the dynamic-LDS attribute must admit them;
+    bit-exact for 2 passes."""
+    rows, cols = shape
+    print("case: LDS limit", shape, use_nlist, flush=True)
+    o = O.sgc_run(rows, cols, use_nlist=use_nlist, temperature=2400.0, mu=0.0, seed=5,
+                  max_count=2)
+    r = run_gpu(rows, cols, 2400.0, 0.0, 5, 2, use_nlist=use_nlist)
+    compare(r, o)
+    r.close()
