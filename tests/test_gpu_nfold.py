@@ -153,3 +153,17 @@ def test_nfold_errors():
     with pytest.raises(cm.McgError, match="exceeded"):
         r.run(2)
     r.close()
+
+
+def test_nfold_at_the_site_limit():
+    """181 x 181 = 32761 sites (positions need 15 bits; bit 15 is the spin):
+    bit-exact for one pass; 182 x 181 is refused."""
+    rows, cols, T, mu, seed = 181, 181, 1700.0, 0.05, 61
+    occ = np.where(np.random.default_rng(seed).random(rows * cols) < 0.5, 1, -1).astype(np.int8)
+    o = O.nfold_run(rows, cols, J, T, mu, 1, occ, O.mt64_state(seed, 0))
+    r = _run_gpu([dict(shape=(rows, cols), temperature=T, exchange_potential=mu, seed=seed)],
+                 [occ], 1)
+    _compare(r, 0, o)
+    r.close()
+    with pytest.raises(cm.McgError, match="NFOLD at most 32768"):
+        cm.Run([dict(shape=(182, 181), temperature=T, seed=1)], mode=cm.MODE_NFOLD, J1=J)
