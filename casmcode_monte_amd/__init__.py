@@ -28,6 +28,7 @@ OBS_POTENTIAL_ENERGY = 2
 OPT_FORCE_STREAMING = 1
 OPT_EXACT_SCALAR_WALK = 2
 OPT_CHECKER_QUAD_PATH = 4
+OPT_NFOLD_SAMPLE_START = 8
 
 # Boltzmann constant, eV/K, as compiled into CASMcode_global (used by the
 # reference's beta = 1/(KB*T), basic_occupation_metropolis.hh:361).

@@ -551,6 +551,7 @@ void fill_nfold_params(mcg_handle *h, int c, NfoldParams &P) {
   P.div_magic = ((1ULL << 40) + (uint64_t)P.rows - 1) / (uint64_t)P.rows;
   P.time_period = h->sample_time_period;
   P.lat_off = h->lat_off[c];
+  P.sample_start = (h->options & MCG_OPT_NFOLD_SAMPLE_START) ? 1 : 0;
   const double beta = 1.0 / (h->kb * ch.temperature);
   const double mu = ch.exchange_potential;
   for (int sidx = 0; sidx < 2; ++sidx)
@@ -1551,7 +1552,12 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
   for (int c = 0; c < h->n_chains; ++c) {
     if (!h->active[c]) continue;
     int64_t p0 = h->n_pass[c];
-    if (!time_grid) h->n_samples[c] += (p0 + n_passes) / h->sample_period - p0 / h->sample_period;
+    // NFOLD by pass with MCG_OPT_NFOLD_SAMPLE_START: the pass-0 sample of a
+    // fresh chain (the kernel's condition: no pass and no sample yet)
+    const bool start = h->nfold() && !time_grid && (h->options & MCG_OPT_NFOLD_SAMPLE_START) &&
+                       p0 == 0 && h->n_samples[c] == 0;
+    if (!time_grid)
+      h->n_samples[c] += (p0 + n_passes) / h->sample_period - p0 / h->sample_period + (start ? 1 : 0);
     h->n_pass[c] = p0 + n_passes;
   }
   if (time_grid) {  // the number of time-grid records is known after the run

@@ -104,6 +104,8 @@ struct NfoldParams {
   double rate[10];      // Metropolis rate of class k = (s > 0) * 5 + (nbr_sum + 4) / 2
   double time_period;   // > 0: samples on the time grid k * time_period; else by pass
   int64_t lat_off;      // lattice byte offset; the class array starts at entry lat_off
+  int32_t sample_start; // by pass: also sample pass 0 (the first event selected then)
+  int32_t pad2;
 };
 
 // class array and positions (16-bit each; a position's bit 15 is the spin)

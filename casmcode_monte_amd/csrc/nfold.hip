@@ -247,18 +247,31 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
     // moving the next move's reads above these writes
     asm volatile("" ::: "memory");
   };
-  auto record = [&](double t, double w) {
+  // a by-pass record also keeps the total rate R in e_run
+  auto record = [&](double t, double w, double R) {
     if (lane == 0 && n_samples < series_cap) {
       mcg_sample rec = {};
       rec.pass = n_pass;
       rec.S1 = S;
       rec.M = M;
+      rec.e_run = R;
       rec.time = t;
       rec.weight = w;
       series[(int64_t)c * series_cap + n_samples] = rec;
     }
     ++n_samples;
   };
+  // by pass: select the pending event and sample the state with its increment
+  auto sample_pass = [&]() {
+    uint32_t l;
+    int l_pos;
+    double u3, R;
+    select(l, l_pos, u3, R);
+    pending_dt = -log1p(-u3) / R;
+    pending = l;
+    record(time, pending_dt, R);
+  };
+  if (!by_time && P->sample_start && n_pass == 0 && n_samples == 0 && pending < 0) sample_pass();
 
   // Deferred time increments: lane j holds event j of the current batch (its
   // u3 and R, or its known increment with R = -1, and the state before it).
@@ -387,15 +400,7 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
     flush();
     if (frozen) break;
     ++n_pass;
-    if (!by_time && n_pass % sample_period == 0) {
-      uint32_t l;
-      int l_pos;
-      double u3, R;
-      select(l, l_pos, u3, R);
-      pending_dt = -log1p(-u3) / R;
-      pending = l;
-      record(time, pending_dt);
-    }
+    if (!by_time && n_pass % sample_period == 0) sample_pass();
   }
 
   __syncthreads();

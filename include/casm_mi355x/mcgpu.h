@@ -136,6 +136,12 @@ typedef struct mcg_desc {
  * 16-site vector path applies (both give identical trajectories; this is the
  * cross-check of the vector path). */
 #define MCG_OPT_CHECKER_QUAD_PATH 4
+/* NFOLD, by-pass sampling: also sample the initial state (pass 0), weighted by
+ * the time increment of the first event, which is selected then -- the
+ * reference's sample at count 0 (nfold.hh:120-122 with a sampling fixture
+ * whose first sample count is 0).  Takes effect in the first mcg_run after
+ * mcg_create or mcg_assign_chain. */
+#define MCG_OPT_NFOLD_SAMPLE_START 8
 
 /* One sampled record of one chain.  Integer observables are exact; the
  * reference's FP64 sampling functions (basic_semigrand_canonical.hh:485-590)

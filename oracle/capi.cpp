@@ -481,7 +481,7 @@ int64_t orc_nfold_run(int64_t rows, int64_t cols, double J, double temperature, 
                       int64_t n_passes, int64_t sample_period, double time_period,
                       int64_t capacity, int8_t *occ, uint64_t *engine, int64_t *S, int64_t *M,
                       double *time, double *weight, int64_t *arr_out, double *scalars,
-                      int64_t *pending) {
+                      int64_t *pending, int64_t sample_start, double *rate) {
   IsingConfiguration cfg(rows, cols, 1);
   for (long l = 0; l < cfg.n_sites; ++l) cfg.occupation[l] = occ[l] > 0 ? 1 : -1;
   std::mt19937_64 eng;
@@ -491,8 +491,8 @@ int64_t orc_nfold_run(int64_t rows, int64_t cols, double J, double temperature, 
     std::istringstream is(os.str());
     is >> eng;
   }
-  NfoldRunOutput r =
-      nfold_run(cfg, J, temperature, mu, eng, n_passes, sample_period, time_period);
+  NfoldRunOutput r = nfold_run(cfg, J, temperature, mu, eng, n_passes, sample_period,
+                               time_period, sample_start != 0);
   for (long l = 0; l < cfg.n_sites; ++l) occ[l] = (int8_t)r.final_occ[l];
   engine_words(eng, engine);
   for (size_t k = 0; k < r.S.size() && (int64_t)k < capacity; ++k) {
@@ -500,6 +500,7 @@ int64_t orc_nfold_run(int64_t rows, int64_t cols, double J, double temperature, 
     M[k] = r.M[k];
     time[k] = r.time[k];
     weight[k] = r.weight[k];
+    if (rate && k < r.rate.size()) rate[k] = r.rate[k];
   }
   for (long l = 0; l < cfg.n_sites; ++l) arr_out[l] = r.arr[l];
   scalars[0] = r.final_time;

@@ -1320,9 +1320,10 @@ struct NfoldChain {
   }
 
   // select an event: the site and its time increment (three words)
-  double select(std::mt19937_64 &engine, long *site) const {
+  double select(std::mt19937_64 &engine, long *site, double *total_rate = nullptr) const {
     double R = 0.0;
     for (int k = 0; k < 10; ++k) R += (double)(start[k + 1] - start[k]) * rate[k];
+    if (total_rate) *total_rate = R;
     const double target = std::generate_canonical<double, 53>(engine) * R;
     double run = 0.0;
     int k = -1, last = -1;
@@ -1358,6 +1359,7 @@ struct NfoldChain {
 struct NfoldRunOutput {
   std::vector<long long> S, M;  // state at each sample
   std::vector<double> time, weight;
+  std::vector<double> rate;  // by pass: the total rate when the sample was taken
   std::vector<int> final_occ;
   std::vector<long> arr;
   double final_time = 0.0;
@@ -1377,15 +1379,29 @@ struct NfoldRunOutput {
 //    each holding the state in force at that time, weight 1
 //    (nfold.hh:105-108, 128-129); the grid points inside one residence
 //    interval are one record, weighted by their number, timed at the first.
+//  * sample_start (by pass): also the sample at count 0, before any event,
+//    weighted by the first event's increment (nfold.hh:120-122 when a
+//    fixture's first sample count is 0).
 inline NfoldRunOutput nfold_run(IsingConfiguration config, double J, double temperature,
                                 double mu, std::mt19937_64 &engine, long n_passes,
-                                long sample_period = 1, double time_period = 0.0) {
+                                long sample_period = 1, double time_period = 0.0,
+                                bool sample_start = false) {
   NfoldChain ch(std::move(config), J, temperature, mu);
   NfoldRunOutput out;
   const long n = ch.config.n_sites;
   long pending = -1;
   double pending_dt = 0.0;
   long next_k = 1;
+  auto sample_pass = [&]() {
+    double R = 0.0;
+    pending_dt = ch.select(engine, &pending, &R);
+    out.S.push_back(bond_sum(ch.config));
+    out.M.push_back(magnetisation(ch.config));
+    out.time.push_back(ch.time);
+    out.weight.push_back(pending_dt);
+    out.rate.push_back(R);
+  };
+  if (time_period <= 0.0 && sample_start) sample_pass();
   for (long p = 1; p <= n_passes; ++p) {
     for (long s = 0; s < n; ++s) {
       long l;
@@ -1416,13 +1432,7 @@ inline NfoldRunOutput nfold_run(IsingConfiguration config, double J, double temp
       ch.apply(l);
       ch.time += dt;
     }
-    if (time_period <= 0.0 && p % sample_period == 0) {
-      pending_dt = ch.select(engine, &pending);
-      out.S.push_back(bond_sum(ch.config));
-      out.M.push_back(magnetisation(ch.config));
-      out.time.push_back(ch.time);
-      out.weight.push_back(pending_dt);
-    }
+    if (time_period <= 0.0 && p % sample_period == 0) sample_pass();
   }
   out.final_occ = ch.config.occupation;
   out.arr = ch.arr;

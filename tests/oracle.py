@@ -124,7 +124,7 @@ def _declare(L):
     L.orc_canonical_run.restype = C.c_int
     L.orc_nfold_run.argtypes = [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double,
                                 C.c_int64, C.c_int64, C.c_double, C.c_int64, P8, PU, P64, P64,
-                                PD, PD, P64, PD, P64]
+                                PD, PD, P64, PD, P64, C.c_int64, PD]
     L.orc_nfold_run.restype = C.c_int64
 
 
@@ -386,27 +386,31 @@ def canonical_run(rows, cols, J, T, n_passes, occ, engine, loc=None, use_nlist=T
 
 
 def nfold_run(rows, cols, J, T, mu, n_passes, occ, engine, sample_period=1, time_period=0.0,
-              capacity=None):
+              capacity=None, sample_start=False):
     """N-fold way (class-based rejection-free selector) on spins occ, sampled by
-    pass (time_period = 0) or on a time grid; returns a dict with S, M, time,
-    weight per sample, final occ, engine, the class array, final time and the
-    pending event (site, dt)."""
+    pass (time_period = 0; sample_start adds the count-0 sample) or on a time
+    grid; returns a dict with S, M, time, weight (and, by pass, the total rate)
+    per sample, final occ, engine, the class array, final time and the pending
+    event (site, dt)."""
     n = rows * cols
     o = np.ascontiguousarray(occ, dtype=np.int8).copy()
     eng = np.ascontiguousarray(engine, dtype=np.uint64).copy()
-    m = capacity if capacity is not None else n_passes // sample_period
+    m = capacity if capacity is not None else n_passes // sample_period + (1 if sample_start else 0)
     S = np.zeros(max(m, 1), dtype=np.int64)
     M = np.zeros(max(m, 1), dtype=np.int64)
     t = np.zeros(max(m, 1), dtype=np.float64)
     w = np.zeros(max(m, 1), dtype=np.float64)
+    rate = np.zeros(max(m, 1), dtype=np.float64)
     arr = np.zeros(n, dtype=np.int64)
     sc = np.zeros(2, dtype=np.float64)
     pend = C.c_int64(0)
     got = lib().orc_nfold_run(rows, cols, J, T, mu, n_passes, sample_period, time_period, m,
                               _p(o, C.c_int8), _p(eng, C.c_uint64), _p(S, C.c_int64),
                               _p(M, C.c_int64), _p(t, C.c_double), _p(w, C.c_double),
-                              _p(arr, C.c_int64), _p(sc, C.c_double), C.byref(pend))
+                              _p(arr, C.c_int64), _p(sc, C.c_double), C.byref(pend),
+                              1 if sample_start else 0, _p(rate, C.c_double))
     m = min(m, got)
-    return dict(S=S[:m], M=M[:m], time=t[:m], weight=w[:m], occ=o, engine=eng, arr=arr,
+    return dict(S=S[:m], M=M[:m], time=t[:m], weight=w[:m], rate=rate[:m], occ=o, engine=eng,
+                arr=arr,
                 n_total=got,
                 final_time=float(sc[0]), pending_dt=float(sc[1]), pending_site=pend.value)

@@ -16,9 +16,10 @@ pytestmark = pytest.mark.gpu
 J = 0.1
 
 
-def _run_gpu(chains, occs, passes, chunks=None, sample_period=1, time_period=0.0, reserve=0):
+def _run_gpu(chains, occs, passes, chunks=None, sample_period=1, time_period=0.0, reserve=0,
+             options=0):
     r = cm.Run(chains, mode=cm.MODE_NFOLD, J1=J, sample_period=sample_period,
-               sample_time_period=time_period)
+               sample_time_period=time_period, options=options)
     for i, o in enumerate(occs):
         r.set_occupation(i, o)
     if reserve:
@@ -67,6 +68,27 @@ def test_nfold_by_pass_matches_oracle(rows, cols, T, mu, seed, passes, sp):
     _compare(r, 0, o)
     n_pass, n_ev, n_rej = r.counters(0)
     assert (n_pass, n_ev, n_rej) == (passes, passes * n, 0)
+    r.close()
+
+
+@pytest.mark.parametrize("rows,cols,seed,passes,sp", [(8, 8, 21, 30, 1), (5, 7, 22, 31, 3)])
+def test_nfold_sample_start_and_total_rate(rows, cols, seed, passes, sp):
+    """MCG_OPT_NFOLD_SAMPLE_START adds the pass-0 record (the initial state
+    weighted by the first event's increment); by-pass records keep the total
+    rate R in e_run."""
+    print("case:", rows, cols, seed, passes, sp, flush=True)
+    T, mu = 1700.0, 0.05
+    occ = np.where(np.random.default_rng(seed).random(rows * cols) < 0.5, 1, -1).astype(np.int8)
+    o = O.nfold_run(rows, cols, J, T, mu, passes, occ, O.mt64_state(seed, 0), sample_period=sp,
+                    sample_start=True)
+    r = _run_gpu([dict(shape=(rows, cols), temperature=T, exchange_potential=mu, seed=seed)],
+                 [occ], passes, chunks=[1, passes - 1], sample_period=sp,
+                 options=cm.OPT_NFOLD_SAMPLE_START)
+    rec = r.samples(0)
+    assert len(rec) == passes // sp + 1
+    assert rec["pass"][0] == 0 and rec["time"][0] == 0.0
+    _compare(r, 0, o)
+    np.testing.assert_allclose(rec["e_run"], o["rate"], rtol=1e-13)
     r.close()
 
 

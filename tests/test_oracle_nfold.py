@@ -125,3 +125,28 @@ def test_weighted_mean_helper():
     assert mean == pytest.approx((Nx * w).sum() / w.sum(), rel=1e-15)
     ex = oracle.lattice_enumerate(2, (3, 5), J, 0.0, 1800.0, 0.05)
     assert abs(mean - ex["Nx"]) < 4 * err
+
+
+def test_sample_start_and_total_rate():
+    """The count-0 sample holds the initial state at time 0, weighted by the first
+    event's increment; selecting that event early changes no later draw, so the
+    rest equals the run without it.  On an all +1 lattice every site is in the
+    class (s = +1, four +1 neighbours), so the recorded total rate is
+    N * exp(-beta * (8 J + mu))."""
+    rows, cols, T, mu, seed = 6, 7, 1800.0, 0.1, 12
+    occ = np.ones(rows * cols, dtype=np.int8)
+    a = oracle.nfold_run(rows, cols, 0.1, T, mu, 5, occ, oracle.mt64_state(seed, 0),
+                         sample_start=True)
+    b = oracle.nfold_run(rows, cols, 0.1, T, mu, 5, occ, oracle.mt64_state(seed, 0))
+    assert len(a["S"]) == len(b["S"]) + 1
+    assert a["S"][0] == 2 * rows * cols and a["M"][0] == rows * cols and a["time"][0] == 0.0
+    np.testing.assert_array_equal(a["S"][1:], b["S"])
+    np.testing.assert_array_equal(a["weight"][1:], b["weight"])
+    np.testing.assert_array_equal(a["occ"], b["occ"])
+    np.testing.assert_array_equal(a["engine"], b["engine"])
+    beta = 1.0 / (oracle.kb() * T)
+    assert a["rate"][0] == pytest.approx(rows * cols * np.exp(-beta * (8 * 0.1 + mu)), rel=1e-13)
+    # the first increment is -log1p(-u)/R with u the canonical double of the
+    # third word the first event draws
+    u = oracle.canonical(oracle.mt64_outputs(seed, 3)[2])
+    assert a["weight"][0] == -np.log1p(-u) / a["rate"][0]
