@@ -41,7 +41,7 @@ EXPORTED = [
     "mcg_stats_weighted",
     "mcg_equilibration", "mcg_last_run_time", "mcg_assign_chain", "mcg_autotune",
     "mcg_get_schedule", "mcg_set_schedule", "mcg_reserve_samples", "mcg_get_occ_location",
-    "mcg_set_occ_location",
+    "mcg_set_occ_location", "mcg_get_nfold_state",
 ]
 
 
@@ -99,6 +99,8 @@ def lib():
         L.mcg_get_counters.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), P(C.c_int64),
                                        P(C.c_int64)]
         L.mcg_n_samples.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64)]
+        L.mcg_get_nfold_state.argtypes = [C.c_void_p, C.c_int64, P(C.c_double), P(C.c_int64),
+                                          P(C.c_double)]
         L.mcg_get_samples.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_void_p]
         L.mcg_clear_samples.argtypes = [C.c_void_p]
         L.mcg_stats_compute.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_double,
@@ -240,6 +242,12 @@ class Run:
         a, b, d = C.c_int64(), C.c_int64(), C.c_int64()
         _check(lib().mcg_get_counters(self.h, c, C.byref(a), C.byref(b), C.byref(d)))
         return a.value, b.value, d.value
+
+    def nfold_state(self, c):
+        """NFOLD: (simulated time, pending event site or -1, its time increment)."""
+        t, site, dt = C.c_double(), C.c_int64(), C.c_double()
+        _check(lib().mcg_get_nfold_state(self.h, c, C.byref(t), C.byref(site), C.byref(dt)))
+        return t.value, site.value, dt.value
 
     def n_samples(self, c):
         n = C.c_int64()

@@ -1475,13 +1475,25 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
   if (n_passes < 0) return fail(MCG_ERR_ARG, "n_passes < 0");
   h->last_ms = 0.0;
   h->last_attempts = 0.0;
-  if (n_passes == 0) return MCG_OK;
+  const bool time_grid = h->nfold() && h->sample_time_period > 0.0;
+  // NFOLD by pass with MCG_OPT_NFOLD_SAMPLE_START: the pass-0 sample of a
+  // fresh chain (the kernel's condition: no pass and no sample yet); a run of
+  // 0 passes takes just that sample
+  auto start_sample = [&](int c) {
+    return h->nfold() && !time_grid && (h->options & MCG_OPT_NFOLD_SAMPLE_START) &&
+           h->n_pass[c] == 0 && h->n_samples[c] == 0;
+  };
+  // NFOLD by pass: a run of 0 passes selects the next event of each active
+  // chain that has none pending and records the current pass
+  const bool sample_now = h->nfold() && !time_grid && n_passes == 0;
+  if (n_passes == 0 && !sample_now) return MCG_OK;
   // grow the series to hold every sample this run can append
   int64_t need = 0;
   for (int c = 0; c < h->n_chains; ++c) {
     if (!h->active[c]) continue;
     int64_t p0 = h->n_pass[c];
-    int64_t add = (p0 + n_passes) / h->sample_period - p0 / h->sample_period;
+    int64_t add = (p0 + n_passes) / h->sample_period - p0 / h->sample_period +
+                  (start_sample(c) || sample_now ? 1 : 0);
     need = std::max(need, h->n_samples[c] + add);
   }
   int rc = ensure_series(h, need);
@@ -1519,7 +1531,8 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
                      h->stream);
   } else if (h->nfold()) {
     launch_nfold(h->d_np, h->d_xs, h->d_lat, h->d_list, h->d_series, h->series_cap,
-                 h->d_active, h->n_chains, n_passes, h->sample_period, h->nfold_lds, h->stream);
+                 h->d_active, h->n_chains, n_passes, h->sample_period, sample_now ? 1 : 0,
+                 h->nfold_lds, h->stream);
   } else if (h->fcc()) {
     HIP_TRY(hipGetLastError());
     launch_fcc(h->d_cp, h->d_cs, h->d_lat, h->d_ftab, h->d_tri, h->d_series, h->series_cap,
@@ -1548,26 +1561,23 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
   HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   h->last_ms = ms;
   h->last_attempts = attempts;
-  const bool time_grid = h->nfold() && h->sample_time_period > 0.0;
   for (int c = 0; c < h->n_chains; ++c) {
     if (!h->active[c]) continue;
     int64_t p0 = h->n_pass[c];
-    // NFOLD by pass with MCG_OPT_NFOLD_SAMPLE_START: the pass-0 sample of a
-    // fresh chain (the kernel's condition: no pass and no sample yet)
-    const bool start = h->nfold() && !time_grid && (h->options & MCG_OPT_NFOLD_SAMPLE_START) &&
-                       p0 == 0 && h->n_samples[c] == 0;
-    if (!time_grid)
-      h->n_samples[c] += (p0 + n_passes) / h->sample_period - p0 / h->sample_period + (start ? 1 : 0);
+    if (!h->nfold())
+      h->n_samples[c] += (p0 + n_passes) / h->sample_period - p0 / h->sample_period;
     h->n_pass[c] = p0 + n_passes;
   }
-  if (time_grid) {  // the number of time-grid records is known after the run
+  if (h->nfold()) {
+    // the kernel counts NFOLD records: time-grid records, the pass-0 sample and
+    // the sample of a run of 0 passes depend on the chain's state
     std::vector<ExactState> xs(h->n_chains);
     HIP_TRY(copy_sync(h, xs.data(), h->d_xs, sizeof(ExactState) * h->n_chains,
                       hipMemcpyDeviceToHost));
     int64_t over = 0;
     for (int c = 0; c < h->n_chains; ++c) {
       if (!h->active[c]) continue;
-      if (xs[c].frozen)
+      if (time_grid && xs[c].frozen)
         return fail(MCG_ERR_STATE,
                     "NFOLD chain " + std::to_string(c) +
                         ": the total rate is 0 (every flip has probability 0 at this "
@@ -1581,6 +1591,19 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
                   "NFOLD time-grid samples exceeded the sample series by " + std::to_string(over) +
                       " records (they were dropped); reserve more with mcg_reserve_samples");
   }
+  return MCG_OK;
+}
+
+int mcg_get_nfold_state(mcg_handle *h, int64_t chain, double *time, int64_t *pending_site,
+                        double *pending_dt) {
+  if (!valid_chain(h, chain)) return fail(MCG_ERR_ARG, "bad chain");
+  if (!h->nfold()) return fail(MCG_ERR_ARG, "mcg_get_nfold_state: not an NFOLD handle");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  ExactState st;
+  HIP_TRY(copy_sync(h, &st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
+  if (time) *time = st.time;
+  if (pending_site) *pending_site = st.pending_site;
+  if (pending_dt) *pending_dt = st.pending_dt;
   return MCG_OK;
 }
 

@@ -113,7 +113,8 @@ inline int nfold_lds_bytes(const NfoldParams &P) { return 4 * P.n + 64; }
 
 void launch_nfold(const NfoldParams *P, ExactState *S, int8_t *lat_pool, int32_t *list_pool,
                   mcg_sample *series, int64_t series_cap, const uint8_t *active, int n_chains,
-                  int64_t n_passes, int64_t sample_period, int lds_bytes, hipStream_t stream);
+                  int64_t n_passes, int64_t sample_period, int sample_now, int lds_bytes,
+                  hipStream_t stream);
 hipError_t nfold_set_max_lds(int bytes);
 
 // ---------------------------------------------------------------------------

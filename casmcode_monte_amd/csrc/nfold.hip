@@ -88,7 +88,8 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
                                                    mcg_sample *__restrict__ series,
                                                    int64_t series_cap,
                                                    const uint8_t *__restrict__ active,
-                                                   int64_t n_passes, int64_t sample_period) {
+                                                   int64_t n_passes, int64_t sample_period,
+                                                   int sample_now) {
   const int c = blockIdx.x;
   if (!active[c]) return;
   const int lane = threadIdx.x;
@@ -271,7 +272,11 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
     pending = l;
     record(time, pending_dt, R);
   };
-  if (!by_time && P->sample_start && n_pass == 0 && n_samples == 0 && pending < 0) sample_pass();
+  // the pass-0 sample, or a sample of the current pass asked for by the host,
+  // unless this pass's first event is already selected
+  if (!by_time && pending < 0 &&
+      (sample_now || (P->sample_start && n_pass == 0 && n_samples == 0)))
+    sample_pass();
 
   // Deferred time increments: lane j holds event j of the current batch (its
   // u3 and R, or its known increment with R = -1, and the state before it).
@@ -431,9 +436,10 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
 
 void launch_nfold(const NfoldParams *P, ExactState *S, int8_t *lat_pool, int32_t *list_pool,
                   mcg_sample *series, int64_t series_cap, const uint8_t *active, int n_chains,
-                  int64_t n_passes, int64_t sample_period, int lds_bytes, hipStream_t stream) {
+                  int64_t n_passes, int64_t sample_period, int sample_now, int lds_bytes,
+                  hipStream_t stream) {
   hipLaunchKernelGGL(nfold_kernel, dim3(n_chains), dim3(64), lds_bytes, stream, P, S, lat_pool,
-                     list_pool, series, series_cap, active, n_passes, sample_period);
+                     list_pool, series, series_cap, active, n_passes, sample_period, sample_now);
 }
 
 hipError_t nfold_set_max_lds(int bytes) {

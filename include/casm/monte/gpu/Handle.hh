@@ -88,6 +88,10 @@ class Handle {
     check(mcg_get_counters(m_h, chain, &n_pass, &n_accept, &n_reject));
   }
   void clear_samples() { check(mcg_clear_samples(m_h)); }
+  /// NFOLD: simulated time, pending event site (-1: none) and its increment
+  void nfold_state(Index chain, double &time, int64_t &pending_site, double &pending_dt) const {
+    check(mcg_get_nfold_state(m_h, chain, &time, &pending_site, &pending_dt));
+  }
 
  private:
   mcg_handle *m_h = nullptr;

@@ -261,10 +261,10 @@ class SamplingFixture {
     m_results.reset();
     if (m_params.sampling_params.sample_mode == SAMPLE_MODE::BY_TIME)
       throw std::runtime_error("SamplingFixture: SAMPLE_MODE::BY_TIME is not supported by "
-                               "the Metropolis GPU loop");
+                               "the GPU loops");
     if (m_params.sampling_params.stochastic_sample_period)
       throw std::runtime_error("SamplingFixture: stochastic sampling periods are not supported "
-                               "by the Metropolis GPU loop");
+                               "by the GPU loops");
     m_next_sample_count =
         static_cast<CountType>(std::round(sample_at(m_results.sample_count.size())));
     if (m_next_sample_count < 0)
@@ -311,8 +311,14 @@ class SamplingFixture {
   void increment_step() { m_counter.increment_step(); }
   CountType next_sample_count() const { return m_next_sample_count; }
 
+  /// Weight of the next sample (time increment of the N-fold way's event
+  /// selected at the sample; SamplingFixture.hh:330-332)
+  void push_back_sample_weight(double weight) { m_results.sample_weight.push_back(weight); }
+  void set_time(double t) { m_counter.set_time(t); }
+
   void sample_data(state_type const &state) {
     m_results.sample_count.push_back(m_counter.count);
+    if (m_params.sampling_params.do_sample_time) m_results.sample_time.push_back(m_counter.time);
     m_results.sample_clocktime.push_back(m_params.method_log.log.time_s());
     if (m_params.sampling_params.do_sample_trajectory)
       m_results.sample_trajectory.push_back(state.configuration);
@@ -411,10 +417,27 @@ struct RunManager {
   void increment_step() {
     for (auto &f : sampling_fixtures) f->increment_step();
   }
-  void sample_data_by_count_if_due(state_type const &state) {
+  struct NullAction {
+    void operator()(sampling_fixture_type const &, state_type const &) {}
+  };
+
+  void set_time(double event_time) {
+    for (auto &f : sampling_fixtures) f->set_time(event_time);
+  }
+
+  /// Sample every fixture due by count (RunManager.hh:165-186); the actions
+  /// run just before and after each fixture's sample.
+  template <typename PreSampleActionType = NullAction, typename PostSampleActionType = NullAction>
+  void sample_data_by_count_if_due(state_type const &state,
+                                   PreSampleActionType pre_sample_f = PreSampleActionType(),
+                                   PostSampleActionType post_sample_f = PostSampleActionType()) {
     for (auto &f : sampling_fixtures)
-      if (f->params().sampling_params.sample_mode != SAMPLE_MODE::BY_TIME)
-        f->sample_data_by_count_if_due(state);
+      if (f->params().sampling_params.sample_mode != SAMPLE_MODE::BY_TIME &&
+          f->counter().count == f->next_sample_count()) {
+        pre_sample_f(*f, state);
+        f->sample_data(state);
+        post_sample_f(*f, state);
+      }
   }
   void finalize(state_type const &final_state) {
     for (auto &f : sampling_fixtures) f->finalize(final_state, run_index);

@@ -140,7 +140,7 @@ typedef struct mcg_desc {
  * the time increment of the first event, which is selected then -- the
  * reference's sample at count 0 (nfold.hh:120-122 with a sampling fixture
  * whose first sample count is 0).  Takes effect in the first mcg_run after
- * mcg_create or mcg_assign_chain. */
+ * mcg_create or mcg_assign_chain (see mcg_run for runs of 0 passes). */
 #define MCG_OPT_NFOLD_SAMPLE_START 8
 
 /* One sampled record of one chain.  Integer observables are exact; the
@@ -223,7 +223,13 @@ int mcg_rng_load(mcg_handle *h, int64_t chain, const uint64_t *words, int64_t n_
  * count reaches a multiple of sample_period.  Replaces the main loop of
  * basic_occupation_metropolis.hh:381-422 between two completion checks (the
  * drop-in classes call it with the number of passes to the next point at
- * which the reference's completion check can change its answer). */
+ * which the reference's completion check can change its answer).
+ * NFOLD by pass: mcg_run(h, 0) selects the next event of every active chain
+ * that has none pending and appends the record of the current pass (the
+ * state, the selected event's time increment as weight and the total rate),
+ * as a sampled pass does; the next mcg_run applies that event first.  This is
+ * the reference's select-then-sample at a count where a fixture is due
+ * (nfold.hh:121-126), taken only once the caller knows the run goes on. */
 int mcg_run(mcg_handle *h, int64_t n_passes);
 
 /* Mark chains active (1) or inactive (0); inactive chains are skipped by
@@ -247,6 +253,16 @@ int mcg_get_counters(mcg_handle *h, int64_t chain, int64_t *n_pass, int64_t *n_a
 
 /* Number of sample records held in a chain's device series. */
 int mcg_n_samples(mcg_handle *h, int64_t chain, int64_t *n_samples);
+
+/* NFOLD: a chain's simulated time and its pending event -- the site of the
+ * event selected at the last by-pass sample and its time increment, which the
+ * next mcg_run applies first (site -1: none pending).  The reference's loop
+ * keeps these in locals of nfold.hh:83-88 (selected_event_id, time_increment)
+ * between selecting an event and applying it; the drop-in nfold() reads them
+ * when a completion check stops the run right after a sample, where the
+ * reference applies that event before it stops. */
+int mcg_get_nfold_state(mcg_handle *h, int64_t chain, double *time, int64_t *pending_site,
+                        double *pending_dt);
 
 /* Copy sample records [first, first+count) of a chain to the host.
  * Replaces the Sampler storage (sampling/Sampler.hh:252-267). */

@@ -3,15 +3,18 @@
 //
 //   dropin_tests --cpu : host logic only (completion check, statistics,
 //                        equilibration, sampler) on synthetic series;
-//   dropin_tests --gpu : SemiGrandCanonicalCalculator::run on the MI355X vs the
-//                        oracle's reference loop, bit for bit.
+//   dropin_tests --gpu : SemiGrandCanonicalCalculator::run, occupation_metropolis
+//                        and nfold on the MI355X vs the oracle's loops (bit for
+//                        bit; N-fold way times to the rounding of log1p).
 // Prints one line per case and "ALL OK" at the end; exits non-zero on failure.
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <iostream>
 #include <numeric>
 
 #include "casm/monte/ising_cpp/basic_semigrand_canonical.hh"
+#include "casm/monte/methods/nfold.hh"
 #include "casm/monte/methods/occupation_metropolis.hh"
 #include "refcpu.hh"
 
@@ -467,6 +470,135 @@ static void gpu_canonical_tests() {
             << std::endl;
 }
 
+static bool close_rel(double a, double b, double rel) {
+  return std::fabs(a - b) <= rel * std::max(std::fabs(a), std::fabs(b));
+}
+
+// nfold() on the MI355X (NFOLD mode) vs refcpu::nfold_run.  Two ways to stop:
+//  * max_count = P: the check at count P comes before that pass's selection,
+//    so the samples are counts 0..P-1 and the engine has drawn exactly the
+//    events of P passes (oracle: P passes, no boundary sample but the first);
+//  * convergence: the check right after the sample at count c stops the run
+//    after that pass's first event (oracle: c passes sampled every pass, then
+//    its pending event applied).
+static void gpu_nfold_case(bool converge_stop) {
+  const int R = 10, C = 12, N = R * C;
+  const long P = 40;
+  const double T = 1800.0, J = 0.1, mu = 0.05;
+  const uint64_t seed = converge_stop ? 52 : 51;
+  typedef ising_cpp::IsingConfiguration config_type;
+  typedef RunManager<config_type, BasicStatistics, std::mt19937_64> rm_type;
+  typedef NfoldData<config_type, BasicStatistics, std::mt19937_64> data_type;
+  std::vector<int> occ0 = random_occupants(N, 57, 13);
+  config_type cfg({R, C}, 0);
+  cfg.set_occupation(occ0);
+  ValueMap cond;
+  cond.scalar_values["temperature"] = T;
+  cond.vector_values["exchange_potential"] = VectorXd{mu};
+  State<config_type> state(cfg, cond);
+  ising_cpp::BinaryLatticeEvents ev(N);
+  OccLocation ol(ev.convert, ev.candidate_list);
+  ol.initialize(state.configuration.occupation());
+  data_type nfold_data;
+  nfold_data.n_events_possible = N;
+
+  StateSamplingFunctionMap fns;
+  StateSamplingFunction m("spin_sum", "sum of spins", {}, [&state]() {
+    double s = 0;
+    for (int v : state.configuration.occupation()) s += 2 * v - 1;
+    return VectorXd{s};
+  });
+  StateSamplingFunction ear("expected_acceptance_rate", "total rate / N", {}, [&nfold_data]() {
+    return VectorXd{nfold_data.expected_acceptance_rate};
+  });
+  fns.emplace(m.name, m);
+  fns.emplace(ear.name, ear);
+  SamplingParams sp;
+  sp.sampler_names = {"spin_sum", "expected_acceptance_rate"};
+  sp.do_sample_time = true;
+  // the cutoff case samples from count 0 (the initial state, weighted by the
+  // first event's increment), the convergence case from count 1
+  sp.begin = converge_stop ? 1.0 : 0.0;
+  const long first = converge_stop ? 1 : 0;
+  CompletionCheckParams<BasicStatistics> cc;
+  if (converge_stop) {
+    cc.cutoff_params.max_count = 20000;
+    cc.check_begin = 50;
+    cc.check_period = 10;
+    converge(fns, cc).set_precision("spin_sum", 4.0);
+  } else {
+    cc.cutoff_params.max_count = P;
+  }
+  std::vector<rm_type::sampling_fixture_params_type> fixtures;
+  fixtures.emplace_back("kinetics", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), sp, cc,
+                        std::vector<std::string>{});
+  auto rm_engine = std::make_shared<std::mt19937_64>(seed + 1000);
+  rm_type rm(rm_engine, fixtures, true);
+  auto engine = std::make_shared<std::mt19937_64>(seed);
+  ising_cpp::IsingNfoldEventSelector<std::mt19937_64> selector(J, engine);
+  OccEvent flip;
+  flip.occ_transform.resize(1);
+  auto get_event_f = [&](Index l) -> OccEvent const & {
+    Mol const &mol = ol.mol(ol.l_to_mol_id(l));
+    flip.linear_site_index = {l};
+    flip.new_occ = {1 - state.configuration.occ(l)};
+    flip.occ_transform[0] = OccTransform{l, mol.id, mol.asym, mol.species_index,
+                                         1 - mol.species_index};
+    return flip;
+  };
+  nfold<Index>(state, ol, nfold_data, selector, get_event_f, rm);
+
+  auto const &f = *rm.sampling_fixtures[0];
+  auto const &res = f.results();
+  const long n_samp = res.samplers.at("spin_sum")->n_samples();
+  const long c_stop = f.counter().count;
+  const std::string tag = converge_stop ? "nfold convergence stop" : "nfold count cutoff";
+  refcpu::IsingConfiguration oc(R, C, -1);
+  for (int l = 0; l < N; ++l) oc.occupation[l] = occ0[l] ? 1 : -1;
+  // the samples: counts first..first+n_samp-1
+  std::mt19937_64 e1(seed);
+  auto o1 = refcpu::nfold_run(oc, J, T, mu, e1, first + n_samp - 1, 1, 0.0, first == 0);
+  bool ok = (long)o1.M.size() == n_samp && (long)res.sample_weight.n_samples() == n_samp &&
+            (long)res.sample_time.size() == n_samp;
+  VectorXd got_m = res.samplers.at("spin_sum")->component(0);
+  VectorXd got_r = res.samplers.at("expected_acceptance_rate")->component(0);
+  VectorXd got_w = res.sample_weight.component(0);
+  for (long i = 0; ok && i < n_samp; ++i)
+    ok = got_m(i) == (double)o1.M[i] && close_rel(got_w(i), o1.weight[i], 1e-12) &&
+         close_rel(got_r(i), o1.rate[i] / N, 1e-13) &&
+         close_rel(res.sample_time[i], o1.time[i], 1e-12) && res.sample_count[i] == first + i;
+  EXPECT(ok, tag << ": samples (" << n_samp << ")");
+  if (converge_stop) {
+    EXPECT(c_stop == n_samp && f.counter().step == 1 && c_stop < 20000,
+           tag << ": stops one event into the pass of its last sample (count " << c_stop
+               << ", step " << f.counter().step << ")");
+    std::vector<int> want(o1.final_occ.size());
+    for (int l = 0; l < N; ++l) want[l] = o1.final_occ[l] > 0 ? 1 : 0;
+    want[o1.pending_site] = 1 - want[o1.pending_site];
+    EXPECT(state.configuration.occupation() == want, tag << ": final configuration");
+    EXPECT(*engine == e1, tag << ": engine state");
+    EXPECT(res.n_accept == (BigCountType)c_stop * N + 1 && res.n_reject == 0,
+           tag << ": event count " << res.n_accept);
+    EXPECT(close_rel(f.counter().time, o1.final_time + o1.pending_dt, 1e-12), tag << ": time");
+  } else {
+    EXPECT(n_samp == P && c_stop == P && f.counter().step == 0, tag << ": stops at count P");
+    std::mt19937_64 e2(seed);
+    auto o2 = refcpu::nfold_run(oc, J, T, mu, e2, P, P + 1, 0.0, true);
+    std::vector<int> want(N);
+    for (int l = 0; l < N; ++l) want[l] = o2.final_occ[l] > 0 ? 1 : 0;
+    EXPECT(state.configuration.occupation() == want, tag << ": final configuration");
+    EXPECT(*engine == e2, tag << ": engine state (no selection at count P)");
+    EXPECT(res.n_accept == (BigCountType)P * N && res.n_reject == 0, tag << ": event count");
+    EXPECT(close_rel(f.counter().time, o2.final_time, 1e-12), tag << ": time");
+  }
+  EXPECT(*rm_engine == std::mt19937_64(seed + 1000), tag << ": run manager engine untouched");
+  EXPECT(occ_location_consistent(ol, state.configuration.occupation()),
+         tag << ": OccLocation invariants after the run");
+  std::cout << "gpu: " << tag << " 10x12 (" << n_samp << " samples)"
+            << (g_fail ? " (failures so far)" : " ok") << std::endl;
+}
+
 static void gpu_tests() {
   // the reference's SemiGrandCanonicalRun1 set-up, converged on two observables
   gpu_case({25, 25, true, 2000.0, 0.0, 42, 1, -1, 0.001, 100, false});
@@ -479,6 +611,8 @@ static void gpu_tests() {
   gpu_case({3, 3, true, 4000.0, 0.0, 11, 1, 3000, 1e-4, 500, false});
   gpu_occupation_metropolis_tests();
   gpu_canonical_tests();
+  gpu_nfold_case(false);
+  gpu_nfold_case(true);
 }
 
 int main(int argc, char **argv) {

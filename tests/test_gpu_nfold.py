@@ -45,6 +45,14 @@ def _compare(r, c, o, exact_time=False):
     np.testing.assert_array_equal(r.rng_dump(c), o["engine"])
 
 
+def _compare_pending(r, c, o):
+    t, site, dt = r.nfold_state(c)
+    assert site == o["pending_site"]
+    assert t == pytest.approx(o["final_time"], rel=1e-12)
+    if site >= 0:
+        assert dt == pytest.approx(o["pending_dt"], rel=1e-12)
+
+
 @pytest.mark.parametrize("rows,cols,T,mu,seed,passes,sp", [
     (8, 8, 2000.0, 0.0, 5, 40, 1),
     (7, 9, 1500.0, 0.1, 6, 40, 3),
@@ -66,6 +74,7 @@ def test_nfold_by_pass_matches_oracle(rows, cols, T, mu, seed, passes, sp):
                  [occ], passes, chunks=[1, passes // 2, passes - 1 - passes // 2],
                  sample_period=sp)
     _compare(r, 0, o)
+    _compare_pending(r, 0, o)
     n_pass, n_ev, n_rej = r.counters(0)
     assert (n_pass, n_ev, n_rej) == (passes, passes * n, 0)
     r.close()
@@ -82,13 +91,35 @@ def test_nfold_sample_start_and_total_rate(rows, cols, seed, passes, sp):
     o = O.nfold_run(rows, cols, J, T, mu, passes, occ, O.mt64_state(seed, 0), sample_period=sp,
                     sample_start=True)
     r = _run_gpu([dict(shape=(rows, cols), temperature=T, exchange_potential=mu, seed=seed)],
-                 [occ], passes, chunks=[1, passes - 1], sample_period=sp,
+                 [occ], passes, chunks=[0, 0, 1, passes - 1], sample_period=sp,
                  options=cm.OPT_NFOLD_SAMPLE_START)
     rec = r.samples(0)
     assert len(rec) == passes // sp + 1
     assert rec["pass"][0] == 0 and rec["time"][0] == 0.0
     _compare(r, 0, o)
+    _compare_pending(r, 0, o)
     np.testing.assert_allclose(rec["e_run"], o["rate"], rtol=1e-13)
+    r.close()
+
+
+def test_nfold_zero_pass_run_takes_the_start_sample():
+    """mcg_run(h, 0) with MCG_OPT_NFOLD_SAMPLE_START records pass 0 and leaves
+    the first event pending; the run that follows applies it first."""
+    rows, cols, seed, T, mu = 6, 10, 23, 1900.0, -0.05
+    occ = np.where(np.random.default_rng(seed).random(rows * cols) < 0.5, 1, -1).astype(np.int8)
+    o0 = O.nfold_run(rows, cols, J, T, mu, 0, occ, O.mt64_state(seed, 0), sample_start=True)
+    r = _run_gpu([dict(shape=(rows, cols), temperature=T, exchange_potential=mu, seed=seed)],
+                 [occ], 0, chunks=[0], options=cm.OPT_NFOLD_SAMPLE_START)
+    assert r.counters(0) == (0, 0, 0)
+    _compare(r, 0, o0)
+    _compare_pending(r, 0, o0)
+    assert o0["pending_site"] >= 0
+    r.run(0)  # no further sample: the chain has one already
+    assert r.n_samples(0) == 1
+    r.run(5)
+    o = O.nfold_run(rows, cols, J, T, mu, 5, occ, O.mt64_state(seed, 0), sample_start=True)
+    _compare(r, 0, o)
+    _compare_pending(r, 0, o)
     r.close()
 
 
