@@ -1487,6 +1487,10 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
   // chain that has none pending and records the current pass
   const bool sample_now = h->nfold() && !time_grid && n_passes == 0;
   if (n_passes == 0 && !sample_now) return MCG_OK;
+  // NFOLD records the host cannot count: time-grid records, and a pass-0 or
+  // asked-for sample, which the kernel skips when an event is pending
+  bool count_on_device = time_grid || sample_now;
+  for (int c = 0; c < h->n_chains; ++c) count_on_device |= h->active[c] && start_sample(c);
   // grow the series to hold every sample this run can append
   int64_t need = 0;
   for (int c = 0; c < h->n_chains; ++c) {
@@ -1564,13 +1568,11 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
   for (int c = 0; c < h->n_chains; ++c) {
     if (!h->active[c]) continue;
     int64_t p0 = h->n_pass[c];
-    if (!h->nfold())
+    if (!count_on_device)
       h->n_samples[c] += (p0 + n_passes) / h->sample_period - p0 / h->sample_period;
     h->n_pass[c] = p0 + n_passes;
   }
-  if (h->nfold()) {
-    // the kernel counts NFOLD records: time-grid records, the pass-0 sample and
-    // the sample of a run of 0 passes depend on the chain's state
+  if (count_on_device) {
     std::vector<ExactState> xs(h->n_chains);
     HIP_TRY(copy_sync(h, xs.data(), h->d_xs, sizeof(ExactState) * h->n_chains,
                       hipMemcpyDeviceToHost));
