@@ -23,8 +23,8 @@
 // Between those points the device runs whole passes.
 //
 // Not supported: SAMPLE_MODE::BY_STEP and BY_TIME fixtures (the device samples
-// by pass; its own time-grid mode serves C-ABI callers) and selected-event
-// statistics.  At the end occ_location is rebuilt from the final occupation:
+// by pass; its own time-grid mode serves C-ABI callers), simulated-time
+// cutoffs, and selected-event statistics.  At the end occ_location is rebuilt from the final occupation:
 // its candidate lists hold the same sites as the reference's, in site order
 // rather than in the order the reference's applies leave them (only a later
 // draw from those lists would notice).
@@ -99,6 +99,12 @@ void nfold(State<ising_cpp::IsingConfiguration> &state, OccLocation &occ_locatio
     auto const &sp = f->params().sampling_params;
     if (sp.sample_mode != SAMPLE_MODE::BY_PASS)
       throw std::runtime_error("nfold (MI355X): only SAMPLE_MODE::BY_PASS sampling is supported");
+    // the reference checks simulated-time cutoffs before every event
+    // (SamplingFixture.hh:272-275); the device stops only between passes
+    auto const &cp = f->params().completion_check_params.cutoff_params;
+    if (sp.do_sample_time && (cp.min_time.has_value() || cp.max_time.has_value()))
+      throw std::runtime_error(
+          "nfold (MI355X): simulated-time cutoffs (min_time, max_time) are not supported");
   }
 
   // nfold.hh:118-119

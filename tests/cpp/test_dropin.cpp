@@ -599,6 +599,53 @@ static void gpu_nfold_case(bool converge_stop) {
             << (g_fail ? " (failures so far)" : " ok") << std::endl;
 }
 
+// nfold's argument checks run before any device call
+static void cpu_nfold_argument_tests() {
+  typedef ising_cpp::IsingConfiguration config_type;
+  typedef RunManager<config_type, BasicStatistics, std::mt19937_64> rm_type;
+  const int N = 16;
+  auto attempt = [&](SAMPLE_MODE mode, bool time_cutoff, int bad_occ) {
+    config_type cfg({4, 4}, 0);
+    if (bad_occ) cfg.set_occ(3, bad_occ);
+    ValueMap cond;
+    cond.scalar_values["temperature"] = 1000.0;
+    cond.vector_values["exchange_potential"] = VectorXd{0.0};
+    State<config_type> state(cfg, cond);
+    ising_cpp::BinaryLatticeEvents ev(N);
+    OccLocation ol(ev.convert, ev.candidate_list);
+    ol.initialize(std::vector<int>(N, 0));
+    SamplingParams sp;
+    sp.sample_mode = mode;
+    sp.do_sample_time = time_cutoff;
+    CompletionCheckParams<BasicStatistics> cc;
+    cc.cutoff_params.max_count = 10;
+    if (time_cutoff) cc.cutoff_params.max_time = 1.0;
+    std::vector<rm_type::sampling_fixture_params_type> fixtures;
+    fixtures.emplace_back("f", StateSamplingFunctionMap(), jsonStateSamplingFunctionMap(),
+                          ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), sp, cc,
+                          std::vector<std::string>{});
+    auto engine = std::make_shared<std::mt19937_64>(1);
+    rm_type rm(engine, fixtures, true);
+    NfoldData<config_type, BasicStatistics, std::mt19937_64> data;
+    ising_cpp::IsingNfoldEventSelector<std::mt19937_64> selector(0.1, engine);
+    OccEvent e;
+    try {
+      nfold<Index>(state, ol, data, selector, [&](Index) -> OccEvent const & { return e; }, rm);
+    } catch (std::runtime_error const &err) {
+      return std::string(err.what());
+    }
+    return std::string();
+  };
+  EXPECT(attempt(SAMPLE_MODE::BY_STEP, false, 0).find("BY_PASS") != std::string::npos,
+         "nfold rejects BY_STEP fixtures");
+  EXPECT(attempt(SAMPLE_MODE::BY_PASS, true, 0).find("max_time") != std::string::npos,
+         "nfold rejects simulated-time cutoffs");
+  EXPECT(attempt(SAMPLE_MODE::BY_PASS, false, -1).find("0/1") != std::string::npos,
+         "nfold rejects spin configurations");
+  std::cout << "cpu: nfold argument checks" << (g_fail ? " (failures so far)" : " ok")
+            << std::endl;
+}
+
 static void gpu_tests() {
   // the reference's SemiGrandCanonicalRun1 set-up, converged on two observables
   gpu_case({25, 25, true, 2000.0, 0.0, 42, 1, -1, 0.001, 100, false});
@@ -622,6 +669,7 @@ int main(int argc, char **argv) {
     if (cpu) {
       cpu_tests();
       cpu_canonical_tests();
+      cpu_nfold_argument_tests();
     }
     if (gpu) gpu_tests();
   } catch (std::exception const &e) {
