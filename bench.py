@@ -376,7 +376,13 @@ def main():
           file=sys.stderr, flush=True)
     value = ex["attempts"] / ex["wall_s"]
     bytes_per_attempt = 6.0  # own read + 4 neighbour reads + write, int8 lattice in LDS
-    achieved = bytes_per_attempt * ex["attempts"] / ex["kernel_s"] / 1e9
+    lds_achieved = bytes_per_attempt * ex["attempts"] / ex["kernel_s"] / 1e9
+    # HBM bytes one launch must move: per chain the int8 lattice in and out,
+    # the engine/counter state (2600 B) in and out, the parameters (~220 B) in,
+    # and one 64 B record per pass
+    n_sites = 128 * 128
+    hbm_bytes = ex["chains"] * (2 * n_sites + 2 * 2600 + 220 + 64 * K)
+    achieved = hbm_bytes / ex["kernel_s"] / 1e9
     line = {
         "metric": "Metropolis site-attempts/s, 1 MI355X, batched vs plain; p50/p99 run latency",
         "value": value,
@@ -395,10 +401,14 @@ def main():
                                "MT19937-64, FP64 running energy, one chain per wavefront",
                    "chains": 64, "sites_per_chain": 16384, "passes": K,
                    "parallelism": "1 GPU, 64 workgroups (one 64-lane wavefront per chain)"},
-        "roofline": {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / LDS_PEAK_GBS, "traffic": None,
-                     "note": "exact mode is a dependent scalar walk per chain "
-                             "(latency bound); 6 B of LDS lattice traffic per attempt"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": hbm_bytes,
+                     "note": "exact mode is a dependent walk per chain held in LDS; neither "
+                             "HBM nor MFMA bounds it (latency of one chain per wavefront); "
+                             "HBM carries only the lattice, engine state and records",
+                     "lds": {"achieved": lds_achieved, "peak": LDS_PEAK_GBS,
+                             "frac": lds_achieved / LDS_PEAK_GBS, "bytes_per_attempt": 6}},
         "latency_s": {"p50": ex["wall_s"], "p99": ex["wall_s"],
                       "note": "all 64 runs complete in the same launch"},
         "plain": {"attempts_per_s": ex["plain"]["attempts_per_s"],
