@@ -220,6 +220,10 @@ class Run:
         _check(lib().mcg_rng_load(self.h, c, w.ctypes.data_as(C.POINTER(C.c_uint64)), w.size))
 
     def run(self, n_passes):
+        """Run n_passes passes of every active chain.  NFOLD sampled by pass:
+        run(0) selects the next event of each chain that has none pending and
+        records the current pass (its state, the event's time increment as
+        weight, the total rate in e_run); the next run applies that event first."""
         _check(lib().mcg_run(self.h, int(n_passes)))
 
     def assign(self, c, chain):

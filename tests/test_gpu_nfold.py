@@ -114,7 +114,7 @@ def test_nfold_zero_pass_run_takes_the_start_sample():
     _compare(r, 0, o0)
     _compare_pending(r, 0, o0)
     assert o0["pending_site"] >= 0
-    r.run(0)  # no further sample: the chain has one already
+    r.run(0)  # no further sample: the first event is already pending
     assert r.n_samples(0) == 1
     r.run(5)
     o = O.nfold_run(rows, cols, J, T, mu, 5, occ, O.mt64_state(seed, 0), sample_start=True)
