@@ -599,6 +599,78 @@ static void gpu_nfold_case(bool converge_stop) {
             << (g_fail ? " (failures so far)" : " ok") << std::endl;
 }
 
+// Two fixtures sampling every pass and every 4th pass from count 0, both cut
+// off at count 30: each holds the oracle's samples at its own counts (the
+// device records only the passes where some fixture is due).
+static void gpu_nfold_two_fixtures() {
+  const int R = 6, C = 9, N = R * C;
+  const long P = 30;
+  const double T = 2200.0, J = 0.1, mu = -0.1;
+  const uint64_t seed = 61;
+  typedef ising_cpp::IsingConfiguration config_type;
+  typedef RunManager<config_type, BasicStatistics, std::mt19937_64> rm_type;
+  std::vector<int> occ0 = random_occupants(N, 20, 17);
+  config_type cfg({R, C}, 0);
+  cfg.set_occupation(occ0);
+  ValueMap cond;
+  cond.scalar_values["temperature"] = T;
+  cond.vector_values["exchange_potential"] = VectorXd{mu};
+  State<config_type> state(cfg, cond);
+  ising_cpp::BinaryLatticeEvents ev(N);
+  OccLocation ol(ev.convert, ev.candidate_list);
+  ol.initialize(state.configuration.occupation());
+  NfoldData<config_type, BasicStatistics, std::mt19937_64> nfold_data;
+  nfold_data.n_events_possible = N;
+  StateSamplingFunctionMap fns;
+  StateSamplingFunction m("spin_sum", "sum of spins", {}, [&state]() {
+    double v = 0;
+    for (int o : state.configuration.occupation()) v += 2 * o - 1;
+    return VectorXd{v};
+  });
+  fns.emplace(m.name, m);
+  std::vector<rm_type::sampling_fixture_params_type> fixtures;
+  for (double period : {1.0, 4.0}) {
+    SamplingParams sp;
+    sp.sampler_names = {"spin_sum"};
+    sp.begin = 0.0;
+    sp.period = period;
+    CompletionCheckParams<BasicStatistics> cc;
+    cc.cutoff_params.max_count = P;
+    fixtures.emplace_back(period == 1.0 ? "every" : "sparse", fns, jsonStateSamplingFunctionMap(),
+                          ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), sp, cc,
+                          std::vector<std::string>{});
+  }
+  auto engine = std::make_shared<std::mt19937_64>(seed);
+  rm_type rm(engine, fixtures, false);
+  ising_cpp::IsingNfoldEventSelector<std::mt19937_64> selector(J, engine);
+  OccEvent e;
+  nfold<Index>(state, ol, nfold_data, selector, [&](Index) -> OccEvent const & { return e; }, rm);
+
+  refcpu::IsingConfiguration oc(R, C, -1);
+  for (int l = 0; l < N; ++l) oc.occupation[l] = occ0[l] ? 1 : -1;
+  std::mt19937_64 e1(seed);
+  auto o1 = refcpu::nfold_run(oc, J, T, mu, e1, P - 1, 1, 0.0, true);
+  for (int fi = 0; fi < 2; ++fi) {
+    auto const &res = rm.sampling_fixtures[fi]->results();
+    const long step = fi == 0 ? 1 : 4, want_n = fi == 0 ? P : (P - 1) / 4 + 1;
+    VectorXd got = res.samplers.at("spin_sum")->component(0);
+    VectorXd w = res.sample_weight.component(0);
+    bool ok = got.size() == want_n && w.size() == want_n;
+    for (long i = 0; ok && i < want_n; ++i)
+      ok = res.sample_count[i] == i * step && got(i) == (double)o1.M[i * step] &&
+           close_rel(w(i), o1.weight[i * step], 1e-12);
+    EXPECT(ok, "nfold two fixtures: fixture " << fi << " samples (" << got.size() << ")");
+  }
+  std::mt19937_64 e2(seed);
+  auto o2 = refcpu::nfold_run(oc, J, T, mu, e2, P, P + 1, 0.0, true);
+  std::vector<int> want(N);
+  for (int l = 0; l < N; ++l) want[l] = o2.final_occ[l] > 0 ? 1 : 0;
+  EXPECT(state.configuration.occupation() == want, "nfold two fixtures: final configuration");
+  EXPECT(*engine == e2, "nfold two fixtures: engine state");
+  std::cout << "gpu: nfold two fixtures 6x9" << (g_fail ? " (failures so far)" : " ok")
+            << std::endl;
+}
+
 // nfold's argument checks run before any device call
 static void cpu_nfold_argument_tests() {
   typedef ising_cpp::IsingConfiguration config_type;
@@ -660,6 +732,7 @@ static void gpu_tests() {
   gpu_canonical_tests();
   gpu_nfold_case(false);
   gpu_nfold_case(true);
+  gpu_nfold_two_fixtures();
 }
 
 int main(int argc, char **argv) {
