@@ -65,6 +65,20 @@ def bench_exact(K, W):
     r.run(W)
     wall, kern_s, attempts = timed_run(r, K)
     out = dict(wall_s=wall, kernel_s=kern_s, attempts=attempts, chains=len(conds))
+    # host buffers across PCIe: the lattices in, then every chain's records and
+    # final lattice out (the run itself keeps its state in HBM)
+    t0 = time.perf_counter()
+    for c in range(len(conds)):
+        r.set_occupation(c, r.get_occupation(c))
+    t1 = time.perf_counter()
+    n_rec = 0
+    for c in range(len(conds)):
+        n_rec += len(r.samples(c))
+        r.get_occupation(c)
+    t2 = time.perf_counter()
+    out["pcie"] = dict(upload_s=t1 - t0, download_s=t2 - t1, records=n_rec,
+                       attempts_per_s_pcie_inclusive=attempts / (wall + (t1 - t0) + (t2 - t1)),
+                       note="upload timed as a lattice round trip per chain (an upper bound)")
     r.close()
     # plain: one chain, same kernel, fewer passes (rate is per-chain latency bound)
     p = cm.Run(conds[:1], mode=cm.MODE_EXACT, J1=0.1, use_nlist=True)
@@ -415,6 +429,7 @@ def main():
                   "batched_over_plain": value / ex["plain"]["attempts_per_s"],
                   "passes": ex["plain"]["passes"]},
         "exact_full_chip": ex["full_chip"],
+        "pcie_inclusive": ex["pcie"],
     }
     if not args.skip_extra:
         scan = bench_checker_scan(200)
