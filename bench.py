@@ -416,7 +416,10 @@ def main():
                    "chains": 64, "sites_per_chain": 16384, "passes": K,
                    "parallelism": "1 GPU, 64 workgroups (one 64-lane wavefront per chain)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 of the 2000-pass launch
+                     # (profiles/exact_block_kernel_64x128sq_2000passes_traffic.md)
+                     "traffic": 12881920 if K == 2000 else None,
                      "bytes_per_launch": hbm_bytes,
                      "note": "exact mode is a dependent walk per chain held in LDS; neither "
                              "HBM nor MFMA bounds it (latency of one chain per wavefront); "
