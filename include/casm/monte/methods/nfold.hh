@@ -172,8 +172,10 @@ void nfold(State<ising_cpp::IsingConfiguration> &state, OccLocation &occ_locatio
       time_increment = cur.weight;
       sync_configuration();
       run_manager.sample_data_by_count_if_due(state, pre_sample_action);
-      // the check after the first event of this pass (nfold.hh:118, :132-141)
-      if (run_manager.is_complete()) {
+      // the check after the first event of this pass (nfold.hh:118, :132-141);
+      // with one site that event ends the pass, and the check at the next
+      // pass boundary (the loop's own) is that check
+      if (N > 1 && run_manager.is_complete()) {
         double t = 0.0, dt = 0.0;
         int64_t site = -1;
         dev.nfold_state(0, t, site, dt);
