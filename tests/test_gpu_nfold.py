@@ -123,6 +123,24 @@ def test_nfold_zero_pass_run_takes_the_start_sample():
     r.close()
 
 
+def test_nfold_record_on_demand():
+    """With no periodic records, mcg_run(h, 0) after pass 3 records that pass
+    and selects its first event (a second call adds nothing); the trajectory is
+    the oracle's with a sample at pass 3 only."""
+    rows, cols, seed, T, mu = 7, 8, 24, 2100.0, 0.1
+    occ = np.where(np.random.default_rng(seed).random(rows * cols) < 0.4, 1, -1).astype(np.int8)
+    o = O.nfold_run(rows, cols, J, T, mu, 5, occ, O.mt64_state(seed, 0), sample_period=3)
+    assert len(o["S"]) == 1
+    r = _run_gpu([dict(shape=(rows, cols), temperature=T, exchange_potential=mu, seed=seed)],
+                 [occ], 0, chunks=[3, 0, 0, 2], sample_period=1 << 40)
+    rec = r.samples(0)
+    assert len(rec) == 1 and rec["pass"][0] == 3
+    _compare(r, 0, o)
+    _compare_pending(r, 0, o)
+    assert r.counters(0) == (5, 5 * rows * cols, 0)
+    r.close()
+
+
 def test_nfold_time_grid_matches_oracle():
     rows, cols, T, mu, seed, passes = 12, 10, 1600.0, 0.05, 21, 30
     occ = np.ones(rows * cols, dtype=np.int8)
