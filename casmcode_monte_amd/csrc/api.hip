@@ -47,6 +47,7 @@ constexpr int64_t LDS_LIMIT = 160 * 1024;     // gfx950 LDS per workgroup
 constexpr int64_t EXACT_LDS_MAX = 144 * 1024;  // lattice + fold scratch (+ ~10 KB static)
 constexpr int64_t CHECKER_LDS_MAX = 128 * 1024;
 constexpr int64_t FCC_LDS_MAX = 142 * 1024;  // lattice; tables and lists take 17 KB static
+constexpr int CHECKER_BAND_MAX_L0 = 8192;     // longest line of the 2D band path
 
 // libstdc++ canonical double of a word compared to prob: the first word whose
 // canonical value is >= prob (2^64 if none), by bisection on the monotone map.
@@ -285,6 +286,7 @@ struct mcg_handle {
   uint32_t *d_tab = nullptr;
   std::vector<CheckerParams> cp;
   int32_t *d_work = nullptr;
+  int64_t work_cap = 0, band_cap = 0, pair_cap = 0;  // int32 entries allocated
   int n_work = 0;
   bool work_dirty = true;
   bool checker_lds = true;
@@ -694,6 +696,12 @@ int build_checker(mcg_handle *h) {
   h->checker_lds = max_n <= CHECKER_LDS_MAX && !(h->options & MCG_OPT_FORCE_STREAMING);
   h->checker_lds_bytes = (int)((max_n + 15) & ~15);
   HIP_TRY(checker_set_max_lds((int)LDS_LIMIT - 4096));
+  // the HBM path's kernels get their largest dynamic LDS once, here: an
+  // 8-line ring of the longest band line (L0 <= 8192) and the largest line
+  // tile; rebuild_work only chooses what a launch uses, so handles driven from
+  // different host threads never change a kernel attribute under each other
+  HIP_TRY(checker_band_set_max_lds(checker_band_ring_lines() * CHECKER_BAND_MAX_L0));
+  HIP_TRY(checker_pair_set_max_lds(checker_tile_max_bytes()));
   HIP_TRY(hipMalloc(&h->d_cp, sizeof(CheckerParams) * h->n_chains));
   HIP_TRY(hipMalloc(&h->d_cs, sizeof(CheckerState) * h->n_chains));
   HIP_TRY(hipMalloc(&h->d_tab, sizeof(uint32_t) * tabs.size()));
@@ -823,6 +831,24 @@ int reset_observables(mcg_handle *h, int64_t c, std::vector<int8_t> const &occ) 
   return MCG_OK;
 }
 
+// Copy a work list to its device buffer, growing the buffer only when the list
+// outgrows it (a retired or re-admitted chain changes the lists on every
+// active-mask change; the buffers keep their largest size so those changes
+// cost no hipFree, which would synchronise the whole device).
+int upload_list(mcg_handle *h, int32_t **buf, int64_t *cap, std::vector<int32_t> const &v) {
+  if ((int64_t)v.size() > *cap) {
+    int32_t *d = nullptr;
+    const int64_t want = std::max<int64_t>((int64_t)v.size(), 2 * *cap);
+    if (hipMalloc(&d, sizeof(int32_t) * want) != hipSuccess)
+      return fail(MCG_ERR_CAPACITY, "out of device memory for work lists");
+    if (*buf) (void)hipFree(*buf);
+    *buf = d;
+    *cap = want;
+  }
+  HIP_TRY(copy_sync(h, *buf, v.data(), sizeof(int32_t) * v.size(), hipMemcpyHostToDevice));
+  return MCG_OK;
+}
+
 int rebuild_work(mcg_handle *h) {
   const int upi = checker_units_per_item();
   std::vector<int32_t> work;
@@ -842,18 +868,13 @@ int rebuild_work(mcg_handle *h) {
       work.push_back((int32_t)it);
     }
   }
-  if (h->d_work) hipFree(h->d_work);
-  h->d_work = nullptr;
   h->n_work = (int)(work.size() / 2);
   if (h->n_work) {
-    HIP_TRY(hipMalloc(&h->d_work, sizeof(int32_t) * work.size()));
-    HIP_TRY(copy_sync(h, h->d_work, work.data(), sizeof(int32_t) * work.size(),
-                      hipMemcpyHostToDevice));
+    int rc = upload_list(h, &h->d_work, &h->work_cap, work);
+    if (rc != MCG_OK) return rc;
   }
   // 2D band path (checker.hip): every active chain square, 16-site chunks,
   // lines short enough for the 8-line ring
-  if (h->d_band) hipFree(h->d_band);
-  h->d_band = nullptr;
   h->n_bands = 0;
   h->band_ring_bytes = 0;
   bool band_ok = h->lattice == MCG_LATTICE_SQUARE && h->band_lines != 0 && !h->fcc() &&
@@ -862,7 +883,7 @@ int rebuild_work(mcg_handle *h) {
   for (int c = 0; c < h->n_chains && band_ok; ++c) {
     if (!h->active[c]) continue;
     auto const &P = h->cp[c];
-    if (P.L0 % 32 != 0 || P.L0 > 8192) band_ok = false;
+    if (P.L0 % 32 != 0 || P.L0 > CHECKER_BAND_MAX_L0) band_ok = false;
     maxL0 = std::max<int64_t>(maxL0, P.L0);
     total_lines += P.L1;
   }
@@ -886,16 +907,12 @@ int rebuild_work(mcg_handle *h) {
     }
     h->n_bands = (int)(band.size() / 3);
     h->band_ring_bytes = (int)(checker_band_ring_lines() * maxL0);
-    HIP_TRY(checker_band_set_max_lds(h->band_ring_bytes));
-    HIP_TRY(hipMalloc(&h->d_band, sizeof(int32_t) * band.size()));
-    HIP_TRY(copy_sync(h, h->d_band, band.data(), sizeof(int32_t) * band.size(),
-                      hipMemcpyHostToDevice));
+    int rc = upload_list(h, &h->d_band, &h->band_cap, band);
+    if (rc != MCG_OK) return rc;
   }
   // line-class path: whole colour-class lines per workgroup (used when the
   // band path is not); LDS tiles of class lines with their neighbour lines on
   // the 16-site chunk path, else lines straight from HBM
-  if (h->d_pair) hipFree(h->d_pair);
-  h->d_pair = nullptr;
   h->n_pairs = 0;
   h->pair_lds = 0;
   if (h->n_bands == 0 && (h->ncolor == 4 || h->ncolor == 8) && !h->fcc()) {
@@ -960,14 +977,12 @@ int rebuild_work(mcg_handle *h) {
     }
     h->n_pairs = (int)(items.size() / 4);
     if (h->pair_lds > 0) {
-      HIP_TRY(checker_pair_set_max_lds(h->pair_lds));
       const int per_cu = std::max(1, std::min(8, (int)(LDS_LIMIT / (h->pair_lds + 1024))));
       h->tile_grid = std::min(h->n_pairs, h->n_cu * per_cu);
     }
     if (h->n_pairs) {
-      HIP_TRY(hipMalloc(&h->d_pair, sizeof(int32_t) * items.size()));
-      HIP_TRY(copy_sync(h, h->d_pair, items.data(), sizeof(int32_t) * items.size(),
-                        hipMemcpyHostToDevice));
+      int rc = upload_list(h, &h->d_pair, &h->pair_cap, items);
+      if (rc != MCG_OK) return rc;
     }
   }
   h->work_dirty = false;
@@ -1675,6 +1690,21 @@ int mcg_clear_samples(mcg_handle *h) {
   return MCG_OK;
 }
 
+// Grow the statistics scratch to `need` doubles.  The new buffer is allocated
+// before the old one is released, and the handle's pointer and capacity change
+// together, so a failed allocation leaves the previous (valid) buffer in place.
+static int ensure_stats(mcg_handle *h, int64_t need) {
+  if (h->d_stats && need <= h->stats_cap) return MCG_OK;
+  const int64_t cap = std::max<int64_t>(need, 2 * h->stats_cap);
+  double *d = nullptr;
+  if (hipMalloc(&d, sizeof(double) * cap) != hipSuccess)
+    return fail(MCG_ERR_CAPACITY, "out of device memory for the statistics scratch");
+  if (h->d_stats) (void)hipFree(h->d_stats);
+  h->d_stats = d;
+  h->stats_cap = cap;
+  return MCG_OK;
+}
+
 static ObsParams obs_params(mcg_handle *h, int64_t chain, int64_t observable) {
   ObsParams op;
   op.n_sites = h->sites(chain);
@@ -1721,11 +1751,9 @@ int mcg_stats_weighted(mcg_handle *h, int64_t chain, int64_t observable, int64_t
   const int64_t count = h->n_samples[chain] - first;
   if (first < 0 || count <= 0)
     return fail(MCG_ERR_ARG, "Error in BasicStatisticsCalculator: observations.size()==0");
-  if (n_resamples + 8 > h->stats_cap) {
-    if (h->d_stats) hipFree(h->d_stats);
-    h->d_stats = nullptr;
-    h->stats_cap = std::max<int64_t>(n_resamples + 8, 2 * h->stats_cap);
-    HIP_TRY(hipMalloc(&h->d_stats, sizeof(double) * h->stats_cap));
+  {
+    int rc = ensure_stats(h, n_resamples + 8);
+    if (rc != MCG_OK) return rc;
   }
   launch_weighted_stats(h->d_series + chain * h->series_cap, first, count,
                         obs_params(h, chain, observable), (int)method, n_resamples, h->d_stats,
@@ -1753,11 +1781,9 @@ int mcg_stats_compute(mcg_handle *h, int64_t chain, int64_t observable, int64_t 
   const int64_t count = h->n_samples[chain] - first;
   if (first < 0 || count <= 0)
     return fail(MCG_ERR_ARG, "Error in BasicStatisticsCalculator: observations.size()==0");
-  if (count + 4 > h->stats_cap) {
-    if (h->d_stats) hipFree(h->d_stats);
-    h->d_stats = nullptr;
-    h->stats_cap = std::max<int64_t>(count + 4, 2 * h->stats_cap);
-    HIP_TRY(hipMalloc(&h->d_stats, sizeof(double) * h->stats_cap));
+  {
+    int rc = ensure_stats(h, count + 4);
+    if (rc != MCG_OK) return rc;
   }
   launch_stats(h->d_series + chain * h->series_cap, first, count,
                obs_params(h, chain, observable), h->d_stats, h->stream);

@@ -175,20 +175,26 @@ void nfold(State<ising_cpp::IsingConfiguration> &state, OccLocation &occ_locatio
       // the check after the first event of this pass (nfold.hh:118, :132-141);
       // with one site that event ends the pass, and the check at the next
       // pass boundary (the loop's own) is that check
-      if (N > 1 && run_manager.is_complete()) {
+      if (N > 1) {
         double t = 0.0, dt = 0.0;
         int64_t site = -1;
         dev.nfold_state(0, t, site, dt);
         if (site < 0) throw std::runtime_error("nfold (MI355X): no pending event at a sample");
-        std::vector<int> occ(config.occupation().begin(), config.occupation().end());
-        occ_location.initialize(occ);
-        run_manager.increment_n_accept();
-        occ_location.apply(get_event_f(static_cast<EventIDType>(site)), occ);
-        config.set_occupation(occ);
+        // that check sees the time after the event (nfold.hh:137-144: the
+        // event time is set before the next loop test); time cutoffs are
+        // refused above, so only the results' recorded time depends on it
         run_manager.set_time(t + dt);
-        run_manager.increment_step();
-        stopped_in_pass = true;
-        break;
+        if (run_manager.is_complete()) {
+          std::vector<int> occ(config.occupation().begin(), config.occupation().end());
+          occ_location.initialize(occ);
+          run_manager.increment_n_accept();
+          occ_location.apply(get_event_f(static_cast<EventIDType>(site)), occ);
+          config.set_occupation(occ);
+          run_manager.increment_step();
+          stopped_in_pass = true;
+          break;
+        }
+        run_manager.set_time(t);
       }
     }
     // the remaining checks of this pass only repeat periodic checks on the same samples
@@ -196,7 +202,10 @@ void nfold(State<ising_cpp::IsingConfiguration> &state, OccLocation &occ_locatio
     if (run_manager.sampling_fixtures.empty()) break;
 
     CountType target = p + 100000;
-    if (pending_checks()) target = p + 1;
+    // one site: the check after this pass's (single) event is the check at the
+    // next pass boundary, and the sample just taken may already complete the
+    // run (max_sample, or min_sample met with a check due), so run one pass
+    if (pending_checks() || (due && N == 1)) target = p + 1;
     for (auto &f : run_manager.sampling_fixtures) {
       target = std::min(target, std::max(f->next_sample_count(), p + 1));
       auto const &cp = f->params().completion_check_params.cutoff_params;

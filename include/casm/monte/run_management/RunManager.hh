@@ -277,8 +277,14 @@ class SamplingFixture {
   bool is_complete() {
     if (m_is_complete) return true;
     Log &log = m_params.method_log.log;
-    m_is_complete = m_completion_check.is_complete(m_results.samplers, m_results.sample_weight,
-                                                   m_counter.count, log);
+    // SamplingFixture.hh:271-279: fixtures that sample time also hand the
+    // counter's time to the check (its results then record it)
+    if (m_params.sampling_params.do_sample_time)
+      m_is_complete = m_completion_check.is_complete(m_results.samplers, m_results.sample_weight,
+                                                     m_counter.count, m_counter.time, log);
+    else
+      m_is_complete = m_completion_check.is_complete(m_results.samplers, m_results.sample_weight,
+                                                     m_counter.count, log);
     return m_is_complete;
   }
 

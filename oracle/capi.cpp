@@ -218,6 +218,65 @@ double orc_autocorrelation_factor(const double *x, int64_t n) {
   return autocorrelation_factor(std::vector<double>(x, x + n));
 }
 
+// autocorrelation_factor with the resampling increment of weighted
+// observations (BasicStatistics.cc:24-48, rho = 2^(-1/(lag * increment)))
+double orc_autocorrelation_factor_inc(const double *x, int64_t n, double increment) {
+  return autocorrelation_factor(std::vector<double>(x, x + n), increment);
+}
+
+// The loop of python/tests/sampling/test_CompletionCheck.py:40-124 on the
+// restated CompletionCheck: min_sample cutoff, absolute precisions for the
+// scalar samplers "e" and "v", a count check every step, one sample of each
+// every 10 steps with e = e_mean + random_real(e_amp) - e_amp/2 and likewise
+// v (both from one RandomNumberGenerator over std::mt19937_64(seed)).
+// out_i: n_steps, n_samples, is_complete, all_equilibrated, n_equilibration
+// results, all_converged, n_convergence results, N_samples_for_statistics;
+// out_d: calculated_precision of e and v, mean of e and v.
+void orc_completion_check_replay(uint64_t seed, int64_t min_sample, double e_mean, double e_amp,
+                                 double e_abs, double v_mean, double v_amp, double v_abs,
+                                 int64_t *out_i, double *out_d) {
+  CompletionCheckParams params;
+  params.cutoff_params.min_sample = min_sample;
+  SamplerComponent e_key{"e", 0}, v_key{"v", 0};
+  RequestedPrecision rp_e, rp_v;
+  rp_e.abs_convergence_is_required = true;
+  rp_e.abs_precision = e_abs;
+  rp_v.abs_convergence_is_required = true;
+  rp_v.abs_precision = v_abs;
+  params.requested_precision[e_key] = rp_e;
+  params.requested_precision[v_key] = rp_v;
+  CompletionCheck check(params);
+  SamplerMap samplers;
+  samplers["e"] = std::make_shared<Sampler>(1);
+  samplers["v"] = std::make_shared<Sampler>(1);
+  RandomNumberGenerator rng(std::make_shared<std::mt19937_64>(seed));
+  CountType n_steps = 0;
+  while (!check.is_complete(samplers, n_steps)) {
+    n_steps += 1;
+    const double e = e_mean + rng.random_real(e_amp) - e_amp / 2.0;
+    const double v = v_mean + rng.random_real(v_amp) - v_amp / 2.0;
+    if (n_steps % 10 == 0) {
+      samplers["e"]->push_back({e});
+      samplers["v"]->push_back({v});
+    }
+  }
+  CompletionCheckResults const &r = check.results();
+  auto const &conv = r.convergence_check_results;
+  out_i[0] = n_steps;
+  out_i[1] = get_n_samples(samplers);
+  out_i[2] = r.is_complete;
+  out_i[3] = r.equilibration_check_results.all_equilibrated;
+  out_i[4] = (int64_t)r.equilibration_check_results.individual_results.size();
+  out_i[5] = conv.all_converged;
+  out_i[6] = (int64_t)conv.individual_results.size();
+  out_i[7] = conv.N_samples_for_statistics;
+  for (int k = 0; k < 2; ++k) {
+    auto it = conv.individual_results.find(k == 0 ? e_key : v_key);
+    out_d[k] = it == conv.individual_results.end() ? -1.0 : it->second.stats.calculated_precision;
+    out_d[2 + k] = it == conv.individual_results.end() ? 0.0 : it->second.stats.mean;
+  }
+}
+
 void orc_equilibration(const double *x, int64_t n, double prec, int64_t *is_eq,
                        int64_t *n_eq) {
   EquilibrationResult r = default_equilibration_check_impl(std::vector<double>(x, x + n), prec);

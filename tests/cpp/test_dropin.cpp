@@ -581,6 +581,10 @@ static void gpu_nfold_case(bool converge_stop) {
     EXPECT(res.n_accept == (BigCountType)c_stop * N + 1 && res.n_reject == 0,
            tag << ": event count " << res.n_accept);
     EXPECT(close_rel(f.counter().time, o1.final_time + o1.pending_dt, 1e-12), tag << ": time");
+    // the stopping check saw the time after that event (SamplingFixture.hh:271-279)
+    auto const &ct = f.completion_check().results().time;
+    EXPECT(ct.has_value() && close_rel(*ct, o1.final_time + o1.pending_dt, 1e-12),
+           tag << ": completion check time");
   } else {
     EXPECT(n_samp == P && c_stop == P && f.counter().step == 0, tag << ": stops at count P");
     std::mt19937_64 e2(seed);
@@ -591,12 +595,83 @@ static void gpu_nfold_case(bool converge_stop) {
     EXPECT(*engine == e2, tag << ": engine state (no selection at count P)");
     EXPECT(res.n_accept == (BigCountType)P * N && res.n_reject == 0, tag << ": event count");
     EXPECT(close_rel(f.counter().time, o2.final_time, 1e-12), tag << ": time");
+    auto const &ct = f.completion_check().results().time;
+    EXPECT(ct.has_value() && close_rel(*ct, o2.final_time, 1e-12),
+           tag << ": completion check time");
   }
   EXPECT(*rm_engine == std::mt19937_64(seed + 1000), tag << ": run manager engine untouched");
   EXPECT(occ_location_consistent(ol, state.configuration.occupation()),
          tag << ": OccLocation invariants after the run");
   std::cout << "gpu: " << tag << " 10x12 (" << n_samp << " samples)"
             << (g_fail ? " (failures so far)" : " ok") << std::endl;
+}
+
+// One site (N = 1), a fixture sampling every 3rd pass from count 0 with
+// max_sample = 1: the reference samples count 0 (selecting the event), applies
+// that event (which ends pass 0), and its check at count 1 meets max_sample.
+// The run must stop there: count 1, one event, the engine after one selection.
+static void gpu_nfold_one_site_max_sample() {
+  const double T = 1500.0, J = 0.1, mu = 0.02;
+  const uint64_t seed = 71;
+  typedef ising_cpp::IsingConfiguration config_type;
+  typedef RunManager<config_type, BasicStatistics, std::mt19937_64> rm_type;
+  config_type cfg({1, 1}, 0);
+  ValueMap cond;
+  cond.scalar_values["temperature"] = T;
+  cond.vector_values["exchange_potential"] = VectorXd{mu};
+  State<config_type> state(cfg, cond);
+  ising_cpp::BinaryLatticeEvents ev(1);
+  OccLocation ol(ev.convert, ev.candidate_list);
+  ol.initialize(state.configuration.occupation());
+  NfoldData<config_type, BasicStatistics, std::mt19937_64> nfold_data;
+  nfold_data.n_events_possible = 1;
+  StateSamplingFunctionMap fns;
+  StateSamplingFunction m("spin_sum", "sum of spins", {}, [&state]() {
+    return VectorXd{2.0 * state.configuration.occ(0) - 1.0};
+  });
+  fns.emplace(m.name, m);
+  SamplingParams sp;
+  sp.sampler_names = {"spin_sum"};
+  sp.begin = 0.0;
+  sp.period = 3.0;
+  sp.do_sample_time = true;
+  CompletionCheckParams<BasicStatistics> cc;
+  cc.cutoff_params.max_sample = 1;
+  std::vector<rm_type::sampling_fixture_params_type> fixtures;
+  fixtures.emplace_back("one", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), sp, cc,
+                        std::vector<std::string>{});
+  rm_type rm(std::make_shared<std::mt19937_64>(seed + 1), fixtures, true);
+  auto engine = std::make_shared<std::mt19937_64>(seed);
+  ising_cpp::IsingNfoldEventSelector<std::mt19937_64> selector(J, engine);
+  OccEvent flip;
+  flip.occ_transform.resize(1);
+  auto get_event_f = [&](Index l) -> OccEvent const & {
+    Mol const &mol = ol.mol(ol.l_to_mol_id(l));
+    flip.linear_site_index = {l};
+    flip.new_occ = {1 - state.configuration.occ(l)};
+    flip.occ_transform[0] = OccTransform{l, mol.id, mol.asym, mol.species_index,
+                                         1 - mol.species_index};
+    return flip;
+  };
+  nfold<Index>(state, ol, nfold_data, selector, get_event_f, rm);
+  auto const &f = *rm.sampling_fixtures[0];
+  auto const &res = f.results();
+  // oracle: the count-0 sample, then one pass (its one event) and no sample at count 1
+  refcpu::IsingConfiguration oc(1, 1, -1);
+  std::mt19937_64 e1(seed);
+  auto o = refcpu::nfold_run(oc, J, T, mu, e1, 1, 3, 0.0, true);
+  const std::string tag = "nfold one site max_sample 1";
+  EXPECT(res.samplers.at("spin_sum")->n_samples() == 1 && o.M.size() == 1 &&
+             res.samplers.at("spin_sum")->component(0)(0) == (double)o.M[0],
+         tag << ": the count-0 sample");
+  EXPECT(f.counter().count == 1 && f.counter().step == 0,
+         tag << ": stops at count 1 (count " << f.counter().count << ")");
+  EXPECT(res.n_accept == 1, tag << ": one event (" << res.n_accept << ")");
+  EXPECT(state.configuration.occ(0) == (o.final_occ[0] > 0 ? 1 : 0), tag << ": configuration");
+  EXPECT(*engine == e1, tag << ": engine after one selection");
+  EXPECT(close_rel(f.counter().time, o.final_time, 1e-12), tag << ": time");
+  std::cout << "gpu: " << tag << (g_fail ? " (failures so far)" : " ok") << std::endl;
 }
 
 // Two fixtures sampling every pass and every 4th pass from count 0, both cut
@@ -733,6 +808,7 @@ static void gpu_tests() {
   gpu_nfold_case(false);
   gpu_nfold_case(true);
   gpu_nfold_two_fixtures();
+  gpu_nfold_one_site_max_sample();
 }
 
 int main(int argc, char **argv) {

@@ -88,6 +88,12 @@ def _declare(L):
     L.orc_resample.restype = None
     L.orc_autocorrelation_factor.argtypes = [C.POINTER(C.c_double), C.c_int64]
     L.orc_autocorrelation_factor.restype = C.c_double
+    L.orc_autocorrelation_factor_inc.argtypes = [C.POINTER(C.c_double), C.c_int64, C.c_double]
+    L.orc_autocorrelation_factor_inc.restype = C.c_double
+    L.orc_completion_check_replay.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_double,
+                                              C.c_double, C.c_double, C.c_double, C.c_double,
+                                              C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    L.orc_completion_check_replay.restype = None
     L.orc_equilibration.argtypes = [C.POINTER(C.c_double), C.c_int64, C.c_double,
                                     C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.orc_approx_erf_inv.argtypes = [C.c_double]
@@ -192,9 +198,26 @@ def resample(x, w, w_sum, m):
     return out
 
 
-def autocorrelation_factor(x):
+def autocorrelation_factor(x, increment=None):
     x = np.ascontiguousarray(x, dtype=np.float64)
-    return lib().orc_autocorrelation_factor(_p(x, C.c_double), len(x))
+    if increment is None:
+        return lib().orc_autocorrelation_factor(_p(x, C.c_double), len(x))
+    return lib().orc_autocorrelation_factor_inc(_p(x, C.c_double), len(x), increment)
+
+
+def completion_check_replay(seed, min_sample=100, e_mean=1.0, e_amp=0.1, e_abs=0.001,
+                            v_mean=20.0, v_amp=1.0, v_abs=0.01):
+    """The loop of the reference's python/tests/sampling/test_CompletionCheck.py:40-124
+    on the oracle's CompletionCheck, with a seeded engine."""
+    oi = np.zeros(8, dtype=np.int64)
+    od = np.zeros(4)
+    lib().orc_completion_check_replay(seed, min_sample, e_mean, e_amp, e_abs, v_mean, v_amp,
+                                      v_abs, _p(oi, C.c_int64), _p(od, C.c_double))
+    keys = ("n_steps", "n_samples", "is_complete", "all_equilibrated", "n_equilibration_results",
+            "all_converged", "n_convergence_results", "N_samples_for_statistics")
+    out = {k: int(v) for k, v in zip(keys, oi)}
+    out.update(prec_e=od[0], prec_v=od[1], mean_e=od[2], mean_v=od[3])
+    return out
 
 
 def equilibration(x, prec):

@@ -11,8 +11,8 @@ here by closed forms:
   * method 1: mean = sum(x w) / W and variance = sum(w (x - mean)^2) / W;
     with unit weights both equal the unweighted ones;
   * the autocorrelation factor of method 1 is that of the resampled series
-    with rho = 2^(-1 / (lag * W / n)): the lag recovered from the unweighted
-    factor of the same series reproduces it;
+    with rho = 2^(-1 / (lag * W / n)), checked on weights whose resampled
+    series is a square wave with a hand-counted lag;
   * method 2 is the unweighted statistics of the resampled series;
   * time-grid records (weight = number of grid points) average like the
     expanded grid.
@@ -81,16 +81,24 @@ def test_unit_weights_reduce_to_unweighted():
 
 
 def test_increment_scales_the_lag():
-    x = _ar1(1500, 9)
-    w = np.random.default_rng(5).exponential(size=1500)
-    W, n = w.sum(), 10000
-    s = oracle.weighted_statistics(x, w, 0.95, 1, n)
-    eq = oracle.resample(x, w, W, n)
-    f1 = oracle.autocorrelation_factor(eq)  # increment 1: rho = 2^(-1/lag)
-    rho1 = (f1 - 1) / (f1 + 1)
-    lag = round(-1.0 / np.log2(rho1))
-    rho = 2.0 ** (-1.0 / (lag * (W / n)))
-    assert s["autocorr_factor"] == pytest.approx((1 + rho) / (1 - rho), rel=1e-12)
+    """Observations 11, 9 with weights 1, 1.5 (W = 2.5) read at 10 points i * 0.25
+    (all exact): points 0..4 lie in the first interval [0, 1] (the strict < keeps
+    i = 4, at 1.0, on it) and 5..9 in the second, so the resampled series is
+    10 + (+1 x 5, -1 x 5), whose first lag with |cov/var0| <= 0.5 is 2
+    (cov_1 = 7/9, cov_2 = 4/8).  With increment W/n = 0.25, rho = 2^(-1/(2*0.25))
+    = 1/4 and the factor is (1 + 1/4)/(1 - 1/4) = 5/3.  Weighted mean
+    (11 + 13.5)/2.5 = 9.8, variance (1.2^2 + 1.5 * 0.8^2)/2.5 = 0.96."""
+    x = np.array([11.0, 9.0])
+    w = np.array([1.0, 1.5])
+    eq = oracle.resample(x, w, 2.5, 10)
+    assert list(eq) == [11.0] * 5 + [9.0] * 5
+    s = oracle.weighted_statistics(x, w, 0.95, 1, 10)
+    assert s["autocorr_factor"] == pytest.approx(5.0 / 3.0, rel=1e-14)
+    assert s["mean"] == pytest.approx(9.8, rel=1e-15)
+    assert s["variance"] == pytest.approx(0.96, rel=1e-14)
+    fc = np.sqrt(2.0) * oracle.lib().orc_approx_erf_inv(0.95)
+    assert s["calculated_precision"] == pytest.approx(fc * np.sqrt(5.0 / 3.0 * 0.96 / 2.5),
+                                                      rel=1e-14)
 
 
 def test_method2_is_unweighted_statistics_of_resampled():
