@@ -311,3 +311,38 @@ def test_line_class_tiles_equal_other_paths(lattice, shape):
                     np.testing.assert_array_equal(other.get_occupation(c), r.get_occupation(c))
                     np.testing.assert_array_equal(other.samples(c), r.samples(c))
                     assert other.counters(c) == r.counters(c)
+
+
+@pytest.mark.parametrize("ndim,L,J2,T,mu", [
+    (2, (32, 32), -0.02, 3000.0, 0.05),
+    (2, (24, 24), 0.0, 3600.0, -0.1),
+    (3, (10, 10, 10), -0.02, 4200.0, 0.0),
+])
+def test_fluctuations_three_sigma_vs_oracle_sequential(ndim, L, J2, T, mu):
+    """Heat capacity C = N var(e) / (kB T^2) and susceptibility chi_x = N var(x) /
+    (kB T) of the checkerboard chain and of the oracle's sequential random-site
+    chain (the reference's CPU path) agree within 3 sigma, sigma from the
+    BasicStatistics precision of the squared deviations of each, combined in
+    quadrature."""
+    from casmcode_monte_amd import analysis
+    J1 = 0.1
+    N = int(np.prod(L))
+    passes, drop = 8000, 500
+    print("case: fluctuations 3sigma", ndim, L, J2, T, mu, flush=True)
+    _, S1, S2, M = O.lattice_run(ndim, L, J1, J2, T, mu, 21, passes, np.ones(N, dtype=np.int8))
+    orec = np.zeros(passes, dtype=cm.SAMPLE_DTYPE)
+    orec["S1"], orec["S2"], orec["M"] = S1, S2, M
+    r = run([dict(shape=L, temperature=T, exchange_potential=mu, seed=22)],
+            cm.LATTICE_SC if ndim == 3 else cm.LATTICE_SQUARE, J1, J2, passes)
+    grec = r.samples(0)
+    r.close()
+
+    def fluct(rec):
+        E, Nx = energy_terms(rec[drop:], J1, J2, mu, N)
+        return (analysis.heat_capacity_stats(E / N, N, T),
+                analysis.susceptibility_stats(Nx / N, N, T))
+
+    for (va, pa), (vb, pb) in zip(fluct(orec), fluct(grec)):
+        sigma = math.hypot(pa, pb) / 1.959964
+        assert va > 0 and vb > 0
+        assert abs(va - vb) < 3 * sigma, (va, vb, sigma)
