@@ -35,6 +35,39 @@ LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz = 78.6 TB/s (GB/
 HBM_PEAK_GBS = 8000.0
 
 
+CLOCK_GHZ = 2.4  # MI355X peak engine clock
+
+
+def measured_peaks():
+    """HBM streaming copy and LDS read bandwidth measured on this device
+    (casmcode_monte_amd/lib/libcasm_monte_probes.so)."""
+    import ctypes as C
+    path = os.path.join(ROOT, "casmcode_monte_amd", "lib", "libcasm_monte_probes.so")
+    L = C.CDLL(path)
+    L.mcgp_hbm_copy.argtypes = [C.c_int64, C.c_int64, C.POINTER(C.c_double),
+                                C.POINTER(C.c_double)]
+    L.mcgp_lds_read.argtypes = [C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.mcgp_last_error.restype = C.c_char_p
+    g, t = C.c_double(), C.c_double()
+    if L.mcgp_hbm_copy(1 << 30, 20, C.byref(g), C.byref(t)):
+        raise RuntimeError(L.mcgp_last_error().decode())
+    hbm = dict(gbs=g.value, ms_per_copy=t.value,
+               note="1 GiB device-to-device copy, 16-byte loads/stores, read+write bytes")
+    if L.mcgp_lds_read(20000, C.byref(g), C.byref(t)):
+        raise RuntimeError(L.mcgp_last_error().decode())
+    lds = dict(gbs=g.value, ms=t.value,
+               note="2 x 1024-thread workgroups per CU reading 16-byte LDS words, no conflicts")
+    return dict(hbm_copy=hbm, lds_read=lds, hbm_datasheet_gbs=HBM_PEAK_GBS,
+                lds_nominal_gbs=LDS_PEAK_GBS)
+
+
+# HBM traffic of one batched exact launch (64 chains of 128x128), FETCH_SIZE x
+# 2.0 + WRITE_SIZE x 1.0 of rocprofv3 --pmc at 20 and 2000 passes
+# (profiles/exact_block_kernel_64x128sq_traffic.md), as fixed + per-pass bytes
+EXACT_TRAFFIC_FIXED_B = 2636800.0
+EXACT_TRAFFIC_PER_PASS_B = 5122.56
+
+
 def c2_conditions(shape=(128, 128)):
     out = []
     for a in range(8):
@@ -58,13 +91,16 @@ def timed_run(run, passes):
     return wall, ms / 1e3, attempts
 
 
-def bench_exact(K, W):
+def bench_exact(K, W, extras=True):
     import casmcode_monte_amd as cm
     conds = c2_conditions()
     r = cm.Run(conds, mode=cm.MODE_EXACT, J1=0.1, use_nlist=True)
     r.run(W)
     wall, kern_s, attempts = timed_run(r, K)
     out = dict(wall_s=wall, kernel_s=kern_s, attempts=attempts, chains=len(conds))
+    if not extras:
+        r.close()
+        return out
     # host buffers across PCIe: the lattices in, then every chain's records and
     # final lattice out (the run itself keeps its state in HBM)
     t0 = time.perf_counter()
@@ -218,6 +254,7 @@ def bench_checker_scan(passes):
     conds = c2_conditions()
     b = cm.Run(conds, mode=cm.MODE_CHECKERBOARD, J1=0.1, J2=0.0)
     b.run(5)
+    sched = b.autotune(20)
     bw, bk, ba = timed_run(b, passes)
     b.close()
     plain_wall, plain_att = 0.0, 0.0
@@ -228,9 +265,15 @@ def bench_checker_scan(passes):
         plain_wall += w
         plain_att += a
         p.close()
+    lds_gbs = 6.0 * ba / bk / 1e9  # own read + 4 neighbour reads + write per attempt
     return dict(passes=passes, batched_attempts_per_s=ba / bw, batched_kernel_s=bk,
+                lds_block=sched["lds_block"],
                 plain_attempts_per_s=plain_att / plain_wall,
-                ratio=(ba / bw) / (plain_att / plain_wall))
+                ratio=(ba / bw) / (plain_att / plain_wall),
+                roofline={"bound": "lds", "achieved": lds_gbs, "peak": LDS_PEAK_GBS,
+                          "unit": "GB/s", "frac": lds_gbs / LDS_PEAK_GBS,
+                          "bytes_per_attempt": 6,
+                          "kernel": "checker_lds_kernel, 64 chains of 128x128 J1"})
 
 
 def bench_checker_family(shape, lattice, passes, chains=16):
@@ -250,7 +293,7 @@ def bench_checker_family(shape, lattice, passes, chains=16):
                 hbm_frac=2.0 * a / k / 1e9 / HBM_PEAK_GBS)
 
 
-def bench_fcc(passes):
+def bench_fcc(passes, plain=True):
     """Binary FCC cluster expansion: 256 chains of L = 16 + (i mod 17) conventional
     cells (16384-131072 sites), T ~ U(300, 2000) K, mu ~ U(-1, 1) eV, default ECIs,
     all in one autotuned launch; plain = one 32^3 chain alone."""
@@ -262,6 +305,9 @@ def bench_fcc(passes):
     sched = r.autotune(8)
     w, k, a = timed_run(r, passes)
     r.close()
+    if not plain:
+        return dict(chains=len(conds), passes=passes, kernel_s=k, attempts_per_s=a / w,
+                    lds_block=sched["lds_block"])
     p = cm.Run([conds[16]], mode=cm.MODE_CHECKERBOARD, lattice=cm.LATTICE_FCC,
                eci=fcc.DEFAULT_ECI)
     p.run(2)
@@ -280,7 +326,7 @@ def bench_fcc(passes):
 
 def bench_served(n_requests):
     """Served scan: a Poisson (50/s) stream of run requests, half 2D Ising (edges
-    log-uniform in [16, 512], rounded to even) and half FCC cluster expansion
+    log-uniform in [16, 512], odd edges with ring colourings) and half FCC cluster expansion
     (edges log-uniform in [8, 48] conventional cells), T in 300-3000 K, mu in
     -1..1 eV, each stopping at <x> precision 1e-3 or 20000 passes."""
     from casmcode_monte_amd import serving
@@ -363,6 +409,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--served-requests", type=int, default=400,
                     help="requests of the served scan (0 skips it)")
+    ap.add_argument("--only", default="",
+                    choices=["", "exact", "scan", "band4096", "sc256", "fcc", "peaks"],
+                    help="run one measurement and print its JSON (for profiling one kernel)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes 10 times "
                          "(the reference point that scales the FETCH_SIZE/WRITE_SIZE counters)")
@@ -385,18 +434,34 @@ def main():
     import casmcode_monte_amd as cm
 
     K, W = args.steps, args.warmup
+    if args.only:
+        import casmcode_monte_amd as cm
+        one = {"exact": lambda: bench_exact(K, W, extras=False),
+               "scan": lambda: bench_checker_scan(K),
+               "band4096": lambda: bench_checker_family((4096, 4096, 1), cm.LATTICE_SQUARE, K),
+               "sc256": lambda: bench_checker_family((256, 256, 256), cm.LATTICE_SC, K),
+               "fcc": lambda: bench_fcc(K, plain=False),
+               "peaks": measured_peaks}[args.only]()
+        print(json.dumps({"only": args.only, "steps": K, **one}))
+        return
     ex = bench_exact(K, W)
     print(f"[bench] exact batched done: {ex['attempts'] / ex['wall_s']:.3e} attempts/s",
           file=sys.stderr, flush=True)
     value = ex["attempts"] / ex["wall_s"]
-    bytes_per_attempt = 6.0  # own read + 4 neighbour reads + write, int8 lattice in LDS
+    # The exact walk is one dependent chain per wavefront with the lattice in
+    # LDS: neither HBM nor MFMA bounds it, the latency of each step does.  The
+    # roofline figure is the algorithmic LDS traffic (own read + 4 neighbour
+    # reads + write = 6 B per attempt) over the measured kernel time.
+    bytes_per_attempt = 6.0
     lds_achieved = bytes_per_attempt * ex["attempts"] / ex["kernel_s"] / 1e9
+    attempts_per_chain = ex["attempts"] / ex["chains"]
+    cycles_per_attempt = ex["kernel_s"] * CLOCK_GHZ * 1e9 / attempts_per_chain
     # HBM bytes one launch must move: per chain the int8 lattice in and out,
     # the engine/counter state (2600 B) in and out, the parameters (~220 B) in,
-    # and one 64 B record per pass
+    # and one 80 B record per pass; measured traffic from the fitted model
     n_sites = 128 * 128
-    hbm_bytes = ex["chains"] * (2 * n_sites + 2 * 2600 + 220 + 64 * K)
-    achieved = hbm_bytes / ex["kernel_s"] / 1e9
+    hbm_bytes = ex["chains"] * (2 * n_sites + 2 * 2600 + 220 + 80 * K)
+    traffic = EXACT_TRAFFIC_FIXED_B + EXACT_TRAFFIC_PER_PASS_B * K
     line = {
         "metric": "Metropolis site-attempts/s, 1 MI355X, batched vs plain; p50/p99 run latency",
         "value": value,
@@ -415,17 +480,20 @@ def main():
                                "MT19937-64, FP64 running energy, one chain per wavefront",
                    "chains": 64, "sites_per_chain": 16384, "passes": K,
                    "parallelism": "1 GPU, 64 workgroups (one 64-lane wavefront per chain)"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 of the 2000-pass launch
-                     # (profiles/exact_block_kernel_64x128sq_2000passes_traffic.md)
-                     "traffic": 12881920 if K == 2000 else None,
-                     "bytes_per_launch": hbm_bytes,
-                     "note": "exact mode is a dependent walk per chain held in LDS; neither "
-                             "HBM nor MFMA bounds it (latency of one chain per wavefront); "
-                             "HBM carries only the lattice, engine state and records",
-                     "lds": {"achieved": lds_achieved, "peak": LDS_PEAK_GBS,
-                             "frac": lds_achieved / LDS_PEAK_GBS, "bytes_per_attempt": 6}},
+        "roofline": {"bound": "latency", "achieved": lds_achieved, "peak": LDS_PEAK_GBS,
+                     "unit": "GB/s", "frac": lds_achieved / LDS_PEAK_GBS,
+                     "traffic": traffic,
+                     "kernel": "exact_block_kernel",
+                     "bytes_per_attempt": 6,
+                     "cycles_per_attempt_per_chain": cycles_per_attempt,
+                     "note": "a dependent walk per chain with the lattice in LDS: the latency "
+                             "of each step binds it; achieved = 6 B of LDS per attempt over "
+                             "the kernel time; traffic = HBM bytes per launch (rocprofv3 "
+                             "FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, fitted fixed + per-pass)",
+                     "hbm": {"algorithmic_bytes_per_launch": hbm_bytes,
+                             "achieved_gbs": hbm_bytes / ex["kernel_s"] / 1e9,
+                             "peak": HBM_PEAK_GBS,
+                             "traffic_over_algorithmic": traffic / hbm_bytes}},
         "latency_s": {"p50": ex["wall_s"], "p99": ex["wall_s"],
                       "note": "all 64 runs complete in the same launch"},
         "plain": {"attempts_per_s": ex["plain"]["attempts_per_s"],
@@ -475,6 +543,11 @@ def main():
             "unit": "GB/s", "frac": fam2["hbm_frac"], "traffic": None,
             "kernel": ("checker_band_kernel + checker_seam_kernel" if fam2["schedule"].get("band_lines")
                        else "checker_color_kernel") + ", 16 x 4096^2 J1+J2 (2 B/attempt algorithmic)"}
+    if not args.skip_extra:
+        line["measured_peaks"] = measured_peaks()
+        pk = line["measured_peaks"]
+        line["roofline"]["frac_of_measured_lds"] = lds_achieved / pk["lds_read"]["gbs"]
+        print("[bench] measured peaks done", file=sys.stderr, flush=True)
     if not args.skip_cpu:
         line["cpu_baseline"] = cpu_baseline(16, 1000)
         print("[bench] cpu baseline done", file=sys.stderr, flush=True)

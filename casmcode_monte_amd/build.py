@@ -17,8 +17,10 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libcasm_monte_mi355x.so")
-SOURCES = ["api.hip", "exact.hip", "canonical.hip", "nfold.hip", "checker.hip", "fcc.hip",
-           "stats.hip"]
+# measured-peak probes (not part of the Metropolis path; loaded by bench.py)
+PROBES_LIB = os.path.join(LIBDIR, "libcasm_monte_probes.so")
+SOURCES = ["api.hip", "exact.hip", "canonical.hip", "nfold.hip", "checker.hip",
+           "checker_ring.hip", "fcc.hip", "stats.hip"]
 HEADERS = ["internal.hh", "rng.hh", "walk.hh", "fcc.hh"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -52,8 +54,13 @@ def build(verbose=False):
             print(out)
         return obj
 
-    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+    with ThreadPoolExecutor(max_workers=len(SOURCES) + 1) as ex:
+        probes = ex.submit(compile_one, "probes.hip")
         objs = list(ex.map(compile_one, SOURCES))
+        probes_obj = probes.result()
+    ptmp = PROBES_LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", ptmp, probes_obj])
+    os.replace(ptmp, PROBES_LIB)
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
