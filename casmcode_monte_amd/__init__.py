@@ -41,7 +41,8 @@ EXPORTED = [
     "mcg_stats_weighted",
     "mcg_equilibration", "mcg_last_run_time", "mcg_assign_chain", "mcg_autotune",
     "mcg_get_schedule", "mcg_set_schedule", "mcg_reserve_samples", "mcg_get_occ_location",
-    "mcg_set_occ_location", "mcg_get_nfold_state",
+    "mcg_set_occ_location", "mcg_get_nfold_state", "mcg_run_steps", "mcg_get_position",
+    "mcg_check_batch",
 ]
 
 
@@ -73,6 +74,11 @@ class Schedule(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("mean", C.c_double), ("calculated_precision", C.c_double),
                 ("variance", C.c_double), ("autocorr_factor", C.c_double), ("n", C.c_int64)]
+
+
+class Check(C.Structure):
+    _fields_ = [("is_equilibrated", C.c_int64), ("n_equilibration", C.c_int64),
+                ("stats", Stats)]
 
 
 _lib = None
@@ -109,6 +115,11 @@ def lib():
                                         P(C.c_int64), P(C.c_int64)]
         L.mcg_stats_weighted.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_double,
                                          C.c_int64, C.c_int64, P(Stats)]
+        L.mcg_run_steps.argtypes = [C.c_void_p, C.c_int64]
+        L.mcg_check_batch.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), C.c_int64, C.c_double,
+                                      C.c_double, P(Check)]
+        L.mcg_get_position.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), P(C.c_int64),
+                                       P(C.c_double)]
         L.mcg_last_run_time.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
         L.mcg_assign_chain.argtypes = [C.c_void_p, C.c_int64, P(ChainDesc)]
         L.mcg_autotune.argtypes = [C.c_void_p, C.c_int64, P(Schedule)]
@@ -226,6 +237,16 @@ class Run:
         weight, the total rate in e_run); the next run applies that event first."""
         _check(lib().mcg_run(self.h, int(n_passes)))
 
+    def run_steps(self, n_steps):
+        """EXACT: exactly n_steps more attempts per active chain (may end inside a pass)."""
+        _check(lib().mcg_run_steps(self.h, n_steps))
+
+    def position(self, c):
+        """(pass count, steps into the current pass, FP64 running energy per unit cell)."""
+        p, st, e = C.c_int64(), C.c_int64(), C.c_double()
+        _check(lib().mcg_get_position(self.h, c, C.byref(p), C.byref(st), C.byref(e)))
+        return p.value, st.value, e.value
+
     def assign(self, c, chain):
         """Re-initialise slot c for a new run (shape within the slot's capacity)."""
         shp = list(chain["shape"]) + [1] * (3 - len(chain["shape"]))
@@ -284,6 +305,22 @@ class Run:
                                         n_resamples, C.byref(s)))
         return dict(mean=s.mean, calculated_precision=s.calculated_precision,
                     variance=s.variance, autocorr_factor=s.autocorr_factor, n=s.n)
+
+    def check_batch(self, chains, observable, prec, confidence=0.95):
+        """Equilibration at precision prec, then the statistics after it, for every
+        listed chain in one launch (mcg_check_batch); a list of dicts."""
+        ch = np.ascontiguousarray(chains, dtype=np.int64)
+        out = (Check * max(len(ch), 1))()
+        _check(lib().mcg_check_batch(self.h, len(ch), ch.ctypes.data_as(C.POINTER(C.c_int64)),
+                                     observable, prec, confidence, out))
+        res = []
+        for k in range(len(ch)):
+            s = out[k].stats
+            res.append(dict(is_equilibrated=bool(out[k].is_equilibrated),
+                            n_equilibration=out[k].n_equilibration, mean=s.mean,
+                            calculated_precision=s.calculated_precision, variance=s.variance,
+                            autocorr_factor=s.autocorr_factor, n=s.n))
+        return res
 
     def equilibration(self, c, observable, prec):
         a, b = C.c_int64(), C.c_int64()

@@ -188,6 +188,29 @@ std::vector<uint8_t> fcc_tri_table() {
   return tab;
 }
 
+// The triangle table in the nearest-neighbour bit order of the bit-line
+// kernel (fcc.hip, fcc_bits_unit): bits 0-7 the lines (dy, dz) = (-1, 0),
+// (1, 0), (0, -1), (0, 1) with dx = -1 then +1, bits 8-11 the lines (-1, -1),
+// (-1, 1), (1, -1), (1, 1) with dx = 0.  The count is a property of the
+// occupied neighbours, so both tables give the same t for the same lattice.
+std::vector<uint8_t> fcc_tri_table_lines() {
+  const FccOffset nn[12] = {{-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {1, 1, 0},
+                            {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1}, {1, 0, 1},
+                            {0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {0, 1, 1}};
+  std::vector<std::pair<int, int>> tp;
+  for (int a = 0; a < 12; ++a)
+    for (int b = a + 1; b < 12; ++b)
+      if (fcc_is_nn(nn[b].dx - nn[a].dx, nn[b].dy - nn[a].dy, nn[b].dz - nn[a].dz))
+        tp.push_back({a, b});
+  std::vector<uint8_t> tab(4096);
+  for (int m = 0; m < 4096; ++m) {
+    int t = 0;
+    for (auto const &pr : tp) t += ((m >> pr.first) & 1) & ((m >> pr.second) & 1);
+    tab[m] = (uint8_t)t;
+  }
+  return tab;
+}
+
 // Counts of an FCC occupation (site order l = h + L0*(y + 2*L1*z)).
 void fcc_counts(std::vector<int8_t> const &occ, const int64_t *L, int64_t &N1, int64_t &P1,
                 int64_t &P2, int64_t &P3, int64_t &T) {
@@ -250,17 +273,29 @@ struct mcg_handle {
   int64_t series_cap = 0;
   std::vector<int64_t> n_samples;  // host mirror of the series length
   std::vector<int64_t> n_pass;     // host mirror of the pass counters
+  std::vector<int64_t> n_step;     // EXACT: host mirror of the steps into the current pass
   std::vector<int64_t> capacity;   // sites each chain slot was allocated for
   double *d_stats = nullptr;
   int64_t stats_cap = 0;
+  // mcg_check_batch: chain list, counts, scratch offsets and observable
+  // parameters of one call (bytes)
+  unsigned char *d_batch = nullptr;
+  int64_t batch_cap = 0;
   int64_t *d_eq = nullptr;
 
   // EXACT (and CANONICAL, which shares the state records d_xs)
   ExactParams *d_xp = nullptr;
   ExactState *d_xs = nullptr;
   std::vector<ExactParams> xp;
-  int exact_lds = 0;
-  bool exact_in_lds = true;
+  // per chain: LDS bytes of the block walk and whether they fit (else the
+  // chain runs the scalar walk on its HBM lattice); device masks of the active
+  // chains on each path, and a second stream and two events that let the HBM
+  // chains run beside the LDS ones
+  std::vector<int> xlds;
+  std::vector<uint8_t> xfit;
+  uint8_t *d_mask_lds = nullptr, *d_mask_hbm = nullptr;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   // CANONICAL: per-chain parameters and the candidate lists (int32 entries at
   // the lattice offsets: species-0 sites then species-1 sites)
@@ -281,6 +316,14 @@ struct mcg_handle {
   }
 
   // CHECKERBOARD
+  // chains with an odd edge run the ring-coloured kernels (checker_ring.hip) on
+  // a natural-layout lattice; masks of the active chains of each kind and the
+  // list of active ring chains for the HBM class launches
+  std::vector<uint8_t> cring;
+  uint8_t *d_mask_even = nullptr, *d_mask_ring = nullptr;
+  int32_t *d_ring_list = nullptr;
+  int n_ring_active = 0, ring_per_chain = 1, ring_lds_bytes = 0;
+  bool any_ring() const { return n_ring_active > 0; }
   CheckerParams *d_cp = nullptr;
   CheckerState *d_cs = nullptr;
   uint32_t *d_tab = nullptr;
@@ -316,19 +359,28 @@ struct mcg_handle {
 
   // FCC cluster expansion (CHECKERBOARD with lattice FCC; shares cp/d_cp/d_cs)
   double *d_ftab = nullptr;  // FCC_NTAB doubles per chain
-  uint8_t *d_tri = nullptr;  // 4096-entry triangle-count table
+  uint8_t *d_tri = nullptr;  // 4096-entry triangle-count table (and the bit-line order one)
+  uint8_t *d_tri_lines = nullptr;
+  // bit-line kernel: every chain L0 <= 32, lattice in LDS as one word per line
+  bool fcc_bits = false;
+  int fcc_bits_lds = 0;
   bool fcc() const { return lattice == MCG_LATTICE_FCC; }
 
   ~mcg_handle() {
     // teardown: errors here have nowhere to go
     if (stream) (void)hipStreamSynchronize(stream);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (stream2) (void)hipStreamSynchronize(stream2);
     void *ptrs[] = {d_lat, d_active, d_series, d_stats, d_eq, d_xp, d_xs, d_cp, d_cs, d_tab,
-                    d_work, d_ftab, d_tri, d_band, d_pair, d_kp, d_list, d_np};
+                    d_work, d_ftab, d_tri, d_tri_lines, d_band, d_pair, d_kp, d_list, d_np,
+                    d_mask_lds, d_mask_even, d_ring_list, d_batch};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -420,20 +472,48 @@ int fill_exact_params(mcg_handle *h, int c, ExactParams &P) {
   }
 }
 
+// Device masks of the active EXACT chains whose lattice fits the block walk's
+// LDS and of the others (scalar walk on the HBM lattice).
+int exact_update_masks(mcg_handle *h) {
+  std::vector<uint8_t> m(2 * (size_t)h->n_chains);
+  for (int c = 0; c < h->n_chains; ++c) {
+    m[c] = h->active[c] && h->xfit[c];
+    m[h->n_chains + c] = h->active[c] && !h->xfit[c];
+  }
+  HIP_TRY(copy_sync(h, h->d_mask_lds, m.data(), 2 * (size_t)h->n_chains, hipMemcpyHostToDevice));
+  return MCG_OK;
+}
+
+// Set chain c's EXACT path from its parameters: the block walk in LDS when its
+// lattice fits, else the scalar walk on HBM (fold scratch in LDS).
+int exact_choose_path(mcg_handle *h, int c, int lds) {
+  auto const &P = h->xp[c];
+  h->xlds[c] = lds;
+  h->xfit[c] = lds <= EXACT_LDS_MAX;
+  if (!h->xfit[c] && 4 * (P.rows + P.cols) > EXACT_LDS_MAX)
+    return fail(MCG_ERR_ARG, "lattice too large for EXACT mode");
+  return MCG_OK;
+}
+
 int build_exact(mcg_handle *h) {
   h->xp.resize(h->n_chains);
-  int max_lds = 0;
-  for (int c = 0; c < h->n_chains; ++c) max_lds = std::max(max_lds, fill_exact_params(h, c, h->xp[c]));
-  h->exact_in_lds = max_lds <= EXACT_LDS_MAX;
-  if (h->exact_in_lds) {
-    h->exact_lds = max_lds;
-  } else {
-    int scratch = 0;
-    for (auto const &P : h->xp) scratch = std::max(scratch, 4 * (P.rows + P.cols));
-    h->exact_lds = scratch;
-    if (scratch > EXACT_LDS_MAX) return fail(MCG_ERR_ARG, "lattice too large for EXACT mode");
+  h->xlds.assign(h->n_chains, 0);
+  h->xfit.assign(h->n_chains, 1);
+  for (int c = 0; c < h->n_chains; ++c) {
+    const int rc = exact_choose_path(h, c, fill_exact_params(h, c, h->xp[c]));
+    if (rc != MCG_OK) return rc;
   }
   HIP_TRY(exact_set_max_lds((int)LDS_LIMIT - 4096));
+  HIP_TRY(hipMalloc(&h->d_mask_lds, 2 * (size_t)h->n_chains));
+  h->d_mask_hbm = h->d_mask_lds + h->n_chains;
+  {
+    const int rc = exact_update_masks(h);
+    if (rc != MCG_OK) return rc;
+  }
+  if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess)
+    return fail(MCG_ERR_HIP, "second stream for the HBM-lattice chains");
   HIP_TRY(hipMalloc(&h->d_xp, sizeof(ExactParams) * h->n_chains));
   HIP_TRY(hipMalloc(&h->d_xs, sizeof(ExactState) * h->n_chains));
   HIP_TRY(copy_sync(h, h->d_xp, h->xp.data(), sizeof(ExactParams) * h->n_chains,
@@ -637,9 +717,10 @@ int fill_checker_params(mcg_handle *h, int c, CheckerParams &P, uint32_t *tab) {
   std::fill(tab, tab + ntab, 0u);
   {
     auto const &ch = h->chains[c];
+    // even edges: parity colourings (checker.hip); an odd edge (>= 3): ring
+    // colourings (checker_ring.hip)
     for (int d = 0; d < ndim; ++d)
-      if (ch.shape[d] < 2 || (ch.shape[d] & 1))
-        return fail(MCG_ERR_ARG, "CHECKERBOARD needs even edge lengths >= 2");
+      if (ch.shape[d] < 2) return fail(MCG_ERR_ARG, "CHECKERBOARD needs edge lengths >= 2");
     std::memset(&P, 0, sizeof(P));
     P.L0 = (int32_t)ch.shape[0];
     P.L1 = (int32_t)ch.shape[1];
@@ -680,6 +761,43 @@ int fill_checker_params(mcg_handle *h, int c, CheckerParams &P, uint32_t *tab) {
   return MCG_OK;
 }
 
+// A CHECKERBOARD chain with an odd edge runs the ring-coloured kernels.
+bool checker_ring_chain(mcg_handle const *h, int c) {
+  const int ndim = h->lattice == MCG_LATTICE_SC ? 3 : 2;
+  for (int d = 0; d < ndim; ++d)
+    if (h->chains[c].shape[d] & 1) return true;
+  return false;
+}
+
+// Masks of the active parity-coloured and ring-coloured chains, the list of the
+// active ring chains and the blocks per chain of their HBM class launches.
+int checker_update_masks(mcg_handle *h) {
+  std::vector<uint8_t> m(2 * (size_t)h->n_chains);
+  std::vector<int32_t> ring;
+  int64_t max_units = 1;
+  for (int c = 0; c < h->n_chains; ++c) {
+    m[c] = h->active[c] && !h->cring[c];
+    m[h->n_chains + c] = h->active[c] && h->cring[c];
+    if (h->active[c] && h->cring[c]) {
+      ring.push_back(c);
+      auto const &P = h->cp[c];
+      // the largest class: ceil(L0/2)/4 units per line, ceil(L1/2) (x ceil(L2/2)) lines
+      const int64_t u = ((P.L0 + 1) / 2 + 3) / 4 * ((P.L1 + 1) / 2) *
+                        (P.ndim == 3 ? (P.L2 + 1) / 2 : 1);
+      max_units = std::max(max_units, u);
+    }
+  }
+  HIP_TRY(copy_sync(h, h->d_mask_even, m.data(), m.size(), hipMemcpyHostToDevice));
+  h->n_ring_active = (int)ring.size();
+  // about four units per thread of a 256-thread block
+  h->ring_per_chain = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (max_units + 1023) / 1024));
+  if (!ring.empty())
+    HIP_TRY(copy_sync(h, h->d_ring_list, ring.data(), sizeof(int32_t) * ring.size(),
+                      hipMemcpyHostToDevice));
+  h->graph_dirty = true;
+  return MCG_OK;
+}
+
 int build_checker(mcg_handle *h) {
   const bool nnn = h->J2 != 0.0;
   const int ndim = h->lattice == MCG_LATTICE_SC ? 3 : 2;
@@ -696,6 +814,16 @@ int build_checker(mcg_handle *h) {
   h->checker_lds = max_n <= CHECKER_LDS_MAX && !(h->options & MCG_OPT_FORCE_STREAMING);
   h->checker_lds_bytes = (int)((max_n + 15) & ~15);
   HIP_TRY(checker_set_max_lds((int)LDS_LIMIT - 4096));
+  HIP_TRY(checker_ring_set_max_lds((int)CHECKER_LDS_MAX));
+  h->cring.assign(h->n_chains, 0);
+  for (int c = 0; c < h->n_chains; ++c) h->cring[c] = checker_ring_chain(h, c) ? 1 : 0;
+  HIP_TRY(hipMalloc(&h->d_mask_even, 2 * (size_t)h->n_chains));
+  h->d_mask_ring = h->d_mask_even + h->n_chains;
+  HIP_TRY(hipMalloc(&h->d_ring_list, sizeof(int32_t) * (size_t)h->n_chains));
+  if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess)
+    return fail(MCG_ERR_HIP, "second stream for the ring-coloured chains");
   // the HBM path's kernels get their largest dynamic LDS once, here: an
   // 8-line ring of the longest band line (L0 <= 8192) and the largest line
   // tile; rebuild_work only chooses what a launch uses, so handles driven from
@@ -710,7 +838,7 @@ int build_checker(mcg_handle *h) {
   HIP_TRY(copy_sync(h, h->d_tab, tabs.data(), sizeof(uint32_t) * tabs.size(),
                     hipMemcpyHostToDevice));
   HIP_TRY(fill_sync(h, h->d_cs, 0, sizeof(CheckerState) * h->n_chains));
-  return MCG_OK;
+  return checker_update_masks(h);
 }
 
 // Parameters and acceptance tables (FCC_NTAB doubles at tab) of one FCC chain:
@@ -777,8 +905,22 @@ int build_fcc(mcg_handle *h) {
   }
   h->checker_lds = max_lds <= FCC_LDS_MAX && !(h->options & MCG_OPT_FORCE_STREAMING);
   h->checker_lds_bytes = h->checker_lds ? (int)max_lds : 0;
+  // the bit-line kernel when every line fits one word
+  h->fcc_bits = !(h->options & MCG_OPT_FORCE_STREAMING);
+  h->fcc_bits_lds = 0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    auto const &P = h->cp[c];
+    if (P.L0 > FCC_BITS_MAX_L0) h->fcc_bits = false;
+    h->fcc_bits_lds = std::max(h->fcc_bits_lds, fcc_bits_lds_bytes(P.L1, P.L2));
+  }
+  if (h->fcc_bits_lds > FCC_LDS_MAX) h->fcc_bits = false;
+  if (h->fcc_bits) h->checker_lds = true;
   HIP_TRY(fcc_set_max_lds((int)FCC_LDS_MAX));
   const std::vector<uint8_t> tri = fcc_tri_table();
+  const std::vector<uint8_t> tri_lines = fcc_tri_table_lines();
+  HIP_TRY(hipMalloc(&h->d_tri_lines, tri_lines.size()));
+  HIP_TRY(copy_sync(h, h->d_tri_lines, tri_lines.data(), tri_lines.size(),
+                    hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc(&h->d_cp, sizeof(CheckerParams) * h->n_chains));
   HIP_TRY(hipMalloc(&h->d_cs, sizeof(CheckerState) * h->n_chains));
   HIP_TRY(hipMalloc(&h->d_ftab, sizeof(double) * tabs.size()));
@@ -852,8 +994,10 @@ int upload_list(mcg_handle *h, int32_t **buf, int64_t *cap, std::vector<int32_t>
 int rebuild_work(mcg_handle *h) {
   const int upi = checker_units_per_item();
   std::vector<int32_t> work;
+  // ring-coloured chains have their own class launches (launch_checker_ring_pass)
+  auto parity_chain = [&](int c) { return h->active[c] && !h->cring[c]; };
   for (int c = 0; c < h->n_chains; ++c) {
-    if (!h->active[c]) continue;
+    if (!parity_chain(c)) continue;
     auto const &P = h->cp[c];
     // work units of one colour: 16-site chunks when the half line is a
     // multiple of 16, else 4-site quads (checker.hip)
@@ -881,7 +1025,7 @@ int rebuild_work(mcg_handle *h) {
                  !(h->options & MCG_OPT_CHECKER_QUAD_PATH);
   int64_t maxL0 = 0, total_lines = 0;
   for (int c = 0; c < h->n_chains && band_ok; ++c) {
-    if (!h->active[c]) continue;
+    if (!parity_chain(c)) continue;
     auto const &P = h->cp[c];
     if (P.L0 % 32 != 0 || P.L0 > CHECKER_BAND_MAX_L0) band_ok = false;
     maxL0 = std::max<int64_t>(maxL0, P.L0);
@@ -895,7 +1039,7 @@ int rebuild_work(mcg_handle *h) {
     }
     std::vector<int32_t> band;
     for (int c = 0; c < h->n_chains; ++c) {
-      if (!h->active[c]) continue;
+      if (!parity_chain(c)) continue;
       const int L1 = h->cp[c].L1;
       int H = std::max(2, target & ~1);
       while (H > 2 && L1 % H != 0) H -= 2;  // an even divisor of L1 (L1 is even)
@@ -924,10 +1068,10 @@ int rebuild_work(mcg_handle *h) {
     // tiles when requested and every active chain runs the 16-site chunk path
     bool tiles = h->line_tiles != 0;
     for (int c = 0; c < h->n_chains; ++c)
-      if (h->active[c] && (h->cp[c].L0 / 2 % 16 != 0 || h->cp[c].quad_only)) tiles = false;
+      if (parity_chain(c) && (h->cp[c].L0 / 2 % 16 != 0 || h->cp[c].quad_only)) tiles = false;
     std::vector<int32_t> items;
     for (int c = 0; c < h->n_chains; ++c) {
-      if (!h->active[c]) continue;
+      if (!parity_chain(c)) continue;
       auto const &P = h->cp[c];
       const int64_t half = P.L0 / 2;
       const bool chunked = half % 16 == 0 && !P.quad_only;
@@ -1003,6 +1147,10 @@ void enqueue_pass(mcg_handle *h) {
     for (int color = 0; color < h->ncolor; ++color)
       launch_checker_color(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_work, h->n_work, color,
                            h->units_per_thread, h->stream);
+  if (h->any_ring())
+    launch_checker_ring_pass(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_ring_list,
+                             h->n_ring_active, h->ring_per_chain,
+                             h->lattice == MCG_LATTICE_SC ? 3 : 2, h->stream);
   launch_checker_record(h->d_cp, h->d_cs, h->d_series, h->series_cap, h->d_active, h->n_chains,
                         h->sample_period, h->stream);
 }
@@ -1101,6 +1249,44 @@ void tune_cache_store(const std::string &key, const mcg_schedule &s) {
 }
 
 bool valid_chain(mcg_handle *h, int64_t c) { return h && c >= 0 && c < h->n_chains; }
+
+// EXACT launches: the chains whose lattice fits in LDS run the block walk on the
+// handle's stream; the others run the scalar walk on their HBM lattices on the
+// second stream at the same time (the two sets touch disjoint chains), and
+// the handle's stream waits for them.
+int exact_launch(mcg_handle *h, int64_t n_passes, int64_t n_steps) {
+  int lds = 0, scratch = 0;
+  bool any_fit = false, any_hbm = false;
+  for (int c = 0; c < h->n_chains; ++c) {
+    if (!h->active[c]) continue;
+    if (h->xfit[c]) {
+      any_fit = true;
+      lds = std::max(lds, h->xlds[c]);
+    } else {
+      any_hbm = true;
+      scratch = std::max(scratch, 4 * (h->xp[c].rows + h->xp[c].cols));
+    }
+  }
+  if (any_hbm) {
+    HIP_TRY(hipEventRecord(h->ev_fork, h->stream));
+    HIP_TRY(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+    launch_exact(h->d_xp, h->d_xs, h->d_lat, h->d_series, h->series_cap, h->d_mask_hbm,
+                 h->n_chains, n_passes, n_steps, h->sample_period, scratch, false, false,
+                 h->stream2);
+    HIP_TRY(hipGetLastError());
+  }
+  if (any_fit) {
+    launch_exact(h->d_xp, h->d_xs, h->d_lat, h->d_series, h->series_cap, h->d_mask_lds,
+                 h->n_chains, n_passes, n_steps, h->sample_period, lds, true,
+                 !(h->options & MCG_OPT_EXACT_SCALAR_WALK), h->stream);
+    HIP_TRY(hipGetLastError());
+  }
+  if (any_hbm) {
+    HIP_TRY(hipEventRecord(h->ev_join, h->stream2));
+    HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_join, 0));
+  }
+  return MCG_OK;
+}
 
 }  // namespace
 
@@ -1214,6 +1400,7 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
     return bail(fail(MCG_ERR_HIP, "active mask allocation failed"));
   h->n_samples.assign(h->n_chains, 0);
   h->n_pass.assign(h->n_chains, 0);
+  h->n_step.assign(h->n_chains, 0);
   for (int c = 0; c < h->n_chains; ++c) h->capacity.push_back(h->sites(c));
   if (hipMalloc(&h->d_eq, 2 * sizeof(int64_t)) != hipSuccess)
     return bail(fail(MCG_ERR_HIP, "allocation failed"));
@@ -1262,7 +1449,10 @@ int mcg_set_occupation(mcg_handle *h, int64_t chain, const int8_t *occ, int64_t 
     }
   } else {
     std::vector<int8_t> dev(n);
-    to_split_rows(v, h->chains[chain].shape[0], dev);
+    if (h->cring[chain])  // natural layout, 1 for +1
+      for (int64_t l = 0; l < n; ++l) dev[l] = v[l] > 0 ? 1 : 0;
+    else
+      to_split_rows(v, h->chains[chain].shape[0], dev);
     HIP_TRY(copy_sync(h, h->d_lat + h->lat_off[chain], dev.data(), n, hipMemcpyHostToDevice));
   }
   return reset_observables(h, chain, v);
@@ -1277,7 +1467,10 @@ int mcg_get_occupation(mcg_handle *h, int64_t chain, int8_t *occ, int64_t n) {
   } else {
     std::vector<int8_t> dev(n), nat(n);
     HIP_TRY(copy_sync(h, dev.data(), h->d_lat + h->lat_off[chain], n, hipMemcpyDeviceToHost));
-    from_split_rows(dev, h->chains[chain].shape[0], nat);
+    if (h->cring[chain])
+      for (int64_t l = 0; l < n; ++l) nat[l] = dev[l] ? 1 : -1;
+    else
+      from_split_rows(dev, h->chains[chain].shape[0], nat);
     std::copy(nat.begin(), nat.end(), occ);
   }
   return MCG_OK;
@@ -1352,13 +1545,20 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
   if (h->mode == MCG_MODE_EXACT) {
     ExactParams P;
     const int lds = fill_exact_params(h, (int)chain, P);
-    const int need = h->exact_in_lds ? lds : 4 * (P.rows + P.cols);
-    if (need > h->exact_lds) {
-      h->chains[chain] = old;
-      return fail(MCG_ERR_CAPACITY, "shape needs more LDS than the handle was created with");
-    }
+    const ExactParams P_old = h->xp[chain];
     h->xp[chain] = P;
+    const int prc = exact_choose_path(h, (int)chain, lds);
+    if (prc != MCG_OK) {
+      h->chains[chain] = old;
+      h->xp[chain] = P_old;
+      exact_choose_path(h, (int)chain, fill_exact_params(h, (int)chain, h->xp[chain]));
+      return prc;
+    }
     HIP_TRY(copy_sync(h, h->d_xp + chain, &P, sizeof(P), hipMemcpyHostToDevice));
+    {
+      const int mrc = exact_update_masks(h);
+      if (mrc != MCG_OK) return mrc;
+    }
     ExactState S;
     std::memset(&S, 0, sizeof(S));
     int pos = 0;
@@ -1401,7 +1601,10 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
       h->chains[chain] = old;
       return rc;
     }
-    if (h->checker_lds && fcc_lds_bytes(P) > h->checker_lds_bytes) {
+    const bool no_room =
+        h->fcc_bits ? (P.L0 > FCC_BITS_MAX_L0 || fcc_bits_lds_bytes(P.L1, P.L2) > h->fcc_bits_lds)
+                    : (h->checker_lds && fcc_lds_bytes(P) > h->checker_lds_bytes);
+    if (no_room) {
       h->chains[chain] = old;
       return fail(MCG_ERR_CAPACITY, "shape needs more LDS than the handle was created with");
     }
@@ -1428,12 +1631,16 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
     std::memset(&S, 0, sizeof(S));
     HIP_TRY(copy_sync(h, h->d_cs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
     h->work_dirty = true;
+    h->cring[chain] = checker_ring_chain(h, (int)chain) ? 1 : 0;
+    const int mrc = checker_update_masks(h);
+    if (mrc != MCG_OK) return mrc;
   }
   const int fill = h->fcc() ? 0 : 1;
   HIP_TRY(fill_sync(h, h->d_lat + h->lat_off[chain], fill,
                     (size_t)((h->capacity[chain] + 255) & ~255LL)));
   h->n_samples[chain] = 0;
   h->n_pass[chain] = 0;
+  h->n_step[chain] = 0;
   const int rc = reset_observables(h, chain, std::vector<int8_t>(n, (int8_t)fill));
   if (rc != MCG_OK || !h->nfold()) return rc;
   return nfold_init_array(h, chain, std::vector<int8_t>(n, (int8_t)fill));
@@ -1482,6 +1689,8 @@ int mcg_set_active(mcg_handle *h, const uint8_t *active, int64_t n_chains) {
   for (int c = 0; c < h->n_chains; ++c) h->active[c] = active[c] ? 1 : 0;
   HIP_TRY(copy_sync(h, h->d_active, h->active.data(), h->n_chains, hipMemcpyHostToDevice));
   h->work_dirty = true;
+  if (h->mode == MCG_MODE_EXACT) return exact_update_masks(h);
+  if (h->mode == MCG_MODE_CHECKERBOARD && !h->fcc()) return checker_update_masks(h);
   return MCG_OK;
 }
 
@@ -1526,7 +1735,9 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
         // OccEventProposal.hh:136-139 (occupation_metropolis.hh:121-127 stops the run)
         return fail(MCG_ERR_STATE, "Error in choose_canonical_swap: No events possible.");
   const bool hbm_path = !h->sequential() && !h->checker_lds && !h->fcc();
-  const int G = h->graph_passes;
+  // the graph holds the parity-coloured launches only: with ring-coloured
+  // chains active, passes are enqueued one by one
+  const int G = (hbm_path && h->any_ring()) ? 1 : h->graph_passes;
   if (hbm_path) {
     if (h->work_dirty) {
       rc = rebuild_work(h);
@@ -1540,10 +1751,8 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
   }
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
   if (h->mode == MCG_MODE_EXACT) {
-    launch_exact(h->d_xp, h->d_xs, h->d_lat, h->d_series, h->series_cap, h->d_active,
-                 h->n_chains, n_passes, h->sample_period, h->exact_lds, h->exact_in_lds,
-                 !(h->options & MCG_OPT_EXACT_SCALAR_WALK),
-                 h->stream);
+    rc = exact_launch(h, n_passes, 0);
+    if (rc != MCG_OK) return rc;
   } else if (h->canonical()) {
     launch_canonical(h->d_kp, h->d_xs, h->d_lat, h->d_list, h->d_series, h->series_cap,
                      h->d_active, h->n_chains, n_passes, h->sample_period, h->canon_lds,
@@ -1554,9 +1763,10 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
                  h->nfold_lds, h->stream);
   } else if (h->fcc()) {
     HIP_TRY(hipGetLastError());
-    launch_fcc(h->d_cp, h->d_cs, h->d_lat, h->d_ftab, h->d_tri, h->d_series, h->series_cap,
-               h->d_active, h->n_chains, n_passes, h->sample_period, h->checker_lds_bytes,
-               h->checker_lds, h->checker_block, h->stream);
+    launch_fcc(h->d_cp, h->d_cs, h->d_lat, h->d_ftab, h->d_tri, h->d_tri_lines, h->d_series,
+               h->series_cap, h->d_active, h->n_chains, n_passes, h->sample_period,
+               h->fcc_bits ? h->fcc_bits_lds : h->checker_lds_bytes, h->checker_lds, h->fcc_bits,
+               h->checker_block, h->stream);
     const hipError_t le = hipGetLastError();
     if (le != hipSuccess)
       return fail(MCG_ERR_HIP, std::string("fcc launch (block ") +
@@ -1564,9 +1774,23 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
                                    std::to_string(h->checker_lds_bytes) +
                                    "): " + hipGetErrorString(le));
   } else if (h->checker_lds) {
+    // parity-coloured chains on the handle's stream, ring-coloured ones beside
+    // them on the second stream (they touch disjoint chains)
+    if (h->any_ring()) {
+      HIP_TRY(hipEventRecord(h->ev_fork, h->stream));
+      HIP_TRY(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+      launch_checker_ring_lds(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_series, h->series_cap,
+                              h->d_mask_ring, h->n_chains, n_passes, h->sample_period,
+                              h->checker_lds_bytes, h->checker_block, h->stream2);
+      HIP_TRY(hipGetLastError());
+    }
     launch_checker_lds(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_series, h->series_cap,
-                       h->d_active, h->n_chains, n_passes, h->sample_period,
+                       h->d_mask_even, h->n_chains, n_passes, h->sample_period,
                        h->checker_lds_bytes, h->checker_block, h->stream);
+    if (h->any_ring()) {
+      HIP_TRY(hipEventRecord(h->ev_join, h->stream2));
+      HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_join, 0));
+    }
   } else {
     int64_t done = 0;
     if (G > 1 && n_passes >= G)
@@ -1608,6 +1832,61 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
                   "NFOLD time-grid samples exceeded the sample series by " + std::to_string(over) +
                       " records (they were dropped); reserve more with mcg_reserve_samples");
   }
+  return MCG_OK;
+}
+
+int mcg_run_steps(mcg_handle *h, int64_t n_steps) {
+  if (!h) return fail(MCG_ERR_ARG, "null handle");
+  if (h->mode != MCG_MODE_EXACT)
+    return fail(MCG_ERR_STATE, "mcg_run_steps: single steps are run in EXACT mode only");
+  if (n_steps < 0) return fail(MCG_ERR_ARG, "n_steps < 0");
+  h->last_ms = 0.0;
+  h->last_attempts = 0.0;
+  if (n_steps == 0) return MCG_OK;
+  // passes each chain completes and the records they append
+  std::vector<int64_t> passes(h->n_chains, 0), added(h->n_chains, 0);
+  int64_t need = 0;
+  double attempts = 0.0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    if (!h->active[c]) continue;
+    const int64_t p0 = h->n_pass[c];
+    passes[c] = (h->n_step[c] + n_steps) / h->sites(c);
+    added[c] = (p0 + passes[c]) / h->sample_period - p0 / h->sample_period;
+    need = std::max(need, h->n_samples[c] + added[c]);
+    attempts += (double)n_steps;
+  }
+  int rc = ensure_series(h, need);
+  if (rc != MCG_OK) return rc;
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  rc = exact_launch(h, 0, n_steps);
+  if (rc != MCG_OK) return rc;
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  h->last_attempts = attempts;
+  for (int c = 0; c < h->n_chains; ++c) {
+    if (!h->active[c]) continue;
+    const int64_t s = h->n_step[c] + n_steps;
+    h->n_step[c] = s - passes[c] * h->sites(c);
+    h->n_pass[c] += passes[c];
+    h->n_samples[c] += added[c];
+  }
+  return MCG_OK;
+}
+
+int mcg_get_position(mcg_handle *h, int64_t chain, int64_t *n_pass, int64_t *step,
+                     double *e_run) {
+  if (!valid_chain(h, chain)) return fail(MCG_ERR_ARG, "bad chain");
+  if (!h->sequential()) return fail(MCG_ERR_STATE, "mcg_get_position: sequential modes only");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  ExactState st;
+  HIP_TRY(copy_sync(h, &st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
+  if (n_pass) *n_pass = st.n_pass;
+  if (step) *step = h->mode == MCG_MODE_EXACT ? st.step : 0;
+  if (e_run) *e_run = st.e_run;
   return MCG_OK;
 }
 
@@ -1798,6 +2077,73 @@ int mcg_stats_compute(mcg_handle *h, int64_t chain, int64_t observable, int64_t 
   out->autocorr_factor = r[2];
   out->n = count;
   out->calculated_precision = f_conf * std::sqrt(r[2] * r[1] / (double)count);
+  return MCG_OK;
+}
+
+int mcg_check_batch(mcg_handle *h, int64_t n, const int64_t *chains, int64_t observable,
+                    double prec, double confidence, mcg_check *out) {
+  if (!h || n < 0 || (n > 0 && (!chains || !out))) return fail(MCG_ERR_ARG, "bad arguments");
+  if (observable < 0 || observable > 2) return fail(MCG_ERR_ARG, "bad observable");
+  if (h->nfold())
+    return fail(MCG_ERR_STATE, "mcg_check_batch: weighted (NFOLD) records are checked one by "
+                               "one (mcg_equilibration, mcg_stats_weighted)");
+  if (n == 0) return MCG_OK;
+  if (n > h->n_chains) return fail(MCG_ERR_ARG, "more chains than the handle holds");
+  std::vector<int64_t> ints(3 * (size_t)n);  // chains, counts, scratch offsets
+  std::vector<ObsParams> ops((size_t)n);
+  int64_t scratch = 0;
+  for (int64_t b = 0; b < n; ++b) {
+    if (!valid_chain(h, chains[b])) return fail(MCG_ERR_ARG, "bad chain in the list");
+    const int64_t count = h->n_samples[chains[b]];
+    if (count <= 0)
+      return fail(MCG_ERR_ARG, "Error in equilibration_check: observations.size()==0");
+    ints[b] = chains[b];
+    ints[n + b] = count;
+    ints[2 * n + b] = scratch;
+    scratch += count;
+    ops[b] = obs_params(h, chains[b], observable);
+  }
+  const size_t ibytes = sizeof(int64_t) * ints.size(), obytes = sizeof(ObsParams) * ops.size();
+  if ((int64_t)(ibytes + obytes) > h->batch_cap) {
+    unsigned char *d = nullptr;
+    const int64_t cap = std::max<int64_t>((int64_t)(ibytes + obytes), 2 * h->batch_cap);
+    if (hipMalloc(&d, cap) != hipSuccess)
+      return fail(MCG_ERR_CAPACITY, "out of device memory for the check list");
+    if (h->d_batch) (void)hipFree(h->d_batch);
+    h->d_batch = d;
+    h->batch_cap = cap;
+  }
+  {
+    const int rc = ensure_stats(h, 8 * n + scratch);
+    if (rc != MCG_OK) return rc;
+  }
+  std::vector<unsigned char> host(ibytes + obytes);
+  std::memcpy(host.data(), ints.data(), ibytes);
+  std::memcpy(host.data() + ibytes, ops.data(), obytes);
+  HIP_TRY(hipMemcpyAsync(h->d_batch, host.data(), host.size(), hipMemcpyHostToDevice,
+                         h->stream));
+  const int64_t *d_ints = reinterpret_cast<const int64_t *>(h->d_batch);
+  launch_check_batch(h->d_series, h->series_cap, d_ints, d_ints + n,
+                     reinterpret_cast<const ObsParams *>(h->d_batch + ibytes), d_ints + 2 * n,
+                     (int)n, prec, h->d_stats + 8 * n, h->d_stats, h->stream);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> r(8 * (size_t)n);
+  HIP_TRY(hipMemcpyAsync(r.data(), h->d_stats, sizeof(double) * r.size(), hipMemcpyDeviceToHost,
+                         h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const double f_conf = confidence_factor(confidence);
+  for (int64_t b = 0; b < n; ++b) {
+    const double *o = r.data() + 8 * b;
+    mcg_check &c = out[b];
+    c.is_equilibrated = (int64_t)o[0];
+    c.n_equilibration = (int64_t)o[1];
+    c.stats.n = (int64_t)o[6];
+    c.stats.mean = o[2];
+    c.stats.variance = o[3];
+    c.stats.autocorr_factor = o[4];
+    c.stats.calculated_precision =
+        c.stats.n > 0 ? f_conf * std::sqrt(o[4] * o[3] / (double)c.stats.n) : 0.0;
+  }
   return MCG_OK;
 }
 

@@ -60,7 +60,8 @@ __global__ __launch_bounds__(64) void exact_kernel(const ExactParams *__restrict
                                                    mcg_sample *__restrict__ series,
                                                    int64_t series_cap,
                                                    const uint8_t *__restrict__ active,
-                                                   int64_t n_passes, int64_t sample_period) {
+                                                   int64_t n_passes, int64_t n_steps,
+                                                   int64_t sample_period) {
   const int c = blockIdx.x;
   if (!active[c]) return;
   const int lane = threadIdx.x;
@@ -100,8 +101,12 @@ __global__ __launch_bounds__(64) void exact_kernel(const ExactParams *__restrict
     return mt_temper(w);
   };
 
-  for (int64_t p = 0; p < n_passes; ++p) {
-    for (int32_t step = 0; step < n; ++step) {
+  // attempts of this launch: n_passes whole passes and n_steps more, from the
+  // chain's position `step` in its current pass
+  int32_t step = (int32_t)St->step;
+  const int64_t total = n_passes * (int64_t)n + n_steps;
+  for (int64_t done = 0; done < total; ++done) {
+    {
       // propose: l = uniform_int_distribution<long>(0, n-1) (Lemire)
       uint64_t x = next_word();
       uint64_t lo = x * N64;
@@ -145,6 +150,8 @@ __global__ __launch_bounds__(64) void exact_kernel(const ExactParams *__restrict
         ++n_rej;
       }
     }
+    if (++step < n) continue;
+    step = 0;
     ++n_pass;
     if (n_pass % sample_period == 0) {
       double ef = 0.0;
@@ -192,6 +199,7 @@ __global__ __launch_bounds__(64) void exact_kernel(const ExactParams *__restrict
     for (int i = lane; i < n; i += 64) g_lat[i] = lat[i];
   if (lane == 0) {
     St->pos = pos;
+    St->step = step;
     St->n_pass = n_pass;
     St->n_accept = n_acc;
     St->n_reject = n_rej;
@@ -233,7 +241,7 @@ __global__ __launch_bounds__(128) void exact_block_kernel(const ExactParams *__r
                                                          mcg_sample *__restrict__ series,
                                                          int64_t series_cap,
                                                          const uint8_t *__restrict__ active,
-                                                         int64_t n_passes,
+                                                         int64_t n_passes, int64_t n_steps,
                                                          int64_t sample_period) {
   const int c = blockIdx.x;
   if (!active[c]) return;
@@ -288,9 +296,11 @@ __global__ __launch_bounds__(128) void exact_block_kernel(const ExactParams *__r
   const uint64_t *ringf = &ring[0][0];
   auto ring_index = [&](int64_t q) -> int { return eng.index(q); };
 
-  int32_t step = 0;
-  int64_t passes_done = 0;
-  while (passes_done < n_passes) {
+  // attempts of this launch: n_passes whole passes and n_steps more, from the
+  // chain's position `step` in its current pass
+  int32_t step = (int32_t)St->step;
+  int64_t remaining = n_passes * (int64_t)n + n_steps;
+  while (remaining > 0) {
     ensure(pos + W + 1);
     const bool inwin = lane < W;
     const int64_t q = pos + lane;
@@ -352,12 +362,13 @@ __global__ __launch_bounds__(128) void exact_block_kernel(const ExactParams *__r
     const uint64_t stops = A & STOP & ~1ull;
     uint64_t exec = (SP & 1ull) ? 0ull : (A & (stops ? ((1ull << __builtin_ctzll(stops)) - 1) : ~0ull));
     int attempts = __popcll(exec);
-    const int to_pass_end = n - step;
-    if (attempts > to_pass_end) {  // end of pass inside the block
+    // the block ends at the end of the pass or of the launch's attempts
+    const int lim = (int)(remaining < (int64_t)(n - step) ? remaining : (int64_t)(n - step));
+    if (attempts > lim) {
       uint64_t rest = exec;
-      for (int r = 0; r < to_pass_end; ++r) rest &= rest - 1;
+      for (int r = 0; r < lim; ++r) rest &= rest - 1;
       exec ^= rest;
-      attempts = to_pass_end;
+      attempts = lim;
     }
     int t = 0;
     if (exec) {
@@ -387,7 +398,10 @@ __global__ __launch_bounds__(128) void exact_block_kernel(const ExactParams *__r
     asm volatile("" ::: "memory");
     pos += t;
     step += attempts;
+    remaining -= attempts;
 
+    // lane 0 special: exec was empty, so this is the block's only attempt and
+    // `remaining` was > 0 at the top of the loop
     if (SP & 1ull) {
       // one attempt with the full Lemire rejection loop (scalar walk)
       uint64_t xs = ringf[ring_index(pos)];
@@ -437,11 +451,11 @@ __global__ __launch_bounds__(128) void exact_block_kernel(const ExactParams *__r
       // here: the producer wave takes part only in the engine hand-offs)
       asm volatile("" ::: "memory");
       ++step;
+      --remaining;
     }
 
     if (step == n) {
       step = 0;
-      ++passes_done;
       ++n_pass;
       if (n_pass % sample_period == 0) {
         double ef = 0.0;
@@ -498,6 +512,7 @@ __global__ __launch_bounds__(128) void exact_block_kernel(const ExactParams *__r
   for (int i2 = lane; i2 < n; i2 += 64) g_lat[i2] = (lat[i2] & SPIN_BIT) ? (int8_t)1 : (int8_t)-1;
   if (lane == 0) {
     St->pos = p_fin;
+    St->step = step;
     St->n_pass = n_pass;
     St->n_accept = n_acc;
     St->n_reject = n_rej;
@@ -510,17 +525,17 @@ __global__ __launch_bounds__(128) void exact_block_kernel(const ExactParams *__r
 
 void launch_exact(const ExactParams *P, ExactState *S, int8_t *lat_pool, mcg_sample *series,
                   int64_t series_cap, const uint8_t *active, int n_chains, int64_t n_passes,
-                  int64_t sample_period, int lds_bytes, bool lattice_in_lds, bool block_walk,
-                  hipStream_t stream) {
+                  int64_t n_steps, int64_t sample_period, int lds_bytes, bool lattice_in_lds,
+                  bool block_walk, hipStream_t stream) {
   if (lattice_in_lds && block_walk) {
     hipLaunchKernelGGL(exact_block_kernel, dim3(n_chains), dim3(128), lds_bytes, stream, P, S,
-                       lat_pool, series, series_cap, active, n_passes, sample_period);
+                       lat_pool, series, series_cap, active, n_passes, n_steps, sample_period);
   } else if (lattice_in_lds) {
     hipLaunchKernelGGL(exact_kernel<true>, dim3(n_chains), dim3(64), lds_bytes, stream, P, S,
-                       lat_pool, series, series_cap, active, n_passes, sample_period);
+                       lat_pool, series, series_cap, active, n_passes, n_steps, sample_period);
   } else {
     hipLaunchKernelGGL(exact_kernel<false>, dim3(n_chains), dim3(64), lds_bytes, stream, P, S,
-                       lat_pool, series, series_cap, active, n_passes, sample_period);
+                       lat_pool, series, series_cap, active, n_passes, n_steps, sample_period);
   }
 }
 

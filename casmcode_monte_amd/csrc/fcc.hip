@@ -52,6 +52,34 @@ __device__ __forceinline__ int fcc_wave_sum(int v) {
 
 __device__ __forceinline__ int wrap_ring(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
+// Thread a (< 4) lists the coordinates of colour a on the three rings.
+__device__ __forceinline__ void fcc_fill_lists(FccLists &Ls, int a, int L0, int L1, int L2) {
+  int k0 = 0, k1 = 0;
+  for (int v = 0; v < 2 * L1; ++v)
+    if (fcc_ring_colour(v, 2 * L1) == a) {
+      if (v & 1) Ls.y[a][1][k1++] = (uint8_t)v;
+      else Ls.y[a][0][k0++] = (uint8_t)v;
+    }
+  Ls.ny[a][0] = k0;
+  Ls.ny[a][1] = k1;
+  k0 = k1 = 0;
+  for (int v = 0; v < 2 * L2; ++v)
+    if (fcc_ring_colour(v, 2 * L2) == a) {
+      if (v & 1) Ls.z[a][1][k1++] = (uint8_t)v;
+      else Ls.z[a][0][k0++] = (uint8_t)v;
+    }
+  Ls.nz[a][0] = k0;
+  Ls.nz[a][1] = k1;
+  k0 = k1 = 0;
+  for (int v = 0; v < 2 * L0; ++v)
+    if (fcc_ring_colour(v, 2 * L0) == a) {
+      if (v & 1) Ls.xh[a][1][k1++] = (uint8_t)(v >> 1);
+      else Ls.xh[a][0][k0++] = (uint8_t)(v >> 1);
+    }
+  Ls.nx[a][0] = k0;
+  Ls.nx[a][1] = k1;
+}
+
 struct FccDelta {
   int N1 = 0, P1 = 0, P2 = 0, P3 = 0, T = 0, acc = 0;
 };
@@ -210,32 +238,7 @@ __global__ __launch_bounds__(1024) void fcc_kernel(
   for (int i = t; i < FCC_NTAB; i += nt) tab[i] = tables[P.tab_off + i];
   for (int i = t; i < 4096; i += nt) tri[i] = tri_g[i];
   const int L0 = P.L0, L1 = P.L1, L2 = P.L2;
-  if (t < 4) {  // thread a lists the coordinates of colour a on the three rings
-    int k0 = 0, k1 = 0;
-    for (int v = 0; v < 2 * L1; ++v)
-      if (fcc_ring_colour(v, 2 * L1) == t) {
-        if (v & 1) Ls.y[t][1][k1++] = (uint8_t)v;
-        else Ls.y[t][0][k0++] = (uint8_t)v;
-      }
-    Ls.ny[t][0] = k0;
-    Ls.ny[t][1] = k1;
-    k0 = k1 = 0;
-    for (int v = 0; v < 2 * L2; ++v)
-      if (fcc_ring_colour(v, 2 * L2) == t) {
-        if (v & 1) Ls.z[t][1][k1++] = (uint8_t)v;
-        else Ls.z[t][0][k0++] = (uint8_t)v;
-      }
-    Ls.nz[t][0] = k0;
-    Ls.nz[t][1] = k1;
-    k0 = k1 = 0;
-    for (int v = 0; v < 2 * L0; ++v)
-      if (fcc_ring_colour(v, 2 * L0) == t) {
-        if (v & 1) Ls.xh[t][1][k1++] = (uint8_t)(v >> 1);
-        else Ls.xh[t][0][k0++] = (uint8_t)(v >> 1);
-      }
-    Ls.nx[t][0] = k0;
-    Ls.nx[t][1] = k1;
-  }
+  if (t < 4) fcc_fill_lists(Ls, t, L0, L1, L2);
   // In LDS every line takes S = L0 + 2 bytes (+1 when that is a multiple of 4)
   // with ghost copies of its end sites: the lines a wavefront reads at once
   // are 4 apart (one colour), and this stride puts them in distinct banks.
@@ -324,13 +327,221 @@ __global__ __launch_bounds__(1024) void fcc_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Bit-line kernel: LDS-resident chains with L0 <= 32.
+//
+// Each lattice line (fixed y, z) is one 64-bit word D = core | core << L0,
+// bit h of core the occupation of site h, so the three sites h-1, h, h+1 of
+// a line -- with the wrap -- are bits [st, st + 3) of D, st = h - 1 (mod L0),
+// one alignbit.  A site's 42 neighbours lie on 21 lines: the even lines
+// (dy + dz even: the site's own line, (+-1, +-1), (+-2, 0), (0, +-2)) hold
+// them at h - 1, h, h + 1 (dx = -2, 0, +2), the odd lines ((+-1, 0),
+// (0, +-1), (+-2, +-1), (+-1, +-2)) at h + p - 1, h + p (dx = -1, +1), p the
+// parity of y + z.  A unit's four sites share their 21 lines: 21 word reads
+// per unit instead of one byte read per neighbour.  Lines of other classes
+// never hold sites of the class being updated (points of one colour are 0 or
+// >= 3 apart in every coordinate), so the only concurrent writes a unit sees
+// are other units' flips in its own line, whose bits it does not read; a
+// unit's flips go into its line with one 64-bit atomic xor.  Units, random
+// words and decisions are those of fcc_kernel: trajectories are identical.
+// Words are stored at ln + ln / 4, so the lines four apart that a wavefront
+// reads together fall into distinct banks.
+__device__ __forceinline__ int fcc_word_index(int ln) { return ln + (ln >> 2); }
+
+__device__ __forceinline__ uint32_t fcc_window(uint64_t D, int st) {
+  return __builtin_amdgcn_alignbit((uint32_t)(D >> 32), (uint32_t)D, (uint32_t)st);
+}
+
+__device__ __forceinline__ void fcc_bits_unit(uint64_t *__restrict__ w,
+                                              const double *__restrict__ tab,
+                                              const uint8_t *__restrict__ tri, const FccLists &Ls,
+                                              int L0, int L1, int L2, const FccClass &k, int u,
+                                              int64_t pass, uint32_t key0, uint32_t key1,
+                                              FccDelta &d) {
+  const int b = (u >= k.start[1]) + (u >= k.start[2]) + (u >= k.start[3]);
+  const int py = b >> 1, pz = b & 1, p = py ^ pz;
+  const int r0 = u - (b == 0 ? 0 : b == 1 ? k.start[1] : b == 2 ? k.start[2] : k.start[3]);
+  const int nq = b == 0 ? k.nq[0] : b == 1 ? k.nq[1] : b == 2 ? k.nq[2] : k.nq[3];
+  const int q = r0 % nq;
+  const int li = r0 / nq;
+  const int nyb = Ls.ny[k.cy][py];
+  const int y = Ls.y[k.cy][py][li % nyb], z = Ls.z[k.cz][pz][li / nyb];
+  const int cnt = Ls.nx[k.cx][p];
+  const int s0 = q << 2;
+  const int ry = 2 * L1, rz = 2 * L2;
+  int ys[5], zs[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    ys[j] = wrap_ring(y + j - 2, ry);
+    zs[j] = wrap_ring(z + j - 2, rz) * ry;
+  }
+  auto line = [&](int dy, int dz) -> uint64_t {
+    return w[fcc_word_index(ys[dy + 2] + zs[dz + 2])];
+  };
+  const int own = fcc_word_index(y + ry * z);
+  const uint64_t self = w[own];
+  // odd lines: nearest neighbours (dx = -1, +1) and third-shell ones
+  const uint64_t om10 = line(-1, 0), op10 = line(1, 0), om01 = line(0, -1), op01 = line(0, 1);
+  const uint64_t t0 = line(-2, -1), t1 = line(-2, 1), t2 = line(2, -1), t3 = line(2, 1);
+  const uint64_t t4 = line(-1, -2), t5 = line(1, -2), t6 = line(-1, 2), t7 = line(1, 2);
+  // even lines: nearest neighbours (dx = 0) and third-shell ones (dx = +-2)
+  const uint64_t emm = line(-1, -1), emp = line(-1, 1), epm = line(1, -1), epp = line(1, 1);
+  // even lines: second-shell neighbours (dx = 0)
+  const uint64_t s20 = line(-2, 0), s21 = line(2, 0), s22 = line(0, -2), s23 = line(0, 2);
+  const u32x4 r = philox4x32_10((uint32_t)((y + ry * z) * 32 + q), (uint32_t)pass,
+                                (uint32_t)(pass >> 32), (uint32_t)k.col, key0, key1);
+  uint64_t flips = 0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int s = s0 + v < cnt ? s0 + v : cnt - 1;  // tail slots repeat the last site
+    const int h = Ls.xh[k.cx][p][s];
+    const int se = h == 0 ? L0 - 1 : h - 1;  // even window: h-1, h, h+1
+    int so = h + p - 1;                      // odd window: h+p-1, h+p
+    so += so < 0 ? L0 : 0;
+    const uint32_t ws = fcc_window(self, se);
+    const int sg = (ws >> 1) & 1;
+    const int n2 = __popc(ws & 5u) + (int)((fcc_window(s20, se) >> 1) & 1u) +
+                   (int)((fcc_window(s21, se) >> 1) & 1u) + (int)((fcc_window(s22, se) >> 1) & 1u) +
+                   (int)((fcc_window(s23, se) >> 1) & 1u);
+    const uint32_t wmm = fcc_window(emm, se), wmp = fcc_window(emp, se);
+    const uint32_t wpm = fcc_window(epm, se), wpp = fcc_window(epp, se);
+    int n3 = __popc(wmm & 5u) + __popc(wmp & 5u) + __popc(wpm & 5u) + __popc(wpp & 5u);
+    n3 += __popc(fcc_window(t0, so) & 3u) + __popc(fcc_window(t1, so) & 3u) +
+          __popc(fcc_window(t2, so) & 3u) + __popc(fcc_window(t3, so) & 3u) +
+          __popc(fcc_window(t4, so) & 3u) + __popc(fcc_window(t5, so) & 3u) +
+          __popc(fcc_window(t6, so) & 3u) + __popc(fcc_window(t7, so) & 3u);
+    // nearest-neighbour mask in the bit order of the host's fcc_tri_table_lines
+    const uint32_t mask = (fcc_window(om10, so) & 3u) | ((fcc_window(op10, so) & 3u) << 2) |
+                          ((fcc_window(om01, so) & 3u) << 4) |
+                          ((fcc_window(op01, so) & 3u) << 6) | (((wmm >> 1) & 1u) << 8) |
+                          (((wmp >> 1) & 1u) << 9) | (((wpm >> 1) & 1u) << 10) |
+                          (((wpp >> 1) & 1u) << 11);
+    const int n1 = __popc(mask);
+    const int t = tri[mask];
+    const double pr = tab[sg * FCC_TAB_A + n1 * (FCC_Z2 + 1) + n2] *
+                      tab[2 * FCC_TAB_A + sg * FCC_TAB_B + n3 * (FCC_ZT + 1) + t];
+    const double uu = ((double)r.v[v] + 0.5) * 0x1p-32;
+    if (s0 + v < cnt && uu < pr) {
+      flips |= (1ull << h) | (1ull << (h + L0));
+      const int ds = 1 - 2 * sg;
+      d.N1 += ds;
+      d.P1 += ds * n1;
+      d.P2 += ds * n2;
+      d.P3 += ds * n3;
+      d.T += ds * t;
+      d.acc += 1;
+    }
+  }
+  if (flips) atomicXor(reinterpret_cast<unsigned long long *>(w + own), (unsigned long long)flips);
+}
+
+__global__ __launch_bounds__(1024) void fcc_bits_kernel(
+    const CheckerParams *__restrict__ Pp, CheckerState *__restrict__ Sp,
+    int8_t *__restrict__ lat_pool, const double *__restrict__ tables,
+    const uint8_t *__restrict__ tri_g, mcg_sample *__restrict__ series, int64_t series_cap,
+    const uint8_t *__restrict__ active, int64_t n_passes, int64_t sample_period) {
+  const int c = blockIdx.x;
+  if (!active[c]) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double tab[FCC_NTAB];
+  __shared__ uint8_t tri[4096];
+  __shared__ FccLists Ls;
+  __shared__ long long red[6][16];
+  const CheckerParams P = Pp[c];
+  CheckerState *St = Sp + c;
+  uint8_t *g_lat = reinterpret_cast<uint8_t *>(lat_pool + P.lat_off);
+  uint64_t *w = reinterpret_cast<uint64_t *>(smem);
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int i = t; i < FCC_NTAB; i += nt) tab[i] = tables[P.tab_off + i];
+  for (int i = t; i < 4096; i += nt) tri[i] = tri_g[i];
+  const int L0 = P.L0, L1 = P.L1, L2 = P.L2;
+  if (t < 4) fcc_fill_lists(Ls, t, L0, L1, L2);
+  const int n_lines = 4 * L1 * L2;
+  for (int ln = t; ln < n_lines; ln += nt) {
+    uint64_t core = 0;
+    for (int h = 0; h < L0; ++h) core |= (uint64_t)(g_lat[ln * L0 + h] & 1) << h;
+    w[fcc_word_index(ln)] = core | (core << L0);
+  }
+  __syncthreads();
+  int64_t pass = St->n_pass;
+  int64_t N1 = St->M, P1 = St->S1, P2 = St->S2, P3 = St->S3, T3 = St->T3;
+  int64_t nacc_tot = St->n_accept, n_samples = St->n_samples;
+  const int wid = t >> 6, lane = t & 63, nw = nt >> 6;
+  for (int64_t it = 0; it < n_passes; ++it) {
+    FccDelta d;
+    for (int col = 0; col < 64; ++col) {
+      const FccClass k = fcc_class(Ls, col);
+      const int units = k.start[4];
+      if (units == 0) continue;  // the same for every thread
+      for (int u = t; u < units; u += nt)
+        fcc_bits_unit(w, tab, tri, Ls, L0, L1, L2, k, u, pass, P.key0, P.key1, d);
+      __syncthreads();
+    }
+    const int v0 = fcc_wave_sum(d.N1), v1 = fcc_wave_sum(d.P1), v2 = fcc_wave_sum(d.P2);
+    const int v3 = fcc_wave_sum(d.P3), v4 = fcc_wave_sum(d.T), v5 = fcc_wave_sum(d.acc);
+    if (lane == 0) {
+      red[0][wid] = v0;
+      red[1][wid] = v1;
+      red[2][wid] = v2;
+      red[3][wid] = v3;
+      red[4][wid] = v4;
+      red[5][wid] = v5;
+    }
+    __syncthreads();
+    long long a[6] = {0, 0, 0, 0, 0, 0};
+    for (int ww = 0; ww < nw; ++ww)
+#pragma unroll
+      for (int kk = 0; kk < 6; ++kk) a[kk] += red[kk][ww];
+    __syncthreads();
+    N1 += a[0];
+    P1 += a[1];
+    P2 += a[2];
+    P3 += a[3];
+    T3 += a[4];
+    nacc_tot += a[5];
+    ++pass;
+    if (pass % sample_period == 0) {
+      if (t == 0 && n_samples < series_cap) {
+        mcg_sample rec = {};
+        rec.pass = pass;
+        rec.S1 = P1;
+        rec.S2 = P2;
+        rec.M = N1;
+        rec.S3 = P3;
+        rec.T3 = T3;
+        series[(int64_t)c * series_cap + n_samples] = rec;
+      }
+      ++n_samples;
+    }
+  }
+  for (int ln = t; ln < n_lines; ln += nt) {
+    const uint64_t core = w[fcc_word_index(ln)];
+    for (int h = 0; h < L0; ++h) g_lat[ln * L0 + h] = (uint8_t)((core >> h) & 1);
+  }
+  if (t == 0) {
+    St->n_pass = pass;
+    St->M = N1;
+    St->S1 = P1;
+    St->S2 = P2;
+    St->S3 = P3;
+    St->T3 = T3;
+    St->n_accept = nacc_tot;
+    St->n_samples = n_samples;
+  }
+}
+
 }  // namespace
 
 void launch_fcc(const CheckerParams *P, CheckerState *St, int8_t *lat_pool, const double *tables,
-                const uint8_t *tri, mcg_sample *series, int64_t series_cap,
-                const uint8_t *active, int n_chains, int64_t n_passes, int64_t sample_period,
-                int lds_bytes, bool lattice_in_lds, int block, hipStream_t stream) {
-  if (lattice_in_lds)
+                const uint8_t *tri, const uint8_t *tri_lines, mcg_sample *series,
+                int64_t series_cap, const uint8_t *active, int n_chains, int64_t n_passes,
+                int64_t sample_period, int lds_bytes, bool lattice_in_lds, bool bit_lines,
+                int block, hipStream_t stream) {
+  if (bit_lines)
+    hipLaunchKernelGGL(fcc_bits_kernel, dim3(n_chains), dim3(block), lds_bytes, stream, P, St,
+                       lat_pool, tables, tri_lines, series, series_cap, active, n_passes,
+                       sample_period);
+  else if (lattice_in_lds)
     hipLaunchKernelGGL(fcc_kernel<true>, dim3(n_chains), dim3(block), lds_bytes, stream, P, St,
                        lat_pool, tables, tri, series, series_cap, active, n_passes,
                        sample_period);
@@ -341,6 +552,9 @@ void launch_fcc(const CheckerParams *P, CheckerState *St, int8_t *lat_pool, cons
 }
 
 hipError_t fcc_set_max_lds(int bytes) {
+  const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(fcc_bits_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return e;
   return hipFuncSetAttribute(reinterpret_cast<const void *>(fcc_kernel<true>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }

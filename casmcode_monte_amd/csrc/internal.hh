@@ -48,12 +48,16 @@ struct ExactState {
   double pending_dt;     // its time increment
   int64_t next_k;        // next time-grid sample index (sample at next_k * period)
   int64_t frozen;        // set when the total rate reached 0 on the time grid
+  // EXACT: attempts done in the current pass (0 <= step < n; runs of whole
+  // passes keep it, runs of single steps move it)
+  int64_t step;
 };
 
+// Every active chain makes n_passes * n + n_steps attempts from where it stands.
 void launch_exact(const ExactParams *P, ExactState *S, int8_t *lat_pool, mcg_sample *series,
                   int64_t series_cap, const uint8_t *active, int n_chains, int64_t n_passes,
-                  int64_t sample_period, int lds_bytes, bool lattice_in_lds, bool block_walk,
-                  hipStream_t stream);
+                  int64_t n_steps, int64_t sample_period, int lds_bytes, bool lattice_in_lds,
+                  bool block_walk, hipStream_t stream);
 
 // ---------------------------------------------------------------------------
 // CANONICAL mode: pair swaps at fixed composition, one chain per wavefront.
@@ -160,6 +164,19 @@ void launch_checker_color(const CheckerParams *P, CheckerState *St, int8_t *lat_
                           const uint32_t *tables, const int32_t *work, int n_work_items,
                           int color, int upt, hipStream_t stream);
 
+// Chains with an odd edge (checker_ring.hip): ring colourings, natural layout.
+void launch_checker_ring_lds(const CheckerParams *P, CheckerState *St, int8_t *lat_pool,
+                             const uint32_t *tables, mcg_sample *series, int64_t series_cap,
+                             const uint8_t *active, int n_chains, int64_t n_passes,
+                             int64_t sample_period, int lds_bytes, int block,
+                             hipStream_t stream);
+// the class launches of one HBM pass of the listed chains (the pass counter and
+// the record follow in launch_checker_record)
+void launch_checker_ring_pass(const CheckerParams *P, CheckerState *St, int8_t *lat_pool,
+                              const uint32_t *tables, const int32_t *chains, int n_ring,
+                              int per_chain, int ndim, hipStream_t stream);
+hipError_t checker_ring_set_max_lds(int bytes);
+
 void launch_checker_record(const CheckerParams *P, CheckerState *St, mcg_sample *series,
                            int64_t series_cap, const uint8_t *active, int n_chains,
                            int64_t sample_period, hipStream_t stream);
@@ -197,10 +214,13 @@ hipError_t checker_band_set_max_lds(int bytes);
 // FCC cluster expansion (fcc.hip): CheckerParams with ncolor = 64, tab_off in
 // doubles into the per-chain acceptance tables, tri = the 4096-entry
 // triangle-count table of the nearest-neighbour occupancy mask.
+// bit_lines: the bit-line kernel (every chain L0 <= 32, lattice in LDS as one
+// 64-bit word per line, tri_lines the triangle table in its mask bit order)
 void launch_fcc(const CheckerParams *P, CheckerState *St, int8_t *lat_pool, const double *tables,
-                const uint8_t *tri, mcg_sample *series, int64_t series_cap,
-                const uint8_t *active, int n_chains, int64_t n_passes, int64_t sample_period,
-                int lds_bytes, bool lattice_in_lds, int block, hipStream_t stream);
+                const uint8_t *tri, const uint8_t *tri_lines, mcg_sample *series,
+                int64_t series_cap, const uint8_t *active, int n_chains, int64_t n_passes,
+                int64_t sample_period, int lds_bytes, bool lattice_in_lds, bool bit_lines,
+                int block, hipStream_t stream);
 
 // ---------------------------------------------------------------------------
 // Statistics on the device sample series
@@ -223,5 +243,9 @@ void launch_weighted_stats(const mcg_sample *series, int64_t first, int64_t coun
                            int method, int64_t n_resamples, double *d_out, hipStream_t stream);
 void launch_equilibration(const mcg_sample *series, int64_t count, ObsParams op, double prec,
                           int weighted, int64_t *d_out, hipStream_t stream);
+// Equilibration and statistics of n chains in one launch (check_batch_kernel).
+void launch_check_batch(const mcg_sample *series, int64_t series_cap, const int64_t *chains,
+                        const int64_t *counts, const ObsParams *ops, const int64_t *scratch_off,
+                        int n, double prec, double *scratch, double *out, hipStream_t stream);
 
 }  // namespace mcg

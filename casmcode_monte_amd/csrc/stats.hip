@@ -17,6 +17,7 @@
 
 namespace mcg {
 
+
 __device__ __forceinline__ double observable_of(const mcg_sample &r, const ObsParams &op) {
   const double N = (double)op.n_sites;
   if (op.fcc) {
@@ -105,13 +106,11 @@ __device__ double lag_factor(const double *__restrict__ d, int64_t count, double
 }
 
 // out[0] = mean, out[1] = var0, out[2] = autocorrelation factor, out[3] = lag
-// d_centered: scratch of `count` doubles
-__global__ __launch_bounds__(256) void stats_kernel(const mcg_sample *__restrict__ series,
-                                                    int64_t first, int64_t count, ObsParams op,
-                                                    double *__restrict__ d_centered,
-                                                    double *__restrict__ out) {
-  __shared__ double red[256];
-  __shared__ int best;
+// d_centered: scratch of `count` doubles; the whole 256-thread block takes part
+__device__ void block_stats(const mcg_sample *__restrict__ series, int64_t first, int64_t count,
+                            const ObsParams &op, double *__restrict__ d_centered,
+                            double *__restrict__ out, double *red, int *best_p) {
+  int &best = *best_p;
   const int t = threadIdx.x;
   double s = 0.0;
   for (int64_t j = t; j < count; j += 256) s = __dadd_rn(s, observable_of(series[first + j], op));
@@ -132,6 +131,16 @@ __global__ __launch_bounds__(256) void stats_kernel(const mcg_sample *__restrict
     out[2] = factor;
     out[3] = lag;
   }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void stats_kernel(const mcg_sample *__restrict__ series,
+                                                    int64_t first, int64_t count, ObsParams op,
+                                                    double *__restrict__ d_centered,
+                                                    double *__restrict__ out) {
+  __shared__ double red[256];
+  __shared__ int best;
+  block_stats(series, first, count, op, d_centered, out, red, &best);
 }
 
 // Weighted observations (the records' weights), BasicStatistics.cc:144-188:
@@ -212,10 +221,9 @@ __global__ __launch_bounds__(256) void weighted_stats_kernel(
 // (default_equilibration_check, EquilibrationCheck.cc:134-160) each observation
 // is scaled by (N/W) * w_i first, W the sum of the weights in order.
 // out[0] = is_equilibrated, out[1] = N_samples_for_equilibration
-__global__ void equilibration_kernel(const mcg_sample *__restrict__ series, int64_t count,
-                                     ObsParams op, double prec, int weighted,
-                                     int64_t *__restrict__ out) {
-  if (threadIdx.x != 0) return;
+__device__ void equilibration_scan(const mcg_sample *__restrict__ series, int64_t count,
+                                   const ObsParams &op, double prec, int weighted,
+                                   int64_t *__restrict__ out) {
   double wf = 0.0;
   if (weighted) {
     double W = 0.0;
@@ -269,6 +277,55 @@ __global__ void equilibration_kernel(const mcg_sample *__restrict__ series, int6
   out[1] = start1;
 }
 
+__global__ void equilibration_kernel(const mcg_sample *__restrict__ series, int64_t count,
+                                     ObsParams op, double prec, int weighted,
+                                     int64_t *__restrict__ out) {
+  if (threadIdx.x != 0) return;
+  equilibration_scan(series, count, op, prec, weighted, out);
+}
+
+// The completion check of a list of chains at once (one block per chain): the
+// equilibration scan of the whole series at precision prec (one lane), then,
+// when equilibrated with samples left, block_stats over the samples after the
+// equilibration index.  out: 8 doubles per chain -- is_equilibrated,
+// n_equilibration, mean, var0, autocorrelation factor, lag, count of the tail
+// (0 when no statistics were computed); scratch: the centred tails at
+// scratch_off[b].
+__global__ __launch_bounds__(256) void check_batch_kernel(
+    const mcg_sample *__restrict__ series, int64_t series_cap,
+    const int64_t *__restrict__ chains, const int64_t *__restrict__ counts,
+    const ObsParams *__restrict__ ops, const int64_t *__restrict__ scratch_off, double prec,
+    double *__restrict__ scratch, double *__restrict__ out) {
+  __shared__ double red[256];
+  __shared__ int best;
+  __shared__ int64_t eq[2];
+  __shared__ double res[4];
+  const int b = blockIdx.x;
+  const int64_t count = counts[b];
+  const mcg_sample *S = series + chains[b] * series_cap;
+  const ObsParams op = ops[b];
+  if (threadIdx.x == 0) equilibration_scan(S, count, op, prec, 0, eq);
+  __syncthreads();
+  const int64_t is_eq = eq[0], n_eq = eq[1];
+  double *o = out + 8 * (int64_t)b;
+  if (is_eq && n_eq < count) {
+    block_stats(S, n_eq, count - n_eq, op, scratch + scratch_off[b], res, red, &best);
+    if (threadIdx.x == 0) {
+      o[2] = res[0];
+      o[3] = res[1];
+      o[4] = res[2];
+      o[5] = res[3];
+      o[6] = (double)(count - n_eq);
+    }
+  } else if (threadIdx.x == 0) {
+    o[2] = o[3] = o[4] = o[5] = o[6] = 0.0;
+  }
+  if (threadIdx.x == 0) {
+    o[0] = (double)is_eq;
+    o[1] = (double)n_eq;
+  }
+}
+
 void launch_stats(const mcg_sample *series, int64_t first, int64_t count, ObsParams op,
                   double *d_out, hipStream_t stream) {
   // d_out: 4 result doubles followed by `count` doubles of scratch
@@ -281,6 +338,13 @@ void launch_weighted_stats(const mcg_sample *series, int64_t first, int64_t coun
   // d_out: 8 result doubles followed by n_resamples doubles of scratch
   hipLaunchKernelGGL(weighted_stats_kernel, dim3(1), dim3(256), 0, stream, series, first, count,
                      op, method, n_resamples, d_out + 8, d_out);
+}
+
+void launch_check_batch(const mcg_sample *series, int64_t series_cap, const int64_t *chains,
+                        const int64_t *counts, const ObsParams *ops, const int64_t *scratch_off,
+                        int n, double prec, double *scratch, double *out, hipStream_t stream) {
+  hipLaunchKernelGGL(check_batch_kernel, dim3(n), dim3(256), 0, stream, series, series_cap,
+                     chains, counts, ops, scratch_off, prec, scratch, out);
 }
 
 void launch_equilibration(const mcg_sample *series, int64_t count, ObsParams op, double prec,

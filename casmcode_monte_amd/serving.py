@@ -98,9 +98,8 @@ class _Pool:
             self.run = Run(chains, mode=MODE_CHECKERBOARD, lattice=LATTICE_FCC, eci=eci,
                            options=opts)
         else:
-            e = max_edge - (max_edge & 1)
-            chains = [dict(shape=(e, e), temperature=1000.0, exchange_potential=0.0, seed=i)
-                      for i in range(n_slots)]
+            chains = [dict(shape=(max_edge, max_edge), temperature=1000.0,
+                           exchange_potential=0.0, seed=i) for i in range(n_slots)]
             self.run = Run(chains, mode=MODE_CHECKERBOARD, lattice=LATTICE_SQUARE, J1=J1,
                            J2=J2, options=opts)
         self.active = np.zeros(n_slots, dtype=np.uint8)
@@ -133,8 +132,8 @@ class Server:
         self.errors = []
 
     def pool_for(self, req):
-        if req.model == "ising" and (req.edge % 2 or req.edge < 2):
-            raise ValueError("Ising edge lengths must be even (checkerboard colouring)")
+        if req.model == "ising" and req.edge < 2:
+            raise ValueError("Ising edge lengths must be >= 2")
         if req.model == "fcc" and req.edge < 3:
             raise ValueError("FCC edges must be >= 3 conventional cells")
         for p in self.pools:
@@ -167,15 +166,30 @@ class Server:
         if changed:
             p.run.set_active(p.active)
 
-    def _check(self, p, s, t0):
-        """Completion check of slot s; returns a Result when the request is done."""
+    def _check_due(self, p, due, t0):
+        """Completion checks of the due slots, batched per requested precision
+        (mcg_check_batch: one launch for all of them); returns {slot: Result} for
+        the requests that are done."""
+        done = {}
+        by_prec = {}
+        for s in due:
+            by_prec.setdefault(p.slot_req[s].precision, []).append(int(s))
+        for prec, slots in by_prec.items():
+            checks = p.run.check_batch(slots, OBS_PARAM_COMPOSITION, prec)
+            for s, ck in zip(slots, checks):
+                res = self._result(p, s, t0, ck)
+                if res is not None:
+                    done[s] = res
+        return done
+
+    def _result(self, p, s, t0, ck):
+        """The Result of slot s from its completion check, or None if it goes on."""
         req = p.slot_req[s]
         n = p.run.n_samples(s)
-        converged, n_eq, mean, prec = False, 0, float("nan"), float("inf")
-        is_eq, n_eq = p.run.equilibration(s, OBS_PARAM_COMPOSITION, req.precision)
+        is_eq, n_eq = ck["is_equilibrated"], ck["n_equilibration"]
+        converged, mean, prec = False, float("nan"), float("inf")
         if is_eq and n_eq < n:
-            st = p.run.stats(s, OBS_PARAM_COMPOSITION, first=n_eq)
-            mean, prec = st["mean"], st["calculated_precision"]
+            mean, prec = ck["mean"], ck["calculated_precision"]
             converged = prec < req.precision
         if not converged and p.passes[s] < req.max_passes:
             return None
@@ -202,13 +216,11 @@ class Server:
                 p.attempts += att
                 p.passes[p.active == 1] += p.chunk
                 retired = False
-                for s in np.flatnonzero(p.active):
+                due = [s for s in np.flatnonzero(p.active)
+                       if p.passes[s] % p.check_period == 0
+                       or p.passes[s] >= p.slot_req[s].max_passes]
+                for s, res in self._check_due(p, due, t0).items():
                     req = p.slot_req[s]
-                    if p.passes[s] % p.check_period and p.passes[s] < req.max_passes:
-                        continue
-                    res = self._check(p, s, t0)
-                    if res is None:
-                        continue
                     self.results[req.rid] = res
                     with self.lock:
                         self.in_use -= req.footprint()
@@ -254,15 +266,16 @@ def poisson_trace(n, rate=50.0, seed=2024, edge_range=(16, 512), T_range=(300.0,
                   mu_range=(-1.0, 1.0), max_passes=20000, precision=1e-3, fcc_fraction=0.0,
                   fcc_edge_range=(8, 48)):
     """Request trace: Poisson arrivals at `rate` per second, T and mu uniform.
-    Ising edges are log-uniform in edge_range rounded to the nearest even
-    integer; with fcc_fraction > 0 each request is an FCC run with that
+    Ising edges are log-uniform in edge_range rounded to the nearest integer
+    (odd edges run with ring colourings); with fcc_fraction > 0 each request is
+    an FCC run with that
     probability (drawn after the Ising trace, so fcc_fraction = 0 reproduces it),
     its edge log-uniform in fcc_edge_range rounded to the nearest integer."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, size=n))
     lo, hi = np.log(edge_range[0]), np.log(edge_range[1])
     edges = np.exp(rng.uniform(lo, hi, size=n))
-    edges = np.clip(2 * np.round(edges / 2), edge_range[0], edge_range[1]).astype(int)
+    edges = np.clip(np.round(edges), edge_range[0], edge_range[1]).astype(int)
     T = rng.uniform(*T_range, size=n)
     mu = rng.uniform(*mu_range, size=n)
     models = ["ising"] * n

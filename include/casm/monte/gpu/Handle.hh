@@ -74,6 +74,12 @@ class Handle {
     set_engine_words(e, w);
   }
   void run(CountType n_passes) { check(mcg_run(m_h, n_passes)); }
+  /// EXACT: exactly n_steps more attempts (may end inside a pass).
+  void run_steps(CountType n_steps) { check(mcg_run_steps(m_h, n_steps)); }
+  /// Pass count, steps into the current pass and FP64 running energy per unit cell.
+  void position(Index chain, int64_t &n_pass, int64_t &step, double &e_run) const {
+    check(mcg_get_position(m_h, chain, &n_pass, &step, &e_run));
+  }
   Index n_samples(Index chain) const {
     int64_t n = 0;
     check(mcg_n_samples(m_h, chain, &n));

@@ -1,25 +1,14 @@
 // Drop-in counterpart of libcasm-monte's SGC Ising calculator
 // (ising_cpp/basic_semigrand_canonical.hh:36-627) and the data/loop of
 // methods/basic_occupation_metropolis.hh:18-425, with the main loop on the
-// MI355X.
+// MI355X (the loop and the argument for running it in device launches are in
+// ising_cpp/sgc_device_run.hh).
 //
 // SemiGrandCanonicalCalculator::run keeps the reference signature and
 // semantics: the same engine stream is consumed (the shared engine is loaded
 // into the device chain and written back at the end), the same samples are
 // taken, and the completion check is called with the same arguments at every
-// point where its answer can change.  The device runs the passes between those
-// points in one launch:
-//   - inside a pass the check sees a constant count and sample count, so only
-//     its first call in a pass can return true; the other calls can only
-//     repeat a periodic check on the same samples, which the host replays;
-//   - between two samples nothing the check reads changes except the count;
-//   - so the loop stops the device at the next sample at which a periodic
-//     check, min/max_sample or a clock-time cutoff is due, or at the next
-//     min/max_count.
-// Sampling functions created by make_*_f are evaluated from the device's
-// exact integer records with the reference's FP64 rounding; any other function
-// is called after the host configuration has been refreshed (one sample per
-// launch in that case).
+// point where its answer can change.
 #ifndef CASM_MI355X_monte_ising_cpp_basic_semigrand_canonical
 #define CASM_MI355X_monte_ising_cpp_basic_semigrand_canonical
 
@@ -28,6 +17,7 @@
 #include "casm/monte/events/OccEvent.hh"
 #include "casm/monte/gpu/Handle.hh"
 #include "casm/monte/ising_cpp/model.hh"
+#include "casm/monte/ising_cpp/sgc_device_run.hh"
 
 namespace CASM {
 namespace monte {
@@ -259,16 +249,7 @@ class SemiGrandCanonicalCalculator {
   /// the reference's FP64 rounding (model.hh:266-295, 412-422;
   /// basic_semigrand_canonical.hh:165-174).
   double device_observable(int which) const {
-    mcg_sample const &r = *current_record;
-    double N = static_cast<double>(state->configuration.n_unitcells);
-    double Ef = formation_energy_calculator->use_nlist()
-                    ? static_cast<double>(r.S1) * -formation_energy_calculator->J
-                    : r.e_fold;
-    if (which == MCG_OBS_FORMATION_ENERGY) return Ef / N;
-    double Nx = static_cast<double>(state->configuration.n_sites + r.M) / 2.0;
-    if (which == MCG_OBS_PARAM_COMPOSITION) return Nx / N;
-    double mu = conditions->exchange_potential(0);
-    return (Ef - mu * Nx) / N;
+    return detail::record_observable(*this, *current_record, which);
   }
 
   template <typename WriteStatusF = void (*)(SemiGrandCanonicalCalculator const &, MethodLog &)>
@@ -284,135 +265,15 @@ class SemiGrandCanonicalCalculator {
     state = &_state;
     conditions = std::make_shared<SemiGrandCanonicalConditions>(
         SemiGrandCanonicalConditions::from_values(state->conditions));
-    if (conditions->exchange_potential.size() != 1)
-      throw std::runtime_error("Ising SGC: exchange_potential must have one component");
-    const double temperature = conditions->temperature;
     const CountType N = state->configuration.n_variable_sites;
     potential.set_state(state, conditions);
     event_generator.set_state(state);
     data = std::make_shared<SemiGrandCanonicalData>(sampling_functions, json_sampling_functions,
                                                     N, completion_check_params);
-    if (sample_period < 1) throw std::runtime_error("sample_period must be >= 1");
-
-    // basic_occupation_metropolis.hh:364-373
-    RandomNumberGenerator<engine_type> rng(random_engine);
-    if (!method_log.has_value()) {
-      method_log = MethodLog();
-      method_log->logfile_path = std::filesystem::current_path() / "status.json";
-      method_log->log_frequency = 600.0;
-    }
-    method_log->log.restart_clock();
-    method_log->log.begin_lap();
-
-    // device chain
-    mcg_chain_desc ch{};
-    ch.shape[0] = state->configuration.shape[0];
-    ch.shape[1] = state->configuration.shape[1];
-    ch.shape[2] = 1;
-    ch.temperature = temperature;
-    ch.exchange_potential = conditions->exchange_potential(0);
-    ch.seed = 0;
-    mcg_desc d{};
-    d.mode = MCG_MODE_EXACT;
-    d.lattice = MCG_LATTICE_SQUARE;
-    d.use_nlist = formation_energy_calculator->use_nlist() ? 1 : 0;
-    d.J1 = formation_energy_calculator->J;
-    d.J2 = 0.0;
-    d.kb = KB;
-    d.sample_period = sample_period;
-    d.n_chains = 1;
-    d.chains = &ch;
-    d.options = 0;
-    gpu::Handle dev(d);
-    {
-      std::vector<int8_t> occ(state->configuration.occupation().begin(),
-                              state->configuration.occupation().end());
-      dev.set_occupation(0, occ);
-    }
-    dev.load_engine(0, *rng.engine);
-
-    bool host_functions = !json_sampling_functions.empty();
-    for (auto const &p : sampling_functions)
-      if (p.second.device_observable < 0) host_functions = true;
-    auto const &cc_params = data->completion_check.params();
-    const bool clock_cutoffs = cc_params.cutoff_params.min_clocktime.has_value() ||
-                               cc_params.cutoff_params.max_clocktime.has_value();
-    Log &log = method_log->log;
-    auto &cc = data->completion_check;
-
-    while (true) {
-      // the first completion check of the current pass
-      if (cc.is_complete(data->samplers, data->sample_weight, data->n_pass, log)) break;
-      // the other calls of this pass: repeat periodic checks on the same samples
-      for (CountType c = 1; c < N; ++c) {
-        CountType ns = get_n_samples(data->samplers);
-        if (!(cc.results().has_all_minimums_met && ns >= cc.next_check_at())) break;
-        cc.is_complete(data->samplers, data->sample_weight, data->n_pass, log);
-      }
-      // next pass at which a completion-check call can answer differently
-      const CountType ns = get_n_samples(data->samplers);
-      const CountType p0 = data->n_pass;
-      auto pass_of_sample = [&](CountType s) { return s * (CountType)sample_period; };
-      CountType s_next = ns + 1;
-      CountType target;
-      if (cc.results().has_all_minimums_met && cc.next_check_at() <= ns) {
-        // periodic checks still pending after a whole pass of calls: replay
-        // the next pass's calls too
-        target = p0 + 1;
-      } else if (host_functions || clock_cutoffs) {
-        target = pass_of_sample(s_next);
-      } else {
-        CountType s_t = std::max(s_next, cc.next_check_at());
-        auto const &cp = cc_params.cutoff_params;
-        if (cp.min_sample.has_value() && *cp.min_sample > ns) s_t = std::min(s_t, *cp.min_sample);
-        if (cp.max_sample.has_value() && *cp.max_sample > ns) s_t = std::min(s_t, *cp.max_sample);
-        target = pass_of_sample(s_t);
-      }
-      auto const &cp = cc_params.cutoff_params;
-      if (cp.min_count.has_value() && *cp.min_count > p0) target = std::min(target, *cp.min_count);
-      if (cp.max_count.has_value() && *cp.max_count > p0) target = std::min(target, *cp.max_count);
-      target = std::max(target, p0 + 1);
-      target = std::min(target, p0 + (CountType)1000000);
-
-      dev.run(target - p0);
-      const Index n_new = dev.n_samples(0) - ns;
-      std::vector<mcg_sample> recs = dev.samples(0, ns, n_new);
-      if (host_functions) {
-        auto occ = dev.occupation(0, state->configuration.n_sites);
-        state->configuration.set_occupation(std::vector<int>(occ.begin(), occ.end()));
-      }
-      for (auto const &rec : recs) {
-        data->n_pass = rec.pass;
-        current_record = rec;
-        for (auto const &p : data->sampling_functions)
-          data->samplers.at(p.second.name)->push_back(p.second());
-        for (auto const &p : data->json_sampling_functions)
-          data->json_samplers.at(p.second.name).values.push_back(p.second());
-        if (method_log->log_frequency.has_value() &&
-            method_log->log.lap_time() >= method_log->log_frequency.value()) {
-          sync_configuration(dev);
-          write_status_f(*this, *method_log);
-        }
-      }
-      current_record.reset();
-      data->n_pass = target;
-    }
-
-    // final state: configuration, engine and counters
-    sync_configuration(dev);
-    dev.store_engine(0, *rng.engine);
-    int64_t n_pass = 0, n_acc = 0, n_rej = 0;
-    dev.counters(0, n_pass, n_acc, n_rej);
-    data->n_accept = n_acc;
-    data->n_reject = n_rej;
-    write_status_f(*this, *method_log);
+    // the main loop (basic_occupation_metropolis.hh:354-425) on the device
+    detail::run_sgc_on_device(*this, *data, sample_period, method_log, random_engine,
+                              write_status_f);
     return data;
-  }
-
- private:
-  void sync_configuration(gpu::Handle const &dev) {
-    auto occ = dev.occupation(0, state->configuration.n_sites);
-    state->configuration.set_occupation(std::vector<int>(occ.begin(), occ.end()));
   }
 };
 

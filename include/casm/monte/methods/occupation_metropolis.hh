@@ -10,9 +10,10 @@
 // engine shared with the RunManager is advanced exactly as the reference
 // advances it and written back at the end.
 //
-// The device runs whole passes between the points where a fixture samples or
-// a completion check can change its answer (see basic_semigrand_canonical.hh
-// for the argument); sampling by step or by time is not supported.
+// The device runs the steps between the points where a fixture samples or a
+// completion check can change its answer (detail::run_device_steps); fixtures
+// sample by pass or, with the single-site SGC potential, by step; sampling by
+// time has no meaning for Metropolis runs (there is no simulated time).
 //
 // Two entry points: the SGC Ising potential with single-site flips (EXACT
 // mode), and the reference's generic signature with an OccLocation and
@@ -53,67 +54,96 @@ class IsingSemiGrandCanonicalPotential {
 
 namespace detail {
 
-/// Drive a RunManager over device passes of chain 0 of `dev` (one pass = N
-/// steps): the device runs whole passes between the points where a fixture
-/// samples or a completion check can change its answer; counters, the running
-/// potential energy and, when a sampling function reads it, the configuration
-/// (sync_configuration) are brought to the host at those points.
+/// Drive a RunManager over device steps of chain 0 of `dev` (one pass = N
+/// steps; g counts the steps since the start of the run).  The reference calls
+/// the completion check before every step and samples after every step
+/// (occupation_metropolis.hh:118-150).  Between two points where a fixture
+/// samples or a count cutoff falls, the checks see the same samples and only
+/// a changing count, so the device runs to the next such point in one launch:
+///  - BY_PASS fixtures only: counts are constant within a pass, so the calls of
+///    a pass after the first only repeat pending periodic checks, which the
+///    host replays; the device runs whole passes;
+///  - with BY_STEP fixtures (`steps_ok`, EXACT handles): a launch ends at any
+///    step (mcg_run_steps); a periodic check still pending after a call is
+///    due again at the next step's call, so the device then runs one step.
+/// Counters, the running potential energy and, when a sampling function reads
+/// it, the configuration (sync_configuration) come to the host at those points.
 template <typename ConfigType, typename StatisticsType, typename EngineType, typename SyncF>
-void run_device_passes(State<ConfigType> &state, gpu::Handle &dev, CountType N,
-                       RunManager<ConfigType, StatisticsType, EngineType> &run_manager,
-                       SyncF sync_configuration) {
+void run_device_steps(State<ConfigType> &state, gpu::Handle &dev, CountType N, bool steps_ok,
+                      RunManager<ConfigType, StatisticsType, EngineType> &run_manager,
+                      SyncF sync_configuration) {
   double &potential_energy_per_unitcell = state.properties.scalar_values["potential_energy"];
   run_manager.initialize(N);
-  bool host_functions = false;
+  bool host_functions = false, by_step = false;
   for (auto &f : run_manager.sampling_fixtures) {
     auto const &sp = f->params().sampling_params;
-    if (sp.sample_mode != SAMPLE_MODE::BY_PASS)
+    if (sp.sample_mode == SAMPLE_MODE::BY_STEP) {
+      if (!steps_ok)
+        throw std::runtime_error(
+            "occupation_metropolis (MI355X): SAMPLE_MODE::BY_STEP sampling runs with the "
+            "single-site SGC potential only");
+      by_step = true;
+    } else if (sp.sample_mode != SAMPLE_MODE::BY_PASS) {
       throw std::runtime_error(
-          "occupation_metropolis (MI355X): only SAMPLE_MODE::BY_PASS sampling is supported");
+          "occupation_metropolis (MI355X): Metropolis runs sample by pass or by step "
+          "(no simulated time)");
+    }
     if (sp.do_sample_trajectory || !sp.json_sampler_names.empty()) host_functions = true;
     for (auto const &name : sp.sampler_names)
       if (f->params().sampling_functions.at(name).device_observable < 0) host_functions = true;
   }
   run_manager.sample_data_by_count_if_due(state);
+  // a fixture's count c in steps since the start
+  auto steps_of = [&](auto const &f, CountType c) {
+    return f->params().sampling_params.sample_mode == SAMPLE_MODE::BY_STEP ? c : c * N;
+  };
+  auto pending = [&]() {
+    for (auto &f : run_manager.sampling_fixtures) {
+      auto const &cc = f->completion_check();
+      if (cc.results().is_complete) continue;
+      if (cc.results().has_all_minimums_met &&
+          get_n_samples(f->results().samplers) >= cc.next_check_at())
+        return true;
+    }
+    return false;
+  };
 
+  CountType g = 0;
   int64_t acc_prev = 0, rej_prev = 0;
   while (true) {
     if (run_manager.is_complete()) break;
-    // the other calls of this pass only repeat periodic checks on the same samples
-    auto pending = [&]() {
-      for (auto &f : run_manager.sampling_fixtures) {
-        auto const &cc = f->completion_check();
-        if (cc.results().is_complete) continue;
-        if (cc.results().has_all_minimums_met &&
-            get_n_samples(f->results().samplers) >= cc.next_check_at())
-          return true;
-      }
-      return false;
-    };
-    for (CountType c = 1; c < N && pending(); ++c) run_manager.is_complete();
+    if (!by_step)
+      for (CountType c = 1; c < N && pending(); ++c) run_manager.is_complete();
     run_manager.write_status_if_due();
 
-    const CountType p0 = run_manager.sampling_fixtures.empty()
-                             ? 0
-                             : run_manager.sampling_fixtures.front()->counter().pass;
-    CountType target = p0 + 1000000;
-    if (pending()) target = p0 + 1;
+    // the earliest next point: the next step, or the next pass boundary when
+    // every fixture counts passes
+    const CountType next = by_step ? g + 1 : g + N;
+    CountType target = g + 1000000 * N;
+    if (pending()) target = next;
     for (auto &f : run_manager.sampling_fixtures) {
-      target = std::min(target, std::max(f->next_sample_count(), p0 + 1));
+      const CountType count = f->counter().count;
+      target = std::min(target, std::max(steps_of(f, f->next_sample_count()), next));
       auto const &cp = f->params().completion_check_params.cutoff_params;
-      if (cp.min_count.has_value() && *cp.min_count > p0) target = std::min(target, *cp.min_count);
-      if (cp.max_count.has_value() && *cp.max_count > p0) target = std::min(target, *cp.max_count);
+      if (cp.min_count.has_value() && *cp.min_count > count)
+        target = std::min(target, steps_of(f, *cp.min_count));
+      if (cp.max_count.has_value() && *cp.max_count > count)
+        target = std::min(target, steps_of(f, *cp.max_count));
     }
-    dev.run(target - p0);
-    mcg_sample last = dev.samples(0, dev.n_samples(0) - 1, 1)[0];
-    dev.clear_samples();
-    int64_t np = 0, na = 0, nr = 0;
+    if (by_step)
+      dev.run_steps(target - g);
+    else
+      dev.run((target - g) / N);
+    int64_t np = 0, step = 0, na = 0, nr = 0;
+    double e_run = 0.0;
     dev.counters(0, np, na, nr);
+    dev.position(0, np, step, e_run);
     for (auto &f : run_manager.sampling_fixtures)
-      f->counter().increment_passes(target - p0, na - acc_prev, nr - rej_prev);
+      f->counter().increment_steps(target - g, na - acc_prev, nr - rej_prev);
     acc_prev = na;
     rej_prev = nr;
-    potential_energy_per_unitcell = last.e_run;
+    g = target;
+    potential_energy_per_unitcell = e_run;
     if (host_functions) sync_configuration();
     run_manager.sample_data_by_count_if_due(state);
   }
@@ -147,7 +177,7 @@ void occupation_metropolis(
   d.use_nlist = potential.use_nlist ? 1 : 0;
   d.J1 = potential.J;
   d.kb = KB;
-  d.sample_period = 1;
+  d.sample_period = (int64_t)1 << 62;  // no records: samples read the chain's position
   d.n_chains = 1;
   d.chains = &ch;
   gpu::Handle dev(d);
@@ -155,7 +185,7 @@ void occupation_metropolis(
   dev.load_engine(0, *random_number_generator.engine);
 
   const CountType N = config.n_sites;
-  detail::run_device_passes(state, dev, N, run_manager, [&]() {
+  detail::run_device_steps(state, dev, N, true, run_manager, [&]() {
     auto occ = dev.occupation(0, N);
     config.set_occupation(std::vector<int>(occ.begin(), occ.end()));
   });
@@ -228,7 +258,7 @@ void occupation_metropolis(
   d.use_nlist = potential.use_nlist ? 1 : 0;
   d.J1 = potential.J;
   d.kb = KB;
-  d.sample_period = 1;
+  d.sample_period = (int64_t)1 << 62;  // no records: samples read the chain's position
   d.n_chains = 1;
   d.chains = &ch;
   gpu::Handle dev(d);
@@ -241,7 +271,7 @@ void occupation_metropolis(
   dev.set_occ_location(0, loc);
   dev.load_engine(0, *random_number_generator.engine);
 
-  detail::run_device_passes(state, dev, N, run_manager, [&]() {
+  detail::run_device_steps(state, dev, N, false, run_manager, [&]() {
     auto occ = dev.occupation(0, N);
     for (Index l = 0; l < N; ++l) config.set_occ(l, occ[l] > 0 ? 1 : 0);
   });

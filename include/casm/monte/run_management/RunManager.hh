@@ -225,6 +225,16 @@ struct MonteCounter {
     n_accept += accepted;
     n_reject += rejected;
   }
+  /// Advance by k steps from any position (k calls of increment_step).
+  void increment_steps(CountType k, BigCountType accepted, BigCountType rejected) {
+    const CountType s = step + k;
+    const CountType passes = s / steps_per_pass;
+    pass += passes;
+    step = s - passes * steps_per_pass;
+    count += (sample_mode == SAMPLE_MODE::BY_STEP) ? k : passes;
+    n_accept += accepted;
+    n_reject += rejected;
+  }
   void set_time(double t) { time = t; }
 };
 

@@ -232,6 +232,24 @@ int mcg_rng_load(mcg_handle *h, int64_t chain, const uint64_t *words, int64_t n_
  * (nfold.hh:121-126), taken only once the caller knows the run goes on. */
 int mcg_run(mcg_handle *h, int64_t n_passes);
 
+/* EXACT: advance every active chain by exactly n_steps attempts from where it
+ * stands, which may end inside a pass; the next mcg_run or mcg_run_steps goes
+ * on from that step (mcg_run then makes whole passes' worth of attempts, so a
+ * chain keeps its offset into the pass).  Records are appended at pass ends as
+ * in mcg_run.  The counterpart of the reference's per-step counting for
+ * fixtures that sample by step (MonteCounter::increment_step with
+ * SAMPLE_MODE::BY_STEP, run_management/SamplingFixture.hh:154-169, sampled
+ * after every step by methods/occupation_metropolis.hh:150).  Other modes:
+ * MCG_ERR_STATE. */
+int mcg_run_steps(mcg_handle *h, int64_t n_steps);
+
+/* EXACT, CANONICAL, NFOLD: a chain's pass count, its steps into the current
+ * pass (EXACT; 0 otherwise) and its FP64 running potential energy per unit
+ * cell (occupation_metropolis.hh:103-106, :141; EXACT and CANONICAL) -- the
+ * state a sample taken between two steps reads. */
+int mcg_get_position(mcg_handle *h, int64_t chain, int64_t *n_pass, int64_t *step,
+                     double *e_run);
+
 /* Mark chains active (1) or inactive (0); inactive chains are skipped by
  * mcg_run.  Used by the continuous-batching server to retire converged
  * chains while others keep running. */
@@ -324,6 +342,25 @@ int mcg_stats_weighted(mcg_handle *h, int64_t chain, int64_t observable, int64_t
  * in-order sum of the weights (EquilibrationCheck.cc:134-160). */
 int mcg_equilibration(mcg_handle *h, int64_t chain, int64_t observable, double prec,
                       int64_t *is_equilibrated, int64_t *n_samples_for_equilibration);
+
+/* The completion check of many chains at once: for each listed chain, the
+ * equilibration check of one observable over its whole series at absolute
+ * precision prec (as mcg_equilibration), then, when it is equilibrated with
+ * samples left, the statistics of the samples after the equilibration index
+ * (as mcg_stats_compute with first = n_equilibration) -- the sequence of
+ * CompletionCheck::_is_complete (checks/CompletionCheck.hh:272-376: the
+ * equilibration check of EquilibrationCheck.cc:180-225, then the convergence
+ * statistics of ConvergenceCheck.hh:139-184 on the samples after it), in one
+ * launch with one workgroup per chain.  stats.n = 0 when no statistics were
+ * computed.  Results equal the single-chain entry points' bit for bit.
+ * Unweighted records only (NFOLD: MCG_ERR_STATE). */
+typedef struct mcg_check {
+  int64_t is_equilibrated;
+  int64_t n_equilibration;
+  mcg_stats stats;
+} mcg_check;
+int mcg_check_batch(mcg_handle *h, int64_t n, const int64_t *chains, int64_t observable,
+                    double prec, double confidence, mcg_check *out);
 
 /* Launch schedule of a CHECKERBOARD handle (EXACT handles have nothing to
  * tune: one wavefront per chain).  Every schedule produces the same

@@ -806,6 +806,45 @@ inline std::vector<double> sgc_running_energy_run(IsingConfiguration config, dou
   return out;
 }
 
+// The loop of sgc_running_energy_run counted in steps, as a fixture sampling
+// by step sees it (SamplingFixture.hh:154-169 with SAMPLE_MODE::BY_STEP,
+// occupation_metropolis.hh:118-150): the running potential energy per unit
+// cell and the magnetisation after each of n_steps steps (index 0: before the
+// first step), and the final configuration.
+struct StepRunOutput {
+  std::vector<double> pe;
+  std::vector<long long> M;
+  IsingConfiguration final_config;
+};
+inline StepRunOutput sgc_step_run(IsingConfiguration config, double J, bool use_nlist,
+                                  double temperature, double mu,
+                                  std::shared_ptr<std::mt19937_64> engine, long n_steps) {
+  SemiGrandCanonicalPotential potential{IsingFormationEnergy(J, use_nlist)};
+  potential.set_state(&config, mu);
+  const double n_unitcells = static_cast<double>(config.n_sites);
+  double pe = potential.per_supercell() / n_unitcells;
+  const double beta = 1.0 / (KB * temperature);
+  RandomNumberGenerator rng(engine);
+  StepRunOutput out;
+  out.pe.push_back(pe);
+  out.M.push_back(magnetisation(config));
+  long long M = out.M.back();
+  for (long s = 0; s < n_steps; ++s) {
+    long l = rng.random_int(config.n_sites - 1);
+    int new_occ = -config.occ(l);
+    double dE = potential.delta(l, new_occ);
+    if (metropolis_acceptance(dE, beta, rng)) {
+      config.set_occ(l, new_occ);
+      pe += (dE / n_unitcells);
+      M += 2 * new_occ;
+    }
+    out.pe.push_back(pe);
+    out.M.push_back(M);
+  }
+  out.final_config = config;
+  return out;
+}
+
 // ---------------------------------------------------------------------------
 // Build-defined Ising models for the statistical modes: 2D square and 3D
 // simple cubic lattices with nearest (J1) and next-nearest (J2) couplings,

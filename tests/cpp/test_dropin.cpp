@@ -14,6 +14,7 @@
 #include <numeric>
 
 #include "casm/monte/ising_cpp/basic_semigrand_canonical.hh"
+#include "casm/monte/ising_cpp/complete_semigrand_canonical.hh"
 #include "casm/monte/methods/nfold.hh"
 #include "casm/monte/methods/occupation_metropolis.hh"
 #include "refcpu.hh"
@@ -250,6 +251,180 @@ static void gpu_case(Case const &c) {
   }
   std::cout << "gpu: " << tag << " n_samples=" << ns << " n_pass=" << data->n_pass
             << (g_fail ? " (failures so far)" : " ok") << std::endl;
+}
+
+// ising_cpp::complete_semigrand_canonical: the reference test's run set-up
+// (Ising_complete_semigrand_canonical_test.cpp:270-345: 25x25 all +1, T 2000 K,
+// mu 0, J 0.1, the row/column fold, min_sample 100, checks from 100 every 10,
+// abs precision 0.001 on param_composition and potential_energy, the JSON
+// configuration sampler, a status writer held in a std::function), with a
+// seeded engine, vs the oracle's loop bit for bit.
+static void gpu_complete_case(bool use_nlist, uint64_t seed, int sample_period) {
+  namespace csgc = CASM::monte::ising_cpp::complete_semigrand_canonical;
+  typedef csgc::SemiGrandCanonicalEventGenerator<std::mt19937_64> gen_type;
+  typedef csgc::SemiGrandCanonicalCalculator<system_type, gen_type> calc_type;
+  auto system = std::make_shared<system_type>(ising_cpp::IsingFormationEnergy(0.1, 1, use_nlist),
+                                              ising_cpp::IsingParamComposition());
+  auto calc = std::make_shared<calc_type>(system);
+  StateSamplingFunctionMap functions;
+  for (auto f : {csgc::make_parametric_composition_f(calc), csgc::make_formation_energy_f(calc),
+                 csgc::make_potential_energy_f(calc)})
+    functions.emplace(f.name, f);
+  jsonStateSamplingFunctionMap json_functions;
+  {
+    auto jf = csgc::make_configuration_json_f(calc);
+    json_functions.emplace(jf.name, jf);
+  }
+  CompletionCheckParams<BasicStatistics> ccp;
+  ccp.cutoff_params.min_sample = 100;
+  ccp.log_spacing = false;
+  ccp.check_begin = 100;
+  ccp.check_period = 10;
+  converge(functions, ccp).set_precision("param_composition", 0.001)
+      .set_precision("potential_energy", 0.001);
+  ValueMap cond;
+  cond.scalar_values["temperature"] = 2000.0;
+  cond.vector_values["exchange_potential"] = VectorXd{0.0};
+  ising_cpp::IsingState state(ising_cpp::IsingConfiguration({25, 25}, 1), cond);
+  auto engine = std::make_shared<std::mt19937_64>(seed);
+  std::optional<MethodLog> mlog = MethodLog();
+  mlog->log_frequency = 10.0;
+  int n_status = 0;
+  std::function<void(calc_type const &, MethodLog &)> status = [&](calc_type const &c,
+                                                                    MethodLog &ml) {
+    ++n_status;
+    csgc::write_no_status(c, ml);
+  };
+  calc->run(state, functions, json_functions, ccp, gen_type(), sample_period, mlog, engine,
+            status);
+  auto const &data = *calc->data;
+
+  refcpu::IsingConfiguration oc(25, 25, 1);
+  refcpu::CompletionCheckParams q;
+  q.cutoff_params.min_sample = 100;
+  q.check_begin = 100;
+  q.check_period = 10;
+  refcpu::RequestedPrecision rp;
+  rp.abs_convergence_is_required = true;
+  rp.abs_precision = 0.001;
+  q.requested_precision[refcpu::SamplerComponent{"potential_energy", 0}] = rp;
+  q.requested_precision[refcpu::SamplerComponent{"param_composition", 0}] = rp;
+  auto oeng = std::make_shared<std::mt19937_64>(seed);
+  auto o = refcpu::sgc_basic_run(oc, 0.1, use_nlist, 2000.0, 0.0, oeng, q, sample_period);
+
+  const std::string tag = std::string("complete_semigrand_canonical ") +
+                          (use_nlist ? "nlist" : "fold") + " seed " + std::to_string(seed) +
+                          " sp " + std::to_string(sample_period);
+  const CountType ns = get_n_samples(data.samplers);
+  EXPECT(ns == refcpu::get_n_samples(o.samplers), tag << ": n_samples " << ns);
+  for (auto name : {"formation_energy", "param_composition", "potential_energy"}) {
+    VectorXd a = data.samplers.at(name)->component(0);
+    auto b = o.samplers.at(name)->component(0);
+    bool ok = a.size() == (Index)b.size();
+    for (Index i = 0; ok && i < a.size(); ++i) ok = same_bits(a(i), b[i]);
+    EXPECT(ok, tag << ": sampled " << name);
+  }
+  EXPECT((CountType)data.json_samplers.at("configuration").values.size() == ns,
+         tag << ": one configuration per sample");
+  EXPECT(data.n_steps_per_pass == 625, tag << ": n_steps_per_pass = n_sites");
+  EXPECT(data.n_pass == o.n_pass && data.n_accept == o.n_accept && data.n_reject == o.n_reject,
+         tag << ": counts");
+  EXPECT(state.configuration.occupation() == o.final_config.occupation, tag << ": configuration");
+  EXPECT(*engine == *oeng, tag << ": engine state");
+  auto const &ra = data.completion_check.results();
+  EXPECT(ra.is_complete && ra.is_complete == o.results.is_complete &&
+             ra.convergence_check_results.all_converged ==
+                 o.results.convergence_check_results.all_converged &&
+             data.completion_check.n_checks() == o.n_checks,
+         tag << ": completion results");
+  EXPECT(n_status >= 1, tag << ": status written at the end");
+  jsonParser json;
+  csgc::to_json(data, json);
+  EXPECT(json.is_obj() && json.contains("completion_check_results") &&
+             json["n_steps_per_pass"].get<long>() == 625,
+         tag << ": data to_json");
+  std::cout << "gpu: " << tag << " n_samples=" << ns << " n_pass=" << data.n_pass
+            << (g_fail ? " (failures so far)" : " ok") << std::endl;
+}
+
+// occupation_metropolis with a fixture sampling by step (period 37 steps on a
+// 12x10 lattice, so samples fall inside passes; a max_count of 3053 steps cuts
+// the run inside a pass) beside a fixture sampling every pass: the samples,
+// counts, final configuration and engine equal the oracle's step-by-step loop.
+static void gpu_occupation_metropolis_by_step() {
+  const int R = 12, C = 10, N = R * C;
+  const long S = 3053;
+  const double T = 2300.0, mu = 0.05;
+  const uint64_t seed = 9;
+  typedef ising_cpp::IsingConfiguration config_type;
+  ValueMap cond;
+  cond.scalar_values["temperature"] = T;
+  cond.vector_values["exchange_potential"] = VectorXd{mu};
+  State<config_type> state(config_type({R, C}, 1), cond);
+  IsingSemiGrandCanonicalPotential potential(0.1, true);
+  potential.set(&state);
+  StateSamplingFunction pe("potential_energy", "running potential energy", {}, [&state]() {
+    return VectorXd{state.properties.scalar_values.at("potential_energy")};
+  });
+  pe.device_observable = MCG_OBS_POTENTIAL_ENERGY;
+  StateSamplingFunction mag("magnetisation", "sum of spins", {}, [&state]() {
+    double s = 0;
+    for (int v : state.configuration.occupation()) s += v;
+    return VectorXd{s};
+  });
+  StateSamplingFunctionMap fns;
+  fns.emplace(pe.name, pe);
+  fns.emplace(mag.name, mag);
+  SamplingParams by_step;
+  by_step.sample_mode = SAMPLE_MODE::BY_STEP;
+  by_step.sampler_names = {"potential_energy", "magnetisation"};
+  by_step.begin = 5.0;
+  by_step.period = 37.0;
+  CompletionCheckParams<BasicStatistics> cc_step;
+  cc_step.cutoff_params.max_count = S;
+  SamplingParams by_pass;
+  by_pass.sampler_names = {"potential_energy"};
+  CompletionCheckParams<BasicStatistics> cc_pass;
+  typedef RunManager<config_type, BasicStatistics, std::mt19937_64> rm_type;
+  std::vector<rm_type::sampling_fixture_params_type> fixtures;
+  fixtures.emplace_back("steps", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), by_step,
+                        cc_step, std::vector<std::string>{});
+  fixtures.emplace_back("passes", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), by_pass,
+                        cc_pass, std::vector<std::string>{});
+  auto engine = std::make_shared<std::mt19937_64>(seed);
+  rm_type rm(engine, fixtures, true);
+  RandomNumberGenerator<std::mt19937_64> rng(engine);
+  occupation_metropolis(state, potential, rng, rm);
+
+  auto e1 = std::make_shared<std::mt19937_64>(seed);
+  auto o = refcpu::sgc_step_run(refcpu::IsingConfiguration(R, C, 1), 0.1, true, T, mu, e1, S);
+  const std::string tag = "occupation_metropolis by step";
+  auto const &a = rm.sampling_fixtures[0]->results();
+  VectorXd ape = a.samplers.at("potential_energy")->component(0);
+  VectorXd am = a.samplers.at("magnetisation")->component(0);
+  const long n_a = (S - 5) / 37 + 1;  // counts 5, 42, ..., <= S
+  bool ok = ape.size() == n_a && am.size() == n_a && (long)a.sample_count.size() == n_a;
+  for (long k = 0; ok && k < n_a; ++k) {
+    const long g = 5 + 37 * k;
+    ok = same_bits(ape(k), o.pe[g]) && am(k) == (double)o.M[g] && a.sample_count[k] == g;
+  }
+  EXPECT(ok, tag << ": step samples (" << ape.size() << " of " << n_a << ")");
+  auto const &b = rm.sampling_fixtures[1]->results();
+  VectorXd bpe = b.samplers.at("potential_energy")->component(0);
+  const long n_b = S / N;  // passes 1..25
+  ok = bpe.size() == n_b;
+  for (long p = 1; ok && p <= n_b; ++p) ok = same_bits(bpe(p - 1), o.pe[p * N]);
+  EXPECT(ok, tag << ": pass samples (" << bpe.size() << " of " << n_b << ")");
+  auto const &ca = rm.sampling_fixtures[0]->counter();
+  auto const &cb = rm.sampling_fixtures[1]->counter();
+  EXPECT(ca.count == S && ca.pass == S / N && ca.step == S % N && cb.count == S / N,
+         tag << ": counters (" << ca.count << ", " << ca.pass << ", " << ca.step << ")");
+  EXPECT(state.configuration.occupation() == o.final_config.occupation, tag << ": configuration");
+  EXPECT(*engine == *e1, tag << ": engine state");
+  EXPECT(a.n_accept + a.n_reject == S, tag << ": attempts");
+  std::cout << "gpu: " << tag << (g_fail ? " (failures so far)" : " ok") << std::endl;
 }
 
 // RunManager-driven loop: two fixtures, the FP64 running potential energy and
@@ -794,6 +969,9 @@ static void cpu_nfold_argument_tests() {
 }
 
 static void gpu_tests() {
+  gpu_complete_case(false, 42, 1);
+  gpu_complete_case(true, 44, 2);
+  gpu_occupation_metropolis_by_step();
   // the reference's SemiGrandCanonicalRun1 set-up, converged on two observables
   gpu_case({25, 25, true, 2000.0, 0.0, 42, 1, -1, 0.001, 100, false});
   gpu_case({25, 25, false, 2000.0, 0.0, 43, 1, -1, 0.001, 100, false});

@@ -179,3 +179,55 @@ def test_lattice_at_the_lds_limit(shape, use_nlist):
     r = run_gpu(rows, cols, 2400.0, 0.0, 5, 2, use_nlist=use_nlist)
     compare(r, o)
     r.close()
+
+
+@pytest.mark.parametrize("use_nlist", [True, False])
+def test_hbm_lattice_chain_beside_lds_chains(use_nlist):
+    """A 400x400 chain (160000 lattice bytes, beyond the block walk's LDS) runs
+    the scalar walk on its HBM lattice while the small chains of the same batch
+    run the block walk in LDS, on a second stream at the same time: every chain
+    bit-exact against the oracle for 2 passes, run in two launches."""
+    conds = [dict(shape=(400, 400), temperature=2300.0, exchange_potential=0.05, seed=401),
+             dict(shape=(24, 24), temperature=1800.0, exchange_potential=-0.1, seed=402),
+             dict(shape=(9, 31), temperature=3100.0, exchange_potential=0.2, seed=403)]
+    print("case: hbm chain + lds chains use_nlist", use_nlist, flush=True)
+    r = cm.Run(conds, mode=cm.MODE_EXACT, J1=0.1, use_nlist=use_nlist)
+    r.run(1)
+    r.run(1)
+    for i, c in enumerate(conds):
+        rows, cols = c["shape"]
+        o = O.sgc_run(rows, cols, use_nlist=use_nlist, temperature=c["temperature"],
+                      mu=c["exchange_potential"], seed=c["seed"], max_count=2)
+        compare(r, o, c=i)
+    r.close()
+
+
+@pytest.mark.parametrize("shape,steps", [((12, 10), [1, 37, 119, 120, 5, 240, 3]),
+                                          ((64, 64), [4095, 1, 4096, 8191, 2])])
+def test_run_steps_inside_passes(shape, steps):
+    """mcg_run_steps ends launches inside passes (the block walk cut at any
+    attempt): the concatenation equals one uninterrupted run; the position
+    (pass, step) and the records at pass ends follow the step counts."""
+    rows, cols = shape
+    n = rows * cols
+    total = sum(steps)
+    print("case: run_steps", shape, steps, flush=True)
+    a = run_gpu(rows, cols, 2100.0, 0.03, 77, 0)
+    for s in steps:
+        a.run_steps(s)
+    p, st, _ = a.position(0)
+    assert (p, st) == (total // n, total % n)
+    b = run_gpu(rows, cols, 2100.0, 0.03, 77, total // n)
+    b.run_steps(total % n)
+    np.testing.assert_array_equal(a.get_occupation(0), b.get_occupation(0))
+    np.testing.assert_array_equal(a.rng_dump(0), b.rng_dump(0))
+    np.testing.assert_array_equal(a.samples(0), b.samples(0))
+    assert a.counters(0) == b.counters(0) and a.position(0) == b.position(0)
+    o = O.sgc_run(rows, cols, use_nlist=True, temperature=2100.0, mu=0.03, seed=77,
+                  max_count=total // n)
+    rec = a.samples(0)
+    assert len(rec) == total // n
+    np.testing.assert_array_equal(rec["S1"], o["S"])
+    # whole passes after a partial one keep the offset into the pass
+    a.run(2)
+    assert a.position(0)[:2] == (total // n + 2, total % n)

@@ -67,3 +67,31 @@ def test_stats_edge_cases():
     assert s2["n"] == 2
     with pytest.raises(cm.McgError):
         r.stats(0, cm.OBS_POTENTIAL_ENERGY, first=200)  # no observations
+
+
+@pytest.mark.parametrize("mode", [cm.MODE_EXACT, cm.MODE_CHECKERBOARD])
+def test_check_batch_equals_single_chain_checks(mode):
+    """mcg_check_batch (one launch for many chains) gives every chain the
+    equilibration index and tail statistics of mcg_equilibration and
+    mcg_stats_compute bit for bit, in any chain order."""
+    print("case: check_batch mode", mode, flush=True)
+    conds = [dict(shape=(16, 16), temperature=1200.0 + 400.0 * k,
+                  exchange_potential=-0.2 + 0.1 * k, seed=70 + k) for k in range(6)]
+    r = cm.Run(conds, mode=mode, J1=0.1)
+    r.run(1500)
+    chains = [4, 0, 5, 2, 1, 3]
+    for obs in (cm.OBS_PARAM_COMPOSITION, cm.OBS_POTENTIAL_ENERGY):
+        for prec in (1e-3, 1e-2):
+            got = r.check_batch(chains, obs, prec)
+            for c, g in zip(chains, got):
+                is_eq, n_eq = r.equilibration(c, obs, prec)
+                assert (g["is_equilibrated"], g["n_equilibration"]) == (is_eq, n_eq)
+                if is_eq and n_eq < r.n_samples(c):
+                    s = r.stats(c, obs, first=n_eq)
+                    assert g["n"] == s["n"]
+                    for k in ("mean", "variance", "autocorr_factor", "calculated_precision"):
+                        assert np.float64(g[k]).view(np.uint64) == \
+                            np.float64(s[k]).view(np.uint64), k
+                else:
+                    assert g["n"] == 0
+    r.close()
