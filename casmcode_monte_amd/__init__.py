@@ -38,7 +38,7 @@ EXPORTED = [
     "mcg_create", "mcg_destroy", "mcg_last_error", "mcg_set_occupation", "mcg_get_occupation",
     "mcg_rng_dump", "mcg_rng_load", "mcg_run", "mcg_set_active", "mcg_get_counters",
     "mcg_n_samples", "mcg_get_samples", "mcg_clear_samples", "mcg_stats_compute",
-    "mcg_stats_weighted",
+    "mcg_stats_weighted", "mcg_stats_compute_noalloc",
     "mcg_equilibration", "mcg_last_run_time", "mcg_assign_chain", "mcg_autotune",
     "mcg_get_schedule", "mcg_set_schedule", "mcg_reserve_samples", "mcg_get_occ_location",
     "mcg_set_occ_location", "mcg_get_nfold_state", "mcg_run_steps", "mcg_get_position",
@@ -115,6 +115,8 @@ def lib():
                                         P(C.c_int64), P(C.c_int64)]
         L.mcg_stats_weighted.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_double,
                                          C.c_int64, C.c_int64, P(Stats)]
+        L.mcg_stats_compute_noalloc.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64,
+                                                C.c_double, P(Stats)]
         L.mcg_run_steps.argtypes = [C.c_void_p, C.c_int64]
         L.mcg_check_batch.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), C.c_int64, C.c_double,
                                       C.c_double, P(Check)]
@@ -293,6 +295,16 @@ class Run:
     def stats(self, c, observable, first=0, confidence=0.95):
         s = Stats()
         _check(lib().mcg_stats_compute(self.h, c, observable, first, confidence, C.byref(s)))
+        return dict(mean=s.mean, calculated_precision=s.calculated_precision,
+                    variance=s.variance, autocorr_factor=s.autocorr_factor, n=s.n)
+
+    def stats_noalloc(self, c, observable, first=0, confidence=0.95):
+        """stats() computed in the library's fixed device scratch
+        (mcg_stats_compute_noalloc: no device allocation; at most
+        MCG_STATS_NOALLOC_MAX - 8 observations)."""
+        s = Stats()
+        _check(lib().mcg_stats_compute_noalloc(self.h, c, observable, first, confidence,
+                                               C.byref(s)))
         return dict(mean=s.mean, calculated_precision=s.calculated_precision,
                     variance=s.variance, autocorr_factor=s.autocorr_factor, n=s.n)
 

@@ -351,6 +351,7 @@ struct mcg_handle {
   int pair_lds = 0;  // dynamic LDS of the line-class launches (largest tile)
   int tile_grid = 0;  // persistent workgroups of the tiled line-class launches
   int n_cu = 256;     // compute units of the device
+  int device = 0;     // the device the handle's memory and stream belong to
   double tuned_ms = 0.0;
   // hipGraph of graph_passes passes of the HBM path (rebuilt when the work
   // list, the series buffer or the schedule change)
@@ -1330,7 +1331,11 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
   if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
     return fail(MCG_ERR_HIP, std::string("device is not gfx950: ") + prop.gcnArchName);
 
+  // the no-allocation statistics scratch: its device address, once per device
+  HIP_TRY(stats_noalloc_scratch_init(dev));
+
   auto h = new mcg_handle();
+  h->device = dev;
   h->n_cu = std::max(1, prop.multiProcessorCount);
   h->mode = desc->mode;
   h->lattice = desc->lattice;
@@ -2077,6 +2082,47 @@ int mcg_stats_compute(mcg_handle *h, int64_t chain, int64_t observable, int64_t 
   out->autocorr_factor = r[2];
   out->n = count;
   out->calculated_precision = f_conf * std::sqrt(r[2] * r[1] / (double)count);
+  return MCG_OK;
+}
+
+int mcg_stats_compute_noalloc(mcg_handle *h, int64_t chain, int64_t observable, int64_t first,
+                              double confidence, mcg_stats *out) {
+  if (!valid_chain(h, chain) || !out) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (observable < 0 || observable > 2) return fail(MCG_ERR_ARG, "bad observable");
+  const int64_t count = h->n_samples[chain] - first;
+  if (first < 0 || count <= 0)
+    return fail(MCG_ERR_ARG, "Error in BasicStatisticsCalculator: observations.size()==0");
+  // the weighted branch reads the series at the calculator's default 10000
+  // resamples (as mcg_stats_compute does for NFOLD records)
+  const int64_t n_resamples = 10000;
+  // 8 result words at most, then the series (or the resamples)
+  const int64_t need = (h->nfold() ? n_resamples : count) + 8;
+  if (need > MCG_STATS_NOALLOC_MAX)
+    return fail(MCG_ERR_CAPACITY,
+                "mcg_stats_compute_noalloc: " + std::to_string(count) +
+                    " observations exceed the fixed scratch (at most " +
+                    std::to_string(MCG_STATS_NOALLOC_MAX - 8) +
+                    "); use mcg_stats_compute");
+  double *ptr = stats_noalloc_scratch(h->device);
+  if (!ptr) return fail(MCG_ERR_STATE, "statistics scratch not initialised on this device");
+  if (h->nfold())
+    launch_weighted_stats(h->d_series + chain * h->series_cap, first, count,
+                          obs_params(h, chain, observable), 1, n_resamples, ptr, h->stream);
+  else
+    launch_stats(h->d_series + chain * h->series_cap, first, count,
+                 obs_params(h, chain, observable), ptr, h->stream);
+  HIP_TRY(hipGetLastError());
+  double r[8];
+  const size_t nr = h->nfold() ? 8 : 4;
+  HIP_TRY(hipMemcpyAsync(r, ptr, sizeof(double) * nr, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  out->mean = r[0];
+  out->variance = r[1];
+  out->autocorr_factor = r[2];
+  out->n = count;
+  // as mcg_stats_weighted (method 1: / W) and mcg_stats_compute (/ n)
+  const double denom = h->nfold() ? r[4] : (double)count;
+  out->calculated_precision = confidence_factor(confidence) * std::sqrt(r[2] * r[1] / denom);
   return MCG_OK;
 }
 

@@ -247,5 +247,10 @@ void launch_equilibration(const mcg_sample *series, int64_t count, ObsParams op,
 void launch_check_batch(const mcg_sample *series, int64_t series_cap, const int64_t *chains,
                         const int64_t *counts, const ObsParams *ops, const int64_t *scratch_off,
                         int n, double prec, double *scratch, double *out, hipStream_t stream);
+// Device address of the fixed statistics scratch (MCG_STATS_NOALLOC_MAX doubles)
+// on device `dev`: looked up once per device by the init call, with that
+// device current; nullptr before it.
+hipError_t stats_noalloc_scratch_init(int dev);
+double *stats_noalloc_scratch(int dev);
 
 }  // namespace mcg

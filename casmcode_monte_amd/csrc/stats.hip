@@ -13,10 +13,44 @@
 // centred series from HBM/L2) and stops at the first batch holding a lag with
 // |cov/var0| <= 0.5.  Sums are reduced in a different order than the
 // reference's sequential loops, so results agree to rounding, not bitwise.
+#include <mutex>
+
 #include "internal.hh"
 
 namespace mcg {
 
+// Fixed statistics scratch of the no-allocation entry point
+// (mcg_stats_compute_noalloc): the kernels' result words followed by the
+// centred or resampled series, MCG_STATS_NOALLOC_MAX doubles in all.  Every GPU
+// the library runs on holds its own copy until the process ends.  Host code
+// never passes `scratch` itself to a kernel (in host code the name is only a
+// stand-in for the symbol); it passes the device address that
+// stats_noalloc_scratch_init looks up once per device.
+__device__ __align__(256) double scratch[MCG_STATS_NOALLOC_MAX];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t stats_noalloc_scratch_init(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev]) return hipSuccess;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!ptr) return hipErrorInvalidSymbol;
+  g_scratch_ptr[dev] = static_cast<double *>(ptr);
+  return hipSuccess;
+}
+
+double *stats_noalloc_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  return g_scratch_ptr[dev];
+}
 
 __device__ __forceinline__ double observable_of(const mcg_sample &r, const ObsParams &op) {
   const double N = (double)op.n_sites;

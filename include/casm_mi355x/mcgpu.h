@@ -335,6 +335,22 @@ int mcg_stats_compute(mcg_handle *h, int64_t chain, int64_t observable, int64_t 
 int mcg_stats_weighted(mcg_handle *h, int64_t chain, int64_t observable, int64_t first,
                        double confidence, int64_t method, int64_t n_resamples, mcg_stats *out);
 
+/* No counterpart in the reference: mcg_stats_compute for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same kernels and results as
+ * mcg_stats_compute (NFOLD records: the weighted statistics with the
+ * calculator's defaults, method 1 and 10000 resamples), computed in a fixed
+ * scratch array compiled into the library -- MCG_STATS_NOALLOC_MAX doubles
+ * (2 MiB) on every GPU the library runs on, held until the process ends --
+ * instead of the handle's growable scratch; no hipMalloc or hipFree is called.
+ * Limit: at most MCG_STATS_NOALLOC_MAX - 8 observations (NFOLD: resamples);
+ * a longer series fails with MCG_ERR_CAPACITY.  All calls on one GPU share
+ * the array and each call leaves its contents there: calls that overlap on
+ * one GPU (from other host threads, on any handle) race. */
+#define MCG_STATS_NOALLOC_MAX 262144
+int mcg_stats_compute_noalloc(mcg_handle *h, int64_t chain, int64_t observable, int64_t first,
+                              double confidence, mcg_stats *out);
+
 /* Equilibration check of one observable over a chain's whole series,
  * as _default_equilibration_check (src/casm/monte/checks/EquilibrationCheck.cc:50-117)
  * with absolute precision prec; the partition scan runs on the device.  NFOLD
