@@ -68,15 +68,21 @@ namespace detail {
 ///    due again at the next step's call, so the device then runs one step.
 /// Counters, the running potential energy and, when a sampling function reads
 /// it, the configuration (sync_configuration) come to the host at those points.
-template <typename ConfigType, typename StatisticsType, typename EngineType, typename SyncF>
+/// A fixture with a stochastic sampling period draws its next sample count
+/// from the run's engine when it samples (SamplingFixture.hh:529-534, between
+/// two steps of the walk): around such samples the walk's engine comes to the
+/// host and goes back (engine_sync(true), engine_sync(false)).
+template <typename ConfigType, typename StatisticsType, typename EngineType, typename SyncF,
+          typename EngineSyncF>
 void run_device_steps(State<ConfigType> &state, gpu::Handle &dev, CountType N, bool steps_ok,
                       RunManager<ConfigType, StatisticsType, EngineType> &run_manager,
-                      SyncF sync_configuration) {
+                      SyncF sync_configuration, EngineSyncF engine_sync) {
   double &potential_energy_per_unitcell = state.properties.scalar_values["potential_energy"];
   run_manager.initialize(N);
-  bool host_functions = false, by_step = false;
+  bool host_functions = false, by_step = false, stochastic = false;
   for (auto &f : run_manager.sampling_fixtures) {
     auto const &sp = f->params().sampling_params;
+    if (sp.stochastic_sample_period) stochastic = true;
     if (sp.sample_mode == SAMPLE_MODE::BY_STEP) {
       if (!steps_ok)
         throw std::runtime_error(
@@ -92,7 +98,18 @@ void run_device_steps(State<ConfigType> &state, gpu::Handle &dev, CountType N, b
     for (auto const &name : sp.sampler_names)
       if (f->params().sampling_functions.at(name).device_observable < 0) host_functions = true;
   }
-  run_manager.sample_data_by_count_if_due(state);
+  auto sample_if_due = [&]() {
+    bool draws = false;
+    if (stochastic)
+      for (auto &f : run_manager.sampling_fixtures)
+        if (f->params().sampling_params.stochastic_sample_period &&
+            f->counter().count == f->next_sample_count())
+          draws = true;
+    if (draws) engine_sync(true);
+    run_manager.sample_data_by_count_if_due(state);
+    if (draws) engine_sync(false);
+  };
+  sample_if_due();
   // a fixture's count c in steps since the start
   auto steps_of = [&](auto const &f, CountType c) {
     return f->params().sampling_params.sample_mode == SAMPLE_MODE::BY_STEP ? c : c * N;
@@ -145,7 +162,7 @@ void run_device_steps(State<ConfigType> &state, gpu::Handle &dev, CountType N, b
     g = target;
     potential_energy_per_unitcell = e_run;
     if (host_functions) sync_configuration();
-    run_manager.sample_data_by_count_if_due(state);
+    sample_if_due();
   }
   sync_configuration();
 }
@@ -185,10 +202,18 @@ void occupation_metropolis(
   dev.load_engine(0, *random_number_generator.engine);
 
   const CountType N = config.n_sites;
-  detail::run_device_steps(state, dev, N, true, run_manager, [&]() {
-    auto occ = dev.occupation(0, N);
-    config.set_occupation(std::vector<int>(occ.begin(), occ.end()));
-  });
+  detail::run_device_steps(
+      state, dev, N, true, run_manager,
+      [&]() {
+        auto occ = dev.occupation(0, N);
+        config.set_occupation(std::vector<int>(occ.begin(), occ.end()));
+      },
+      [&](bool to_host) {
+        if (to_host)
+          dev.store_engine(0, *random_number_generator.engine);
+        else
+          dev.load_engine(0, *random_number_generator.engine);
+      });
   dev.store_engine(0, *random_number_generator.engine);
   run_manager.finalize(state);
 }
@@ -271,10 +296,18 @@ void occupation_metropolis(
   dev.set_occ_location(0, loc);
   dev.load_engine(0, *random_number_generator.engine);
 
-  detail::run_device_steps(state, dev, N, false, run_manager, [&]() {
-    auto occ = dev.occupation(0, N);
-    for (Index l = 0; l < N; ++l) config.set_occ(l, occ[l] > 0 ? 1 : 0);
-  });
+  detail::run_device_steps(
+      state, dev, N, false, run_manager,
+      [&]() {
+        auto occ = dev.occupation(0, N);
+        for (Index l = 0; l < N; ++l) config.set_occ(l, occ[l] > 0 ? 1 : 0);
+      },
+      [&](bool to_host) {
+        if (to_host)
+          dev.store_engine(0, *random_number_generator.engine);
+        else
+          dev.load_engine(0, *random_number_generator.engine);
+      });
   auto dl = dev.occ_location(0, N);
   std::vector<std::vector<Index>> lists(2);
   const Index nA = occ_location.cand_size(0);

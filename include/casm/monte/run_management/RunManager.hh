@@ -250,6 +250,7 @@ class SamplingFixture {
                   std::shared_ptr<engine_type> engine)
       : m_params(params),
         m_engine(engine),
+        m_random_number_generator(engine),
         m_completion_check(m_params.completion_check_params),
         m_results(m_params.sampling_params.sampler_names, m_params.sampling_functions,
                   m_params.sampling_params.json_sampler_names, m_params.json_sampling_functions,
@@ -272,9 +273,6 @@ class SamplingFixture {
     if (m_params.sampling_params.sample_mode == SAMPLE_MODE::BY_TIME)
       throw std::runtime_error("SamplingFixture: SAMPLE_MODE::BY_TIME is not supported by "
                                "the GPU loops");
-    if (m_params.sampling_params.stochastic_sample_period)
-      throw std::runtime_error("SamplingFixture: stochastic sampling periods are not supported "
-                               "by the GPU loops");
     m_next_sample_count =
         static_cast<CountType>(std::round(sample_at(m_results.sample_count.size())));
     if (m_next_sample_count < 0)
@@ -354,7 +352,13 @@ class SamplingFixture {
     if (m_counter.count == m_next_sample_count) sample_data(state);
   }
 
-  double sample_at(CountType sample_index) const {
+  /// SamplingFixture.hh:570-588: a stochastic period draws the next sample's
+  /// count from the fixture's generator, which shares the run's engine
+  double sample_at(CountType sample_index) {
+    if (m_params.sampling_params.stochastic_sample_period)
+      return stochastic_sample_at(sample_index, m_params.sampling_params,
+                                  m_random_number_generator, m_results.sample_count,
+                                  m_results.sample_time);
     return monte::sample_at(sample_index, m_params.sampling_params);
   }
 
@@ -374,6 +378,7 @@ class SamplingFixture {
  private:
   SamplingFixtureParams<config_type, stats_type> m_params;
   std::shared_ptr<engine_type> m_engine;
+  RandomNumberGenerator<engine_type> m_random_number_generator;
   Index m_n_samples = 0;
   Index m_count = 0;
   bool m_is_complete = false;

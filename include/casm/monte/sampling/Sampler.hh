@@ -4,10 +4,12 @@
 //  (:372-408), default component names (:333-363);
 //  StateSamplingFunction (sampling/StateSamplingFunction.hh:20-54) and its
 //  JSON-valued sibling (values kept as JSON text);
-//  SamplingParams and sample_at (sampling/SamplingParams.hh:14-180).
+//  SamplingParams and sample_at (sampling/SamplingParams.hh:14-180);
+//  stochastic sampling periods (SamplingParams.hh:186-272).
 #ifndef CASM_MI355X_monte_Sampler
 #define CASM_MI355X_monte_Sampler
 
+#include "casm/monte/RandomNumberGenerator.hh"
 #include "casm/monte/definitions.hh"
 
 namespace CASM {
@@ -234,6 +236,51 @@ inline double sample_at(CountType sample_index, SamplingParams const &s) {
     throw std::runtime_error(
         "Error in sample_at: sample_method==SAMPLE_METHOD::CUSTOM and !custom_sample_at");
   return s.custom_sample_at(n);
+}
+
+/// Counts until the next sample of a stochastic sampling period
+/// (SamplingParams.hh:186-204): the first dn >= 1 whose uniform draw on [0, 1)
+/// falls below the rate, so dn is geometric with mean 1 / rate.
+template <typename EngineType>
+CountType stochastic_count_step(double sample_rate,
+                                RandomNumberGenerator<EngineType> &random_number_generator) {
+  CountType dn = 1;
+  while (!(random_number_generator.random_real(1.0) < sample_rate)) ++dn;
+  return dn;
+}
+
+/// Time until the next sample of a stochastic sampling period
+/// (SamplingParams.hh:206-219): exponential with the given rate.
+template <typename EngineType>
+TimeType stochastic_time_step(TimeType sample_rate,
+                              RandomNumberGenerator<EngineType> &random_number_generator) {
+  return -std::log(random_number_generator.random_real(1.0)) / sample_rate;
+}
+
+/// Count or time of the sample_index-th sample with a stochastic period
+/// (SamplingParams.hh:221-272): `begin` for the first, then the last sample's
+/// count (time) plus a stochastic step at the rate 1 / period (LINEAR) or
+/// 1 / (ln(base) base^(n + shift)) (LOG).  The reference computes no rate for
+/// SAMPLE_METHOD::CUSTOM (its CUSTOM branch repeats the LOG test, so the rate
+/// stays unset); that case is refused here.
+template <typename EngineType>
+double stochastic_sample_at(CountType sample_index, SamplingParams const &s,
+                            RandomNumberGenerator<EngineType> &random_number_generator,
+                            std::vector<CountType> const &sample_count,
+                            std::vector<TimeType> const &sample_time) {
+  if (sample_index == 0) return s.begin;
+  const double n = static_cast<double>(sample_index);
+  double rate;
+  if (s.sample_method == SAMPLE_METHOD::LINEAR)
+    rate = 1.0 / s.period;
+  else if (s.sample_method == SAMPLE_METHOD::LOG)
+    rate = 1.0 / (std::log(s.base) * std::pow(s.base, (n + s.shift)));
+  else
+    throw std::runtime_error(
+        "Error in stochastic_sample_at: SAMPLE_METHOD::CUSTOM has no sampling rate");
+  if (s.sample_mode == SAMPLE_MODE::BY_TIME)
+    return sample_time.back() + stochastic_time_step(rate, random_number_generator);
+  return sample_count.back() + stochastic_count_step(rate, random_number_generator);
 }
 
 }  // namespace monte

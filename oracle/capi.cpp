@@ -166,6 +166,16 @@ long orc_uniform_int_replay(const uint64_t *xs, int64_t n, long maximum, int64_t
   return v;
 }
 
+// n successive stochastic_count_step(rate) calls on an engine seeded with
+// `seed`; dn[i] = the counts drawn, *next_output = the engine's next output
+// after them.
+void orc_stochastic_count_steps(uint64_t seed, double rate, int64_t n, int64_t *dn,
+                                uint64_t *next_output) {
+  RandomNumberGenerator rng(std::make_shared<std::mt19937_64>(seed));
+  for (int64_t i = 0; i < n; ++i) dn[i] = stochastic_count_step(rate, rng);
+  *next_output = (*rng.engine)();
+}
+
 // std::uniform_real_distribution<double>(0, 1)(engine) on one replayed word.
 double orc_canonical(uint64_t x) {
   ReplayEngine e{&x, 1, 0};

@@ -85,6 +85,18 @@ bool metropolis_acceptance(double delta_potential_energy, double beta, G &rng) {
   return rand < prob;
 }
 
+// SamplingParams.hh:193-204: counts to the next sample of a stochastic
+// sampling period -- one uniform draw on [0, 1) per count until a draw falls
+// below the rate
+template <typename G>
+CountType stochastic_count_step(double sample_rate, G &rng) {
+  CountType dn = 1;
+  while (true) {
+    if (rng.random_real(1.0) < sample_rate) return dn;
+    ++dn;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Ising model: model.hh:19-436
 // ---------------------------------------------------------------------------
@@ -816,9 +828,12 @@ struct StepRunOutput {
   std::vector<long long> M;
   IsingConfiguration final_config;
 };
+// after_step(g), when given, runs after step g (g = 1..n_steps) and may draw
+// from the same engine (a fixture's stochastic sampling period does).
 inline StepRunOutput sgc_step_run(IsingConfiguration config, double J, bool use_nlist,
                                   double temperature, double mu,
-                                  std::shared_ptr<std::mt19937_64> engine, long n_steps) {
+                                  std::shared_ptr<std::mt19937_64> engine, long n_steps,
+                                  std::function<void(long)> after_step = nullptr) {
   SemiGrandCanonicalPotential potential{IsingFormationEnergy(J, use_nlist)};
   potential.set_state(&config, mu);
   const double n_unitcells = static_cast<double>(config.n_sites);
@@ -840,6 +855,7 @@ inline StepRunOutput sgc_step_run(IsingConfiguration config, double J, bool use_
     }
     out.pe.push_back(pe);
     out.M.push_back(M);
+    if (after_step) after_step(s + 1);
   }
   out.final_config = config;
   return out;

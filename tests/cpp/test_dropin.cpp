@@ -427,6 +427,91 @@ static void gpu_occupation_metropolis_by_step() {
   std::cout << "gpu: " << tag << (g_fail ? " (failures so far)" : " ok") << std::endl;
 }
 
+// Stochastic sampling periods: a by-step and a by-pass fixture draw their next
+// sample counts from the run's engine between steps of the walk, so the walk's
+// stream and the sample counts both depend on where those draws fall.
+static void gpu_occupation_metropolis_stochastic() {
+  const int R = 10, C = 12, N = R * C;
+  const long S = 2503;
+  const double T = 2600.0, mu = -0.05;
+  const uint64_t seed = 21;
+  typedef ising_cpp::IsingConfiguration config_type;
+  ValueMap cond;
+  cond.scalar_values["temperature"] = T;
+  cond.vector_values["exchange_potential"] = VectorXd{mu};
+  State<config_type> state(config_type({R, C}, 1), cond);
+  IsingSemiGrandCanonicalPotential potential(0.1, false);
+  potential.set(&state);
+  StateSamplingFunction pe("potential_energy", "running potential energy", {}, [&state]() {
+    return VectorXd{state.properties.scalar_values.at("potential_energy")};
+  });
+  pe.device_observable = MCG_OBS_POTENTIAL_ENERGY;
+  StateSamplingFunctionMap fns;
+  fns.emplace(pe.name, pe);
+  SamplingParams by_step;
+  by_step.sample_mode = SAMPLE_MODE::BY_STEP;
+  by_step.sampler_names = {"potential_energy"};
+  by_step.begin = 3.0;
+  by_step.period = 25.0;
+  by_step.stochastic_sample_period = true;
+  CompletionCheckParams<BasicStatistics> cc_step;
+  cc_step.cutoff_params.max_count = S;
+  SamplingParams by_pass;
+  by_pass.sampler_names = {"potential_energy"};
+  by_pass.period = 2.0;
+  by_pass.stochastic_sample_period = true;
+  CompletionCheckParams<BasicStatistics> cc_pass;
+  typedef RunManager<config_type, BasicStatistics, std::mt19937_64> rm_type;
+  std::vector<rm_type::sampling_fixture_params_type> fixtures;
+  fixtures.emplace_back("steps", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), by_step,
+                        cc_step, std::vector<std::string>{});
+  fixtures.emplace_back("passes", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), by_pass,
+                        cc_pass, std::vector<std::string>{});
+  auto engine = std::make_shared<std::mt19937_64>(seed);
+  rm_type rm(engine, fixtures, true);
+  RandomNumberGenerator<std::mt19937_64> rng(engine);
+  occupation_metropolis(state, potential, rng, rm);
+
+  // the oracle's walk with the two fixtures' draws after each step, in
+  // fixture order (SamplingFixture.hh:529-534, SamplingParams.hh:242-272)
+  auto e1 = std::make_shared<std::mt19937_64>(seed);
+  refcpu::RandomNumberGenerator orng(e1);
+  long next_a = 3, next_b = 1;
+  std::vector<long> counts_a, counts_b;
+  auto o = refcpu::sgc_step_run(
+      refcpu::IsingConfiguration(R, C, 1), 0.1, false, T, mu, e1, S, [&](long g) {
+        if (g == next_a) {
+          counts_a.push_back(g);
+          next_a = g + refcpu::stochastic_count_step(1.0 / 25.0, orng);
+        }
+        if (g % N == 0 && g / N == next_b) {
+          counts_b.push_back(g / N);
+          next_b = g / N + refcpu::stochastic_count_step(1.0 / 2.0, orng);
+        }
+      });
+  const std::string tag = "occupation_metropolis stochastic periods";
+  auto const &a = rm.sampling_fixtures[0]->results();
+  VectorXd ape = a.samplers.at("potential_energy")->component(0);
+  bool ok = ape.size() == (Index)counts_a.size() && a.sample_count.size() == counts_a.size() &&
+            counts_a.size() > 50;
+  for (size_t k = 0; ok && k < counts_a.size(); ++k)
+    ok = a.sample_count[k] == counts_a[k] && same_bits(ape(k), o.pe[counts_a[k]]);
+  EXPECT(ok, tag << ": step samples (" << ape.size() << " of " << counts_a.size() << ")");
+  auto const &b = rm.sampling_fixtures[1]->results();
+  VectorXd bpe = b.samplers.at("potential_energy")->component(0);
+  ok = bpe.size() == (Index)counts_b.size() && b.sample_count.size() == counts_b.size() &&
+       counts_b.size() > 3;
+  for (size_t k = 0; ok && k < counts_b.size(); ++k)
+    ok = b.sample_count[k] == counts_b[k] && same_bits(bpe(k), o.pe[counts_b[k] * N]);
+  EXPECT(ok, tag << ": pass samples (" << bpe.size() << " of " << counts_b.size() << ")");
+  EXPECT(state.configuration.occupation() == o.final_config.occupation, tag << ": configuration");
+  EXPECT(*engine == *e1, tag << ": engine state");
+  EXPECT(a.n_accept + a.n_reject == S, tag << ": attempts");
+  std::cout << "gpu: " << tag << (g_fail ? " (failures so far)" : " ok") << std::endl;
+}
+
 // RunManager-driven loop: two fixtures, the FP64 running potential energy and
 // a configuration-reading function, vs the oracle.
 static void gpu_occupation_metropolis_tests() {
@@ -972,6 +1057,7 @@ static void gpu_tests() {
   gpu_complete_case(false, 42, 1);
   gpu_complete_case(true, 44, 2);
   gpu_occupation_metropolis_by_step();
+  gpu_occupation_metropolis_stochastic();
   // the reference's SemiGrandCanonicalRun1 set-up, converged on two observables
   gpu_case({25, 25, true, 2000.0, 0.0, 42, 1, -1, 0.001, 100, false});
   gpu_case({25, 25, false, 2000.0, 0.0, 43, 1, -1, 0.001, 100, false});

@@ -72,6 +72,9 @@ def _declare(L):
     L.orc_uniform_int_replay.argtypes = [C.POINTER(C.c_uint64), C.c_int64, C.c_long,
                                          C.POINTER(C.c_int64)]
     L.orc_uniform_int_replay.restype = C.c_long
+    L.orc_stochastic_count_steps.argtypes = [C.c_uint64, C.c_double, C.c_int64,
+                                             C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]
+    L.orc_stochastic_count_steps.restype = None
     L.orc_canonical.argtypes = [C.c_uint64]
     L.orc_canonical.restype = C.c_double
     L.orc_accept_prob.argtypes = [C.c_double, C.c_double]
@@ -196,6 +199,15 @@ def resample(x, w, w_sum, m):
     lib().orc_resample(_p(x, C.c_double), _p(w, C.c_double), len(x), w_sum, m,
                        _p(out, C.c_double))
     return out
+
+
+def stochastic_count_steps(seed, rate, n):
+    """n successive stochastic_count_step(rate) draws (SamplingParams.hh:193-204)
+    from a fresh engine, and the engine's next output after them."""
+    dn = np.zeros(n, dtype=np.int64)
+    nxt = C.c_uint64(0)
+    lib().orc_stochastic_count_steps(seed, rate, n, _p(dn, C.c_int64), C.byref(nxt))
+    return dn, nxt.value
 
 
 def autocorrelation_factor(x, increment=None):
