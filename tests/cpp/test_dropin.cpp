@@ -13,10 +13,13 @@
 #include <iostream>
 #include <numeric>
 
+#include <unistd.h>
+
 #include "casm/monte/ising_cpp/basic_semigrand_canonical.hh"
 #include "casm/monte/ising_cpp/complete_semigrand_canonical.hh"
 #include "casm/monte/methods/nfold.hh"
 #include "casm/monte/methods/occupation_metropolis.hh"
+#include "casm/monte/run_management/io/json/jsonResultsIO.hh"
 #include "refcpu.hh"
 
 using namespace CASM;
@@ -509,6 +512,76 @@ static void gpu_occupation_metropolis_stochastic() {
   EXPECT(state.configuration.occupation() == o.final_config.occupation, tag << ": configuration");
   EXPECT(*engine == *e1, tag << ": engine state");
   EXPECT(a.n_accept + a.n_reject == S, tag << ": attempts");
+  std::cout << "gpu: " << tag << (g_fail ? " (failures so far)" : " ok") << std::endl;
+}
+
+// Results files of a run on the MI355X (jsonResultsIO as the fixture's
+// results_io): run.0/observations.json holds the sampled series bit for bit as
+// the oracle's walk gives it, and summary.json the conditions and the
+// BasicStatistics of that series (the oracle's calculator on the oracle's
+// series, same bits).
+static void gpu_results_files() {
+  const int R = 16, C = 16, N = R * C;
+  const long P = 400;
+  const double T = 1900.0, mu = 0.1;
+  const uint64_t seed = 31;
+  typedef ising_cpp::IsingConfiguration config_type;
+  ValueMap cond;
+  cond.scalar_values["temperature"] = T;
+  cond.vector_values["exchange_potential"] = VectorXd{mu};
+  State<config_type> state(config_type({R, C}, 1), cond);
+  IsingSemiGrandCanonicalPotential potential(0.1, true);
+  potential.set(&state);
+  StateSamplingFunction pe("potential_energy", "running potential energy", {}, [&state]() {
+    return VectorXd{state.properties.scalar_values.at("potential_energy")};
+  });
+  pe.device_observable = MCG_OBS_POTENTIAL_ENERGY;
+  StateSamplingFunctionMap fns;
+  fns.emplace(pe.name, pe);
+  SamplingParams sp;
+  sp.sampler_names = {"potential_energy"};
+  CompletionCheckParams<BasicStatistics> cc;
+  cc.cutoff_params.max_count = P;
+  typedef RunManager<config_type, BasicStatistics, std::mt19937_64> rm_type;
+  typedef Results<config_type, BasicStatistics> results_type;
+  std::filesystem::path dir = std::filesystem::temp_directory_path() /
+                              ("mcg_results_" + std::to_string(::getpid()));
+  std::filesystem::remove_all(dir);
+  std::filesystem::create_directories(dir);
+  jsonResultsIO<results_type> io(dir, false, true);
+  std::vector<rm_type::sampling_fixture_params_type> fixtures;
+  fixtures.emplace_back("passes", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), sp, cc,
+                        std::vector<std::string>{}, io.writer());
+  auto engine = std::make_shared<std::mt19937_64>(seed);
+  rm_type rm(engine, fixtures, true);
+  RandomNumberGenerator<std::mt19937_64> rng(engine);
+  occupation_metropolis(state, potential, rng, rm);
+
+  auto e1 = std::make_shared<std::mt19937_64>(seed);
+  auto o = refcpu::sgc_step_run(refcpu::IsingConfiguration(R, C, 1), 0.1, true, T, mu, e1, P * N);
+  std::vector<double> series;
+  for (long p = 1; p <= P; ++p) series.push_back(o.pe[p * N]);
+  auto st = refcpu::basic_statistics(series, 0.95);
+
+  const std::string tag = "results files from a GPU run";
+  jsonParser obs = jsonParser::read(dir / "run.0" / "observations.json");
+  auto vals = obs["potential_energy"]["value"].get<std::vector<double>>();
+  auto counts = obs["count"].get<std::vector<long>>();
+  bool ok = vals.size() == series.size() && counts.size() == series.size();
+  for (size_t k = 0; ok && k < series.size(); ++k)
+    ok = same_bits(vals[k], series[k]) && counts[k] == (long)k + 1;
+  EXPECT(ok, tag << ": observations.json (" << vals.size() << " samples)");
+  jsonParser sum = jsonParser::read(dir / "summary.json");
+  auto means = sum["statistics"]["potential_energy"]["value"]["mean"].get<std::vector<double>>();
+  auto precs = sum["statistics"]["potential_energy"]["value"]["calculated_precision"]
+                   .get<std::vector<double>>();
+  EXPECT(means.size() == 1 && precs.size() == 1 && same_bits(means[0], st.mean) &&
+             same_bits(precs[0], st.calculated_precision),
+         tag << ": summary.json statistics");
+  auto temps = sum["conditions"]["temperature"]["value"].get<std::vector<double>>();
+  EXPECT(temps.size() == 1 && temps[0] == T, tag << ": summary.json conditions");
+  std::filesystem::remove_all(dir);
   std::cout << "gpu: " << tag << (g_fail ? " (failures so far)" : " ok") << std::endl;
 }
 
@@ -1058,6 +1131,7 @@ static void gpu_tests() {
   gpu_complete_case(true, 44, 2);
   gpu_occupation_metropolis_by_step();
   gpu_occupation_metropolis_stochastic();
+  gpu_results_files();
   // the reference's SemiGrandCanonicalRun1 set-up, converged on two observables
   gpu_case({25, 25, true, 2000.0, 0.0, 42, 1, -1, 0.001, 100, false});
   gpu_case({25, 25, false, 2000.0, 0.0, 43, 1, -1, 0.001, 100, false});
