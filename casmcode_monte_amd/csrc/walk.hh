@@ -157,10 +157,13 @@ struct MtRing3 {
       sG = sG == 2 ? 0 : sG + 1;
     }
   }
-  // offset of stream position q, 312(G-1) <= q < 312(G+1), in ring[3][MT_N]
+  // offset of stream position q, 312(G-1) <= q < 312(G+1), in ring[3][MT_N]:
+  // generation G sits in slot sG, generation G-1 in slot (sG + 2) % 3, so a
+  // negative r lands at r + 312 sG except when sG = 0 (slot 2: r + 936); one
+  // select, no branch
   __device__ __forceinline__ int index(int64_t q) const {
     const int r = (int)(q - (int64_t)MT_N * G);
-    return r >= 0 ? r + MT_N * sG : r + MT_N * (sG == 0 ? 3 : sG);
+    return r + MT_N * sG + ((r < 0 && sG == 0) ? 3 * MT_N : 0);
   }
   // the walk is done: release the producer (lane 0 writes the final barrier's
   // number; every lane of the walking wave passes the barrier)
