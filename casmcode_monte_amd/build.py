@@ -20,7 +20,7 @@ LIB = os.path.join(LIBDIR, "libcasm_monte_mi355x.so")
 # measured-peak probes (not part of the Metropolis path; loaded by bench.py)
 PROBES_LIB = os.path.join(LIBDIR, "libcasm_monte_probes.so")
 SOURCES = ["api.hip", "exact.hip", "canonical.hip", "nfold.hip", "checker.hip",
-           "checker_ring.hip", "fcc.hip", "stats.hip"]
+           "checker_ring.hip", "checker_slab.hip", "fcc.hip", "stats.hip"]
 HEADERS = ["internal.hh", "rng.hh", "walk.hh", "fcc.hh"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

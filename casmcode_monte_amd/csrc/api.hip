@@ -345,6 +345,7 @@ struct mcg_handle {
   int32_t *d_band = nullptr;
   int n_bands = 0;
   int band_ring_bytes = 0;
+  bool band_slabs = false;  // the band list holds 3D slabs (checker_slab.hip)
   // line-class path (4/8 colourings): (chain, first line, lines) items
   int32_t *d_pair = nullptr;
   int n_pairs = 0;
@@ -831,6 +832,7 @@ int build_checker(mcg_handle *h) {
   // different host threads never change a kernel attribute under each other
   HIP_TRY(checker_band_set_max_lds(checker_band_ring_lines() * CHECKER_BAND_MAX_L0));
   HIP_TRY(checker_pair_set_max_lds(checker_tile_max_bytes()));
+  HIP_TRY(checker_slab_set_max_lds(CHECKER_SLAB_MAX_LDS));
   HIP_TRY(hipMalloc(&h->d_cp, sizeof(CheckerParams) * h->n_chains));
   HIP_TRY(hipMalloc(&h->d_cs, sizeof(CheckerState) * h->n_chains));
   HIP_TRY(hipMalloc(&h->d_tab, sizeof(uint32_t) * tabs.size()));
@@ -1032,6 +1034,44 @@ int rebuild_work(mcg_handle *h) {
     maxL0 = std::max<int64_t>(maxL0, P.L0);
     total_lines += P.L1;
   }
+  // 3D J1+J2 slab path (checker_slab.hip): every active chain bit-packable;
+  // band_lines counts planes per slab here
+  h->band_slabs = false;
+  bool slab_ok = h->lattice == MCG_LATTICE_SC && h->ncolor == 8 && h->band_lines != 0 &&
+                 !(h->options & MCG_OPT_CHECKER_QUAD_PATH);
+  int64_t total_planes = 0;
+  int slab_bytes = 0;
+  for (int c = 0; c < h->n_chains && slab_ok; ++c) {
+    if (!parity_chain(c)) continue;
+    auto const &P = h->cp[c];
+    if (!checker_slab_fits(P)) slab_ok = false;
+    total_planes += P.L2;
+    slab_bytes = std::max(slab_bytes, checker_slab_ring_bytes(P.L0, P.L1));
+  }
+  if (slab_ok && total_planes > 0) {
+    int target = h->band_lines;
+    if (target < 0) {  // about two slabs per CU or more, slabs of at most 16 planes
+      target = 16;
+      while (target > 2 && total_planes / target < 2 * h->n_cu) target /= 2;
+    }
+    std::vector<int32_t> slabs;
+    for (int c = 0; c < h->n_chains; ++c) {
+      if (!parity_chain(c)) continue;
+      const int L2 = h->cp[c].L2;
+      int H = std::max(2, target & ~1);
+      while (H > 2 && L2 % H != 0) H -= 2;  // an even divisor of L2 (L2 is even)
+      for (int k0 = 0; k0 < L2; k0 += H) {
+        slabs.push_back(c);
+        slabs.push_back(k0);
+        slabs.push_back(H);
+      }
+    }
+    h->n_bands = (int)(slabs.size() / 3);
+    h->band_ring_bytes = slab_bytes;
+    h->band_slabs = true;
+    int rc = upload_list(h, &h->d_band, &h->band_cap, slabs);
+    if (rc != MCG_OK) return rc;
+  }
   if (band_ok && total_lines > 0) {
     int target = h->band_lines;
     if (target < 0) {  // about 2048 bands or more, bands of at most 64 lines
@@ -1137,7 +1177,10 @@ int rebuild_work(mcg_handle *h) {
 // One pass of the HBM path: one launch per colour over the work list, then
 // the pass-counter/sample-record launch.
 void enqueue_pass(mcg_handle *h) {
-  if (h->n_bands > 0)
+  if (h->n_bands > 0 && h->band_slabs)
+    launch_checker_slab(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_band, h->n_bands,
+                        h->band_ring_bytes, h->stream);
+  else if (h->n_bands > 0)
     launch_checker_band(h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_band, h->n_bands,
                         h->band_ring_bytes, h->stream);
   else if (h->n_pairs > 0)
@@ -1169,7 +1212,8 @@ int build_graph(mcg_handle *h) {
   hipError_t e = checker_graph_add_passes(
       g, h->d_cp, h->d_cs, h->d_lat, h->d_tab, h->d_work, h->n_work, h->ncolor,
       h->units_per_thread, h->d_series, h->series_cap, h->d_active, h->n_chains,
-      h->sample_period, h->graph_passes, h->d_band, h->n_bands, h->band_ring_bytes, h->d_pair,
+      h->sample_period, h->graph_passes, h->d_band, h->n_bands, h->band_ring_bytes,
+      h->band_slabs, h->d_pair,
       h->n_pairs, h->pair_lds, h->tile_grid);
   if (e == hipSuccess) e = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
@@ -2286,6 +2330,10 @@ int mcg_autotune(mcg_handle *h, int64_t trial_passes, mcg_schedule *chosen) {
       }
       if (h->lattice == MCG_LATTICE_SQUARE)
         for (int64_t bl : {8, 16, 32, 64})
+          for (int64_t g : {1, 8})
+            cands.push_back({h->checker_block, h->units_per_thread, g, 0.0, bl, 0});
+      if (h->lattice == MCG_LATTICE_SC && h->ncolor == 8)  // slabs of 4 to 16 planes
+        for (int64_t bl : {4, 8, 16})
           for (int64_t g : {1, 8})
             cands.push_back({h->checker_block, h->units_per_thread, g, 0.0, bl, 0});
     }

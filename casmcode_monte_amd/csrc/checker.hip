@@ -1074,8 +1074,8 @@ hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, Checke
                                     mcg_sample *series, int64_t series_cap,
                                     const uint8_t *active, int n_chains, int64_t sample_period,
                                     int passes, const int32_t *band_work, int n_bands,
-                                    int ring_bytes, const int32_t *pair_work, int n_pairs,
-                                    int pair_lds, int tile_grid) {
+                                    int ring_bytes, bool slabs, const int32_t *pair_work,
+                                    int n_pairs, int pair_lds, int tile_grid) {
   void *color_fn = upt == 1   ? reinterpret_cast<void *>(&checker_color_kernel<1>)
                    : upt == 2 ? reinterpret_cast<void *>(&checker_color_kernel<2>)
                               : reinterpret_cast<void *>(&checker_color_kernel<4>);
@@ -1095,7 +1095,17 @@ hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, Checke
     return e;
   };
   for (int p = 0; p < passes; ++p) {
-    if (n_bands > 0) {
+    if (n_bands > 0 && slabs) {
+      void *args[] = {(void *)&P, (void *)&St, (void *)&lat_pool, (void *)&tables,
+                      (void *)&band_work};
+      const unsigned blk = (unsigned)checker_slab_block();
+      hipError_t e = add(checker_slab_kernel_fn(), dim3(n_bands), dim3(blk), args,
+                         (unsigned)ring_bytes);
+      if (e == hipSuccess)
+        e = add(checker_slab_seam_kernel_fn(), dim3(n_bands), dim3(blk), args,
+                (unsigned)(3 * ring_bytes / checker_slab_ring_planes()));
+      if (e != hipSuccess) return e;
+    } else if (n_bands > 0) {
       void *args[] = {(void *)&P, (void *)&St, (void *)&lat_pool, (void *)&tables,
                       (void *)&band_work};
       hipError_t e = add(reinterpret_cast<void *>(&checker_band_kernel), dim3(n_bands),

@@ -187,8 +187,8 @@ hipError_t checker_graph_add_passes(hipGraph_t g, const CheckerParams *P, Checke
                                     mcg_sample *series, int64_t series_cap,
                                     const uint8_t *active, int n_chains, int64_t sample_period,
                                     int passes, const int32_t *band_work, int n_bands,
-                                    int ring_bytes, const int32_t *pair_work, int n_pairs,
-                                    int pair_lds, int tile_grid);
+                                    int ring_bytes, bool slabs, const int32_t *pair_work,
+                                    int n_pairs, int pair_lds, int tile_grid);
 
 // Line-class path (4 and 8 colourings): one launch per line class, both
 // halves of each line.  lds_bytes == 0: items (chain, first colour-class line,
@@ -210,6 +210,23 @@ void launch_checker_band(const CheckerParams *P, CheckerState *St, int8_t *lat_p
                          int ring_bytes, hipStream_t stream);
 int checker_band_ring_lines();
 hipError_t checker_band_set_max_lds(int bytes);
+
+// 3D J1+J2 slab path (checker_slab.hip): one launch over (chain, first plane,
+// planes) slabs with the planes bit-packed in an LDS ring, then the slab-edge
+// launch; ring_bytes = checker_slab_ring_bytes(L0, L1) of the largest plane.
+// checker_slab_fits: 8 colours, L0 % 64 == 0, even L1 and L2, the ring within
+// CHECKER_SLAB_MAX_LDS.
+constexpr int CHECKER_SLAB_MAX_LDS = 144 * 1024;
+int checker_slab_ring_bytes(int L0, int L1);
+bool checker_slab_fits(const CheckerParams &P);
+void launch_checker_slab(const CheckerParams *P, CheckerState *St, int8_t *lat_pool,
+                         const uint32_t *tables, const int32_t *slab_work, int n_slabs,
+                         int ring_bytes, hipStream_t stream);
+void *checker_slab_kernel_fn();
+void *checker_slab_seam_kernel_fn();
+int checker_slab_block();
+int checker_slab_ring_planes();
+hipError_t checker_slab_set_max_lds(int bytes);
 
 // FCC cluster expansion (fcc.hip): CheckerParams with ncolor = 64, tab_off in
 // doubles into the per-chain acceptance tables, tri = the 4096-entry

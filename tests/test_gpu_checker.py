@@ -281,6 +281,40 @@ def test_band_path_equals_colour_kernels(J2, band_lines):
                 assert ref.counters(c) == r.counters(c)
 
 
+@pytest.mark.parametrize("band_lines", [2, 4, 6, -1])
+def test_slab_path_equals_other_paths(band_lines):
+    """The 3D J1+J2 slab kernels (planes bit-packed in an LDS ring, slabs of
+    band_lines planes, the last plane of each slab in the edge launch) give the
+    LDS-resident kernel's and the line-class kernel's trajectories bit for bit,
+    for a batch of uneven chains (rows of 1 to 3 words per half, 2 to 12 planes,
+    slabs that hold a whole chain, planes of two rows)."""
+    print("case: slab path", band_lines, flush=True)
+    chains = [dict(shape=(64, 6, 4), temperature=2300.0, exchange_potential=0.04, seed=81),
+              dict(shape=(128, 2, 2), temperature=1700.0, exchange_potential=-0.2, seed=82),
+              dict(shape=(192, 10, 12), temperature=3100.0, exchange_potential=0.0, seed=83),
+              dict(shape=(64, 4, 6), temperature=900.0, exchange_potential=0.1, seed=84)]
+    ref = cm.Run(chains, mode=cm.MODE_CHECKERBOARD, lattice=cm.LATTICE_SC, J1=0.1, J2=-0.02)
+    ref.run(17)
+    line = cm.Run(chains, mode=cm.MODE_CHECKERBOARD, lattice=cm.LATTICE_SC, J1=0.1, J2=-0.02,
+                  options=cm.OPT_FORCE_STREAMING)
+    line.set_schedule(band_lines=0)
+    line.run(17)
+    for G in (1, 8):
+        r = cm.Run(chains, mode=cm.MODE_CHECKERBOARD, lattice=cm.LATTICE_SC, J1=0.1, J2=-0.02,
+                   options=cm.OPT_FORCE_STREAMING)
+        r.set_schedule(band_lines=band_lines, graph_passes=G)
+        r.run(17)
+        for c in range(len(chains)):
+            for other in (ref, line):
+                np.testing.assert_array_equal(other.get_occupation(c), r.get_occupation(c))
+                np.testing.assert_array_equal(other.samples(c), r.samples(c))
+                assert other.counters(c) == r.counters(c)
+            S1, S2, M = O.lattice_observables(3, chains[c]["shape"], r.get_occupation(c))
+            last = r.samples(c)[-1]
+            assert (int(last["S1"]), int(last["S2"]), int(last["M"])) == (S1, S2, M)
+        r.close()
+
+
 @pytest.mark.parametrize("lattice,shape", [
     (cm.LATTICE_SC, (64, 10, 6)),
     (cm.LATTICE_SC, (32, 14, 10)),
@@ -304,7 +338,7 @@ def test_line_class_tiles_equal_other_paths(lattice, shape):
         for G in (1, 8):
             r = cm.Run(ch, mode=cm.MODE_CHECKERBOARD, lattice=lattice, J1=0.1, J2=-0.02,
                        options=cm.OPT_FORCE_STREAMING)
-            r.set_schedule(units_per_thread=u, graph_passes=G, line_tiles=1)
+            r.set_schedule(units_per_thread=u, graph_passes=G, line_tiles=1, band_lines=0)
             r.run(19)
             for c in range(2):
                 for other in (ref, quad):

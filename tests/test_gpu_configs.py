@@ -3,9 +3,10 @@
 Parallel checkerboard (2D 4096^2 and 3D 256^3, J1 = 0.1, J2 = -0.02 eV, the
 temperatures T_k = 1000 + 200 k K of the 16-temperature family):
   - two chains of each family for 3 passes: the 2D band kernels equal the
-    per-colour kernels and the 4-site byte path; the 3D XCD-ordered line-class
-    kernel equals the LDS-tiled line-class kernel and the byte path; the final
-    integer observables equal the oracle's recount of the final lattice;
+    per-colour kernels and the 4-site byte path; the 3D bit-plane slab kernels,
+    the XCD-ordered line-class kernel, the LDS-tiled line-class kernel and the
+    byte path are equal; the final integer observables equal the oracle's
+    recount of the final lattice;
   - at a mid size (256^2, 32^3) and two of the family's temperatures, <e> and
     <x> agree with the oracle's sequential random-site chain within 3 sigma
     (BasicStatistics precision of each, combined in quadrature).
@@ -77,10 +78,15 @@ def test_c3_square_4096_paths_equal():
 
 def test_c3_cubic_256_paths_equal():
     chains = family((256, 256, 256), (0, 15))
+    print("case: c3 256^3 slab path", flush=True)
+    slab = checker(chains, cm.LATTICE_SC, 3, cm.OPT_FORCE_STREAMING, band_lines=-1)
+    assert slab.schedule()["band_lines"] != 0
     print("case: c3 256^3 line-class path", flush=True)
-    line = checker(chains, cm.LATTICE_SC, 3, cm.OPT_FORCE_STREAMING, line_tiles=0)
+    line = checker(chains, cm.LATTICE_SC, 3, cm.OPT_FORCE_STREAMING, band_lines=0, line_tiles=0)
+    same_runs(slab, line, 2)
+    slab.close()
     print("case: c3 256^3 tiled line-class path", flush=True)
-    tiled = checker(chains, cm.LATTICE_SC, 3, cm.OPT_FORCE_STREAMING, line_tiles=1)
+    tiled = checker(chains, cm.LATTICE_SC, 3, cm.OPT_FORCE_STREAMING, band_lines=0, line_tiles=1)
     same_runs(line, tiled, 2)
     tiled.close()
     print("case: c3 256^3 byte path", flush=True)
