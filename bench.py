@@ -31,8 +31,12 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz = 78.6 TB/s (GB/s units)
+LDS_128B_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz = 78.6 TB/s (GB/s units)
 HBM_PEAK_GBS = 8000.0
+# LDS peak for the rooflines: the 16-byte read rate measured on the device
+# (measured_peaks(), set by main()); it exceeds 128 B/clk per CU on MI355X, so
+# that figure is no ceiling there.  Until measured: 128 B/clk.
+LDS_PEAK = {"gbs": LDS_128B_GBS, "source": "256 CUs x 128 B/clk x 2.4 GHz"}
 
 
 CLOCK_GHZ = 2.4  # MI355X peak engine clock
@@ -58,7 +62,15 @@ def measured_peaks():
     lds = dict(gbs=g.value, ms=t.value,
                note="2 x 1024-thread workgroups per CU reading 16-byte LDS words, no conflicts")
     return dict(hbm_copy=hbm, lds_read=lds, hbm_datasheet_gbs=HBM_PEAK_GBS,
-                lds_nominal_gbs=LDS_PEAK_GBS)
+                lds_128B_per_clk_gbs=LDS_128B_GBS)
+
+
+def set_lds_peak():
+    """Measure the LDS read rate once and use it as the LDS roofline peak."""
+    pk = measured_peaks()
+    LDS_PEAK["gbs"] = pk["lds_read"]["gbs"]
+    LDS_PEAK["source"] = "measured 16-byte LDS read rate (measured_peaks.lds_read)"
+    return pk
 
 
 # HBM traffic of one batched exact launch (64 chains of 128x128), FETCH_SIZE x
@@ -201,21 +213,14 @@ def bench_canonical(passes):
 
 def bench_nfold(passes):
     """N-fold way chains (NFOLD mode): the 64 conditions of the exact scan
-    (128x128, 8 T x 8 mu) in one launch vs one chain alone.  Besides events/s
-    the Metropolis-equivalent rate: a random-site Metropolis attempt accepts
-    with probability R/N, so the simulated time (sum of 1/R per event on
-    average) times N counts the attempts the events stand for."""
+    (128x128, 8 T x 8 mu) in one launch vs one chain alone, in events/s."""
     import casmcode_monte_amd as cm
     conds = c2_conditions()
     r = cm.Run(conds, mode=cm.MODE_NFOLD, J1=0.1)
     r.run(2)
-    t0 = [r.samples(c)["time"][-1] for c in range(len(conds))]
     w, k, a = timed_run(r, passes)
-    t1 = [r.samples(c)["time"][-1] for c in range(len(conds))]
-    n = 128 * 128
-    equiv = sum(n * (b - e) for b, e in zip(t1, t0))
     out = dict(chains=len(conds), passes=passes, wall_s=w, kernel_s=k, events=a,
-               events_per_s=a / w, metropolis_equivalent_attempts_per_s=equiv / w)
+               events_per_s=a / w)
     r.close()
     # the chip holds two 128x128 chains per CU: 8 independent replicas (seeds)
     # of each condition fill it
@@ -270,8 +275,8 @@ def bench_checker_scan(passes):
                 lds_block=sched["lds_block"],
                 plain_attempts_per_s=plain_att / plain_wall,
                 ratio=(ba / bw) / (plain_att / plain_wall),
-                roofline={"bound": "lds", "achieved": lds_gbs, "peak": LDS_PEAK_GBS,
-                          "unit": "GB/s", "frac": lds_gbs / LDS_PEAK_GBS,
+                roofline={"bound": "lds", "achieved": lds_gbs, "peak": LDS_PEAK["gbs"],
+                          "unit": "GB/s", "frac": lds_gbs / LDS_PEAK["gbs"],
                           "bytes_per_attempt": 6,
                           "kernel": "checker_lds_kernel, 64 chains of 128x128 J1"})
 
@@ -319,8 +324,8 @@ def bench_fcc(passes, plain=True):
                 attempts_per_s=a / w, lds_block=sched["lds_block"],
                 plain_attempts_per_s=pa / pw, plain_passes=pp,
                 batched_over_plain=(a / w) / (pa / pw),
-                roofline={"bound": "lds", "achieved": lds_gbs, "peak": LDS_PEAK_GBS,
-                          "unit": "GB/s", "frac": lds_gbs / LDS_PEAK_GBS,
+                roofline={"bound": "lds", "achieved": lds_gbs, "peak": LDS_PEAK["gbs"],
+                          "unit": "GB/s", "frac": lds_gbs / LDS_PEAK["gbs"],
                           "bytes_per_attempt": 44})
 
 
@@ -410,7 +415,7 @@ def main():
     ap.add_argument("--served-requests", type=int, default=400,
                     help="requests of the served scan (0 skips it)")
     ap.add_argument("--only", default="",
-                    choices=["", "exact", "scan", "band4096", "sc256", "fcc", "peaks"],
+                    choices=["", "exact", "scan", "band4096", "sc256", "fcc", "served", "peaks"],
                     help="run one measurement and print its JSON (for profiling one kernel)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes 10 times "
@@ -436,14 +441,18 @@ def main():
     K, W = args.steps, args.warmup
     if args.only:
         import casmcode_monte_amd as cm
+        if args.only in ("scan", "fcc"):
+            set_lds_peak()
         one = {"exact": lambda: bench_exact(K, W, extras=False),
                "scan": lambda: bench_checker_scan(K),
                "band4096": lambda: bench_checker_family((4096, 4096, 1), cm.LATTICE_SQUARE, K),
                "sc256": lambda: bench_checker_family((256, 256, 256), cm.LATTICE_SC, K),
                "fcc": lambda: bench_fcc(K, plain=False),
+               "served": lambda: bench_served(args.served_requests),
                "peaks": measured_peaks}[args.only]()
         print(json.dumps({"only": args.only, "steps": K, **one}))
         return
+    peaks = set_lds_peak()
     ex = bench_exact(K, W)
     print(f"[bench] exact batched done: {ex['attempts'] / ex['wall_s']:.3e} attempts/s",
           file=sys.stderr, flush=True)
@@ -480,8 +489,9 @@ def main():
                                "MT19937-64, FP64 running energy, one chain per wavefront",
                    "chains": 64, "sites_per_chain": 16384, "passes": K,
                    "parallelism": "1 GPU, 64 workgroups (one 64-lane wavefront per chain)"},
-        "roofline": {"bound": "latency", "achieved": lds_achieved, "peak": LDS_PEAK_GBS,
-                     "unit": "GB/s", "frac": lds_achieved / LDS_PEAK_GBS,
+        "roofline": {"bound": "latency", "achieved": lds_achieved, "peak": LDS_PEAK["gbs"],
+                     "unit": "GB/s", "frac": lds_achieved / LDS_PEAK["gbs"],
+                     "peak_source": LDS_PEAK["source"],
                      "traffic": traffic,
                      "kernel": "exact_block_kernel",
                      "bytes_per_attempt": 6,
@@ -543,11 +553,14 @@ def main():
             "unit": "GB/s", "frac": fam2["hbm_frac"], "traffic": None,
             "kernel": ("checker_band_kernel + checker_seam_kernel" if fam2["schedule"].get("band_lines")
                        else "checker_color_kernel") + ", 16 x 4096^2 J1+J2 (2 B/attempt algorithmic)"}
+        line["roofline_checkerboard_3d"] = {
+            "bound": "hbm", "achieved": fam3["hbm_gbs_algorithmic"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": fam3["hbm_frac"], "traffic": None,
+            "kernel": ("checker_slab_kernel + checker_slab_seam_kernel"
+                       if fam3["schedule"].get("band_lines") else "checker_pair_kernel")
+                      + ", 16 x 256^3 J1+J2 (2 B/attempt algorithmic)"}
     if not args.skip_extra:
-        line["measured_peaks"] = measured_peaks()
-        pk = line["measured_peaks"]
-        line["roofline"]["frac_of_measured_lds"] = lds_achieved / pk["lds_read"]["gbs"]
-        print("[bench] measured peaks done", file=sys.stderr, flush=True)
+        line["measured_peaks"] = peaks
     if not args.skip_cpu:
         line["cpu_baseline"] = cpu_baseline(16, 1000)
         print("[bench] cpu baseline done", file=sys.stderr, flush=True)

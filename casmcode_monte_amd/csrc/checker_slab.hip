@@ -13,7 +13,9 @@
 // which neither read nor write each other's planes.  B on the slab's last
 // plane needs the next slab's first plane after its A, so it runs in a
 // second launch (checker_slab_seam_kernel).  Within a plane the four colours
-// run in order with a barrier between them.  Every site reads the neighbour
+// run in order with a barrier between them; a workgroup of 512 threads (two
+// words per thread and colour on 256^2 planes) leaves room for two slabs per
+// CU, whose barriers do not wait for each other.  Every site reads the neighbour
 // values of the colour-by-colour sweep and the same Philox words (counters of
 // colour-class line ln = (j>>1) + (L1/2)(k>>1), as in checker.hip), so the
 // trajectory equals the per-colour and line-class kernels' bit for bit.
@@ -37,7 +39,7 @@ namespace mcg {
 
 namespace {
 
-constexpr int SLAB_BLOCK = 1024;
+constexpr int SLAB_BLOCK = 512;
 constexpr int SLAB_RING = 8;
 
 struct SlabGeom {
@@ -122,7 +124,7 @@ __device__ __forceinline__ void csa(uint32_t x, uint32_t y, uint32_t z, uint32_t
 
 // nibble n of x spread to the low bit of four bytes
 __device__ __forceinline__ uint32_t spread(uint32_t x, int n) {
-  return unpack4((x >> (4 * n)) & 15u);
+  return __umul24((x >> (4 * n)) & 15u, 0x00204081u) & 0x01010101u;
 }
 
 struct SlabDelta {
