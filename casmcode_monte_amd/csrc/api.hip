@@ -365,6 +365,7 @@ struct mcg_handle {
   uint8_t *d_tri_lines = nullptr;
   // bit-line kernel: every chain L0 <= 32, lattice in LDS as one word per line
   bool fcc_bits = false;
+  bool fcc_wide = false;  // bit-line kernel with lines of up to 64 sites
   int fcc_bits_lds = 0;
   bool fcc() const { return lattice == MCG_LATTICE_FCC; }
 
@@ -910,14 +911,18 @@ int build_fcc(mcg_handle *h) {
   h->checker_lds_bytes = h->checker_lds ? (int)max_lds : 0;
   // the bit-line kernel when every line fits one word
   h->fcc_bits = !(h->options & MCG_OPT_FORCE_STREAMING);
+  h->fcc_wide = false;
   h->fcc_bits_lds = 0;
   for (int c = 0; c < h->n_chains; ++c) {
     auto const &P = h->cp[c];
     if (P.L0 > FCC_BITS_MAX_L0) h->fcc_bits = false;
+    if (P.L0 > FCC_BITS_NARROW_L0) h->fcc_wide = true;
     h->fcc_bits_lds = std::max(h->fcc_bits_lds, fcc_bits_lds_bytes(P.L1, P.L2));
   }
   if (h->fcc_bits_lds > FCC_LDS_MAX) h->fcc_bits = false;
+  if (!h->fcc_bits) h->fcc_wide = false;
   if (h->fcc_bits) h->checker_lds = true;
+  if (h->fcc_wide) h->checker_block = 512;  // the wide kernel's launch bound
   HIP_TRY(fcc_set_max_lds((int)FCC_LDS_MAX));
   const std::vector<uint8_t> tri = fcc_tri_table();
   const std::vector<uint8_t> tri_lines = fcc_tri_table_lines();
@@ -1651,7 +1656,8 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
       return rc;
     }
     const bool no_room =
-        h->fcc_bits ? (P.L0 > FCC_BITS_MAX_L0 || fcc_bits_lds_bytes(P.L1, P.L2) > h->fcc_bits_lds)
+        h->fcc_bits ? (P.L0 > (h->fcc_wide ? FCC_BITS_MAX_L0 : FCC_BITS_NARROW_L0) ||
+                       fcc_bits_lds_bytes(P.L1, P.L2) > h->fcc_bits_lds)
                     : (h->checker_lds && fcc_lds_bytes(P) > h->checker_lds_bytes);
     if (no_room) {
       h->chains[chain] = old;
@@ -1815,7 +1821,7 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
     launch_fcc(h->d_cp, h->d_cs, h->d_lat, h->d_ftab, h->d_tri, h->d_tri_lines, h->d_series,
                h->series_cap, h->d_active, h->n_chains, n_passes, h->sample_period,
                h->fcc_bits ? h->fcc_bits_lds : h->checker_lds_bytes, h->checker_lds, h->fcc_bits,
-               h->checker_block, h->stream);
+               h->fcc_wide, h->checker_block, h->stream);
     const hipError_t le = hipGetLastError();
     if (le != hipSuccess)
       return fail(MCG_ERR_HIP, std::string("fcc launch (block ") +
@@ -2279,6 +2285,8 @@ int mcg_set_schedule(mcg_handle *h, const mcg_schedule *s) {
   if (s->band_lines != -1 && (s->band_lines < 0 || s->band_lines > 4096 || (s->band_lines & 1)))
     return fail(MCG_ERR_ARG, "band_lines must be -1 (automatic), 0 (off) or even <= 4096");
   if (s->line_tiles != 0 && s->line_tiles != 1) return fail(MCG_ERR_ARG, "line_tiles must be 0 or 1");
+  if (h->fcc_wide && s->lds_block > 512)
+    return fail(MCG_ERR_ARG, "lds_block: FCC lines of more than 32 sites run at most 512 threads");
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->checker_block = (int)s->lds_block;
   if (h->band_lines != (int)s->band_lines || h->units_per_thread != (int)s->units_per_thread ||
@@ -2320,7 +2328,8 @@ int mcg_autotune(mcg_handle *h, int64_t trial_passes, mcg_schedule *chosen) {
     std::vector<mcg_schedule> cands;
     if (h->checker_lds || h->fcc()) {
       for (int64_t b : {256, 512, 1024})
-        cands.push_back({b, h->units_per_thread, 1, 0.0, h->band_lines, h->line_tiles});
+        if (!(h->fcc_wide && b > 512))
+          cands.push_back({b, h->units_per_thread, 1, 0.0, h->band_lines, h->line_tiles});
     } else {
       const bool line_class = h->ncolor == 4 || h->ncolor == 8;
       for (int64_t lt : {0, 1}) {

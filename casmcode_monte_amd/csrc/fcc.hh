@@ -52,9 +52,12 @@ __host__ __device__ inline int fcc_lds_stride(int L0) {
   return (s & 3) == 0 ? s + 1 : s;
 }
 
-// Bit-line kernel (fcc.hip, LDS-resident chains with L0 <= 32): one 64-bit
-// word per lattice line, stored at ln + ln / 4; dynamic LDS bytes of a chain.
-constexpr int FCC_BITS_MAX_L0 = 32;
+// Bit-line kernel (fcc.hip, LDS-resident chains with L0 <= 64): one 64-bit
+// word per lattice line, stored at ln + ln / 4 -- the line doubled when
+// L0 <= FCC_BITS_NARROW_L0, else the line once (doubled in registers);
+// dynamic LDS bytes of a chain.
+constexpr int FCC_BITS_NARROW_L0 = 32;
+constexpr int FCC_BITS_MAX_L0 = 64;
 __host__ __device__ inline int fcc_bits_lds_bytes(int L1, int L2) {
   const int n_lines = 4 * L1 * L2;
   return 8 * (n_lines + (n_lines >> 2) + 1);

@@ -328,7 +328,7 @@ __global__ __launch_bounds__(1024) void fcc_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Bit-line kernel: LDS-resident chains with L0 <= 32.
+// Bit-line kernel: LDS-resident chains with L0 <= 64.
 //
 // Each lattice line (fixed y, z) is one 64-bit word D = core | core << L0,
 // bit h of core the occupation of site h, so the three sites h-1, h, h+1 of
@@ -346,12 +346,48 @@ __global__ __launch_bounds__(1024) void fcc_kernel(
 // words and decisions are those of fcc_kernel: trajectories are identical.
 // Words are stored at ln + ln / 4, so the lines four apart that a wavefront
 // reads together fall into distinct banks.
+//
+// Lines of 33 to 64 sites (WIDE): the word holds the line once (bits 0..L0-1)
+// and the doubled line is formed in registers when a line is read, as the
+// 96 bits lo = core | core << L0 (bits 0..63) and hi = core >> (64 - L0)
+// (bits 64..95); a window starting at st < L0 <= 64 then lies in lo for
+// st < 32 and in the top half of lo and hi otherwise.
 __device__ __forceinline__ int fcc_word_index(int ln) { return ln + (ln >> 2); }
 
 __device__ __forceinline__ uint32_t fcc_window(uint64_t D, int st) {
   return __builtin_amdgcn_alignbit((uint32_t)(D >> 32), (uint32_t)D, (uint32_t)st);
 }
 
+// one line as the windows read it: the doubled line (narrow: the stored word
+// itself; WIDE: lo and hi formed from the stored line)
+template <bool WIDE>
+struct FccLine {
+  uint64_t lo;
+  uint32_t hi;
+};
+
+template <bool WIDE>
+__device__ __forceinline__ FccLine<WIDE> fcc_line(uint64_t stored, int L0) {
+  FccLine<WIDE> l;
+  if (WIDE) {
+    l.lo = L0 < 64 ? stored | (stored << L0) : stored;
+    l.hi = (uint32_t)(stored >> (64 - L0));
+  } else {
+    l.lo = stored;
+    l.hi = 0u;
+  }
+  return l;
+}
+
+template <bool WIDE>
+__device__ __forceinline__ uint32_t fcc_win(const FccLine<WIDE> &l, int st) {
+  if (!WIDE) return fcc_window(l.lo, st);
+  const uint32_t a = (uint32_t)l.lo, b = (uint32_t)(l.lo >> 32);
+  return st < 32 ? __builtin_amdgcn_alignbit(b, a, (uint32_t)st)
+                 : __builtin_amdgcn_alignbit(l.hi, b, (uint32_t)(st - 32));
+}
+
+template <bool WIDE>
 __device__ __forceinline__ void fcc_bits_unit(uint64_t *__restrict__ w,
                                               const double *__restrict__ tab,
                                               const uint8_t *__restrict__ tri, const FccLists &Ls,
@@ -375,59 +411,100 @@ __device__ __forceinline__ void fcc_bits_unit(uint64_t *__restrict__ w,
     ys[j] = wrap_ring(y + j - 2, ry);
     zs[j] = wrap_ring(z + j - 2, rz) * ry;
   }
-  auto line = [&](int dy, int dz) -> uint64_t {
-    return w[fcc_word_index(ys[dy + 2] + zs[dz + 2])];
+  typedef FccLine<WIDE> Line;
+  auto line = [&](int dy, int dz) -> Line {
+    return fcc_line<WIDE>(w[fcc_word_index(ys[dy + 2] + zs[dz + 2])], L0);
   };
   const int own = fcc_word_index(y + ry * z);
-  const uint64_t self = w[own];
-  // odd lines: nearest neighbours (dx = -1, +1) and third-shell ones
-  const uint64_t om10 = line(-1, 0), op10 = line(1, 0), om01 = line(0, -1), op01 = line(0, 1);
-  const uint64_t t0 = line(-2, -1), t1 = line(-2, 1), t2 = line(2, -1), t3 = line(2, 1);
-  const uint64_t t4 = line(-1, -2), t5 = line(1, -2), t6 = line(-1, 2), t7 = line(1, 2);
-  // even lines: nearest neighbours (dx = 0) and third-shell ones (dx = +-2)
-  const uint64_t emm = line(-1, -1), emp = line(-1, 1), epm = line(1, -1), epp = line(1, 1);
-  // even lines: second-shell neighbours (dx = 0)
-  const uint64_t s20 = line(-2, 0), s21 = line(2, 0), s22 = line(0, -2), s23 = line(0, 2);
+  // the unit's four sites (tail slots repeat the last site): h, the even
+  // window start se (h-1, h, h+1) and the odd one so (h+p-1, h+p)
+  int hh[4], se[4], so[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int s = s0 + v < cnt ? s0 + v : cnt - 1;
+    hh[v] = Ls.xh[k.cx][p][s];
+    se[v] = hh[v] == 0 ? L0 - 1 : hh[v] - 1;
+    so[v] = hh[v] + p - 1 + (hh[v] + p - 1 < 0 ? L0 : 0);
+  }
+  // the counts of the four sites, line by line: own spin sg, second shell n2,
+  // third shell n3, and the nearest-neighbour mask in the bit order of the
+  // host's fcc_tri_table_lines (bits 0-7 the odd lines (-1,0), (1,0), (0,-1),
+  // (0,1) at x-1, x+1; bits 8-11 the even lines (-1,-1), (-1,1), (1,-1), (1,1))
+  int sg[4], n2[4], n3[4];
+  uint32_t mask[4];
+  {
+    const Line L = fcc_line<WIDE>(w[own], L0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const uint32_t ws = fcc_win<WIDE>(L, se[v]);
+      sg[v] = (ws >> 1) & 1;
+      n2[v] = __popc(ws & 5u);  // x -+ 2 on the own line
+      n3[v] = 0;
+      mask[v] = 0u;
+    }
+  }
+  auto even2 = [&](int dy, int dz) {  // second shell: (0, +-2, 0), (0, 0, +-2)
+    const Line L = line(dy, dz);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) n2[v] += (int)((fcc_win<WIDE>(L, se[v]) >> 1) & 1u);
+  };
+  even2(-2, 0);
+  even2(2, 0);
+  even2(0, -2);
+  even2(0, 2);
+  auto even1 = [&](int dy, int dz, int bit) {  // nearest (dx = 0) and third (dx = +-2)
+    const Line L = line(dy, dz);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const uint32_t wv = fcc_win<WIDE>(L, se[v]);
+      n3[v] += __popc(wv & 5u);
+      mask[v] |= ((wv >> 1) & 1u) << bit;
+    }
+  };
+  even1(-1, -1, 8);
+  even1(-1, 1, 9);
+  even1(1, -1, 10);
+  even1(1, 1, 11);
+  auto odd3 = [&](int dy, int dz) {  // third shell (dx = +-1)
+    const Line L = line(dy, dz);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) n3[v] += __popc(fcc_win<WIDE>(L, so[v]) & 3u);
+  };
+  odd3(-2, -1);
+  odd3(-2, 1);
+  odd3(2, -1);
+  odd3(2, 1);
+  odd3(-1, -2);
+  odd3(1, -2);
+  odd3(-1, 2);
+  odd3(1, 2);
+  auto odd1 = [&](int dy, int dz, int shift) {  // nearest (dx = +-1)
+    const Line L = line(dy, dz);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) mask[v] |= (fcc_win<WIDE>(L, so[v]) & 3u) << shift;
+  };
+  odd1(-1, 0, 0);
+  odd1(1, 0, 2);
+  odd1(0, -1, 4);
+  odd1(0, 1, 6);
   const u32x4 r = philox4x32_10((uint32_t)((y + ry * z) * 32 + q), (uint32_t)pass,
                                 (uint32_t)(pass >> 32), (uint32_t)k.col, key0, key1);
   uint64_t flips = 0;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const int s = s0 + v < cnt ? s0 + v : cnt - 1;  // tail slots repeat the last site
-    const int h = Ls.xh[k.cx][p][s];
-    const int se = h == 0 ? L0 - 1 : h - 1;  // even window: h-1, h, h+1
-    int so = h + p - 1;                      // odd window: h+p-1, h+p
-    so += so < 0 ? L0 : 0;
-    const uint32_t ws = fcc_window(self, se);
-    const int sg = (ws >> 1) & 1;
-    const int n2 = __popc(ws & 5u) + (int)((fcc_window(s20, se) >> 1) & 1u) +
-                   (int)((fcc_window(s21, se) >> 1) & 1u) + (int)((fcc_window(s22, se) >> 1) & 1u) +
-                   (int)((fcc_window(s23, se) >> 1) & 1u);
-    const uint32_t wmm = fcc_window(emm, se), wmp = fcc_window(emp, se);
-    const uint32_t wpm = fcc_window(epm, se), wpp = fcc_window(epp, se);
-    int n3 = __popc(wmm & 5u) + __popc(wmp & 5u) + __popc(wpm & 5u) + __popc(wpp & 5u);
-    n3 += __popc(fcc_window(t0, so) & 3u) + __popc(fcc_window(t1, so) & 3u) +
-          __popc(fcc_window(t2, so) & 3u) + __popc(fcc_window(t3, so) & 3u) +
-          __popc(fcc_window(t4, so) & 3u) + __popc(fcc_window(t5, so) & 3u) +
-          __popc(fcc_window(t6, so) & 3u) + __popc(fcc_window(t7, so) & 3u);
-    // nearest-neighbour mask in the bit order of the host's fcc_tri_table_lines
-    const uint32_t mask = (fcc_window(om10, so) & 3u) | ((fcc_window(op10, so) & 3u) << 2) |
-                          ((fcc_window(om01, so) & 3u) << 4) |
-                          ((fcc_window(op01, so) & 3u) << 6) | (((wmm >> 1) & 1u) << 8) |
-                          (((wmp >> 1) & 1u) << 9) | (((wpm >> 1) & 1u) << 10) |
-                          (((wpp >> 1) & 1u) << 11);
-    const int n1 = __popc(mask);
-    const int t = tri[mask];
-    const double pr = tab[sg * FCC_TAB_A + n1 * (FCC_Z2 + 1) + n2] *
-                      tab[2 * FCC_TAB_A + sg * FCC_TAB_B + n3 * (FCC_ZT + 1) + t];
+    const int n1 = __popc(mask[v]);
+    const int t = tri[mask[v]];
+    const double pr = tab[sg[v] * FCC_TAB_A + n1 * (FCC_Z2 + 1) + n2[v]] *
+                      tab[2 * FCC_TAB_A + sg[v] * FCC_TAB_B + n3[v] * (FCC_ZT + 1) + t];
     const double uu = ((double)r.v[v] + 0.5) * 0x1p-32;
     if (s0 + v < cnt && uu < pr) {
-      flips |= (1ull << h) | (1ull << (h + L0));
-      const int ds = 1 - 2 * sg;
+      const int h = hh[v];
+      flips |= WIDE ? (1ull << h) : ((1ull << h) | (1ull << (h + L0)));
+      const int ds = 1 - 2 * sg[v];
       d.N1 += ds;
       d.P1 += ds * n1;
-      d.P2 += ds * n2;
-      d.P3 += ds * n3;
+      d.P2 += ds * n2[v];
+      d.P3 += ds * n3[v];
       d.T += ds * t;
       d.acc += 1;
     }
@@ -435,7 +512,10 @@ __device__ __forceinline__ void fcc_bits_unit(uint64_t *__restrict__ w,
   if (flips) atomicXor(reinterpret_cast<unsigned long long *>(w + own), (unsigned long long)flips);
 }
 
-__global__ __launch_bounds__(1024) void fcc_bits_kernel(
+// WIDE chains run 512-thread workgroups: the doubled lines formed in registers
+// need more than the 128 VGPRs a 1024-thread workgroup leaves
+template <bool WIDE>
+__global__ __launch_bounds__(WIDE ? 512 : 1024) void fcc_bits_kernel(
     const CheckerParams *__restrict__ Pp, CheckerState *__restrict__ Sp,
     int8_t *__restrict__ lat_pool, const double *__restrict__ tables,
     const uint8_t *__restrict__ tri_g, mcg_sample *__restrict__ series, int64_t series_cap,
@@ -460,7 +540,7 @@ __global__ __launch_bounds__(1024) void fcc_bits_kernel(
   for (int ln = t; ln < n_lines; ln += nt) {
     uint64_t core = 0;
     for (int h = 0; h < L0; ++h) core |= (uint64_t)(g_lat[ln * L0 + h] & 1) << h;
-    w[fcc_word_index(ln)] = core | (core << L0);
+    w[fcc_word_index(ln)] = WIDE ? core : (core | (core << L0));
   }
   __syncthreads();
   int64_t pass = St->n_pass;
@@ -474,7 +554,7 @@ __global__ __launch_bounds__(1024) void fcc_bits_kernel(
       const int units = k.start[4];
       if (units == 0) continue;  // the same for every thread
       for (int u = t; u < units; u += nt)
-        fcc_bits_unit(w, tab, tri, Ls, L0, L1, L2, k, u, pass, P.key0, P.key1, d);
+        fcc_bits_unit<WIDE>(w, tab, tri, Ls, L0, L1, L2, k, u, pass, P.key0, P.key1, d);
       __syncthreads();
     }
     const int v0 = fcc_wave_sum(d.N1), v1 = fcc_wave_sum(d.P1), v2 = fcc_wave_sum(d.P2);
@@ -536,10 +616,14 @@ void launch_fcc(const CheckerParams *P, CheckerState *St, int8_t *lat_pool, cons
                 const uint8_t *tri, const uint8_t *tri_lines, mcg_sample *series,
                 int64_t series_cap, const uint8_t *active, int n_chains, int64_t n_passes,
                 int64_t sample_period, int lds_bytes, bool lattice_in_lds, bool bit_lines,
-                int block, hipStream_t stream) {
-  if (bit_lines)
-    hipLaunchKernelGGL(fcc_bits_kernel, dim3(n_chains), dim3(block), lds_bytes, stream, P, St,
-                       lat_pool, tables, tri_lines, series, series_cap, active, n_passes,
+                bool wide_lines, int block, hipStream_t stream) {
+  if (bit_lines && wide_lines)
+    hipLaunchKernelGGL(fcc_bits_kernel<true>, dim3(n_chains), dim3(block), lds_bytes, stream, P,
+                       St, lat_pool, tables, tri_lines, series, series_cap, active, n_passes,
+                       sample_period);
+  else if (bit_lines)
+    hipLaunchKernelGGL(fcc_bits_kernel<false>, dim3(n_chains), dim3(block), lds_bytes, stream, P,
+                       St, lat_pool, tables, tri_lines, series, series_cap, active, n_passes,
                        sample_period);
   else if (lattice_in_lds)
     hipLaunchKernelGGL(fcc_kernel<true>, dim3(n_chains), dim3(block), lds_bytes, stream, P, St,
@@ -552,8 +636,11 @@ void launch_fcc(const CheckerParams *P, CheckerState *St, int8_t *lat_pool, cons
 }
 
 hipError_t fcc_set_max_lds(int bytes) {
-  const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(fcc_bits_kernel),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(fcc_bits_kernel<false>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(reinterpret_cast<const void *>(fcc_bits_kernel<true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute(reinterpret_cast<const void *>(fcc_kernel<true>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);

@@ -231,13 +231,13 @@ hipError_t checker_slab_set_max_lds(int bytes);
 // FCC cluster expansion (fcc.hip): CheckerParams with ncolor = 64, tab_off in
 // doubles into the per-chain acceptance tables, tri = the 4096-entry
 // triangle-count table of the nearest-neighbour occupancy mask.
-// bit_lines: the bit-line kernel (every chain L0 <= 32, lattice in LDS as one
+// bit_lines: the bit-line kernel (every chain L0 <= 64, lattice in LDS as one
 // 64-bit word per line, tri_lines the triangle table in its mask bit order)
 void launch_fcc(const CheckerParams *P, CheckerState *St, int8_t *lat_pool, const double *tables,
                 const uint8_t *tri, const uint8_t *tri_lines, mcg_sample *series,
                 int64_t series_cap, const uint8_t *active, int n_chains, int64_t n_passes,
                 int64_t sample_period, int lds_bytes, bool lattice_in_lds, bool bit_lines,
-                int block, hipStream_t stream);
+                bool wide_lines, int block, hipStream_t stream);
 
 // ---------------------------------------------------------------------------
 // Statistics on the device sample series

@@ -13,8 +13,9 @@ equilibration samples must reach calculated_precision < requested
 on the device over the device-resident sample series.
 
 Slots: requests are routed by model and lattice size to pools (one library
-handle and one host thread each): LDS-resident classes up to 128 KB (Ising) or
-142 KB (FCC) lattices, and an HBM class above.  A pool runs `chunk` passes of
+handle and one host thread each): LDS-resident classes up to 128 KB Ising
+lattices and, for FCC, every class (the bit-line kernel holds a 48^3-cell
+lattice in 92 KB of LDS), and an HBM class above 362 for Ising.  A pool runs `chunk` passes of
 all its active slots per round (one launch), then checks the slots whose own
 pass count reached a multiple
 of `check_period`, retires finished requests and admits waiting ones into the
@@ -83,8 +84,8 @@ class Result:
 # (max edge, slots, LDS-resident) per size class
 DEFAULT_CLASSES = ((64, 96, True), (128, 64, True), (256, 48, True), (362, 32, True),
                    (512, 16, False))
-# FCC: edges in conventional cells (4 L^3 sites; LDS-resident up to L = 32)
-DEFAULT_FCC_CLASSES = ((12, 64, True), (20, 48, True), (32, 32, True), (48, 12, False))
+# FCC: edges in conventional cells (4 L^3 sites, bit lines in LDS up to L = 48)
+DEFAULT_FCC_CLASSES = ((12, 64, True), (20, 48, True), (32, 32, True), (48, 12, True))
 
 
 class _Pool:

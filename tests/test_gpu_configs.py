@@ -15,8 +15,9 @@ autotuned launch): every chain's counts equal the oracle's recount, and three
 chains (16^3, 24^3, 32^3) equal the same chains run alone.
 Served scan (the Poisson trace of 50/50 2D Ising / FCC requests, edges in
 [16, 512] and [8, 48]): the first 200 requests with arrivals compressed 20x;
-a sample including FCC edges >= 8, FCC and Ising requests of the HBM classes,
-gives each request's result alone and in the mix bit for bit.
+a sample including FCC edges >= 8, the FCC class above 32 cells (wide bit
+lines) and the Ising HBM class, gives each request's result alone and in the
+mix bit for bit.
 """
 import math
 
@@ -176,7 +177,7 @@ def test_c5_served_trace_mix_equals_alone():
     fcc_reqs = [q for q in reqs if q.model == "fcc"]
     ising = [q for q in reqs if q.model == "ising"]
     assert fcc_reqs and min(q.edge for q in fcc_reqs) >= 8
-    # the sample: the largest FCC requests (the HBM class above 32 cells), FCC
+    # the sample: the largest FCC requests (the class above 32 cells), FCC
     # requests near the low end, the largest Ising requests (the 512 HBM class)
     # and a few small Ising requests
     by_edge = lambda qs: sorted(qs, key=lambda q: (q.edge, q.rid))

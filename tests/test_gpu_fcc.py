@@ -66,6 +66,32 @@ def test_lds_global_and_block_sizes_identical(shape):
         assert ref.counters(0) == got.counters(0)
 
 
+@pytest.mark.parametrize("shapes", [
+    [(40, 6, 8)],
+    [(33, 4, 4), (47, 5, 7), (64, 4, 6)],
+    [(48, 10, 10), (16, 6, 4), (5, 3, 4)],  # narrow chains in a launch of wide lines
+])
+def test_wide_bit_lines_equal_global_kernel(shapes):
+    """Lines of 33 to 64 sites in the bit-line kernel (the line once per word,
+    doubled in registers, 512-thread workgroups): the trajectories equal the
+    global-memory byte kernel's and the counts the oracle's recount, for
+    chains of both line lengths in one launch and two block sizes."""
+    print("case: fcc wide bit lines", shapes, flush=True)
+    chains = [dict(shape=sh, temperature=900.0 + 300.0 * i, exchange_potential=0.05 - 0.1 * i,
+                   seed=30 + i) for i, sh in enumerate(shapes)]
+    ref = run(chains, 12, options=cm.OPT_FORCE_STREAMING)
+    for block in (None, 256):
+        got = run(chains, 12, block=block)
+        for c in range(len(chains)):
+            np.testing.assert_array_equal(ref.get_occupation(c), got.get_occupation(c))
+            np.testing.assert_array_equal(ref.samples(c), got.samples(c))
+            assert ref.counters(c) == got.counters(c)
+            assert counts_of(got.samples(c)[-1]) == O.fcc_counts(chains[c]["shape"],
+                                                                  got.get_occupation(c))
+        got.close()
+    ref.close()
+
+
 def test_batched_equals_alone():
     chains = [dict(shape=(3, 3, 3), temperature=800.0, exchange_potential=0.05, seed=1),
               dict(shape=(6, 4, 5), temperature=1700.0, exchange_potential=-0.3, seed=2),
