@@ -75,9 +75,9 @@ def set_lds_peak():
 
 # HBM traffic of one batched exact launch (64 chains of 128x128), FETCH_SIZE x
 # 2.0 + WRITE_SIZE x 1.0 of rocprofv3 --pmc at 20 and 2000 passes
-# (profiles/exact_block_kernel_64x128sq_traffic.md), as fixed + per-pass bytes
-EXACT_TRAFFIC_FIXED_B = 2636800.0
-EXACT_TRAFFIC_PER_PASS_B = 5122.56
+# (profiles/exact_block_kernel_64x128sq.md), as fixed + per-pass bytes
+EXACT_TRAFFIC_FIXED_B = 2647478.0
+EXACT_TRAFFIC_PER_PASS_B = 5120.5
 
 
 def c2_conditions(shape=(128, 128)):
