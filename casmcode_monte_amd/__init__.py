@@ -19,6 +19,7 @@ MODE_EXACT = 0
 MODE_CHECKERBOARD = 1
 MODE_CANONICAL = 2
 MODE_NFOLD = 3
+MODE_OCCUPANT = 4
 LATTICE_SQUARE = 2
 LATTICE_SC = 3
 LATTICE_FCC = 4
@@ -42,7 +43,7 @@ EXPORTED = [
     "mcg_equilibration", "mcg_last_run_time", "mcg_assign_chain", "mcg_autotune",
     "mcg_get_schedule", "mcg_set_schedule", "mcg_reserve_samples", "mcg_get_occ_location",
     "mcg_set_occ_location", "mcg_get_nfold_state", "mcg_run_steps", "mcg_get_position",
-    "mcg_check_batch",
+    "mcg_check_batch", "mcg_set_site_asym",
 ]
 
 
@@ -51,12 +52,21 @@ class ChainDesc(C.Structure):
                 ("exchange_potential", C.c_double), ("seed", C.c_uint64)]
 
 
+class OccupantDesc(C.Structure):
+    _fields_ = [("n_asym", C.c_int64), ("occ_size", C.POINTER(C.c_int64)),
+                ("species", C.POINTER(C.c_int64)), ("n_species", C.c_int64),
+                ("canonical", C.c_int64), ("n_swaps", C.c_int64),
+                ("swaps", C.POINTER(C.c_int64)), ("exchange_potential", C.POINTER(C.c_double)),
+                ("sites_per_unitcell", C.c_int64)]
+
+
 class Desc(C.Structure):
     _fields_ = [("mode", C.c_int64), ("lattice", C.c_int64), ("use_nlist", C.c_int64),
                 ("J1", C.c_double), ("J2", C.c_double), ("kb", C.c_double),
                 ("sample_period", C.c_int64), ("n_chains", C.c_int64),
                 ("chains", C.POINTER(ChainDesc)), ("options", C.c_int64),
-                ("eci", C.POINTER(C.c_double)), ("sample_time_period", C.c_double)]
+                ("eci", C.POINTER(C.c_double)), ("sample_time_period", C.c_double),
+                ("occupant", C.POINTER(OccupantDesc))]
 
 
 SAMPLE_DTYPE = np.dtype([("pass", np.int64), ("S1", np.int64), ("S2", np.int64),
@@ -130,6 +140,7 @@ def lib():
         L.mcg_set_schedule.argtypes = [C.c_void_p, P(Schedule)]
         L.mcg_get_occ_location.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), C.c_int64]
         L.mcg_set_occ_location.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), C.c_int64]
+        L.mcg_set_site_asym.argtypes = [C.c_void_p, C.c_int64, P(C.c_int8), C.c_int64]
         _lib = L
     return _lib
 
@@ -148,10 +159,14 @@ class Run:
 
     def __init__(self, chains, mode=MODE_EXACT, lattice=LATTICE_SQUARE, J1=0.1, J2=0.0,
                  kb=KB, use_nlist=True, sample_period=1, options=0, eci=None,
-                 sample_time_period=0.0):
+                 sample_time_period=0.0, occupant=None):
         """chains: sequence of dicts with shape, temperature, exchange_potential, seed.
         FCC (lattice=LATTICE_FCC, mode CHECKERBOARD): shape in conventional cells and
-        eci = (V0, V1, V2, V3, V4, Vt)."""
+        eci = (V0, V1, V2, V3, V4, Vt).  OCCUPANT: occupant = dict(asym_species=
+        [[species of occupant 0, occupant 1], ...] per unit (one entry for a
+        unit with one occupant), n_species, canonical (bool), swaps = [(cand_a,
+        cand_b), ...], exchange_potential = per chain a list of n_species values
+        (optional), sites_per_unitcell (optional))."""
         L = lib()
         self.n_chains = len(chains)
         arr = (ChainDesc * self.n_chains)()
@@ -171,8 +186,33 @@ class Run:
                 raise ValueError("the FCC cluster expansion needs 6 ECIs")
             self.eci = tuple(float(v) for v in eci)
             eci_arr = (C.c_double * 6)(*self.eci)
+        occ_ptr = None
+        self._occ_keep = None
+        self.occupant = None
+        if mode == MODE_OCCUPANT:
+            if occupant is None:
+                raise ValueError("OCCUPANT mode needs the occupant system")
+            spec = occupant["asym_species"]
+            na = len(spec)
+            osz = (C.c_int64 * na)(*[len(x) for x in spec])
+            sp = (C.c_int64 * (2 * na))(*[v for x in spec for v in (list(x) + [0])[:2]])
+            sw = [v for pr in occupant["swaps"] for v in pr]
+            swa = (C.c_int64 * len(sw))(*sw)
+            ns = int(occupant["n_species"])
+            mu = occupant.get("exchange_potential")
+            mua = None
+            if mu is not None:
+                flat = [float(v) for row in mu for v in row]
+                if len(flat) != self.n_chains * ns:
+                    raise ValueError("exchange_potential: n_chains x n_species values")
+                mua = (C.c_double * len(flat))(*flat)
+            od = OccupantDesc(na, osz, sp, ns, int(bool(occupant.get("canonical", False))),
+                              len(sw) // 2, swa, mua, int(occupant.get("sites_per_unitcell", 1)))
+            self._occ_keep = (osz, sp, swa, mua, od)
+            occ_ptr = C.pointer(od)
+            self.occupant = dict(occupant)
         d = Desc(mode, lattice, int(use_nlist), J1, J2, kb, sample_period, self.n_chains, arr,
-                 options, eci_arr, sample_time_period)
+                 options, eci_arr, sample_time_period, occ_ptr)
         self._arr, self._eci_arr = arr, eci_arr
         h = C.c_void_p()
         _check(L.mcg_create(C.byref(d), C.byref(h)))
@@ -210,9 +250,27 @@ class Run:
         _check(lib().mcg_get_occupation(self.h, c, o.ctypes.data_as(C.POINTER(C.c_int8)), o.size))
         return o
 
+    def set_site_asym(self, c, asym):
+        """OCCUPANT: the asymmetric unit of every site (resets the occupation to
+        occupant 0 and the lists)."""
+        a = np.ascontiguousarray(asym, dtype=np.int8)
+        _check(lib().mcg_set_site_asym(self.h, c, a.ctypes.data_as(C.POINTER(C.c_int8)), a.size))
+        self._units = getattr(self, "_units", {})
+        self._units[c] = a.copy()
+
+    def n_mutating(self, c):
+        """OCCUPANT: sites whose unit allows two occupants."""
+        units = getattr(self, "_units", {}).get(c)
+        if units is None:
+            units = np.zeros(self.n_sites(c), dtype=np.int8)
+        spec = self.occupant["asym_species"]
+        return int(sum(1 for u in units if len(spec[u]) > 1))
+
     def occ_location(self, c):
-        """CANONICAL: the candidate lists (species-0 sites in list order, then species 1)."""
-        v = np.zeros(self.n_sites(c), dtype=np.int64)
+        """CANONICAL: the candidate lists (species-0 sites in list order, then species 1).
+        OCCUPANT: the candidate lists in candidate order, as site indices."""
+        n = self.n_mutating(c) if self.mode == MODE_OCCUPANT else self.n_sites(c)
+        v = np.zeros(n, dtype=np.int64)
         _check(lib().mcg_get_occ_location(self.h, c, v.ctypes.data_as(C.POINTER(C.c_int64)),
                                           v.size))
         return v
@@ -223,7 +281,7 @@ class Run:
                                           v.size))
 
     def rng_dump(self, c):
-        n = 313 if self.mode in (MODE_EXACT, MODE_CANONICAL, MODE_NFOLD) else 4
+        n = 313 if self.mode in (MODE_EXACT, MODE_CANONICAL, MODE_NFOLD, MODE_OCCUPANT) else 4
         w = np.zeros(n, dtype=np.uint64)
         _check(lib().mcg_rng_dump(self.h, c, w.ctypes.data_as(C.POINTER(C.c_uint64)), n))
         return w
@@ -240,7 +298,8 @@ class Run:
         _check(lib().mcg_run(self.h, int(n_passes)))
 
     def run_steps(self, n_steps):
-        """EXACT: exactly n_steps more attempts per active chain (may end inside a pass)."""
+        """EXACT, OCCUPANT: exactly n_steps more attempts per active chain (may end
+        inside a pass)."""
         _check(lib().mcg_run_steps(self.h, n_steps))
 
     def position(self, c):
@@ -258,6 +317,7 @@ class Run:
         d.exchange_potential = chain.get("exchange_potential", 0.0)
         d.seed = chain.get("seed", 0)
         _check(lib().mcg_assign_chain(self.h, c, C.byref(d)))
+        getattr(self, "_units", {}).pop(c, None)
         self.shapes[c] = tuple(shp)
         self.chains[c] = dict(chain)
 

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <limits>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -47,6 +48,8 @@ constexpr int64_t LDS_LIMIT = 160 * 1024;     // gfx950 LDS per workgroup
 constexpr int64_t EXACT_LDS_MAX = 144 * 1024;  // lattice + fold scratch (+ ~10 KB static)
 constexpr int64_t CHECKER_LDS_MAX = 128 * 1024;
 constexpr int64_t FCC_LDS_MAX = 142 * 1024;  // lattice; tables and lists take 17 KB static
+constexpr int64_t OCC_LDS_MAX = 136 * 1024;  // lattice + lists (+ ~23 KB static)
+constexpr int64_t OCC_MAX_SITES = 40000;
 constexpr int CHECKER_BAND_MAX_L0 = 8192;     // longest line of the 2D band path
 
 // libstdc++ canonical double of a word compared to prob: the first word whose
@@ -310,9 +313,25 @@ struct mcg_handle {
   int nfold_lds = 0;
   double sample_time_period = 0.0;
   bool nfold() const { return mode == MCG_MODE_NFOLD; }
+  // OCCUPANT: the system (units, candidates, swap types, exchange potentials
+  // per chain and species), per-chain parameters and class tables, and each
+  // chain's site units (the lists share d_list)
+  OccParams *d_op = nullptr;
+  OccTables *d_ot = nullptr;
+  std::vector<OccParams> op;
+  int occ_lds = 0;
+  int occ_n_asym = 0, occ_n_species = 0, occ_canonical = 0;
+  int64_t occ_spu = 1;
+  int occ_size[OCC_MAX_ASYM] = {};
+  int occ_species[OCC_MAX_ASYM][2] = {};
+  std::vector<int32_t> occ_cand_asym, occ_cand_occ, occ_swap_a, occ_swap_b;
+  std::vector<double> occ_mu;  // n_chains x n_species
+  std::vector<std::vector<int8_t>> site_asym;
+  bool occupant() const { return mode == MCG_MODE_OCCUPANT; }
   // one sequential chain per wavefront, MT19937-64 state, native lattice layout
   bool sequential() const {
-    return mode == MCG_MODE_EXACT || mode == MCG_MODE_CANONICAL || mode == MCG_MODE_NFOLD;
+    return mode == MCG_MODE_EXACT || mode == MCG_MODE_CANONICAL || mode == MCG_MODE_NFOLD ||
+           mode == MCG_MODE_OCCUPANT;
   }
 
   // CHECKERBOARD
@@ -376,7 +395,7 @@ struct mcg_handle {
     if (stream2) (void)hipStreamSynchronize(stream2);
     void *ptrs[] = {d_lat, d_active, d_series, d_stats, d_eq, d_xp, d_xs, d_cp, d_cs, d_tab,
                     d_work, d_ftab, d_tri, d_tri_lines, d_band, d_pair, d_kp, d_list, d_np,
-                    d_mask_lds, d_mask_even, d_ring_list, d_batch};
+                    d_mask_lds, d_mask_even, d_ring_list, d_batch, d_op, d_ot};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -390,6 +409,8 @@ struct mcg_handle {
   int64_t sites(int64_t c) const {
     return (fcc() ? 4 : 1) * chains[c].shape[0] * chains[c].shape[1] * chains[c].shape[2];
   }
+  // attempts per pass: every site, or in OCCUPANT the mutating sites
+  int64_t per_pass(int64_t c) const { return occupant() ? (int64_t)op[c].n_mut : sites(c); }
 };
 
 namespace {
@@ -621,6 +642,259 @@ int canonical_init_lists(mcg_handle *h, int64_t c, std::vector<int8_t> const &oc
                     hipMemcpyHostToDevice));
   fill_canonical_params(h, (int)c, nA, h->kp[c]);
   HIP_TRY(copy_sync(h, h->d_kp + c, &h->kp[c], sizeof(CanonParams), hipMemcpyHostToDevice));
+  return MCG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// OCCUPANT: the system's units and candidates, per-chain list segments and
+// class tables, and the conversion between the candidate-order lists of the
+// C ABI (OccLocation's m_loc) and the device's segments.
+// ---------------------------------------------------------------------------
+
+// Geometry, list segments (units in order, sized by their mutating sites),
+// candidates and swap types of one chain.
+void fill_occupant_params(mcg_handle *h, int c, OccParams &P) {
+  auto const &ch = h->chains[c];
+  std::memset(&P, 0, sizeof(P));
+  P.rows = (int32_t)ch.shape[0];
+  P.cols = (int32_t)ch.shape[1];
+  P.n = (int32_t)(ch.shape[0] * ch.shape[1]);
+  P.canonical = h->occ_canonical;
+  P.n_asym = h->occ_n_asym;
+  P.n_cand = (int32_t)h->occ_cand_asym.size();
+  P.n_swaps = (int32_t)h->occ_swap_a.size();
+  P.window = (h->options & MCG_OPT_EXACT_SCALAR_WALK) ? 1 : 64;
+  P.div_magic = ((1ULL << 40) + (uint64_t)P.rows - 1) / (uint64_t)P.rows;
+  auto const &ua = h->site_asym[c];
+  int32_t off = 0;
+  for (int b = 0; b < h->occ_n_asym; ++b) {
+    int32_t size = 0;
+    if (h->occ_size[b] == 2)
+      for (int8_t u : ua) size += u == b ? 1 : 0;
+    P.asym_off[b] = off;
+    P.asym_size[b] = size;
+    off += size;
+  }
+  P.n_mut = off;
+  for (int i = 0; i < P.n_cand; ++i) {
+    P.cand_asym[i] = h->occ_cand_asym[i];
+    P.cand_occ[i] = h->occ_cand_occ[i];
+  }
+  for (int i = 0; i < P.n_swaps; ++i) {
+    P.swap_a[i] = h->occ_swap_a[i];
+    P.swap_b[i] = h->occ_swap_b[i];
+  }
+  P.lat_off = h->lat_off[c];
+}
+
+// Class tables of one chain: per swap type and neighbour sums, the potential
+// delta in the order of its definition (per transformed site,
+// dE = (dE + (-J ((2 o' - 1) - (2 o - 1)) * sum)) - (mu[to] - mu[from]);
+// the single-site term as model.hh:305-314 writes it), fl(dE / n_unitcells)
+// (occupation_metropolis.hh:141) and the acceptance threshold of
+// metropolis.hh:25-35.
+void fill_occupant_tables(mcg_handle *h, int c, OccTables &T) {
+  std::memset(&T, 0, sizeof(T));
+  auto const &ch = h->chains[c];
+  const double beta = 1.0 / (h->kb * ch.temperature);  // occupation_metropolis.hh:108-109
+  const double n_uc = static_cast<double>(h->sites(c) / h->occ_spu);
+  const double *mu = h->occ_mu.data() + (size_t)c * h->occ_n_species;
+  const int nb_max = h->occ_canonical ? 5 : 1;
+  for (size_t i = 0; i < h->occ_swap_a.size(); ++i) {
+    const int ca = h->occ_swap_a[i], cb = h->occ_swap_b[i];
+    const int ua = h->occ_cand_asym[ca], oa = h->occ_cand_occ[ca];
+    const int ub = h->occ_cand_asym[cb], ob = h->occ_cand_occ[cb];
+    const int from_a = h->occ_species[ua][oa], to_a = h->occ_species[ua][1 - oa];
+    const int from_b = h->occ_species[ub][ob], to_b = h->occ_species[ub][1 - ob];
+    for (int ia = 0; ia < 5; ++ia)
+      for (int ib = 0; ib < nb_max; ++ib) {
+        const int Sa = 2 * ia - 4, Sb = 2 * ib - 4;
+        double dE = 0.0;
+        dE = (dE + -h->J1 * ((2 * (1 - oa) - 1) - (2 * oa - 1)) * Sa) - (mu[to_a] - mu[from_a]);
+        if (h->occ_canonical)
+          dE = (dE + -h->J1 * ((2 * (1 - ob) - 1) - (2 * ob - 1)) * Sb) - (mu[to_b] - mu[from_b]);
+        const int k = (int)i * 25 + ia * 5 + ib;
+        T.dq[k] = dE / n_uc;
+        if (dE < 0.0) {
+          T.flags[k] = 1;
+          continue;
+        }
+        uint64_t thr = 0;
+        if (threshold_for(std::exp(-dE * beta), &thr))
+          T.thr[k] = thr;
+        else
+          T.flags[k] = 2;
+      }
+  }
+}
+
+// The device segments of candidate-order lists (each list's entries in
+// order), and back.
+void occ_lists_to_segments(OccParams const &P, std::vector<std::vector<int32_t>> const &lists,
+                           std::vector<int32_t> &seg) {
+  seg.assign(P.n_mut, 0);
+  for (int ci = 0; ci < P.n_cand; ++ci) {
+    const int b = P.cand_asym[ci], o = P.cand_occ[ci];
+    for (size_t j = 0; j < lists[ci].size(); ++j)
+      seg[o ? P.asym_off[b] + P.asym_size[b] - 1 - (int32_t)j : P.asym_off[b] + (int32_t)j] =
+          lists[ci][j];
+  }
+}
+
+void occ_segments_to_lists(OccParams const &P, const int32_t *cnt0,
+                           std::vector<int32_t> const &seg,
+                           std::vector<std::vector<int32_t>> &lists) {
+  lists.assign(P.n_cand, {});
+  for (int ci = 0; ci < P.n_cand; ++ci) {
+    const int b = P.cand_asym[ci], o = P.cand_occ[ci];
+    const int32_t cnt = o ? P.asym_size[b] - cnt0[b] : cnt0[b];
+    for (int32_t j = 0; j < cnt; ++j)
+      lists[ci].push_back(seg[o ? P.asym_off[b] + P.asym_size[b] - 1 - j : P.asym_off[b] + j]);
+  }
+}
+
+// candidate index of (unit, occupant), -1 for units with one occupant
+int occ_cand_of(mcg_handle const *h, int b, int o) {
+  for (size_t i = 0; i < h->occ_cand_asym.size(); ++i)
+    if (h->occ_cand_asym[i] == b && h->occ_cand_occ[i] == o) return (int)i;
+  return -1;
+}
+
+// Upload candidate-order lists and the matching occupant-0 counts of one chain.
+int occupant_upload_lists(mcg_handle *h, int64_t c, std::vector<std::vector<int32_t>> const &lists) {
+  OccParams const &P = h->op[c];
+  std::vector<int32_t> seg;
+  occ_lists_to_segments(P, lists, seg);
+  if (!seg.empty())
+    HIP_TRY(copy_sync(h, h->d_list + h->lat_off[c], seg.data(), sizeof(int32_t) * seg.size(),
+                      hipMemcpyHostToDevice));
+  ExactState st;
+  HIP_TRY(copy_sync(h, &st, h->d_xs + c, sizeof(st), hipMemcpyDeviceToHost));
+  for (int b = 0; b < OCC_MAX_ASYM; ++b) st.occ_cnt0[b] = 0;
+  for (int ci = 0; ci < P.n_cand; ++ci)
+    if (P.cand_occ[ci] == 0) st.occ_cnt0[P.cand_asym[ci]] = (int32_t)lists[ci].size();
+  HIP_TRY(copy_sync(h, h->d_xs + c, &st, sizeof(st), hipMemcpyHostToDevice));
+  return MCG_OK;
+}
+
+// OccLocation::initialize (OccLocation.cc:39-118) for one chain: every
+// mutating site, in site order, at the back of its candidate's list.
+int occupant_init_lists(mcg_handle *h, int64_t c, std::vector<int8_t> const &occ) {
+  std::vector<std::vector<int32_t>> lists(h->op[c].n_cand);
+  auto const &ua = h->site_asym[c];
+  for (int64_t l = 0; l < (int64_t)occ.size(); ++l) {
+    const int ci = occ_cand_of(h, ua[l], occ[l]);
+    if (ci >= 0) lists[ci].push_back((int32_t)l);
+  }
+  return occupant_upload_lists(h, c, lists);
+}
+
+// Parameters, tables, state and lists of one chain (occupation all occupant 0).
+int occupant_setup_chain(mcg_handle *h, int c, bool fresh_state) {
+  OccParams P;
+  fill_occupant_params(h, c, P);
+  if (occ_lds_bytes(P) > h->occ_lds)
+    return fail(MCG_ERR_CAPACITY, "shape needs more LDS than the handle was created with");
+  h->op[c] = P;
+  HIP_TRY(copy_sync(h, h->d_op + c, &P, sizeof(P), hipMemcpyHostToDevice));
+  auto T = std::make_unique<OccTables>();
+  fill_occupant_tables(h, c, *T);
+  HIP_TRY(copy_sync(h, h->d_ot + c, T.get(), sizeof(OccTables), hipMemcpyHostToDevice));
+  if (fresh_state) {
+    ExactState S;
+    std::memset(&S, 0, sizeof(S));
+    int pos = 0;
+    mt_seed(h->chains[c].seed, S.mt, &pos);
+    S.pos = pos;
+    HIP_TRY(copy_sync(h, h->d_xs + c, &S, sizeof(S), hipMemcpyHostToDevice));
+  }
+  return MCG_OK;
+}
+
+int build_occupant(mcg_handle *h) {
+  h->op.resize(h->n_chains);
+  h->site_asym.resize(h->n_chains);
+  int max_lds = 0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    h->site_asym[c].assign(h->sites(c), 0);
+    OccParams P;
+    fill_occupant_params(h, c, P);
+    // room for every site in a list, whatever the units
+    P.n_mut = P.n;
+    max_lds = std::max(max_lds, occ_lds_bytes(P));
+  }
+  if (max_lds > OCC_LDS_MAX)
+    return fail(MCG_ERR_ARG, "OCCUPANT mode keeps a chain's lattice and lists in LDS: "
+                             "lattice too large");
+  h->occ_lds = max_lds;
+  HIP_TRY(occupant_set_max_lds((int)OCC_LDS_MAX));
+  HIP_TRY(hipMalloc(&h->d_op, sizeof(OccParams) * h->n_chains));
+  HIP_TRY(hipMalloc(&h->d_ot, sizeof(OccTables) * h->n_chains));
+  HIP_TRY(hipMalloc(&h->d_xs, sizeof(ExactState) * h->n_chains));
+  HIP_TRY(hipMalloc(&h->d_list, sizeof(int32_t) * (size_t)h->lat_bytes));
+  for (int c = 0; c < h->n_chains; ++c) {
+    const int rc = occupant_setup_chain(h, c, true);
+    if (rc != MCG_OK) return rc;
+  }
+  return MCG_OK;
+}
+
+// Check and copy the OCCUPANT system of a descriptor.
+int occupant_configure(mcg_handle *h, const mcg_desc *desc) {
+  const mcg_occupant_desc *o = desc->occupant;
+  if (!o) return fail(MCG_ERR_ARG, "OCCUPANT mode needs mcg_desc.occupant");
+  if (o->n_asym < 1 || o->n_asym > OCC_MAX_ASYM || !o->occ_size || !o->species)
+    return fail(MCG_ERR_ARG, "OCCUPANT: 1..8 asymmetric units with occ_size and species");
+  if (o->n_species < 1 || o->n_species > 8)
+    return fail(MCG_ERR_ARG, "OCCUPANT: 1..8 species");
+  if (o->n_swaps < 1 || o->n_swaps > OCC_MAX_SWAPS || !o->swaps)
+    return fail(MCG_ERR_ARG, "OCCUPANT: 1..16 swap types");
+  if (o->sites_per_unitcell < 1) return fail(MCG_ERR_ARG, "OCCUPANT: sites_per_unitcell < 1");
+  h->occ_n_asym = (int)o->n_asym;
+  h->occ_n_species = (int)o->n_species;
+  h->occ_canonical = o->canonical ? 1 : 0;
+  h->occ_spu = o->sites_per_unitcell;
+  for (int b = 0; b < h->occ_n_asym; ++b) {
+    const int64_t k = o->occ_size[b];
+    if (k != 1 && k != 2) return fail(MCG_ERR_ARG, "OCCUPANT: a unit allows 1 or 2 occupants");
+    h->occ_size[b] = (int)k;
+    for (int j = 0; j < k; ++j) {
+      const int64_t sp = o->species[2 * b + j];
+      if (sp < 0 || sp >= o->n_species) return fail(MCG_ERR_ARG, "OCCUPANT: species out of range");
+      h->occ_species[b][j] = (int)sp;
+    }
+    if (k == 2 && h->occ_species[b][0] == h->occ_species[b][1])
+      return fail(MCG_ERR_ARG, "OCCUPANT: a unit's two occupants must be different species");
+    if (k == 2)
+      for (int j = 0; j < 2; ++j) {  // OccCandidateList order (OccCandidate.cc:31-52)
+        h->occ_cand_asym.push_back(b);
+        h->occ_cand_occ.push_back(j);
+      }
+  }
+  const int nc = (int)h->occ_cand_asym.size();
+  auto species_of = [&](int ci) { return h->occ_species[h->occ_cand_asym[ci]][h->occ_cand_occ[ci]]; };
+  auto other_of = [&](int ci) { return h->occ_species[h->occ_cand_asym[ci]][1 - h->occ_cand_occ[ci]]; };
+  for (int64_t i = 0; i < o->n_swaps; ++i) {
+    const int64_t a = o->swaps[2 * i], b = o->swaps[2 * i + 1];
+    if (a < 0 || a >= nc || b < 0 || b >= nc)
+      return fail(MCG_ERR_ARG, "OCCUPANT: swap candidate out of range");
+    const bool ok = h->occ_canonical
+                        // allowed_canonical_swap (OccCandidate.cc:90-97)
+                        ? (species_of((int)a) != species_of((int)b) &&
+                           other_of((int)a) == species_of((int)b) &&
+                           other_of((int)b) == species_of((int)a))
+                        // allowed_semigrand_canonical_swap (OccCandidate.cc:126-133)
+                        : (h->occ_cand_asym[a] == h->occ_cand_asym[b] && a != b);
+    if (!ok)
+      return fail(MCG_ERR_ARG, h->occ_canonical
+                                   ? "OCCUPANT: not an allowed canonical swap"
+                                   : "OCCUPANT: not an allowed semi-grand canonical swap");
+    h->occ_swap_a.push_back((int32_t)a);
+    h->occ_swap_b.push_back((int32_t)b);
+  }
+  h->occ_mu.assign((size_t)desc->n_chains * o->n_species, 0.0);
+  if (o->exchange_potential)
+    std::copy(o->exchange_potential, o->exchange_potential + h->occ_mu.size(), h->occ_mu.begin());
   return MCG_OK;
 }
 
@@ -955,6 +1229,24 @@ int reset_observables(mcg_handle *h, int64_t c, std::vector<int8_t> const &occ) 
   }
   LatticeGeom g{h->lattice == MCG_LATTICE_SC ? 3 : 2, {ch.shape[0], ch.shape[1], ch.shape[2]}};
   int64_t S1, S2, M;
+  if (h->occupant()) {
+    // spins 2o - 1 of the occupant indices; the potential's per_supercell:
+    // the formation energy, then minus each site's exchange potential in site
+    // order; per unit cell (occupation_metropolis.hh:103-106)
+    std::vector<int8_t> spins(occ.size());
+    for (size_t l = 0; l < occ.size(); ++l) spins[l] = (int8_t)(2 * occ[l] - 1);
+    integer_observables(spins, g, false, S1, S2, M);
+    double E = ising_per_supercell(spins, ch.shape[0], ch.shape[1], h->J1, h->use_nlist);
+    const double *mu = h->occ_mu.data() + (size_t)c * h->occ_n_species;
+    for (size_t l = 0; l < occ.size(); ++l) E -= mu[h->occ_species[h->site_asym[c][l]][occ[l]]];
+    ExactState st;
+    HIP_TRY(copy_sync(h, &st, h->d_xs + c, sizeof(ExactState), hipMemcpyDeviceToHost));
+    st.S = S1;
+    st.M = M;
+    st.e_run = E / static_cast<double>(h->sites(c) / h->occ_spu);
+    HIP_TRY(copy_sync(h, h->d_xs + c, &st, sizeof(ExactState), hipMemcpyHostToDevice));
+    return MCG_OK;
+  }
   integer_observables(occ, g, h->J2 != 0.0, S1, S2, M);
   if (h->sequential()) {
     ExactState st;
@@ -1348,8 +1640,12 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
   if (!desc || !out) return fail(MCG_ERR_ARG, "null argument");
   *out = nullptr;
   if (desc->mode != MCG_MODE_EXACT && desc->mode != MCG_MODE_CHECKERBOARD &&
-      desc->mode != MCG_MODE_CANONICAL && desc->mode != MCG_MODE_NFOLD)
+      desc->mode != MCG_MODE_CANONICAL && desc->mode != MCG_MODE_NFOLD &&
+      desc->mode != MCG_MODE_OCCUPANT)
     return fail(MCG_ERR_ARG, "unknown mode");
+  if (desc->mode == MCG_MODE_OCCUPANT &&
+      (desc->lattice != MCG_LATTICE_SQUARE || desc->J2 != 0.0))
+    return fail(MCG_ERR_ARG, "OCCUPANT mode supports the 2D square nearest-neighbour model only");
   if (desc->mode == MCG_MODE_NFOLD && (desc->lattice != MCG_LATTICE_SQUARE || desc->J2 != 0.0))
     return fail(MCG_ERR_ARG, "NFOLD mode supports the 2D square nearest-neighbour model only");
   if (desc->sample_time_period < 0.0 ||
@@ -1397,6 +1693,13 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
   h->sample_time_period = desc->sample_time_period;
   if (h->fcc())
     for (int i = 0; i < 6; ++i) h->eci[i] = desc->eci[i];
+  if (h->occupant()) {
+    const int orc = occupant_configure(h, desc);
+    if (orc != MCG_OK) {
+      delete h;
+      return orc;
+    }
+  }
   h->n_chains = (int)desc->n_chains;
   h->chains.assign(desc->chains, desc->chains + desc->n_chains);
   for (auto &ch : h->chains) {
@@ -1426,6 +1729,11 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
       return fail(MCG_ERR_ARG, "CANONICAL mode supports at most 65536 sites per chain, "
                                "NFOLD at most 32768");
     }
+    if (h->occupant() && (n > OCC_MAX_SITES || n % h->occ_spu != 0)) {
+      delete h;
+      return fail(MCG_ERR_ARG, "OCCUPANT mode supports at most 40000 sites per chain, a "
+                               "multiple of sites_per_unitcell");
+    }
     if (n > (1LL << 31)) {
       delete h;
       return fail(MCG_ERR_ARG, "too many sites in one chain");
@@ -1445,8 +1753,10 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
     return bail(fail(MCG_ERR_HIP, "hipEventCreate failed"));
   if (hipMalloc(&h->d_lat, (size_t)h->lat_bytes) != hipSuccess)
     return bail(fail(MCG_ERR_CAPACITY, "out of device memory for lattices"));
-  // Ising lattices start all +1 (stored as 1), FCC lattices empty
-  if (fill_sync(h, h->d_lat, h->fcc() ? 0 : 1, (size_t)h->lat_bytes) != hipSuccess)
+  // Ising lattices start all +1 (stored as 1), FCC lattices empty, OCCUPANT
+  // lattices at occupant 0
+  const int fill0 = (h->fcc() || h->occupant()) ? 0 : 1;
+  if (fill_sync(h, h->d_lat, fill0, (size_t)h->lat_bytes) != hipSuccess)
     return bail(fail(MCG_ERR_HIP, "hipMemset failed"));
   h->active.assign(h->n_chains, 1);
   if (hipMalloc(&h->d_active, h->n_chains) != hipSuccess ||
@@ -1461,14 +1771,16 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
   rc = (h->mode == MCG_MODE_EXACT)  ? build_exact(h)
        : h->canonical()              ? build_canonical(h)
        : h->nfold()                  ? build_nfold(h)
+       : h->occupant()               ? build_occupant(h)
        : h->fcc()                    ? build_fcc(h)
                                      : build_checker(h);
   if (rc != MCG_OK) return bail(rc);
   // initial lattices: set observables
   for (int c = 0; c < h->n_chains; ++c) {
-    std::vector<int8_t> occ(h->sites(c), h->fcc() ? 0 : 1);
+    std::vector<int8_t> occ(h->sites(c), (int8_t)fill0);
     rc = reset_observables(h, c, occ);
     if (rc == MCG_OK && h->nfold()) rc = nfold_init_array(h, c, occ);
+    if (rc == MCG_OK && h->occupant()) rc = occupant_init_lists(h, c, occ);
     if (rc != MCG_OK) return bail(rc);
   }
   rc = ensure_series(h, 64);
@@ -1483,7 +1795,12 @@ int mcg_set_occupation(mcg_handle *h, int64_t chain, const int8_t *occ, int64_t 
   if (!valid_chain(h, chain) || !occ) return fail(MCG_ERR_ARG, "bad chain or buffer");
   if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "occupation size mismatch");
   std::vector<int8_t> v(occ, occ + n);
-  if (h->fcc()) {
+  if (h->occupant()) {
+    for (int64_t l = 0; l < n; ++l)
+      if (v[l] < 0 || v[l] >= h->occ_size[h->site_asym[chain][l]])
+        return fail(MCG_ERR_ARG, "OCCUPANT occupation values must be occupant indices of the "
+                                 "site's asymmetric unit");
+  } else if (h->fcc()) {
     for (int8_t s : v)
       if (s != 0 && s != 1) return fail(MCG_ERR_ARG, "FCC occupation values must be 0 or 1");
   } else {
@@ -1499,6 +1816,10 @@ int mcg_set_occupation(mcg_handle *h, int64_t chain, const int8_t *occ, int64_t 
     }
     if (h->nfold()) {
       const int rc = nfold_init_array(h, chain, v);
+      if (rc != MCG_OK) return rc;
+    }
+    if (h->occupant()) {
+      const int rc = occupant_init_lists(h, chain, v);
       if (rc != MCG_OK) return rc;
     }
   } else {
@@ -1592,6 +1913,8 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
   if ((h->canonical() && n > (1LL << 16)) || (h->nfold() && n > (1LL << 15)))
     return fail(MCG_ERR_ARG, "CANONICAL mode supports at most 65536 sites per chain, "
                              "NFOLD at most 32768");
+  if (h->occupant() && n % h->occ_spu != 0)
+    return fail(MCG_ERR_ARG, "OCCUPANT: sites must be a multiple of sites_per_unitcell");
 
   HIP_TRY(hipStreamSynchronize(h->stream));
   const mcg_chain_desc old = h->chains[chain];
@@ -1631,6 +1954,15 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
     ExactState S;
     nfold_reset_state(S, nd.seed);
     HIP_TRY(copy_sync(h, h->d_xs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
+  } else if (h->occupant()) {
+    const std::vector<int8_t> old_units = h->site_asym[chain];
+    h->site_asym[chain].assign(n, 0);
+    const int orc = occupant_setup_chain(h, (int)chain, true);
+    if (orc != MCG_OK) {
+      h->chains[chain] = old;
+      h->site_asym[chain] = old_units;
+      return orc;
+    }
   } else if (h->canonical()) {
     CanonParams P;
     fill_canonical_params(h, (int)chain, 0, P);
@@ -1690,19 +2022,39 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
     const int mrc = checker_update_masks(h);
     if (mrc != MCG_OK) return mrc;
   }
-  const int fill = h->fcc() ? 0 : 1;
+  const int fill = (h->fcc() || h->occupant()) ? 0 : 1;
   HIP_TRY(fill_sync(h, h->d_lat + h->lat_off[chain], fill,
                     (size_t)((h->capacity[chain] + 255) & ~255LL)));
   h->n_samples[chain] = 0;
   h->n_pass[chain] = 0;
   h->n_step[chain] = 0;
   const int rc = reset_observables(h, chain, std::vector<int8_t>(n, (int8_t)fill));
+  if (rc == MCG_OK && h->occupant())
+    return occupant_init_lists(h, chain, std::vector<int8_t>(n, (int8_t)fill));
   if (rc != MCG_OK || !h->nfold()) return rc;
   return nfold_init_array(h, chain, std::vector<int8_t>(n, (int8_t)fill));
 }
 
 int mcg_get_occ_location(mcg_handle *h, int64_t chain, int64_t *loc, int64_t n) {
-  if (!valid_chain(h, chain) || !loc) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (!valid_chain(h, chain) || (!loc && n)) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (h->occupant()) {
+    OccParams const &P = h->op[chain];
+    if (n != P.n_mut) return fail(MCG_ERR_ARG, "list size mismatch (mutating sites)");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    std::vector<int32_t> seg(n);
+    if (n)
+      HIP_TRY(copy_sync(h, seg.data(), h->d_list + h->lat_off[chain], sizeof(int32_t) * n,
+                        hipMemcpyDeviceToHost));
+    ExactState st;
+    HIP_TRY(copy_sync(h, &st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
+    std::vector<std::vector<int32_t>> lists;
+    occ_segments_to_lists(P, st.occ_cnt0, seg, lists);
+    int64_t k = 0;
+    for (auto const &v : lists)
+      for (int32_t l : v) loc[k++] = l;
+    return MCG_OK;
+  }
+  if (!loc) return fail(MCG_ERR_ARG, "bad chain or buffer");
   if (!h->canonical() && !h->nfold())
     return fail(MCG_ERR_STATE, "site lists exist in CANONICAL and NFOLD modes only");
   if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "list size mismatch");
@@ -1715,7 +2067,37 @@ int mcg_get_occ_location(mcg_handle *h, int64_t chain, int64_t *loc, int64_t n) 
 }
 
 int mcg_set_occ_location(mcg_handle *h, int64_t chain, const int64_t *loc, int64_t n) {
-  if (!valid_chain(h, chain) || !loc) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (!valid_chain(h, chain) || (!loc && n)) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (h->occupant()) {
+    OccParams const &P = h->op[chain];
+    if (n != P.n_mut) return fail(MCG_ERR_ARG, "list size mismatch (mutating sites)");
+    const int64_t ns = h->sites(chain);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    std::vector<int8_t> occ(ns);
+    HIP_TRY(copy_sync(h, occ.data(), h->d_lat + h->lat_off[chain], ns, hipMemcpyDeviceToHost));
+    // list sizes from the occupation, then every entry a mutating site of the
+    // list's candidate, each once
+    std::vector<int64_t> size(P.n_cand, 0);
+    for (int64_t l = 0; l < ns; ++l) {
+      const int ci = occ_cand_of(h, h->site_asym[chain][l], occ[l]);
+      if (ci >= 0) ++size[ci];
+    }
+    std::vector<std::vector<int32_t>> lists(P.n_cand);
+    std::vector<uint8_t> seen(ns, 0);
+    int64_t k = 0;
+    for (int ci = 0; ci < P.n_cand; ++ci)
+      for (int64_t j = 0; j < size[ci]; ++j, ++k) {
+        const int64_t l = loc[k];
+        if (l < 0 || l >= ns || seen[l])
+          return fail(MCG_ERR_ARG, "occupant lists must hold every mutating site exactly once");
+        if (occ_cand_of(h, h->site_asym[chain][l], occ[l]) != ci)
+          return fail(MCG_ERR_ARG, "occupant lists do not match the occupation");
+        seen[l] = 1;
+        lists[ci].push_back((int32_t)l);
+      }
+    return occupant_upload_lists(h, chain, lists);
+  }
+  if (!loc) return fail(MCG_ERR_ARG, "bad chain or buffer");
   if (!h->canonical()) return fail(MCG_ERR_STATE, "occupant lists exist in CANONICAL mode only");
   if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "list size mismatch");
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1738,6 +2120,28 @@ int mcg_set_occ_location(mcg_handle *h, int64_t chain, const int64_t *loc, int64
   return MCG_OK;
 }
 
+int mcg_set_site_asym(mcg_handle *h, int64_t chain, const int8_t *asym, int64_t n) {
+  if (!valid_chain(h, chain) || !asym) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (!h->occupant()) return fail(MCG_ERR_STATE, "site units exist in OCCUPANT mode only");
+  if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "unit array size mismatch");
+  for (int64_t l = 0; l < n; ++l)
+    if (asym[l] < 0 || asym[l] >= h->occ_n_asym)
+      return fail(MCG_ERR_ARG, "asymmetric unit index out of range");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const std::vector<int8_t> old_units = h->site_asym[chain];
+  h->site_asym[chain].assign(asym, asym + n);
+  const int rc = occupant_setup_chain(h, (int)chain, false);
+  if (rc != MCG_OK) {
+    h->site_asym[chain] = old_units;
+    return rc;
+  }
+  HIP_TRY(fill_sync(h, h->d_lat + h->lat_off[chain], 0, (size_t)n));
+  std::vector<int8_t> zero(n, 0);
+  const int rc2 = reset_observables(h, chain, zero);
+  if (rc2 != MCG_OK) return rc2;
+  return occupant_init_lists(h, chain, zero);
+}
+
 int mcg_set_active(mcg_handle *h, const uint8_t *active, int64_t n_chains) {
   if (!h || !active || n_chains != h->n_chains) return fail(MCG_ERR_ARG, "bad active mask");
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1749,9 +2153,73 @@ int mcg_set_active(mcg_handle *h, const uint8_t *active, int64_t n_chains) {
   return MCG_OK;
 }
 
+// OCCUPANT runs of n_passes passes plus n_steps steps per active chain.  A
+// chain whose swap types have no events (OccEventProposal.hh:136-139,
+// 290-293) stops there; the run then fails with the reference's message after
+// every chain has gone as far as it can.
+int occupant_run(mcg_handle *h, int64_t n_passes, int64_t n_steps) {
+  h->last_ms = 0.0;
+  h->last_attempts = 0.0;
+  if (n_passes == 0 && n_steps == 0) return MCG_OK;
+  const char *no_events = h->occ_canonical
+                              ? "Error in choose_canonical_swap: No events possible."
+                              : "Error in choose_semigrand_canonical_swap: No events possible.";
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  std::vector<ExactState> xs(h->n_chains);
+  HIP_TRY(copy_sync(h, xs.data(), h->d_xs, sizeof(ExactState) * h->n_chains,
+                    hipMemcpyDeviceToHost));
+  int64_t need = 0;
+  double attempts = 0.0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    if (!h->active[c]) continue;
+    OccParams const &P = h->op[c];
+    // the swap types' event count at the start (OccEventProposal.hh:127-134)
+    int64_t total = 0;
+    for (int i = 0; i < P.n_swaps; ++i) {
+      auto cnt = [&](int ci) -> int64_t {
+        const int b = P.cand_asym[ci];
+        return P.cand_occ[ci] ? P.asym_size[b] - xs[c].occ_cnt0[b] : xs[c].occ_cnt0[b];
+      };
+      total += P.canonical ? cnt(P.swap_a[i]) * cnt(P.swap_b[i]) : cnt(P.swap_a[i]);
+    }
+    if (total == 0) return fail(MCG_ERR_STATE, no_events);
+    const int64_t p0 = h->n_pass[c];
+    const int64_t passes = n_passes + (h->n_step[c] + n_steps) / P.n_mut;
+    need = std::max(need, h->n_samples[c] + (p0 + passes) / h->sample_period -
+                              p0 / h->sample_period);
+    attempts += (double)P.n_mut * (double)n_passes + (double)n_steps;
+  }
+  int rc = ensure_series(h, need);
+  if (rc != MCG_OK) return rc;
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  launch_occupant(h->d_op, h->d_ot, h->d_xs, h->d_lat, h->d_list, h->d_series, h->series_cap,
+                  h->d_active, h->n_chains, n_passes, n_steps, h->sample_period, h->occ_lds,
+                  h->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  h->last_attempts = attempts;
+  HIP_TRY(copy_sync(h, xs.data(), h->d_xs, sizeof(ExactState) * h->n_chains,
+                    hipMemcpyDeviceToHost));
+  bool frozen = false;
+  for (int c = 0; c < h->n_chains; ++c) {
+    if (!h->active[c]) continue;
+    h->n_pass[c] = xs[c].n_pass;
+    h->n_step[c] = xs[c].step;
+    h->n_samples[c] = std::min(xs[c].n_samples, h->series_cap);
+    frozen = frozen || xs[c].frozen;
+  }
+  if (frozen) return fail(MCG_ERR_STATE, no_events);
+  return MCG_OK;
+}
+
 int mcg_run(mcg_handle *h, int64_t n_passes) {
   if (!h) return fail(MCG_ERR_ARG, "null handle");
   if (n_passes < 0) return fail(MCG_ERR_ARG, "n_passes < 0");
+  if (h->occupant()) return occupant_run(h, n_passes, 0);
   h->last_ms = 0.0;
   h->last_attempts = 0.0;
   const bool time_grid = h->nfold() && h->sample_time_period > 0.0;
@@ -1892,9 +2360,11 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
 
 int mcg_run_steps(mcg_handle *h, int64_t n_steps) {
   if (!h) return fail(MCG_ERR_ARG, "null handle");
-  if (h->mode != MCG_MODE_EXACT)
-    return fail(MCG_ERR_STATE, "mcg_run_steps: single steps are run in EXACT mode only");
+  if (h->mode != MCG_MODE_EXACT && !h->occupant())
+    return fail(MCG_ERR_STATE,
+                "mcg_run_steps: single steps are run in EXACT and OCCUPANT modes only");
   if (n_steps < 0) return fail(MCG_ERR_ARG, "n_steps < 0");
+  if (h->occupant()) return occupant_run(h, 0, n_steps);
   h->last_ms = 0.0;
   h->last_attempts = 0.0;
   if (n_steps == 0) return MCG_OK;
@@ -1940,7 +2410,7 @@ int mcg_get_position(mcg_handle *h, int64_t chain, int64_t *n_pass, int64_t *ste
   ExactState st;
   HIP_TRY(copy_sync(h, &st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
   if (n_pass) *n_pass = st.n_pass;
-  if (step) *step = h->mode == MCG_MODE_EXACT ? st.step : 0;
+  if (step) *step = (h->mode == MCG_MODE_EXACT || h->occupant()) ? st.step : 0;
   if (e_run) *e_run = st.e_run;
   return MCG_OK;
 }
@@ -2049,7 +2519,7 @@ static ObsParams obs_params(mcg_handle *h, int64_t chain, int64_t observable) {
   op.fold = ((h->mode == MCG_MODE_EXACT || h->canonical()) && !h->use_nlist) ? 1 : 0;
   op.observable = (int32_t)observable;
   op.fcc = h->fcc() ? 1 : 0;
-  op.pad = 0;
+  op.running = h->occupant() ? 1 : 0;
   op.coef[0] = h->eci[0];
   op.coef[1] = h->eci[1];
   op.coef[2] = h->eci[2] / 6.0;

@@ -47,10 +47,14 @@ struct ExactState {
   int64_t pending_site;  // event selected at the end of a sampled pass, or -1
   double pending_dt;     // its time increment
   int64_t next_k;        // next time-grid sample index (sample at next_k * period)
-  int64_t frozen;        // set when the total rate reached 0 on the time grid
+  int64_t frozen;        // NFOLD: the total rate reached 0 on the time grid; OCCUPANT: the
+                         // swap types ran out of events
   // EXACT: attempts done in the current pass (0 <= step < n; runs of whole
-  // passes keep it, runs of single steps move it)
+  // passes keep it, runs of single steps move it); OCCUPANT: steps done in the
+  // current pass (0 <= step < n_mut)
   int64_t step;
+  // OCCUPANT: sites of each asymmetric unit holding its occupant 0
+  int32_t occ_cnt0[8];
 };
 
 // Every active chain makes n_passes * n + n_steps attempts from where it stands.
@@ -58,6 +62,57 @@ void launch_exact(const ExactParams *P, ExactState *S, int8_t *lat_pool, mcg_sam
                   int64_t series_cap, const uint8_t *active, int n_chains, int64_t n_passes,
                   int64_t n_steps, int64_t sample_period, int lds_bytes, bool lattice_in_lds,
                   bool block_walk, hipStream_t stream);
+
+// ---------------------------------------------------------------------------
+// OCCUPANT mode: occupant events over an OccLocation with several asymmetric
+// units (semi-grand canonical single-site events or canonical pair swaps),
+// one chain per wavefront.
+// ---------------------------------------------------------------------------
+constexpr int OCC_MAX_ASYM = 8;
+constexpr int OCC_MAX_CAND = 2 * OCC_MAX_ASYM;
+constexpr int OCC_MAX_SWAPS = 16;
+// classes: swap type i, neighbour sum of a, and (pairs) of b after a changed,
+// k = i * 25 + ((Sa + 4) / 2) * 5 + (Sb + 4) / 2
+constexpr int OCC_NCLASS = OCC_MAX_SWAPS * 25;
+
+struct OccParams {
+  int32_t rows, cols, n, n_mut;
+  int32_t canonical;  // 1: pair swaps (propose_canonical_event), 0: single sites
+  int32_t n_asym, n_cand, n_swaps;
+  int32_t window;     // stream positions evaluated per block
+  int32_t pad;
+  uint64_t div_magic; // floor(l / rows) = (l * div_magic) >> 40
+  // per unit: segment of the list array (units with one occupant: size 0)
+  int32_t asym_off[OCC_MAX_ASYM], asym_size[OCC_MAX_ASYM];
+  // per candidate (OccCandidateList order): unit and occupant index
+  int32_t cand_asym[OCC_MAX_CAND], cand_occ[OCC_MAX_CAND];
+  // swap types: candidate indices (cand_a, cand_b)
+  int32_t swap_a[OCC_MAX_SWAPS], swap_b[OCC_MAX_SWAPS];
+  int64_t lat_off;    // lattice byte offset; the lists start at entry lat_off of the list pool
+};
+
+// per chain: the class thresholds (first word whose canonical double reaches
+// exp(-beta dE)), fl(dE / n_unitcells), and flags (bit 0: dE < 0, accepted
+// without a draw; bit 1: every word accepts)
+struct OccTables {
+  uint64_t thr[OCC_NCLASS];
+  double dq[OCC_NCLASS];
+  uint8_t flags[OCC_NCLASS];
+};
+
+// LDS bytes of one chain: lattice and the list array.
+inline int occ_lds_bytes(const OccParams &P) {
+  return ((P.n + 15) & ~15) + ((2 * P.n_mut + 15) & ~15);
+}
+
+// Every active chain makes n_passes * n_mut + n_steps steps from where it
+// stands (fewer if its swap types run out of events: ExactState::frozen).
+void launch_occupant(const OccParams *P, const OccTables *T, ExactState *S, int8_t *lat_pool,
+                     int32_t *list_pool, mcg_sample *series, int64_t series_cap,
+                     const uint8_t *active, int n_chains, int64_t n_passes, int64_t n_steps,
+                     int64_t sample_period, int lds_bytes, hipStream_t stream);
+// Dynamic LDS limit of the occupant kernel (its static LDS is ~23 KB).
+hipError_t occupant_set_max_lds(int bytes);
 
 // ---------------------------------------------------------------------------
 // CANONICAL mode: pair swaps at fixed composition, one chain per wavefront.
@@ -249,7 +304,7 @@ struct ObsParams {
   int32_t fold;  // use e_fold as E_f
   int32_t observable;
   int32_t fcc;   // FCC cluster expansion: E_f from the counts and coef
-  int32_t pad;
+  int32_t running;  // OCCUPANT: the potential energy is the record's running e_run
   double coef[6];  // FCC: V0, V1, V2/6, V3/3, V4/12, Vt/8
 };
 

@@ -77,6 +77,9 @@ __device__ __forceinline__ double observable_of(const mcg_sample &r, const ObsPa
     if (op.negJ2 != 0.0) ef = __dadd_rn(ef, __dmul_rn((double)r.S2, op.negJ2));
   }
   if (op.observable == MCG_OBS_FORMATION_ENERGY) return __ddiv_rn(ef, N);
+  // OCCUPANT: the potential has an exchange potential per species, carried in
+  // the running energy per unit cell
+  if (op.running && op.observable == MCG_OBS_POTENTIAL_ENERGY) return r.e_run;
   const double Nx = __ddiv_rn((double)(op.n_sites + r.M), 2.0);  // model.hh:415
   if (op.observable == MCG_OBS_PARAM_COMPOSITION) return __ddiv_rn(Nx, N);
   // basic_semigrand_canonical.hh:165-174

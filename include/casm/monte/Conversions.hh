@@ -1,10 +1,14 @@
 // Stand-in for the part of libcasm-monte's Conversions (Conversions.hh:43-131)
 // that the occupant-location classes read, for the lattice models this library
-// runs: one asymmetric unit whose sites all allow the same ordered occupants,
-// species index == occupant index.  The reference builds Conversions from an
+// runs: every site belongs to one asymmetric unit, and each unit allows an
+// ordered list of occupants (species names).  Species are numbered in the order
+// they first appear over the units; a site's occupant index is the position of
+// its species in its unit's list.  The reference builds Conversions from an
 // xtal::BasicStructure prim and a supercell matrix (CASMcode_crystallography,
-// not a dependency of this library); the member names and meanings used by
-// OccCandidateList, OccLocation and the event proposal functions are kept.
+// not a dependency of this library), where the units come from the prim's
+// basis under symmetry; here the caller gives them.  The member names and
+// meanings used by OccCandidateList, OccLocation and the event proposal
+// functions are kept.
 #ifndef CASM_MI355X_monte_Conversions
 #define CASM_MI355X_monte_Conversions
 
@@ -15,22 +19,46 @@ namespace monte {
 
 class Conversions {
  public:
-  /// n_sites sites, each allowing `occupants` (species names) in this order
+  /// n_sites sites of one asymmetric unit, each allowing `occupants` (species
+  /// names) in this order
   Conversions(Index n_sites, std::vector<std::string> occupants)
-      : m_n_sites(n_sites), m_species(std::move(occupants)) {
-    if (n_sites < 1 || m_species.empty())
-      throw std::runtime_error("Error in Conversions: no sites or no occupants");
+      : Conversions(std::vector<Index>(n_sites < 0 ? 0 : n_sites, 0), {std::move(occupants)}) {}
+
+  /// site l in unit l_to_asym[l]; unit b allows asym_occupants[b] in this order
+  Conversions(std::vector<Index> l_to_asym, std::vector<std::vector<std::string>> asym_occupants)
+      : m_l_to_asym(std::move(l_to_asym)), m_asym_occ(asym_occupants.size()) {
+    if (m_l_to_asym.empty() || asym_occupants.empty())
+      throw std::runtime_error("Error in Conversions: no sites or no asymmetric units");
+    for (size_t b = 0; b < asym_occupants.size(); ++b) {
+      if (asym_occupants[b].empty())
+        throw std::runtime_error("Error in Conversions: a unit allows no occupant");
+      for (auto const &name : asym_occupants[b]) {
+        Index s = species_index(name);
+        if (s == species_size()) m_species.push_back(name);
+        for (Index x : m_asym_occ[b])
+          if (x == s) throw std::runtime_error("Error in Conversions: repeated occupant");
+        m_asym_occ[b].push_back(s);
+      }
+    }
+    for (Index b : m_l_to_asym)
+      if (b < 0 || b >= asym_size())
+        throw std::runtime_error("Error in Conversions: asymmetric unit out of range");
   }
 
-  Index l_size() const { return m_n_sites; }
-  Index l_to_asym(Index l) const { return 0; }
-  Index asym_size() const { return 1; }
+  Index l_size() const { return static_cast<Index>(m_l_to_asym.size()); }
+  Index l_to_asym(Index l) const { return m_l_to_asym[l]; }
+  Index asym_size() const { return static_cast<Index>(m_asym_occ.size()); }
 
-  Index occ_size(Index asym) const { return static_cast<Index>(m_species.size()); }
-  Index species_index(Index asym, Index occ_index) const { return occ_index; }
-  Index occ_index(Index asym, Index species_index) const { return species_index; }
+  Index occ_size(Index asym) const { return static_cast<Index>(m_asym_occ[asym].size()); }
+  Index species_index(Index asym, Index occ_index) const { return m_asym_occ[asym][occ_index]; }
+  /// occ_size(asym) if the species is not allowed on the unit
+  Index occ_index(Index asym, Index species_index) const {
+    for (Index o = 0; o < occ_size(asym); ++o)
+      if (m_asym_occ[asym][o] == species_index) return o;
+    return occ_size(asym);
+  }
   bool species_allowed(Index asym, Index species_index) const {
-    return asym == 0 && species_index >= 0 && species_index < species_size();
+    return asym >= 0 && asym < asym_size() && occ_index(asym, species_index) < occ_size(asym);
   }
   Index species_size() const { return static_cast<Index>(m_species.size()); }
   Index species_index(std::string species_name) const {
@@ -41,7 +69,8 @@ class Conversions {
   std::string const &species_name(Index species_index) const { return m_species.at(species_index); }
 
  private:
-  Index m_n_sites;
+  std::vector<Index> m_l_to_asym;
+  std::vector<std::vector<Index>> m_asym_occ;  // species index of each occupant, per unit
   std::vector<std::string> m_species;
 };
 

@@ -128,15 +128,17 @@ class OccLocation {
   /// The candidate lists (Mol ids per candidate index, in list order)
   std::vector<std::vector<Index>> const &locations() const { return m_loc; }
 
-  /// Replace the candidate lists (same sizes, each Mol in the list of its
-  /// current species) and the Mol positions they imply
+  /// Replace the candidate lists (every Mol once, in the list of its current
+  /// species) and the Mol positions they imply
   void set_locations(std::vector<std::vector<Index>> const &loc) {
     if (loc.size() != m_loc.size())
       throw std::runtime_error("OccLocation::set_locations: wrong number of lists");
     std::vector<char> seen(m_mol.size(), 0);
+    size_t total = 0;
+    for (auto const &v : loc) total += v.size();
+    if (total != m_mol.size())
+      throw std::runtime_error("OccLocation::set_locations: every Mol must be listed once");
     for (size_t c = 0; c < loc.size(); ++c) {
-      if (loc[c].size() != m_loc[c].size())
-        throw std::runtime_error("OccLocation::set_locations: list size changed");
       for (Index id : loc[c]) {
         if (id < 0 || id >= (Index)m_mol.size() || seen[id] ||
             m_candidate_list.index(m_mol[id].asym, m_mol[id].species_index) != (Index)c)

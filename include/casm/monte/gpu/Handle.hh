@@ -55,7 +55,8 @@ class Handle {
     check(mcg_get_occupation(m_h, chain, occ.data(), n));
     return occ;
   }
-  /// CANONICAL: the candidate lists, species-0 sites then species-1 sites
+  /// CANONICAL: the candidate lists, species-0 sites then species-1 sites;
+  /// OCCUPANT: the candidate lists in candidate order (n = mutating sites)
   std::vector<int64_t> occ_location(Index chain, Index n) const {
     std::vector<int64_t> loc(n);
     check(mcg_get_occ_location(m_h, chain, loc.data(), n));
@@ -74,8 +75,15 @@ class Handle {
     set_engine_words(e, w);
   }
   void run(CountType n_passes) { check(mcg_run(m_h, n_passes)); }
-  /// EXACT: exactly n_steps more attempts (may end inside a pass).
+  /// EXACT, OCCUPANT: exactly n_steps more attempts (may end inside a pass).
   void run_steps(CountType n_steps) { check(mcg_run_steps(m_h, n_steps)); }
+  /// As run / run_steps, returning the C ABI's code instead of throwing
+  int try_run(CountType n_passes) { return mcg_run(m_h, n_passes); }
+  int try_run_steps(CountType n_steps) { return mcg_run_steps(m_h, n_steps); }
+  /// OCCUPANT: the asymmetric unit of every site (resets occupation and lists)
+  void set_site_asym(Index chain, std::vector<int8_t> const &asym) {
+    check(mcg_set_site_asym(m_h, chain, asym.data(), (int64_t)asym.size()));
+  }
   /// Pass count, steps into the current pass and FP64 running energy per unit cell.
   void position(Index chain, int64_t &n_pass, int64_t &step, double &e_run) const {
     check(mcg_get_position(m_h, chain, &n_pass, &step, &e_run));

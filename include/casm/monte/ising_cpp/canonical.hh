@@ -29,13 +29,30 @@ inline IsingConfiguration spins_of_occupants(IsingConfiguration const &c) {
   return s;
 }
 
-/// Formation energy of the square-lattice Ising model over occupant indices
+/// Formation energy of the square-lattice Ising model over occupant indices,
+/// optionally minus an exchange potential per species (semi-grand canonical
+/// runs over an OccLocation with several asymmetric units): per supercell,
+/// E_f - mu[species(l)] for every site l in site order; the delta of an event,
+/// per transformed site in event order with the earlier ones applied,
+///   dE = (dE + (-J ((2 o' - 1) - s_l) sum_l)) - (mu[to] - mu[from]),
+/// which without exchange potentials is the canonical delta of
+/// model.hh:354-379 on the spins 2o - 1.
 class IsingOccupantPotential {
  public:
   explicit IsingOccupantPotential(double _J, bool _use_nlist = true)
       : J(_J), use_nlist(_use_nlist) {}
+  /// with exchange potentials, one per species of `_convert`
+  IsingOccupantPotential(double _J, Conversions const &_convert,
+                         std::vector<double> _exchange_potential, bool _use_nlist = true)
+      : J(_J), use_nlist(_use_nlist), exchange_potential(std::move(_exchange_potential)),
+        m_convert(&_convert) {
+    if ((Index)exchange_potential.size() != _convert.species_size())
+      throw std::runtime_error("IsingOccupantPotential: one exchange potential per species");
+  }
   double J;
   bool use_nlist;
+  std::vector<double> exchange_potential;  // per species (empty: none)
+  Conversions const *convert() const { return m_convert; }
 
   void set(State<IsingConfiguration> const *_state) {
     m_state = throw_if_null(_state, "Error in IsingOccupantPotential::set: state==nullptr");
@@ -44,7 +61,13 @@ class IsingOccupantPotential {
 
   double per_supercell() const {
     IsingState st(spins_of_occupants(m_state->configuration), ValueMap());
-    return IsingFormationEnergy(J, 1, use_nlist, &st).per_supercell();
+    double E = IsingFormationEnergy(J, 1, use_nlist, &st).per_supercell();
+    if (m_convert) {
+      auto const &c = m_state->configuration;
+      for (Index l = 0; l < c.n_sites; ++l)
+        E -= exchange_potential[m_convert->species_index(m_convert->l_to_asym(l), c.occ(l))];
+    }
+    return E;
   }
   double per_unitcell() const { return per_supercell() / m_state->configuration.n_unitcells; }
 
@@ -63,7 +86,15 @@ class IsingOccupantPotential {
       const Index rows = c.shape[0], i = l % rows, j = l / rows;
       const int sum = spin(c.within(i + 1, 0) + rows * j) + spin(i + rows * c.within(j + 1, 1)) +
                       spin(c.within(i - 1, 0) + rows * j) + spin(i + rows * c.within(j - 1, 1));
-      dE += -J * ((2 * new_occ[k] - 1) - spin(l)) * sum;
+      if (m_convert) {
+        const Index b = m_convert->l_to_asym(l);
+        const int o = (spin(l) + 1) / 2;
+        dE = (dE + -J * ((2 * new_occ[k] - 1) - spin(l)) * sum) -
+             (exchange_potential[m_convert->species_index(b, new_occ[k])] -
+              exchange_potential[m_convert->species_index(b, o)]);
+      } else {
+        dE += -J * ((2 * new_occ[k] - 1) - spin(l)) * sum;
+      }
       changed.emplace_back(l, new_occ[k]);
     }
     return dE;
@@ -71,6 +102,7 @@ class IsingOccupantPotential {
 
  private:
   State<IsingConfiguration> const *m_state = nullptr;
+  Conversions const *m_convert = nullptr;
 };
 
 /// The occupation the occupant events read and write

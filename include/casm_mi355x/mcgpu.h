@@ -73,6 +73,24 @@ extern "C" {
  *   On the time grid, a chain whose total rate reaches 0 (every flip's
  *   probability underflows) fails the run with MCG_ERR_STATE. */
 #define MCG_MODE_NFOLD 3
+/* OCCUPANT: one sequential chain per wavefront running the reference's generic
+ *   occupant-event Metropolis bit for bit: the loop of
+ *   occupation_metropolis.hh:87-153 over an OccLocation (OccLocation.cc:39-118,
+ *   216-283) with several asymmetric units, proposing either semi-grand
+ *   canonical single-site events (propose_semigrand_canonical_event,
+ *   OccEventProposal.hh:261-352) or canonical pair swaps
+ *   (propose_canonical_event, :109-236) from the caller's swap types,
+ *   std::mt19937_64.  Square lattice; every site belongs to one asymmetric unit
+ *   (mcg_set_site_asym), and a unit allows one or two occupants
+ *   (mcg_desc.occupant).  Occupations are occupant indices (0 or 1).  The
+ *   potential: the square-lattice formation energy (J1, model.hh:259-290) on
+ *   the spins 2o - 1 of the occupant indices o, minus the exchange potential of
+ *   each site's species; the delta of an event, per transformed site in event
+ *   order with the earlier ones applied, is
+ *     dE = (dE + (-J1 ((2 o' - 1) - (2 o - 1)) * nbr_sum)) - (mu[to] - mu[from]).
+ *   One pass = the number of mutating sites (OccLocation::mol_size) steps.  At
+ *   most 40000 sites per chain (lattice and lists held in LDS). */
+#define MCG_MODE_OCCUPANT 4
 
 #define MCG_LATTICE_SQUARE 2 /* 2D, shape[2] == 1 */
 #define MCG_LATTICE_SC 3     /* 3D simple cubic */
@@ -101,6 +119,26 @@ typedef struct mcg_chain_desc {
   uint64_t seed;             /* std::mt19937_64::seed value (EXACT) or Philox key */
 } mcg_chain_desc;
 
+/* OCCUPANT system.  Replaces the Conversions / OccCandidateList / swap-list
+ * set-up of a generic occupation_metropolis run (OccCandidate.cc:31-155): the
+ * candidates are, in OccCandidateList order, (unit, occupant) of every unit
+ * with two occupants, occupant 0 first; swap types name two candidates. */
+typedef struct mcg_occupant_desc {
+  int64_t n_asym;            /* asymmetric units, 1..8                            */
+  const int64_t *occ_size;   /* n_asym entries: occupants each unit allows, 1 or 2 */
+  const int64_t *species;    /* n_asym x 2: species index of occupant 0 and 1 of
+                                each unit (the second ignored if occ_size is 1)  */
+  int64_t n_species;         /* species, 1..8                                     */
+  int64_t canonical;         /* 1: pair swaps (propose_canonical_event); 0: single
+                                sites (propose_semigrand_canonical_event)         */
+  int64_t n_swaps;           /* swap types, 1..16                                 */
+  const int64_t *swaps;      /* n_swaps x 2 candidate indices (cand_a, cand_b), in
+                                the order of the caller's swap list (e.g. what
+                                make_semigrand_canonical_swaps builds)            */
+  const double *exchange_potential; /* n_chains x n_species, eV (NULL: all 0)    */
+  int64_t sites_per_unitcell; /* n_unitcells = sites / sites_per_unitcell (>= 1)  */
+} mcg_occupant_desc;
+
 /* Run description.  Replaces IsingFormationEnergy / IsingParamComposition /
  * SemiGrandCanonicalPotential construction (model.hh:164-436,
  * basic_semigrand_canonical.hh:104-199) and the beta = 1/(KB*T) set-up of
@@ -123,6 +161,7 @@ typedef struct mcg_desc {
                             otherwise ignored                                 */
   double sample_time_period; /* NFOLD: > 0 samples on the time grid k*period
                                 (nfold.hh:105-108 BY_TIME); 0 samples by pass   */
+  const mcg_occupant_desc *occupant; /* OCCUPANT: the system; otherwise ignored */
 } mcg_desc;
 
 /* CHECKERBOARD: always use the HBM work-list kernels, even for lattices that
@@ -205,6 +244,15 @@ int mcg_get_occupation(mcg_handle *h, int64_t chain, int8_t *occ, int64_t n);
  * the site array partitioned by rate class, in class order. */
 int mcg_get_occ_location(mcg_handle *h, int64_t chain, int64_t *loc, int64_t n);
 int mcg_set_occ_location(mcg_handle *h, int64_t chain, const int64_t *loc, int64_t n);
+/* OCCUPANT lists (get and set): n = the chain's mutating sites, the candidate
+ * lists in candidate order, each list in its order, as site indices (Mol.l of
+ * OccLocation's m_loc entries; the caller maps Mol ids). */
+
+/* OCCUPANT: the asymmetric unit of every site of one chain (n = sites; the
+ * reference's Conversions::l_to_asym).  Resets the chain's occupation to
+ * occupant 0 everywhere and its lists (OccLocation::initialize); set the
+ * occupation afterwards.  A new handle starts with every site in unit 0. */
+int mcg_set_site_asym(mcg_handle *h, int64_t chain, const int8_t *asym, int64_t n);
 
 /* Dump / load the RNG state of one chain.  EXACT: 313 words, the
  * std::mt19937_64 state words followed by the index of the next output (the
@@ -216,9 +264,13 @@ int mcg_rng_load(mcg_handle *h, int64_t chain, const uint64_t *words, int64_t n_
 
 /* Advance every active chain by n_passes passes (one pass = one attempt per
  * site: N random-site attempts in EXACT, N pair-swap attempts in CANONICAL,
- * one coloured sweep in CHECKERBOARD).  CANONICAL fails with MCG_ERR_STATE
- * when an active chain has no site of one species (the reference's
- * "No events possible").
+ * one coloured sweep in CHECKERBOARD; mol_size() steps in OCCUPANT).
+ * CANONICAL fails with MCG_ERR_STATE when an active chain has no site of one
+ * species (the reference's "No events possible"); OCCUPANT fails the same way
+ * when a chain's swap types have no events at the start, or stops a chain at
+ * the step where they run out (the reference's loop stops there,
+ * occupation_metropolis.hh:121-127) and then fails with MCG_ERR_STATE after
+ * every chain has run (mcg_get_position gives where each chain stands).
  * A sample record is appended to a chain's device series whenever its pass
  * count reaches a multiple of sample_period.  Replaces the main loop of
  * basic_occupation_metropolis.hh:381-422 between two completion checks (the
@@ -232,8 +284,8 @@ int mcg_rng_load(mcg_handle *h, int64_t chain, const uint64_t *words, int64_t n_
  * (nfold.hh:121-126), taken only once the caller knows the run goes on. */
 int mcg_run(mcg_handle *h, int64_t n_passes);
 
-/* EXACT: advance every active chain by exactly n_steps attempts from where it
- * stands, which may end inside a pass; the next mcg_run or mcg_run_steps goes
+/* EXACT and OCCUPANT: advance every active chain by exactly n_steps attempts
+ * from where it stands, which may end inside a pass; the next mcg_run or mcg_run_steps goes
  * on from that step (mcg_run then makes whole passes' worth of attempts, so a
  * chain keeps its offset into the pass).  Records are appended at pass ends as
  * in mcg_run.  The counterpart of the reference's per-step counting for
@@ -243,10 +295,11 @@ int mcg_run(mcg_handle *h, int64_t n_passes);
  * MCG_ERR_STATE. */
 int mcg_run_steps(mcg_handle *h, int64_t n_steps);
 
-/* EXACT, CANONICAL, NFOLD: a chain's pass count, its steps into the current
- * pass (EXACT; 0 otherwise) and its FP64 running potential energy per unit
- * cell (occupation_metropolis.hh:103-106, :141; EXACT and CANONICAL) -- the
- * state a sample taken between two steps reads. */
+/* EXACT, CANONICAL, NFOLD, OCCUPANT: a chain's pass count, its steps into the
+ * current pass (EXACT and OCCUPANT; 0 otherwise) and its FP64 running
+ * potential energy per unit cell (occupation_metropolis.hh:103-106, :141;
+ * EXACT, CANONICAL and OCCUPANT) -- the state a sample taken between two steps
+ * reads. */
 int mcg_get_position(mcg_handle *h, int64_t chain, int64_t *n_pass, int64_t *step,
                      double *e_run);
 
@@ -299,7 +352,10 @@ int mcg_reserve_samples(mcg_handle *h, int64_t n_records);
  *   MCG_OBS_PARAM_COMPOSITION  x  = fl(fl((N+M)/2)/N)        (model.hh:412-422)
  *   MCG_OBS_FORMATION_ENERGY   ef = fl(E_f/N)                 (model.hh:293-295)
  *   MCG_OBS_POTENTIAL_ENERGY   e  = fl(fl(E_f - fl(mu*Nx))/N) (basic_semigrand_canonical.hh:165-174)
- * where E_f = fl(-J1*S1) - ... per the model (EXACT use_nlist==0: e_fold). */
+ * where E_f = fl(-J1*S1) - ... per the model (EXACT use_nlist==0: e_fold).
+ * OCCUPANT: MCG_OBS_POTENTIAL_ENERGY is the record's running potential energy
+ * per unit cell (e_run; the potential has an exchange potential per species),
+ * and x is the fraction of sites at occupant 1. */
 #define MCG_OBS_FORMATION_ENERGY 0
 #define MCG_OBS_PARAM_COMPOSITION 1
 #define MCG_OBS_POTENTIAL_ENERGY 2

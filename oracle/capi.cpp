@@ -542,6 +542,86 @@ int orc_canonical_run(int64_t rows, int64_t cols, double J, double temperature,
   }
 }
 
+// Occupant-event run over an OccLocation with several asymmetric units
+// (refcpu::occ_run).  site_asym: n = rows*cols unit indices; asym_species:
+// n_asym x 2 species indices (occupant 0, occupant 1; the second ignored
+// where occ_size is 1); occ_size: n_asym entries (1 or 2); mu: n_species
+// exchange potentials; swaps: n_swaps pairs of candidate indices.  occ: n
+// occupant indices in, final ones out.  engine: 313 words in and out.
+// loc_in: NULL (OccLocation::initialize) or the candidate lists' mol ids in
+// candidate order, each list's length given by the configuration; loc_out:
+// the final lists in the same form (n_mut entries).  Per completed pass
+// (at most cap entries): e_run, S, M.  counters: n_accept, n_reject, n_steps,
+// passes recorded.  Returns 0, or 1 with the message in err (256 bytes) when
+// the reference's loop would stop on an exception (the counters still hold
+// the steps made) or the input is invalid.
+int orc_occ_run(int64_t rows, int64_t cols, const int8_t *site_asym, int64_t n_asym,
+                const int64_t *asym_species, const int64_t *occ_size, int64_t n_species,
+                const double *mu, const int64_t *swaps, int64_t n_swaps, int64_t canonical,
+                double J, double temperature, double n_unitcells, int64_t n_steps, int8_t *occ,
+                uint64_t *engine, const int64_t *loc_in, int64_t *loc_out, double *e_run,
+                int64_t *S, int64_t *M, int64_t cap, int64_t *counters, double *initial,
+                int64_t use_nlist, char *err) {
+  try {
+    const int64_t n = rows * cols;
+    std::vector<int> sa(site_asym, site_asym + n);
+    std::vector<std::vector<int>> spec(n_asym);
+    for (int64_t b = 0; b < n_asym; ++b)
+      for (int64_t o = 0; o < occ_size[b]; ++o) spec[b].push_back((int)asym_species[2 * b + o]);
+    OccSystem sys(sa, spec, (int)n_species);
+    std::vector<int> o(occ, occ + n);
+    std::vector<double> m(mu, mu + n_species);
+    std::vector<std::pair<int, int>> sw;
+    for (int64_t i = 0; i < n_swaps; ++i) sw.push_back({(int)swaps[2 * i], (int)swaps[2 * i + 1]});
+    std::mt19937_64 eng;
+    {
+      std::ostringstream os;
+      for (int i = 0; i < 313; ++i) os << engine[i] << (i < 312 ? " " : "");
+      std::istringstream is(os.str());
+      is >> eng;
+    }
+    OccLocationG ol0;
+    const OccLocationG *init = nullptr;
+    if (loc_in) {
+      ol0.initialize(sys, o);
+      int64_t k = 0;
+      for (auto &v : ol0.loc)
+        for (size_t i = 0; i < v.size(); ++i) {
+          Index id = loc_in[k++];
+          v[i] = id;
+          ol0.mol[id].loc = (Index)i;
+        }
+      init = &ol0;
+    }
+    OccRunOutput r = occ_run(rows, cols, sys, o, J, m, sw, canonical != 0, temperature,
+                             n_unitcells, eng, n_steps, init, use_nlist != 0);
+    for (int64_t l = 0; l < n; ++l) occ[l] = (int8_t)r.final_occ[l];
+    engine_words(eng, engine);
+    int64_t k = 0;
+    for (auto const &v : r.ol.loc)
+      for (Index id : v) loc_out[k++] = id;
+    const int64_t np = std::min<int64_t>((int64_t)r.e_run.size(), cap);
+    for (int64_t p = 0; p < np; ++p) {
+      e_run[p] = r.e_run[p];
+      S[p] = r.S[p];
+      M[p] = r.M[p];
+    }
+    counters[0] = r.n_accept;
+    counters[1] = r.n_reject;
+    counters[2] = r.n_steps;
+    counters[3] = (int64_t)r.e_run.size();
+    *initial = r.initial;
+    if (!r.error.empty()) {
+      std::snprintf(err, 256, "%s", r.error.c_str());
+      return 1;
+    }
+    return 0;
+  } catch (std::exception &ex) {
+    std::snprintf(err, 256, "%s", ex.what());
+    return 1;
+  }
+}
+
 // N-fold way run (refcpu::nfold_run).  occ: rows*cols spins in, final spins
 // out; engine: 313 words in and out.  Per sample (at most capacity entries;
 // returns the number of samples): S, M, time, weight.  arr_out: the class-partitioned site array

@@ -1285,6 +1285,291 @@ inline CanonicalRunOutput canonical_run(Index rows, Index cols, std::vector<int>
 }
 
 // ---------------------------------------------------------------------------
+// Occupant events over an OccLocation with several asymmetric units: the
+// generic occupation_metropolis loop (occupation_metropolis.hh:87-153) with
+// propose_semigrand_canonical_event (OccEventProposal.hh:261-352: one
+// random_real over the summed candidate counts, then one choose_mol) or
+// propose_canonical_event (:109-236: one random_real over the summed count
+// products, then choose_mol on cand_a and on cand_b), OccLocation::initialize
+// and apply (OccLocation.cc:39-118, 216-283; no atom tracking), the candidate
+// list of OccCandidate.cc:31-52 and the swaps the caller gives (those of
+// make_canonical_swaps / make_semigrand_canonical_swaps, OccCandidate.cc:
+// 99-155, by default).  The system: a square lattice whose sites each belong
+// to one asymmetric unit; each unit allows one or two occupants (species
+// indices in occupant order); the configuration holds occupant indices.
+// The potential is the square-lattice Ising formation energy on spins
+// s = 2o - 1 of the occupant indices (model.hh:259-290, the multi-site delta
+// of model.hh:354-379) minus the exchange potential of each species:
+//   per_supercell: E_f, then - mu[species(l)] for l = 0 .. n-1 in site order;
+//   delta, per transformed site k in event order, the earlier ones applied:
+//     dE = (dE + (-J ((2 o' - 1) - s_l) sum_l)) - (mu[to] - mu[from]).
+// ---------------------------------------------------------------------------
+struct OccSystem {
+  std::vector<int> site_asym;                  // Conversions::l_to_asym
+  std::vector<std::vector<int>> asym_species;  // species index of each occupant index
+  int n_species = 0;
+  std::vector<std::pair<int, int>> cand;       // OccCandidateList: (asym, species)
+  std::vector<std::vector<int>> cand_of;       // [asym][species] -> candidate index
+
+  OccSystem(std::vector<int> _site_asym, std::vector<std::vector<int>> _asym_species,
+            int _n_species)
+      : site_asym(std::move(_site_asym)),
+        asym_species(std::move(_asym_species)),
+        n_species(_n_species) {
+    // OccCandidate.cc:31-52: asymmetric units with more than one occupant, each
+    // occupant in occupant order
+    cand_of.assign(asym_species.size(), std::vector<int>(n_species, -1));
+    for (int b = 0; b < (int)asym_species.size(); ++b) {
+      if (asym_species[b].size() < 2) continue;
+      for (int s : asym_species[b]) {
+        cand_of[b][s] = (int)cand.size();
+        cand.push_back({b, s});
+      }
+    }
+  }
+  int occ_size(int b) const { return (int)asym_species[b].size(); }
+  int species_index(int b, int o) const { return asym_species[b][o]; }
+  int occ_index(int b, int s) const {
+    for (int o = 0; o < occ_size(b); ++o)
+      if (asym_species[b][o] == s) return o;
+    throw std::runtime_error("refcpu: species not allowed");
+  }
+  bool species_allowed(int b, int s) const {
+    for (int x : asym_species[b])
+      if (x == s) return true;
+    return false;
+  }
+};
+
+// OccCandidate.cc:99-110 (cand_a < cand_b in candidate order, species differ,
+// each allowed on the other's unit) and :129-143 (same unit, species differ)
+inline std::vector<std::pair<int, int>> occ_canonical_swaps(OccSystem const &sys) {
+  std::vector<std::pair<int, int>> out;
+  for (int a = 0; a < (int)sys.cand.size(); ++a)
+    for (int b = 0; b < (int)sys.cand.size(); ++b) {
+      auto ca = sys.cand[a], cb = sys.cand[b];
+      if (ca < cb && ca.second != cb.second && sys.species_allowed(ca.first, cb.second) &&
+          sys.species_allowed(cb.first, ca.second))
+        out.push_back({a, b});
+    }
+  return out;
+}
+inline std::vector<std::pair<int, int>> occ_semigrand_swaps(OccSystem const &sys) {
+  std::vector<std::pair<int, int>> out;
+  for (int a = 0; a < (int)sys.cand.size(); ++a)
+    for (int b = 0; b < (int)sys.cand.size(); ++b) {
+      auto ca = sys.cand[a], cb = sys.cand[b];
+      if (ca.first == cb.first && ca.second != cb.second && sys.species_allowed(ca.first, cb.second))
+        out.push_back({a, b});
+    }
+  return out;
+}
+
+struct OccLocationG {
+  OccSystem const *sys = nullptr;
+  std::vector<OccMol> mol;
+  std::vector<std::vector<Index>> loc;  // m_loc[cand_index] -> mol ids
+  std::vector<Index> l_to_mol;
+
+  // OccLocation.cc:39-118 without atoms: mutating sites in site order get mol
+  // ids 0, 1, ...; fixed sites map to mol_size()
+  void initialize(OccSystem const &s, std::vector<int> const &occupation) {
+    sys = &s;
+    mol.clear();
+    l_to_mol.clear();
+    loc.assign(s.cand.size(), {});
+    Index nmut = 0;
+    for (Index l = 0; l < (Index)occupation.size(); ++l)
+      if (s.occ_size(s.site_asym[l]) > 1) ++nmut;
+    mol.resize(nmut);
+    Index id = 0;
+    for (Index l = 0; l < (Index)occupation.size(); ++l) {
+      const int b = s.site_asym[l];
+      if (s.occ_size(b) > 1) {
+        const int sp = s.species_index(b, occupation[l]);
+        const int ci = s.cand_of[b][sp];
+        OccMol &m = mol[id];
+        m.id = id;
+        m.l = l;
+        m.asym = b;
+        m.species_index = sp;
+        m.loc = (Index)loc[ci].size();
+        loc[ci].push_back(id);
+        l_to_mol.push_back(id);
+        ++id;
+      } else {
+        l_to_mol.push_back(nmut);
+      }
+    }
+  }
+  Index cand_size(int ci) const { return (Index)loc[ci].size(); }
+  // OccLocation.hh:257-261
+  OccMol const &choose_mol(int ci, std::mt19937_64 &engine) const {
+    std::size_t maxv = loc[ci].size() - 1;
+    std::size_t i = std::uniform_int_distribution<std::size_t>(0, maxv)(engine);
+    return mol[loc[ci][i]];
+  }
+  // OccLocation.cc:254-283 (the occ_transform part)
+  void apply_transform(Index mol_id, Index from_species, Index to_species,
+                       std::vector<int> &occupation) {
+    OccMol &m = mol[mol_id];
+    if (m.species_index != from_species)
+      throw std::runtime_error("refcpu: species mismatch in apply");
+    occupation[m.l] = sys->occ_index((int)m.asym, (int)to_species);
+    int ci = sys->cand_of[m.asym][m.species_index];
+    Index back = loc[ci].back();
+    loc[ci][m.loc] = back;
+    mol[back].loc = m.loc;
+    loc[ci].pop_back();
+    m.species_index = to_species;
+    ci = sys->cand_of[m.asym][m.species_index];
+    m.loc = (Index)loc[ci].size();
+    loc[ci].push_back(m.id);
+  }
+};
+
+struct OccEventG {  // OccEvent: 1 (semi-grand) or 2 (canonical) transforms
+  int n = 0;
+  Index mol_id[2], l[2], from[2], to[2];
+  int asym[2], new_occ[2];
+};
+
+// OccEventProposal.hh:109-152 / 261-303: the swap whose cumulative count range
+// holds one random_real draw over the total
+inline int occ_choose_swap(OccLocationG const &ol, std::vector<std::pair<int, int>> const &swaps,
+                           bool canonical, std::mt19937_64 &engine) {
+  std::vector<double> tsum(swaps.size() + 1);
+  tsum[0] = 0.;
+  for (size_t i = 0; i < swaps.size(); ++i)
+    tsum[i + 1] = tsum[i] + (canonical ? ((double)ol.cand_size(swaps[i].first)) *
+                                             ((double)ol.cand_size(swaps[i].second))
+                                       : ((double)ol.cand_size(swaps[i].first)));
+  if (tsum.back() == 0.0)
+    throw std::runtime_error(canonical
+                                 ? "Error in choose_canonical_swap: No events possible."
+                                 : "Error in choose_semigrand_canonical_swap: No events possible.");
+  double rand = std::uniform_real_distribution<double>(0, tsum.back())(engine);
+  for (size_t i = 0; i < swaps.size(); ++i)
+    if (rand < tsum[i + 1]) return (int)i;
+  throw std::runtime_error("refcpu: no swap chosen");
+}
+
+// :173-208 (canonical, choose_mol on cand_a then cand_b) and :314-336
+// (semi-grand, one choose_mol on cand_a)
+inline void occ_propose(OccEventG &e, OccLocationG const &ol,
+                        std::vector<std::pair<int, int>> const &swaps, bool canonical,
+                        std::mt19937_64 &engine) {
+  OccSystem const &s = *ol.sys;
+  const auto &sw = swaps[occ_choose_swap(ol, swaps, canonical, engine)];
+  const int ca[2] = {sw.first, sw.second}, cb[2] = {sw.second, sw.first};
+  e.n = canonical ? 2 : 1;
+  for (int i = 0; i < e.n; ++i) {
+    OccMol const &m = ol.choose_mol(ca[i], engine);
+    e.mol_id[i] = m.id;
+    e.l[i] = m.l;
+    e.asym[i] = s.cand[ca[i]].first;
+    e.from[i] = s.cand[ca[i]].second;
+    e.to[i] = s.cand[cb[i]].second;
+  }
+  for (int i = 0; i < e.n; ++i) e.new_occ[i] = s.occ_index(e.asym[i], (int)e.to[i]);
+}
+
+// the potential described above; occ is restored before returning
+inline double occ_potential_delta(IsingConfiguration const &shape, std::vector<int> &occ, double J,
+                                  std::vector<double> const &mu, OccEventG const &e) {
+  double dE = 0.0;
+  int orig[2];
+  for (int i = 0; i < e.n; ++i) {
+    dE = (dE + lattice_gas_single_delta(shape, occ, J, e.l[i], e.new_occ[i])) -
+         (mu[e.to[i]] - mu[e.from[i]]);
+    orig[i] = occ[e.l[i]];
+    occ[e.l[i]] = e.new_occ[i];
+  }
+  for (int i = e.n - 1; i >= 0; --i) occ[e.l[i]] = orig[i];
+  return dE;
+}
+
+inline double occ_potential_per_supercell(IsingConfiguration const &shape, OccSystem const &s,
+                                          std::vector<int> const &occ, double J,
+                                          std::vector<double> const &mu, bool use_nlist) {
+  IsingConfiguration spins(shape.rows, shape.cols, 1);
+  for (Index l = 0; l < shape.n_sites; ++l) spins.occupation[l] = 2 * occ[l] - 1;
+  IsingFormationEnergy fe(J, use_nlist);
+  fe.set_state(&spins);
+  double E = fe.per_supercell();
+  for (Index l = 0; l < shape.n_sites; ++l)
+    E -= mu[s.species_index(s.site_asym[l], occ[l])];
+  return E;
+}
+
+struct OccRunOutput {
+  std::vector<double> e_run;   // potential energy per unit cell after each pass
+  std::vector<long long> S;    // bond sum of the spins after each pass
+  std::vector<long long> M;    // sum of the spins after each pass
+  std::vector<int> final_occ;  // occupant indices
+  OccLocationG ol;
+  BigCountType n_accept = 0, n_reject = 0, n_steps = 0;
+  double initial = 0.0;
+  std::string error;  // what the proposal threw, if the loop stopped on it
+};
+
+// occupation_metropolis.hh:96-153 for n_steps steps (mol_size() steps per
+// pass; a record after each completed pass)
+inline OccRunOutput occ_run(Index rows, Index cols, OccSystem const &sys, std::vector<int> occ,
+                            double J, std::vector<double> const &mu,
+                            std::vector<std::pair<int, int>> const &swaps, bool canonical,
+                            double temperature, double n_unitcells, std::mt19937_64 &engine,
+                            long n_steps, OccLocationG const *init_ol = nullptr,
+                            bool use_nlist = true) {
+  OccRunOutput out;
+  IsingConfiguration shape(rows, cols, 1);
+  double pe = occ_potential_per_supercell(shape, sys, occ, J, mu, use_nlist) / n_unitcells;
+  out.initial = pe;
+  OccLocationG ol;
+  if (init_ol) {
+    ol = *init_ol;
+    ol.sys = &sys;
+  } else {
+    ol.initialize(sys, occ);
+  }
+  const double beta = 1.0 / (KB * temperature);
+  RandomNumberGenerator rng(std::shared_ptr<std::mt19937_64>(&engine, [](std::mt19937_64 *) {}));
+  const long per_pass = (long)ol.mol.size();
+  OccEventG e;
+  IsingConfiguration spins(rows, cols, 1);
+  for (long t = 0; t < n_steps; ++t) {
+    try {
+      occ_propose(e, ol, swaps, canonical, engine);
+    } catch (std::exception const &x) {
+      out.error = x.what();
+      break;
+    }
+    double dE = occ_potential_delta(shape, occ, J, mu, e);
+    if (metropolis_acceptance(dE, beta, rng)) {
+      out.n_accept++;
+      for (int i = 0; i < e.n; ++i) ol.apply_transform(e.mol_id[i], e.from[i], e.to[i], occ);
+      pe += (dE / n_unitcells);
+    } else {
+      out.n_reject++;
+    }
+    out.n_steps++;
+    if (per_pass > 0 && out.n_steps % per_pass == 0) {
+      out.e_run.push_back(pe);
+      long long m = 0;
+      for (Index l = 0; l < shape.n_sites; ++l) {
+        spins.occupation[l] = 2 * occ[l] - 1;
+        m += spins.occupation[l];
+      }
+      out.S.push_back(bond_sum(spins));
+      out.M.push_back(m);
+    }
+  }
+  out.final_occ = occ;
+  out.ol = ol;
+  return out;
+}
+
+// ---------------------------------------------------------------------------
 // N-fold way (rejection-free) for the SGC Ising model: the loop of
 // nfold.hh:78-149 (select an event and its time increment, sample by count
 // before applying it with the increment as the sample weight, apply, advance

@@ -1126,6 +1126,259 @@ static void cpu_nfold_argument_tests() {
             << std::endl;
 }
 
+// ---------------------------------------------------------------------------
+// Several asymmetric units: units (A, B), (B, A) and (A) on a square lattice
+// (the unit of each site drawn at random)
+// ---------------------------------------------------------------------------
+static std::vector<Index> random_units(int n, uint64_t seed) {
+  std::mt19937_64 e(seed);
+  std::vector<Index> ua(n);
+  for (auto &b : ua) b = (Index)(e() % 3);
+  return ua;
+}
+
+static std::vector<int> unit_occupants(std::vector<Index> const &ua, uint64_t seed) {
+  std::mt19937_64 e(seed);
+  std::vector<int> occ(ua.size());
+  for (size_t l = 0; l < ua.size(); ++l) occ[l] = ua[l] == 2 ? 0 : (int)(e() % 2);
+  return occ;
+}
+
+static const std::vector<std::vector<std::string>> kThreeUnits = {{"A", "B"}, {"B", "A"}, {"A"}};
+
+// OccLocation_test.cpp:16-66 for several units: fixed sites map to mol_size(),
+// every Mol in the list of its unit and species at its loc
+static bool occ_location_consistent_units(OccLocation const &ol, std::vector<int> const &occ) {
+  auto const &cv = ol.convert();
+  for (Index id = 0; id < ol.mol_size(); ++id)
+    if (ol.mol(id).id != id) return false;
+  for (Index l = 0; l < (Index)occ.size(); ++l) {
+    const Index b = cv.l_to_asym(l);
+    if (cv.occ_size(b) < 2) {
+      if (ol.l_to_mol_id(l) != ol.mol_size()) return false;
+      continue;
+    }
+    Mol const &m = ol.mol(ol.l_to_mol_id(l));
+    if (m.l != l || m.asym != b || m.species_index != cv.species_index(b, occ[l])) return false;
+    if (ol.mol_id(ol.candidate_list().index(m.asym, m.species_index), m.loc) != m.id) return false;
+  }
+  return true;
+}
+
+static refcpu::OccSystem oracle_system(std::vector<Index> const &ua) {
+  return refcpu::OccSystem(std::vector<int>(ua.begin(), ua.end()), {{0, 1}, {1, 0}, {0}}, 2);
+}
+
+static std::vector<std::pair<int, int>> swap_indices(OccCandidateList const &cl,
+                                                     std::vector<OccSwap> const &swaps) {
+  std::vector<std::pair<int, int>> out;
+  for (auto const &s : swaps) out.push_back({(int)cl.index(s.cand_a), (int)cl.index(s.cand_b)});
+  return out;
+}
+
+static void cpu_occupant_tests() {
+  const int R = 8, C = 9, N = R * C;
+  auto ua = random_units(N, 5);
+  Conversions cv(ua, kThreeUnits);
+  OccCandidateList cl(cv);
+  EXPECT(cv.species_size() == 2 && cv.occ_index(1, 0) == 1 && !cv.species_allowed(2, 1),
+         "Conversions with three units");
+  EXPECT(cl.size() == 4 && cl[2] == OccCandidate(1, 1) && cl[3] == OccCandidate(1, 0),
+         "candidate list in unit and occupant order");
+  auto sgc = make_semigrand_canonical_swaps(cv, cl);
+  auto can = make_canonical_swaps(cv, cl);
+  refcpu::OccSystem rs = oracle_system(ua);
+  EXPECT(swap_indices(cl, sgc) == refcpu::occ_semigrand_swaps(rs), "semi-grand swap list");
+  EXPECT(swap_indices(cl, can) == refcpu::occ_canonical_swaps(rs), "canonical swap list");
+  const double J = 0.1;
+  const std::vector<double> mu = {0.02, -0.05};
+  for (int canonical = 0; canonical < 2; ++canonical) {
+    auto const &swaps = canonical ? can : sgc;
+    auto idx = swap_indices(cl, swaps);
+    std::vector<int> occ = unit_occupants(ua, 6 + canonical);
+    ising_cpp::IsingConfiguration cfg({R, C}, 0);
+    cfg.set_occupation(occ);
+    State<ising_cpp::IsingConfiguration> state(cfg, ValueMap());
+    ising_cpp::IsingOccupantPotential pot(J, cv, mu, true);
+    pot.set(&state);
+    OccLocation ol(cv, cl);
+    ol.initialize(state.configuration.occupation());
+    refcpu::OccLocationG ro;
+    ro.initialize(rs, occ);
+    std::vector<int> rocc = occ;
+    refcpu::IsingConfiguration shape(R, C, 1);
+    EXPECT(same_bits(pot.per_supercell(),
+                     refcpu::occ_potential_per_supercell(shape, rs, rocc, J, mu, true)),
+           "potential per supercell");
+    auto eng = std::make_shared<std::mt19937_64>(77 + canonical);
+    std::mt19937_64 reng(77 + canonical);
+    RandomNumberGenerator<std::mt19937_64> rng(eng);
+    OccEvent e;
+    refcpu::OccEventG re;
+    bool same = true, delta = true, inv = true;
+    for (int k = 0; k < 20000; ++k) {
+      if (canonical)
+        propose_canonical_event(e, ol, swaps, rng);
+      else
+        propose_semigrand_canonical_event(e, ol, swaps, rng);
+      refcpu::occ_propose(re, ro, idx, canonical != 0, reng);
+      same = same && (int)e.linear_site_index.size() == re.n;
+      for (int i = 0; same && i < re.n; ++i)
+        same = e.linear_site_index[i] == re.l[i] && e.new_occ[i] == re.new_occ[i];
+      delta = delta && same_bits(pot.occ_delta_per_supercell(e.linear_site_index, e.new_occ),
+                                 refcpu::occ_potential_delta(shape, rocc, J, mu, re));
+      if (k % 3 != 2) {
+        ol.apply(e, state.configuration.occupation());
+        for (int i = 0; i < re.n; ++i) ro.apply_transform(re.mol_id[i], re.from[i], re.to[i], rocc);
+      }
+      if (k % 500 == 0) inv = inv && occ_location_consistent_units(ol, state.configuration.occupation());
+    }
+    std::string tag = canonical ? "canonical" : "semi-grand";
+    EXPECT(same, tag << " proposals match the oracle's stream use");
+    EXPECT(delta, tag << " potential deltas equal the oracle's bit for bit");
+    EXPECT(inv && occ_location_consistent_units(ol, state.configuration.occupation()),
+           tag << " OccLocation invariants");
+    bool lists = state.configuration.occupation() == rocc;
+    for (Index c = 0; lists && c < cl.size(); ++c)
+      lists = ol.locations()[c] == ro.loc[c];
+    EXPECT(lists, tag << " occupation and lists after the applied events");
+    EXPECT(*eng == reng, tag << " engine state after 20000 proposals");
+  }
+  std::cout << "cpu: occupant events over three units" << (g_fail ? " (failures so far)" : " ok")
+            << std::endl;
+}
+
+// occupation_metropolis through the reference's generic signature with
+// propose_semigrand_canonical_event or propose_canonical_event over three
+// units (OCCUPANT mode) vs refcpu::occ_run
+static void gpu_occupant_case(bool canonical, double T, uint64_t seed) {
+  const int R = 14, C = 12, N = R * C;
+  const long P = 30;
+  const double J = 0.1;
+  const std::vector<double> mu = {0.0, 0.04};
+  typedef ising_cpp::IsingConfiguration config_type;
+  auto ua = random_units(N, seed);
+  Conversions cv(ua, kThreeUnits);
+  OccCandidateList cl(cv);
+  auto swaps = canonical ? make_canonical_swaps(cv, cl) : make_semigrand_canonical_swaps(cv, cl);
+  std::vector<int> occ0 = unit_occupants(ua, seed + 1);
+  config_type cfg({R, C}, 0);
+  cfg.set_occupation(occ0);
+  ValueMap cond;
+  cond.scalar_values["temperature"] = T;
+  State<config_type> state(cfg, cond);
+  ising_cpp::IsingOccupantPotential potential(J, cv, mu, true);
+  potential.set(&state);
+  OccLocation ol(cv, cl);
+  ol.initialize(state.configuration.occupation());
+  const Index n_mut = ol.mol_size();
+
+  StateSamplingFunction pe("potential_energy", "running potential energy", {}, [&state]() {
+    return VectorXd{state.properties.scalar_values.at("potential_energy")};
+  });
+  pe.device_observable = MCG_OBS_POTENTIAL_ENERGY;
+  StateSamplingFunctionMap fns;
+  fns.emplace(pe.name, pe);
+  SamplingParams sp;
+  sp.sampler_names = {"potential_energy"};
+  CompletionCheckParams<BasicStatistics> cc;
+  cc.cutoff_params.max_count = P;
+  typedef RunManager<config_type, BasicStatistics, std::mt19937_64> rm_type;
+  std::vector<rm_type::sampling_fixture_params_type> fixtures;
+  fixtures.emplace_back("thermo", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), sp, cc,
+                        std::vector<std::string>{});
+  auto engine = std::make_shared<std::mt19937_64>(seed);
+  rm_type rm(engine, fixtures, true);
+  typedef RandomNumberGenerator<std::mt19937_64> generator_type;
+  generator_type rng(engine);
+  if (canonical)
+    occupation_metropolis(state, ol, potential, swaps, propose_canonical_event<generator_type>, rng,
+                          rm);
+  else
+    occupation_metropolis(state, ol, potential, swaps,
+                          propose_semigrand_canonical_event<generator_type>, rng, rm);
+
+  refcpu::OccSystem rs = oracle_system(ua);
+  std::mt19937_64 oe(seed);
+  auto o = refcpu::occ_run(R, C, rs, occ0, J, mu, swap_indices(cl, swaps), canonical, T, N, oe,
+                           P * n_mut);
+  std::string tag = std::string(canonical ? "canonical" : "semi-grand") + " three units";
+  auto const &res = rm.sampling_fixtures[0]->results();
+  VectorXd got = res.samplers.at("potential_energy")->component(0);
+  bool ok = got.size() == P && o.error.empty();
+  for (long i = 0; ok && i < P; ++i) ok = same_bits(got(i), o.e_run[i]);
+  EXPECT(ok, tag << ": running energy (" << got.size() << " samples)");
+  EXPECT(state.configuration.occupation() == o.final_occ, tag << ": final configuration");
+  bool lists = true;
+  for (Index c = 0; lists && c < cl.size(); ++c) lists = ol.locations()[c] == o.ol.loc[c];
+  EXPECT(lists, tag << ": OccLocation lists");
+  EXPECT(occ_location_consistent_units(ol, state.configuration.occupation()),
+         tag << ": OccLocation invariants after the run");
+  EXPECT(*engine == oe, tag << ": engine state");
+  EXPECT(res.n_accept == o.n_accept && res.n_reject == o.n_reject, tag << ": acceptance counts");
+  std::cout << "gpu: occupation_metropolis, " << tag << (g_fail ? " (failures so far)" : " ok")
+            << std::endl;
+}
+
+// a canonical run whose only swap type runs out of events: the loop stops
+// there (occupation_metropolis.hh:118-127) with the steps made so far
+static void gpu_occupant_no_events() {
+  const int R = 6, C = 6, N = R * C;
+  const double T = 50000.0;
+  typedef ising_cpp::IsingConfiguration config_type;
+  auto ua = random_units(N, 18);
+  Conversions cv(ua, kThreeUnits);
+  OccCandidateList cl(cv);
+  std::vector<OccSwap> swaps = {OccSwap(OccCandidate(0, 0), OccCandidate(1, 1))};
+  std::vector<int> occ0 = unit_occupants(ua, 19);
+  config_type cfg({R, C}, 0);
+  cfg.set_occupation(occ0);
+  ValueMap cond;
+  cond.scalar_values["temperature"] = T;
+  State<config_type> state(cfg, cond);
+  ising_cpp::IsingOccupantPotential potential(0.0, true);
+  potential.set(&state);
+  OccLocation ol(cv, cl);
+  ol.initialize(state.configuration.occupation());
+  StateSamplingFunction pe("potential_energy", "running potential energy", {}, [&state]() {
+    return VectorXd{state.properties.scalar_values.at("potential_energy")};
+  });
+  pe.device_observable = MCG_OBS_POTENTIAL_ENERGY;
+  StateSamplingFunctionMap fns;
+  fns.emplace(pe.name, pe);
+  SamplingParams sp;
+  sp.sampler_names = {"potential_energy"};
+  CompletionCheckParams<BasicStatistics> cc;
+  cc.cutoff_params.max_count = 100;
+  typedef RunManager<config_type, BasicStatistics, std::mt19937_64> rm_type;
+  std::vector<rm_type::sampling_fixture_params_type> fixtures;
+  fixtures.emplace_back("thermo", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), sp, cc,
+                        std::vector<std::string>{});
+  auto engine = std::make_shared<std::mt19937_64>(20);
+  rm_type rm(engine, fixtures, true);
+  typedef RandomNumberGenerator<std::mt19937_64> generator_type;
+  generator_type rng(engine);
+  occupation_metropolis(state, ol, potential, swaps, propose_canonical_event<generator_type>, rng,
+                        rm);
+  refcpu::OccSystem rs = oracle_system(ua);
+  std::mt19937_64 oe(20);
+  auto o = refcpu::occ_run(R, C, rs, occ0, 0.0, {0.0, 0.0}, swap_indices(cl, swaps), true, T, N,
+                           oe, 100 * ol.mol_size());
+  EXPECT(o.error == "Error in choose_canonical_swap: No events possible.", "oracle stops");
+  auto const &res = rm.sampling_fixtures[0]->results();
+  EXPECT(res.n_accept == o.n_accept && res.n_reject == o.n_reject,
+         "no-events stop: steps made (" << res.n_accept << " vs " << o.n_accept << ")");
+  EXPECT(rm.sampling_fixtures[0]->counter().step + rm.sampling_fixtures[0]->counter().pass *
+                                                       ol.mol_size() == o.n_steps,
+         "no-events stop: the fixture's count");
+  EXPECT(state.configuration.occupation() == o.final_occ, "no-events stop: configuration");
+  EXPECT(*engine == oe, "no-events stop: engine");
+  std::cout << "gpu: occupation_metropolis stops when no events are possible"
+            << (g_fail ? " (failures so far)" : " ok") << std::endl;
+}
+
 static void gpu_tests() {
   gpu_complete_case(false, 42, 1);
   gpu_complete_case(true, 44, 2);
@@ -1143,6 +1396,10 @@ static void gpu_tests() {
   gpu_case({3, 3, true, 4000.0, 0.0, 11, 1, 3000, 1e-4, 500, false});
   gpu_occupation_metropolis_tests();
   gpu_canonical_tests();
+  gpu_occupant_case(false, 1800.0, 41);
+  gpu_occupant_case(true, 2200.0, 43);
+  gpu_occupant_case(false, 400.0, 45);
+  gpu_occupant_no_events();
   gpu_nfold_case(false);
   gpu_nfold_case(true);
   gpu_nfold_two_fixtures();
@@ -1156,6 +1413,7 @@ int main(int argc, char **argv) {
     if (cpu) {
       cpu_tests();
       cpu_canonical_tests();
+      cpu_occupant_tests();
       cpu_nfold_argument_tests();
     }
     if (gpu) gpu_tests();

@@ -131,6 +131,13 @@ def _declare(L):
                                     P8, PU, P64, P64, P64, PD, P64, P64, PD, C.c_int64,
                                     C.c_char_p]
     L.orc_canonical_run.restype = C.c_int
+    L.orc_occ_run.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
+                              C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
+                              C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int64,
+                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                              C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                              C.c_int64, C.c_char_p]
+    L.orc_occ_run.restype = C.c_int
     L.orc_nfold_run.argtypes = [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double,
                                 C.c_int64, C.c_int64, C.c_double, C.c_int64, P8, PU, P64, P64,
                                 PD, PD, P64, PD, P64, C.c_int64, PD]
@@ -418,6 +425,93 @@ def canonical_run(rows, cols, J, T, n_passes, occ, engine, loc=None, use_nlist=T
         raise CanonicalError(err.value.decode())
     return dict(e_run=e[:n_passes], S=S[:n_passes], occ=o, engine=eng, loc=lo, mol_loc=ml,
                 n_accept=int(cnt[0]), n_reject=int(cnt[1]), initial=init.value)
+
+
+class OccError(RuntimeError):
+    """The reference's occupant-event loop would stop with this exception."""
+
+
+def occ_run(rows, cols, site_asym, asym_species, mu, swaps, canonical, J, T, n_steps, occ,
+            engine, loc=None, n_unitcells=None, use_nlist=True, cap=None):
+    """Occupant events over an OccLocation with several asymmetric units
+    (occupation_metropolis with propose_semigrand_canonical_event, or with
+    propose_canonical_event when canonical) on occupant indices occ
+    (column-major).
+
+    site_asym: unit of each site; asym_species: per unit, the species index of
+    each occupant (1 or 2 entries); mu: exchange potential per species; swaps:
+    (cand_a, cand_b) pairs of candidate indices.  loc: None (lists built by
+    OccLocation::initialize) or the candidate lists' mol ids in candidate order
+    (mol ids number the mutating sites in site order).  Returns a dict with
+    e_run, S, M per completed pass, final occ, engine, loc (mol ids), loc_sites
+    (the same lists as site indices), n_accept, n_reject, n_steps, initial and
+    error (the exception text, or None).
+    """
+    n = rows * cols
+    sa = np.ascontiguousarray(site_asym, dtype=np.int8)
+    n_asym = len(asym_species)
+    spec = np.zeros(2 * n_asym, dtype=np.int64)
+    osz = np.zeros(n_asym, dtype=np.int64)
+    for b, sp in enumerate(asym_species):
+        osz[b] = len(sp)
+        spec[2 * b:2 * b + len(sp)] = sp
+    m = np.ascontiguousarray(mu, dtype=np.float64)
+    sw = np.ascontiguousarray(np.asarray(swaps, dtype=np.int64).reshape(-1))
+    o = np.ascontiguousarray(occ, dtype=np.int8).copy()
+    eng = np.ascontiguousarray(engine, dtype=np.uint64).copy()
+    li = None if loc is None else np.ascontiguousarray(loc, dtype=np.int64)
+    lo = np.zeros(n, dtype=np.int64)
+    n_mut = int(sum(osz[b] > 1 for b in sa))
+    if cap is None:
+        cap = n_steps // max(1, n_mut) + 2
+    e = np.zeros(max(cap, 1), dtype=np.float64)
+    S = np.zeros(max(cap, 1), dtype=np.int64)
+    M = np.zeros(max(cap, 1), dtype=np.int64)
+    cnt = np.zeros(4, dtype=np.int64)
+    init = C.c_double(0.0)
+    err = C.create_string_buffer(256)
+    rc = lib().orc_occ_run(rows, cols, _p(sa, C.c_int8), n_asym, _p(spec, C.c_int64),
+                           _p(osz, C.c_int64), len(m), _p(m, C.c_double), _p(sw, C.c_int64),
+                           len(sw) // 2, int(bool(canonical)), J, T,
+                           float(n if n_unitcells is None else n_unitcells), n_steps,
+                           _p(o, C.c_int8), _p(eng, C.c_uint64),
+                           None if li is None else _p(li, C.c_int64), _p(lo, C.c_int64),
+                           _p(e, C.c_double), _p(S, C.c_int64), _p(M, C.c_int64), cap,
+                           _p(cnt, C.c_int64), C.byref(init), int(use_nlist), err)
+    np_ = min(int(cnt[3]), cap)
+    mol_site = np.array([l for l in range(n) if osz[sa[l]] > 1], dtype=np.int64)
+    out = dict(e_run=e[:np_], S=S[:np_], M=M[:np_], occ=o, engine=eng, loc=lo[:n_mut],
+               loc_sites=mol_site[lo[:n_mut]],
+               n_accept=int(cnt[0]), n_reject=int(cnt[1]), n_steps=int(cnt[2]),
+               initial=init.value, error=None)
+    if rc:
+        msg = err.value.decode()
+        if int(cnt[2]) == 0 and not msg.startswith("Error in choose_"):
+            raise OccError(msg)
+        out["error"] = msg
+    return out
+
+
+def occ_candidates(asym_species):
+    """OccCandidateList order (OccCandidate.cc:31-52): (unit, species) of every
+    unit with more than one occupant, in occupant order."""
+    return [(b, s) for b, sp in enumerate(asym_species) if len(sp) > 1 for s in sp]
+
+
+def occ_semigrand_swaps(asym_species):
+    """make_semigrand_canonical_swaps (OccCandidate.cc:129-143) as candidate pairs."""
+    c = occ_candidates(asym_species)
+    return [(i, j) for i in range(len(c)) for j in range(len(c))
+            if c[i][0] == c[j][0] and c[i][1] != c[j][1]]
+
+
+def occ_canonical_swaps(asym_species):
+    """make_canonical_swaps (OccCandidate.cc:99-110) as candidate pairs."""
+    c = occ_candidates(asym_species)
+    allowed = lambda b, s: s in asym_species[b]
+    return [(i, j) for i in range(len(c)) for j in range(len(c))
+            if c[i] < c[j] and c[i][1] != c[j][1] and allowed(c[i][0], c[j][1])
+            and allowed(c[j][0], c[i][1])]
 
 
 def nfold_run(rows, cols, J, T, mu, n_passes, occ, engine, sample_period=1, time_period=0.0,
