@@ -211,6 +211,77 @@ def bench_canonical(passes):
     return out
 
 
+def occupant_conditions(shape=(128, 128)):
+    """Two asymmetric units on the two sublattices of the square lattice, unit
+    0 allowing (A, B) and unit 1 (B, A); 8 temperatures x 8 exchange potentials
+    of B (semi-grand canonical) with random initial occupations (seeded)."""
+    import numpy as np
+    rows, cols = shape
+    n = rows * cols
+    units = np.array([((l % rows) + (l // rows)) % 2 for l in range(n)], dtype=np.int8)
+    conds, mus, occs = [], [], []
+    for a in range(8):
+        for b in range(8):
+            conds.append(dict(shape=shape, temperature=500.0 + a * 2500.0 / 7.0,
+                              seed=3000 + 8 * a + b))
+            mus.append([0.0, -0.2 + 0.4 * b / 7.0])
+            occs.append(np.random.default_rng(3000 + 8 * a + b).integers(0, 2, n).astype(np.int8))
+    return conds, mus, occs, units
+
+
+def bench_occupant(passes, canonical=False):
+    """Occupant events over an OccLocation with two asymmetric units
+    (OCCUPANT mode, bit-exact): the 64 conditions in one launch, one chain per
+    CU over the whole chip, one chain alone, and the oracle on one host core."""
+    import casmcode_monte_amd as cm
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as O
+    conds, mus, occs, units = occupant_conditions()
+    spec = [[0, 1], [1, 0]]
+    swaps = O.occ_canonical_swaps(spec) if canonical else O.occ_semigrand_swaps(spec)
+
+    def make(cs, ms, os_):
+        r = cm.Run(cs, mode=cm.MODE_OCCUPANT, J1=0.1,
+                   occupant=dict(asym_species=spec, n_species=2, canonical=canonical,
+                                 swaps=swaps, exchange_potential=ms))
+        for i, o in enumerate(os_):
+            r.set_site_asym(i, units)
+            r.set_occupation(i, o)
+        return r
+
+    r = make(conds, mus, occs)
+    r.run(2)
+    w, k, a = timed_run(r, passes)
+    out = dict(chains=len(conds), passes=passes, wall_s=w, kernel_s=k, attempts=a,
+               attempts_per_s=a / w, canonical=canonical,
+               system="128x128, units (A,B) and (B,A) on the two sublattices")
+    r.close()
+    reps = [dict(c, seed=c["seed"] + 100000 * q) for q in range(4) for c in conds]
+    f = make(reps, mus * 4, occs * 4)
+    f.run(1)
+    fp = max(passes // 4, 10)
+    fw, fk, fa = timed_run(f, fp)
+    out["full_chip"] = dict(chains=len(reps), passes=fp, wall_s=fw, kernel_s=fk, attempts=fa,
+                            attempts_per_s=fa / fw,
+                            note="4 seeds of each of the 64 conditions, one chain per CU")
+    f.close()
+    p = make(conds[:1], mus[:1], occs[:1])
+    p.run(1)
+    pp = max(passes // 10, 10)
+    pw, pk, pa = timed_run(p, pp)
+    out["plain_attempts_per_s"] = pa / pw
+    out["batched_over_plain"] = out["attempts_per_s"] / out["plain_attempts_per_s"]
+    p.close()
+    cp = 10
+    t0 = time.perf_counter()
+    O.occ_run(128, 128, units, spec, mus[0], swaps, canonical, 0.1, conds[0]["temperature"],
+              cp * 128 * 128, occs[0], O.mt64_state(3000, 0))
+    cw = time.perf_counter() - t0
+    out["cpu_port"] = dict(attempts_per_s=cp * 128 * 128 / cw, cores=1,
+                           sample=f"1 chain 128x128, {cp} passes, oracle restatement")
+    return out
+
+
 def bench_nfold(passes):
     """N-fold way chains (NFOLD mode): the 64 conditions of the exact scan
     (128x128, 8 T x 8 mu) in one launch vs one chain alone, in events/s."""
@@ -415,7 +486,8 @@ def main():
     ap.add_argument("--served-requests", type=int, default=400,
                     help="requests of the served scan (0 skips it)")
     ap.add_argument("--only", default="",
-                    choices=["", "exact", "scan", "band4096", "sc256", "fcc", "served", "peaks"],
+                    choices=["", "exact", "scan", "band4096", "sc256", "fcc", "served", "peaks",
+                             "occupant"],
                     help="run one measurement and print its JSON (for profiling one kernel)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes 10 times "
@@ -449,6 +521,7 @@ def main():
                "sc256": lambda: bench_checker_family((256, 256, 256), cm.LATTICE_SC, K),
                "fcc": lambda: bench_fcc(K, plain=False),
                "served": lambda: bench_served(args.served_requests),
+               "occupant": lambda: bench_occupant(K),
                "peaks": measured_peaks}[args.only]()
         print(json.dumps({"only": args.only, "steps": K, **one}))
         return
@@ -533,6 +606,10 @@ def main():
         print(f"[bench] nfold batch done: {nf['events_per_s']:.3e} events/s", file=sys.stderr,
               flush=True)
         line["nfold_64_chains"] = nf
+        oc = bench_occupant(max(K // 10, 20))
+        print(f"[bench] occupant batch done: {oc['attempts_per_s']:.3e} attempts/s",
+              file=sys.stderr, flush=True)
+        line["occupant_sgc_two_units_64_chains"] = oc
         line["checkerboard"] = {
             "scan_128x128_64_conditions": scan,
             "square_4096x4096_J1J2_16T": fam2,

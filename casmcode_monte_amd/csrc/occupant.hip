@@ -41,10 +41,12 @@ namespace mcg {
 namespace {
 
 // one swap type's lists at the block start: sizes, slot -> position maps
-// (position = base + dir * slot), and the occupant change of each site
+// (position = base + dir * slot) and units.  dir is +1 for a unit's occupant-0
+// list (the front of its segment) and -1 for its occupant-1 list, so it is
+// also the occupant change of a transformed site (0 -> 1 or 1 -> 0).
 struct SwapTab {
-  int32_t na, base_a, dir_a, da;
-  int32_t nb, base_b, dir_b, db;
+  int32_t na, base_a, dir_a, ua;
+  int32_t nb, base_b, dir_b, ub;
 };
 
 __device__ __forceinline__ int spin_of(uint8_t v) { return (v & 1) ? 1 : -1; }
@@ -78,6 +80,61 @@ __device__ __forceinline__ void nbr_sums(const uint8_t *lat, uint32_t l, const N
     const int s = s4[r] == x ? sx : spin_of(lat[s4[r]]);
     all += s;
     ex += s4[r] == l ? 0 : s;
+  }
+}
+
+// One proposed event evaluated from a state: the swap type, the chosen sites
+// with their list positions and the positions (and entries) of their lists'
+// last slots, the class, and the bond-sum and spin-sum changes.
+struct Proposal {
+  SwapTab e;
+  int32_t pa, pba, pb, pbb;
+  uint32_t a, la, b, lb;
+  int k, dS, dM;
+};
+
+// The neighbour sums, class and observable changes of a proposal whose sites
+// are known (the site and list-entry reads done).
+__device__ __forceinline__ void classify(Proposal &p, int si, int canonical, const uint8_t *lat,
+                                         int32_t rows, int32_t cols, uint64_t magic) {
+  const int oa = p.e.dir_a < 0 ? 1 : 0;
+  const int sa_old = 2 * oa - 1, sa_new = -sa_old;
+  int Sa, Sa_ex;
+  nbr_sums(lat, p.a, nbrs_of(p.a, rows, cols, magic), 0xFFFFFFFFu, 0, Sa, Sa_ex);
+  p.k = si * 25 + ((Sa + 4) >> 1) * 5;
+  p.dS = (sa_new - sa_old) * Sa_ex;
+  p.dM = sa_new - sa_old;
+  if (canonical) {
+    const int sb_old = p.e.dir_b < 0 ? 1 : -1, sb_new = -sb_old;
+    int Sb, Sb_ex;
+    nbr_sums(lat, p.b, nbrs_of(p.b, rows, cols, magic), p.a, sa_new, Sb, Sb_ex);
+    p.k += (Sb + 4) >> 1;
+    p.dS += (sb_new - sb_old) * Sb_ex;
+    p.dM += sb_new - sb_old;
+  }
+}
+
+// OccLocation.cc:254-283 for a, then for b (the second transform sees the
+// first's lists), by one lane: each removal moves its list's last entry into
+// the freed slot, and the position the last entry leaves is where the other
+// list of the unit appends.  With a and b on one unit, b's list has gained a
+// at that position, so b's removal moves a into b's slot and b lands there.
+__device__ __forceinline__ void apply_proposal(const Proposal &p, int canonical, uint8_t *lat,
+                                               uint16_t *list, int32_t *cnt) {
+  list[p.pa] = (uint16_t)p.la;
+  list[p.pba] = (uint16_t)p.a;
+  lat[p.a] = (uint8_t)(p.e.dir_a > 0 ? 1 : 0);
+  atomicAdd(&cnt[p.e.ua], -p.e.dir_a);
+  if (canonical) {
+    if (p.e.ub == p.e.ua) {
+      list[p.pb] = (uint16_t)p.a;
+      list[p.pba] = (uint16_t)p.b;
+    } else {
+      list[p.pb] = (uint16_t)p.lb;
+      list[p.pbb] = (uint16_t)p.b;
+    }
+    lat[p.b] = (uint8_t)(p.e.dir_b > 0 ? 1 : 0);
+    atomicAdd(&cnt[p.e.ub], -p.e.dir_b);
   }
 }
 
@@ -139,18 +196,26 @@ __global__ __launch_bounds__(128) void occupant_kernel(const OccParams *__restri
     return;
   }
 
-  // this lane's swap type (lanes < n_swaps): its candidates' units, occupants
-  // and list segments
-  int my_ua = 0, my_oa = 0, my_ub = 0, my_ob = 0;
+  // Lane j < n_swaps keeps swap type j's table entry (its two lists' sizes
+  // change with every accepted event of their units) and writes it to tab_s
+  // when it changes; the cumulative counts tsum[j+1] are exact integers, kept
+  // uniform.
+  SwapTab mine = {};
   if (lane < n_swaps) {
     const int ca = P->swap_a[lane], cb = P->swap_b[lane];
-    my_ua = P->cand_asym[ca];
-    my_oa = P->cand_occ[ca];
-    my_ub = P->cand_asym[cb];
-    my_ob = P->cand_occ[cb];
+    const int ua = P->cand_asym[ca], oa = P->cand_occ[ca];
+    const int ub = P->cand_asym[cb], ob = P->cand_occ[cb];
+    const int32_t c0a = St->occ_cnt0[ua], c0b = St->occ_cnt0[ub];
+    const int32_t za = P->asym_size[ua], zb = P->asym_size[ub];
+    mine.na = oa ? za - c0a : c0a;
+    mine.base_a = P->asym_off[ua] + (oa ? za - 1 : 0);
+    mine.dir_a = oa ? -1 : 1;
+    mine.ua = ua;
+    mine.nb = ob ? zb - c0b : c0b;
+    mine.base_b = P->asym_off[ub] + (ob ? zb - 1 : 0);
+    mine.dir_b = ob ? -1 : 1;
+    mine.ub = ub;
   }
-  const int32_t my_off_a = P->asym_off[my_ua], my_size_a = P->asym_size[my_ua];
-  const int32_t my_off_b = P->asym_off[my_ub], my_size_b = P->asym_size[my_ub];
 
   const int64_t pos0 = St->pos;
   int64_t pos = pos0;
@@ -183,47 +248,44 @@ __global__ __launch_bounds__(128) void occupant_kernel(const OccParams *__restri
     return (uint32_t)hi;
   };
 
-  // the swap choice's cumulative counts tsum[j+1] (exact integers), uniform
-  double ts[OCC_MAX_SWAPS];
+  uint32_t ts[OCC_MAX_SWAPS];
   double Ttot = 0.0;
-  // OccEventProposal.hh:127-139 / 276-288 from the current list sizes; the
-  // table of each swap type's lists for the lanes
-  auto rebuild = [&]() {
-    int64_t w = 0;
+  // OccEventProposal.hh:127-139 / 276-288: the cumulative counts from the
+  // lanes' entries (n(cand_a), or n(cand_a) n(cand_b) for canonical swaps)
+  auto publish = [&]() {
+    uint32_t w = 0;
     if (lane < n_swaps) {
-      const int c0a = cnt_s[my_ua], c0b = cnt_s[my_ub];
-      const int na = my_oa ? my_size_a - c0a : c0a;
-      const int nb = my_ob ? my_size_b - c0b : c0b;
-      SwapTab e;
-      e.na = na;
-      e.base_a = my_off_a + (my_oa ? my_size_a - 1 : 0);
-      e.dir_a = my_oa ? -1 : 1;
-      e.da = 1 - 2 * my_oa;  // a unit has two occupants: a transform swaps them
-      e.nb = nb;
-      e.base_b = my_off_b + (my_ob ? my_size_b - 1 : 0);
-      e.dir_b = my_ob ? -1 : 1;
-      e.db = 1 - 2 * my_ob;
-      tab_s[lane] = e;
-      w = canonical ? (int64_t)na * (int64_t)nb : (int64_t)na;
+      tab_s[lane] = mine;
+      w = canonical ? (uint32_t)mine.na * (uint32_t)mine.nb : (uint32_t)mine.na;
     }
-    int64_t run = 0;
+    uint32_t run = 0;
 #pragma unroll
     for (int j = 0; j < OCC_MAX_SWAPS; ++j) {
-      const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)w, j);
-      const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)((uint64_t)w >> 32), j);
-      run += (int64_t)(((uint64_t)hi << 32) | lo);
-      ts[j] = (double)run;
+      run += __builtin_amdgcn_readlane(w, j);
+      ts[j] = run;
     }
     Ttot = (double)run;
     asm volatile("" ::: "memory");
   };
-  // first j with rand < tsum[j+1]: the number of tsum[j+1] <= rand (they do
-  // not decrease, and rand < tsum[n_swaps])
+  // the swap type of a draw: the first j with rand < tsum[j+1]; for an integer
+  // t, rand < t exactly when floor(rand) < t, so the comparisons are integer
+  // ones: the number of tsum[j+1] <= floor(rand) (they do not decrease, and
+  // rand < tsum[n_swaps])
   auto choose = [&](double rand) -> int {
+    const uint32_t r = (uint32_t)rand;
     int i = 0;
 #pragma unroll
-    for (int j = 0; j < OCC_MAX_SWAPS - 1; ++j) i += (j < n_swaps - 1 && ts[j] <= rand) ? 1 : 0;
+    for (int j = 0; j < OCC_MAX_SWAPS - 1; ++j) i += (j < n_swaps - 1 && ts[j] <= r) ? 1 : 0;
     return i;
+  };
+  // every lane's entry after an accepted event changed the occupant-0 counts
+  // of units ua (by -dir_a) and, canonical, ub (by -dir_b)
+  auto account = [&](int ua, int dir_a, int ub, int dir_b) {
+    if (lane < n_swaps) {
+      auto d = [&](int u) { return (u == ua ? -dir_a : 0) + (canonical && u == ub ? -dir_b : 0); };
+      mine.na += d(mine.ua) * mine.dir_a;
+      mine.nb += d(mine.ub) * mine.dir_b;
+    }
   };
 
   const int base_len = canonical ? 3 : 2;
@@ -232,7 +294,7 @@ __global__ __launch_bounds__(128) void occupant_kernel(const OccParams *__restri
   bool dirty = true;
   while (done_steps < target) {
     if (dirty) {
-      rebuild();
+      publish();
       dirty = false;
     }
     if (Ttot == 0.0) {  // OccEventProposal.hh:136-139 / 290-293: no events possible
@@ -246,57 +308,45 @@ __global__ __launch_bounds__(128) void occupant_kernel(const OccParams *__restri
     const uint64_t x3 = canonical ? word(q + 3) : 0ULL;
     const double rand = __dmul_rn(canonical_from_word(x0), Ttot);  // random_real(tsum.back())
     const int si = choose(rand);
-    const SwapTab e = tab_s[si];
-    const uint64_t na = (uint64_t)e.na, nb = (uint64_t)e.nb;
+    Proposal p;
+    p.e = tab_s[si];
+    const uint64_t na = (uint64_t)p.e.na, nb = (uint64_t)p.e.nb;
     const uint32_t sa = (uint32_t)__umul64hi(x1, na);
     bool special = inwin && (x1 * na) < na;
-    const int32_t pa = e.base_a + e.dir_a * (int32_t)sa;
-    const uint32_t a = list[pa];
-    const int oa = lat[a] & 1;
-    const int sa_new = 2 * (oa + e.da) - 1;
-    const Nbrs ma = nbrs_of(a, rows, cols, magic);
-    int Sa, Sa_ex;
-    nbr_sums(lat, a, ma, 0xFFFFFFFFu, 0, Sa, Sa_ex);
-    int k = si * 25 + ((Sa + 4) >> 1) * 5;
-    int dS = (sa_new - (2 * oa - 1)) * Sa_ex, dM = sa_new - (2 * oa - 1);
-    uint32_t b = 0;
-    int32_t pb = 0;
-    uint32_t sb = 0;
+    p.pa = p.e.base_a + p.e.dir_a * (int32_t)sa;
+    p.pba = p.e.base_a + p.e.dir_a * (p.e.na - 1);
+    p.a = list[p.pa];
+    p.la = list[p.pba];
     uint64_t xm = x2;
     if (canonical) {
-      sb = (uint32_t)__umul64hi(x2, nb);
+      const uint32_t sb = (uint32_t)__umul64hi(x2, nb);
       special = special || (inwin && (x2 * nb) < nb);
-      pb = e.base_b + e.dir_b * (int32_t)sb;
-      b = list[pb];
-      const int ob = lat[b] & 1;
-      const int sb_new = 2 * (ob + e.db) - 1;
-      const Nbrs mb = nbrs_of(b, rows, cols, magic);
-      int Sb, Sb_ex;
-      nbr_sums(lat, b, mb, a, sa_new, Sb, Sb_ex);
-      k += (Sb + 4) >> 1;
-      dS += (sb_new - (2 * ob - 1)) * Sb_ex;
-      dM += sb_new - (2 * ob - 1);
+      p.pb = p.e.base_b + p.e.dir_b * (int32_t)sb;
+      p.pbb = p.e.base_b + p.e.dir_b * (p.e.nb - 1);
+      p.b = list[p.pb];
+      p.lb = list[p.pbb];
       xm = x3;
     }
-    const uint32_t fl = flg_s[k];
+    classify(p, si, canonical, lat, rows, cols, magic);
+    const uint32_t fl = flg_s[p.k];
+    const uint64_t thr = thr_s[p.k];
+    const double dq = dq_s[p.k];
     const bool nd = !(fl & 1u);
-    const bool acc = !nd || (fl & 2u) || (xm < thr_s[k]);
+    const bool acc = !nd || (fl & 2u) || (xm < thr);
     const bool valid = inwin && !special;
     const uint64_t ND = __ballot(valid && nd);
     const uint64_t AC = __ballot(valid && acc);
     const uint64_t SP = __ballot(special);
 
-    // step starts: 0, then +base_len or +base_len+1 (a uniform drawn); the
-    // block runs to the first accepted step or the first special one
-    uint64_t A = 0;
-    for (int t = 0; t < W; t += base_len + (int)((ND >> t) & 1ull)) A |= 1ull << t;
-    const uint64_t stops_sp = A & SP;
-    const uint64_t accs = A & AC & ~SP;
-    uint64_t exec = A;
-    if (stops_sp) exec &= (1ull << __builtin_ctzll(stops_sp)) - 1ull;
-    if (accs) {
-      const int fa = __builtin_ctzll(accs);
-      exec &= fa == 63 ? ~0ull : ((2ull << fa) - 1ull);
+    // step starts: 0, then +base_len or +base_len+1 (a uniform drawn), up to
+    // the first accepted step or before the first special one
+    uint64_t exec = 0;
+    for (int t = 0; t < W;) {
+      const uint64_t bit = 1ull << t;
+      if (SP & bit) break;
+      exec |= bit;
+      if (AC & bit) break;
+      t += base_len + (int)((ND >> t) & 1ull);
     }
     int steps = __popcll(exec);
     const int64_t to_pass_end = (int64_t)n_mut - step;
@@ -315,35 +365,14 @@ __global__ __launch_bounds__(128) void occupant_kernel(const OccParams *__restri
     }
     const uint64_t F = exec & AC;  // at most one step: the last one
     if (F) {
-      const int fl_lane = __builtin_ctzll(F);
-      if (lane == fl_lane) {
-        // OccLocation.cc:254-283 for a, then for b (the second transform sees
-        // the first's lists)
-        const int ua = P->cand_asym[P->swap_a[si]];
-        asm volatile("" ::: "memory");
-        const int32_t back_a = e.base_a + e.dir_a * (int32_t)(e.na - 1);
-        list[pa] = list[back_a];
-        asm volatile("" ::: "memory");
-        list[back_a] = (uint16_t)a;
-        lat[a] = (uint8_t)(oa + e.da);
-        cnt_s[ua] += oa == 0 ? -1 : 1;
-        if (canonical) {
-          const int ub = P->cand_asym[P->swap_b[si]];
-          const int32_t nb2 = e.nb + (ub == ua ? 1 : 0);
-          const int32_t back_b = e.base_b + e.dir_b * (nb2 - 1);
-          const int ob = lat[b] & 1;
-          asm volatile("" ::: "memory");
-          list[pb] = list[back_b];
-          asm volatile("" ::: "memory");
-          list[back_b] = (uint16_t)b;
-          lat[b] = (uint8_t)(ob + e.db);
-          cnt_s[ub] += ob == 0 ? -1 : 1;
-        }
-      }
+      const int f = __builtin_ctzll(F);
+      if (lane == f) apply_proposal(p, canonical, lat, list, cnt_s);
       asm volatile("" ::: "memory");
-      e_run = __dadd_rn(e_run, readlane_f64(dq_s[k], fl_lane));
-      S += __builtin_amdgcn_readlane(dS, fl_lane);
-      M += __builtin_amdgcn_readlane(dM, fl_lane);
+      e_run = __dadd_rn(e_run, readlane_f64(dq, f));
+      S += __builtin_amdgcn_readlane(p.dS, f);
+      M += __builtin_amdgcn_readlane(p.dM, f);
+      account(__builtin_amdgcn_readlane(p.e.ua, f), __builtin_amdgcn_readlane(p.e.dir_a, f),
+              __builtin_amdgcn_readlane(p.e.ub, f), __builtin_amdgcn_readlane(p.e.dir_b, f));
       ++n_acc;
       dirty = true;
     }
@@ -359,67 +388,39 @@ __global__ __launch_bounds__(128) void occupant_kernel(const OccParams *__restri
       const double r1 = __dmul_rn(canonical_from_word(word(pos)), Ttot);
       ++pos;
       const int i1 = choose(r1);
-      const SwapTab e1 = tab_s[i1];
-      const uint32_t s1a = lemire_scalar((uint64_t)e1.na);
-      const int32_t p1a = e1.base_a + e1.dir_a * (int32_t)s1a;
-      const uint32_t a1 = list[p1a];
-      const int o1a = lat[a1] & 1;
-      const int s1a_new = 2 * (o1a + e1.da) - 1;
-      int Sa1, Sa1_ex;
-      nbr_sums(lat, a1, nbrs_of(a1, rows, cols, magic), 0xFFFFFFFFu, 0, Sa1, Sa1_ex);
-      int k1 = i1 * 25 + ((Sa1 + 4) >> 1) * 5;
-      int dS1 = (s1a_new - (2 * o1a - 1)) * Sa1_ex, dM1 = s1a_new - (2 * o1a - 1);
-      uint32_t b1 = 0;
-      int32_t p1b = 0;
-      int o1b = 0;
+      Proposal q1;
+      q1.e = tab_s[i1];
+      const uint32_t s1a = lemire_scalar((uint64_t)q1.e.na);
+      q1.pa = q1.e.base_a + q1.e.dir_a * (int32_t)s1a;
+      q1.pba = q1.e.base_a + q1.e.dir_a * (q1.e.na - 1);
+      q1.a = list[q1.pa];
+      q1.la = list[q1.pba];
       if (canonical) {
-        const uint32_t s1b = lemire_scalar((uint64_t)e1.nb);
-        p1b = e1.base_b + e1.dir_b * (int32_t)s1b;
-        b1 = list[p1b];
-        o1b = lat[b1] & 1;
-        const int s1b_new = 2 * (o1b + e1.db) - 1;
-        int Sb1, Sb1_ex;
-        nbr_sums(lat, b1, nbrs_of(b1, rows, cols, magic), a1, s1a_new, Sb1, Sb1_ex);
-        k1 += (Sb1 + 4) >> 1;
-        dS1 += (s1b_new - (2 * o1b - 1)) * Sb1_ex;
-        dM1 += s1b_new - (2 * o1b - 1);
+        const uint32_t s1b = lemire_scalar((uint64_t)q1.e.nb);
+        q1.pb = q1.e.base_b + q1.e.dir_b * (int32_t)s1b;
+        q1.pbb = q1.e.base_b + q1.e.dir_b * (q1.e.nb - 1);
+        q1.b = list[q1.pb];
+        q1.lb = list[q1.pbb];
       }
+      classify(q1, i1, canonical, lat, rows, cols, magic);
       bool ok;
-      if (flg_s[k1] & 1u) {
+      if (flg_s[q1.k] & 1u) {
         ok = true;
       } else {
         ensure(pos + 1);
         const uint64_t xr = word(pos);
         ++pos;
-        ok = (flg_s[k1] & 2u) || (xr < thr_s[k1]);
+        ok = (flg_s[q1.k] & 2u) || (xr < thr_s[q1.k]);
       }
       // one wavefront walks: its LDS operations execute in issue order and the
       // asm keeps the compiler from reordering them
       asm volatile("" ::: "memory");
       if (ok) {
-        const int ua = P->cand_asym[P->swap_a[i1]];
-        if (lane == 0) {
-          const int32_t back_a = e1.base_a + e1.dir_a * (int32_t)(e1.na - 1);
-          list[p1a] = list[back_a];
-          asm volatile("" ::: "memory");
-          list[back_a] = (uint16_t)a1;
-          lat[a1] = (uint8_t)(o1a + e1.da);
-          cnt_s[ua] += o1a == 0 ? -1 : 1;
-          if (canonical) {
-            const int ub = P->cand_asym[P->swap_b[i1]];
-            const int32_t nb2 = e1.nb + (ub == ua ? 1 : 0);
-            const int32_t back_b = e1.base_b + e1.dir_b * (nb2 - 1);
-            asm volatile("" ::: "memory");
-            list[p1b] = list[back_b];
-            asm volatile("" ::: "memory");
-            list[back_b] = (uint16_t)b1;
-            lat[b1] = (uint8_t)(o1b + e1.db);
-            cnt_s[ub] += o1b == 0 ? -1 : 1;
-          }
-        }
-        e_run = __dadd_rn(e_run, dq_s[k1]);
-        S += dS1;
-        M += dM1;
+        if (lane == 0) apply_proposal(q1, canonical, lat, list, cnt_s);
+        e_run = __dadd_rn(e_run, dq_s[q1.k]);
+        S += q1.dS;
+        M += q1.dM;
+        account(q1.e.ua, q1.e.dir_a, q1.e.ub, q1.e.dir_b);
         ++n_acc;
         dirty = true;
       } else {
