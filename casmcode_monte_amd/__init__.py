@@ -20,6 +20,7 @@ MODE_CHECKERBOARD = 1
 MODE_CANONICAL = 2
 MODE_NFOLD = 3
 MODE_OCCUPANT = 4
+MODE_KMC = 5
 LATTICE_SQUARE = 2
 LATTICE_SC = 3
 LATTICE_FCC = 4
@@ -43,7 +44,7 @@ EXPORTED = [
     "mcg_equilibration", "mcg_last_run_time", "mcg_assign_chain", "mcg_autotune",
     "mcg_get_schedule", "mcg_set_schedule", "mcg_reserve_samples", "mcg_get_occ_location",
     "mcg_set_occ_location", "mcg_get_nfold_state", "mcg_run_steps", "mcg_get_position",
-    "mcg_check_batch", "mcg_set_site_asym",
+    "mcg_check_batch", "mcg_set_site_asym", "mcg_get_atoms", "mcg_kmc_set_spin_limit",
 ]
 
 
@@ -141,6 +142,8 @@ def lib():
         L.mcg_get_occ_location.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), C.c_int64]
         L.mcg_set_occ_location.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), C.c_int64]
         L.mcg_set_site_asym.argtypes = [C.c_void_p, C.c_int64, P(C.c_int8), C.c_int64]
+        L.mcg_get_atoms.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), C.c_int64]
+        L.mcg_kmc_set_spin_limit.argtypes = [C.c_void_p, C.c_int64]
         _lib = L
     return _lib
 
@@ -258,6 +261,16 @@ class Run:
         self._units = getattr(self, "_units", {})
         self._units[c] = a.copy()
 
+    def atoms(self, c, n_atoms):
+        """KMC: (n_atoms, 4) array of site, displacement along rows and columns,
+        jumps, in atom order."""
+        v = np.zeros(4 * n_atoms, dtype=np.int64)
+        _check(lib().mcg_get_atoms(self.h, c, v.ctypes.data_as(C.POINTER(C.c_int64)), n_atoms))
+        return v.reshape(n_atoms, 4)
+
+    def kmc_set_spin_limit(self, attempts):
+        _check(lib().mcg_kmc_set_spin_limit(self.h, int(attempts)))
+
     def n_mutating(self, c):
         """OCCUPANT: sites whose unit allows two occupants."""
         units = getattr(self, "_units", {}).get(c)
@@ -281,7 +294,7 @@ class Run:
                                           v.size))
 
     def rng_dump(self, c):
-        n = 313 if self.mode in (MODE_EXACT, MODE_CANONICAL, MODE_NFOLD, MODE_OCCUPANT) else 4
+        n = 4 if self.mode == MODE_CHECKERBOARD else 313
         w = np.zeros(n, dtype=np.uint64)
         _check(lib().mcg_rng_dump(self.h, c, w.ctypes.data_as(C.POINTER(C.c_uint64)), n))
         return w
@@ -298,8 +311,8 @@ class Run:
         _check(lib().mcg_run(self.h, int(n_passes)))
 
     def run_steps(self, n_steps):
-        """EXACT, OCCUPANT: exactly n_steps more attempts per active chain (may end
-        inside a pass)."""
+        """EXACT, OCCUPANT, KMC: exactly n_steps more attempts (KMC: events) per
+        active chain (may end inside a pass)."""
         _check(lib().mcg_run_steps(self.h, n_steps))
 
     def position(self, c):

@@ -19,7 +19,8 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libcasm_monte_mi355x.so")
 # measured-peak probes (not part of the Metropolis path; loaded by bench.py)
 PROBES_LIB = os.path.join(LIBDIR, "libcasm_monte_probes.so")
-SOURCES = ["api.hip", "exact.hip", "canonical.hip", "occupant.hip", "nfold.hip", "checker.hip",
+SOURCES = ["api.hip", "exact.hip", "canonical.hip", "occupant.hip", "kmc.hip", "nfold.hip",
+           "checker.hip",
            "checker_ring.hip", "checker_slab.hip", "fcc.hip", "stats.hip"]
 HEADERS = ["internal.hh", "rng.hh", "walk.hh", "fcc.hh"]
 

@@ -50,6 +50,7 @@ constexpr int64_t CHECKER_LDS_MAX = 128 * 1024;
 constexpr int64_t FCC_LDS_MAX = 142 * 1024;  // lattice; tables and lists take 17 KB static
 constexpr int64_t OCC_LDS_MAX = 136 * 1024;  // lattice + lists (+ ~23 KB static)
 constexpr int64_t OCC_MAX_SITES = 40000;
+constexpr int64_t KMC_MAX_SITES = 7500;  // lists, positions, site atoms, atoms: 18 B a site
 constexpr int CHECKER_BAND_MAX_L0 = 8192;     // longest line of the 2D band path
 
 // libstdc++ canonical double of a word compared to prob: the first word whose
@@ -328,10 +329,19 @@ struct mcg_handle {
   std::vector<double> occ_mu;  // n_chains x n_species
   std::vector<std::vector<int8_t>> site_asym;
   bool occupant() const { return mode == MCG_MODE_OCCUPANT; }
+  // KMC: per-chain parameters, the atom pool (4 int32 per atom: site, dx, dy,
+  // jumps; the lists share d_list) and the attempts without an event after
+  // which a chain stops
+  KmcParams *d_kp_kmc = nullptr;
+  std::vector<KmcParams> kp_kmc;
+  int32_t *d_atoms = nullptr;
+  int kmc_lds = 0;
+  int64_t kmc_spin_limit = int64_t(1) << 26;
+  bool kmc() const { return mode == MCG_MODE_KMC; }
   // one sequential chain per wavefront, MT19937-64 state, native lattice layout
   bool sequential() const {
     return mode == MCG_MODE_EXACT || mode == MCG_MODE_CANONICAL || mode == MCG_MODE_NFOLD ||
-           mode == MCG_MODE_OCCUPANT;
+           mode == MCG_MODE_OCCUPANT || mode == MCG_MODE_KMC;
   }
 
   // CHECKERBOARD
@@ -395,7 +405,8 @@ struct mcg_handle {
     if (stream2) (void)hipStreamSynchronize(stream2);
     void *ptrs[] = {d_lat, d_active, d_series, d_stats, d_eq, d_xp, d_xs, d_cp, d_cs, d_tab,
                     d_work, d_ftab, d_tri, d_tri_lines, d_band, d_pair, d_kp, d_list, d_np,
-                    d_mask_lds, d_mask_even, d_ring_list, d_batch, d_op, d_ot};
+                    d_mask_lds, d_mask_even, d_ring_list, d_batch, d_op, d_ot, d_kp_kmc,
+                    d_atoms};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -895,6 +906,172 @@ int occupant_configure(mcg_handle *h, const mcg_desc *desc) {
   h->occ_mu.assign((size_t)desc->n_chains * o->n_species, 0.0);
   if (o->exchange_potential)
     std::copy(o->exchange_potential, o->exchange_potential + h->occ_mu.size(), h->occ_mu.begin());
+  return MCG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// KMC: lattice-gas kinetic Monte Carlo with atom tracking.
+// ---------------------------------------------------------------------------
+
+// Parameters of one KMC chain with n_atoms atoms: geometry, the attempt rate
+// R = 4 n_atoms and the seven hop classes' acceptance (dE = J (a - b),
+// class a - b + 3: accepted without a draw when dE <= 0, else when
+// random_real(1.0) < exp(-beta dE)).
+void fill_kmc_params(mcg_handle *h, int c, int64_t n_atoms, KmcParams &P) {
+  auto const &ch = h->chains[c];
+  std::memset(&P, 0, sizeof(P));
+  P.rows = (int32_t)ch.shape[0];
+  P.cols = (int32_t)ch.shape[1];
+  P.n = (int32_t)(ch.shape[0] * ch.shape[1]);
+  P.n_atoms = (int32_t)n_atoms;
+  P.window = (h->options & MCG_OPT_EXACT_SCALAR_WALK) ? 1 : 64;
+  P.div_magic = ((1ULL << 40) + (uint64_t)P.rows - 1) / (uint64_t)P.rows;
+  P.attempt_rate = 4.0 * (double)n_atoms;
+  P.time_period = h->sample_time_period;
+  P.J = h->J1;
+  P.spin_limit = h->kmc_spin_limit;
+  P.lat_off = h->lat_off[c];
+  const double beta = 1.0 / (h->kb * ch.temperature);
+  for (int k = 0; k < KMC_NCLASS; ++k) {
+    const double dE = h->J1 * (double)(k - 3);
+    if (dE <= 0.0) continue;
+    P.draw_mask |= 1u << k;
+    uint64_t thr = 0;
+    if (threshold_for(std::exp(-dE * beta), &thr))
+      P.thr[k] = thr;
+    else
+      P.full_mask |= 1u << k;
+  }
+}
+
+// OccLocation::initialize with atoms (OccLocation.cc:39-118) for one chain:
+// vacancies and atoms in site order (the vacancy list from the front of the
+// list array, the atom list from the back), atoms numbered in site order with
+// no displacement; the loop's time and sums start again.
+int kmc_init(mcg_handle *h, int64_t c, std::vector<int8_t> const &occ) {
+  const int64_t n = (int64_t)occ.size();
+  std::vector<int32_t> seg(n), atoms;
+  int64_t k0 = 0, k1 = 0;
+  for (int64_t l = 0; l < n; ++l) {
+    if (occ[l]) {
+      seg[n - 1 - k1++] = (int32_t)l;
+      atoms.insert(atoms.end(), {(int32_t)l, 0, 0, 0});
+    } else {
+      seg[k0++] = (int32_t)l;
+    }
+  }
+  HIP_TRY(copy_sync(h, h->d_list + h->lat_off[c], seg.data(), sizeof(int32_t) * n,
+                    hipMemcpyHostToDevice));
+  if (!atoms.empty())
+    HIP_TRY(copy_sync(h, h->d_atoms + 4 * h->lat_off[c], atoms.data(),
+                      sizeof(int32_t) * atoms.size(), hipMemcpyHostToDevice));
+  fill_kmc_params(h, (int)c, k1, h->kp_kmc[c]);
+  HIP_TRY(copy_sync(h, h->d_kp_kmc + c, &h->kp_kmc[c], sizeof(KmcParams), hipMemcpyHostToDevice));
+  ExactState st;
+  HIP_TRY(copy_sync(h, &st, h->d_xs + c, sizeof(st), hipMemcpyDeviceToHost));
+  st.time = 0.0;
+  st.next_k = 0;
+  st.kmc_k = 0;
+  st.kmc_sum_sq = st.kmc_sum_dx = st.kmc_sum_dy = 0;
+  st.frozen = 0;
+  int64_t S = 0;
+  const int64_t rows = h->chains[c].shape[0], cols = h->chains[c].shape[1];
+  for (int64_t l = 0; l < n; ++l) {
+    const int64_t i = l % rows, j = l / rows;
+    if (occ[l]) S += occ[(i + 1) % rows + rows * j] + occ[i + rows * ((j + 1) % cols)];
+  }
+  st.S = S;
+  st.M = k1;
+  st.e_run = 0.0;
+  HIP_TRY(copy_sync(h, h->d_xs + c, &st, sizeof(st), hipMemcpyHostToDevice));
+  return MCG_OK;
+}
+
+int build_kmc(mcg_handle *h) {
+  h->kp_kmc.resize(h->n_chains);
+  int max_lds = 0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    fill_kmc_params(h, c, 0, h->kp_kmc[c]);
+    max_lds = std::max(max_lds, kmc_lds_bytes((int)h->sites(c)));
+  }
+  if (max_lds > OCC_LDS_MAX)
+    return fail(MCG_ERR_ARG, "KMC mode keeps a chain's lists and atoms in LDS: lattice too large");
+  h->kmc_lds = max_lds;
+  HIP_TRY(kmc_set_max_lds((int)OCC_LDS_MAX));
+  HIP_TRY(hipMalloc(&h->d_kp_kmc, sizeof(KmcParams) * h->n_chains));
+  HIP_TRY(hipMalloc(&h->d_xs, sizeof(ExactState) * h->n_chains));
+  HIP_TRY(hipMalloc(&h->d_list, sizeof(int32_t) * (size_t)h->lat_bytes));
+  HIP_TRY(hipMalloc(&h->d_atoms, 4 * sizeof(int32_t) * (size_t)h->lat_bytes));
+  std::vector<ExactState> xs(h->n_chains);
+  for (int c = 0; c < h->n_chains; ++c) {
+    std::memset(&xs[c], 0, sizeof(ExactState));
+    int pos = 0;
+    mt_seed(h->chains[c].seed, xs[c].mt, &pos);
+    xs[c].pos = pos;
+  }
+  HIP_TRY(copy_sync(h, h->d_xs, xs.data(), sizeof(ExactState) * h->n_chains,
+                    hipMemcpyHostToDevice));
+  return MCG_OK;
+}
+
+// KMC runs of n_passes passes (n events each) plus n_steps events per active
+// chain.  Fails before running when a chain has no atom or no vacancy (no
+// hop is possible), and after running when a chain froze (no event in the
+// spin limit's attempts) or when the time grid's records overflowed the
+// series.
+int kmc_run(mcg_handle *h, int64_t n_passes, int64_t n_steps) {
+  h->last_ms = 0.0;
+  h->last_attempts = 0.0;
+  if (n_passes == 0 && n_steps == 0) return MCG_OK;
+  const bool grid = h->sample_time_period > 0.0;
+  int64_t need = 0;
+  double events = 0.0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    if (!h->active[c]) continue;
+    const int64_t n = h->sites(c), na = h->kp_kmc[c].n_atoms;
+    if (na == 0 || na == n)
+      return fail(MCG_ERR_STATE, "KMC: no hop is possible (no atom or no vacancy)");
+    const int64_t p0 = h->n_pass[c];
+    const int64_t passes = n_passes + (h->n_step[c] + n_steps) / n;
+    if (!grid)
+      need = std::max(need, h->n_samples[c] + (p0 + passes) / h->sample_period -
+                                p0 / h->sample_period);
+    events += (double)n * (double)n_passes + (double)n_steps;
+  }
+  int rc = ensure_series(h, need);
+  if (rc != MCG_OK) return rc;
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  launch_kmc(h->d_kp_kmc, h->d_xs, h->d_lat, h->d_list, h->d_atoms, h->d_series, h->series_cap,
+             h->d_active, h->n_chains, n_passes, n_steps, h->sample_period, h->kmc_lds,
+             h->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  h->last_attempts = events;
+  std::vector<ExactState> xs(h->n_chains);
+  HIP_TRY(copy_sync(h, xs.data(), h->d_xs, sizeof(ExactState) * h->n_chains,
+                    hipMemcpyDeviceToHost));
+  bool frozen = false;
+  int64_t over = 0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    if (!h->active[c]) continue;
+    h->n_pass[c] = xs[c].n_pass;
+    h->n_step[c] = xs[c].step;
+    over = std::max(over, xs[c].n_samples - h->series_cap);
+    h->n_samples[c] = std::min(xs[c].n_samples, h->series_cap);
+    frozen = frozen || xs[c].frozen;
+  }
+  if (frozen)
+    return fail(MCG_ERR_STATE,
+                "KMC: a chain made no event in the spin limit's attempts (every hop of its "
+                "state has a vanishing probability at this temperature); it stopped there");
+  if (over > 0)
+    return fail(MCG_ERR_CAPACITY,
+                "KMC time-grid samples exceeded the sample series by " + std::to_string(over) +
+                    " records (they were dropped); reserve more with mcg_reserve_samples");
   return MCG_OK;
 }
 
@@ -1641,16 +1818,19 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
   *out = nullptr;
   if (desc->mode != MCG_MODE_EXACT && desc->mode != MCG_MODE_CHECKERBOARD &&
       desc->mode != MCG_MODE_CANONICAL && desc->mode != MCG_MODE_NFOLD &&
-      desc->mode != MCG_MODE_OCCUPANT)
+      desc->mode != MCG_MODE_OCCUPANT && desc->mode != MCG_MODE_KMC)
     return fail(MCG_ERR_ARG, "unknown mode");
+  if (desc->mode == MCG_MODE_KMC && (desc->lattice != MCG_LATTICE_SQUARE || desc->J2 != 0.0))
+    return fail(MCG_ERR_ARG, "KMC mode supports the 2D square nearest-neighbour lattice gas only");
   if (desc->mode == MCG_MODE_OCCUPANT &&
       (desc->lattice != MCG_LATTICE_SQUARE || desc->J2 != 0.0))
     return fail(MCG_ERR_ARG, "OCCUPANT mode supports the 2D square nearest-neighbour model only");
   if (desc->mode == MCG_MODE_NFOLD && (desc->lattice != MCG_LATTICE_SQUARE || desc->J2 != 0.0))
     return fail(MCG_ERR_ARG, "NFOLD mode supports the 2D square nearest-neighbour model only");
   if (desc->sample_time_period < 0.0 ||
-      (desc->sample_time_period > 0.0 && desc->mode != MCG_MODE_NFOLD))
-    return fail(MCG_ERR_ARG, "sample_time_period must be 0, or > 0 in NFOLD mode");
+      (desc->sample_time_period > 0.0 && desc->mode != MCG_MODE_NFOLD &&
+       desc->mode != MCG_MODE_KMC))
+    return fail(MCG_ERR_ARG, "sample_time_period must be 0, or > 0 in NFOLD and KMC modes");
   if (desc->lattice != MCG_LATTICE_SQUARE && desc->lattice != MCG_LATTICE_SC &&
       desc->lattice != MCG_LATTICE_FCC)
     return fail(MCG_ERR_ARG, "unknown lattice");
@@ -1729,6 +1909,10 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
       return fail(MCG_ERR_ARG, "CANONICAL mode supports at most 65536 sites per chain, "
                                "NFOLD at most 32768");
     }
+    if (h->kmc() && (n > KMC_MAX_SITES || h->chains[c].shape[0] < 3 || h->chains[c].shape[1] < 3)) {
+      delete h;
+      return fail(MCG_ERR_ARG, "KMC mode supports 3x3 to 7500 sites per chain");
+    }
     if (h->occupant() && (n > OCC_MAX_SITES || n % h->occ_spu != 0)) {
       delete h;
       return fail(MCG_ERR_ARG, "OCCUPANT mode supports at most 40000 sites per chain, a "
@@ -1755,7 +1939,7 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
     return bail(fail(MCG_ERR_CAPACITY, "out of device memory for lattices"));
   // Ising lattices start all +1 (stored as 1), FCC lattices empty, OCCUPANT
   // lattices at occupant 0
-  const int fill0 = (h->fcc() || h->occupant()) ? 0 : 1;
+  const int fill0 = (h->fcc() || h->occupant() || h->kmc()) ? 0 : 1;
   if (fill_sync(h, h->d_lat, fill0, (size_t)h->lat_bytes) != hipSuccess)
     return bail(fail(MCG_ERR_HIP, "hipMemset failed"));
   h->active.assign(h->n_chains, 1);
@@ -1772,6 +1956,7 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
        : h->canonical()              ? build_canonical(h)
        : h->nfold()                  ? build_nfold(h)
        : h->occupant()               ? build_occupant(h)
+       : h->kmc()                    ? build_kmc(h)
        : h->fcc()                    ? build_fcc(h)
                                      : build_checker(h);
   if (rc != MCG_OK) return bail(rc);
@@ -1781,6 +1966,7 @@ int mcg_create(const mcg_desc *desc, mcg_handle **out) {
     rc = reset_observables(h, c, occ);
     if (rc == MCG_OK && h->nfold()) rc = nfold_init_array(h, c, occ);
     if (rc == MCG_OK && h->occupant()) rc = occupant_init_lists(h, c, occ);
+    if (rc == MCG_OK && h->kmc()) rc = kmc_init(h, c, occ);
     if (rc != MCG_OK) return bail(rc);
   }
   rc = ensure_series(h, 64);
@@ -1800,9 +1986,10 @@ int mcg_set_occupation(mcg_handle *h, int64_t chain, const int8_t *occ, int64_t 
       if (v[l] < 0 || v[l] >= h->occ_size[h->site_asym[chain][l]])
         return fail(MCG_ERR_ARG, "OCCUPANT occupation values must be occupant indices of the "
                                  "site's asymmetric unit");
-  } else if (h->fcc()) {
+  } else if (h->fcc() || h->kmc()) {
     for (int8_t s : v)
-      if (s != 0 && s != 1) return fail(MCG_ERR_ARG, "FCC occupation values must be 0 or 1");
+      if (s != 0 && s != 1)
+        return fail(MCG_ERR_ARG, "FCC and KMC occupation values must be 0 or 1");
   } else {
     for (int8_t s : v)
       if (s != 1 && s != -1) return fail(MCG_ERR_ARG, "occupation values must be +1 or -1");
@@ -1822,6 +2009,7 @@ int mcg_set_occupation(mcg_handle *h, int64_t chain, const int8_t *occ, int64_t 
       const int rc = occupant_init_lists(h, chain, v);
       if (rc != MCG_OK) return rc;
     }
+    if (h->kmc()) return kmc_init(h, chain, v);
   } else {
     std::vector<int8_t> dev(n);
     if (h->cring[chain])  // natural layout, 1 for +1
@@ -1915,6 +2103,8 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
                              "NFOLD at most 32768");
   if (h->occupant() && n % h->occ_spu != 0)
     return fail(MCG_ERR_ARG, "OCCUPANT: sites must be a multiple of sites_per_unitcell");
+  if (h->kmc() && (nd.shape[0] < 3 || nd.shape[1] < 3))
+    return fail(MCG_ERR_ARG, "KMC mode supports lattices of at least 3x3 sites");
 
   HIP_TRY(hipStreamSynchronize(h->stream));
   const mcg_chain_desc old = h->chains[chain];
@@ -1953,6 +2143,13 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
     HIP_TRY(copy_sync(h, h->d_np + chain, &P, sizeof(P), hipMemcpyHostToDevice));
     ExactState S;
     nfold_reset_state(S, nd.seed);
+    HIP_TRY(copy_sync(h, h->d_xs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
+  } else if (h->kmc()) {
+    ExactState S;
+    std::memset(&S, 0, sizeof(S));
+    int pos = 0;
+    mt_seed(nd.seed, S.mt, &pos);
+    S.pos = pos;
     HIP_TRY(copy_sync(h, h->d_xs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
   } else if (h->occupant()) {
     const std::vector<int8_t> old_units = h->site_asym[chain];
@@ -2022,7 +2219,7 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
     const int mrc = checker_update_masks(h);
     if (mrc != MCG_OK) return mrc;
   }
-  const int fill = (h->fcc() || h->occupant()) ? 0 : 1;
+  const int fill = (h->fcc() || h->occupant() || h->kmc()) ? 0 : 1;
   HIP_TRY(fill_sync(h, h->d_lat + h->lat_off[chain], fill,
                     (size_t)((h->capacity[chain] + 255) & ~255LL)));
   h->n_samples[chain] = 0;
@@ -2031,6 +2228,7 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
   const int rc = reset_observables(h, chain, std::vector<int8_t>(n, (int8_t)fill));
   if (rc == MCG_OK && h->occupant())
     return occupant_init_lists(h, chain, std::vector<int8_t>(n, (int8_t)fill));
+  if (rc == MCG_OK && h->kmc()) return kmc_init(h, chain, std::vector<int8_t>(n, (int8_t)fill));
   if (rc != MCG_OK || !h->nfold()) return rc;
   return nfold_init_array(h, chain, std::vector<int8_t>(n, (int8_t)fill));
 }
@@ -2055,6 +2253,17 @@ int mcg_get_occ_location(mcg_handle *h, int64_t chain, int64_t *loc, int64_t n) 
     return MCG_OK;
   }
   if (!loc) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (h->kmc()) {
+    if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "list size mismatch");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    std::vector<int32_t> seg(n);
+    HIP_TRY(copy_sync(h, seg.data(), h->d_list + h->lat_off[chain], sizeof(int32_t) * n,
+                      hipMemcpyDeviceToHost));
+    const int64_t na = h->kp_kmc[chain].n_atoms;
+    for (int64_t i = 0; i < n - na; ++i) loc[i] = seg[i];
+    for (int64_t j = 0; j < na; ++j) loc[n - na + j] = seg[n - 1 - j];
+    return MCG_OK;
+  }
   if (!h->canonical() && !h->nfold())
     return fail(MCG_ERR_STATE, "site lists exist in CANONICAL and NFOLD modes only");
   if (n != h->sites(chain)) return fail(MCG_ERR_ARG, "list size mismatch");
@@ -2116,6 +2325,30 @@ int mcg_set_occ_location(mcg_handle *h, int64_t chain, const int64_t *loc, int64
     v[i] = (int32_t)l;
   }
   HIP_TRY(copy_sync(h, h->d_list + h->lat_off[chain], v.data(), sizeof(int32_t) * n,
+                    hipMemcpyHostToDevice));
+  return MCG_OK;
+}
+
+int mcg_get_atoms(mcg_handle *h, int64_t chain, int64_t *atoms, int64_t n_atoms) {
+  if (!valid_chain(h, chain) || (!atoms && n_atoms)) return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (!h->kmc()) return fail(MCG_ERR_STATE, "atoms are tracked in KMC mode only");
+  if (n_atoms != h->kp_kmc[chain].n_atoms) return fail(MCG_ERR_ARG, "atom count mismatch");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  std::vector<int32_t> v(4 * n_atoms);
+  if (n_atoms)
+    HIP_TRY(copy_sync(h, v.data(), h->d_atoms + 4 * h->lat_off[chain],
+                      sizeof(int32_t) * v.size(), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < v.size(); ++i) atoms[i] = v[i];
+  return MCG_OK;
+}
+
+int mcg_kmc_set_spin_limit(mcg_handle *h, int64_t attempts) {
+  if (!h || attempts < 1) return fail(MCG_ERR_ARG, "bad handle or spin limit");
+  if (!h->kmc()) return fail(MCG_ERR_STATE, "the spin limit exists in KMC mode only");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->kmc_spin_limit = attempts;
+  for (int c = 0; c < h->n_chains; ++c) h->kp_kmc[c].spin_limit = attempts;
+  HIP_TRY(copy_sync(h, h->d_kp_kmc, h->kp_kmc.data(), sizeof(KmcParams) * h->n_chains,
                     hipMemcpyHostToDevice));
   return MCG_OK;
 }
@@ -2220,6 +2453,7 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
   if (!h) return fail(MCG_ERR_ARG, "null handle");
   if (n_passes < 0) return fail(MCG_ERR_ARG, "n_passes < 0");
   if (h->occupant()) return occupant_run(h, n_passes, 0);
+  if (h->kmc()) return kmc_run(h, n_passes, 0);
   h->last_ms = 0.0;
   h->last_attempts = 0.0;
   const bool time_grid = h->nfold() && h->sample_time_period > 0.0;
@@ -2360,11 +2594,12 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
 
 int mcg_run_steps(mcg_handle *h, int64_t n_steps) {
   if (!h) return fail(MCG_ERR_ARG, "null handle");
-  if (h->mode != MCG_MODE_EXACT && !h->occupant())
+  if (h->mode != MCG_MODE_EXACT && !h->occupant() && !h->kmc())
     return fail(MCG_ERR_STATE,
-                "mcg_run_steps: single steps are run in EXACT and OCCUPANT modes only");
+                "mcg_run_steps: single steps are run in EXACT, OCCUPANT and KMC modes only");
   if (n_steps < 0) return fail(MCG_ERR_ARG, "n_steps < 0");
   if (h->occupant()) return occupant_run(h, 0, n_steps);
+  if (h->kmc()) return kmc_run(h, 0, n_steps);
   h->last_ms = 0.0;
   h->last_attempts = 0.0;
   if (n_steps == 0) return MCG_OK;
@@ -2410,7 +2645,7 @@ int mcg_get_position(mcg_handle *h, int64_t chain, int64_t *n_pass, int64_t *ste
   ExactState st;
   HIP_TRY(copy_sync(h, &st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
   if (n_pass) *n_pass = st.n_pass;
-  if (step) *step = (h->mode == MCG_MODE_EXACT || h->occupant()) ? st.step : 0;
+  if (step) *step = (h->mode == MCG_MODE_EXACT || h->occupant() || h->kmc()) ? st.step : 0;
   if (e_run) *e_run = st.e_run;
   return MCG_OK;
 }
@@ -2418,12 +2653,13 @@ int mcg_get_position(mcg_handle *h, int64_t chain, int64_t *n_pass, int64_t *ste
 int mcg_get_nfold_state(mcg_handle *h, int64_t chain, double *time, int64_t *pending_site,
                         double *pending_dt) {
   if (!valid_chain(h, chain)) return fail(MCG_ERR_ARG, "bad chain");
-  if (!h->nfold()) return fail(MCG_ERR_ARG, "mcg_get_nfold_state: not an NFOLD handle");
+  if (!h->nfold() && !h->kmc())
+    return fail(MCG_ERR_ARG, "mcg_get_nfold_state: not an NFOLD or KMC handle");
   HIP_TRY(hipStreamSynchronize(h->stream));
   ExactState st;
   HIP_TRY(copy_sync(h, &st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
   if (time) *time = st.time;
-  if (pending_site) *pending_site = st.pending_site;
+  if (pending_site) *pending_site = h->kmc() ? -1 : st.pending_site;
   if (pending_dt) *pending_dt = st.pending_dt;
   return MCG_OK;
 }

@@ -55,6 +55,10 @@ struct ExactState {
   int64_t step;
   // OCCUPANT: sites of each asymmetric unit holding its occupant 0
   int32_t occ_cnt0[8];
+  // KMC: attempts since the last event, and the sums over atoms of dx^2 +
+  // dy^2, dx and dy (time, next_k and frozen are shared with NFOLD)
+  int64_t kmc_k;
+  int64_t kmc_sum_sq, kmc_sum_dx, kmc_sum_dy;
 };
 
 // Every active chain makes n_passes * n + n_steps attempts from where it stands.
@@ -113,6 +117,45 @@ void launch_occupant(const OccParams *P, const OccTables *T, ExactState *S, int8
                      int64_t sample_period, int lds_bytes, hipStream_t stream);
 // Dynamic LDS limit of the occupant kernel (its static LDS is ~23 KB).
 hipError_t occupant_set_max_lds(int bytes);
+
+// ---------------------------------------------------------------------------
+// KMC mode: kinetic Monte Carlo with atom tracking for a square lattice gas,
+// one chain per wavefront.
+// ---------------------------------------------------------------------------
+// classes of a hop: a - b + 3 (a, b: atoms next to the origin other than at
+// the target, next to the target other than at the origin), dE = J (a - b)
+constexpr int KMC_NCLASS = 7;
+
+struct KmcParams {
+  int32_t rows, cols, n, n_atoms;
+  int32_t window;      // stream positions evaluated per block
+  uint32_t draw_mask;  // bit k: dE > 0, the class draws an acceptance word
+  uint32_t full_mask;  // bit k: every drawn word accepts
+  int32_t pad;
+  uint64_t div_magic;  // floor(l / rows) = (l * div_magic) >> 40
+  double attempt_rate; // R = 4 * n_atoms: attempts per unit time
+  double time_period;  // > 0: samples at k * time_period; 0: by pass
+  double J;
+  int64_t spin_limit;  // attempts without an event after which a chain stops (frozen)
+  int64_t lat_off;     // lattice byte offset; the lists start at entry lat_off of the
+                       // list pool, the atoms (4 int32 each) at entry 4 lat_off of the atom pool
+  uint64_t thr[KMC_NCLASS];
+};
+
+// LDS bytes of one chain: list, positions and site atoms (16-bit), atom
+// displacements and jumps (32-bit).
+inline int kmc_lds_bytes(int n) {
+  const int a = (n + 7) & ~7;
+  return 3 * 2 * a + 3 * 4 * a;
+}
+
+// Every active chain makes n_passes * n + n_steps events from where it stands
+// (fewer if it freezes: ExactState::frozen).
+void launch_kmc(const KmcParams *P, ExactState *S, int8_t *lat_pool, int32_t *list_pool,
+                int32_t *atom_pool, mcg_sample *series, int64_t series_cap, const uint8_t *active,
+                int n_chains, int64_t n_passes, int64_t n_steps, int64_t sample_period,
+                int lds_bytes, hipStream_t stream);
+hipError_t kmc_set_max_lds(int bytes);
 
 // ---------------------------------------------------------------------------
 // CANONICAL mode: pair swaps at fixed composition, one chain per wavefront.

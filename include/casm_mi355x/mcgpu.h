@@ -91,6 +91,33 @@ extern "C" {
  *   One pass = the number of mutating sites (OccLocation::mol_size) steps.  At
  *   most 40000 sites per chain (lattice and lists held in LDS). */
 #define MCG_MODE_OCCUPANT 4
+/* KMC: kinetic Monte Carlo with atom tracking, one sequential chain per
+ *   wavefront: the loop of kinetic_monte_carlo.hh:136-229 (select an event and
+ *   its time increment, sample by time at every grid time the event's time
+ *   reaches, apply the event through the OccLocation with atom tracking,
+ *   advance the time) for a square lattice gas of at least 3x3 and at most
+ *   7500 sites: occupant 0 = vacancy, 1 = atom, E = -J1 sum over nearest-
+ *   neighbour bonds of n_i n_j.  The reference takes its event selector from
+ *   outside the repository; the one here is rejection kinetic Monte Carlo:
+ *   attempts at 1/R each (R = 4 x atoms), an atom chosen uniformly from the
+ *   OccLocation's atom list (one Lemire word), a direction random_int(3)
+ *   (0: +row, 1: -row, 2: +column, 3: -column), rejected when the target holds
+ *   an atom, otherwise dE = J1 (a - b) (atoms next to the origin other than at
+ *   the target, next to the target other than at the origin) accepted without
+ *   a draw when dE <= 0, else when random_real(1.0) < exp(-beta dE); the
+ *   event's time increment is fl(k / R) for the k attempts it took.  One pass
+ *   = N events.  Samples: by pass (the state after the pass with the time of
+ *   its last event) or, with mcg_desc.sample_time_period > 0, at the times
+ *   k * period (k = 0, 1, ...), each taken before the event whose time reaches
+ *   it.  Records: pass = passes (by pass) or events applied (time grid),
+ *   S1 = sum over bonds of n_i n_j, M = atoms, S2 = sum over atoms of
+ *   dx^2 + dy^2, S3 = sum of dx, T3 = sum of dy (displacements in lattice
+ *   units since the occupation was set), time.  Occupations, lists, atoms,
+ *   records, times and engine words are bit-identical to the oracle's.  A
+ *   chain that makes no event in the spin limit's attempts (2^26 unless set
+ *   with mcg_kmc_set_spin_limit; every hop of its state has a vanishing
+ *   probability) stops, and the run fails with MCG_ERR_STATE. */
+#define MCG_MODE_KMC 5
 
 #define MCG_LATTICE_SQUARE 2 /* 2D, shape[2] == 1 */
 #define MCG_LATTICE_SC 3     /* 3D simple cubic */
@@ -246,7 +273,17 @@ int mcg_get_occ_location(mcg_handle *h, int64_t chain, int64_t *loc, int64_t n);
 int mcg_set_occ_location(mcg_handle *h, int64_t chain, const int64_t *loc, int64_t n);
 /* OCCUPANT lists (get and set): n = the chain's mutating sites, the candidate
  * lists in candidate order, each list in its order, as site indices (Mol.l of
- * OccLocation's m_loc entries; the caller maps Mol ids). */
+ * OccLocation's m_loc entries; the caller maps Mol ids).  KMC (get): n =
+ * sites, the vacancy list then the atom list. */
+
+/* KMC: the atoms of one chain, 4 entries each in atom order (atoms are
+ * numbered in site order when the occupation is set, OccLocation.cc:68-106):
+ * current site, displacement along rows and along columns since then, jumps
+ * (Atom::translation and n_jumps, OccEvent.hh:13-20; OccLocation.cc:286-330).
+ * n_atoms must equal the chain's atom count. */
+int mcg_get_atoms(mcg_handle *h, int64_t chain, int64_t *atoms, int64_t n_atoms);
+/* KMC: attempts without an event after which a chain stops (default 2^26). */
+int mcg_kmc_set_spin_limit(mcg_handle *h, int64_t attempts);
 
 /* OCCUPANT: the asymmetric unit of every site of one chain (n = sites; the
  * reference's Conversions::l_to_asym).  Resets the chain's occupation to
@@ -284,8 +321,8 @@ int mcg_rng_load(mcg_handle *h, int64_t chain, const uint64_t *words, int64_t n_
  * (nfold.hh:121-126), taken only once the caller knows the run goes on. */
 int mcg_run(mcg_handle *h, int64_t n_passes);
 
-/* EXACT and OCCUPANT: advance every active chain by exactly n_steps attempts
- * from where it stands, which may end inside a pass; the next mcg_run or mcg_run_steps goes
+/* EXACT, OCCUPANT and KMC: advance every active chain by exactly n_steps
+ * attempts (KMC: events) from where it stands, which may end inside a pass; the next mcg_run or mcg_run_steps goes
  * on from that step (mcg_run then makes whole passes' worth of attempts, so a
  * chain keeps its offset into the pass).  Records are appended at pass ends as
  * in mcg_run.  The counterpart of the reference's per-step counting for
@@ -295,8 +332,8 @@ int mcg_run(mcg_handle *h, int64_t n_passes);
  * MCG_ERR_STATE. */
 int mcg_run_steps(mcg_handle *h, int64_t n_steps);
 
-/* EXACT, CANONICAL, NFOLD, OCCUPANT: a chain's pass count, its steps into the
- * current pass (EXACT and OCCUPANT; 0 otherwise) and its FP64 running
+/* Sequential modes: a chain's pass count, its steps into the current pass
+ * (EXACT, OCCUPANT and KMC; 0 otherwise) and its FP64 running
  * potential energy per unit cell (occupation_metropolis.hh:103-106, :141;
  * EXACT, CANONICAL and OCCUPANT) -- the state a sample taken between two steps
  * reads. */
@@ -331,7 +368,9 @@ int mcg_n_samples(mcg_handle *h, int64_t chain, int64_t *n_samples);
  * keeps these in locals of nfold.hh:83-88 (selected_event_id, time_increment)
  * between selecting an event and applying it; the drop-in nfold() reads them
  * when a completion check stops the run right after a sample, where the
- * reference applies that event before it stops. */
+ * reference applies that event before it stops.  KMC: the loop's time
+ * (kinetic_monte_carlo.hh:218, the time of the last event); no event is ever
+ * pending (site -1). */
 int mcg_get_nfold_state(mcg_handle *h, int64_t chain, double *time, int64_t *pending_site,
                         double *pending_dt);
 

@@ -622,6 +622,52 @@ int orc_occ_run(int64_t rows, int64_t cols, const int8_t *site_asym, int64_t n_a
   }
 }
 
+// Lattice-gas kinetic Monte Carlo (refcpu::kmc_run).  occ: n = rows*cols
+// occupant indices (0 vacancy, 1 atom) in, final ones out.  engine: 313 words
+// in and out.  loc_out: n entries, the vacancy list then the atom list.
+// atoms_out: 4 per atom (site, dx, dy, n_jumps) in atom order.  Per record
+// (at most cap; the return value is the number made): events, time, S,
+// sum_sq, sum_dx, sum_dy.  scalars: final time; counts: attempts.
+int64_t orc_kmc_run(int64_t rows, int64_t cols, double J, double temperature, int64_t n_events,
+                    double time_period, int8_t *occ, uint64_t *engine, int64_t *loc_out,
+                    int64_t *atoms_out, int64_t cap, int64_t *rec_events, double *rec_time,
+                    int64_t *rec_S, int64_t *rec_sq, int64_t *rec_dx, int64_t *rec_dy,
+                    double *final_time, int64_t *attempts) {
+  const int64_t n = rows * cols;
+  std::vector<int> o(occ, occ + n);
+  std::mt19937_64 eng;
+  {
+    std::ostringstream os;
+    for (int i = 0; i < 313; ++i) os << engine[i] << (i < 312 ? " " : "");
+    std::istringstream is(os.str());
+    is >> eng;
+  }
+  KmcRunOutput r = kmc_run(rows, cols, o, J, temperature, eng, n_events, time_period);
+  for (int64_t l = 0; l < n; ++l) occ[l] = (int8_t)r.final_occ[l];
+  engine_words(eng, engine);
+  int64_t k = 0;
+  for (int c = 0; c < 2; ++c)
+    for (Index id : r.ol.loc[c]) loc_out[k++] = id;
+  for (size_t a = 0; a < r.atoms.size(); ++a) {
+    atoms_out[4 * a + 0] = r.atoms[a].site;
+    atoms_out[4 * a + 1] = r.atoms[a].dx;
+    atoms_out[4 * a + 2] = r.atoms[a].dy;
+    atoms_out[4 * a + 3] = r.atoms[a].n_jumps;
+  }
+  const int64_t nr = std::min<int64_t>(cap, (int64_t)r.records.size());
+  for (int64_t i = 0; i < nr; ++i) {
+    rec_events[i] = r.records[i].events;
+    rec_time[i] = r.records[i].time;
+    rec_S[i] = r.records[i].S;
+    rec_sq[i] = r.records[i].sum_sq;
+    rec_dx[i] = r.records[i].sum_dx;
+    rec_dy[i] = r.records[i].sum_dy;
+  }
+  *final_time = r.time;
+  *attempts = r.attempts;
+  return (int64_t)r.records.size();
+}
+
 // N-fold way run (refcpu::nfold_run).  occ: rows*cols spins in, final spins
 // out; engine: 313 words in and out.  Per sample (at most capacity entries;
 // returns the number of samples): S, M, time, weight.  arr_out: the class-partitioned site array

@@ -1570,6 +1570,154 @@ inline OccRunOutput occ_run(Index rows, Index cols, OccSystem const &sys, std::v
 }
 
 // ---------------------------------------------------------------------------
+// Kinetic Monte Carlo with atom tracking: the loop of kinetic_monte_carlo.hh:
+// 136-229 (select an event and its time increment, sample by time at every
+// sample time the event's time reaches, apply the event through the
+// OccLocation with atom tracking, advance the time, count the step) for a
+// square lattice gas, with an event selector defined here (the reference
+// takes its selector from outside the repository).
+//
+// Model: occupant 0 = vacancy, 1 = atom; E = -J * sum over nearest-neighbour
+// bonds of n_i n_j; lattices of at least 3 x 3 sites.
+// Selector (rejection kinetic Monte Carlo): attempts follow one another at
+// 1/R, R = 4 * (number of atoms).  An attempt chooses an atom uniformly from
+// the OccLocation list of atoms (choose_mol, OccLocation.hh:257-261) and a
+// direction d = random_int(3) (0: +i, 1: -i, 2: +j, 3: -j); it is rejected if
+// the target site holds an atom, otherwise dE = J (a - b) (a: atoms next to
+// the origin other than at the target, b: atoms next to the target other
+// than at the origin) and it is accepted without a draw when dE <= 0, else
+// when random_real(1.0) < exp(-beta dE).  The accepted attempt is the selected
+// event; its time increment is fl(k / R), k the attempts it took.
+// The event (OccEvent.hh): occ_transform (origin: 1 -> 0), (target: 0 -> 1);
+// atom_traj: the origin's atom to the target with delta_ijk the direction.
+// Atoms are numbered in site order at the start (OccLocation.cc:68-106); each
+// keeps its site, its displacement (the translation change since the start)
+// and its jump count (OccLocation.cc:286-330).
+// ---------------------------------------------------------------------------
+struct KmcAtom {
+  Index site = 0;
+  long dx = 0, dy = 0, n_jumps = 0;
+};
+
+struct KmcRecord {  // the state a sample sees
+  long events = 0;             // events applied
+  double time = 0.0;           // sample time (by time) or the last event's time (by count)
+  long long S = 0;             // sum over bonds of n_i n_j
+  long long sum_sq = 0;        // sum over atoms of dx^2 + dy^2
+  long long sum_dx = 0, sum_dy = 0;
+};
+
+struct KmcRunOutput {
+  std::vector<KmcRecord> records;
+  std::vector<int> final_occ;
+  BinaryOccLocation ol;
+  std::vector<KmcAtom> atoms;
+  std::vector<Index> site_atom;  // atom at each site, -1 for a vacancy
+  long events = 0;
+  long long attempts = 0;
+  double time = 0.0;
+};
+
+inline Index kmc_neighbour(Index l, int d, Index rows, Index cols) {
+  const Index i = l % rows, j = l / rows;
+  switch (d) {
+    case 0: return (i + 1) % rows + rows * j;
+    case 1: return (i + rows - 1) % rows + rows * j;
+    case 2: return i + rows * ((j + 1) % cols);
+    default: return i + rows * ((j + cols - 1) % cols);
+  }
+}
+
+inline KmcRecord kmc_record(std::vector<int> const &occ, std::vector<KmcAtom> const &atoms,
+                            Index rows, Index cols, long events, double time) {
+  KmcRecord r;
+  r.events = events;
+  r.time = time;
+  for (Index l = 0; l < rows * cols; ++l)
+    if (occ[l]) r.S += occ[kmc_neighbour(l, 0, rows, cols)] + occ[kmc_neighbour(l, 2, rows, cols)];
+  for (auto const &a : atoms) {
+    r.sum_sq += a.dx * a.dx + a.dy * a.dy;
+    r.sum_dx += a.dx;
+    r.sum_dy += a.dy;
+  }
+  return r;
+}
+
+// kinetic_monte_carlo.hh:185-226 for n_events events.  time_period > 0:
+// samples at the times k * time_period (k = 0, 1, ...; the sample index
+// continues from next_k) taken before the event whose time reaches them;
+// otherwise a sample after every `per_pass` events (by count, the state after
+// the pass with the time of its last event).
+inline KmcRunOutput kmc_run(Index rows, Index cols, std::vector<int> occ, double J,
+                            double temperature, std::mt19937_64 &engine, long n_events,
+                            double time_period = 0.0, long next_k = 0) {
+  KmcRunOutput out;
+  const Index n = rows * cols;
+  out.ol.initialize(occ);
+  out.site_atom.assign(n, -1);
+  for (Index l = 0; l < n; ++l)
+    if (occ[l]) {
+      out.site_atom[l] = (Index)out.atoms.size();
+      out.atoms.push_back(KmcAtom{l, 0, 0, 0});
+    }
+  const double beta = 1.0 / (KB * temperature);
+  const double R = 4.0 * (double)out.atoms.size();
+  const long per_pass = (long)n;
+  RandomNumberGenerator rng(std::shared_ptr<std::mt19937_64>(&engine, [](std::mt19937_64 *) {}));
+  double time = 0.0;
+  long k_next = next_k;
+  for (long ev = 0; ev < n_events; ++ev) {
+    // select_event: attempts until one is accepted
+    long k = 0;
+    Index from = 0, to = 0;
+    int dir = 0;
+    while (true) {
+      ++k;
+      ++out.attempts;
+      OccMol const &m = out.ol.choose_mol(1, engine);
+      from = m.l;
+      dir = (int)rng.random_int(3);
+      to = kmc_neighbour(from, dir, rows, cols);
+      if (occ[to]) continue;
+      int a = 0, b = 0;
+      for (int e = 0; e < 4; ++e) {
+        const Index x = kmc_neighbour(from, e, rows, cols), y = kmc_neighbour(to, e, rows, cols);
+        if (x != to) a += occ[x];
+        if (y != from) b += occ[y];
+      }
+      const double dE = J * (double)(a - b);
+      if (dE <= 0.0) break;
+      if (rng.random_real(1.0) < std::exp(-dE * beta)) break;
+    }
+    const double event_time = time + (double)k / R;
+    if (time_period > 0.0)
+      while (event_time >= (double)k_next * time_period) {
+        out.records.push_back(kmc_record(occ, out.atoms, rows, cols, ev, (double)k_next * time_period));
+        ++k_next;
+      }
+    // OccLocation::apply with atom tracking: the two transforms in order,
+    // then the atom's move
+    out.ol.apply_transform(from, 1, 0, occ);
+    out.ol.apply_transform(to, 0, 1, occ);
+    const Index id = out.site_atom[from];
+    out.site_atom[from] = -1;
+    out.site_atom[to] = id;
+    KmcAtom &at = out.atoms[id];
+    at.site = to;
+    at.dx += dir == 0 ? 1 : dir == 1 ? -1 : 0;
+    at.dy += dir == 2 ? 1 : dir == 3 ? -1 : 0;
+    at.n_jumps += 1;
+    time = event_time;
+    if (time_period <= 0.0 && (ev + 1) % per_pass == 0)
+      out.records.push_back(kmc_record(occ, out.atoms, rows, cols, ev + 1, time));
+  }
+  out.final_occ = occ;
+  out.events = n_events;
+  out.time = time;
+  return out;
+}
+
+// ---------------------------------------------------------------------------
 // N-fold way (rejection-free) for the SGC Ising model: the loop of
 // nfold.hh:78-149 (select an event and its time increment, sample by count
 // before applying it with the increment as the sample weight, apply, advance

@@ -138,6 +138,11 @@ def _declare(L):
                               C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                               C.c_int64, C.c_char_p]
     L.orc_occ_run.restype = C.c_int
+    L.orc_kmc_run.argtypes = [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_int64,
+                              C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                              C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_kmc_run.restype = C.c_int64
     L.orc_nfold_run.argtypes = [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double,
                                 C.c_int64, C.c_int64, C.c_double, C.c_int64, P8, PU, P64, P64,
                                 PD, PD, P64, PD, P64, C.c_int64, PD]
@@ -513,6 +518,42 @@ def occ_canonical_swaps(asym_species):
             if c[i] < c[j] and c[i][1] != c[j][1] and allowed(c[i][0], c[j][1])
             and allowed(c[j][0], c[i][1])]
 
+
+
+def kmc_run(rows, cols, J, T, n_events, occ, engine, time_period=0.0, cap=None):
+    """Lattice-gas kinetic Monte Carlo (kinetic_monte_carlo.hh loop with the
+    rejection selector of refcpu::kmc_run) on occupant indices occ (0 vacancy,
+    1 atom).  time_period > 0 samples on the grid k * time_period, otherwise
+    after every pass (rows*cols events).  Returns a dict with the records
+    (events, time, S, sum_sq, sum_dx, sum_dy), final occ, engine, loc (vacancy
+    list then atom list), atoms (site, dx, dy, n_jumps per atom), time and
+    attempts."""
+    n = rows * cols
+    o = np.ascontiguousarray(occ, dtype=np.int8).copy()
+    eng = np.ascontiguousarray(engine, dtype=np.uint64).copy()
+    n_atoms = int(o.sum())
+    if cap is None:
+        cap = 1 << 16
+    lo = np.zeros(n, dtype=np.int64)
+    at = np.zeros(4 * max(n_atoms, 1), dtype=np.int64)
+    ev = np.zeros(cap, dtype=np.int64)
+    tm = np.zeros(cap, dtype=np.float64)
+    S = np.zeros(cap, dtype=np.int64)
+    sq = np.zeros(cap, dtype=np.int64)
+    dx = np.zeros(cap, dtype=np.int64)
+    dy = np.zeros(cap, dtype=np.int64)
+    ft = C.c_double(0.0)
+    att = C.c_int64(0)
+    nr = lib().orc_kmc_run(rows, cols, J, T, n_events, time_period, _p(o, C.c_int8),
+                           _p(eng, C.c_uint64), _p(lo, C.c_int64), _p(at, C.c_int64), cap,
+                           _p(ev, C.c_int64), _p(tm, C.c_double), _p(S, C.c_int64),
+                           _p(sq, C.c_int64), _p(dx, C.c_int64), _p(dy, C.c_int64),
+                           C.byref(ft), C.byref(att))
+    k = min(nr, cap)
+    return dict(events=ev[:k], time=tm[:k], S=S[:k], sum_sq=sq[:k], sum_dx=dx[:k],
+                sum_dy=dy[:k], n_records=nr, occ=o, engine=eng, loc=lo,
+                atoms=at[:4 * n_atoms].reshape(n_atoms, 4), final_time=ft.value,
+                attempts=att.value)
 
 def nfold_run(rows, cols, J, T, mu, n_passes, occ, engine, sample_period=1, time_period=0.0,
               capacity=None, sample_start=False):
