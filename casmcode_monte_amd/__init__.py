@@ -45,6 +45,7 @@ EXPORTED = [
     "mcg_get_schedule", "mcg_set_schedule", "mcg_reserve_samples", "mcg_get_occ_location",
     "mcg_set_occ_location", "mcg_get_nfold_state", "mcg_run_steps", "mcg_get_position",
     "mcg_check_batch", "mcg_set_site_asym", "mcg_get_atoms", "mcg_kmc_set_spin_limit",
+    "mcg_kmc_advance",
 ]
 
 
@@ -144,6 +145,7 @@ def lib():
         L.mcg_set_site_asym.argtypes = [C.c_void_p, C.c_int64, P(C.c_int8), C.c_int64]
         L.mcg_get_atoms.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), C.c_int64]
         L.mcg_kmc_set_spin_limit.argtypes = [C.c_void_p, C.c_int64]
+        L.mcg_kmc_advance.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_int64]
         _lib = L
     return _lib
 
@@ -267,6 +269,13 @@ class Run:
         v = np.zeros(4 * n_atoms, dtype=np.int64)
         _check(lib().mcg_get_atoms(self.h, c, v.ctypes.data_as(C.POINTER(C.c_int64)), n_atoms))
         return v.reshape(n_atoms, 4)
+
+    def kmc_advance(self, max_events, stop_time=float("inf"), select_at_end=False):
+        """KMC: apply the pending event and events up to max_events; an event
+        reaching stop_time stays pending; select_at_end leaves one more selected
+        (mcg_kmc_advance)."""
+        _check(lib().mcg_kmc_advance(self.h, int(max_events), float(stop_time),
+                                     int(bool(select_at_end))))
 
     def kmc_set_spin_limit(self, attempts):
         _check(lib().mcg_kmc_set_spin_limit(self.h, int(attempts)))

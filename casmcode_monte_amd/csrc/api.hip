@@ -974,6 +974,9 @@ int kmc_init(mcg_handle *h, int64_t c, std::vector<int8_t> const &occ) {
   st.kmc_k = 0;
   st.kmc_sum_sq = st.kmc_sum_dx = st.kmc_sum_dy = 0;
   st.frozen = 0;
+  st.pending_site = -1;
+  st.pending_dt = 0.0;
+  st.kmc_pdir = st.kmc_pk = 0;
   int64_t S = 0;
   const int64_t rows = h->chains[c].shape[0], cols = h->chains[c].shape[1];
   for (int64_t l = 0; l < n; ++l) {
@@ -1015,14 +1018,16 @@ int build_kmc(mcg_handle *h) {
 }
 
 // KMC runs of n_passes passes (n events each) plus n_steps events per active
-// chain.  Fails before running when a chain has no atom or no vacancy (no
-// hop is possible), and after running when a chain froze (no event in the
-// spin limit's attempts) or when the time grid's records overflowed the
-// series.
-int kmc_run(mcg_handle *h, int64_t n_passes, int64_t n_steps) {
+// chain, a pending event first; an event reaching stop_time stays pending,
+// and select_at_end leaves one more selected.  Fails before running when a
+// chain has no atom or no vacancy (no hop is possible), and after running
+// when a chain froze (no event in the spin limit's attempts) or when the time
+// grid's records overflowed the series.
+int kmc_run(mcg_handle *h, int64_t n_passes, int64_t n_steps,
+            double stop_time = std::numeric_limits<double>::infinity(), bool select_at_end = false) {
   h->last_ms = 0.0;
   h->last_attempts = 0.0;
-  if (n_passes == 0 && n_steps == 0) return MCG_OK;
+  if (n_passes == 0 && n_steps == 0 && !select_at_end) return MCG_OK;
   const bool grid = h->sample_time_period > 0.0;
   int64_t need = 0;
   double events = 0.0;
@@ -1032,7 +1037,7 @@ int kmc_run(mcg_handle *h, int64_t n_passes, int64_t n_steps) {
     if (na == 0 || na == n)
       return fail(MCG_ERR_STATE, "KMC: no hop is possible (no atom or no vacancy)");
     const int64_t p0 = h->n_pass[c];
-    const int64_t passes = n_passes + (h->n_step[c] + n_steps) / n;
+    const int64_t passes = n_passes + (h->n_step[c] + n_steps + 1) / n;
     if (!grid)
       need = std::max(need, h->n_samples[c] + (p0 + passes) / h->sample_period -
                                 p0 / h->sample_period);
@@ -1042,8 +1047,8 @@ int kmc_run(mcg_handle *h, int64_t n_passes, int64_t n_steps) {
   if (rc != MCG_OK) return rc;
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
   launch_kmc(h->d_kp_kmc, h->d_xs, h->d_lat, h->d_list, h->d_atoms, h->d_series, h->series_cap,
-             h->d_active, h->n_chains, n_passes, n_steps, h->sample_period, h->kmc_lds,
-             h->stream);
+             h->d_active, h->n_chains, n_passes, n_steps, h->sample_period, stop_time,
+             select_at_end ? 1 : 0, h->kmc_lds, h->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   HIP_TRY(hipEventSynchronize(h->ev1));
@@ -2342,6 +2347,13 @@ int mcg_get_atoms(mcg_handle *h, int64_t chain, int64_t *atoms, int64_t n_atoms)
   return MCG_OK;
 }
 
+int mcg_kmc_advance(mcg_handle *h, int64_t max_events, double stop_time, int64_t select_at_end) {
+  if (!h) return fail(MCG_ERR_ARG, "null handle");
+  if (!h->kmc()) return fail(MCG_ERR_STATE, "mcg_kmc_advance: KMC mode only");
+  if (max_events < 0 || std::isnan(stop_time)) return fail(MCG_ERR_ARG, "bad event count or stop time");
+  return kmc_run(h, 0, max_events, stop_time, select_at_end != 0);
+}
+
 int mcg_kmc_set_spin_limit(mcg_handle *h, int64_t attempts) {
   if (!h || attempts < 1) return fail(MCG_ERR_ARG, "bad handle or spin limit");
   if (!h->kmc()) return fail(MCG_ERR_STATE, "the spin limit exists in KMC mode only");
@@ -2659,7 +2671,7 @@ int mcg_get_nfold_state(mcg_handle *h, int64_t chain, double *time, int64_t *pen
   ExactState st;
   HIP_TRY(copy_sync(h, &st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
   if (time) *time = st.time;
-  if (pending_site) *pending_site = h->kmc() ? -1 : st.pending_site;
+  if (pending_site) *pending_site = st.pending_site;
   if (pending_dt) *pending_dt = st.pending_dt;
   return MCG_OK;
 }

@@ -59,6 +59,9 @@ struct ExactState {
   // dy^2, dx and dy (time, next_k and frozen are shared with NFOLD)
   int64_t kmc_k;
   int64_t kmc_sum_sq, kmc_sum_dx, kmc_sum_dy;
+  // KMC pending event (pending_site >= 0): its direction and attempts; its
+  // time is in pending_dt
+  int64_t kmc_pdir, kmc_pk;
 };
 
 // Every active chain makes n_passes * n + n_steps attempts from where it stands.
@@ -149,12 +152,15 @@ inline int kmc_lds_bytes(int n) {
   return 3 * 2 * a + 3 * 4 * a;
 }
 
-// Every active chain makes n_passes * n + n_steps events from where it stands
-// (fewer if it freezes: ExactState::frozen).
+// Every active chain applies its pending event, if any, then makes events up
+// to n_passes * n + n_steps in all (fewer if it freezes: ExactState::frozen).
+// An event whose time reaches stop_time is left pending instead of applied
+// (the chain stops there); with select_at_end a chain that reached the count
+// selects one more event and leaves it pending.
 void launch_kmc(const KmcParams *P, ExactState *S, int8_t *lat_pool, int32_t *list_pool,
                 int32_t *atom_pool, mcg_sample *series, int64_t series_cap, const uint8_t *active,
                 int n_chains, int64_t n_passes, int64_t n_steps, int64_t sample_period,
-                int lds_bytes, hipStream_t stream);
+                double stop_time, int select_at_end, int lds_bytes, hipStream_t stream);
 hipError_t kmc_set_max_lds(int bytes);
 
 // ---------------------------------------------------------------------------

@@ -119,7 +119,8 @@ __global__ __launch_bounds__(128) void kmc_kernel(const KmcParams *__restrict__ 
                                                  int64_t series_cap,
                                                  const uint8_t *__restrict__ active,
                                                  int64_t n_passes, int64_t n_steps,
-                                                 int64_t sample_period) {
+                                                 int64_t sample_period, double stop_time,
+                                                 int select_at_end) {
   const int c = blockIdx.x;
   if (!active[c]) return;
   const int lane = threadIdx.x & 63;
@@ -189,6 +190,10 @@ __global__ __launch_bounds__(128) void kmc_kernel(const KmcParams *__restrict__ 
   const uint32_t draw_mask = P->draw_mask, full_mask = P->full_mask;
   const int64_t spin_limit = P->spin_limit;
   const int32_t pba = n - n_atoms;  // the atom list's last position (fixed: hops keep the count)
+  const int64_t target = n_passes * (int64_t)n + n_steps;
+  int64_t done_ev = 0;
+  int64_t pending_site = St->pending_site, pending_dir = St->kmc_pdir, pending_k = St->kmc_pk;
+  double pending_time = St->pending_dt;
 
   auto ensure = [&](int64_t upto) { eng.ensure(upto); };
   const uint64_t *ringf = &ring[0][0];
@@ -241,9 +246,48 @@ __global__ __launch_bounds__(128) void kmc_kernel(const KmcParams *__restrict__ 
     sum_sq += 2 * (int64_t)dx * ddx + 2 * (int64_t)dy * ddy + 1;
   };
 
-  const int64_t target = n_passes * (int64_t)n + n_steps;
-  int64_t done_ev = 0;
-  while (done_ev < target && n_atoms > 0) {
+  // a selected event: left pending (its site, direction, attempts and time
+  // kept for the next launch) or applied
+  bool stopped = false;
+  auto pend = [&](uint32_t from, int dir, int64_t k, double event_time) {
+    pending_site = from;
+    pending_dir = dir;
+    pending_k = k;
+    pending_time = event_time;
+    stopped = true;
+  };
+  auto bump = [&]() {
+    ++n_acc;
+    ++step;
+    ++done_ev;
+    if (step == n) {
+      step = 0;
+      ++n_pass;
+      if (period <= 0.0 && n_pass % sample_period == 0) record(n_pass, time);
+    }
+  };
+
+  // the event selected at the end of the previous launch comes first
+  if (pending_site >= 0 && n_atoms > 0) {
+    if (pending_time >= stop_time) {
+      stopped = true;
+    } else {
+      Attempt a0;
+      const uint32_t from = (uint32_t)pending_site;
+      kmc_eval(a0, (uint32_t)(n - 1 - (int32_t)pos[from]), pending_dir, n, list, site_atom, adx,
+               ady, rows, cols, magic);
+      account(pending_k, a0.cls, a0.dir, a0.dx, a0.dy);
+      if (lane == 0) kmc_apply(a0, pba, list, pos, site_atom, adx, ady, ajump);
+      asm volatile("" ::: "memory");
+      pending_site = -1;
+      bump();
+    }
+  }
+
+  while (!stopped && n_atoms > 0) {
+    // past the target: select one more event and leave it pending, if asked
+    const bool select_only = done_ev >= target;
+    if (select_only && !select_at_end) break;
     ensure(ps + W + 2);
     const bool inwin = lane < W;
     const int64_t q = inwin ? ps + lane : ps;
@@ -272,35 +316,35 @@ __global__ __launch_bounds__(128) void kmc_kernel(const KmcParams *__restrict__ 
       if (AC & bit) break;
       t += 2 + (int)((ND >> t) & 1ull);
     }
-    int attempts = __popcll(exec);
+    const int attempts = __popcll(exec);
     const uint64_t F = exec & AC;  // at most one: the last
-    // an executed event ends the attempts it closes; the count of events to
-    // the pass end and the target only limits executed ones
     int used = 0;
     if (exec) {
       const int last = 63 - __builtin_clzll(exec);
       used = last + 2 + (int)((ND >> last) & 1ull);
     }
     k_run += attempts;
+    n_rej += attempts - (F ? 1 : 0);
+    ps += used;
     if (F) {
       const int f = __builtin_ctzll(F);
       const int cls = __builtin_amdgcn_readlane(at.cls, f);
       const int dir = __builtin_amdgcn_readlane(at.dir, f);
       const int32_t dx = __builtin_amdgcn_readlane(at.dx, f), dy = __builtin_amdgcn_readlane(at.dy, f);
-      account(k_run, cls, dir, dx, dy);
-      if (lane == f) kmc_apply(at, pba, list, pos, site_atom, adx, ady, ajump);
-      asm volatile("" ::: "memory");
+      const uint32_t from = __builtin_amdgcn_readlane(at.from, f);
+      const double event_time = __dadd_rn(time, __ddiv_rn((double)k_run, R));
+      if (select_only || event_time >= stop_time) {
+        pend(from, dir, k_run, event_time);
+      } else {
+        account(k_run, cls, dir, dx, dy);
+        if (lane == f) kmc_apply(at, pba, list, pos, site_atom, adx, ady, ajump);
+        asm volatile("" ::: "memory");
+        bump();
+      }
       k_run = 0;
-      ++n_acc;
-      n_rej += attempts - 1;
-      ++step;
-      ++done_ev;
-    } else {
-      n_rej += attempts;
     }
-    ps += used;
 
-    if ((SP & 1ull) && attempts == 0) {
+    if (!stopped && (SP & 1ull) && attempts == 0) {
       // an attempt whose Lemire draw may reject, with the full rejection loop;
       // every lane computes it, lane 0 writes
       const uint32_t s1 = lemire_scalar(na);
@@ -320,22 +364,19 @@ __global__ __launch_bounds__(128) void kmc_kernel(const KmcParams *__restrict__ 
       // one wavefront walks: its LDS operations execute in issue order
       asm volatile("" ::: "memory");
       if (ok) {
-        account(k_run, a1.cls, a1.dir, a1.dx, a1.dy);
-        if (lane == 0) kmc_apply(a1, pba, list, pos, site_atom, adx, ady, ajump);
-        asm volatile("" ::: "memory");
+        const double event_time = __dadd_rn(time, __ddiv_rn((double)k_run, R));
+        if (select_only || event_time >= stop_time) {
+          pend(a1.from, a1.dir, k_run, event_time);
+        } else {
+          account(k_run, a1.cls, a1.dir, a1.dx, a1.dy);
+          if (lane == 0) kmc_apply(a1, pba, list, pos, site_atom, adx, ady, ajump);
+          asm volatile("" ::: "memory");
+          bump();
+        }
         k_run = 0;
-        ++n_acc;
-        ++step;
-        ++done_ev;
       } else {
         ++n_rej;
       }
-    }
-
-    if (step == n) {
-      step = 0;
-      ++n_pass;
-      if (period <= 0.0 && n_pass % sample_period == 0) record(n_pass, time);
     }
     // no event in spin_limit attempts (every move of the state has a
     // vanishing probability): stop instead of spinning
@@ -380,16 +421,20 @@ __global__ __launch_bounds__(128) void kmc_kernel(const KmcParams *__restrict__ 
     St->kmc_sum_dx = sum_dx;
     St->kmc_sum_dy = sum_dy;
     St->frozen = frozen;
+    St->pending_site = pending_site;
+    St->pending_dt = pending_time;
+    St->kmc_pdir = pending_dir;
+    St->kmc_pk = pending_k;
   }
 }
 
 void launch_kmc(const KmcParams *P, ExactState *S, int8_t *lat_pool, int32_t *list_pool,
                 int32_t *atom_pool, mcg_sample *series, int64_t series_cap, const uint8_t *active,
                 int n_chains, int64_t n_passes, int64_t n_steps, int64_t sample_period,
-                int lds_bytes, hipStream_t stream) {
+                double stop_time, int select_at_end, int lds_bytes, hipStream_t stream) {
   hipLaunchKernelGGL(kmc_kernel, dim3(n_chains), dim3(128), lds_bytes, stream, P, S, lat_pool,
                      list_pool, atom_pool, series, series_cap, active, n_passes, n_steps,
-                     sample_period);
+                     sample_period, stop_time, select_at_end);
 }
 
 hipError_t kmc_set_max_lds(int bytes) {

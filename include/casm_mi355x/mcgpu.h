@@ -284,6 +284,17 @@ int mcg_set_occ_location(mcg_handle *h, int64_t chain, const int64_t *loc, int64
 int mcg_get_atoms(mcg_handle *h, int64_t chain, int64_t *atoms, int64_t n_atoms);
 /* KMC: attempts without an event after which a chain stops (default 2^26). */
 int mcg_kmc_set_spin_limit(mcg_handle *h, int64_t attempts);
+/* KMC: the reference loop's select-then-sample order (kinetic_monte_carlo.hh:
+ * 185-226).  Every active chain applies its pending event, if any, then
+ * selects and applies events until it has applied max_events in this call;
+ * an event whose time reaches stop_time is selected but left pending (the
+ * chain stops there), and with select_at_end a chain that applied max_events
+ * selects one more and leaves it pending.  A pending event's site, and its
+ * time (in place of the increment), are read with mcg_get_nfold_state; the
+ * next mcg_run, mcg_run_steps or mcg_kmc_advance applies it first (and
+ * counts it).  The drop-in kinetic_monte_carlo() samples by count and by time
+ * between the selection and the application, as the reference does. */
+int mcg_kmc_advance(mcg_handle *h, int64_t max_events, double stop_time, int64_t select_at_end);
 
 /* OCCUPANT: the asymmetric unit of every site of one chain (n = sites; the
  * reference's Conversions::l_to_asym).  Resets the chain's occupation to
@@ -369,8 +380,9 @@ int mcg_n_samples(mcg_handle *h, int64_t chain, int64_t *n_samples);
  * between selecting an event and applying it; the drop-in nfold() reads them
  * when a completion check stops the run right after a sample, where the
  * reference applies that event before it stops.  KMC: the loop's time
- * (kinetic_monte_carlo.hh:218, the time of the last event); no event is ever
- * pending (site -1). */
+ * (kinetic_monte_carlo.hh:218, the time of the last applied event), the
+ * origin site of the pending event left by mcg_kmc_advance (-1: none) and, in
+ * pending_dt, that event's time. */
 int mcg_get_nfold_state(mcg_handle *h, int64_t chain, double *time, int64_t *pending_site,
                         double *pending_dt);
 

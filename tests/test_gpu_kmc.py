@@ -174,3 +174,53 @@ def test_no_hop_possible_is_refused():
             r.run(1)
     with pytest.raises(M.McgError, match="3x3"):
         M.Run([dict(shape=(2, 5), temperature=1000.0)], mode=M.MODE_KMC, J1=0.05)
+
+
+def test_advance_selects_then_applies():
+    # mcg_kmc_advance: the reference loop's order -- an event is selected (its
+    # words drawn) before the samples, and applied after them
+    rows, cols, J, T, seed = 7, 9, 0.06, 900.0, 13
+    n = rows * cols
+    occ = _occ(n, 0.4, seed)
+    na = int(occ.sum())
+
+    def ref(k, **kw):
+        return O.kmc_run(rows, cols, J, T, k, occ, O.mt64_state(seed, 0), **kw)
+
+    with M.Run([dict(shape=(rows, cols), temperature=T, seed=seed)], mode=M.MODE_KMC, J1=J,
+               sample_period=1 << 40) as r:
+        r.set_occupation(0, occ)
+        r.kmc_advance(10, select_at_end=True)
+        t, site, t_pending = r.nfold_state(0)
+        o10, o11 = ref(10), ref(11)
+        assert np.array_equal(r.get_occupation(0), o10["occ"])
+        assert np.array_equal(r.rng_dump(0), o11["engine"])  # the 11th event's words are drawn
+        assert t == o10["final_time"] and t_pending == o11["final_time"]
+        moved_from = np.flatnonzero((o10["occ"] == 1) & (o11["occ"] == 0))
+        assert list(moved_from) == [site]
+        r.kmc_advance(5)  # the pending event, then four more
+        o15 = ref(15)
+        g = _gpu_state(r, 0, na)
+        assert np.array_equal(g["occ"], o15["occ"]) and np.array_equal(g["atoms"], o15["atoms"])
+        assert np.array_equal(g["engine"], o15["engine"]) and g["time"] == o15["final_time"]
+        assert g["pending"] == -1 and g["position"][1] == 15
+        # stop at a time: the event that reaches it is selected, not applied
+        t_stop = o15["final_time"] + 3.0
+        r.kmc_advance(1 << 40, stop_time=t_stop)
+        grid = O.kmc_run(rows, cols, J, T, 5000, occ, O.mt64_state(seed, 0), time_period=t_stop)
+        k = int(grid["events"][1])  # events applied before the one reaching t_stop
+        ok, ok1 = ref(k), ref(k + 1)
+        t, site, t_pending = r.nfold_state(0)
+        assert np.array_equal(r.get_occupation(0), ok["occ"])
+        assert np.array_equal(r.rng_dump(0), ok1["engine"])
+        assert t == ok["final_time"] < t_stop <= t_pending == ok1["final_time"]
+        assert site >= 0
+        # an advance whose stop time the pending event reaches does nothing
+        r.kmc_advance(100, stop_time=t_stop)
+        assert np.array_equal(r.get_occupation(0), ok["occ"])
+        # an ordinary run applies it first
+        r.run_steps(7)
+        o = ref(k + 7)
+        assert np.array_equal(r.get_occupation(0), o["occ"])
+        assert np.array_equal(r.rng_dump(0), o["engine"])
+        assert r.nfold_state(0)[0] == o["final_time"]
