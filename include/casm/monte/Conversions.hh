@@ -12,6 +12,8 @@
 #ifndef CASM_MI355X_monte_Conversions
 #define CASM_MI355X_monte_Conversions
 
+#include <array>
+
 #include "casm/monte/definitions.hh"
 
 namespace CASM {
@@ -23,6 +25,15 @@ class Conversions {
   /// names) in this order
   Conversions(Index n_sites, std::vector<std::string> occupants)
       : Conversions(std::vector<Index>(n_sites < 0 ? 0 : n_sites, 0), {std::move(occupants)}) {}
+
+  /// a rows x cols square lattice (site l = i + rows j, lattice constant 1)
+  /// of one asymmetric unit allowing `occupants`; it knows its sites'
+  /// coordinates (l_to_ijk), as the kinetic Monte Carlo atom tracking needs
+  Conversions(Index rows, Index cols, std::vector<std::string> occupants)
+      : Conversions(rows * cols, std::move(occupants)) {
+    m_rows = rows;
+    m_cols = cols;
+  }
 
   /// site l in unit l_to_asym[l]; unit b allows asym_occupants[b] in this order
   Conversions(std::vector<Index> l_to_asym, std::vector<std::vector<std::string>> asym_occupants)
@@ -67,8 +78,20 @@ class Conversions {
     return species_size();
   }
   std::string const &species_name(Index species_index) const { return m_species.at(species_index); }
+  /// atoms of a species: none for a vacancy ("Va"), one otherwise (the
+  /// lattice models here have single-atom occupants)
+  Index components_size(Index species_index) const {
+    return species_name(species_index) == "Va" ? 0 : 1;
+  }
+  /// unit cell of site l of a square-lattice Conversions (its third index 0)
+  std::array<long, 3> l_to_ijk(Index l) const {
+    if (m_rows <= 0)
+      throw std::runtime_error("Conversions::l_to_ijk: the lattice shape is not known");
+    return {{(long)(l % m_rows), (long)(l / m_rows), 0}};
+  }
 
  private:
+  Index m_rows = 0, m_cols = 0;
   std::vector<Index> m_l_to_asym;
   std::vector<std::vector<Index>> m_asym_occ;  // species index of each occupant, per unit
   std::vector<std::string> m_species;

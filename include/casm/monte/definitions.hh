@@ -112,6 +112,30 @@ class VectorXd {
   std::vector<double> m_v;
 };
 
+/// Minimal dense column-major matrix of doubles with the Eigen::MatrixXd
+/// operations the kinetic Monte Carlo data uses (atom positions, 3 x n).
+class MatrixXd {
+ public:
+  MatrixXd() = default;
+  MatrixXd(Index rows, Index cols, double value = 0.0)
+      : m_rows(rows), m_cols(cols), m_v(static_cast<size_t>(rows * cols), value) {}
+  static MatrixXd Zero(Index rows, Index cols) { return MatrixXd(rows, cols, 0.0); }
+  Index rows() const { return m_rows; }
+  Index cols() const { return m_cols; }
+  double &operator()(Index i, Index j) { return m_v[static_cast<size_t>(i + m_rows * j)]; }
+  double operator()(Index i, Index j) const { return m_v[static_cast<size_t>(i + m_rows * j)]; }
+  VectorXd col(Index j) const {
+    return VectorXd(std::vector<double>(m_v.begin() + m_rows * j, m_v.begin() + m_rows * (j + 1)));
+  }
+  bool operator==(MatrixXd const &o) const {
+    return m_rows == o.m_rows && m_cols == o.m_cols && m_v == o.m_v;
+  }
+
+ private:
+  Index m_rows = 0, m_cols = 0;
+  std::vector<double> m_v;
+};
+
 typedef std::vector<Index> Shape;
 
 }  // namespace monte

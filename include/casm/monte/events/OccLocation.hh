@@ -10,10 +10,15 @@
 // slot, then is appended to the list of its new species.  The order of the
 // lists decides which site later draws pick, so it is kept exactly.
 //
-// Not supported here: atom tracking (update_atoms, unique atom ids
-// and atom info; only kinetic Monte Carlo with molecules uses them).  Added
-// for moving the lists to and from the device: locations() and
-// set_locations().
+// Atom tracking (update_atoms, OccLocation.cc:68-106, 216-330): every
+// non-vacancy occupant holds one atom, created at initialize in Mol order with
+// its site's unit cell as translation; apply moves the atoms of the event's
+// trajectories, adding delta_ijk to their translations and counting their
+// jumps; atom_positions_cart() gives the positions on the unit square lattice
+// of a Conversions that knows its shape.  Not supported here: unique atom ids
+// and atom info (track_unique_atoms, save_atom_info), atoms moving to or from
+// a reservoir.  Added for moving the lists to and from the device:
+// locations(), set_locations() and set_atoms().
 #ifndef CASM_MI355X_monte_OccLocation
 #define CASM_MI355X_monte_OccLocation
 
@@ -32,15 +37,20 @@ class OccLocation {
   OccLocation(Conversions const &_convert, OccCandidateList const &_candidate_list,
               bool _update_atoms = false, bool _track_unique_atoms = false,
               bool _save_atom_info = false)
-      : m_convert(_convert), m_candidate_list(_candidate_list), m_loc(_candidate_list.size()) {
-    if (_update_atoms || _track_unique_atoms || _save_atom_info)
-      throw std::runtime_error("OccLocation (MI355X build): atom tracking is not supported");
+      : m_convert(_convert),
+        m_candidate_list(_candidate_list),
+        m_loc(_candidate_list.size()),
+        m_update_atoms(_update_atoms) {
+    if (_track_unique_atoms || _save_atom_info)
+      throw std::runtime_error(
+          "OccLocation (MI355X build): unique atom ids and atom info are not supported");
   }
 
   /// Fill the Mol list and the candidate lists from an occupation (occupant indices)
   void initialize(std::vector<int> const &occupation, std::optional<double> time = std::nullopt) {
     m_mol.clear();
     m_l_to_mol.clear();
+    m_atoms.clear();
     for (auto &v : m_loc) v.clear();
     Index n_mut = 0;
     for (Index l = 0; l < (Index)occupation.size(); ++l)
@@ -58,6 +68,15 @@ class OccLocation {
         mol.asym = asym;
         mol.species_index = species;
         mol.loc = (Index)m_loc[cand].size();
+        mol.component.clear();
+        if (m_update_atoms)
+          for (Index k = 0; k < m_convert.components_size(species); ++k) {
+            mol.component.push_back((Index)m_atoms.size());
+            Atom atom;
+            atom.translation = m_convert.l_to_ijk(l);
+            atom.n_jumps = 0;
+            m_atoms.push_back(atom);
+          }
         m_loc[cand].push_back(mol_id);
         m_l_to_mol.push_back(mol_id);
         ++mol_id;
@@ -70,6 +89,15 @@ class OccLocation {
   /// Update the occupation and the lists for an event that occurred
   void apply(OccEvent const &e, std::vector<int> &occupation,
              std::optional<double> time = std::nullopt) {
+    // the moving atoms, read before the transforms resize the Mols' components
+    std::vector<Index> moving;
+    if (m_update_atoms)
+      for (auto const &traj : e.atom_traj) {
+        if (traj.from.l == -1 || traj.to.l == -1)
+          throw std::runtime_error(
+              "OccLocation (MI355X build): atoms moving to or from a reservoir are not supported");
+        moving.push_back(m_mol[traj.from.mol_id].component[traj.from.mol_comp]);
+      }
     for (auto const &t : e.occ_transform) {
       Mol &mol = m_mol[t.mol_id];
       if (mol.species_index != t.from_species)
@@ -81,10 +109,44 @@ class OccLocation {
       m_mol[back].loc = mol.loc;
       m_loc[cand].pop_back();
       mol.species_index = t.to_species;
+      if (m_update_atoms) mol.component.resize(m_convert.components_size(mol.species_index));
       cand = m_candidate_list.index(mol.asym, mol.species_index);
       mol.loc = (Index)m_loc[cand].size();
       m_loc[cand].push_back(mol.id);
     }
+    if (m_update_atoms)
+      for (size_t i = 0; i < e.atom_traj.size(); ++i) {
+        auto const &traj = e.atom_traj[i];
+        const Index id = moving[i];
+        m_mol[traj.to.mol_id].component[traj.to.mol_comp] = id;
+        for (int d = 0; d < 3; ++d) m_atoms[id].translation[d] += traj.delta_ijk[d];
+        m_atoms[id].n_jumps += 1;
+      }
+  }
+
+  /// Number of tracked atoms and access by atom id
+  Index atom_size() const { return (Index)m_atoms.size(); }
+  Atom &atom(Index atom_id) { return m_atoms[atom_id]; }
+  Atom const &atom(Index atom_id) const { return m_atoms[atom_id]; }
+
+  /// Atom positions, 3 x atom_size(), as if there were no periodic boundary
+  /// (OccLocation.cc:385-403 on the unit square lattice: one site per unit
+  /// cell at its origin, point atoms)
+  MatrixXd atom_positions_cart() const {
+    MatrixXd R = MatrixXd::Zero(3, atom_size());
+    for (Index a = 0; a < atom_size(); ++a)
+      for (int d = 0; d < 3; ++d) R(d, a) = (double)m_atoms[a].translation[d];
+    return R;
+  }
+
+  /// Replace the atoms and the Mols' components (the device's atom records at
+  /// the end of a kinetic run): component[mol] holds the atom ids of each Mol
+  void set_atoms(std::vector<Atom> const &atoms,
+                 std::vector<std::vector<Index>> const &component) {
+    if (component.size() != m_mol.size())
+      throw std::runtime_error("OccLocation::set_atoms: one component list per Mol");
+    m_atoms = atoms;
+    for (size_t i = 0; i < m_mol.size(); ++i) m_mol[i].component = component[i];
   }
 
   template <typename GeneratorType>
@@ -157,6 +219,8 @@ class OccLocation {
   std::vector<std::vector<Index>> m_loc;
   std::vector<Mol> m_mol;
   std::vector<Index> m_l_to_mol;
+  bool m_update_atoms = false;
+  std::vector<Atom> m_atoms;
 };
 
 }  // namespace monte

@@ -80,6 +80,17 @@ class Handle {
   /// As run / run_steps, returning the C ABI's code instead of throwing
   int try_run(CountType n_passes) { return mcg_run(m_h, n_passes); }
   int try_run_steps(CountType n_steps) { return mcg_run_steps(m_h, n_steps); }
+  /// KMC: apply the pending event and events up to max_events; an event
+  /// reaching stop_time stays pending; select_at_end leaves one more selected
+  void kmc_advance(CountType max_events, double stop_time, bool select_at_end) {
+    check(mcg_kmc_advance(m_h, max_events, stop_time, select_at_end ? 1 : 0));
+  }
+  /// KMC: per atom (site, displacement along rows, along columns, jumps)
+  std::vector<int64_t> atoms(Index chain, Index n_atoms) const {
+    std::vector<int64_t> a(4 * n_atoms);
+    check(mcg_get_atoms(m_h, chain, a.data(), n_atoms));
+    return a;
+  }
   /// OCCUPANT: the asymmetric unit of every site (resets occupation and lists)
   void set_site_asym(Index chain, std::vector<int8_t> const &asym) {
     check(mcg_set_site_asym(m_h, chain, asym.data(), (int64_t)asym.size()));

@@ -270,13 +270,19 @@ class SamplingFixture {
     m_counter.reset(m_params.sampling_params.sample_mode, steps_per_pass);
     m_completion_check.reset();
     m_results.reset();
-    if (m_params.sampling_params.sample_mode == SAMPLE_MODE::BY_TIME)
-      throw std::runtime_error("SamplingFixture: SAMPLE_MODE::BY_TIME is not supported by "
-                               "the GPU loops");
-    m_next_sample_count =
-        static_cast<CountType>(std::round(sample_at(m_results.sample_count.size())));
-    if (m_next_sample_count < 0)
-      throw std::runtime_error("Error: sampling period parameter error, next_sample_count < 0");
+    // SamplingFixture.hh:219-235
+    if (m_params.sampling_params.sample_mode == SAMPLE_MODE::BY_TIME) {
+      m_next_sample_count = 0;
+      m_next_sample_time = sample_at(m_results.sample_time.size());
+      if (m_next_sample_time < 0.0)
+        throw std::runtime_error("Error: sampling period parameter error, next_sample_time < 0.0");
+    } else {
+      m_next_sample_time = 0.0;
+      m_next_sample_count =
+          static_cast<CountType>(std::round(sample_at(m_results.sample_count.size())));
+      if (m_next_sample_count < 0)
+        throw std::runtime_error("Error: sampling period parameter error, next_sample_count < 0");
+    }
     Log &log = m_params.method_log.log;
     log.restart_clock();
     log.begin_lap();
@@ -324,6 +330,7 @@ class SamplingFixture {
   void increment_n_reject() { m_counter.increment_n_reject(); }
   void increment_step() { m_counter.increment_step(); }
   CountType next_sample_count() const { return m_next_sample_count; }
+  TimeType next_sample_time() const { return m_next_sample_time; }
 
   /// Weight of the next sample (time increment of the N-fold way's event
   /// selected at the sample; SamplingFixture.hh:330-332)
@@ -341,15 +348,25 @@ class SamplingFixture {
     for (auto const &name : m_params.sampling_params.json_sampler_names)
       m_results.json_samplers.at(name)->values.push_back(
           m_params.json_sampling_functions.at(name)());
-    m_next_sample_count =
-        static_cast<CountType>(std::round(sample_at(m_results.sample_count.size())));
-    if (m_next_sample_count <= m_counter.count)
-      throw std::runtime_error(
-          "Error: state sampling period parameter error, next_sample_count <= current count");
+    // SamplingFixture.hh:516-538
+    if (m_params.sampling_params.sample_mode == SAMPLE_MODE::BY_TIME) {
+      m_next_sample_time = sample_at(m_results.sample_time.size());
+      if (m_next_sample_time <= m_counter.time)
+        throw std::runtime_error(
+            "Error: state sampling period parameter error, next_sample_time <= current time");
+    } else {
+      m_next_sample_count =
+          static_cast<CountType>(std::round(sample_at(m_results.sample_count.size())));
+      if (m_next_sample_count <= m_counter.count)
+        throw std::runtime_error(
+            "Error: state sampling period parameter error, next_sample_count <= current count");
+    }
   }
 
   void sample_data_by_count_if_due(state_type const &state) {
-    if (m_counter.count == m_next_sample_count) sample_data(state);
+    if (m_params.sampling_params.sample_mode != SAMPLE_MODE::BY_TIME &&
+        m_counter.count == m_next_sample_count)
+      sample_data(state);
   }
 
   /// SamplingFixture.hh:570-588: a stochastic period draws the next sample's
@@ -384,6 +401,7 @@ class SamplingFixture {
   bool m_is_complete = false;
   MonteCounter m_counter;
   CountType m_next_sample_count = 0;
+  TimeType m_next_sample_time = 0.0;
   CompletionCheck<stats_type> m_completion_check;
   Results<config_type, stats_type> m_results;
 };
@@ -460,6 +478,37 @@ struct RunManager {
         post_sample_f(*f, state);
       }
   }
+  /// Sample every fixture due by time, earliest first, at its sample time
+  /// (RunManager.hh:190-209): the fixture's time is set to its sample time
+  /// before it samples.
+  template <typename PreSampleActionType = NullAction, typename PostSampleActionType = NullAction>
+  void sample_data_by_time_if_due(TimeType event_time, state_type const &state,
+                                  PreSampleActionType pre_sample_f = PreSampleActionType(),
+                                  PostSampleActionType post_sample_f = PostSampleActionType()) {
+    while (next_sampling_fixture != nullptr && event_time >= next_sample_time) {
+      auto &fixture = *next_sampling_fixture;
+      pre_sample_f(fixture, state);
+      fixture.set_time(next_sample_time);
+      fixture.sample_data(state);
+      post_sample_f(fixture, state);
+      update_next_sampling_fixture();
+    }
+  }
+
+  /// The BY_TIME fixture that samples next and its sample time
+  /// (RunManager.hh:211-225)
+  void update_next_sampling_fixture() {
+    next_sampling_fixture = nullptr;
+    for (auto &f : sampling_fixtures)
+      if (f->params().sampling_params.sample_mode == SAMPLE_MODE::BY_TIME &&
+          (next_sampling_fixture == nullptr || f->next_sample_time() < next_sample_time)) {
+        next_sample_time = f->next_sample_time();
+        next_sampling_fixture = f.get();
+      }
+  }
+  sampling_fixture_type *next_sampling_fixture = nullptr;
+  TimeType next_sample_time = 0.0;
+
   void finalize(state_type const &final_state) {
     for (auto &f : sampling_fixtures) f->finalize(final_state, run_index);
   }

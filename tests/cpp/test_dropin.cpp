@@ -17,6 +17,7 @@
 
 #include "casm/monte/ising_cpp/basic_semigrand_canonical.hh"
 #include "casm/monte/ising_cpp/complete_semigrand_canonical.hh"
+#include "casm/monte/methods/kinetic_monte_carlo.hh"
 #include "casm/monte/methods/nfold.hh"
 #include "casm/monte/methods/occupation_metropolis.hh"
 #include "casm/monte/run_management/io/json/jsonResultsIO.hh"
@@ -1379,6 +1380,164 @@ static void gpu_occupant_no_events() {
             << (g_fail ? " (failures so far)" : " ok") << std::endl;
 }
 
+// kinetic_monte_carlo() on the MI355X (KMC mode) vs refcpu::kmc_run: a fixture
+// sampling by pass (completing the run at its max_count) and one sampling by
+// time, with sampling functions that read the configuration and KMCData
+static void gpu_kmc_case(uint64_t seed) {
+  const int R = 10, C = 9, N = R * C;
+  const long P = 12;
+  const double J = 0.05, T = 900.0, tau = 0.75;
+  typedef ising_cpp::IsingConfiguration config_type;
+  typedef RunManager<config_type, BasicStatistics, std::mt19937_64> rm_type;
+  std::vector<int> occ0(N, 0);
+  {
+    std::mt19937_64 e(seed);
+    for (auto &o : occ0) o = (e() % 100) < 35 ? 1 : 0;
+  }
+  config_type cfg({R, C}, 0);
+  cfg.set_occupation(occ0);
+  ValueMap cond;
+  cond.scalar_values["temperature"] = T;
+  State<config_type> state(cfg, cond);
+  Conversions cv(R, C, {"Va", "A"});
+  OccCandidateList cl(cv);
+  OccLocation ol(cv, cl, true);
+  ol.initialize(state.configuration.occupation());
+  const Index n_atoms = ol.atom_size();
+  auto engine = std::make_shared<std::mt19937_64>(seed);
+  ising_cpp::LatticeGasKmcEventSelector selector(J, T, engine, &state, &ol);
+  KMCData<config_type, BasicStatistics, std::mt19937_64> kd;
+  MatrixXd R0 = ol.atom_positions_cart();
+
+  StateSamplingFunctionMap fns;
+  fns.emplace("S", StateSamplingFunction("S", "bond sum of the atoms", {}, [&]() {
+                auto const &c = state.configuration;
+                double s = 0.0;
+                for (Index l = 0; l < N; ++l)
+                  if (c.occ(l))
+                    s += c.occ(ising_cpp::lattice_gas_neighbour(l, 0, R, C)) +
+                         c.occ(ising_cpp::lattice_gas_neighbour(l, 2, R, C));
+                return VectorXd{s};
+              }));
+  fns.emplace("sum_sq", StateSamplingFunction("sum_sq", "sum of squared displacements", {}, [&]() {
+                double s = 0.0;
+                for (Index a = 0; a < kd.atom_positions_cart.cols(); ++a)
+                  for (int k = 0; k < 3; ++k) {
+                    const double d = kd.atom_positions_cart(k, a) - R0(k, a);
+                    s += d * d;
+                  }
+                return VectorXd{s};
+              }));
+  fns.emplace("time", StateSamplingFunction("time", "KMC time", {}, [&]() { return VectorXd{kd.time}; }));
+  SamplingParams by_pass;
+  by_pass.sampler_names = {"S", "sum_sq", "time"};
+  by_pass.begin = 0.0;
+  CompletionCheckParams<BasicStatistics> cc_pass;
+  cc_pass.cutoff_params.max_count = P;
+  SamplingParams by_time = by_pass;
+  by_time.sample_mode = SAMPLE_MODE::BY_TIME;
+  by_time.period = tau;
+  by_time.do_sample_time = true;
+  std::vector<rm_type::sampling_fixture_params_type> fixtures;
+  fixtures.emplace_back("by_pass", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), by_pass,
+                        cc_pass, std::vector<std::string>{});
+  fixtures.emplace_back("by_time", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), by_time,
+                        CompletionCheckParams<BasicStatistics>(), std::vector<std::string>{});
+  rm_type rm(engine, fixtures, true);
+  kinetic_monte_carlo<ising_cpp::LatticeGasHop>(
+      state, ol, kd, selector,
+      [&](ising_cpp::LatticeGasHop h) -> OccEvent const & { return selector.get_event(h); }, rm);
+
+  std::mt19937_64 e1(seed), e2(seed);
+  auto o = refcpu::kmc_run(R, C, occ0, J, T, e1, P * N);
+  auto og = refcpu::kmc_run(R, C, occ0, J, T, e2, P * N, tau);
+  std::string tag = "kinetic_monte_carlo seed " + std::to_string(seed);
+  auto const &rp = rm.sampling_fixtures[0]->results();
+  VectorXd S = rp.samplers.at("S")->component(0), sq = rp.samplers.at("sum_sq")->component(0),
+           tm = rp.samplers.at("time")->component(0);
+  bool ok = S.size() == P;
+  for (long c = 1; ok && c < P; ++c)
+    ok = S(c) == (double)o.records[c - 1].S && sq(c) == (double)o.records[c - 1].sum_sq &&
+         same_bits(tm(c), o.records[c - 1].time);
+  ok = ok && sq(0) == 0.0 && tm(0) == 0.0;
+  EXPECT(ok, tag << ": by-pass samples (" << S.size() << ")");
+  auto const &rt = rm.sampling_fixtures[1]->results();
+  VectorXd St = rt.samplers.at("S")->component(0), sqt = rt.samplers.at("sum_sq")->component(0),
+           tt = rt.samplers.at("time")->component(0);
+  bool okt = St.size() == (Index)og.records.size() && rt.sample_time.size() == og.records.size();
+  for (Index k = 0; okt && k < St.size(); ++k)
+    okt = St(k) == (double)og.records[k].S && sqt(k) == (double)og.records[k].sum_sq &&
+          same_bits(tt(k), og.records[k].time) && same_bits(rt.sample_time[k], og.records[k].time);
+  EXPECT(okt, tag << ": by-time samples (" << St.size() << " vs " << og.records.size() << ")");
+  EXPECT(state.configuration.occupation() == o.final_occ, tag << ": final configuration");
+  EXPECT(*engine == e1, tag << ": engine");
+  bool atoms_ok = n_atoms == (Index)o.atoms.size();
+  for (Index a = 0; atoms_ok && a < n_atoms; ++a) {
+    auto const &at = ol.atom(a);
+    atoms_ok = at.translation[0] - (long)R0(0, a) == o.atoms[a].dx &&
+               at.translation[1] - (long)R0(1, a) == o.atoms[a].dy && at.n_jumps == o.atoms[a].n_jumps &&
+               ol.mol(ol.l_to_mol_id(o.atoms[a].site)).component == std::vector<Index>{a};
+  }
+  EXPECT(atoms_ok, tag << ": atoms (translations, jumps, components)");
+  bool lists_ok = ol.locations()[0] == std::vector<Index>(o.ol.loc[0].begin(), o.ol.loc[0].end()) &&
+                  ol.locations()[1] == std::vector<Index>(o.ol.loc[1].begin(), o.ol.loc[1].end());
+  EXPECT(lists_ok, tag << ": OccLocation lists");
+  EXPECT(rp.n_accept == P * N, tag << ": events counted as accepted");
+  std::cout << "gpu: " << tag << (g_fail ? " (failures so far)" : " ok") << std::endl;
+}
+
+// the drop-in OccLocation's atom tracking against the oracle's, over the
+// selector's own events applied on the host
+static void cpu_kmc_atom_tracking() {
+  const int R = 6, C = 7, N = R * C;
+  std::vector<int> occ(N, 0);
+  for (int l = 0; l < N; l += 3) occ[l] = 1;
+  typedef ising_cpp::IsingConfiguration config_type;
+  config_type cfg({R, C}, 0);
+  cfg.set_occupation(occ);
+  ValueMap cond;
+  cond.scalar_values["temperature"] = 1000.0;
+  State<config_type> state(cfg, cond);
+  Conversions cv(R, C, {"Va", "A"});
+  OccCandidateList cl(cv);
+  OccLocation ol(cv, cl, true);
+  ol.initialize(state.configuration.occupation());
+  ising_cpp::LatticeGasKmcEventSelector sel(0.05, 1000.0, std::make_shared<std::mt19937_64>(1),
+                                            &state, &ol);
+  std::mt19937_64 e(5);
+  std::vector<std::array<long, 2>> disp(ol.atom_size(), {{0, 0}});
+  std::vector<Index> where;
+  for (Index l = 0; l < N; ++l)
+    if (occ[l]) where.push_back(l);
+  int applied = 0;
+  for (int k = 0; k < 3000; ++k) {
+    const Index from = where[e() % where.size()];
+    const int dir = (int)(e() % 4);
+    const Index to = ising_cpp::lattice_gas_neighbour(from, dir, R, C);
+    if (state.configuration.occ(to)) continue;
+    ol.apply(sel.get_event(ising_cpp::LatticeGasHop{from, dir}), state.configuration.occupation());
+    const Index a = std::find(where.begin(), where.end(), from) - where.begin();
+    where[a] = to;
+    disp[a][0] += dir == 0 ? 1 : dir == 1 ? -1 : 0;
+    disp[a][1] += dir == 2 ? 1 : dir == 3 ? -1 : 0;
+    ++applied;
+  }
+  bool ok = applied > 500 && occ_location_consistent(ol, state.configuration.occupation());
+  MatrixXd Rn = ol.atom_positions_cart();
+  Index a0 = 0;
+  for (Index l = 0; l < N; ++l)
+    if (occ[l]) {
+      ok = ok && Rn(0, a0) == (double)(l % R + disp[a0][0]) && Rn(1, a0) == (double)(l / R + disp[a0][1]) &&
+           ol.mol(ol.l_to_mol_id(where[a0])).component == std::vector<Index>{a0};
+      ++a0;
+    }
+  EXPECT(ok, "OccLocation atom tracking over " << applied << " hops");
+  EXPECT(sel.total_rate() > 0.0, "lattice gas total rate");
+  std::cout << "cpu: OccLocation atom tracking" << (g_fail ? " (failures so far)" : " ok") << std::endl;
+}
+
 static void gpu_tests() {
   gpu_complete_case(false, 42, 1);
   gpu_complete_case(true, 44, 2);
@@ -1400,6 +1559,8 @@ static void gpu_tests() {
   gpu_occupant_case(true, 2200.0, 43);
   gpu_occupant_case(false, 400.0, 45);
   gpu_occupant_no_events();
+  gpu_kmc_case(61);
+  gpu_kmc_case(62);
   gpu_nfold_case(false);
   gpu_nfold_case(true);
   gpu_nfold_two_fixtures();
@@ -1414,6 +1575,7 @@ int main(int argc, char **argv) {
       cpu_tests();
       cpu_canonical_tests();
       cpu_occupant_tests();
+      cpu_kmc_atom_tracking();
       cpu_nfold_argument_tests();
     }
     if (gpu) gpu_tests();
