@@ -315,12 +315,76 @@ def bench_nfold(passes):
     import numpy as np
     import oracle as O
     cp = 5
+    n = 128 * 128
     tc = time.perf_counter()
     O.nfold_run(128, 128, 0.1, conds[0]["temperature"], conds[0]["exchange_potential"], cp,
                 np.ones(n, dtype=np.int8), O.mt64_state(1000, 0))
     cw = time.perf_counter() - tc
     out["cpu_port"] = dict(events_per_s=cp * n / cw, cores=1,
                            sample=f"1 chain 128x128, {cp} passes, oracle restatement")
+    return out
+
+
+def kmc_conditions(shape=(64, 64)):
+    """Lattice-gas KMC scan: 8 temperatures (500-3000 K) x 8 atom fractions
+    (0.1-0.8), seeds 5000+i, J = 0.05 eV, random initial occupations."""
+    import numpy as np
+    n = shape[0] * shape[1]
+    conds, occs = [], []
+    for a in range(8):
+        for b in range(8):
+            i = 8 * a + b
+            conds.append(dict(shape=shape, temperature=500.0 + a * 2500.0 / 7.0, seed=5000 + i))
+            occs.append((np.random.default_rng(5000 + i).random(n) < 0.1 * (b + 1))
+                        .astype(np.int8))
+    return conds, occs
+
+
+def bench_kmc(passes):
+    """Kinetic Monte Carlo with atom tracking (KMC mode): the 64-condition
+    lattice-gas scan (64x64, 8 T x 8 atom fractions) in one launch vs one chain
+    alone, in events/s (one pass = 4096 events per chain)."""
+    import casmcode_monte_amd as cm
+    import numpy as np
+    conds, occs = kmc_conditions()
+    n = 64 * 64
+
+    def make(cs, os_):
+        r = cm.Run(cs, mode=cm.MODE_KMC, J1=0.05)
+        for c, o in enumerate(os_):
+            r.set_occupation(c, o)
+        r.run(1)
+        return r
+
+    r = make(conds, occs)
+    w, k, ev = timed_run(r, passes)
+    att = sum(sum(r.counters(c)[1:]) for c in range(len(conds)))
+    out = dict(chains=len(conds), passes=passes, wall_s=w, kernel_s=k, events=ev,
+               events_per_s=ev / w, attempts_total=att,
+               note="attempts_total counts every attempt since the occupation was set "
+                    "(one warm-up pass included)")
+    r.close()
+    reps = [dict(c, seed=c["seed"] + 100000 * j) for j in range(4) for c in conds]
+    f = make(reps, occs * 4)
+    fp = max(passes // 2, 5)
+    fw, fk, fe = timed_run(f, fp)
+    out["full_chip"] = dict(chains=len(reps), passes=fp, wall_s=fw, kernel_s=fk, events=fe,
+                            events_per_s=fe / fw, note="4 seeds of each of the 64 conditions")
+    f.close()
+    p = make(conds[:1], occs[:1])
+    pp = max(passes // 4, 5)
+    pw, pk, pe = timed_run(p, pp)
+    out["plain_events_per_s"] = pe / pw
+    out["batched_over_plain"] = out["events_per_s"] / out["plain_events_per_s"]
+    p.close()
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as O
+    cp = 10
+    tc = time.perf_counter()
+    O.kmc_run(64, 64, 0.05, conds[0]["temperature"], cp * n, occs[0], O.mt64_state(5000, 0))
+    cw = time.perf_counter() - tc
+    out["cpu_port"] = dict(events_per_s=cp * n / cw, cores=1,
+                           sample=f"1 chain 64x64, {cp} passes, oracle restatement")
     return out
 
 
@@ -487,7 +551,7 @@ def main():
                     help="requests of the served scan (0 skips it)")
     ap.add_argument("--only", default="",
                     choices=["", "exact", "scan", "band4096", "sc256", "fcc", "served", "peaks",
-                             "occupant"],
+                             "occupant", "kmc"],
                     help="run one measurement and print its JSON (for profiling one kernel)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes 10 times "
@@ -522,6 +586,7 @@ def main():
                "fcc": lambda: bench_fcc(K, plain=False),
                "served": lambda: bench_served(args.served_requests),
                "occupant": lambda: bench_occupant(K),
+               "kmc": lambda: bench_kmc(K),
                "peaks": measured_peaks}[args.only]()
         print(json.dumps({"only": args.only, "steps": K, **one}))
         return
@@ -610,6 +675,10 @@ def main():
         print(f"[bench] occupant batch done: {oc['attempts_per_s']:.3e} attempts/s",
               file=sys.stderr, flush=True)
         line["occupant_sgc_two_units_64_chains"] = oc
+        km = bench_kmc(max(K // 20, 20))
+        print(f"[bench] kmc batch done: {km['events_per_s']:.3e} events/s", file=sys.stderr,
+              flush=True)
+        line["kmc_lattice_gas_64_chains"] = km
         line["checkerboard"] = {
             "scan_128x128_64_conditions": scan,
             "square_4096x4096_J1J2_16T": fam2,
