@@ -45,7 +45,7 @@ EXPORTED = [
     "mcg_get_schedule", "mcg_set_schedule", "mcg_reserve_samples", "mcg_get_occ_location",
     "mcg_set_occ_location", "mcg_get_nfold_state", "mcg_run_steps", "mcg_get_position",
     "mcg_check_batch", "mcg_set_site_asym", "mcg_get_atoms", "mcg_kmc_set_spin_limit",
-    "mcg_kmc_advance",
+    "mcg_kmc_advance", "mcg_fcc_class_schedule",
 ]
 
 
@@ -144,6 +144,8 @@ def lib():
         L.mcg_set_occ_location.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), C.c_int64]
         L.mcg_set_site_asym.argtypes = [C.c_void_p, C.c_int64, P(C.c_int8), C.c_int64]
         L.mcg_get_atoms.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), C.c_int64]
+        L.mcg_fcc_class_schedule.argtypes = [C.c_int64, C.c_int64, C.c_int64, P(C.c_int64),
+                                             P(C.c_int64), P(C.c_int64)]
         L.mcg_kmc_set_spin_limit.argtypes = [C.c_void_p, C.c_int64]
         L.mcg_kmc_advance.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_int64]
         _lib = L
@@ -157,6 +159,18 @@ class McgError(RuntimeError):
 def _check(rc):
     if rc != 0:
         raise McgError(f"mcg error {rc}: {lib().mcg_last_error().decode()}")
+
+
+def fcc_class_schedule(shape):
+    """The class schedule an FCC chain of shape (L0, L1, L2) conventional cells
+    sweeps each pass: a list of steps, each a list of colour classes
+    (col = cx + 4 cy + 16 cz) that update together (a host computation)."""
+    n = C.c_int64()
+    start = (C.c_int64 * 65)()
+    cls = (C.c_int64 * 64)()
+    _check(lib().mcg_fcc_class_schedule(int(shape[0]), int(shape[1]), int(shape[2]),
+                                        C.byref(n), start, cls))
+    return [[cls[k] for k in range(start[s], start[s + 1])] for s in range(n.value)]
 
 
 class Run:

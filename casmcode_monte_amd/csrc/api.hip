@@ -48,6 +48,31 @@ constexpr int64_t LDS_LIMIT = 160 * 1024;     // gfx950 LDS per workgroup
 constexpr int64_t EXACT_LDS_MAX = 144 * 1024;  // lattice + fold scratch (+ ~10 KB static)
 constexpr int64_t CHECKER_LDS_MAX = 128 * 1024;
 constexpr int64_t FCC_LDS_MAX = 142 * 1024;  // lattice; tables and lists take 17 KB static
+// dynamic LDS above half a CU's 160 KB: a workgroup then has its CU to itself
+constexpr int FCC_ONE_PER_CU_LDS = 81 * 1024;
+
+// The most lattice lines any step of a chain's class schedule sweeps: the
+// class-line kernel runs up to one thread per line of the step.
+int fcc_max_step_lines(int L0, int L1, int L2, const FccSched &S) {
+  auto per_parity = [](int L, int a, int par) {
+    int k = 0;
+    for (int v = 0; v < 2 * L; ++v) k += fcc_ring_colour(v, 2 * L) == a && (v & 1) == par;
+    return k;
+  };
+  int most = 0;
+  for (int s = 0; s < S.n_steps; ++s) {
+    int lines = 0;
+    for (int k = S.start[s]; k < S.start[s + 1]; ++k) {
+      const int col = S.cls[k], cx = col & 3, cy = (col >> 2) & 3, cz = col >> 4;
+      for (int py = 0; py < 2; ++py)
+        for (int pz = 0; pz < 2; ++pz)
+          if (per_parity(L0, cx, py ^ pz) > 0)
+            lines += per_parity(L1, cy, py) * per_parity(L2, cz, pz);
+    }
+    most = std::max(most, lines);
+  }
+  return most;
+}
 constexpr int64_t OCC_LDS_MAX = 136 * 1024;  // lattice + lists (+ ~23 KB static)
 constexpr int64_t OCC_MAX_SITES = 40000;
 constexpr int64_t KMC_MAX_SITES = 7500;  // lists, positions, site atoms, atoms: 18 B a site
@@ -172,6 +197,83 @@ std::vector<FccOffset> fcc_offsets(int shell) {
 }
 
 bool fcc_is_nn(int dx, int dy, int dz) { return dx * dx + dy * dy + dz * dz == 2; }
+
+// The class schedule of an L0 x L1 x L2 chain (FccSched, fcc.hh).  Two classes
+// interact when some site of one has a site of the other among its 42
+// neighbours (shells 1-3; the triangles lie within the first shell).  Per
+// ring the pairs (d, v mod 2) with colour(v) = a and colour(v + d) = b are
+// listed for |d| <= 2; classes (ax, ay, az) and (bx, by, bz) interact when an
+// offset (dx, dy, dz) has such pairs in the three rings whose parities sum to
+// an even number (x + y + z is even on every site).  The test agrees with a
+// brute-force search over all site pairs on every shape with edges up to 9.
+void fcc_class_schedule(int L0, int L1, int L2, FccSched &S) {
+  const int L[3] = {L0, L1, L2};
+  // pairs[dim][a][b]: bit (d + 2) * 2 + parity
+  uint32_t pairs[3][4][4] = {};
+  bool has[3][4][2] = {};  // colour a occurs at parity p on ring dim
+  for (int dim = 0; dim < 3; ++dim) {
+    const int n = 2 * L[dim];
+    for (int v = 0; v < n; ++v) {
+      const int a = fcc_ring_colour(v, n);
+      has[dim][a][v & 1] = true;
+      for (int d = -2; d <= 2; ++d) {
+        const int b = fcc_ring_colour(((v + d) % n + n) % n, n);
+        pairs[dim][a][b] |= 1u << ((d + 2) * 2 + (v & 1));
+      }
+    }
+  }
+  std::vector<FccOffset> off;
+  for (int sh = 1; sh <= 3; ++sh)
+    for (auto const &o : fcc_offsets(sh)) off.push_back(o);
+  auto interact = [&](int a, int b) {
+    const int ac[3] = {a & 3, (a >> 2) & 3, a >> 4}, bc[3] = {b & 3, (b >> 2) & 3, b >> 4};
+    for (auto const &o : off) {
+      const int dd[3] = {o.dx, o.dy, o.dz};
+      for (int px = 0; px < 2; ++px)
+        for (int py = 0; py < 2; ++py) {
+          const int pz = (px + py) & 1, pp[3] = {px, py, pz};
+          bool ok = true;
+          for (int dim = 0; dim < 3 && ok; ++dim)
+            ok = (pairs[dim][ac[dim]][bc[dim]] >> ((dd[dim] + 2) * 2 + pp[dim])) & 1u;
+          if (ok) return true;
+        }
+    }
+    return false;
+  };
+  auto nonempty = [&](int col) {
+    const int c[3] = {col & 3, (col >> 2) & 3, col >> 4};
+    for (int px = 0; px < 2; ++px)
+      for (int py = 0; py < 2; ++py)
+        if (has[0][c[0]][px] && has[1][c[1]][py] && has[2][c[2]][(px + py) & 1]) return true;
+    return false;
+  };
+  std::vector<std::vector<int>> steps;
+  for (int col = 0; col < 64; ++col) {
+    if (!nonempty(col)) continue;
+    bool placed = false;
+    for (auto &st : steps) {
+      if ((int)st.size() >= FCC_STEP_MAX) continue;
+      bool free = true;
+      for (int o : st) free = free && !interact(col, o);
+      if (free) {
+        st.push_back(col);
+        placed = true;
+        break;
+      }
+    }
+    if (!placed) steps.push_back({col});
+  }
+  S = FccSched{};
+  S.n_steps = (int32_t)steps.size();
+  int k = 0;
+  for (size_t s = 0; s < steps.size(); ++s) {
+    S.start[s] = (uint8_t)k;
+    for (int col : steps[s]) S.cls[k++] = (uint8_t)col;
+  }
+  S.start[steps.size()] = (uint8_t)k;
+  S.n_classes = k;
+}
+
 
 // Triangle count of a nearest-neighbour occupancy mask (bit k = neighbour k of
 // MCG_FCC_NN): the occupied pairs among the neighbour pairs that are
@@ -392,10 +494,17 @@ struct mcg_handle {
   double *d_ftab = nullptr;  // FCC_NTAB doubles per chain
   uint8_t *d_tri = nullptr;  // 4096-entry triangle-count table (and the bit-line order one)
   uint8_t *d_tri_lines = nullptr;
+  FccSched *d_fcc_sched = nullptr;  // per chain: the class steps of a pass
   // bit-line kernel: every chain L0 <= 32, lattice in LDS as one word per line
   bool fcc_bits = false;
   bool fcc_wide = false;  // bit-line kernel with lines of up to 64 sites
   int fcc_bits_lds = 0;
+  // class-line kernel (narrow bit lines, schedule line_tiles = 0): threads per
+  // workgroup (the most lines of any step, rounded up to a wavefront, at most
+  // 512) and the dynamic LDS it requests (raised so that one chain takes a CU
+  // when the batch has no more chains than the device has CUs)
+  int fcc_line_block = 64;
+  int fcc_line_lds = 0;
   bool fcc() const { return lattice == MCG_LATTICE_FCC; }
 
   ~mcg_handle() {
@@ -406,7 +515,7 @@ struct mcg_handle {
     void *ptrs[] = {d_lat, d_active, d_series, d_stats, d_eq, d_xp, d_xs, d_cp, d_cs, d_tab,
                     d_work, d_ftab, d_tri, d_tri_lines, d_band, d_pair, d_kp, d_list, d_np,
                     d_mask_lds, d_mask_even, d_ring_list, d_batch, d_op, d_ot, d_kp_kmc,
-                    d_atoms};
+                    d_atoms, d_fcc_sched};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -1382,6 +1491,25 @@ int build_fcc(mcg_handle *h) {
   HIP_TRY(fcc_set_max_lds((int)FCC_LDS_MAX));
   const std::vector<uint8_t> tri = fcc_tri_table();
   const std::vector<uint8_t> tri_lines = fcc_tri_table_lines();
+  if (!fcc_tri_pairs_check(tri_lines))
+    return fail(MCG_ERR_STATE, "FCC triangle pairs disagree with the triangle table");
+  std::vector<FccSched> sched(h->n_chains);
+  for (int c = 0; c < h->n_chains; ++c)
+    fcc_class_schedule(h->cp[c].L0, h->cp[c].L1, h->cp[c].L2, sched[c]);
+  HIP_TRY(hipMalloc(&h->d_fcc_sched, sizeof(FccSched) * h->n_chains));
+  HIP_TRY(copy_sync(h, h->d_fcc_sched, sched.data(), sizeof(FccSched) * h->n_chains,
+                    hipMemcpyHostToDevice));
+  if (h->fcc_bits && !h->fcc_wide) {
+    int most = 1;
+    for (int c = 0; c < h->n_chains; ++c)
+      most = std::max(most, fcc_max_step_lines(h->cp[c].L0, h->cp[c].L1, h->cp[c].L2, sched[c]));
+    h->fcc_line_block = std::min(512, 64 * ((most + 63) / 64));
+    int dev = 0, cus = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    h->fcc_line_lds = h->fcc_bits_lds;
+    if (h->n_chains <= cus) h->fcc_line_lds = std::max(h->fcc_line_lds, FCC_ONE_PER_CU_LDS);
+  }
   HIP_TRY(hipMalloc(&h->d_tri_lines, tri_lines.size()));
   HIP_TRY(copy_sync(h, h->d_tri_lines, tri_lines.data(), tri_lines.size(),
                     hipMemcpyHostToDevice));
@@ -2201,6 +2329,9 @@ int mcg_assign_chain(mcg_handle *h, int64_t chain, const mcg_chain_desc *desc) {
     HIP_TRY(copy_sync(h, h->d_cp + chain, &P, sizeof(P), hipMemcpyHostToDevice));
     HIP_TRY(copy_sync(h, h->d_ftab + P.tab_off, tab.data(), sizeof(double) * tab.size(),
                       hipMemcpyHostToDevice));
+    FccSched sched;
+    fcc_class_schedule(P.L0, P.L1, P.L2, sched);
+    HIP_TRY(copy_sync(h, h->d_fcc_sched + chain, &sched, sizeof(sched), hipMemcpyHostToDevice));
     CheckerState S;
     std::memset(&S, 0, sizeof(S));
     HIP_TRY(copy_sync(h, h->d_cs + chain, &S, sizeof(S), hipMemcpyHostToDevice));
@@ -2532,10 +2663,15 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
                  h->nfold_lds, h->stream);
   } else if (h->fcc()) {
     HIP_TRY(hipGetLastError());
-    launch_fcc(h->d_cp, h->d_cs, h->d_lat, h->d_ftab, h->d_tri, h->d_tri_lines, h->d_series,
+    const bool lines = h->fcc_bits && !h->fcc_wide && h->line_tiles == 0;
+    launch_fcc(h->d_cp, h->d_cs, h->d_lat, h->d_ftab, h->d_tri, h->d_tri_lines,
+               h->d_fcc_sched, h->d_series,
                h->series_cap, h->d_active, h->n_chains, n_passes, h->sample_period,
-               h->fcc_bits ? h->fcc_bits_lds : h->checker_lds_bytes, h->checker_lds, h->fcc_bits,
-               h->fcc_wide, h->checker_block, h->stream);
+               lines ? h->fcc_line_lds : h->fcc_bits ? h->fcc_bits_lds : h->checker_lds_bytes,
+               h->checker_lds, h->fcc_bits, h->fcc_wide, lines,
+               lines ? std::min(512, std::max(h->checker_block, h->fcc_line_block))
+                     : h->checker_block,
+               h->stream);
     const hipError_t le = hipGetLastError();
     if (le != hipSuccess)
       return fail(MCG_ERR_HIP, std::string("fcc launch (block ") +
@@ -3044,7 +3180,14 @@ int mcg_autotune(mcg_handle *h, int64_t trial_passes, mcg_schedule *chosen) {
       return MCG_OK;
     };
     std::vector<mcg_schedule> cands;
-    if (h->checker_lds || h->fcc()) {
+    if (h->fcc() && h->fcc_bits && !h->fcc_wide) {
+      // the class-line kernel (its block follows from the lines per class),
+      // then the 4-site unit kernel at three block sizes
+      for (int64_t b : {256, 512})
+        cands.push_back({b, h->units_per_thread, 1, 0.0, h->band_lines, 0});
+      for (int64_t b : {256, 512, 1024})
+        cands.push_back({b, h->units_per_thread, 1, 0.0, h->band_lines, 1});
+    } else if (h->checker_lds || h->fcc()) {
       for (int64_t b : {256, 512, 1024})
         if (!(h->fcc_wide && b > 512))
           cands.push_back({b, h->units_per_thread, 1, 0.0, h->band_lines, h->line_tiles});
@@ -3094,6 +3237,19 @@ int mcg_last_run_time(mcg_handle *h, double *ms, double *attempts) {
   if (!h) return fail(MCG_ERR_ARG, "null handle");
   if (ms) *ms = h->last_ms;
   if (attempts) *attempts = h->last_attempts;
+  return MCG_OK;
+}
+
+int mcg_fcc_class_schedule(int64_t L0, int64_t L1, int64_t L2, int64_t *n_steps,
+                           int64_t *start, int64_t *cls) {
+  if (!n_steps || !start || !cls) return fail(MCG_ERR_ARG, "null argument");
+  if (L0 < 1 || L1 < 1 || L2 < 1 || L0 > FCC_LMAX || L1 > FCC_LMAX || L2 > FCC_LMAX)
+    return fail(MCG_ERR_ARG, "FCC edges must lie in [1, 127]");
+  FccSched S;
+  fcc_class_schedule((int)L0, (int)L1, (int)L2, S);
+  *n_steps = S.n_steps;
+  for (int i = 0; i <= S.n_steps; ++i) start[i] = S.start[i];
+  for (int i = 0; i < S.n_classes; ++i) cls[i] = S.cls[i];
   return MCG_OK;
 }
 

@@ -52,6 +52,35 @@ __host__ __device__ inline int fcc_lds_stride(int L0) {
   return (s & 3) == 0 ? s + 1 : s;
 }
 
+// The 24 nearest-neighbour triangles through a site as pairs of bits of the
+// nearest-neighbour mask of the bit-line kernels (bit i: offset (dx, dy, dz) =
+// (-1,-1,0), (1,-1,0), (-1,1,0), (1,1,0), (-1,0,-1), (1,0,-1), (-1,0,1),
+// (1,0,1), (0,-1,-1), (0,-1,1), (0,1,-1), (0,1,1)): the pairs whose offsets
+// differ by a nearest-neighbour offset.  A handle checks them against the
+// host's triangle table in that bit order when it is built.
+constexpr uint8_t FCC_TRI_PAIRS[24][2] = {
+    {0, 4}, {0, 6}, {0, 8}, {0, 9}, {1, 5}, {1, 7}, {1, 8}, {1, 9},
+    {2, 4}, {2, 6}, {2, 10}, {2, 11}, {3, 5}, {3, 7}, {3, 10}, {3, 11},
+    {4, 8}, {4, 10}, {5, 8}, {5, 10}, {6, 9}, {6, 11}, {7, 9}, {7, 11}};
+
+// Class schedule of a pass: the non-empty colour classes in the order they
+// are swept, grouped into steps of classes that do not interact (no site of
+// one lies within the interaction range of a site of another), so the
+// classes of a step update together, with one barrier per step.  Built on the
+// host for each chain's shape (fcc_class_schedule in api.hip): in class order,
+// each class joins the first step none of whose classes it interacts with (at
+// most FCC_STEP_MAX per step).  Every FCC kernel sweeps this schedule, so all
+// give one trajectory: 8 steps of 4 classes when every edge is even, 14-32
+// when some edge is odd.
+constexpr int FCC_STEP_MAX = 8;
+struct FccSched {
+  int32_t n_steps;
+  int32_t n_classes;
+  uint8_t start[65];  // step s holds cls[start[s] .. start[s+1])
+  uint8_t cls[64];    // classes (col = cx + 4 cy + 16 cz) in sweep order
+  uint8_t pad[5];
+};
+
 // Bit-line kernel (fcc.hip, LDS-resident chains with L0 <= 64): one 64-bit
 // word per lattice line, stored at ln + ln / 4 -- the line doubled when
 // L0 <= FCC_BITS_NARROW_L0, else the line once (doubled in registers);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "../../include/casm_mi355x/mcgpu.h"
 #include "rng.hh"
@@ -335,13 +336,20 @@ hipError_t checker_slab_set_max_lds(int bytes);
 // FCC cluster expansion (fcc.hip): CheckerParams with ncolor = 64, tab_off in
 // doubles into the per-chain acceptance tables, tri = the 4096-entry
 // triangle-count table of the nearest-neighbour occupancy mask.
-// bit_lines: the bit-line kernel (every chain L0 <= 64, lattice in LDS as one
-// 64-bit word per line, tri_lines the triangle table in its mask bit order)
+// bit_lines: the bit-line kernels (every chain L0 <= 64, lattice in LDS as one
+// 64-bit word per line, tri_lines the triangle table in its mask bit order);
+// class_lines (with bit_lines, every chain L0 <= 32): one thread per class
+// line with bit-sliced counts, else 4-site units
+struct FccSched;
 void launch_fcc(const CheckerParams *P, CheckerState *St, int8_t *lat_pool, const double *tables,
-                const uint8_t *tri, const uint8_t *tri_lines, mcg_sample *series,
-                int64_t series_cap, const uint8_t *active, int n_chains, int64_t n_passes,
-                int64_t sample_period, int lds_bytes, bool lattice_in_lds, bool bit_lines,
-                bool wide_lines, int block, hipStream_t stream);
+                const uint8_t *tri, const uint8_t *tri_lines, const FccSched *sched,
+                mcg_sample *series, int64_t series_cap, const uint8_t *active, int n_chains,
+                int64_t n_passes, int64_t sample_period, int lds_bytes, bool lattice_in_lds,
+                bool bit_lines, bool wide_lines, bool class_lines, int block,
+                hipStream_t stream);
+// the triangle pairs of the class-line kernel against the host's table in the
+// bit-line mask order (true: they agree for every mask)
+bool fcc_tri_pairs_check(const std::vector<uint8_t> &tri_lines);
 
 // ---------------------------------------------------------------------------
 // Statistics on the device sample series

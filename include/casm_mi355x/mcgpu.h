@@ -498,7 +498,11 @@ typedef struct mcg_schedule {
                                (-1 = chosen from the sizes, 0 = one launch per colour)  */
   int64_t line_tiles;       /* 4/8-colour HBM path (not banded): 1 = stage tiles of class
                                lines with their neighbour lines in LDS, 0 = read lines
-                               from HBM per class line                                  */
+                               from HBM per class line.  FCC chains of lines of at most
+                               32 sites: 0 = class lines with bit-sliced counts, one to
+                               four threads per line (at least as many threads per
+                               workgroup as the most lines of a class, lds_block if
+                               larger, at most 512), 1 = 4-site units at lds_block  */
 } mcg_schedule;
 
 /* Autotuner: times the candidate schedules on `trial_passes` passes of this
@@ -514,6 +518,15 @@ int mcg_set_schedule(mcg_handle *h, const mcg_schedule *s);
 /* Time of the last mcg_run's kernels on the handle's stream, in ms
  * (HIP events around the launches), and the number of site attempts done. */
 int mcg_last_run_time(mcg_handle *h, double *ms, double *attempts);
+
+/* FCC: the class schedule a chain of L0 x L1 x L2 conventional cells sweeps
+ * each pass (no counterpart in the reference, whose models have no FCC
+ * checkerboard; a host computation, no device needed): the non-empty colour
+ * classes (col = cx + 4 cy + 16 cz) grouped into steps of classes no two of
+ * which interact, updated together.  n_steps; start[0..n_steps] (65 entries
+ * suffice): step s holds cls[start[s] .. start[s+1]); cls: 64 entries. */
+int mcg_fcc_class_schedule(int64_t L0, int64_t L1, int64_t L2, int64_t *n_steps,
+                           int64_t *start, int64_t *cls);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,66 @@
+"""The FCC class schedule (mcg_fcc_class_schedule, a host computation): every
+colour class holding sites appears exactly once, and no two classes of one
+step interact -- checked by brute force over every pair of sites and the 42
+offsets of shells 1-3 on small lattices of even and odd edges -- so updating
+a step's classes together gives the result of updating them one after the
+other.  Even edges need 8 steps of 4 classes."""
+import itertools
+
+import pytest
+
+import casmcode_monte_amd as cm
+
+
+def ring_colour(v, n):
+    if n % 4 == 0 or v < n - 2:
+        return v & 3
+    return 1 if v == n - 2 else 2
+
+
+def offsets():
+    out = []
+    for d in itertools.product(range(-2, 3), repeat=3):
+        a = sorted(abs(x) for x in d)
+        if a in ([0, 1, 1], [0, 0, 2], [1, 1, 2]):
+            out.append(d)
+    return out
+
+
+def classes(shape):
+    n = [2 * L for L in shape]
+    cls = {}
+    for x in range(n[0]):
+        for y in range(n[1]):
+            for z in range(n[2]):
+                if (x + y + z) % 2 == 0:
+                    col = ring_colour(x, n[0]) + 4 * ring_colour(y, n[1]) + 16 * ring_colour(z, n[2])
+                    cls.setdefault(col, set()).add((x, y, z))
+    return cls
+
+
+@pytest.mark.parametrize("shape", [(3, 3, 3), (4, 4, 4), (5, 3, 4), (3, 4, 6), (5, 5, 5),
+                                   (6, 4, 4), (7, 3, 5), (4, 6, 3)])
+def test_schedule_covers_every_class_once_and_steps_commute(shape):
+    off = offsets()
+    assert len(off) == 42
+    cls = classes(shape)
+    steps = cm.fcc_class_schedule(shape)
+    flat = [c for st in steps for c in st]
+    assert sorted(flat) == sorted(cls)
+    n = [2 * L for L in shape]
+    for st in steps:
+        for a, b in itertools.combinations(st, 2):
+            sb = cls[b]
+            for (x, y, z) in cls[a]:
+                for dx, dy, dz in off:
+                    assert ((x + dx) % n[0], (y + dy) % n[1], (z + dz) % n[2]) not in sb, \
+                        (shape, a, b)
+    if all(L % 2 == 0 for L in shape):
+        assert len(steps) == 8 and all(len(st) == 4 for st in steps)
+
+
+def test_batch_shapes_step_counts():
+    # the 16^3-32^3 batch: even edges 8 steps, odd edges 32
+    for L in range(16, 33):
+        steps = cm.fcc_class_schedule((L, L, L))
+        assert len(steps) == (8 if L % 2 == 0 else 32), L

@@ -23,11 +23,11 @@ pytestmark = pytest.mark.gpu
 ECI = fcc.DEFAULT_ECI
 
 
-def run(chains, passes, options=0, eci=ECI, block=None):
+def run(chains, passes, options=0, eci=ECI, block=None, line_tiles=None):
     r = cm.Run(chains, mode=cm.MODE_CHECKERBOARD, lattice=cm.LATTICE_FCC, eci=eci,
                options=options)
-    if block:
-        r.set_schedule(lds_block=block)
+    if block or line_tiles is not None:
+        r.set_schedule(lds_block=block, line_tiles=line_tiles)
     r.run(passes)
     return r
 
@@ -90,6 +90,37 @@ def test_wide_bit_lines_equal_global_kernel(shapes):
                                                                   got.get_occupation(c))
         got.close()
     ref.close()
+
+
+@pytest.mark.parametrize("shapes", [
+    [(3, 3, 3)],
+    [(5, 3, 4), (7, 9, 5), (4, 4, 4)],
+    [(32, 4, 6), (17, 16, 18), (31, 5, 3), (16, 16, 16)],
+])
+def test_class_line_kernel_equals_unit_kernels(shapes):
+    """The class-line kernel (one thread per class line, bit-sliced counts;
+    schedule line_tiles = 0, the default for lines of at most 32 sites) gives
+    the trajectories of the 4-site unit bit-line kernel (line_tiles = 1) and of
+    the global-memory byte kernel, for even and odd edges (whose colourings
+    put sites of one colour on lines of both parities), and counts equal to
+    the oracle's recount."""
+    print("case: fcc class lines", shapes, flush=True)
+    chains = [dict(shape=sh, temperature=700.0 + 400.0 * i, exchange_potential=0.1 - 0.15 * i,
+                   seed=60 + i) for i, sh in enumerate(shapes)]
+    lines = run(chains, 15)
+    assert lines.schedule()["line_tiles"] == 0
+    for other in (run(chains, 15, line_tiles=1, block=512),
+                  run(chains, 15, options=cm.OPT_FORCE_STREAMING)):
+        for c in range(len(chains)):
+            np.testing.assert_array_equal(lines.get_occupation(c), other.get_occupation(c))
+            np.testing.assert_array_equal(lines.samples(c), other.samples(c))
+            assert lines.counters(c) == other.counters(c)
+        other.close()
+    for c in range(len(chains)):
+        assert counts_of(lines.samples(c)[-1]) == O.fcc_counts(chains[c]["shape"],
+                                                                lines.get_occupation(c))
+        assert lines.counters(c)[1] > 0
+    lines.close()
 
 
 def test_batched_equals_alone():
