@@ -164,10 +164,10 @@ def _check(rc):
 def fcc_class_schedule(shape):
     """The class schedule an FCC chain of shape (L0, L1, L2) conventional cells
     sweeps each pass: a list of steps, each a list of colour classes
-    (col = cx + 4 cy + 16 cz) that update together (a host computation)."""
+    (col = cx + 6 cy + 36 cz) that update together (a host computation)."""
     n = C.c_int64()
-    start = (C.c_int64 * 65)()
-    cls = (C.c_int64 * 64)()
+    start = (C.c_int64 * 129)()
+    cls = (C.c_int64 * 128)()
     _check(lib().mcg_fcc_class_schedule(int(shape[0]), int(shape[1]), int(shape[2]),
                                         C.byref(n), start, cls))
     return [[cls[k] for k in range(start[s], start[s + 1])] for s in range(n.value)]

@@ -47,7 +47,7 @@ int fail(int code, std::string const &msg) {
 constexpr int64_t LDS_LIMIT = 160 * 1024;     // gfx950 LDS per workgroup
 constexpr int64_t EXACT_LDS_MAX = 144 * 1024;  // lattice + fold scratch (+ ~10 KB static)
 constexpr int64_t CHECKER_LDS_MAX = 128 * 1024;
-constexpr int64_t FCC_LDS_MAX = 142 * 1024;  // lattice; tables and lists take 17 KB static
+constexpr int64_t FCC_LDS_MAX = 140 * 1024;  // lattice; tables, lists and schedule take 19.2 KB static
 // dynamic LDS above half a CU's 160 KB: a workgroup then has its CU to itself
 constexpr int FCC_ONE_PER_CU_LDS = 81 * 1024;
 
@@ -63,7 +63,8 @@ int fcc_max_step_lines(int L0, int L1, int L2, const FccSched &S) {
   for (int s = 0; s < S.n_steps; ++s) {
     int lines = 0;
     for (int k = S.start[s]; k < S.start[s + 1]; ++k) {
-      const int col = S.cls[k], cx = col & 3, cy = (col >> 2) & 3, cz = col >> 4;
+      const int col = S.cls[k], cx = col % FCC_NCOL, cy = (col / FCC_NCOL) % FCC_NCOL,
+                cz = col / (FCC_NCOL * FCC_NCOL);
       for (int py = 0; py < 2; ++py)
         for (int pz = 0; pz < 2; ++pz)
           if (per_parity(L0, cx, py ^ pz) > 0)
@@ -205,17 +206,19 @@ bool fcc_is_nn(int dx, int dy, int dz) { return dx * dx + dy * dy + dz * dz == 2
 // listed for |d| <= 2; classes (ax, ay, az) and (bx, by, bz) interact when an
 // offset (dx, dy, dz) has such pairs in the three rings whose parities sum to
 // an even number (x + y + z is even on every site).  The test agrees with a
-// brute-force search over all site pairs on every shape with edges up to 9.
+// brute-force search over all site pairs on small lattices of even and odd
+// edges.  Classes are placed in order of decreasing size, then class id.
 void fcc_class_schedule(int L0, int L1, int L2, FccSched &S) {
+  constexpr int NC = FCC_NCOL;
   const int L[3] = {L0, L1, L2};
-  // pairs[dim][a][b]: bit (d + 2) * 2 + parity
-  uint32_t pairs[3][4][4] = {};
-  bool has[3][4][2] = {};  // colour a occurs at parity p on ring dim
+  // pairs[dim][a][b]: bit (d + 2) * 2 + parity; cnt[dim][a][p]: points of colour a, parity p
+  uint32_t pairs[3][NC][NC] = {};
+  int cnt[3][NC][2] = {};
   for (int dim = 0; dim < 3; ++dim) {
     const int n = 2 * L[dim];
     for (int v = 0; v < n; ++v) {
       const int a = fcc_ring_colour(v, n);
-      has[dim][a][v & 1] = true;
+      ++cnt[dim][a][v & 1];
       for (int d = -2; d <= 2; ++d) {
         const int b = fcc_ring_colour(((v + d) % n + n) % n, n);
         pairs[dim][a][b] |= 1u << ((d + 2) * 2 + (v & 1));
@@ -225,13 +228,20 @@ void fcc_class_schedule(int L0, int L1, int L2, FccSched &S) {
   std::vector<FccOffset> off;
   for (int sh = 1; sh <= 3; ++sh)
     for (auto const &o : fcc_offsets(sh)) off.push_back(o);
+  auto coords = [](int col, int *c) {
+    c[0] = col % NC;
+    c[1] = (col / NC) % NC;
+    c[2] = col / (NC * NC);
+  };
   auto interact = [&](int a, int b) {
-    const int ac[3] = {a & 3, (a >> 2) & 3, a >> 4}, bc[3] = {b & 3, (b >> 2) & 3, b >> 4};
+    int ac[3], bc[3];
+    coords(a, ac);
+    coords(b, bc);
     for (auto const &o : off) {
       const int dd[3] = {o.dx, o.dy, o.dz};
       for (int px = 0; px < 2; ++px)
         for (int py = 0; py < 2; ++py) {
-          const int pz = (px + py) & 1, pp[3] = {px, py, pz};
+          const int pp[3] = {px, py, (px + py) & 1};
           bool ok = true;
           for (int dim = 0; dim < 3 && ok; ++dim)
             ok = (pairs[dim][ac[dim]][bc[dim]] >> ((dd[dim] + 2) * 2 + pp[dim])) & 1u;
@@ -240,16 +250,22 @@ void fcc_class_schedule(int L0, int L1, int L2, FccSched &S) {
     }
     return false;
   };
-  auto nonempty = [&](int col) {
-    const int c[3] = {col & 3, (col >> 2) & 3, col >> 4};
+  auto sites = [&](int col) {
+    int c[3];
+    coords(col, c);
+    int64_t k = 0;
     for (int px = 0; px < 2; ++px)
       for (int py = 0; py < 2; ++py)
-        if (has[0][c[0]][px] && has[1][c[1]][py] && has[2][c[2]][(px + py) & 1]) return true;
-    return false;
+        k += (int64_t)cnt[0][c[0]][px] * cnt[1][c[1]][py] * cnt[2][c[2]][(px + py) & 1];
+    return k;
   };
+  std::vector<int> order;
+  for (int col = 0; col < FCC_NCLASS; ++col)
+    if (sites(col) > 0) order.push_back(col);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return sites(a) > sites(b); });
   std::vector<std::vector<int>> steps;
-  for (int col = 0; col < 64; ++col) {
-    if (!nonempty(col)) continue;
+  for (int col : order) {
     bool placed = false;
     for (auto &st : steps) {
       if ((int)st.size() >= FCC_STEP_MAX) continue;

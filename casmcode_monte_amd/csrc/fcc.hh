@@ -38,11 +38,15 @@ constexpr int FCC_LIST = 64;   // longest per-colour coordinate list (2L <= 254)
 constexpr int FCC_LMAX = 127;  // conventional cells per edge
 
 // Colour of doubled coordinate v on a ring of n = 2L points: v mod 4, except
-// that when n = 2 (mod 4) the last two points get 1 and 2, so that points of
-// one colour are 0 or >= 3 apart around the ring.
+// that when n = 2 (mod 4) the last two points get colours of their own, 4
+// and 5, so that points of one colour are 0 or >= 3 apart around the ring and
+// colours 0-3 keep one parity each (as on rings of n = 0 mod 4, no two points
+// of colours 0-3 within distance 2 of each other straddle the seam).
+constexpr int FCC_NCOL = 6;
+constexpr int FCC_NCLASS = FCC_NCOL * FCC_NCOL * FCC_NCOL;  // col = cx + 6 cy + 36 cz
 __host__ __device__ inline int fcc_ring_colour(int v, int n) {
   if ((n & 3) == 0 || v < n - 2) return v & 3;
-  return v == n - 2 ? 1 : 2;
+  return v == n - 2 ? 4 : 5;
 }
 
 // LDS line stride of the FCC kernel: L0 sites plus two ghost bytes, plus one
@@ -67,18 +71,20 @@ constexpr uint8_t FCC_TRI_PAIRS[24][2] = {
 // are swept, grouped into steps of classes that do not interact (no site of
 // one lies within the interaction range of a site of another), so the
 // classes of a step update together, with one barrier per step.  Built on the
-// host for each chain's shape (fcc_class_schedule in api.hip): in class order,
-// each class joins the first step none of whose classes it interacts with (at
-// most FCC_STEP_MAX per step).  Every FCC kernel sweeps this schedule, so all
-// give one trajectory: 8 steps of 4 classes when every edge is even, 14-32
-// when some edge is odd.
+// host for each chain's shape (fcc_class_schedule in api.hip): classes in
+// order of decreasing size (then class id), each joining the first step none
+// of whose classes it interacts with (at most FCC_STEP_MAX per step).  Every
+// FCC kernel sweeps this schedule, so all give one trajectory: 8 steps of 4
+// classes when every edge is even, 20 (8 large, 12 on the odd rings' seams)
+// when every edge is odd.
 constexpr int FCC_STEP_MAX = 8;
+constexpr int FCC_SCHED_MAX = 128;  // non-empty classes: 32 to 108
 struct FccSched {
   int32_t n_steps;
   int32_t n_classes;
-  uint8_t start[65];  // step s holds cls[start[s] .. start[s+1])
-  uint8_t cls[64];    // classes (col = cx + 4 cy + 16 cz) in sweep order
-  uint8_t pad[5];
+  uint8_t start[FCC_SCHED_MAX + 1];  // step s holds cls[start[s] .. start[s+1])
+  uint8_t cls[FCC_SCHED_MAX];        // classes (col = cx + 6 cy + 36 cz) in sweep order
+  uint8_t pad[7];
 };
 
 // Bit-line kernel (fcc.hip, LDS-resident chains with L0 <= 64): one 64-bit

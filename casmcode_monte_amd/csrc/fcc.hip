@@ -11,12 +11,13 @@
 // A site flips iff (r + 1/2) 2^-32 < A*B for its 32-bit Philox word r.
 //
 // Colouring: every doubled coordinate is coloured on its ring with
-// fcc_ring_colour (points of one colour are 0 or >= 3 apart); a class is the
-// product of the three coordinate colours, so two sites of one class differ
-// by >= 3 in some coordinate while every interaction offset is <= 2 in each:
-// a class updates in parallel.  One pass visits the 64 classes in order
-// col = cx + 4 cy + 16 cz (the 32 with cx+cy+cz odd are empty when every edge
-// is even).
+// fcc_ring_colour (points of one colour are 0 or >= 3 apart; six colours, the
+// last two only on the seams of odd rings); a class is the product of the
+// three coordinate colours, col = cx + 6 cy + 36 cz, so two sites of one class
+// differ by >= 3 in some coordinate while every interaction offset is <= 2 in
+// each: a class updates in parallel.  One pass sweeps the chain's class
+// schedule (FccSched, fcc.hh): steps of classes that do not interact, each
+// step updated at once, one barrier per step.
 //
 // Work unit: up to four sites of one class on one lattice line (fixed y, z),
 // slots 4q..4q+3 of that line's list of class x values; one Philox4x32-10
@@ -38,10 +39,10 @@ namespace mcg {
 namespace {
 
 struct FccLists {
-  uint8_t y[4][2][FCC_LIST];   // y values of each colour, by parity
-  uint8_t z[4][2][FCC_LIST];   // z values of each colour, by parity
-  uint8_t xh[4][2][FCC_LIST];  // h = x >> 1 of each colour, by parity of x
-  int ny[4][2], nz[4][2], nx[4][2];
+  uint8_t y[FCC_NCOL][2][FCC_LIST];   // y values of each colour, by parity
+  uint8_t z[FCC_NCOL][2][FCC_LIST];   // z values of each colour, by parity
+  uint8_t xh[FCC_NCOL][2][FCC_LIST];  // h = x >> 1 of each colour, by parity of x
+  int ny[FCC_NCOL][2], nz[FCC_NCOL][2], nx[FCC_NCOL][2];
 };
 
 __device__ __forceinline__ int fcc_wave_sum(int v) {
@@ -52,7 +53,7 @@ __device__ __forceinline__ int fcc_wave_sum(int v) {
 
 __device__ __forceinline__ int wrap_ring(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
-// Thread a (< 4) lists the coordinates of colour a on the three rings.
+// Thread a (< FCC_NCOL) lists the coordinates of colour a on the three rings.
 __device__ __forceinline__ void fcc_fill_lists(FccLists &Ls, int a, int L0, int L1, int L2) {
   int k0 = 0, k1 = 0;
   for (int v = 0; v < 2 * L1; ++v)
@@ -95,9 +96,9 @@ struct FccClass {
 __device__ __forceinline__ FccClass fcc_class(const FccLists &Ls, int col) {
   FccClass k;
   k.col = col;
-  k.cx = col & 3;
-  k.cy = (col >> 2) & 3;
-  k.cz = col >> 4;
+  k.cx = col % FCC_NCOL;
+  k.cy = (col / FCC_NCOL) % FCC_NCOL;
+  k.cz = col / (FCC_NCOL * FCC_NCOL);
   k.start[0] = 0;
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
@@ -266,7 +267,7 @@ __global__ __launch_bounds__(1024) void fcc_kernel(
   __shared__ uint8_t tri[4096];
   __shared__ FccLists Ls;
   __shared__ FccSched S;
-  __shared__ int item_start[65];
+  __shared__ int item_start[FCC_SCHED_MAX + 1];
   __shared__ long long red[6][16];
   const CheckerParams P = Pp[c];
   CheckerState *St = Sp + c;
@@ -275,7 +276,7 @@ __global__ __launch_bounds__(1024) void fcc_kernel(
   for (int i = t; i < FCC_NTAB; i += nt) tab[i] = tables[P.tab_off + i];
   for (int i = t; i < 4096; i += nt) tri[i] = tri_g[i];
   const int L0 = P.L0, L1 = P.L1, L2 = P.L2;
-  if (t < 4) fcc_fill_lists(Ls, t, L0, L1, L2);
+  if (t < FCC_NCOL) fcc_fill_lists(Ls, t, L0, L1, L2);
   __syncthreads();
   fcc_sched_setup<true>(sched, c, S, item_start, Ls, t);
   // In LDS every line takes L0 + 2 bytes (+1 when that is a multiple of 4)
@@ -570,7 +571,7 @@ __global__ __launch_bounds__(WIDE ? 512 : 1024) void fcc_bits_kernel(
   __shared__ uint8_t tri[4096];
   __shared__ FccLists Ls;
   __shared__ FccSched S;
-  __shared__ int item_start[65];
+  __shared__ int item_start[FCC_SCHED_MAX + 1];
   __shared__ long long red[6][16];
   const CheckerParams P = Pp[c];
   CheckerState *St = Sp + c;
@@ -580,7 +581,7 @@ __global__ __launch_bounds__(WIDE ? 512 : 1024) void fcc_bits_kernel(
   for (int i = t; i < FCC_NTAB; i += nt) tab[i] = tables[P.tab_off + i];
   for (int i = t; i < 4096; i += nt) tri[i] = tri_g[i];
   const int L0 = P.L0, L1 = P.L1, L2 = P.L2;
-  if (t < 4) fcc_fill_lists(Ls, t, L0, L1, L2);
+  if (t < FCC_NCOL) fcc_fill_lists(Ls, t, L0, L1, L2);
   const int n_lines = 4 * L1 * L2;
   for (int ln = t; ln < n_lines; ln += nt) {
     uint64_t core = 0;
@@ -892,8 +893,8 @@ __global__ __launch_bounds__(512) void fcc_lines_kernel(
   __shared__ double tab[FCC_NTAB];
   __shared__ FccLists Ls;
   __shared__ FccSched S;
-  __shared__ int item_start[65];
-  __shared__ uint32_t xmask[4][2];
+  __shared__ int item_start[FCC_SCHED_MAX + 1];
+  __shared__ uint32_t xmask[FCC_NCOL][2];
   __shared__ long long red[6][8];
   const CheckerParams P = Pp[c];
   CheckerState *St = Sp + c;
@@ -902,7 +903,7 @@ __global__ __launch_bounds__(512) void fcc_lines_kernel(
   const int t = threadIdx.x, nt = blockDim.x;
   for (int i = t; i < FCC_NTAB; i += nt) tab[i] = tables[P.tab_off + i];
   const int L0 = P.L0, L1 = P.L1, L2 = P.L2;
-  if (t < 4) fcc_fill_lists(Ls, t, L0, L1, L2);
+  if (t < FCC_NCOL) fcc_fill_lists(Ls, t, L0, L1, L2);
   const int n_lines = 4 * L1 * L2;
   for (int ln = t; ln < n_lines; ln += nt) {
     uint64_t core = 0;
@@ -910,7 +911,7 @@ __global__ __launch_bounds__(512) void fcc_lines_kernel(
     w[fcc_word_index(ln)] = core | (core << L0);
   }
   __syncthreads();
-  if (t < 8) {  // positions of colour t>>1 on lines of x parity t&1
+  if (t < 2 * FCC_NCOL) {  // positions of colour t>>1 on lines of x parity t&1
     const int a = t >> 1, pr = t & 1;
     uint32_t m = 0u;
     for (int s = 0; s < Ls.nx[a][pr]; ++s) m |= 1u << Ls.xh[a][pr][s];
@@ -934,7 +935,8 @@ __global__ __launch_bounds__(512) void fcc_lines_kernel(
       for (int li = t / g; li < n_items; li += nt / g) {
         const int k = fcc_item_class(item_start, k0, k1, i0 + li);
         const int col = S.cls[k];
-        const int cx = col & 3, cy = (col >> 2) & 3, cz = col >> 4;
+        const int cx = col % FCC_NCOL, cy = (col / FCC_NCOL) % FCC_NCOL,
+                  cz = col / (FCC_NCOL * FCC_NCOL);
         int lstart[4];
         lstart[0] = 0;
 #pragma unroll

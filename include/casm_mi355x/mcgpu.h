@@ -522,9 +522,10 @@ int mcg_last_run_time(mcg_handle *h, double *ms, double *attempts);
 /* FCC: the class schedule a chain of L0 x L1 x L2 conventional cells sweeps
  * each pass (no counterpart in the reference, whose models have no FCC
  * checkerboard; a host computation, no device needed): the non-empty colour
- * classes (col = cx + 4 cy + 16 cz) grouped into steps of classes no two of
- * which interact, updated together.  n_steps; start[0..n_steps] (65 entries
- * suffice): step s holds cls[start[s] .. start[s+1]); cls: 64 entries. */
+ * classes (col = cx + 6 cy + 36 cz; colours 4 and 5 are the two seam points
+ * of a ring of odd L) grouped into steps of classes no two of which interact,
+ * updated together.  n_steps; start[0..n_steps] (129 entries suffice): step s
+ * holds cls[start[s] .. start[s+1]); cls: 128 entries. */
 int mcg_fcc_class_schedule(int64_t L0, int64_t L1, int64_t L2, int64_t *n_steps,
                            int64_t *start, int64_t *cls);
 

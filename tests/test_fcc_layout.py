@@ -1,10 +1,11 @@
 """CPU checks of the FCC kernel's work decomposition (a Python emulation of the
 index arithmetic of casmcode_monte_amd/csrc/fcc.hip):
 
-  - the ring colouring puts equal colours 0 or >= 3 apart on every ring 2L >= 6;
+  - the ring colouring (six colours, the last two on the seam of a ring of odd
+    L) puts equal colours 0 or >= 3 apart on every ring 2L >= 6;
   - no two sites of one colour class interact (no interaction offset joins
     them), for odd and even edges;
-  - the 64 classes' work units visit every site exactly once, every index stays
+  - the 216 classes' work units visit every site exactly once, every index stays
     inside the lattice and the per-colour lists fit their 64-entry buffers;
   - the Philox counters (line*32 + q, class) of the units are distinct.
 """
@@ -21,7 +22,7 @@ FCC_LIST = 64
 def ring_colour(v, n):
     if n % 4 == 0 or v < n - 2:
         return v & 3
-    return 1 if v == n - 2 else 2
+    return 4 if v == n - 2 else 5
 
 
 def offsets():
@@ -44,17 +45,17 @@ def emulate_units(L):
     L0, L1, L2 = L
     lists = {}
     for axis, Lx in (("y", L1), ("z", L2)):
-        for c in range(4):
+        for c in range(6):
             lists[axis, c] = [v for v in range(2 * Lx) if ring_colour(v, 2 * Lx) == c]
-    for c in range(4):
+    for c in range(6):
         for p in (0, 1):
             lists["x", c, p] = [v >> 1 for v in range(2 * L0)
                                 if ring_colour(v, 2 * L0) == c and (v & 1) == p]
     for key, v in lists.items():
         assert len(v) <= FCC_LIST, key
     visits, counters = [], set()
-    for col in range(64):
-        cx, cy, cz = col & 3, (col >> 2) & 3, col >> 4
+    for col in range(216):
+        cx, cy, cz = col % 6, (col // 6) % 6, col // 36
         ny, nz = len(lists["y", cy]), len(lists["z", cz])
         maxq = (max(len(lists["x", cx, 0]), len(lists["x", cx, 1])) + 3) >> 2
         for u in range(ny * nz * maxq):
@@ -88,7 +89,7 @@ def test_units_cover_every_site_once(L):
 def test_classes_do_not_interact(L):
     x, y, z = O.fcc_coords(L)
     n = [2 * v for v in L]
-    cls = np.array([ring_colour(a, n[0]) + 4 * ring_colour(b, n[1]) + 16 * ring_colour(c, n[2])
+    cls = np.array([ring_colour(a, n[0]) + 6 * ring_colour(b, n[1]) + 36 * ring_colour(c, n[2])
                     for a, b, c in zip(x, y, z)])
     index = {}
     for l, (a, b, c) in enumerate(zip(x, y, z)):

@@ -1,5 +1,6 @@
 """The FCC class schedule (mcg_fcc_class_schedule, a host computation): every
-colour class holding sites appears exactly once, and no two classes of one
+colour class holding sites appears exactly once (six colours per ring, the
+last two on the seam of a ring of odd L), and no two classes of one
 step interact -- checked by brute force over every pair of sites and the 42
 offsets of shells 1-3 on small lattices of even and odd edges -- so updating
 a step's classes together gives the result of updating them one after the
@@ -14,7 +15,7 @@ import casmcode_monte_amd as cm
 def ring_colour(v, n):
     if n % 4 == 0 or v < n - 2:
         return v & 3
-    return 1 if v == n - 2 else 2
+    return 4 if v == n - 2 else 5
 
 
 def offsets():
@@ -33,7 +34,8 @@ def classes(shape):
         for y in range(n[1]):
             for z in range(n[2]):
                 if (x + y + z) % 2 == 0:
-                    col = ring_colour(x, n[0]) + 4 * ring_colour(y, n[1]) + 16 * ring_colour(z, n[2])
+                    col = (ring_colour(x, n[0]) + 6 * ring_colour(y, n[1]) +
+                           36 * ring_colour(z, n[2]))
                     cls.setdefault(col, set()).add((x, y, z))
     return cls
 
@@ -60,7 +62,13 @@ def test_schedule_covers_every_class_once_and_steps_commute(shape):
 
 
 def test_batch_shapes_step_counts():
-    # the 16^3-32^3 batch: even edges 8 steps, odd edges 32
+    # the 16^3-32^3 batch: even edges 8 steps of 4 classes; odd edges 8 steps
+    # holding the 32 large classes (four each, with seam classes that fit) and
+    # 12 more steps of seam classes
     for L in range(16, 33):
         steps = cm.fcc_class_schedule((L, L, L))
-        assert len(steps) == (8 if L % 2 == 0 else 32), L
+        assert len(steps) == (8 if L % 2 == 0 else 20), L
+        large = sorted(c for c in range(216) if c % 6 < 4 and (c // 6) % 6 < 4 and c // 36 < 4
+                       and (c % 6 + c // 6 + c // 36) % 2 == 0)
+        assert len(large) == 32
+        assert all(sum(c in large for c in st) == 4 for st in steps[:8]), L
