@@ -2494,10 +2494,45 @@ int mcg_get_atoms(mcg_handle *h, int64_t chain, int64_t *atoms, int64_t n_atoms)
   return MCG_OK;
 }
 
+// NFOLD event by event (the kernel's max_events >= 0 mode); no records
+int nfold_advance(mcg_handle *h, int64_t max_events, double stop_time, bool select_at_end) {
+  h->last_ms = 0.0;
+  h->last_attempts = 0.0;
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  launch_nfold(h->d_np, h->d_xs, h->d_lat, h->d_list, h->d_series, h->series_cap, h->d_active,
+               h->n_chains, 0, h->sample_period, 0, max_events, stop_time, select_at_end ? 1 : 0,
+               h->nfold_lds, h->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  std::vector<ExactState> xs(h->n_chains);
+  HIP_TRY(copy_sync(h, xs.data(), h->d_xs, sizeof(ExactState) * h->n_chains,
+                    hipMemcpyDeviceToHost));
+  bool frozen = false;
+  double events = 0.0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    if (!h->active[c]) continue;
+    events += (double)(xs[c].n_accept - (h->n_pass[c] * h->sites(c) + h->n_step[c]));
+    h->n_pass[c] = xs[c].n_pass;
+    h->n_step[c] = xs[c].step;
+    frozen = frozen || xs[c].frozen;
+  }
+  h->last_attempts = events;
+  if (frozen)
+    return fail(MCG_ERR_STATE,
+                "NFOLD: the total rate is 0 (every flip has probability 0 at this temperature), "
+                "so the next event never comes; the chain stopped there");
+  return MCG_OK;
+}
+
 int mcg_kmc_advance(mcg_handle *h, int64_t max_events, double stop_time, int64_t select_at_end) {
   if (!h) return fail(MCG_ERR_ARG, "null handle");
-  if (!h->kmc()) return fail(MCG_ERR_STATE, "mcg_kmc_advance: KMC mode only");
+  if (!h->kmc() && !h->nfold()) return fail(MCG_ERR_STATE, "mcg_kmc_advance: KMC and NFOLD modes only");
   if (max_events < 0 || std::isnan(stop_time)) return fail(MCG_ERR_ARG, "bad event count or stop time");
+  if (h->nfold()) return nfold_advance(h, max_events, stop_time, select_at_end != 0);
   return kmc_run(h, 0, max_events, stop_time, select_at_end != 0);
 }
 
@@ -2675,8 +2710,8 @@ int mcg_run(mcg_handle *h, int64_t n_passes) {
                      h->stream);
   } else if (h->nfold()) {
     launch_nfold(h->d_np, h->d_xs, h->d_lat, h->d_list, h->d_series, h->series_cap,
-                 h->d_active, h->n_chains, n_passes, h->sample_period, sample_now ? 1 : 0,
-                 h->nfold_lds, h->stream);
+                 h->d_active, h->n_chains, n_passes, h->sample_period, sample_now ? 1 : 0, -1,
+                 std::numeric_limits<double>::infinity(), 0, h->nfold_lds, h->stream);
   } else if (h->fcc()) {
     HIP_TRY(hipGetLastError());
     const bool lines = h->fcc_bits && !h->fcc_wide && h->line_tiles == 0;
@@ -2809,7 +2844,8 @@ int mcg_get_position(mcg_handle *h, int64_t chain, int64_t *n_pass, int64_t *ste
   ExactState st;
   HIP_TRY(copy_sync(h, &st, h->d_xs + chain, sizeof(st), hipMemcpyDeviceToHost));
   if (n_pass) *n_pass = st.n_pass;
-  if (step) *step = (h->mode == MCG_MODE_EXACT || h->occupant() || h->kmc()) ? st.step : 0;
+  if (step)
+    *step = (h->mode == MCG_MODE_EXACT || h->occupant() || h->kmc() || h->nfold()) ? st.step : 0;
   if (e_run) *e_run = st.e_run;
   return MCG_OK;
 }

@@ -41,7 +41,8 @@ struct ExactState {
   int64_t pos;        // index of the next output (312: twist first)
   int64_t n_pass, n_accept, n_reject;
   int64_t S, M;       // exact bond sum and magnetisation
-  double e_run;       // running potential energy per unit cell
+  double e_run;       // running potential energy per unit cell (NFOLD: the total rate
+                      // at the pending event's selection)
   int64_t n_samples;  // records the run produced (written while below the capacity)
   // NFOLD
   double time;           // simulated time
@@ -52,7 +53,8 @@ struct ExactState {
                          // swap types ran out of events
   // EXACT: attempts done in the current pass (0 <= step < n; runs of whole
   // passes keep it, runs of single steps move it); OCCUPANT: steps done in the
-  // current pass (0 <= step < n_mut)
+  // current pass (0 <= step < n_mut); NFOLD and KMC: events done in the
+  // current pass
   int64_t step;
   // OCCUPANT: sites of each asymmetric unit holding its occupant 0
   int32_t occ_cnt0[8];
@@ -220,10 +222,14 @@ struct NfoldParams {
 // class array and positions (16-bit each; a position's bit 15 is the spin)
 inline int nfold_lds_bytes(const NfoldParams &P) { return 4 * P.n + 64; }
 
+// max_events >= 0: event by event (no records, n_passes ignored): apply the
+// pending event unless its time reaches stop_time, then up to max_events
+// events, leaving pending the one that reaches stop_time or, with
+// select_at_end, the one after the last; max_events < 0: whole passes.
 void launch_nfold(const NfoldParams *P, ExactState *S, int8_t *lat_pool, int32_t *list_pool,
                   mcg_sample *series, int64_t series_cap, const uint8_t *active, int n_chains,
-                  int64_t n_passes, int64_t sample_period, int sample_now, int lds_bytes,
-                  hipStream_t stream);
+                  int64_t n_passes, int64_t sample_period, int sample_now, int64_t max_events,
+                  double stop_time, int select_at_end, int lds_bytes, hipStream_t stream);
 hipError_t nfold_set_max_lds(int bytes);
 
 // ---------------------------------------------------------------------------

@@ -89,7 +89,8 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
                                                    int64_t series_cap,
                                                    const uint8_t *__restrict__ active,
                                                    int64_t n_passes, int64_t sample_period,
-                                                   int sample_now) {
+                                                   int sample_now, int64_t max_events,
+                                                   double stop_time, int select_at_end) {
   const int c = blockIdx.x;
   if (!active[c]) return;
   const int lane = threadIdx.x;
@@ -136,6 +137,8 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
   double time = St->time;
   int64_t pending = St->pending_site;
   double pending_dt = St->pending_dt;
+  double pending_R = St->e_run;  // the total rate at the pending event's selection
+  int64_t step = St->step;       // events done in the current pass
   const int self_nb = 2 * (rows == 1) + 2 * (cols == 1);
   const bool narrow = rows <= 2 || cols <= 2;  // a neighbour can be listed twice
   bool frozen = false;  // total rate 0: the next event never comes
@@ -270,11 +273,12 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
     select(l, l_pos, u3, R);
     pending_dt = -log1p(-u3) / R;
     pending = l;
+    pending_R = R;
     record(time, pending_dt, R);
   };
   // the pass-0 sample, or a sample of the current pass asked for by the host,
   // unless this pass's first event is already selected
-  if (!by_time && pending < 0 &&
+  if (max_events < 0 && !by_time && pending < 0 &&
       (sample_now || (P->sample_start && n_pass == 0 && n_samples == 0)))
     sample_pass();
 
@@ -377,8 +381,68 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
     }
   };
 
+  // Event by event (max_events >= 0; no records: the host samples), the
+  // select-then-sample protocol of nfold.hh:121-141: the pending event is
+  // applied first unless its time reaches stop_time; then events are selected
+  // and, each time increment formed at once, applied until max_events are
+  // done; an event whose time reaches stop_time, or (select_at_end) the one
+  // after the last, is selected and left pending with its increment and the
+  // total rate at its selection.  A total rate of 0 (no event ever comes)
+  // stops the chain as frozen.
+  if (max_events >= 0) {
+    int64_t done = 0;
+    bool stopped = false;
+    auto bump = [&]() {
+      ++n_ev;
+      ++done;
+      if (++step == n) {
+        step = 0;
+        ++n_pass;
+      }
+    };
+    if (pending >= 0) {
+      const double et = __dadd_rn(time, pending_dt);
+      if (et >= stop_time) {
+        stopped = true;
+      } else {
+        apply((uint32_t)pending, -1);
+        time = et;
+        pending = -1;
+        bump();
+      }
+    }
+    while (!stopped) {
+      const bool select_only = done >= max_events;
+      if (select_only && !select_at_end) break;
+      uint32_t l;
+      int l_pos;
+      double u3, R;
+      select(l, l_pos, u3, R);
+      const double dt = -log1p(-u3) / R;
+      const double et = __dadd_rn(time, dt);
+      if (!(et <= 1.79769313486231570815e308)) {
+        frozen = true;
+        break;
+      }
+      if (select_only || et >= stop_time) {
+        pending = l;
+        pending_dt = dt;
+        pending_R = R;
+        stopped = true;
+      } else {
+        apply(l, l_pos);
+        time = et;
+        bump();
+      }
+    }
+    n_passes = 0;
+  }
+
   for (int64_t pp = 0; pp < n_passes; ++pp) {
-    for (int32_t st = 0; st < n; ++st) {
+    // a pass begun by event-by-event advances ends at its boundary
+    const int32_t st0 = pp == 0 ? (int32_t)step : 0;
+    step = 0;
+    for (int32_t st = st0; st < n; ++st) {
       uint32_t l;
       int l_pos = -1;
       double u3, R;
@@ -429,6 +493,8 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
     St->time = time;
     St->pending_site = pending;
     St->pending_dt = pending_dt;
+    St->e_run = pending_R;
+    St->step = step;
     St->next_k = next_k;
     St->frozen = frozen ? 1 : 0;
   }
@@ -436,10 +502,11 @@ __global__ __launch_bounds__(64) void nfold_kernel(const NfoldParams *__restrict
 
 void launch_nfold(const NfoldParams *P, ExactState *S, int8_t *lat_pool, int32_t *list_pool,
                   mcg_sample *series, int64_t series_cap, const uint8_t *active, int n_chains,
-                  int64_t n_passes, int64_t sample_period, int sample_now, int lds_bytes,
-                  hipStream_t stream) {
+                  int64_t n_passes, int64_t sample_period, int sample_now, int64_t max_events,
+                  double stop_time, int select_at_end, int lds_bytes, hipStream_t stream) {
   hipLaunchKernelGGL(nfold_kernel, dim3(n_chains), dim3(64), lds_bytes, stream, P, S, lat_pool,
-                     list_pool, series, series_cap, active, n_passes, sample_period, sample_now);
+                     list_pool, series, series_cap, active, n_passes, sample_period, sample_now,
+                     max_events, stop_time, select_at_end);
 }
 
 hipError_t nfold_set_max_lds(int bytes) {

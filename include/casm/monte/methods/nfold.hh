@@ -22,9 +22,14 @@
 // reference's loop does; the fixtures' time is the time of the last event.
 // Between those points the device runs whole passes.
 //
-// Not supported: SAMPLE_MODE::BY_STEP and BY_TIME fixtures (the device samples
-// by pass; its own time-grid mode serves C-ABI callers), simulated-time
-// cutoffs, and selected-event statistics.  At the end occ_location is rebuilt from the final occupation:
+// Fixtures that sample by time (SAMPLE_MODE::BY_TIME) sample with weight 1 at
+// every sample time an event's time reaches, before that event is applied
+// (nfold.hh:93-110, :128-130); with such a fixture the device runs event by
+// event between the points where anything can change and stops with the event
+// that reaches the next sample time selected (mcg_kmc_advance).
+//
+// Not supported: SAMPLE_MODE::BY_STEP fixtures, stochastic sample periods,
+// simulated-time cutoffs, and selected-event statistics.  At the end occ_location is rebuilt from the final occupation:
 // its candidate lists hold the same sites as the reference's, in site order
 // rather than in the order the reference's applies leave them (only a later
 // draw from those lists would notice).
@@ -32,6 +37,7 @@
 #define CASM_MI355X_monte_methods_nfold
 
 #include <algorithm>
+#include <limits>
 #include <memory>
 #include <string>
 
@@ -95,10 +101,15 @@ void nfold(State<ising_cpp::IsingConfiguration> &state, OccLocation &occ_locatio
   for (Index l = 0; l < N; ++l)
     if (config.occ(l) != 0 && config.occ(l) != 1)
       throw std::runtime_error("nfold (MI355X): configurations hold occupant indices 0/1");
+  bool by_time = false;
   for (auto &f : run_manager.sampling_fixtures) {
     auto const &sp = f->params().sampling_params;
-    if (sp.sample_mode != SAMPLE_MODE::BY_PASS)
-      throw std::runtime_error("nfold (MI355X): only SAMPLE_MODE::BY_PASS sampling is supported");
+    if (sp.sample_mode == SAMPLE_MODE::BY_STEP)
+      throw std::runtime_error(
+          "nfold (MI355X): SAMPLE_MODE::BY_PASS and BY_TIME sampling are supported, BY_STEP is not");
+    if (sp.sample_mode == SAMPLE_MODE::BY_TIME) by_time = true;
+    if (sp.stochastic_sample_period)
+      throw std::runtime_error("nfold (MI355X): stochastic sample periods are not supported");
     // the reference checks simulated-time cutoffs before every event
     // (SamplingFixture.hh:272-275); the device stops only between passes
     auto const &cp = f->params().completion_check_params.cutoff_params;
@@ -153,6 +164,92 @@ void nfold(State<ising_cpp::IsingConfiguration> &state, OccLocation &occ_locatio
     }
     return false;
   };
+
+  if (by_time) {
+    // Event by event (nfold.hh:118-149): the device runs the iterations up to
+    // the next point where something can change (a count due, a count cutoff,
+    // a periodic check, or an event whose time reaches the next sample time)
+    // and stops there with that event selected; the fixtures sample by count,
+    // then by time, and the event is applied.
+    auto steps_of = [&](CountType c) { return (int64_t)c * (int64_t)N; };
+    int64_t acc = 0, g = 0;
+    double time = 0.0;
+    auto account = [&]() {
+      int64_t np = 0, step = 0, na = 0, nr = 0;
+      double rate = 0.0, dt = 0.0;
+      dev.position(0, np, step, rate);
+      dev.counters(0, np, na, nr);
+      const int64_t reached = np * (int64_t)N + step;
+      for (auto &f : run_manager.sampling_fixtures) f->counter().increment_steps(reached - g, na - acc, 0);
+      acc = na;
+      g = reached;
+      int64_t site = -1;
+      dev.nfold_state(0, time, site, dt);
+      run_manager.set_time(time);
+      total_rate = rate;
+      time_increment = dt;
+      return site;
+    };
+    auto pre_sample_time = [&](fixture_type &fixture, State<config_type> const &) {
+      nfold_data.sampling_fixture_label = fixture.label();
+      nfold_data.sampling_fixture = &fixture;
+      nfold_data.expected_acceptance_rate = total_rate / nfold_data.n_events_possible;
+      if (fixture.params().sampling_params.sample_mode == SAMPLE_MODE::BY_TIME)
+        fixture.push_back_sample_weight(1.0);
+      else
+        fixture.push_back_sample_weight(time_increment);
+    };
+    typename RunManager<config_type, StatisticsType, EngineType>::NullAction post_sample;
+    run_manager.update_next_sampling_fixture();
+    const double inf = std::numeric_limits<double>::infinity();
+    while (!run_manager.is_complete()) {
+      run_manager.write_status_if_due();
+      bool due = false;
+      for (auto &f : run_manager.sampling_fixtures)
+        if (f->params().sampling_params.sample_mode != SAMPLE_MODE::BY_TIME &&
+            f->counter().count == f->next_sample_count())
+          due = true;
+      int64_t site = -1;
+      if (due) {
+        dev.kmc_advance(0, inf, true);  // select the event, leave it pending
+        site = account();
+      } else {
+        int64_t target = g + (int64_t)1000000 * (int64_t)N;
+        if (pending_checks()) target = g + 1;
+        for (auto &f : run_manager.sampling_fixtures) {
+          if (f->params().sampling_params.sample_mode != SAMPLE_MODE::BY_TIME)
+            target = std::min<int64_t>(target, std::max<int64_t>(steps_of(f->next_sample_count()), g + 1));
+          auto const &cp = f->params().completion_check_params.cutoff_params;
+          const CountType count = f->counter().count;
+          if (cp.min_count.has_value() && *cp.min_count > count)
+            target = std::min<int64_t>(target, steps_of(*cp.min_count));
+          if (cp.max_count.has_value() && *cp.max_count > count)
+            target = std::min<int64_t>(target, steps_of(*cp.max_count));
+        }
+        const double stop = run_manager.next_sampling_fixture ? run_manager.next_sample_time : inf;
+        dev.kmc_advance(target - g, stop, false);
+        site = account();
+        if (site < 0) continue;  // a count point: the completion check comes first
+      }
+      // the selected event: samples by count, then by time (nfold.hh:121-126)
+      const double event_time = time + time_increment;
+      sync_configuration();
+      run_manager.sample_data_by_count_if_due(state, pre_sample_time, post_sample);
+      run_manager.sample_data_by_time_if_due(event_time, state, pre_sample_time, post_sample);
+      // apply it, set the time, count the step (:128-141)
+      dev.kmc_advance(1, inf, false);
+      account();
+    }
+    sync_configuration();
+    {
+      std::vector<int> occ(config.occupation().begin(), config.occupation().end());
+      occ_location.initialize(occ);
+    }
+    dev.store_engine(0, *event_selector.engine);
+    (void)get_event_f;
+    run_manager.finalize(state);
+    return;
+  }
 
   CountType p = 0;
   int64_t acc_prev = 0;

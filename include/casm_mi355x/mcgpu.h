@@ -284,16 +284,21 @@ int mcg_set_occ_location(mcg_handle *h, int64_t chain, const int64_t *loc, int64
 int mcg_get_atoms(mcg_handle *h, int64_t chain, int64_t *atoms, int64_t n_atoms);
 /* KMC: attempts without an event after which a chain stops (default 2^26). */
 int mcg_kmc_set_spin_limit(mcg_handle *h, int64_t attempts);
-/* KMC: the reference loop's select-then-sample order (kinetic_monte_carlo.hh:
- * 185-226).  Every active chain applies its pending event, if any, then
- * selects and applies events until it has applied max_events in this call;
- * an event whose time reaches stop_time is selected but left pending (the
- * chain stops there), and with select_at_end a chain that applied max_events
- * selects one more and leaves it pending.  A pending event's site, and its
- * time (in place of the increment), are read with mcg_get_nfold_state; the
- * next mcg_run, mcg_run_steps or mcg_kmc_advance applies it first (and
- * counts it).  The drop-in kinetic_monte_carlo() samples by count and by time
- * between the selection and the application, as the reference does. */
+/* KMC and NFOLD: the reference loops' select-then-sample order
+ * (kinetic_monte_carlo.hh:185-226, nfold.hh:121-141).  Every active chain
+ * applies its pending event, if any, unless that event's time reaches
+ * stop_time, then selects and applies events until it has applied max_events
+ * in this call; an event whose time reaches stop_time is selected but left
+ * pending (the chain stops there), and with select_at_end a chain that
+ * applied max_events selects one more and leaves it pending.  A pending
+ * event's site is read with mcg_get_nfold_state, with its time (KMC) or its
+ * time increment (NFOLD; the total rate at its selection is the running
+ * energy of mcg_get_position); the next mcg_run, mcg_run_steps or
+ * mcg_kmc_advance applies it first (and counts it).  No records are written
+ * (NFOLD: a chain whose total rate is 0 stops and the call fails with
+ * MCG_ERR_STATE).  The drop-ins kinetic_monte_carlo() and nfold() sample by
+ * count and by time between the selection and the application, as the
+ * reference does. */
 int mcg_kmc_advance(mcg_handle *h, int64_t max_events, double stop_time, int64_t select_at_end);
 
 /* OCCUPANT: the asymmetric unit of every site of one chain (n = sites; the
@@ -344,10 +349,10 @@ int mcg_run(mcg_handle *h, int64_t n_passes);
 int mcg_run_steps(mcg_handle *h, int64_t n_steps);
 
 /* Sequential modes: a chain's pass count, its steps into the current pass
- * (EXACT, OCCUPANT and KMC; 0 otherwise) and its FP64 running
+ * (EXACT, OCCUPANT, KMC and NFOLD; 0 otherwise) and its FP64 running
  * potential energy per unit cell (occupation_metropolis.hh:103-106, :141;
- * EXACT, CANONICAL and OCCUPANT) -- the state a sample taken between two steps
- * reads. */
+ * EXACT, CANONICAL and OCCUPANT; NFOLD: the total rate at the selection of
+ * the pending event) -- the state a sample taken between two steps reads. */
 int mcg_get_position(mcg_handle *h, int64_t chain, int64_t *n_pass, int64_t *step,
                      double *e_run);
 

@@ -1011,6 +1011,124 @@ static void gpu_nfold_one_site_max_sample() {
 // Two fixtures sampling every pass and every 4th pass from count 0, both cut
 // off at count 30: each holds the oracle's samples at its own counts (the
 // device records only the passes where some fixture is due).
+// nfold() with a fixture sampling by time (period tau from tau, weight 1,
+// the state before the event whose time reaches each sample time) beside a
+// fixture sampling by pass that ends the run at count P: both against the
+// oracle's time-grid and by-pass runs of the same chain (sampling draws
+// nothing from the engine, so both runs follow one trajectory).
+static void gpu_nfold_by_time() {
+  const int R = 10, C = 12, N = R * C;
+  const long P = 30;
+  const double T = 1600.0, J = 0.1, mu = -0.03, tau = 0.37;
+  const uint64_t seed = 81;
+  typedef ising_cpp::IsingConfiguration config_type;
+  typedef RunManager<config_type, BasicStatistics, std::mt19937_64> rm_type;
+  std::vector<int> occ0 = random_occupants(N, 43, 17);
+  config_type cfg({R, C}, 0);
+  cfg.set_occupation(occ0);
+  ValueMap cond;
+  cond.scalar_values["temperature"] = T;
+  cond.vector_values["exchange_potential"] = VectorXd{mu};
+  State<config_type> state(cfg, cond);
+  ising_cpp::BinaryLatticeEvents ev(N);
+  OccLocation ol(ev.convert, ev.candidate_list);
+  ol.initialize(state.configuration.occupation());
+  NfoldData<config_type, BasicStatistics, std::mt19937_64> nfold_data;
+  nfold_data.n_events_possible = N;
+  StateSamplingFunctionMap fns;
+  StateSamplingFunction m("spin_sum", "sum of spins", {}, [&state]() {
+    double s = 0;
+    for (int v : state.configuration.occupation()) s += 2 * v - 1;
+    return VectorXd{s};
+  });
+  StateSamplingFunction ear("expected_acceptance_rate", "total rate / N", {}, [&nfold_data]() {
+    return VectorXd{nfold_data.expected_acceptance_rate};
+  });
+  fns.emplace(m.name, m);
+  fns.emplace(ear.name, ear);
+  SamplingParams sp_time;
+  sp_time.sampler_names = {"spin_sum", "expected_acceptance_rate"};
+  sp_time.sample_mode = SAMPLE_MODE::BY_TIME;
+  sp_time.begin = tau;
+  sp_time.period = tau;
+  sp_time.do_sample_time = true;
+  SamplingParams sp_pass;
+  sp_pass.sampler_names = {"spin_sum"};
+  sp_pass.begin = 1.0;
+  sp_pass.do_sample_time = true;
+  CompletionCheckParams<BasicStatistics> cc_pass;
+  cc_pass.cutoff_params.max_count = P;
+  std::vector<rm_type::sampling_fixture_params_type> fixtures;
+  fixtures.emplace_back("by_time", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), sp_time,
+                        CompletionCheckParams<BasicStatistics>(), std::vector<std::string>{});
+  fixtures.emplace_back("by_pass", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), sp_pass,
+                        cc_pass, std::vector<std::string>{});
+  rm_type rm(std::make_shared<std::mt19937_64>(seed + 1000), fixtures, true);
+  auto engine = std::make_shared<std::mt19937_64>(seed);
+  ising_cpp::IsingNfoldEventSelector<std::mt19937_64> selector(J, engine);
+  OccEvent flip;
+  flip.occ_transform.resize(1);
+  auto get_event_f = [&](Index l) -> OccEvent const & {
+    Mol const &mol = ol.mol(ol.l_to_mol_id(l));
+    flip.linear_site_index = {l};
+    flip.new_occ = {1 - state.configuration.occ(l)};
+    flip.occ_transform[0] = OccTransform{l, mol.id, mol.asym, mol.species_index,
+                                         1 - mol.species_index};
+    return flip;
+  };
+  nfold<Index>(state, ol, nfold_data, selector, get_event_f, rm);
+
+  const std::string tag = "nfold by time";
+  refcpu::IsingConfiguration oc(R, C, -1);
+  for (int l = 0; l < N; ++l) oc.occupation[l] = occ0[l] ? 1 : -1;
+  std::mt19937_64 eg(seed), ep(seed);
+  auto og = refcpu::nfold_run(oc, J, T, mu, eg, P, 1, tau);
+  auto op = refcpu::nfold_run(oc, J, T, mu, ep, P - 1, 1, 0.0, false);
+  // the grid records expanded: one sample per grid time, weight 1
+  std::vector<double> want_m, want_t;
+  for (size_t r = 0; r < og.M.size(); ++r)
+    for (long k = 0; k < (long)og.weight[r]; ++k) {
+      want_m.push_back((double)og.M[r]);
+      want_t.push_back(og.time[r] + k * tau);
+    }
+  auto const &rt = rm.sampling_fixtures[0]->results();
+  VectorXd got_m = rt.samplers.at("spin_sum")->component(0);
+  VectorXd got_w = rt.sample_weight.component(0);
+  bool ok = (size_t)got_m.size() == want_m.size() && rt.sample_time.size() == want_m.size() &&
+            (size_t)got_w.size() == want_m.size() && !want_m.empty();
+  for (size_t i = 0; ok && i < want_m.size(); ++i)
+    ok = got_m(i) == want_m[i] && got_w(i) == 1.0 &&
+         close_rel(rt.sample_time[i], (double)(i + 1) * tau, 1e-12) &&
+         close_rel(rt.sample_time[i], want_t[i], 1e-9);
+  EXPECT(ok, tag << ": samples by time (" << got_m.size() << " of " << want_m.size() << ")");
+  VectorXd got_r = rt.samplers.at("expected_acceptance_rate")->component(0);
+  bool rate_ok = got_r.size() == (Index)want_m.size();
+  for (Index i = 0; rate_ok && i < got_r.size(); ++i) rate_ok = got_r(i) > 0.0 && got_r(i) <= 1.0;
+  EXPECT(rate_ok, tag << ": expected acceptance rates in (0, 1]");
+  // by pass: counts 1..P-1 (the run ends at count P before sampling it)
+  auto const &rp = rm.sampling_fixtures[1]->results();
+  VectorXd pm = rp.samplers.at("spin_sum")->component(0);
+  VectorXd pw = rp.sample_weight.component(0);
+  bool okp = pm.size() == P - 1 && (long)op.M.size() == P - 1;
+  for (long i = 0; okp && i < P - 1; ++i)
+    okp = pm(i) == (double)op.M[i] && close_rel(pw(i), op.weight[i], 1e-12) &&
+          rp.sample_count[i] == i + 1;
+  EXPECT(okp, tag << ": samples by pass (" << pm.size() << ")");
+  std::vector<int> want(N);
+  for (int l = 0; l < N; ++l) want[l] = og.final_occ[l] > 0 ? 1 : 0;
+  EXPECT(state.configuration.occupation() == want, tag << ": final configuration");
+  EXPECT(*engine == eg, tag << ": engine state");
+  EXPECT(rp.n_accept == (BigCountType)P * N, tag << ": event count " << rp.n_accept);
+  EXPECT(close_rel(rm.sampling_fixtures[1]->counter().time, og.final_time, 1e-12),
+         tag << ": time");
+  EXPECT(occ_location_consistent(ol, state.configuration.occupation()),
+         tag << ": OccLocation invariants after the run");
+  std::cout << "gpu: " << tag << " 10x12 (" << got_m.size() << " time samples)"
+            << (g_fail ? " (failures so far)" : " ok") << std::endl;
+}
+
 static void gpu_nfold_two_fixtures() {
   const int R = 6, C = 9, N = R * C;
   const long P = 30;
@@ -1565,6 +1683,7 @@ static void gpu_tests() {
   gpu_nfold_case(true);
   gpu_nfold_two_fixtures();
   gpu_nfold_one_site_max_sample();
+  gpu_nfold_by_time();
 }
 
 int main(int argc, char **argv) {
