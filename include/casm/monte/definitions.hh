@@ -136,6 +136,41 @@ class MatrixXd {
   std::vector<double> m_v;
 };
 
+/// Minimal dense vector of longs (Eigen::VectorXl's role in histogram keys)
+class VectorXl {
+ public:
+  VectorXl() = default;
+  explicit VectorXl(Index n, long value = 0) : m_v(static_cast<size_t>(n), value) {}
+  VectorXl(std::initializer_list<long> l) : m_v(l) {}
+  explicit VectorXl(std::vector<long> v) : m_v(std::move(v)) {}
+  Index size() const { return static_cast<Index>(m_v.size()); }
+  long &operator()(Index i) { return m_v[i]; }
+  long operator()(Index i) const { return m_v[i]; }
+  long const *data() const { return m_v.data(); }
+  std::vector<long> const &vector() const { return m_v; }
+  bool operator==(VectorXl const &o) const { return m_v == o.m_v; }
+
+ private:
+  std::vector<long> m_v;
+};
+
+/// Minimal dense column-major matrix of longs (Eigen::MatrixXl)
+class MatrixXl {
+ public:
+  MatrixXl() = default;
+  MatrixXl(Index rows, Index cols, long value = 0)
+      : m_rows(rows), m_cols(cols), m_v(static_cast<size_t>(rows * cols), value) {}
+  static MatrixXl Zero(Index rows, Index cols) { return MatrixXl(rows, cols, 0); }
+  Index rows() const { return m_rows; }
+  Index cols() const { return m_cols; }
+  long &operator()(Index i, Index j) { return m_v[static_cast<size_t>(i + m_rows * j)]; }
+  long operator()(Index i, Index j) const { return m_v[static_cast<size_t>(i + m_rows * j)]; }
+
+ private:
+  Index m_rows = 0, m_cols = 0;
+  std::vector<long> m_v;
+};
+
 typedef std::vector<Index> Shape;
 
 }  // namespace monte

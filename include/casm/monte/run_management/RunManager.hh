@@ -10,6 +10,7 @@
 
 #include "casm/monte/checks/CompletionCheck.hh"
 #include "casm/monte/checks/io/json/CompletionCheck_json_io.hh"
+#include "casm/monte/sampling/SelectedEventFunctions.hh"
 
 namespace CASM {
 namespace monte {
@@ -93,9 +94,12 @@ struct Results {
   CompletionCheckResults<StatisticsType> completion_check_results;
   BigCountType n_accept = 0;
   BigCountType n_reject = 0;
+  /// selected-event data a kinetic run collected (Results.hh:73-74)
+  std::optional<SelectedEventData> selected_event_data;
 
   void reset() {
     elapsed_clocktime.reset();
+    selected_event_data.reset();
     initial_memory_used_MiB = memory_used_MiB();
     final_memory_used_MiB = 0.0;
     samplers.clear();
@@ -335,6 +339,10 @@ class SamplingFixture {
   /// Weight of the next sample (time increment of the N-fold way's event
   /// selected at the sample; SamplingFixture.hh:330-332)
   void push_back_sample_weight(double weight) { m_results.sample_weight.push_back(weight); }
+  /// store the selected-event data a run collected (SamplingFixture.hh:340-343)
+  void set_selected_event_data(SelectedEventData &&selected_event_data) {
+    m_results.selected_event_data = std::move(selected_event_data);
+  }
   void set_time(double t) { m_counter.set_time(t); }
 
   void sample_data(state_type const &state) {

@@ -21,6 +21,7 @@
 #include "casm/monte/methods/nfold.hh"
 #include "casm/monte/methods/occupation_metropolis.hh"
 #include "casm/monte/run_management/io/json/jsonResultsIO.hh"
+#include "casm/monte/sampling/io/json/SelectedEventFunctions_json_io.hh"
 #include "refcpu.hh"
 
 using namespace CASM;
@@ -1608,6 +1609,182 @@ static void gpu_kmc_case(uint64_t seed) {
 
 // the drop-in OccLocation's atom tracking against the oracle's, over the
 // selector's own events applied on the host
+// Selected-event statistics (SelectedEventFunctions.hh): every expected value
+// below is worked out by hand from the binning rules (the first value places
+// begin on the bin_width grid just below it, a value exactly on an edge goes
+// to the bin above, smaller values prepend bins, at most max_size bins).
+static void cpu_selected_event_tests() {
+  {  // linear 1d histogram: growth both ways, then the max_size limit
+    Histogram1D h(0.0, 1.0, false, 5);
+    h.insert(2.5);
+    EXPECT(h.begin() == 2.0 && h.count() == std::vector<double>({1.0}), "hist1d: first value");
+    h.insert(0.3);
+    EXPECT(h.begin() == 0.0 && h.count() == std::vector<double>({1.0, 0.0, 1.0}),
+           "hist1d: bins prepended");
+    h.insert(6.7);  // would need bin 6 of at most 5
+    EXPECT(h.count() == std::vector<double>({1.0, 0.0, 1.0, 0.0, 0.0}) && h.max_size_exceeded() &&
+               h.out_of_range_count() == 1.0,
+           "hist1d: appended to max_size, the value out of range");
+    h.insert(-2.2);  // no room below either
+    EXPECT(h.begin() == 0.0 && h.out_of_range_count() == 2.0 && h.sum() == 4.0,
+           "hist1d: below the full range");
+    EXPECT(h.bin_coords() == std::vector<double>({0.0, 1.0, 2.0, 3.0, 4.0}) &&
+               h.density() == std::vector<double>({0.25, 0.0, 0.25, 0.0, 0.0}),
+           "hist1d: coordinates and density");
+  }
+  {  // log space: log10(10) = 1 lies on an edge and goes to the bin above it
+    Histogram1D h(0.0, 0.5, true);
+    h.insert(10.0);
+    EXPECT(h.begin() == 0.5 && h.count() == std::vector<double>({0.0, 1.0}), "hist1d log: edge");
+    h.insert_log_value(0.2, 2.0);
+    EXPECT(h.begin() == 0.0 && h.count() == std::vector<double>({2.0, 0.0, 1.0}),
+           "hist1d log: weighted log value");
+    bool threw = false;
+    try {
+      Histogram1D(0.0, 1.0, false).insert_log_value(1.0);
+    } catch (std::runtime_error const &) {
+      threw = true;
+    }
+    EXPECT(threw, "hist1d: insert_log_value on a linear histogram throws");
+  }
+  {  // partitions: combined range, every partition aligned to it
+    PartitionedHistogram1D p({"A", "B"}, 0.0, 1.0, false);
+    p.insert(0, 1.5);
+    p.insert(1, 4.2);
+    p.insert(1, 4.7);
+    auto const &c = p.combined_histogram();
+    EXPECT(c.begin() == 1.0 && c.count() == std::vector<double>({1.0, 0.0, 0.0, 2.0}),
+           "partitioned: combined histogram");
+    auto const &hs = p.histograms();
+    EXPECT(hs[0].begin() == 1.0 && hs[0].count() == std::vector<double>({1.0, 0.0, 0.0, 0.0}) &&
+               hs[1].begin() == 1.0 && hs[1].count() == std::vector<double>({0.0, 0.0, 0.0, 2.0}),
+           "partitioned: partitions span the combined range");
+    jsonParser j;
+    to_json(p, j);
+    EXPECT(j["partition_names"].size() == 2 && j["partitions"].size() == 2 &&
+               j["partitions"][1]["partition_fraction"].get<double>() == 2.0 / 3.0 &&
+               j["combined"]["sum"].get<double>() == 3.0,
+           "partitioned: JSON");
+  }
+  {  // discrete histograms: lexicographic keys, max_size, tolerance
+    DiscreteVectorIntHistogram h({"a", "b"}, {2}, 2);
+    h.insert(VectorXl{1, 2});
+    h.insert(VectorXl{0, 5});
+    h.insert(VectorXl{1, 2});
+    h.insert(VectorXl{3, 3});
+    EXPECT(h.size() == 2 && h.values()[0] == VectorXl({0, 5}) &&
+               h.count() == std::vector<double>({1.0, 2.0}) && h.out_of_range_count() == 1.0 &&
+               h.max_size_exceeded() && h.fraction() == std::vector<double>({0.25, 0.5}),
+           "discrete int histogram");
+    DiscreteVectorFloatHistogram f({"0"}, {}, 1e-5);
+    f.insert(VectorXd{0.1});
+    f.insert(VectorXd{0.1 + 1e-7});
+    f.insert(VectorXd{0.2});
+    EXPECT(f.size() == 2 && f.count() == std::vector<double>({2.0, 1.0}) && f.tol() == 1e-5,
+           "discrete float histogram: values within tol share a bin");
+    jsonParser j;
+    to_json(h, j);
+    EXPECT(j["values"].size() == 2 && j["values"][0][1].get<long>() == 5 &&
+               j["out_of_range_count"].get<double>() == 1.0 && j["component_names"].size() == 2,
+           "discrete histogram: JSON");
+  }
+  {  // hop correlations: every 2nd jump, at most 2 positions per atom
+    CorrelationsData d;
+    d.initialize(2, 2, 2, false);
+    d.insert(0, 2, VectorXd{1.0, 0.0, 0.0}, 5, 1, 0.5);
+    EXPECT(d.n_position_samples == std::vector<CountType>({1, 0}) && d.n_complete_samples == 0,
+           "correlations: one atom sampled");
+    d.insert(1, 1, VectorXd{0.0, 1.0, 0.0}, 6, 1, 0.7);  // odd jump count: not sampled
+    d.insert(1, 2, VectorXd{0.0, 2.0, 0.0}, 7, 1, 0.9);
+    EXPECT(d.n_position_samples == std::vector<CountType>({1, 1}) && d.n_complete_samples == 1 &&
+               d.step(0, 1) == 7 && d.time(0, 1) == 0.9 && d.atom_positions_cart[0](1, 1) == 2.0,
+           "correlations: both atoms, one complete sample");
+    d.insert(0, 4, VectorXd{2.0, 0.0, 0.0}, 8, 2, 1.1);
+    d.insert(0, 6, VectorXd{3.0, 0.0, 0.0}, 9, 2, 1.3);  // atom 0 full
+    EXPECT(d.n_position_samples == std::vector<CountType>({2, 1}) && d.n_complete_samples == 1,
+           "correlations: the full atom stops, the complete count waits for the other");
+    jsonParser j;
+    to_json(d, j);
+    EXPECT(j["step"].size() == 1 && j["atom_positions_cart"].size() == 1 &&
+               j["atom_positions_cart"][0].size() == 3,
+           "correlations: JSON holds the complete samples");
+  }
+  {  // the collector: generic functions in order, histograms by name
+    std::vector<std::string> trace;
+    long type = 0;
+    double rate = 1.0;
+    SelectedEventFunctions fns;
+    fns.insert(GenericSelectedEventFunction("second", "", false, [&]() { trace.push_back("second"); },
+                                            []() { return true; }, 2));
+    fns.insert(GenericSelectedEventFunction("first", "", false, [&]() { trace.push_back("first"); },
+                                            []() { return true; }, 1));
+    fns.insert(DiscreteVectorIntHistogramFunction("event_type", "", {}, std::nullopt, false,
+                                                  [&]() { return VectorXl{type}; },
+                                                  []() { return true; }, 100));
+    fns.insert(PartitionedHistogramFunction<double>("rate", "", false, [&]() { return rate; },
+                                                    {"all"}, []() { return 0; }, true, 0.0, 1.0,
+                                                    100));
+    SelectedEventFunctionParams params;
+    params.evaluate("second", std::nullopt).evaluate("first", std::nullopt);
+    params.collect("event_type", std::nullopt, std::nullopt, std::nullopt, std::nullopt, 3);
+    params.collect("rate", std::nullopt, 0.5, std::nullopt, "log", std::nullopt);
+    auto data = std::make_shared<SelectedEventData>();
+    SelectedEventDataCollector collector(fns, params, data);
+    for (long k = 0; k < 5; ++k) {
+      type = k % 4;
+      rate = std::pow(10.0, (double)k);
+      collector.collect();
+    }
+    EXPECT(trace.size() == 10 && trace[0] == "first" && trace[1] == "second",
+           "collector: generic functions by order");
+    auto const &et = data->discrete_vector_int_histograms.at("event_type");
+    EXPECT(et.max_size() == 3 && et.size() == 3 && et.out_of_range_count() == 1.0 &&
+               et.count() == std::vector<double>({2.0, 1.0, 1.0}),
+           "collector: discrete histogram with the params' max_size");
+    auto const &rh = data->continuous_1d_histograms.at("rate").combined_histogram();
+    EXPECT(rh.is_log() && rh.bin_width() == 0.5 && rh.sum() == 5.0 && rh.begin() == 0.0,
+           "collector: log histogram with the params' bin width");
+    bool threw = false;
+    try {
+      SelectedEventFunctionParams bad;
+      bad.evaluate("missing", std::nullopt);
+      SelectedEventDataCollector c2(fns, bad, data);
+    } catch (std::runtime_error const &) {
+      threw = true;
+    }
+    EXPECT(threw, "collector: unknown quantity throws");
+  }
+  {  // parameters: collect / do_not_collect and JSON round trip
+    SelectedEventFunctionParams p;
+    p.collect("event_type", std::nullopt, std::nullopt, std::nullopt, std::nullopt, std::nullopt);
+    EXPECT(p.function_names == std::vector<std::string>({"event_type"}), "params: collect");
+    p.do_not_collect("event_type");
+    EXPECT(p.function_names.empty(), "params: do_not_collect");
+    p.collect("event_type", std::nullopt, std::nullopt, std::nullopt, std::nullopt, std::nullopt);
+    p.collect("dE_activated", std::nullopt, 1.0, 0.01, "linear", 10);
+    p.correlations_data_params = CorrelationsDataParams{};
+    p.correlations_data_params->jumps_per_position_sample = 10;
+    jsonParser j;
+    to_json(p, j);
+    auto q = selected_event_function_params_from_json(j);
+    EXPECT(q.function_names == p.function_names && q.bin_width == p.bin_width &&
+               q.initial_begin == p.initial_begin && q.is_log == p.is_log &&
+               q.max_size == p.max_size && q.correlations_data_params.has_value() &&
+               q.correlations_data_params->jumps_per_position_sample == 10 &&
+               j["spacing"]["dE_activated"].get<std::string>() == "linear",
+           "params: JSON round trip");
+    bool threw = false;
+    try {
+      p.collect("x", std::nullopt, std::nullopt, std::nullopt, "cubic", std::nullopt);
+    } catch (std::runtime_error const &) {
+      threw = true;
+    }
+    EXPECT(threw, "params: a spacing other than log or linear throws");
+  }
+  std::cout << "cpu: selected-event statistics" << (g_fail ? " (failures so far)" : " ok")
+            << std::endl;
+}
+
 static void cpu_kmc_atom_tracking() {
   const int R = 6, C = 7, N = R * C;
   std::vector<int> occ(N, 0);
@@ -1695,6 +1872,7 @@ int main(int argc, char **argv) {
       cpu_canonical_tests();
       cpu_occupant_tests();
       cpu_kmc_atom_tracking();
+      cpu_selected_event_tests();
       cpu_nfold_argument_tests();
     }
     if (gpu) gpu_tests();
