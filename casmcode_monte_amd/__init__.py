@@ -45,7 +45,7 @@ EXPORTED = [
     "mcg_get_schedule", "mcg_set_schedule", "mcg_reserve_samples", "mcg_get_occ_location",
     "mcg_set_occ_location", "mcg_get_nfold_state", "mcg_run_steps", "mcg_get_position",
     "mcg_check_batch", "mcg_set_site_asym", "mcg_get_atoms", "mcg_kmc_set_spin_limit",
-    "mcg_kmc_advance", "mcg_fcc_class_schedule",
+    "mcg_kmc_advance", "mcg_fcc_class_schedule", "mcg_kmc_collect_hops", "mcg_kmc_get_hops",
 ]
 
 
@@ -146,6 +146,9 @@ def lib():
         L.mcg_get_atoms.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64), C.c_int64]
         L.mcg_fcc_class_schedule.argtypes = [C.c_int64, C.c_int64, C.c_int64, P(C.c_int64),
                                              P(C.c_int64), P(C.c_int64)]
+        L.mcg_kmc_collect_hops.argtypes = [C.c_void_p, C.c_int64, C.c_int64]
+        L.mcg_kmc_get_hops.argtypes = [C.c_void_p, C.c_int64, C.c_int64, P(C.c_int64),
+                                       P(C.c_int64), P(C.c_double)]
         L.mcg_kmc_set_spin_limit.argtypes = [C.c_void_p, C.c_int64]
         L.mcg_kmc_advance.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_int64]
         _lib = L
@@ -283,6 +286,25 @@ class Run:
         v = np.zeros(4 * n_atoms, dtype=np.int64)
         _check(lib().mcg_get_atoms(self.h, c, v.ctypes.data_as(C.POINTER(C.c_int64)), n_atoms))
         return v.reshape(n_atoms, 4)
+
+    def kmc_collect_hops(self, jumps_per_position_sample, max_n_position_samples=100):
+        """KMC: record hop correlations on the device (0 turns recording off)."""
+        self._max_hops = int(max_n_position_samples)
+        _check(lib().mcg_kmc_collect_hops(self.h, int(jumps_per_position_sample),
+                                          int(max_n_position_samples)))
+
+    def kmc_hops(self, c, n_atoms):
+        """KMC: (counts[n_atoms], records[max, n_atoms, 4] = dx, dy, step, pass,
+        times[max, n_atoms]) of chain c."""
+        m = self._max_hops
+        counts = np.zeros(n_atoms, dtype=np.int64)
+        rec = np.zeros(m * n_atoms * 4, dtype=np.int64)
+        t = np.zeros(m * n_atoms, dtype=np.float64)
+        _check(lib().mcg_kmc_get_hops(self.h, c, n_atoms,
+                                      counts.ctypes.data_as(C.POINTER(C.c_int64)),
+                                      rec.ctypes.data_as(C.POINTER(C.c_int64)),
+                                      t.ctypes.data_as(C.POINTER(C.c_double))))
+        return counts, rec.reshape(m, n_atoms, 4), t.reshape(m, n_atoms)
 
     def kmc_advance(self, max_events, stop_time=float("inf"), select_at_end=False):
         """KMC: apply the pending event and events up to max_events; an event

@@ -453,6 +453,12 @@ struct mcg_handle {
   KmcParams *d_kp_kmc = nullptr;
   std::vector<KmcParams> kp_kmc;
   int32_t *d_atoms = nullptr;
+  // hop correlations (mcg_kmc_collect_hops): every kmc_hop_every-th jump, at
+  // most kmc_hop_max records per atom; records kmc_hop_max x lat_bytes, counts
+  // one per site slot of the lattice pool
+  int32_t kmc_hop_every = 0, kmc_hop_max = 0;
+  KmcHop *d_hop = nullptr;
+  int32_t *d_hop_cnt = nullptr;
   int kmc_lds = 0;
   int64_t kmc_spin_limit = int64_t(1) << 26;
   bool kmc() const { return mode == MCG_MODE_KMC; }
@@ -531,7 +537,7 @@ struct mcg_handle {
     void *ptrs[] = {d_lat, d_active, d_series, d_stats, d_eq, d_xp, d_xs, d_cp, d_cs, d_tab,
                     d_work, d_ftab, d_tri, d_tri_lines, d_band, d_pair, d_kp, d_list, d_np,
                     d_mask_lds, d_mask_even, d_ring_list, d_batch, d_op, d_ot, d_kp_kmc,
-                    d_atoms, d_fcc_sched};
+                    d_atoms, d_fcc_sched, d_hop, d_hop_cnt};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -1056,6 +1062,9 @@ void fill_kmc_params(mcg_handle *h, int c, int64_t n_atoms, KmcParams &P) {
   P.J = h->J1;
   P.spin_limit = h->kmc_spin_limit;
   P.lat_off = h->lat_off[c];
+  P.hop_every = h->kmc_hop_every;
+  P.hop_max = h->kmc_hop_max;
+  P.hop_off = (int64_t)h->kmc_hop_max * h->lat_off[c];
   const double beta = 1.0 / (h->kb * ch.temperature);
   for (int k = 0; k < KMC_NCLASS; ++k) {
     const double dE = h->J1 * (double)(k - 3);
@@ -1092,6 +1101,8 @@ int kmc_init(mcg_handle *h, int64_t c, std::vector<int8_t> const &occ) {
                       sizeof(int32_t) * atoms.size(), hipMemcpyHostToDevice));
   fill_kmc_params(h, (int)c, k1, h->kp_kmc[c]);
   HIP_TRY(copy_sync(h, h->d_kp_kmc + c, &h->kp_kmc[c], sizeof(KmcParams), hipMemcpyHostToDevice));
+  if (h->d_hop_cnt)  // the chain's hop correlations start again with its atoms
+    HIP_TRY(fill_sync(h, h->d_hop_cnt + h->lat_off[c], 0, sizeof(int32_t) * n));
   ExactState st;
   HIP_TRY(copy_sync(h, &st, h->d_xs + c, sizeof(st), hipMemcpyDeviceToHost));
   st.time = 0.0;
@@ -1171,7 +1182,8 @@ int kmc_run(mcg_handle *h, int64_t n_passes, int64_t n_steps,
   int rc = ensure_series(h, need);
   if (rc != MCG_OK) return rc;
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
-  launch_kmc(h->d_kp_kmc, h->d_xs, h->d_lat, h->d_list, h->d_atoms, h->d_series, h->series_cap,
+  launch_kmc(h->d_kp_kmc, h->d_xs, h->d_lat, h->d_list, h->d_atoms, h->d_hop, h->d_hop_cnt,
+             h->d_series, h->series_cap,
              h->d_active, h->n_chains, n_passes, n_steps, h->sample_period, stop_time,
              select_at_end ? 1 : 0, h->kmc_lds, h->stream);
   HIP_TRY(hipGetLastError());
@@ -2534,6 +2546,73 @@ int mcg_kmc_advance(mcg_handle *h, int64_t max_events, double stop_time, int64_t
   if (max_events < 0 || std::isnan(stop_time)) return fail(MCG_ERR_ARG, "bad event count or stop time");
   if (h->nfold()) return nfold_advance(h, max_events, stop_time, select_at_end != 0);
   return kmc_run(h, 0, max_events, stop_time, select_at_end != 0);
+}
+
+int mcg_kmc_collect_hops(mcg_handle *h, int64_t jumps_per_position_sample,
+                         int64_t max_n_position_samples) {
+  if (!h) return fail(MCG_ERR_ARG, "null handle");
+  if (!h->kmc()) return fail(MCG_ERR_STATE, "hop correlations exist in KMC mode only");
+  if (jumps_per_position_sample < 0 || jumps_per_position_sample > (1 << 30))
+    return fail(MCG_ERR_ARG, "jumps_per_position_sample must lie in [0, 2^30]");
+  if (jumps_per_position_sample > 0 &&
+      (max_n_position_samples < 1 || max_n_position_samples > 100000))
+    return fail(MCG_ERR_ARG, "max_n_position_samples must lie in [1, 100000]");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (jumps_per_position_sample == 0) {
+    h->kmc_hop_every = 0;
+  } else {
+    if (h->d_hop) (void)hipFree(h->d_hop);
+    h->d_hop = nullptr;
+    const size_t recs = (size_t)max_n_position_samples * (size_t)h->lat_bytes;
+    if (hipMalloc(&h->d_hop, sizeof(KmcHop) * recs) != hipSuccess) {
+      h->d_hop = nullptr;
+      h->kmc_hop_every = 0;
+      return fail(MCG_ERR_CAPACITY, "out of device memory for the hop-correlation records");
+    }
+    if (!h->d_hop_cnt) HIP_TRY(hipMalloc(&h->d_hop_cnt, sizeof(int32_t) * (size_t)h->lat_bytes));
+    HIP_TRY(fill_sync(h, h->d_hop_cnt, 0, sizeof(int32_t) * (size_t)h->lat_bytes));
+    h->kmc_hop_every = (int32_t)jumps_per_position_sample;
+    h->kmc_hop_max = (int32_t)max_n_position_samples;
+  }
+  for (int c = 0; c < h->n_chains; ++c) {
+    h->kp_kmc[c].hop_every = h->kmc_hop_every;
+    h->kp_kmc[c].hop_max = h->kmc_hop_max;
+    h->kp_kmc[c].hop_off = (int64_t)h->kmc_hop_max * h->lat_off[c];
+  }
+  HIP_TRY(copy_sync(h, h->d_kp_kmc, h->kp_kmc.data(), sizeof(KmcParams) * h->n_chains,
+                    hipMemcpyHostToDevice));
+  return MCG_OK;
+}
+
+int mcg_kmc_get_hops(mcg_handle *h, int64_t chain, int64_t n_atoms, int64_t *counts,
+                     int64_t *records, double *times) {
+  if (!valid_chain(h, chain) || !counts || !records || !times)
+    return fail(MCG_ERR_ARG, "bad chain or buffer");
+  if (!h->kmc() || !h->d_hop || !h->d_hop_cnt)
+    return fail(MCG_ERR_STATE, "no hop correlations collected (mcg_kmc_collect_hops)");
+  if (n_atoms != h->kp_kmc[chain].n_atoms) return fail(MCG_ERR_ARG, "atom count mismatch");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const int64_t n = h->sites(chain), m = h->kmc_hop_max;
+  std::vector<int32_t> cnt(n_atoms);
+  if (n_atoms > 0)
+    HIP_TRY(copy_sync(h, cnt.data(), h->d_hop_cnt + h->lat_off[chain], sizeof(int32_t) * n_atoms,
+                      hipMemcpyDeviceToHost));
+  std::vector<KmcHop> rec((size_t)(m * n));
+  HIP_TRY(copy_sync(h, rec.data(), h->d_hop + (size_t)m * (size_t)h->lat_off[chain],
+                    sizeof(KmcHop) * rec.size(), hipMemcpyDeviceToHost));
+  for (int64_t a = 0; a < n_atoms; ++a) counts[a] = cnt[a];
+  for (int64_t k = 0; k < m; ++k)
+    for (int64_t a = 0; a < n_atoms; ++a) {
+      const bool have = k < cnt[a];
+      KmcHop const &r = rec[(size_t)(k * n + a)];
+      int64_t *o = records + 4 * (k * n_atoms + a);
+      o[0] = have ? r.dx : 0;
+      o[1] = have ? r.dy : 0;
+      o[2] = have ? r.step : 0;
+      o[3] = have ? r.pass : 0;
+      times[k * n_atoms + a] = have ? r.time : 0.0;
+    }
+  return MCG_OK;
 }
 
 int mcg_kmc_set_spin_limit(mcg_handle *h, int64_t attempts) {

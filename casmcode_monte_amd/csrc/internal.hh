@@ -146,6 +146,19 @@ struct KmcParams {
   int64_t lat_off;     // lattice byte offset; the lists start at entry lat_off of the
                        // list pool, the atoms (4 int32 each) at entry 4 lat_off of the atom pool
   uint64_t thr[KMC_NCLASS];
+  // hop correlations (0: off): every hop_every-th jump of an atom is recorded,
+  // at most hop_max per atom; record k of atom a at hop pool entry
+  // hop_off + k * n + a, the atoms' record counts at count pool entry lat_off + a
+  int32_t hop_every, hop_max;
+  int64_t hop_off;
+};
+
+// One hop-correlation record: the atom's displacement since its occupation was
+// set, the step within the pass and the pass of the event, and its time.
+struct KmcHop {
+  int32_t dx, dy;
+  int64_t step, pass;
+  double time;
 };
 
 // LDS bytes of one chain: list, positions and site atoms (16-bit), atom
@@ -161,9 +174,10 @@ inline int kmc_lds_bytes(int n) {
 // (the chain stops there); with select_at_end a chain that reached the count
 // selects one more event and leaves it pending.
 void launch_kmc(const KmcParams *P, ExactState *S, int8_t *lat_pool, int32_t *list_pool,
-                int32_t *atom_pool, mcg_sample *series, int64_t series_cap, const uint8_t *active,
-                int n_chains, int64_t n_passes, int64_t n_steps, int64_t sample_period,
-                double stop_time, int select_at_end, int lds_bytes, hipStream_t stream);
+                int32_t *atom_pool, KmcHop *hop_pool, int32_t *hop_count_pool,
+                mcg_sample *series, int64_t series_cap, const uint8_t *active, int n_chains,
+                int64_t n_passes, int64_t n_steps, int64_t sample_period, double stop_time,
+                int select_at_end, int lds_bytes, hipStream_t stream);
 hipError_t kmc_set_max_lds(int bytes);
 
 // ---------------------------------------------------------------------------

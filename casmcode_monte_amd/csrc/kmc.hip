@@ -115,6 +115,8 @@ __global__ __launch_bounds__(128) void kmc_kernel(const KmcParams *__restrict__ 
                                                  int8_t *__restrict__ lat_pool,
                                                  int32_t *__restrict__ list_pool,
                                                  int32_t *__restrict__ atom_pool,
+                                                 KmcHop *__restrict__ hop_pool,
+                                                 int32_t *__restrict__ hop_count_pool,
                                                  mcg_sample *__restrict__ series,
                                                  int64_t series_cap,
                                                  const uint8_t *__restrict__ active,
@@ -246,6 +248,25 @@ __global__ __launch_bounds__(128) void kmc_kernel(const KmcParams *__restrict__ 
     sum_sq += 2 * (int64_t)dx * ddx + 2 * (int64_t)dy * ddy + 1;
   };
 
+  // hop correlations: after an atom's hop (by the lane that applied it, time
+  // already the event's, step and pass still the event's own), its
+  // displacement when its jump count is a multiple of hop_every
+  const int32_t hop_every = P->hop_every, hop_max = P->hop_max;
+  auto hop_record = [&](int32_t atom) {
+    if (hop_every <= 0 || ajump[atom] % hop_every != 0) return;
+    int32_t *cnt = hop_count_pool + P->lat_off;
+    const int32_t k = cnt[atom];
+    if (k >= hop_max) return;
+    KmcHop r;
+    r.dx = adx[atom];
+    r.dy = ady[atom];
+    r.step = step;
+    r.pass = n_pass;
+    r.time = time;
+    hop_pool[P->hop_off + (int64_t)k * n + atom] = r;
+    cnt[atom] = k + 1;
+  };
+
   // a selected event: left pending (its site, direction, attempts and time
   // kept for the next launch) or applied
   bool stopped = false;
@@ -277,7 +298,10 @@ __global__ __launch_bounds__(128) void kmc_kernel(const KmcParams *__restrict__ 
       kmc_eval(a0, (uint32_t)(n - 1 - (int32_t)pos[from]), pending_dir, n, list, site_atom, adx,
                ady, rows, cols, magic);
       account(pending_k, a0.cls, a0.dir, a0.dx, a0.dy);
-      if (lane == 0) kmc_apply(a0, pba, list, pos, site_atom, adx, ady, ajump);
+      if (lane == 0) {
+        kmc_apply(a0, pba, list, pos, site_atom, adx, ady, ajump);
+        hop_record(a0.atom);
+      }
       asm volatile("" ::: "memory");
       pending_site = -1;
       bump();
@@ -337,7 +361,10 @@ __global__ __launch_bounds__(128) void kmc_kernel(const KmcParams *__restrict__ 
         pend(from, dir, k_run, event_time);
       } else {
         account(k_run, cls, dir, dx, dy);
-        if (lane == f) kmc_apply(at, pba, list, pos, site_atom, adx, ady, ajump);
+        if (lane == f) {
+          kmc_apply(at, pba, list, pos, site_atom, adx, ady, ajump);
+          hop_record(at.atom);
+        }
         asm volatile("" ::: "memory");
         bump();
       }
@@ -369,7 +396,10 @@ __global__ __launch_bounds__(128) void kmc_kernel(const KmcParams *__restrict__ 
           pend(a1.from, a1.dir, k_run, event_time);
         } else {
           account(k_run, a1.cls, a1.dir, a1.dx, a1.dy);
-          if (lane == 0) kmc_apply(a1, pba, list, pos, site_atom, adx, ady, ajump);
+          if (lane == 0) {
+            kmc_apply(a1, pba, list, pos, site_atom, adx, ady, ajump);
+            hop_record(a1.atom);
+          }
           asm volatile("" ::: "memory");
           bump();
         }
@@ -429,12 +459,13 @@ __global__ __launch_bounds__(128) void kmc_kernel(const KmcParams *__restrict__ 
 }
 
 void launch_kmc(const KmcParams *P, ExactState *S, int8_t *lat_pool, int32_t *list_pool,
-                int32_t *atom_pool, mcg_sample *series, int64_t series_cap, const uint8_t *active,
-                int n_chains, int64_t n_passes, int64_t n_steps, int64_t sample_period,
-                double stop_time, int select_at_end, int lds_bytes, hipStream_t stream) {
+                int32_t *atom_pool, KmcHop *hop_pool, int32_t *hop_count_pool,
+                mcg_sample *series, int64_t series_cap, const uint8_t *active, int n_chains,
+                int64_t n_passes, int64_t n_steps, int64_t sample_period, double stop_time,
+                int select_at_end, int lds_bytes, hipStream_t stream) {
   hipLaunchKernelGGL(kmc_kernel, dim3(n_chains), dim3(128), lds_bytes, stream, P, S, lat_pool,
-                     list_pool, atom_pool, series, series_cap, active, n_passes, n_steps,
-                     sample_period, stop_time, select_at_end);
+                     list_pool, atom_pool, hop_pool, hop_count_pool, series, series_cap, active,
+                     n_passes, n_steps, sample_period, stop_time, select_at_end);
 }
 
 hipError_t kmc_set_max_lds(int bytes) {

@@ -20,3 +20,5 @@ EquilibrationCheckResults = _s.EquilibrationCheckResults
 ConvergenceCheckResults = _s.ConvergenceCheckResults
 converge = _s.converge
 get_n_samples = _s.get_n_samples
+CorrelationsDataParams = _s.CorrelationsDataParams
+SelectedEventFunctionParams = _s.SelectedEventFunctionParams

@@ -15,6 +15,8 @@
 // library through the C++ drop-in (include/casm/monte/ising_cpp/
 // basic_semigrand_canonical.hh); nothing here runs a Metropolis step on the
 // host.
+#include <sstream>
+
 #include <pybind11/functional.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -23,6 +25,7 @@
 
 #include "casm/monte/events/OccEventProposal.hh"
 #include "casm/monte/ising_cpp/basic_semigrand_canonical.hh"
+#include "casm/monte/sampling/io/json/SelectedEventFunctions_json_io.hh"
 
 namespace py = pybind11;
 using namespace CASM;
@@ -56,6 +59,16 @@ py::array_t<double> to_numpy(VectorXd const &v) {
 }
 
 VectorXd from_sequence(std::vector<double> const &v) { return VectorXd(v); }
+
+// JSON values to and from Python objects (through their text)
+py::object json_to_py(jsonParser const &j) {
+  std::ostringstream ss;
+  j.print(ss, -1);
+  return py::module_::import("json").attr("loads")(ss.str());
+}
+jsonParser py_to_json(py::object const &o) {
+  return jsonParser::parse(py::module_::import("json").attr("dumps")(o).cast<std::string>());
+}
 
 py::dict to_dict(RequestedPrecision const &r) {
   py::dict d;
@@ -377,6 +390,83 @@ PYBIND11_MODULE(_libcasm_monte, m) {
       .def_readonly("component_names", &StateSamplingFunction::component_names)
       .def("__call__", [](StateSamplingFunction const &f) { return to_numpy(f()); });
   py::bind_map<StateSamplingFunctionMap>(ms, "StateSamplingFunctionMap");
+
+  // selected-event statistics parameters (python/src/monte_sampling.cpp:
+  // 3267-3371, 4059-4330)
+  py::class_<CorrelationsDataParams>(ms, "CorrelationsDataParams")
+      .def(py::init([](Index j, Index m, bool o, bool st) {
+             return CorrelationsDataParams{j, m, o, st};
+           }),
+           py::arg("jumps_per_position_sample") = 1, py::arg("max_n_position_samples") = 100,
+           py::arg("output_incomplete_samples") = false,
+           py::arg("stop_run_when_complete") = false)
+      .def_readwrite("jumps_per_position_sample", &CorrelationsDataParams::jumps_per_position_sample)
+      .def_readwrite("max_n_position_samples", &CorrelationsDataParams::max_n_position_samples)
+      .def_readwrite("output_incomplete_samples", &CorrelationsDataParams::output_incomplete_samples)
+      .def_readwrite("stop_run_when_complete", &CorrelationsDataParams::stop_run_when_complete)
+      .def("copy", [](CorrelationsDataParams const &p) { return CorrelationsDataParams(p); })
+      .def("to_dict",
+           [](CorrelationsDataParams const &p) {
+             jsonParser j;
+             to_json(p, j);
+             return json_to_py(j);
+           })
+      .def_static("from_dict", [](py::dict d) {
+        return correlations_data_params_from_json(py_to_json(d));
+      });
+  py::class_<SelectedEventFunctionParams, std::shared_ptr<SelectedEventFunctionParams>>(
+      ms, "SelectedEventFunctionParams")
+      .def(py::init<>())
+      .def_readwrite("correlations_data_params",
+                     &SelectedEventFunctionParams::correlations_data_params)
+      .def_readonly("function_names", &SelectedEventFunctionParams::function_names)
+      .def(
+          "collect_hop_correlations",
+          [](SelectedEventFunctionParams &self, Index j, Index m, bool o, bool st) {
+            self.correlations_data_params = CorrelationsDataParams{j, m, o, st};
+            return self;
+          },
+          py::arg("jumps_per_position_sample") = 1, py::arg("max_n_position_samples") = 100,
+          py::arg("output_incomplete_samples") = false, py::arg("stop_run_when_complete") = false)
+      .def("do_not_collect_hop_correlations",
+           [](SelectedEventFunctionParams &self) {
+             self.correlations_data_params = std::nullopt;
+             return self;
+           })
+      .def("evaluate", &SelectedEventFunctionParams::evaluate, py::arg("name"),
+           py::arg("order") = std::nullopt)
+      .def("collect", &SelectedEventFunctionParams::collect, py::arg("name"),
+           py::arg("tol") = std::nullopt, py::arg("bin_width") = std::nullopt,
+           py::arg("initial_begin") = std::nullopt, py::arg("spacing") = std::nullopt,
+           py::arg("max_size") = std::nullopt)
+      .def("do_not_collect", &SelectedEventFunctionParams::do_not_collect, py::arg("name"))
+      .def(
+          "get_parameters",
+          [](SelectedEventFunctionParams const &self, std::string name) {
+            jsonParser j = jsonParser::object();
+            if (self.tol.count(name)) j["tol"] = self.tol.at(name);
+            if (self.bin_width.count(name)) j["bin_width"] = self.bin_width.at(name);
+            if (self.initial_begin.count(name)) j["initial_begin"] = self.initial_begin.at(name);
+            if (self.is_log.count(name)) j["spacing"] = self.is_log.at(name) ? "log" : "linear";
+            if (self.max_size.count(name)) j["max_size"] = self.max_size.at(name);
+            return json_to_py(j);
+          },
+          py::arg("name"))
+      .def("reset", &SelectedEventFunctionParams::reset)
+      .def("copy",
+           [](SelectedEventFunctionParams const &self) {
+             return std::make_shared<SelectedEventFunctionParams>(self);
+           })
+      .def("to_dict",
+           [](SelectedEventFunctionParams const &self) {
+             jsonParser j;
+             to_json(self, j);
+             return json_to_py(j);
+           })
+      .def_static("from_dict", [](py::dict d) {
+        return std::make_shared<SelectedEventFunctionParams>(
+            selected_event_function_params_from_json(py_to_json(d)));
+      });
 
   py::class_<jsonStateSamplingFunction>(ms, "jsonStateSamplingFunction")
       .def_readonly("name", &jsonStateSamplingFunction::name)

@@ -85,6 +85,19 @@ class Handle {
   void kmc_advance(CountType max_events, double stop_time, bool select_at_end) {
     check(mcg_kmc_advance(m_h, max_events, stop_time, select_at_end ? 1 : 0));
   }
+  /// KMC: record hop correlations on the device (0 turns recording off)
+  void kmc_collect_hops(Index jumps_per_position_sample, Index max_n_position_samples) {
+    check(mcg_kmc_collect_hops(m_h, jumps_per_position_sample, max_n_position_samples));
+  }
+  /// KMC: hop-correlation records of a chain: counts, (dx, dy, step, pass) x
+  /// max x n_atoms and times
+  void kmc_hops(Index chain, Index n_atoms, Index max_n, std::vector<int64_t> &counts,
+                std::vector<int64_t> &records, std::vector<double> &times) const {
+    counts.assign(n_atoms, 0);
+    records.assign(4 * max_n * n_atoms, 0);
+    times.assign(max_n * n_atoms, 0.0);
+    check(mcg_kmc_get_hops(m_h, chain, n_atoms, counts.data(), records.data(), times.data()));
+  }
   /// KMC: per atom (site, displacement along rows, along columns, jumps)
   std::vector<int64_t> atoms(Index chain, Index n_atoms) const {
     std::vector<int64_t> a(4 * n_atoms);

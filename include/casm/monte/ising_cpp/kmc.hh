@@ -25,6 +25,7 @@
 #include "casm/monte/events/OccLocation.hh"
 #include "casm/monte/ising_cpp/model.hh"
 #include "casm/monte/run_management/RunManager.hh"
+#include "casm/monte/sampling/SelectedEventFunctions.hh"
 
 namespace CASM {
 namespace monte {
@@ -66,6 +67,13 @@ class LatticeGasKmcEventSelector {
   double J;
   double temperature;
   std::shared_ptr<std::mt19937_64> engine;
+
+  /// Hop correlations to collect during kinetic_monte_carlo(), on the device
+  /// (the role of SelectedEventFunctionParams::correlations_data_params); at
+  /// the end of the run the data is in selected_event_data->correlations_data
+  /// (stop_run_when_complete is not supported)
+  std::optional<CorrelationsDataParams> correlations_data_params;
+  std::shared_ptr<SelectedEventData> selected_event_data;
 
   /// Rate of one hop in the current configuration (0 if the target holds an atom)
   double rate(LatticeGasHop h) const {

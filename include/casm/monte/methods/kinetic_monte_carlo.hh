@@ -19,8 +19,12 @@
 // selected and not applied (mcg_kmc_advance), so the samples see the
 // reference's state, time and engine.
 //
+// Hop correlations (the CorrelationsData of the selected-event statistics) are
+// recorded by the device when the selector's correlations_data_params is set,
+// and left in its selected_event_data.
+//
 // Not supported: other event selectors and configurations, simulated-time
-// cutoffs (min_time, max_time), selected-event statistics, unique atom ids.
+// cutoffs (min_time, max_time), selected-event histograms, unique atom ids.
 #ifndef CASM_MI355X_monte_methods_kinetic_monte_carlo
 #define CASM_MI355X_monte_methods_kinetic_monte_carlo
 
@@ -135,6 +139,19 @@ void kinetic_monte_carlo(
   gpu::Handle dev(d);
   dev.set_occupation(0, std::vector<int8_t>(config.occupation().begin(), config.occupation().end()));
   dev.load_engine(0, *event_selector.engine);
+  // hop correlations, recorded by the device at every jumps_per_position_sample-th
+  // jump of each atom
+  auto const &hop_params = event_selector.correlations_data_params;
+  if (hop_params.has_value()) {
+    if (hop_params->stop_run_when_complete)
+      throw std::runtime_error(
+          "kinetic_monte_carlo (MI355X): stop_run_when_complete is not supported");
+    if (!event_selector.selected_event_data)
+      throw std::runtime_error(
+          "kinetic_monte_carlo (MI355X): correlations_data_params without selected_event_data");
+    dev.kmc_collect_hops(hop_params->jumps_per_position_sample,
+                         hop_params->max_n_position_samples);
+  }
 
   auto positions = [&]() {
     auto a = dev.atoms(0, n_atoms);
@@ -272,6 +289,35 @@ void kinetic_monte_carlo(
     occ_location.set_locations(lists);
     occ_location.set_atoms(atoms, component);
     kmc_data.atom_positions_cart = occ_location.atom_positions_cart();
+  }
+  if (hop_params.has_value()) {
+    // the device's records as CorrelationsData (positions as if there were no
+    // periodic boundary: the initial translation plus the displacement); the
+    // complete samples are those every atom has
+    const Index m = hop_params->max_n_position_samples;
+    std::vector<int64_t> counts, rec;
+    std::vector<double> times;
+    dev.kmc_hops(0, n_atoms, m, counts, rec, times);
+    CorrelationsData cd;
+    cd.initialize(n_atoms, hop_params->jumps_per_position_sample, m,
+                  hop_params->output_incomplete_samples);
+    cd.stop_run_when_complete = false;
+    CountType complete = n_atoms > 0 ? m : 0;
+    for (Index a = 0; a < n_atoms; ++a) {
+      cd.n_position_samples[a] = counts[a];
+      complete = std::min<CountType>(complete, counts[a]);
+      for (Index k = 0; k < counts[a]; ++k) {
+        int64_t const *r = rec.data() + 4 * (k * n_atoms + a);
+        cd.atom_positions_cart[k](0, a) = (double)(atoms0[a].translation[0] + r[0]);
+        cd.atom_positions_cart[k](1, a) = (double)(atoms0[a].translation[1] + r[1]);
+        cd.atom_positions_cart[k](2, a) = (double)atoms0[a].translation[2];
+        cd.step(k, a) = r[2];
+        cd.pass(k, a) = r[3];
+        cd.time(k, a) = times[k * n_atoms + a];
+      }
+    }
+    cd.n_complete_samples = complete;
+    event_selector.selected_event_data->correlations_data = std::move(cd);
   }
   dev.store_engine(0, *event_selector.engine);
   (void)get_event_f;

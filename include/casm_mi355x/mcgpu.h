@@ -282,6 +282,24 @@ int mcg_set_occ_location(mcg_handle *h, int64_t chain, const int64_t *loc, int64
  * (Atom::translation and n_jumps, OccEvent.hh:13-20; OccLocation.cc:286-330).
  * n_atoms must equal the chain's atom count. */
 int mcg_get_atoms(mcg_handle *h, int64_t chain, int64_t *atoms, int64_t n_atoms);
+/* KMC: hop correlations collected on the device (the CorrelationsData of
+ * sampling/SelectedEventFunctions.hh:37-96): from now on, every
+ * jumps_per_position_sample-th jump of each atom records the atom's
+ * displacement since its chain's occupation was set, the index of the event
+ * within its pass, the pass and the event's time, at most
+ * max_n_position_samples records per atom (later jumps are not recorded).
+ * Every call starts the records afresh; setting a chain's occupation starts
+ * that chain's afresh; 0 turns recording off (the records stay readable).
+ * Device memory: 32 B x max_n_position_samples per lattice site. */
+int mcg_kmc_collect_hops(mcg_handle *h, int64_t jumps_per_position_sample,
+                         int64_t max_n_position_samples);
+/* KMC: one chain's hop-correlation records: counts[a] records of atom a;
+ * record k of atom a is records[4 (k n_atoms + a) + 0..3] = (displacement
+ * along rows, along columns, step within the pass, pass) and
+ * times[k n_atoms + a], for k < max_n_position_samples (zeros past counts[a]).
+ * n_atoms must equal the chain's atom count. */
+int mcg_kmc_get_hops(mcg_handle *h, int64_t chain, int64_t n_atoms, int64_t *counts,
+                     int64_t *records, double *times);
 /* KMC: attempts without an event after which a chain stops (default 2^26). */
 int mcg_kmc_set_spin_limit(mcg_handle *h, int64_t attempts);
 /* KMC and NFOLD: the reference loops' select-then-sample order

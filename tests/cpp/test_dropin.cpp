@@ -1785,6 +1785,84 @@ static void cpu_selected_event_tests() {
             << std::endl;
 }
 
+// kinetic_monte_carlo() with hop correlations recorded by the device (the
+// selector's correlations_data_params): every 2nd jump, at most 3 positions
+// per atom.  Each atom has min(3, jumps / 2) positions, the complete count is
+// the least of them, an atom whose jumps are a multiple of 2 and that has
+// fewer than 3 positions ends at its last stored position, positions and
+// times only grow, and the run itself equals the oracle's.
+static void gpu_kmc_hops() {
+  const int R = 8, C = 7, N = R * C;
+  const long P = 6;
+  const double J = 0.05, T = 800.0;
+  const uint64_t seed = 77;
+  typedef ising_cpp::IsingConfiguration config_type;
+  typedef RunManager<config_type, BasicStatistics, std::mt19937_64> rm_type;
+  std::vector<int> occ0 = random_occupants(N, 20, 5);
+  config_type cfg({R, C}, 0);
+  cfg.set_occupation(occ0);
+  ValueMap cond;
+  cond.scalar_values["temperature"] = T;
+  State<config_type> state(cfg, cond);
+  Conversions cv(R, C, {"Va", "A"});
+  OccCandidateList cl(cv);
+  OccLocation ol(cv, cl, true);
+  ol.initialize(state.configuration.occupation());
+  const Index n_atoms = ol.atom_size();
+  auto engine = std::make_shared<std::mt19937_64>(seed);
+  ising_cpp::LatticeGasKmcEventSelector selector(J, T, engine, &state, &ol);
+  selector.correlations_data_params = CorrelationsDataParams{2, 3, false, false};
+  selector.selected_event_data = std::make_shared<SelectedEventData>();
+  KMCData<config_type, BasicStatistics, std::mt19937_64> kd;
+  StateSamplingFunctionMap fns;
+  fns.emplace("time", StateSamplingFunction("time", "KMC time", {}, [&]() { return VectorXd{kd.time}; }));
+  SamplingParams sp;
+  sp.sampler_names = {"time"};
+  CompletionCheckParams<BasicStatistics> cc;
+  cc.cutoff_params.max_count = P;
+  std::vector<rm_type::sampling_fixture_params_type> fixtures;
+  fixtures.emplace_back("kmc", fns, jsonStateSamplingFunctionMap(),
+                        ResultsAnalysisFunctionMap<config_type, BasicStatistics>(), sp, cc,
+                        std::vector<std::string>{});
+  rm_type rm(engine, fixtures, true);
+  kinetic_monte_carlo<ising_cpp::LatticeGasHop>(
+      state, ol, kd, selector,
+      [&](ising_cpp::LatticeGasHop h) -> OccEvent const & { return selector.get_event(h); }, rm);
+  const std::string tag = "kinetic_monte_carlo hop correlations";
+  std::mt19937_64 e1(seed);
+  auto o = refcpu::kmc_run(R, C, occ0, J, T, e1, P * N);
+  EXPECT(*engine == e1, tag << ": engine state");
+  auto const &cdo = selector.selected_event_data->correlations_data;
+  EXPECT(cdo.has_value(), tag << ": data present");
+  if (cdo.has_value()) {
+    auto const &cd = *cdo;
+    bool ok = (Index)cd.n_position_samples.size() == n_atoms;
+    CountType least = 3;
+    MatrixXd fin = ol.atom_positions_cart();
+    for (Index a = 0; ok && a < n_atoms; ++a) {
+      const Index jumps = ol.atom(a).n_jumps;
+      const CountType want = std::min<CountType>(3, jumps / 2);
+      ok = cd.n_position_samples[a] == want;
+      least = std::min(least, want);
+      for (CountType k = 1; ok && k < want; ++k)
+        ok = cd.time(k, a) > cd.time(k - 1, a) &&
+             cd.pass(k, a) * N + cd.step(k, a) > cd.pass(k - 1, a) * N + cd.step(k - 1, a);
+      if (ok && want > 0 && want < 3 && jumps % 2 == 0)
+        ok = cd.atom_positions_cart[want - 1](0, a) == fin(0, a) &&
+             cd.atom_positions_cart[want - 1](1, a) == fin(1, a);
+      if (ok && want > 0) ok = cd.time(want - 1, a) <= o.time;
+    }
+    EXPECT(ok && cd.n_complete_samples == least && cd.jumps_per_position_sample == 2,
+           tag << ": records against the final atoms");
+    jsonParser j;
+    to_json(*selector.selected_event_data, j);
+    EXPECT(j.contains("correlations_data") &&
+               j["correlations_data"]["n_complete_samples"].get<long>() == (long)least,
+           tag << ": JSON");
+  }
+  std::cout << "gpu: " << tag << " 8x7" << (g_fail ? " (failures so far)" : " ok") << std::endl;
+}
+
 static void cpu_kmc_atom_tracking() {
   const int R = 6, C = 7, N = R * C;
   std::vector<int> occ(N, 0);
@@ -1856,6 +1934,7 @@ static void gpu_tests() {
   gpu_occupant_no_events();
   gpu_kmc_case(61);
   gpu_kmc_case(62);
+  gpu_kmc_hops();
   gpu_nfold_case(false);
   gpu_nfold_case(true);
   gpu_nfold_two_fixtures();

@@ -224,3 +224,43 @@ def test_advance_selects_then_applies():
         assert np.array_equal(r.get_occupation(0), o["occ"])
         assert np.array_equal(r.rng_dump(0), o["engine"])
         assert r.nfold_state(0)[0] == o["final_time"]
+
+
+@pytest.mark.parametrize("every,max_samples", [(1, 3), (3, 4)])
+def test_hop_correlations_on_device(every, max_samples):
+    """Hop correlations recorded by the kernel (mcg_kmc_collect_hops): every
+    `every`-th jump of each atom, at most `max_samples` per atom, the atom's
+    displacement, the event's step within its pass, its pass and its time --
+    against the oracle replayed one event at a time (each prefix run shows
+    which atom the event moved, its displacement, jumps and the time)."""
+    rows, cols, J, T, seed, passes = 6, 5, 0.05, 900.0, 91, 5
+    n = rows * cols
+    occ = _occ(n, 0.4, 92)
+    na = int(occ.sum())
+    print(f"kmc hops every={every} max={max_samples} atoms={na}", flush=True)
+    with M.Run([dict(shape=(rows, cols), temperature=T, seed=seed)], mode=M.MODE_KMC, J1=J,
+               sample_period=1) as r:
+        r.set_occupation(0, occ)
+        r.kmc_collect_hops(every, max_samples)
+        r.reserve_samples(64)
+        r.run(passes)
+        counts, rec, times = r.kmc_hops(0, na)
+        g_atoms = r.atoms(0, na)
+    want = [[] for _ in range(na)]
+    prev = O.kmc_run(rows, cols, J, T, 0, occ, O.mt64_state(seed, 0))["atoms"]
+    for e in range(1, passes * n + 1):
+        o = O.kmc_run(rows, cols, J, T, e, occ, O.mt64_state(seed, 0))
+        moved = np.nonzero(o["atoms"][:, 3] != prev[:, 3])[0]
+        assert len(moved) == 1
+        a = int(moved[0])
+        _, dx, dy, jumps = o["atoms"][a]
+        if jumps % every == 0 and len(want[a]) < max_samples:
+            want[a].append((dx, dy, (e - 1) % n, (e - 1) // n, o["final_time"]))
+        prev = o["atoms"]
+    assert np.array_equal(g_atoms, prev)
+    for a in range(na):
+        assert counts[a] == len(want[a]), a
+        for k, (dx, dy, st, ps, t) in enumerate(want[a]):
+            assert tuple(rec[k, a]) == (dx, dy, st, ps), (a, k)
+            assert times[k, a] == t, (a, k)
+        assert np.all(rec[counts[a]:, a] == 0)
