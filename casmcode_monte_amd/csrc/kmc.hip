@@ -330,46 +330,106 @@ __global__ __launch_bounds__(128) void kmc_kernel(const KmcParams *__restrict__ 
     const uint64_t AC = __ballot(valid && acc);
     const uint64_t SP = __ballot(special);
 
-    // attempt starts: 0, then +2 or +3 (a uniform drawn), up to the first
-    // accepted attempt or before the first special one
-    uint64_t exec = 0;
+    // attempt starts over the window: 0, then +2 or +3 (a uniform drawn), up
+    // to the first special one
+    uint64_t A = 0;
     for (int t = 0; t < W;) {
       const uint64_t bit = 1ull << t;
       if (SP & bit) break;
-      exec |= bit;
-      if (AC & bit) break;
+      A |= bit;
       t += 2 + (int)((ND >> t) & 1ull);
     }
-    const int attempts = __popcll(exec);
-    const uint64_t F = exec & AC;  // at most one: the last
-    int used = 0;
-    if (exec) {
-      const int last = 63 - __builtin_clzll(exec);
-      used = last + 2 + (int)((ND >> last) & 1ull);
-    }
-    k_run += attempts;
-    n_rej += attempts - (F ? 1 : 0);
-    ps += used;
-    if (F) {
-      const int f = __builtin_ctzll(F);
-      const int cls = __builtin_amdgcn_readlane(at.cls, f);
-      const int dir = __builtin_amdgcn_readlane(at.dir, f);
-      const int32_t dx = __builtin_amdgcn_readlane(at.dx, f), dy = __builtin_amdgcn_readlane(at.dy, f);
-      const uint32_t from = __builtin_amdgcn_readlane(at.from, f);
-      const double event_time = __dadd_rn(time, __ddiv_rn((double)k_run, R));
-      if (select_only || event_time >= stop_time) {
-        pend(from, dir, k_run, event_time);
-      } else {
-        account(k_run, cls, dir, dx, dy);
-        if (lane == f) {
-          kmc_apply(at, pba, list, pos, site_atom, adx, ady, ajump);
-          hop_record(at.atom);
-        }
-        asm volatile("" ::: "memory");
-        bump();
+    // An attempt evaluated from the block-start state stays exact until an
+    // earlier accepted hop changes what it reads: the hop rewrites its own
+    // list slot and the atom list's last slot, and changes its origin and
+    // target sites.  So an attempt is hit when its slot is one of those two,
+    // or its origin, target or their neighbours include the hop's origin or
+    // target (its class, its free target and its atom are read there).  The
+    // block runs up to the first hit attempt; hits found from accepted lanes
+    // past it change nothing.  A window without an accepted attempt skips all
+    // of this.
+    const uint64_t AA = A & AC;
+    uint64_t exec = A;
+    if (AA & (AA - 1ull) ? true : (AA && (A >> __builtin_ctzll(AA)) > 1ull)) {
+      uint32_t nf[4], ntg[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        nf[e] = kmc_nbr(at.from, e, rows, cols, magic);
+        ntg[e] = kmc_nbr(at.to, e, rows, cols, magic);
       }
-      k_run = 0;
+      const bool is_att = (A >> lane) & 1ull;
+      bool hit = false;
+      uint64_t HIT = 0;
+      for (uint64_t rem = AA; rem; rem &= rem - 1) {
+        const int e = __builtin_ctzll(rem);
+        if (HIT & ((1ull << e) - 1)) break;  // past the first hit attempt
+        const int32_t pa_e = __builtin_amdgcn_readlane(at.pa, e);
+        const uint32_t fe = (uint32_t)__builtin_amdgcn_readlane((int)at.from, e);
+        const uint32_t te = (uint32_t)__builtin_amdgcn_readlane((int)at.to, e);
+        bool h = at.pa == pa_e || at.pa == pba || at.from == fe || at.from == te ||
+                 at.to == fe || at.to == te;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          h = h || nf[k] == fe || nf[k] == te || ntg[k] == fe || ntg[k] == te;
+        hit = hit || (lane > e && h);
+        HIT = __ballot(is_att && hit);
+      }
+      if (HIT) exec = A & ((1ull << __builtin_ctzll(HIT)) - 1);
     }
+
+    // the attempts in order: rejected ones (counted in bulk between events)
+    // add to the next event's increment; accepted ones are the events,
+    // applied in order by their lanes (their slots and sites are disjoint from
+    // every later attempt run here)
+    int64_t kk = k_run;
+    int last = -1;
+    int attempts = 0;
+    uint64_t consumed = 0;
+    bool cut = false;
+    for (uint64_t ev = exec & AC; ev; ev &= ev - 1) {
+      const int t = __builtin_ctzll(ev);
+      const uint64_t upto = t == 63 ? ~0ull : ((2ull << t) - 1ull);
+      const uint64_t span = exec & upto & ~consumed;  // attempts since the last event, with t
+      const int cnt = __popcll(span);
+      consumed |= span;
+      kk += cnt;
+      attempts += cnt;
+      n_rej += cnt - 1;
+      last = t;
+      const int cls = __builtin_amdgcn_readlane(at.cls, t);
+      const int dir = __builtin_amdgcn_readlane(at.dir, t);
+      const int32_t dx = __builtin_amdgcn_readlane(at.dx, t), dy = __builtin_amdgcn_readlane(at.dy, t);
+      const uint32_t from = __builtin_amdgcn_readlane((int)at.from, t);
+      const double event_time = __dadd_rn(time, __ddiv_rn((double)kk, R));
+      if (done_ev >= target || event_time >= stop_time) {
+        pend(from, dir, kk, event_time);
+        kk = 0;
+        cut = true;
+        break;
+      }
+      account(kk, cls, dir, dx, dy);
+      if (lane == t) {
+        kmc_apply(at, pba, list, pos, site_atom, adx, ady, ajump);
+        hop_record(at.atom);
+      }
+      asm volatile("" ::: "memory");
+      bump();
+      kk = 0;
+      if (done_ev >= target && !select_at_end) {  // the launch's events are done
+        cut = true;
+        break;
+      }
+    }
+    if (!cut) {  // rejected attempts after the last event
+      const uint64_t rest = exec & ~consumed;
+      const int cnt = __popcll(rest);
+      kk += cnt;
+      attempts += cnt;
+      n_rej += cnt;
+      if (rest) last = 63 - __builtin_clzll(rest);
+    }
+    k_run = kk;
+    if (last >= 0) ps += last + 2 + (int)((ND >> last) & 1ull);
 
     if (!stopped && (SP & 1ull) && attempts == 0) {
       // an attempt whose Lemire draw may reject, with the full rejection loop;
