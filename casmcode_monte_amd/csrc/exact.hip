@@ -393,8 +393,14 @@ __global__ __launch_bounds__(128) void exact_block_kernel(const ExactParams *__r
     M += (int64_t)(psum >> 16) - 2 * nflip;
     n_acc += nflip;
     n_rej += attempts - nflip;
-    if (flipped) atomicXor(&latw[site >> 2], SPIN_BIT << sh);
-    if (att) atomicAnd(&latw[site >> 2], ~(0xFEu << sh));
+    // one add per attempt lane takes its own mark back out of its site's byte
+    // and, for a flipped site (drawn once in the window, so its byte is
+    // b0 + MARK_A), turns the spin bit: the byte never goes below zero, so
+    // no borrow reaches the next site
+    if (att) {
+      const uint32_t dv = flipped ? (b0 ? 0u - (MARK_A + SPIN_BIT) : SPIN_BIT - MARK_A) : 0u - MARK_A;
+      atomicAdd(&latw[site >> 2], dv << sh);
+    }
     asm volatile("" ::: "memory");
     pos += t;
     step += attempts;
