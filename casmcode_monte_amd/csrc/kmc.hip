@@ -26,8 +26,9 @@
 //
 // Block walk: lane t evaluates stream position P+t as if an attempt started
 // there, from the block-start state.  Rejected attempts change nothing but the
-// clock, so every attempt up to and including the first accepted one is exact;
-// the block commits those and starts again after them.
+// clock, and an accepted hop changes only what a later attempt reads at its
+// own slots and sites, so the block commits every attempt up to the first one
+// an earlier accepted hop changes and starts again after them.
 #include "internal.hh"
 #include "walk.hh"
 
@@ -332,13 +333,7 @@ __global__ __launch_bounds__(128) void kmc_kernel(const KmcParams *__restrict__ 
 
     // attempt starts over the window: 0, then +2 or +3 (a uniform drawn), up
     // to the first special one
-    uint64_t A = 0;
-    for (int t = 0; t < W;) {
-      const uint64_t bit = 1ull << t;
-      if (SP & bit) break;
-      A |= bit;
-      t += 2 + (int)((ND >> t) & 1ull);
-    }
+    const uint64_t A = window_starts2(ND, SP, 0ull, W);
     // An attempt evaluated from the block-start state stays exact until an
     // earlier accepted hop changes what it reads: the hop rewrites its own
     // list slot and the atom list's last slot, and changes its origin and

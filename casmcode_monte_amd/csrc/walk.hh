@@ -173,4 +173,35 @@ struct MtRing3 {
   }
 };
 
+// Starts of the attempts in a window of W stream words (bit t: an attempt
+// starts at word t) when an attempt takes two words or, if it draws (bit t of
+// ND), three: 0, then 2 or 3 words on, up to the first start in SX (excluded:
+// the walk stops before it) or in SI (included: the walk stops after it).
+// Starts keep their parity until one draws, so they are taken a segment at a
+// time, one iteration per drawing attempt instead of one per attempt.
+__device__ __forceinline__ uint64_t window_starts2(uint64_t ND, uint64_t SX, uint64_t SI, int W) {
+  const uint64_t win = W >= 64 ? ~0ull : ((1ull << W) - 1ull);
+  uint64_t A = 0;
+  int t = 0;
+  while (t < W) {
+    const uint64_t res = (t & 1) ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull;
+    const uint64_t cand = res & (~0ull << t) & win;
+    const uint64_t stop = cand & (SX | SI | ND);
+    if (!stop) {
+      A |= cand;
+      break;
+    }
+    const int f = __builtin_ctzll(stop);
+    const uint64_t below = (1ull << f) - 1ull;
+    if ((SX >> f) & 1ull) {
+      A |= cand & below;
+      break;
+    }
+    A |= cand & (below | (1ull << f));
+    if ((SI >> f) & 1ull) break;
+    t = f + 3;
+  }
+  return A;
+}
+
 }  // namespace mcg
